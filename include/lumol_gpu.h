@@ -1,0 +1,212 @@
+/*
+ * lumol_gpu.h -- C ABI of the MI355X (gfx950) force-and-energy backend for
+ * Lumol.
+ *
+ * The library keeps one system resident in HBM per handle: positions,
+ * velocities, masses, charges, kinds, molecule topology, the potential
+ * tables, the cell list and the Ewald k-space caches.  Host buffers cross
+ * the boundary only in lg_system_set_* / lg_system_get_* / lg_forces /
+ * lg_potential_energy / lg_virial; MD steps and MC moves run on the device.
+ *
+ * Every entry point returns an int status (LG_OK = 0).  No exception, panic
+ * or abort crosses the boundary; lg_last_error() returns a thread-local
+ * message for the last failure on the calling thread.  Calls on one handle
+ * are ordered on the handle's stream and must not overlap from several host
+ * threads (the reference serialises Ewald behind a write lock,
+ * lumol-core/src/energy/global/ewald.rs:847-874; the handle does the same by
+ * contract).
+ *
+ * Units are Lumol's internal units (A, fs, u, K; energy in u*A^2/fs^2), and
+ * vectors are AoS triples of doubles, as Lumol's Vector3D
+ * (lumol-core/src/types/vectors.rs:59).
+ *
+ * Citations are to the reference checkout (lumol-org/lumol).
+ */
+#ifndef LUMOL_GPU_H
+#define LUMOL_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    LG_OK = 0,
+    LG_E_INVALID = 1,     /* bad argument or inconsistent system            */
+    LG_E_UNSUPPORTED = 2, /* valid for Lumol but outside the GPU kernel set */
+    LG_E_DEVICE = 3,      /* HIP runtime error                              */
+    LG_E_STATE = 4,       /* call order (e.g. commit without a cost)        */
+    LG_E_NONFINITE = 5    /* energy or coordinates are not finite           */
+};
+
+/* ---- closed set of GPU potential kinds ----------------------------------
+ * Replaces the open `Potential`/`PairPotential` trait objects
+ * (lumol-core/src/energy/mod.rs:98-163); kernels cannot call host closures.
+ * params: LJ {sigma, epsilon} (functions.rs:72-107);
+ *         HARMONIC {k, x0} (functions.rs:128-160); NULL {} (functions.rs:30-50).
+ */
+enum { LG_POT_NULL = 0, LG_POT_LJ = 1, LG_POT_HARMONIC = 2 };
+
+/* PairRestriction (lumol-core/src/energy/restrictions.rs:13-33). */
+enum {
+    LG_RESTRICT_NONE = 0,
+    LG_RESTRICT_INTRA = 1,
+    LG_RESTRICT_INTER = 2,
+    LG_RESTRICT_EXCLUDE12 = 3,
+    LG_RESTRICT_EXCLUDE13 = 4,
+    LG_RESTRICT_EXCLUDE14 = 5,
+    LG_RESTRICT_SCALE14 = 6
+};
+
+/* Arithmetic form of the evaluation. */
+enum {
+    LG_PRECISION_FP64 = 0,  /* all FP64: matches the reference to 1e-10 rel   */
+    LG_PRECISION_MIXED = 1  /* FP32 pair/erfc/trig, FP64 positions/sums        */
+};
+
+enum { LG_INTEGRATOR_VELOCITY_VERLET = 0, LG_INTEGRATOR_LEAP_FROG = 1 };
+
+/* Energy terms of EnergyEvaluator (lumol-core/src/sys/energy.rs:47-158). */
+typedef struct lg_energy_terms {
+    double pairs;      /* EnergyEvaluator::pairs      energy.rs:47   */
+    double pairs_tail; /* EnergyEvaluator::pairs_tail energy.rs:63   */
+    double bonds;      /* EnergyEvaluator::bonds      energy.rs:90   */
+    double angles;     /* EnergyEvaluator::angles     energy.rs:110  */
+    double dihedrals;  /* EnergyEvaluator::dihedrals  energy.rs:131  */
+    double coulomb;    /* EnergyEvaluator::coulomb    energy.rs:145  */
+    double global;     /* EnergyEvaluator::global     energy.rs:152  */
+    double total;      /* PotentialEnergy::compute    compute.rs:114 */
+    double coulomb_real; /* Ewald real space   ewald.rs:431 (Wolf: pairs+self) */
+    double coulomb_self; /* Ewald self energy  ewald.rs:619              */
+    double coulomb_kspace; /* Ewald k-space    ewald.rs:677              */
+} lg_energy_terms;
+
+typedef struct lg_context lg_context;
+typedef struct lg_system lg_system;
+
+/* ---- errors --------------------------------------------------------------- */
+/* Message of the last failed call on this thread (never NULL). */
+const char* lg_last_error(void);
+/* Library version string. */
+const char* lg_version(void);
+
+/* ---- context: one per GPU ---------------------------------------------------- */
+/* Opens device `device` and creates the stream the context's systems use.
+ * No counterpart in the reference (rayon's global pool, lumol-core/Cargo.toml). */
+int lg_context_create(int device, lg_context** out);
+int lg_context_destroy(lg_context* ctx);
+
+/* ---- system: replaces System / Configuration ----------------------------------
+ * System::new / with_cell (lumol-core/src/sys/system.rs:41-80). */
+int lg_system_create(lg_context* ctx, size_t natoms, lg_system** out);
+int lg_system_destroy(lg_system* sys);
+int lg_system_size(const lg_system* sys, size_t* natoms);
+
+/* UnitCell::ortho / cubic (lumol-core/src/sys/config/cells.rs:60-100).
+ * Orthorhombic only; lengths all zero = UnitCell::infinite (cells.rs:50).
+ * Triclinic cells return LG_E_UNSUPPORTED. */
+int lg_system_set_cell(lg_system* sys, double a, double b, double c);
+
+/* Particle SoA (lumol-core/src/sys/config/particles.rs:31-46). */
+int lg_system_set_positions(lg_system* sys, const double* xyz /* 3N */);
+int lg_system_get_positions(lg_system* sys, double* xyz /* 3N */);
+int lg_system_set_velocities(lg_system* sys, const double* v /* 3N */);
+int lg_system_get_velocities(lg_system* sys, double* v /* 3N */);
+int lg_system_set_masses(lg_system* sys, const double* mass /* N */);
+int lg_system_set_charges(lg_system* sys, const double* q /* N */);
+/* ParticleKind per atom (particles.rs:36, interactions.rs:94-103). */
+int lg_system_set_kinds(lg_system* sys, const uint32_t* kinds /* N */, uint32_t nkinds);
+
+/* System::add_bond (lumol-core/src/sys/config/configuration.rs:150-200).
+ * Molecules are the connected components of the bond graph and must be
+ * contiguous atom ranges; angles, dihedrals and bond paths follow
+ * Bonding::rebuild (lumol-core/src/sys/config/bonding.rs:78-155). */
+int lg_system_set_bonds(lg_system* sys, size_t nbonds, const uint32_t* pairs /* 2*nbonds */);
+int lg_system_molecule_count(const lg_system* sys, size_t* nmol);
+
+/* Evaluation form; see LG_PRECISION_*. */
+int lg_system_set_precision(lg_system* sys, int precision);
+
+/* ---- potentials ------------------------------------------------------------------
+ * System::set_pair_potential (system.rs:122-131) with a PairInteraction
+ * (lumol-core/src/energy/pairs.rs:57-297): cutoff `r >= cutoff -> 0`,
+ * `shifted` subtracts V(cutoff) from the energy only, `tail` enables
+ * tail corrections, `restriction`/`scale14` as PairRestriction.
+ * Returns LG_E_INVALID when the cutoff is above half the smallest cell
+ * length (system.rs:123-129 panics there). */
+int lg_pair_set(lg_system* sys, uint32_t kind_i, uint32_t kind_j, int pot_kind, const double* params,
+                double cutoff, int shifted, int tail, int restriction, double scale14);
+/* System::set_bond_potential / set_angle_potential / set_dihedral_potential
+ * (system.rs:133-170). */
+int lg_bond_set(lg_system* sys, uint32_t kind_i, uint32_t kind_j, int pot_kind, const double* params);
+int lg_angle_set(lg_system* sys, uint32_t kind_i, uint32_t kind_j, uint32_t kind_k, int pot_kind,
+                 const double* params);
+int lg_dihedral_set(lg_system* sys, uint32_t kind_i, uint32_t kind_j, uint32_t kind_k, uint32_t kind_m,
+                    int pot_kind, const double* params);
+
+/* System::set_coulomb_potential with SharedEwald::new(Ewald::new(cutoff,
+ * kmax, alpha)) (ewald.rs:278-305, 847-956); alpha NaN selects pi/cutoff
+ * (ewald.rs:279).  `restriction` as CoulombicPotential::set_restriction
+ * (global/mod.rs:197-202); Scale14 is rejected as ewald.rs:388 asserts. */
+int lg_coulomb_ewald(lg_system* sys, double cutoff, int kmax, double alpha, int restriction);
+/* Wolf::new(cutoff) (wolf.rs:68-84). */
+int lg_coulomb_wolf(lg_system* sys, double cutoff, int restriction, double scale14);
+int lg_coulomb_clear(lg_system* sys);
+
+/* ---- evaluation --------------------------------------------------------------------
+ * System::forces (system.rs:316 -> compute.rs:30-108).  Writes 3N doubles;
+ * with accumulate != 0 adds into `forces` instead (GlobalPotential::forces
+ * accumulates, global/mod.rs:95). */
+int lg_forces(lg_system* sys, double* forces /* 3N */, int accumulate);
+/* System::potential_energy and the EnergyEvaluator terms
+ * (system.rs:263, compute.rs:111-128, energy.rs:47-158). */
+int lg_potential_energy(lg_system* sys, lg_energy_terms* out);
+/* System::virial with the atomic definition (system.rs:286,
+ * compute.rs:195-255), row-major 3x3. */
+int lg_virial(lg_system* sys, double* virial /* 9 */);
+/* KineticEnergy / Temperature (compute.rs:133-172; dof = 3N, system.rs:249). */
+int lg_kinetic_energy(lg_system* sys, double* out);
+int lg_temperature(lg_system* sys, double* out);
+/* Pair set of the device neighbour search: every inter-molecular (i < j)
+ * with r < cutoff, r computed by the kernels' own minimum image and norm.
+ * Replaces the brute-force `for j in i+1..n` candidate loop
+ * (compute.rs:40) as a checkable set; `cutoff` must not exceed the largest
+ * interaction cutoff.  Writes min(count, capacity) pairs (2 uint32 each,
+ * unordered) and the total in *count. */
+int lg_pairs_within(lg_system* sys, double cutoff, uint32_t* pairs, uint64_t capacity, uint64_t* count);
+/* Number of k-vectors of the Ewald half-space set (ewald.rs:130-183). */
+int lg_ewald_kvector_count(lg_system* sys, int* out);
+
+/* ---- molecular dynamics --------------------------------------------------------------
+ * MolecularDynamics::new(integrator) + Simulation::run (lumol-sim/src/md/
+ * molecular_dynamics.rs:11-83, simulations.rs:69-97).  `integrator` is
+ * VelocityVerlet (integrators.rs:22-70) or LeapFrog (:128-169); the first
+ * step starts from zero accelerations as both setups do (:40-42, :146-148).
+ * berendsen_temperature > 0 enables BerendsenThermostat::new(T, tau)
+ * (thermostats.rs:91-121), applied after each step. */
+int lg_md_setup(lg_system* sys, int integrator, double timestep, double berendsen_temperature,
+                double berendsen_tau);
+/* Runs `nsteps` steps on the device (positions/velocities stay in HBM). */
+int lg_md_run(lg_system* sys, uint64_t nsteps);
+
+/* ---- Monte Carlo energy cache ---------------------------------------------------------
+ * EnergyCache::init (lumol-core/src/sys/cache.rs:75-98).  Computes the
+ * cached energy terms and the Ewald rho(k) cache on the device. */
+int lg_cache_init(lg_system* sys, double* energy);
+/* EnergyCache::move_molecule_cost (cache.rs:145-220): energy change of
+ * moving molecule `molid` to `newpos` (3 * molecule size doubles). */
+int lg_move_cost(lg_system* sys, uint32_t molid, const double* newpos, double* cost);
+/* EnergyCache::update (cache.rs:118-127) + Translate::apply
+ * (lumol-sim/src/mc/moves/translate.rs:115-127, COM wrap when wrap != 0). */
+int lg_move_commit(lg_system* sys, int wrap);
+/* Rejected move: drop the staged update (monte_carlo.rs:318-321). */
+int lg_move_discard(lg_system* sys);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LUMOL_GPU_H */

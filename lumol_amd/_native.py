@@ -1,0 +1,155 @@
+"""ctypes binding of liblumol_gpu.so (the C ABI of include/lumol_gpu.h).
+
+There is no fallback: if the library is missing or the device cannot be
+opened, every call raises.  The library is built in-tree by
+``lumol_amd.build.build()``.
+"""
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblumol_gpu.so")
+
+LG_OK = 0
+LG_E_INVALID = 1
+LG_E_UNSUPPORTED = 2
+LG_E_DEVICE = 3
+LG_E_STATE = 4
+LG_E_NONFINITE = 5
+
+LG_POT_NULL, LG_POT_LJ, LG_POT_HARMONIC = 0, 1, 2
+LG_PRECISION_FP64, LG_PRECISION_MIXED = 0, 1
+LG_INTEGRATOR_VELOCITY_VERLET, LG_INTEGRATOR_LEAP_FROG = 0, 1
+
+# every symbol include/lumol_gpu.h declares
+EXPORTS = [
+    "lg_last_error", "lg_version", "lg_context_create", "lg_context_destroy",
+    "lg_system_create", "lg_system_destroy", "lg_system_size", "lg_system_set_cell",
+    "lg_system_set_positions", "lg_system_get_positions", "lg_system_set_velocities",
+    "lg_system_get_velocities", "lg_system_set_masses", "lg_system_set_charges",
+    "lg_system_set_kinds", "lg_system_set_bonds", "lg_system_molecule_count",
+    "lg_system_set_precision", "lg_pair_set", "lg_bond_set", "lg_angle_set", "lg_dihedral_set",
+    "lg_coulomb_ewald", "lg_coulomb_wolf", "lg_coulomb_clear", "lg_forces",
+    "lg_potential_energy", "lg_virial", "lg_kinetic_energy", "lg_temperature",
+    "lg_ewald_kvector_count", "lg_pairs_within", "lg_md_setup", "lg_md_run", "lg_cache_init", "lg_move_cost",
+    "lg_move_commit", "lg_move_discard",
+]
+
+
+class EnergyTerms(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_double) for name in (
+        "pairs", "pairs_tail", "bonds", "angles", "dihedrals", "coulomb", "global_",
+        "total", "coulomb_real", "coulomb_self", "coulomb_kspace")]
+
+
+class LumolGpuError(RuntimeError):
+    """Error returned by the GPU library (status code and message)."""
+
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+
+P = ctypes.c_void_p
+D = ctypes.c_double
+DP = ctypes.POINTER(ctypes.c_double)
+U32 = ctypes.c_uint32
+U32P = ctypes.POINTER(ctypes.c_uint32)
+SZ = ctypes.c_size_t
+I = ctypes.c_int
+
+
+def _declare(lib):
+    sig = {
+        "lg_last_error": (ctypes.c_char_p, []),
+        "lg_version": (ctypes.c_char_p, []),
+        "lg_context_create": (I, [I, ctypes.POINTER(P)]),
+        "lg_context_destroy": (I, [P]),
+        "lg_system_create": (I, [P, SZ, ctypes.POINTER(P)]),
+        "lg_system_destroy": (I, [P]),
+        "lg_system_size": (I, [P, ctypes.POINTER(SZ)]),
+        "lg_system_set_cell": (I, [P, D, D, D]),
+        "lg_system_set_positions": (I, [P, DP]),
+        "lg_system_get_positions": (I, [P, DP]),
+        "lg_system_set_velocities": (I, [P, DP]),
+        "lg_system_get_velocities": (I, [P, DP]),
+        "lg_system_set_masses": (I, [P, DP]),
+        "lg_system_set_charges": (I, [P, DP]),
+        "lg_system_set_kinds": (I, [P, U32P, U32]),
+        "lg_system_set_bonds": (I, [P, SZ, U32P]),
+        "lg_system_molecule_count": (I, [P, ctypes.POINTER(SZ)]),
+        "lg_system_set_precision": (I, [P, I]),
+        "lg_pair_set": (I, [P, U32, U32, I, DP, D, I, I, I, D]),
+        "lg_bond_set": (I, [P, U32, U32, I, DP]),
+        "lg_angle_set": (I, [P, U32, U32, U32, I, DP]),
+        "lg_dihedral_set": (I, [P, U32, U32, U32, U32, I, DP]),
+        "lg_coulomb_ewald": (I, [P, D, I, D, I]),
+        "lg_coulomb_wolf": (I, [P, D, I, D]),
+        "lg_coulomb_clear": (I, [P]),
+        "lg_forces": (I, [P, DP, I]),
+        "lg_potential_energy": (I, [P, ctypes.POINTER(EnergyTerms)]),
+        "lg_virial": (I, [P, DP]),
+        "lg_kinetic_energy": (I, [P, DP]),
+        "lg_temperature": (I, [P, DP]),
+        "lg_ewald_kvector_count": (I, [P, ctypes.POINTER(I)]),
+        "lg_pairs_within": (I, [P, D, U32P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+        "lg_md_setup": (I, [P, I, D, D, D]),
+        "lg_md_run": (I, [P, ctypes.c_uint64]),
+        "lg_cache_init": (I, [P, DP]),
+        "lg_move_cost": (I, [P, U32, DP, DP]),
+        "lg_move_commit": (I, [P, I]),
+        "lg_move_discard": (I, [P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build it with lumol_amd.build.build() "
+                    "(python __graft_entry__.py build)")
+            handle = ctypes.CDLL(LIB_PATH)
+            _declare(handle)
+            _lib = handle
+        return _lib
+
+
+def check(status):
+    if status != LG_OK:
+        msg = lib().lg_last_error().decode("utf-8", "replace")
+        raise LumolGpuError(status, msg)
+
+
+def dptr(array):
+    return array.ctypes.data_as(DP)
+
+
+def u32ptr(array):
+    return array.ctypes.data_as(U32P)
+
+
+_contexts = {}
+
+
+def context(device=0):
+    """One lg_context per device, created on first use and kept for the process."""
+    with _lock:
+        ctx = _contexts.get(device)
+    if ctx is None:
+        handle = P()
+        check(lib().lg_context_create(device, ctypes.byref(handle)))
+        with _lock:
+            _contexts[device] = handle
+        ctx = handle
+    return ctx

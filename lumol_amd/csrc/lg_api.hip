@@ -1,0 +1,1403 @@
+// lg_api.hip -- host side of the C ABI declared in include/lumol_gpu.h.
+//
+// Owns the device-resident state of each system and sequences the kernels of
+// lg_kernels.hpp on the context's stream.  Host code restates the parts of
+// the reference that are set-up rather than hot path:
+//   Bonding::rebuild (angles, dihedrals, bond distances)
+//       lumol-core/src/sys/config/bonding.rs:78-155
+//   Ewald k-vector set and factors  lumol-core/src/energy/global/ewald.rs:130-183, 353-378
+//   tail corrections                lumol-core/src/sys/energy.rs:63-79, compute.rs:218-228
+//   Ewald / Wolf self terms         ewald.rs:619-626, wolf.rs:100-102, 177-207
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <numeric>
+#include <set>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/lumol_gpu.h"
+#include "lg_kernels.hpp"
+
+using namespace lg;
+
+namespace {
+
+thread_local std::string g_error = "";
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define LG_HIP(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return fail(LG_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LG_TRY(expr)              \
+    do {                          \
+        int r_ = (expr);          \
+        if (r_ != LG_OK) return r_; \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t count) {
+        if (count <= cap && p) return LG_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return fail(LG_E_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+        cap = std::max<size_t>(count, 1);
+        return LG_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct PairSpec {
+    int pot;
+    double p0, p1, rc;
+    int shifted, tail, restriction;
+    double scale14;
+};
+struct BondedSpec {
+    int pot;
+    double p0, p1;
+};
+
+double host_pot_energy(int pot, double p0, double p1, double r) {
+    if (pot == LG_POT_LJ) {
+        double x = p0 / r;
+        double x2 = x * x;
+        double x4 = x2 * x2;
+        double s6 = x2 * x4;
+        return 4.0 * p1 * (s6 * s6 - s6);
+    }
+    if (pot == LG_POT_HARMONIC) {
+        double dx = r - p1;
+        return 0.5 * p0 * dx * dx;
+    }
+    return 0.0;
+}
+// functions.rs:92-106 (LJ only; Null/Harmonic tails are 0)
+double host_tail_energy(const PairSpec& p) {
+    if (!p.tail || p.pot != LG_POT_LJ) return 0.0;
+    double s3 = p.p0 * p.p0 * p.p0, rc3 = p.rc * p.rc * p.rc;
+    double s9 = s3 * s3 * s3, rc9 = rc3 * rc3 * rc3;
+    return 4.0 / 3.0 * p.p1 * s3 * (1.0 / 3.0 * s9 / rc9 - s3 / rc3);
+}
+double host_tail_virial(const PairSpec& p) {
+    if (!p.tail || p.pot != LG_POT_LJ) return 0.0;
+    double s3 = p.p0 * p.p0 * p.p0, rc3 = p.rc * p.rc * p.rc;
+    double s9 = s3 * s3 * s3, rc9 = rc3 * rc3 * rc3;
+    return 8.0 * p.p1 * s3 * (2.0 / 3.0 * s9 / rc9 - s3 / rc3);
+}
+
+}  // namespace
+
+struct lg_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t max_shared = 65536;
+};
+
+struct lg_system {
+    lg_context* ctx = nullptr;
+    size_t n = 0;
+    // host mirrors of what the caller set
+    std::vector<double> mass, charge;
+    std::vector<uint32_t> kind;
+    uint32_t nkinds = 1;
+    double L[3] = {0, 0, 0};
+    bool periodic = false;
+    std::vector<std::pair<uint32_t, uint32_t>> bonds;
+    std::map<std::pair<uint32_t, uint32_t>, PairSpec> pairs;
+    std::map<std::pair<uint32_t, uint32_t>, BondedSpec> bond_pots;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, BondedSpec> angle_pots;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, BondedSpec> dihedral_pots;
+    int coul_kind = COUL_NONE;
+    double coul_rc = 0, coul_alpha = 0;
+    int coul_kmax = 0, coul_restr = 0;
+    double coul_s14 = 1.0;
+    int precision = LG_PRECISION_FP64;
+
+    // derived topology
+    std::vector<int> molid, mol_start;
+    std::vector<std::tuple<int, int, int>> angles;
+    std::vector<std::tuple<int, int, int, int>> dihedrals;
+    bool topo_dirty = true, params_dirty = true, kspace_dirty = true, nlist_dirty = true, meta_dirty = true;
+
+    // device state
+    DevBuf<double4> pos, vel, acc, force, ref;
+    DevBuf<int2> meta;
+    // neighbour list
+    Grid grid{};
+    Box box{};
+    double rlist = 0, skin = 0;
+    int cap = 0;
+    DevBuf<int> cellid, slot, count, start, tmp, sorted, nbr, nnbr, flags;  // flags[0] rebuild, [1] overflow
+    // parameters
+    DevBuf<PairParam> table;
+    DevBuf<int> pair_restr;
+    DevBuf<double> pair_s14;
+    std::vector<PairParam> h_table;
+    CoulParam coul{};
+    // molecules
+    int nmol = 0, nmol_multi = 0;
+    DevBuf<int> mol_list, d_mol_start, ip_start, b_start, a_start, d_start, d_param;
+    DevBuf<int4> ipairs, d_bonds, d_angles, d_dihedrals;
+    DevBuf<BondedParam> bparam, aparam, dparam;
+    // k-space
+    int nk = 0, nrows = 0, rho_width = 64, force_width = 64;
+    DevBuf<int4> kidx;
+    DevBuf<KRow> rows;
+    DevBuf<double> efac, vfac;
+    DevBuf<double4> field;
+    DevBuf<double2> rho, rho_partial;
+    // energy partials
+    DevBuf<double> epart_pairs, epart_mol, epart_k, ke_part, lambda;
+    // MD
+    int integrator = LG_INTEGRATOR_VELOCITY_VERLET;
+    double dt = 1.0, berendsen_T = 0.0, berendsen_tau = 1.0;
+    bool md_ready = false;
+    // snapshot for neighbour-list overflow recovery
+    DevBuf<double4> snap_pos, snap_vel, snap_acc;
+};
+
+namespace {
+
+hipStream_t S(lg_system* s) { return s->ctx->stream; }
+
+int blocks_for(size_t n, int b) { return int((n + b - 1) / b); }
+
+// ---- topology (bonding.rs:78-155, configuration.rs:124-144) ---------------------------
+int build_topology(lg_system* s, std::vector<std::tuple<int, int, int>>& ipairs_out) {
+    size_t n = s->n;
+    std::vector<size_t> parent(n);
+    std::iota(parent.begin(), parent.end(), size_t(0));
+    std::function<size_t(size_t)> find = [&](size_t x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    for (auto& b : s->bonds) {
+        size_t a = find(b.first), c = find(b.second);
+        if (a != c) parent[std::max(a, c)] = std::min(a, c);
+    }
+    s->molid.assign(n, 0);
+    s->mol_start.clear();
+    int m = 0;
+    for (size_t i = 0; i < n; i++) {
+        size_t root = find(i);
+        if (root == i) {
+            s->mol_start.push_back(int(i));
+            s->molid[i] = m++;
+        } else {
+            if (s->molid[root] != m - 1 || find(i - 1) != root)
+                return fail(LG_E_INVALID, "molecules must be contiguous ranges of atoms");
+            s->molid[i] = s->molid[root];
+        }
+    }
+    s->mol_start.push_back(int(n));
+    s->nmol = m;
+
+    std::set<std::tuple<int, int, int>> angles;
+    std::set<std::tuple<int, int, int, int>> dihedrals;
+    auto make_angle = [](int a, int b, int c) { return std::make_tuple(std::min(a, c), b, std::max(a, c)); };
+    auto make_dihedral = [](int a, int b, int c, int d) {
+        if (std::max(a, b) < std::max(c, d)) return std::make_tuple(a, b, c, d);
+        return std::make_tuple(d, c, b, a);
+    };
+    std::vector<std::pair<int, int>> bonds;
+    for (auto& b : s->bonds) bonds.push_back({int(b.first), int(b.second)});
+    for (auto& b1 : bonds)
+        for (auto& b2 : bonds) {
+            if (b1 == b2) continue;
+            std::tuple<int, int, int> angle;
+            if (b1.first == b2.second)
+                angle = make_angle(b2.first, b2.second, b1.second);
+            else if (b1.second == b2.first)
+                angle = make_angle(b1.first, b1.second, b2.second);
+            else if (b1.second == b2.second)
+                angle = make_angle(b1.first, b1.second, b2.first);
+            else if (b1.first == b2.first)
+                angle = make_angle(b1.second, b1.first, b2.second);
+            else
+                continue;
+            angles.insert(angle);
+            auto [ai, aj, ak] = angle;
+            for (auto& b3 : bonds) {
+                if (b2 == b3) continue;
+                if (ak == b3.first && aj != b3.second)
+                    dihedrals.insert(make_dihedral(ai, aj, ak, b3.second));
+                else if (ak == b3.second && aj != b3.first)
+                    dihedrals.insert(make_dihedral(ai, aj, ak, b3.first));
+                else if (ai == b3.second && aj != b3.first)
+                    dihedrals.insert(make_dihedral(b3.first, ai, aj, ak));
+                else if (ai == b3.first && aj != b3.second)
+                    dihedrals.insert(make_dihedral(b3.second, ai, aj, ak));
+            }
+        }
+    s->angles.assign(angles.begin(), angles.end());
+    s->dihedrals.assign(dihedrals.begin(), dihedrals.end());
+    std::map<std::pair<int, int>, int> flags;
+    for (auto& [i, j] : bonds) {
+        flags[{i, j}] |= 1;
+        flags[{j, i}] |= 1;
+    }
+    for (auto& [i, j, k] : s->angles) {
+        flags[{i, k}] |= 2;
+        flags[{k, i}] |= 2;
+    }
+    for (auto& [i, j, k, mm] : s->dihedrals) {
+        flags[{i, mm}] |= 4;
+        flags[{mm, i}] |= 4;
+    }
+    ipairs_out.clear();
+    for (int mol = 0; mol < s->nmol; mol++) {
+        int m0 = s->mol_start[mol], m1 = s->mol_start[mol + 1];
+        for (int i = m0; i < m1; i++)
+            for (int j = i + 1; j < m1; j++) {
+                auto it = flags.find({i, j});
+                int f = it == flags.end() ? 0 : it->second;
+                int path = (f & 1) ? BP_ONE : (f & 2) ? BP_TWO : (f & 4) ? BP_THREE : BP_FAR;
+                ipairs_out.push_back({i, j, path});
+            }
+    }
+    return LG_OK;
+}
+
+template <class T>
+int upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
+    LG_TRY(b.ensure(v.size()));
+    if (!v.empty()) LG_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return LG_OK;
+}
+
+int prepare_topology(lg_system* s) {
+    std::vector<std::tuple<int, int, int>> ip;
+    LG_TRY(build_topology(s, ip));
+    hipStream_t st = S(s);
+    std::vector<int> mol_list;
+    std::vector<int> ip_start(s->nmol + 1, 0), b_start(s->nmol + 1, 0), a_start(s->nmol + 1, 0),
+        d_start(s->nmol + 1, 0);
+    std::vector<int4> h_ip, h_b, h_a, h_d;
+    std::vector<int> h_dparam;
+    std::vector<BondedParam> h_bp, h_ap, h_dp;
+    auto kind_of = [&](int a) { return s->kind[a]; };
+    auto find_bond = [&](int a, int b) -> BondedParam {
+        uint32_t x = kind_of(a), y = kind_of(b);
+        auto it = s->bond_pots.find({std::min(x, y), std::max(x, y)});
+        if (it == s->bond_pots.end()) return {POT_NONE, 0, 0};
+        return {it->second.pot, it->second.p0, it->second.p1};
+    };
+    auto find_angle = [&](int a, int b, int c) -> BondedParam {
+        uint32_t x = kind_of(a), y = kind_of(b), z = kind_of(c);
+        if (x > z) std::swap(x, z);
+        auto it = s->angle_pots.find({x, y, z});
+        if (it == s->angle_pots.end()) return {POT_NONE, 0, 0};
+        return {it->second.pot, it->second.p0, it->second.p1};
+    };
+    auto dkey = [](uint32_t i, uint32_t j, uint32_t k, uint32_t m) {
+        uint32_t ij = std::max(i, j), km = std::max(k, m);
+        if (ij == km) {
+            if (std::min(i, j) < std::min(k, m)) return std::make_tuple(i, j, k, m);
+            return std::make_tuple(m, k, j, i);
+        }
+        if (ij < km) return std::make_tuple(i, j, k, m);
+        return std::make_tuple(m, k, j, i);
+    };
+    auto find_dihedral = [&](int a, int b, int c, int d) -> BondedParam {
+        auto it = s->dihedral_pots.find(dkey(kind_of(a), kind_of(b), kind_of(c), kind_of(d)));
+        if (it == s->dihedral_pots.end()) return {POT_NONE, 0, 0};
+        return {it->second.pot, it->second.p0, it->second.p1};
+    };
+    // group per molecule
+    std::vector<std::vector<int4>> per_ip(s->nmol), per_b(s->nmol), per_a(s->nmol), per_d(s->nmol);
+    std::vector<std::vector<int>> per_dp(s->nmol);
+    for (auto& [i, j, path] : ip) per_ip[s->molid[i]].push_back(make_int4(i, j, path, 0));
+    for (auto& b : s->bonds) {
+        int i = int(b.first), j = int(b.second);
+        h_bp.push_back(find_bond(i, j));
+        per_b[s->molid[i]].push_back(make_int4(i, j, int(h_bp.size() - 1), 0));
+    }
+    for (auto& [i, j, k] : s->angles) {
+        h_ap.push_back(find_angle(i, j, k));
+        per_a[s->molid[i]].push_back(make_int4(i, j, k, int(h_ap.size() - 1)));
+    }
+    for (auto& [i, j, k, m] : s->dihedrals) {
+        h_dp.push_back(find_dihedral(i, j, k, m));
+        per_d[s->molid[i]].push_back(make_int4(i, j, k, m));
+        per_dp[s->molid[i]].push_back(int(h_dp.size() - 1));
+    }
+    for (int m = 0; m < s->nmol; m++) {
+        ip_start[m] = int(h_ip.size());
+        h_ip.insert(h_ip.end(), per_ip[m].begin(), per_ip[m].end());
+        b_start[m] = int(h_b.size());
+        h_b.insert(h_b.end(), per_b[m].begin(), per_b[m].end());
+        a_start[m] = int(h_a.size());
+        h_a.insert(h_a.end(), per_a[m].begin(), per_a[m].end());
+        d_start[m] = int(h_d.size());
+        h_d.insert(h_d.end(), per_d[m].begin(), per_d[m].end());
+        h_dparam.insert(h_dparam.end(), per_dp[m].begin(), per_dp[m].end());
+        if (s->mol_start[m + 1] - s->mol_start[m] >= 2) mol_list.push_back(m);
+    }
+    ip_start[s->nmol] = int(h_ip.size());
+    b_start[s->nmol] = int(h_b.size());
+    a_start[s->nmol] = int(h_a.size());
+    d_start[s->nmol] = int(h_d.size());
+    s->nmol_multi = int(mol_list.size());
+    LG_TRY(upload(s->mol_list, mol_list, st));
+    LG_TRY(upload(s->d_mol_start, s->mol_start, st));
+    LG_TRY(upload(s->ip_start, ip_start, st));
+    LG_TRY(upload(s->b_start, b_start, st));
+    LG_TRY(upload(s->a_start, a_start, st));
+    LG_TRY(upload(s->d_start, d_start, st));
+    LG_TRY(upload(s->ipairs, h_ip, st));
+    LG_TRY(upload(s->d_bonds, h_b, st));
+    LG_TRY(upload(s->d_angles, h_a, st));
+    LG_TRY(upload(s->d_dihedrals, h_d, st));
+    LG_TRY(upload(s->d_param, h_dparam, st));
+    LG_TRY(upload(s->bparam, h_bp, st));
+    LG_TRY(upload(s->aparam, h_ap, st));
+    LG_TRY(upload(s->dparam, h_dp, st));
+    LG_TRY(s->epart_mol.ensure(size_t(std::max(1, blocks_for(s->nmol_multi, 256))) * 11));
+    s->meta_dirty = true;
+    s->topo_dirty = false;
+    return LG_OK;
+}
+
+int prepare_meta(lg_system* s) {
+    std::vector<int2> m(s->n);
+    for (size_t i = 0; i < s->n; i++) m[i] = make_int2(int(s->kind[i]), s->molid.empty() ? int(i) : s->molid[i]);
+    LG_TRY(upload(s->meta, m, S(s)));
+    s->meta_dirty = false;
+    return LG_OK;
+}
+
+int prepare_params(lg_system* s) {
+    uint32_t K = s->nkinds;
+    if (K > uint32_t(kMaxKinds)) return fail(LG_E_UNSUPPORTED, "more than 16 particle kinds");
+    s->h_table.assign(size_t(K) * K, PairParam{POT_NONE, 0, 0, 0, 0, 0});
+    std::vector<int> restr(size_t(K) * K, 0);
+    std::vector<double> s14(size_t(K) * K, 1.0);
+    for (auto& [key, p] : s->pairs) {
+        for (int o = 0; o < 2; o++) {
+            uint32_t a = o ? key.second : key.first, b = o ? key.first : key.second;
+            if (a >= K || b >= K) return fail(LG_E_INVALID, "pair potential kind out of range");
+            PairParam& pp = s->h_table[size_t(a) * K + b];
+            pp.pot = p.pot;
+            pp.inter_excl = p.restriction == LG_RESTRICT_INTRA ? 1 : 0;
+            pp.p0 = p.p0;
+            pp.p1 = p.p1;
+            pp.rc = p.rc;
+            // pairs.rs:86-95: shift = V(cutoff) of the bare potential
+            pp.shift = p.shifted ? host_pot_energy(p.pot, p.p0, p.p1, p.rc) : 0.0;
+            restr[size_t(a) * K + b] = p.restriction;
+            s14[size_t(a) * K + b] = p.scale14;
+        }
+    }
+    LG_TRY(upload(s->table, s->h_table, S(s)));
+    LG_TRY(upload(s->pair_restr, restr, S(s)));
+    LG_TRY(upload(s->pair_s14, s14, S(s)));
+    CoulParam c{};
+    c.kind = s->coul_kind;
+    c.inter_excl = s->coul_restr == LG_RESTRICT_INTRA ? 1 : 0;
+    c.rc = s->coul_rc;
+    if (s->coul_kind == COUL_EWALD) {
+        c.alpha = s->coul_alpha;
+    } else if (s->coul_kind == COUL_WOLF) {
+        // wolf.rs:68-84
+        c.alpha = kPi / s->coul_rc;
+        double ac = c.alpha * s->coul_rc;
+        double ac2 = ac * ac;
+        c.e_const = std::erfc(ac) / s->coul_rc;
+        c.f_const = std::erfc(ac) / (s->coul_rc * s->coul_rc) + kFrac2SqrtPi * c.alpha * std::exp(-ac2) / s->coul_rc;
+    }
+    s->coul = c;
+    s->params_dirty = false;
+    s->nlist_dirty = true;
+    return LG_OK;
+}
+
+// ewald.rs:130-183 and :353-378 for an orthorhombic cell
+int prepare_kspace(lg_system* s) {
+    s->kspace_dirty = false;
+    s->nk = 0;
+    s->nrows = 0;
+    if (s->coul_kind != COUL_EWALD) return LG_OK;
+    if (!s->periodic) return fail(LG_E_INVALID, "Ewald is not defined with infinite unit cell");
+    double det = s->L[0] * (s->L[1] * s->L[2] - 0.0 * 0.0) - 0.0 + 0.0;
+    double inv_det = 1.0 / det;
+    double inv[3] = {(s->L[1] * s->L[2] - 0.0 * 0.0) * inv_det, (s->L[0] * s->L[2] - 0.0 * 0.0) * inv_det,
+                     (s->L[0] * s->L[1] - 0.0 * 0.0) * inv_det};
+    auto kvec = [&](double a, double b, double c, double out[3]) {
+        out[0] = 2.0 * kPi * inv[0] * a;
+        out[1] = 2.0 * kPi * inv[1] * b;
+        out[2] = 2.0 * kPi * inv[2] * c;
+    };
+    double k111[3];
+    kvec(1, 1, 1, k111);
+    int kmax = s->coul_kmax;
+    double mx = std::max(k111[0], std::max(k111[1], k111[2])) * double(kmax);
+    double kmax2 = 1.0001 * mx * mx;
+    double alpha = s->coul_alpha;
+    double a4 = 0.25 / (alpha * alpha);
+    double volume = s->L[0] * s->L[1] * s->L[2];
+    double four_pi_v = 4.0 * kPi / volume;
+    std::vector<int4> idx;
+    std::vector<double> efac;
+    std::vector<double4> field;
+    std::vector<double> vf[6];
+    auto push = [&](int x, int y, int z) {
+        double k[3];
+        kvec(x, y, z, k);
+        double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        if (k2 > kmax2) return;
+        double energy = four_pi_v * std::exp(-k2 * a4) / k2;
+        idx.push_back(make_int4(x, y, z, 0));
+        efac.push_back(energy);
+        field.push_back(make_double4((2.0 * energy) * k[0], (2.0 * energy) * k[1], (2.0 * energy) * k[2], 0.0));
+        double f = -2.0 * (1.0 / k2 + a4);
+        // virial = energy * (I + f k (x) k); components xx yy zz xy xz yz
+        vf[0].push_back(energy * (1.0 + f * (k[0] * k[0])));
+        vf[1].push_back(energy * (1.0 + f * (k[1] * k[1])));
+        vf[2].push_back(energy * (1.0 + f * (k[2] * k[2])));
+        vf[3].push_back(energy * (0.0 + f * (k[0] * k[1])));
+        vf[4].push_back(energy * (0.0 + f * (k[0] * k[2])));
+        vf[5].push_back(energy * (0.0 + f * (k[1] * k[2])));
+    };
+    for (int x = 1; x < kmax; x++)
+        for (int y = -kmax; y < kmax; y++)
+            for (int z = -kmax; z < kmax; z++) push(x, y, z);
+    for (int y = 1; y < kmax; y++)
+        for (int z = -kmax; z < kmax; z++) push(0, y, z);
+    for (int z = 1; z < kmax; z++) push(0, 0, z);
+    s->nk = int(idx.size());
+    std::vector<KRow> rows;
+    for (int k = 0; k < s->nk; k++) {
+        if (rows.empty() || rows.back().ikx != idx[k].x || rows.back().iky != idx[k].y ||
+            idx[k - 1].z + 1 != idx[k].z)
+            rows.push_back(KRow{idx[k].x, idx[k].y, k, 0});
+        rows.back().count++;
+    }
+    s->nrows = int(rows.size());
+    std::vector<double> vflat;
+    for (int q = 0; q < 6; q++) vflat.insert(vflat.end(), vf[q].begin(), vf[q].end());
+    hipStream_t st = S(s);
+    LG_TRY(upload(s->kidx, idx, st));
+    LG_TRY(upload(s->rows, rows, st));
+    LG_TRY(upload(s->efac, efac, st));
+    LG_TRY(upload(s->vfac, vflat, st));
+    LG_TRY(upload(s->field, field, st));
+    LG_TRY(s->rho.ensure(s->nk));
+    // LDS budget for the phase tables
+    size_t rows_bytes = size_t(phase_rows(kmax)) * sizeof(double2);
+    size_t maxs = s->ctx->max_shared;
+    s->rho_width = 64;
+    while (s->rho_width > 8 && rows_bytes * s->rho_width + s->rho_width * sizeof(double) > maxs) s->rho_width /= 2;
+    s->force_width = 64;
+    while (s->force_width > 8 && rows_bytes * s->force_width > maxs) s->force_width /= 2;
+    if (rows_bytes * 8 > maxs) return fail(LG_E_UNSUPPORTED, "kmax too large for the k-space LDS tables");
+    int nchunks = blocks_for(s->n, s->rho_width);
+    LG_TRY(s->rho_partial.ensure(size_t(nchunks) * s->nk));
+    LG_TRY(s->epart_k.ensure(size_t(blocks_for(s->nk, 256)) * 7));
+    return LG_OK;
+}
+
+double max_cutoff(lg_system* s) {
+    double rc = 0.0;
+    for (auto& [k, p] : s->pairs)
+        if (p.pot != LG_POT_NULL) rc = std::max(rc, p.rc);
+    if (s->coul_kind != COUL_NONE) rc = std::max(rc, s->coul_rc);
+    return rc;
+}
+
+int prepare_nlist(lg_system* s) {
+    size_t n = s->n;
+    double rc = max_cutoff(s);
+    s->skin = rc > 0 ? 0.08 * rc : 0.0;
+    s->rlist = rc + s->skin;
+    Box& b = s->box;
+    b.periodic = s->periodic ? 1 : 0;
+    for (int d = 0; d < 3; d++) {
+        b.L[d] = s->periodic ? s->L[d] : 0.0;
+        b.inv[d] = s->periodic ? 1.0 / s->L[d] : 0.0;
+    }
+    Grid& g = s->grid;
+    for (int d = 0; d < 3; d++) {
+        int nc = 1;
+        if (s->periodic && s->rlist > 0) {
+            nc = int(std::floor(s->L[d] / (s->rlist * (1.0 + 1e-12))));
+            if (nc < 3) nc = 1;
+            nc = std::min(nc, 256);
+        }
+        g.n[d] = nc;
+        g.size[d] = s->periodic ? s->L[d] / nc : 0.0;
+    }
+    g.ncell = g.n[0] * g.n[1] * g.n[2];
+    double rl2 = s->rlist * s->rlist * (1.0 + 1e-9);
+    if (!s->periodic || s->rlist <= 0) rl2 = INFINITY;
+    // capacity from the mean density, with margin; exact sizes are checked
+    // after every build (overflow flag) and grown on demand.
+    size_t capacity = n;
+    if (s->periodic && s->rlist > 0) {
+        double volume = s->L[0] * s->L[1] * s->L[2];
+        double expect = 4.0 / 3.0 * kPi * s->rlist * s->rlist * s->rlist * double(n) / volume;
+        capacity = size_t(std::ceil(expect * 1.5 + 64));
+        capacity = std::min(capacity, n);
+    }
+    if (s->cap > int(capacity)) capacity = size_t(s->cap);
+    s->cap = int(std::max<size_t>(capacity, 1));
+    LG_TRY(s->cellid.ensure(n));
+    LG_TRY(s->slot.ensure(n));
+    LG_TRY(s->count.ensure(g.ncell));
+    LG_TRY(s->start.ensure(g.ncell + 1));
+    LG_TRY(s->tmp.ensure(n));
+    LG_TRY(s->sorted.ensure(n));
+    LG_TRY(s->nbr.ensure(size_t(s->cap) * n));
+    LG_TRY(s->nnbr.ensure(n));
+    LG_TRY(s->flags.ensure(2));
+    LG_TRY(s->ref.ensure(n));
+    LG_TRY(s->epart_pairs.ensure(size_t(blocks_for(n, kPairBlock)) * 8));
+    int one[2] = {1, 0};  // force a rebuild, clear overflow
+    LG_HIP(hipMemcpyAsync(s->flags.p, one, sizeof(one), hipMemcpyHostToDevice, S(s)));
+    s->nlist_dirty = false;
+    (void)rl2;
+    return LG_OK;
+}
+
+double rlist2_of(lg_system* s) {
+    if (!s->periodic || s->rlist <= 0) return INFINITY;
+    return s->rlist * s->rlist * (1.0 + 1e-9);
+}
+
+int prepare(lg_system* s) {
+    if (s->n == 0) return fail(LG_E_INVALID, "empty system");
+    if (s->topo_dirty) LG_TRY(prepare_topology(s));
+    if (s->meta_dirty) LG_TRY(prepare_meta(s));
+    if (s->params_dirty) LG_TRY(prepare_params(s));
+    if (s->kspace_dirty) LG_TRY(prepare_kspace(s));
+    if (s->periodic) {
+        double half = std::min(s->L[0], std::min(s->L[1], s->L[2])) / 2.0;
+        for (auto& [k, p] : s->pairs)
+            if (p.rc > half)
+                return fail(LG_E_INVALID, "The cutoff for this pair potential is bigger than half the cell length");
+    }
+    if (s->nlist_dirty) LG_TRY(prepare_nlist(s));
+    return LG_OK;
+}
+
+// ---- kernel sequences --------------------------------------------------------------------
+int launch_nlist(lg_system* s, bool force_rebuild) {
+    hipStream_t st = S(s);
+    int n = int(s->n);
+    int* flag = s->flags.p;
+    if (force_rebuild) {
+        int one = 1;
+        LG_HIP(hipMemcpyAsync(flag, &one, sizeof(int), hipMemcpyHostToDevice, st));
+    } else {
+        double hs = 0.5 * s->skin;
+        k_nlist_check<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->ref.p, n, hs * hs, flag);
+    }
+    const Grid& g = s->grid;
+    k_cell_zero<<<blocks_for(g.ncell, 256), 256, 0, st>>>(s->count.p, g.ncell, flag);
+    k_cell_count<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, n, g, s->box, s->cellid.p, s->slot.p, s->count.p,
+                                                      flag);
+    k_cell_scan<<<1, 1024, 0, st>>>(s->count.p, s->start.p, g.ncell, flag);
+    k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(s->cellid.p, s->slot.p, s->start.p, s->tmp.p, n, flag);
+    k_cell_rank<<<g.ncell, 256, 0, st>>>(s->tmp.p, s->start.p, s->sorted.p, g.ncell, flag);
+    k_nlist_build<<<blocks_for(n, 128), 128, 0, st>>>(s->pos.p, s->meta.p, s->sorted.p, s->cellid.p, s->start.p, g,
+                                                       s->box, rlist2_of(s), n, s->cap, s->nbr.p, s->nnbr.p,
+                                                       flag + 1, s->ref.p, flag);
+    k_flag_clear<<<1, 64, 0, st>>>(flag);
+    LG_HIP(hipGetLastError());
+    return LG_OK;
+}
+
+int launch_forces(lg_system* s, bool energy) {
+    hipStream_t st = S(s);
+    int n = int(s->n);
+    PairArgs pa;
+    pa.pos = s->pos.p;
+    pa.meta = s->meta.p;
+    pa.sorted = s->sorted.p;
+    pa.nbr = s->nbr.p;
+    pa.nnbr = s->nnbr.p;
+    pa.table = s->table.p;
+    pa.nkinds = int(s->nkinds);
+    pa.coul = s->coul;
+    pa.box = s->box;
+    pa.n = n;
+    pa.force = s->force.p;
+    pa.epart = s->epart_pairs.p;
+    if (energy)
+        k_pairs<true><<<blocks_for(n, kPairBlock), kPairBlock, 0, st>>>(pa);
+    else
+        k_pairs<false><<<blocks_for(n, kPairBlock), kPairBlock, 0, st>>>(pa);
+    if (s->nmol_multi > 0) {
+        MolArgs ma;
+        ma.pos = s->pos.p;
+        ma.meta = s->meta.p;
+        ma.nmol_multi = s->nmol_multi;
+        ma.mol_list = s->mol_list.p;
+        ma.mol_start = s->d_mol_start.p;
+        ma.ipairs = s->ipairs.p;
+        ma.ip_start = s->ip_start.p;
+        ma.bonds = s->d_bonds.p;
+        ma.b_start = s->b_start.p;
+        ma.angles = s->d_angles.p;
+        ma.a_start = s->a_start.p;
+        ma.dihedrals = s->d_dihedrals.p;
+        ma.d_param = s->d_param.p;
+        ma.d_start = s->d_start.p;
+        ma.table = s->table.p;
+        ma.nkinds = int(s->nkinds);
+        ma.pair_restr = s->pair_restr.p;
+        ma.pair_s14 = s->pair_s14.p;
+        ma.bparam = s->bparam.p;
+        ma.aparam = s->aparam.p;
+        ma.dparam = s->dparam.p;
+        ma.coul = s->coul;
+        ma.coul_restr = s->coul_restr;
+        ma.coul_s14 = s->coul_s14;
+        ma.box = s->box;
+        ma.force = s->force.p;
+        ma.epart = s->epart_mol.p;
+        int nb = blocks_for(s->nmol_multi, 256);
+        if (energy)
+            k_molecules<true><<<nb, 256, 0, st>>>(ma);
+        else
+            k_molecules<false><<<nb, 256, 0, st>>>(ma);
+    }
+    if (s->coul_kind == COUL_EWALD && s->nk > 0) {
+        int kmax = s->coul_kmax;
+        size_t rows_bytes = size_t(phase_rows(kmax)) * sizeof(double2);
+        int w = s->rho_width;
+        int nchunks = blocks_for(n, w);
+        dim3 grid(blocks_for(s->nk, kRhoBlock), nchunks);
+        size_t shm = rows_bytes * w + w * sizeof(double);
+        k_rho_partial<<<grid, kRhoBlock, shm, st>>>(s->pos.p, n, s->box, kmax, w, s->kidx.p, s->nk,
+                                                    s->rho_partial.p);
+        if (energy)
+            k_rho_reduce<true><<<blocks_for(s->nk, 256), 256, 0, st>>>(s->rho_partial.p, nchunks, s->nk, s->rho.p,
+                                                                      s->efac.p, s->vfac.p, s->epart_k.p);
+        else
+            k_rho_reduce<false><<<blocks_for(s->nk, 256), 256, 0, st>>>(s->rho_partial.p, nchunks, s->nk, s->rho.p,
+                                                                       s->efac.p, s->vfac.p, s->epart_k.p);
+        int fw = s->force_width;
+        k_kspace_force<<<blocks_for(n, fw), fw, rows_bytes * fw, st>>>(s->pos.p, n, s->box, kmax, s->rows.p,
+                                                                       s->nrows, s->kidx.p, s->rho.p, s->field.p,
+                                                                       s->force.p);
+    }
+    LG_HIP(hipGetLastError());
+    return LG_OK;
+}
+
+int check_overflow(lg_system* s, bool* overflowed) {
+    int ov = 0;
+    LG_HIP(hipMemcpyAsync(&ov, s->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, S(s)));
+    LG_HIP(hipStreamSynchronize(S(s)));
+    *overflowed = ov > 0;
+    if (ov > 0) {
+        s->cap = int(std::min<size_t>(s->n, size_t(ov) + size_t(ov) / 4 + 32));
+        LG_TRY(s->nbr.ensure(size_t(s->cap) * s->n));
+        int z[2] = {1, 0};
+        LG_HIP(hipMemcpyAsync(s->flags.p, z, sizeof(z), hipMemcpyHostToDevice, S(s)));
+    }
+    return LG_OK;
+}
+
+// Forces (and optionally energy partials) for the current positions.
+int compute(lg_system* s, bool energy) {
+    LG_TRY(prepare(s));
+    for (int attempt = 0; attempt < 4; attempt++) {
+        LG_TRY(launch_nlist(s, false));
+        LG_TRY(launch_forces(s, energy));
+        bool ov = false;
+        LG_TRY(check_overflow(s, &ov));
+        if (!ov) return LG_OK;
+    }
+    return fail(LG_E_DEVICE, "neighbour list overflow could not be resolved");
+}
+
+struct Energies {
+    double pairs = 0, coul_real = 0, bonds = 0, angles = 0, dihedrals = 0, kspace = 0;
+    double w[6] = {0, 0, 0, 0, 0, 0};      // pairs + coulomb real + bonds
+    double wk[6] = {0, 0, 0, 0, 0, 0};     // k-space
+};
+
+int gather_energies(lg_system* s, Energies& E) {
+    hipStream_t st = S(s);
+    int nbp = blocks_for(s->n, kPairBlock);
+    std::vector<double> hp(size_t(nbp) * 8);
+    LG_HIP(hipMemcpyAsync(hp.data(), s->epart_pairs.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    std::vector<double> hm;
+    int nbm = s->nmol_multi > 0 ? blocks_for(s->nmol_multi, 256) : 0;
+    if (nbm) {
+        hm.resize(size_t(nbm) * 11);
+        LG_HIP(hipMemcpyAsync(hm.data(), s->epart_mol.p, hm.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    std::vector<double> hk;
+    int nbk = (s->coul_kind == COUL_EWALD && s->nk > 0) ? blocks_for(s->nk, 256) : 0;
+    if (nbk) {
+        hk.resize(size_t(nbk) * 7);
+        LG_HIP(hipMemcpyAsync(hk.data(), s->epart_k.p, hk.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    LG_HIP(hipStreamSynchronize(st));
+    for (int b = 0; b < nbp; b++) {
+        E.pairs += hp[b * 8 + 0];
+        E.coul_real += hp[b * 8 + 1];
+        for (int q = 0; q < 6; q++) E.w[q] += hp[b * 8 + 2 + q];
+    }
+    for (int b = 0; b < nbm; b++) {
+        E.pairs += hm[b * 11 + 0];
+        E.coul_real += hm[b * 11 + 1];
+        E.bonds += hm[b * 11 + 2];
+        E.angles += hm[b * 11 + 3];
+        E.dihedrals += hm[b * 11 + 4];
+        for (int q = 0; q < 6; q++) E.w[q] += hm[b * 11 + 5 + q];
+    }
+    for (int b = 0; b < nbk; b++) {
+        E.kspace += hk[b * 7 + 0];
+        for (int q = 0; q < 6; q++) E.wk[q] += hk[b * 7 + 1 + q];
+    }
+    E.kspace /= kFourPiEps0;
+    for (int q = 0; q < 6; q++) E.wk[q] /= kFourPiEps0;
+    return LG_OK;
+}
+
+// composition.rs:123-128 then energy.rs:63-79 / compute.rs:218-228
+void tail_terms(lg_system* s, double* e_tail, double* w_tail_diag) {
+    *e_tail = 0.0;
+    *w_tail_diag = 0.0;
+    if (!s->periodic) return;
+    std::vector<double> count(s->nkinds, 0.0);
+    for (size_t i = 0; i < s->n; i++) count[s->kind[i]] += 1.0;
+    double volume = s->L[0] * s->L[1] * s->L[2];
+    for (uint32_t a = 0; a < s->nkinds; a++) {
+        if (count[a] == 0) continue;
+        for (uint32_t b = 0; b < s->nkinds; b++) {
+            if (count[b] == 0) continue;
+            double tpd = 2.0 * kPi * count[a] * count[b] / volume;
+            auto it = s->pairs.find({std::min(a, b), std::max(a, b)});
+            if (it == s->pairs.end()) continue;
+            *e_tail += tpd * host_tail_energy(it->second);
+            *w_tail_diag += tpd * (host_tail_virial(it->second) * (1.0 / 3.0));
+        }
+    }
+}
+
+double coulomb_self(lg_system* s) {
+    if (s->coul_kind == COUL_EWALD) {
+        double q2 = 0.0;
+        for (size_t i = 0; i < s->n; i++) q2 += s->charge[i] * s->charge[i];
+        return -s->coul_alpha / std::sqrt(kPi) * q2 / kFourPiEps0;
+    }
+    if (s->coul_kind == COUL_WOLF) {
+        double e = 0.0;
+        const CoulParam& c = s->coul;
+        for (size_t i = 0; i < s->n; i++) {
+            double q = s->charge[i];
+            if (q == 0.0) continue;
+            e -= q * q * 0.5 * (c.e_const + c.alpha * kFrac2SqrtPi) / kFourPiEps0;
+        }
+        return e;
+    }
+    return 0.0;
+}
+
+int push_vec3(DevBuf<double4>& buf, size_t n, const double* xyz, const std::vector<double>& w, hipStream_t st) {
+    std::vector<double4> h(n);
+    for (size_t i = 0; i < n; i++) h[i] = make_double4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], w[i]);
+    LG_HIP(hipMemcpyAsync(buf.p, h.data(), n * sizeof(double4), hipMemcpyHostToDevice, st));
+    LG_HIP(hipStreamSynchronize(st));
+    return LG_OK;
+}
+
+int pull_vec3(DevBuf<double4>& buf, size_t n, double* xyz, hipStream_t st, bool accumulate = false) {
+    std::vector<double4> h(n);
+    LG_HIP(hipMemcpyAsync(h.data(), buf.p, n * sizeof(double4), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipStreamSynchronize(st));
+    for (size_t i = 0; i < n; i++) {
+        if (accumulate) {
+            xyz[3 * i] += h[i].x;
+            xyz[3 * i + 1] += h[i].y;
+            xyz[3 * i + 2] += h[i].z;
+        } else {
+            xyz[3 * i] = h[i].x;
+            xyz[3 * i + 1] = h[i].y;
+            xyz[3 * i + 2] = h[i].z;
+        }
+    }
+    return LG_OK;
+}
+
+// replace the .w lane of a double4 buffer
+__global__ void k_set_w(double4* __restrict__ b, const double* __restrict__ w, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i].w = w[i];
+}
+
+int set_w(lg_system* s, DevBuf<double4>& b, const std::vector<double>& w) {
+    DevBuf<double> tmp;
+    LG_TRY(tmp.ensure(s->n));
+    LG_HIP(hipMemcpyAsync(tmp.p, w.data(), s->n * sizeof(double), hipMemcpyHostToDevice, S(s)));
+    k_set_w<<<blocks_for(s->n, 256), 256, 0, S(s)>>>(b.p, tmp.p, int(s->n));
+    LG_HIP(hipGetLastError());
+    LG_HIP(hipStreamSynchronize(S(s)));
+    tmp.release();
+    return LG_OK;
+}
+
+}  // namespace
+
+// ===================================================================================
+// C ABI
+// ===================================================================================
+extern "C" {
+
+const char* lg_last_error(void) { return g_error.c_str(); }
+const char* lg_version(void) { return "lumol_gpu 0.1 gfx950"; }
+
+int lg_context_create(int device, lg_context** out) {
+    if (!out) return fail(LG_E_INVALID, "null output pointer");
+    int count = 0;
+    LG_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(LG_E_INVALID, "device index out of range");
+    LG_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    LG_HIP(hipGetDeviceProperties(&prop, device));
+    lg_context* c = new lg_context();
+    c->device = device;
+    c->max_shared = prop.sharedMemPerBlock;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(LG_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return LG_OK;
+}
+
+int lg_context_destroy(lg_context* ctx) {
+    if (!ctx) return LG_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return LG_OK;
+}
+
+int lg_system_create(lg_context* ctx, size_t natoms, lg_system** out) {
+    if (!ctx || !out) return fail(LG_E_INVALID, "null argument");
+    if (natoms == 0 || natoms > (size_t(1) << 30)) return fail(LG_E_INVALID, "invalid number of atoms");
+    LG_HIP(hipSetDevice(ctx->device));
+    lg_system* s = new lg_system();
+    s->ctx = ctx;
+    s->n = natoms;
+    s->mass.assign(natoms, 1.0);
+    s->charge.assign(natoms, 0.0);
+    s->kind.assign(natoms, 0);
+    int r;
+    if ((r = s->pos.ensure(natoms)) || (r = s->vel.ensure(natoms)) || (r = s->acc.ensure(natoms)) ||
+        (r = s->force.ensure(natoms)) || (r = s->lambda.ensure(1)) ||
+        (r = s->ke_part.ensure(size_t(blocks_for(natoms, 256))))) {
+        delete s;
+        return r;
+    }
+    LG_HIP(hipMemsetAsync(s->pos.p, 0, natoms * sizeof(double4), ctx->stream));
+    LG_HIP(hipMemsetAsync(s->acc.p, 0, natoms * sizeof(double4), ctx->stream));
+    LG_HIP(hipMemsetAsync(s->force.p, 0, natoms * sizeof(double4), ctx->stream));
+    std::vector<double4> v(natoms, make_double4(0, 0, 0, 1.0));
+    LG_HIP(hipMemcpyAsync(s->vel.p, v.data(), natoms * sizeof(double4), hipMemcpyHostToDevice, ctx->stream));
+    LG_HIP(hipStreamSynchronize(ctx->stream));
+    *out = s;
+    return LG_OK;
+}
+
+int lg_system_destroy(lg_system* s) {
+    if (!s) return LG_OK;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    DevBuf<int>* ibufs[] = {&s->cellid, &s->slot, &s->count, &s->start, &s->tmp, &s->sorted, &s->nbr, &s->nnbr,
+                            &s->flags, &s->pair_restr, &s->mol_list, &s->d_mol_start, &s->ip_start, &s->b_start,
+                            &s->a_start, &s->d_start, &s->d_param};
+    for (auto* b : ibufs) b->release();
+    DevBuf<double4>* d4[] = {&s->pos, &s->vel, &s->acc, &s->force, &s->ref, &s->field,
+                             &s->snap_pos, &s->snap_vel, &s->snap_acc};
+    for (auto* b : d4) b->release();
+    DevBuf<double>* d1[] = {&s->pair_s14, &s->efac, &s->vfac, &s->epart_pairs, &s->epart_mol, &s->epart_k,
+                            &s->ke_part, &s->lambda};
+    for (auto* b : d1) b->release();
+    s->meta.release();
+    s->table.release();
+    s->ipairs.release();
+    s->d_bonds.release();
+    s->d_angles.release();
+    s->d_dihedrals.release();
+    s->bparam.release();
+    s->aparam.release();
+    s->dparam.release();
+    s->kidx.release();
+    s->rows.release();
+    s->rho.release();
+    s->rho_partial.release();
+    delete s;
+    return LG_OK;
+}
+
+int lg_system_size(const lg_system* s, size_t* n) {
+    if (!s || !n) return fail(LG_E_INVALID, "null argument");
+    *n = s->n;
+    return LG_OK;
+}
+
+int lg_system_set_cell(lg_system* s, double a, double b, double c) {
+    if (!s) return fail(LG_E_INVALID, "null system");
+    if (a == 0.0 && b == 0.0 && c == 0.0) {
+        s->periodic = false;
+    } else {
+        if (!(a > 0.0 && b > 0.0 && c > 0.0) || !std::isfinite(a) || !std::isfinite(b) || !std::isfinite(c))
+            return fail(LG_E_INVALID, "Cell lengths must be positive");
+        s->periodic = true;
+    }
+    s->L[0] = a;
+    s->L[1] = b;
+    s->L[2] = c;
+    s->kspace_dirty = true;
+    s->nlist_dirty = true;
+    return LG_OK;
+}
+
+int lg_system_set_positions(lg_system* s, const double* xyz) {
+    if (!s || !xyz) return fail(LG_E_INVALID, "null argument");
+    for (size_t i = 0; i < 3 * s->n; i++)
+        if (!std::isfinite(xyz[i])) return fail(LG_E_NONFINITE, "non-finite position");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(push_vec3(s->pos, s->n, xyz, s->charge, S(s)));
+    if (!s->nlist_dirty && s->flags.p) {
+        int one = 1;
+        LG_HIP(hipMemcpy(s->flags.p, &one, sizeof(int), hipMemcpyHostToDevice));
+    }
+    return LG_OK;
+}
+
+int lg_system_get_positions(lg_system* s, double* xyz) {
+    if (!s || !xyz) return fail(LG_E_INVALID, "null argument");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    return pull_vec3(s->pos, s->n, xyz, S(s));
+}
+
+int lg_system_set_velocities(lg_system* s, const double* v) {
+    if (!s || !v) return fail(LG_E_INVALID, "null argument");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    return push_vec3(s->vel, s->n, v, s->mass, S(s));
+}
+
+int lg_system_get_velocities(lg_system* s, double* v) {
+    if (!s || !v) return fail(LG_E_INVALID, "null argument");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    return pull_vec3(s->vel, s->n, v, S(s));
+}
+
+int lg_system_set_masses(lg_system* s, const double* m) {
+    if (!s || !m) return fail(LG_E_INVALID, "null argument");
+    for (size_t i = 0; i < s->n; i++)
+        if (!(m[i] > 0.0) || !std::isfinite(m[i])) return fail(LG_E_INVALID, "masses must be positive");
+    s->mass.assign(m, m + s->n);
+    LG_HIP(hipSetDevice(s->ctx->device));
+    return set_w(s, s->vel, s->mass);
+}
+
+int lg_system_set_charges(lg_system* s, const double* q) {
+    if (!s || !q) return fail(LG_E_INVALID, "null argument");
+    for (size_t i = 0; i < s->n; i++)
+        if (!std::isfinite(q[i])) return fail(LG_E_NONFINITE, "non-finite charge");
+    s->charge.assign(q, q + s->n);
+    LG_HIP(hipSetDevice(s->ctx->device));
+    return set_w(s, s->pos, s->charge);
+}
+
+int lg_system_set_kinds(lg_system* s, const uint32_t* kinds, uint32_t nkinds) {
+    if (!s || !kinds || nkinds == 0) return fail(LG_E_INVALID, "null argument");
+    if (nkinds > uint32_t(kMaxKinds)) return fail(LG_E_UNSUPPORTED, "more than 16 particle kinds");
+    for (size_t i = 0; i < s->n; i++)
+        if (kinds[i] >= nkinds) return fail(LG_E_INVALID, "kind index out of range");
+    s->kind.assign(kinds, kinds + s->n);
+    s->nkinds = nkinds;
+    s->meta_dirty = true;
+    s->params_dirty = true;
+    s->topo_dirty = true;
+    return LG_OK;
+}
+
+int lg_system_set_bonds(lg_system* s, size_t nbonds, const uint32_t* ij) {
+    if (!s || (nbonds && !ij)) return fail(LG_E_INVALID, "null argument");
+    std::vector<std::pair<uint32_t, uint32_t>> b;
+    for (size_t k = 0; k < nbonds; k++) {
+        uint32_t i = ij[2 * k], j = ij[2 * k + 1];
+        if (i >= s->n || j >= s->n || i == j) return fail(LG_E_INVALID, "invalid bond");
+        b.push_back({std::min(i, j), std::max(i, j)});
+    }
+    std::sort(b.begin(), b.end());
+    b.erase(std::unique(b.begin(), b.end()), b.end());
+    std::swap(s->bonds, b);
+    s->topo_dirty = true;
+    LG_HIP(hipSetDevice(s->ctx->device));
+    std::vector<std::tuple<int, int, int>> ip;
+    int r = build_topology(s, ip);  // validate now, as System::add_bond does
+    if (r != LG_OK) {
+        std::swap(s->bonds, b);
+        s->topo_dirty = true;
+        return r;
+    }
+    s->nlist_dirty = true;
+    return LG_OK;
+}
+
+int lg_system_molecule_count(const lg_system* s, size_t* nmol) {
+    if (!s || !nmol) return fail(LG_E_INVALID, "null argument");
+    if (s->topo_dirty) {
+        lg_system* ms = const_cast<lg_system*>(s);
+        std::vector<std::tuple<int, int, int>> ip;
+        LG_TRY(build_topology(ms, ip));
+    }
+    *nmol = size_t(s->nmol);
+    return LG_OK;
+}
+
+int lg_system_set_precision(lg_system* s, int precision) {
+    if (!s) return fail(LG_E_INVALID, "null system");
+    if (precision != LG_PRECISION_FP64 && precision != LG_PRECISION_MIXED)
+        return fail(LG_E_INVALID, "unknown precision");
+    if (precision == LG_PRECISION_MIXED) return fail(LG_E_UNSUPPORTED, "mixed precision form not built yet");
+    s->precision = precision;
+    return LG_OK;
+}
+
+static int check_pot(int pot) {
+    if (pot != LG_POT_NULL && pot != LG_POT_LJ && pot != LG_POT_HARMONIC)
+        return fail(LG_E_UNSUPPORTED, "potential kind not in the GPU kernel set");
+    return LG_OK;
+}
+
+int lg_pair_set(lg_system* s, uint32_t ki, uint32_t kj, int pot, const double* params, double cutoff, int shifted,
+                int tail, int restriction, double scale14) {
+    if (!s) return fail(LG_E_INVALID, "null system");
+    LG_TRY(check_pot(pot));
+    if (ki >= s->nkinds || kj >= s->nkinds) return fail(LG_E_INVALID, "kind index out of range");
+    if (pot != LG_POT_NULL && !params) return fail(LG_E_INVALID, "missing potential parameters");
+    if (restriction < 0 || restriction > 6) return fail(LG_E_INVALID, "unknown restriction");
+    if (!(cutoff >= 0.0) || !std::isfinite(cutoff)) return fail(LG_E_INVALID, "invalid cutoff");
+    if (s->periodic) {
+        double half = std::min(s->L[0], std::min(s->L[1], s->L[2])) / 2.0;
+        if (cutoff > half)
+            return fail(LG_E_INVALID, "The cutoff for this pair potential is bigger than half the cell length");
+    }
+    PairSpec p{pot, params ? params[0] : 0.0, params ? params[1] : 0.0, cutoff, shifted, tail, restriction, scale14};
+    s->pairs[{std::min(ki, kj), std::max(ki, kj)}] = p;
+    s->params_dirty = true;
+    s->nlist_dirty = true;
+    return LG_OK;
+}
+
+int lg_bond_set(lg_system* s, uint32_t ki, uint32_t kj, int pot, const double* params) {
+    if (!s || !params) return fail(LG_E_INVALID, "null argument");
+    LG_TRY(check_pot(pot));
+    s->bond_pots[{std::min(ki, kj), std::max(ki, kj)}] = BondedSpec{pot, params[0], params[1]};
+    s->topo_dirty = true;
+    return LG_OK;
+}
+
+int lg_angle_set(lg_system* s, uint32_t ki, uint32_t kj, uint32_t kk, int pot, const double* params) {
+    if (!s || !params) return fail(LG_E_INVALID, "null argument");
+    LG_TRY(check_pot(pot));
+    if (pot == LG_POT_LJ) return fail(LG_E_UNSUPPORTED, "Lennard-Jones is not an angle potential");
+    s->angle_pots[{std::min(ki, kk), kj, std::max(ki, kk)}] = BondedSpec{pot, params[0], params[1]};
+    s->topo_dirty = true;
+    return LG_OK;
+}
+
+int lg_dihedral_set(lg_system* s, uint32_t ki, uint32_t kj, uint32_t kk, uint32_t km, int pot,
+                    const double* params) {
+    if (!s || !params) return fail(LG_E_INVALID, "null argument");
+    LG_TRY(check_pot(pot));
+    if (pot == LG_POT_LJ) return fail(LG_E_UNSUPPORTED, "Lennard-Jones is not a dihedral potential");
+    uint32_t ij = std::max(ki, kj), kmm = std::max(kk, km);
+    std::tuple<uint32_t, uint32_t, uint32_t, uint32_t> key;
+    if (ij == kmm)
+        key = std::min(ki, kj) < std::min(kk, km) ? std::make_tuple(ki, kj, kk, km) : std::make_tuple(km, kk, kj, ki);
+    else if (ij < kmm)
+        key = std::make_tuple(ki, kj, kk, km);
+    else
+        key = std::make_tuple(km, kk, kj, ki);
+    s->dihedral_pots[key] = BondedSpec{pot, params[0], params[1]};
+    s->topo_dirty = true;
+    return LG_OK;
+}
+
+int lg_coulomb_ewald(lg_system* s, double cutoff, int kmax, double alpha, int restriction) {
+    if (!s) return fail(LG_E_INVALID, "null system");
+    if (std::isnan(alpha)) alpha = kPi / cutoff;  // ewald.rs:279
+    if (!(cutoff >= 0.0)) return fail(LG_E_INVALID, "the cutoff can not be negative in Ewald");
+    if (alpha < 0.0) return fail(LG_E_INVALID, "alpha can not be negative in Ewald");
+    if (kmax <= 0) return fail(LG_E_INVALID, "kmax can not be 0 in Ewald");
+    if (restriction == LG_RESTRICT_SCALE14)
+        return fail(LG_E_UNSUPPORTED, "Scaling restriction scheme using Ewald are not implemented");
+    if (restriction < 0 || restriction > 6) return fail(LG_E_INVALID, "unknown restriction");
+    s->coul_kind = COUL_EWALD;
+    s->coul_rc = cutoff;
+    s->coul_alpha = alpha;
+    s->coul_kmax = kmax;
+    s->coul_restr = restriction;
+    s->coul_s14 = 1.0;
+    s->params_dirty = s->kspace_dirty = s->nlist_dirty = true;
+    return LG_OK;
+}
+
+int lg_coulomb_wolf(lg_system* s, double cutoff, int restriction, double scale14) {
+    if (!s) return fail(LG_E_INVALID, "null system");
+    if (!(cutoff > 0.0)) return fail(LG_E_INVALID, "Got a negative cutoff in Wolf summation");
+    if (restriction < 0 || restriction > 6) return fail(LG_E_INVALID, "unknown restriction");
+    s->coul_kind = COUL_WOLF;
+    s->coul_rc = cutoff;
+    s->coul_alpha = kPi / cutoff;
+    s->coul_kmax = 0;
+    s->coul_restr = restriction;
+    s->coul_s14 = scale14;
+    s->params_dirty = s->kspace_dirty = s->nlist_dirty = true;
+    return LG_OK;
+}
+
+int lg_coulomb_clear(lg_system* s) {
+    if (!s) return fail(LG_E_INVALID, "null system");
+    s->coul_kind = COUL_NONE;
+    s->params_dirty = s->kspace_dirty = s->nlist_dirty = true;
+    return LG_OK;
+}
+
+int lg_forces(lg_system* s, double* out, int accumulate) {
+    if (!s || !out) return fail(LG_E_INVALID, "null argument");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(compute(s, false));
+    return pull_vec3(s->force, s->n, out, S(s), accumulate != 0);
+}
+
+int lg_potential_energy(lg_system* s, lg_energy_terms* out) {
+    if (!s || !out) return fail(LG_E_INVALID, "null argument");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(compute(s, true));
+    Energies E;
+    LG_TRY(gather_energies(s, E));
+    double e_tail, w_tail;
+    tail_terms(s, &e_tail, &w_tail);
+    double self = coulomb_self(s);
+    lg_energy_terms t{};
+    t.pairs = E.pairs;
+    t.pairs_tail = e_tail;
+    t.bonds = E.bonds;
+    t.angles = E.angles;
+    t.dihedrals = E.dihedrals;
+    t.coulomb_real = E.coul_real;
+    t.coulomb_self = self;
+    t.coulomb_kspace = E.kspace;
+    t.coulomb = s->coul_kind == COUL_NONE ? 0.0 : (E.coul_real + self + E.kspace);
+    t.global = 0.0;
+    double total = t.pairs;
+    total += t.pairs_tail;
+    total += t.bonds;
+    total += t.angles;
+    total += t.dihedrals;
+    total += t.coulomb;
+    total += t.global;
+    t.total = total;
+    *out = t;
+    if (!std::isfinite(total)) return fail(LG_E_NONFINITE, "Potential energy is infinite!");
+    return LG_OK;
+}
+
+int lg_virial(lg_system* s, double* w9) {
+    if (!s || !w9) return fail(LG_E_INVALID, "null argument");
+    if (!s->periodic) return fail(LG_E_INVALID, "Can not compute virial for infinite cell");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(compute(s, true));
+    Energies E;
+    LG_TRY(gather_energies(s, E));
+    double e_tail, w_tail;
+    tail_terms(s, &e_tail, &w_tail);
+    double w[6];
+    for (int q = 0; q < 6; q++) w[q] = E.w[q] + E.wk[q];
+    w[0] += w_tail;
+    w[1] += w_tail;
+    w[2] += w_tail;
+    double m[9] = {w[0], w[3], w[4], w[3], w[1], w[5], w[4], w[5], w[2]};
+    std::memcpy(w9, m, sizeof(m));
+    return LG_OK;
+}
+
+int lg_kinetic_energy(lg_system* s, double* out) {
+    if (!s || !out) return fail(LG_E_INVALID, "null argument");
+    std::vector<double> v(3 * s->n);
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(pull_vec3(s->vel, s->n, v.data(), S(s)));
+    double e = 0.0;  // compute.rs:137-142, sequential
+    for (size_t i = 0; i < s->n; i++)
+        e += 0.5 * s->mass[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+    *out = e;
+    if (!std::isfinite(e)) return fail(LG_E_NONFINITE, "Kinetic energy is infinite!");
+    return LG_OK;
+}
+
+int lg_temperature(lg_system* s, double* out) {
+    double ke;
+    LG_TRY(lg_kinetic_energy(s, &ke));
+    *out = 2.0 * ke / (double(3 * s->n) * kBoltzmann);
+    return LG_OK;
+}
+
+int lg_pairs_within(lg_system* s, double cutoff, uint32_t* pairs, uint64_t capacity, uint64_t* count) {
+    if (!s || !count || (capacity && !pairs)) return fail(LG_E_INVALID, "null argument");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(prepare(s));
+    if (!(cutoff > 0.0) || cutoff > s->rlist) return fail(LG_E_INVALID, "cutoff above the neighbour-list range");
+    LG_TRY(compute(s, false));
+    DevBuf<unsigned long long> cnt;
+    DevBuf<uint2> out;
+    LG_TRY(cnt.ensure(1));
+    LG_TRY(out.ensure(std::max<uint64_t>(capacity, 1)));
+    LG_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), S(s)));
+    int n = int(s->n);
+    k_pairs_within<<<blocks_for(n, 256), 256, 0, S(s)>>>(s->pos.p, s->sorted.p, s->nbr.p, s->nnbr.p, n, s->box,
+                                                         cutoff, out.p, capacity, cnt.p);
+    LG_HIP(hipGetLastError());
+    unsigned long long c = 0;
+    LG_HIP(hipMemcpyAsync(&c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, S(s)));
+    LG_HIP(hipStreamSynchronize(S(s)));
+    uint64_t w = std::min<uint64_t>(c, capacity);
+    if (w) LG_HIP(hipMemcpy(pairs, out.p, w * sizeof(uint2), hipMemcpyDeviceToHost));
+    *count = c;
+    cnt.release();
+    out.release();
+    return LG_OK;
+}
+
+int lg_ewald_kvector_count(lg_system* s, int* out) {
+    if (!s || !out) return fail(LG_E_INVALID, "null argument");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(prepare(s));
+    *out = s->nk;
+    return LG_OK;
+}
+
+int lg_md_setup(lg_system* s, int integrator, double timestep, double bT, double btau) {
+    if (!s) return fail(LG_E_INVALID, "null system");
+    if (integrator != LG_INTEGRATOR_VELOCITY_VERLET && integrator != LG_INTEGRATOR_LEAP_FROG)
+        return fail(LG_E_UNSUPPORTED, "unknown integrator");
+    if (!(timestep > 0.0)) return fail(LG_E_INVALID, "timestep must be positive");
+    if (bT > 0.0 && !(btau >= 1.0))
+        return fail(LG_E_INVALID, "The timestep must be larger than 1 in berendsen thermostat.");
+    s->integrator = integrator;
+    s->dt = timestep;
+    s->berendsen_T = bT > 0.0 ? bT : 0.0;
+    s->berendsen_tau = btau;
+    LG_HIP(hipSetDevice(s->ctx->device));
+    // Integrator::setup: accelerations start at zero (integrators.rs:40-42)
+    LG_HIP(hipMemsetAsync(s->acc.p, 0, s->n * sizeof(double4), S(s)));
+    LG_HIP(hipStreamSynchronize(S(s)));
+    s->md_ready = true;
+    return LG_OK;
+}
+
+// Steps in chunks; a chunk whose neighbour list overflowed is replayed from a
+// snapshot with a larger list.
+int lg_md_run(lg_system* s, uint64_t nsteps) {
+    if (!s) return fail(LG_E_INVALID, "null system");
+    if (!s->md_ready) return fail(LG_E_STATE, "lg_md_setup must be called before lg_md_run");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(prepare(s));
+    hipStream_t st = S(s);
+    int n = int(s->n);
+    bool thermo = s->berendsen_T > 0.0;
+    const uint64_t chunk = 200;
+    LG_TRY(s->snap_pos.ensure(s->n));
+    LG_TRY(s->snap_vel.ensure(s->n));
+    LG_TRY(s->snap_acc.ensure(s->n));
+    uint64_t done = 0;
+    int retries = 0;
+    while (done < nsteps) {
+        uint64_t todo = std::min<uint64_t>(chunk, nsteps - done);
+        LG_HIP(hipMemcpyAsync(s->snap_pos.p, s->pos.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+        LG_HIP(hipMemcpyAsync(s->snap_vel.p, s->vel.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+        LG_HIP(hipMemcpyAsync(s->snap_acc.p, s->acc.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+        for (uint64_t k = 0; k < todo; k++) {
+            const double* lam = (thermo && k > 0) ? s->lambda.p : nullptr;
+            if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
+                k_vv_first<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, lam);
+            else
+                k_lf_first<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, lam);
+            LG_TRY(launch_nlist(s, false));
+            LG_TRY(launch_forces(s, false));
+            double* kp = thermo ? s->ke_part.p : nullptr;
+            if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
+                k_vv_second<<<blocks_for(n, 256), 256, 0, st>>>(s->force.p, s->vel.p, s->acc.p, n, s->dt, kp);
+            else
+                k_lf_second<<<blocks_for(n, 256), 256, 0, st>>>(s->force.p, s->vel.p, s->acc.p, n, s->dt, kp);
+            if (thermo)
+                k_berendsen<<<1, 256, 0, st>>>(s->ke_part.p, blocks_for(n, 256), double(3 * s->n), s->berendsen_T,
+                                               s->berendsen_tau, s->lambda.p);
+        }
+        if (thermo) k_scale_vel<<<blocks_for(n, 256), 256, 0, st>>>(s->vel.p, n, s->lambda.p);
+        LG_HIP(hipGetLastError());
+        bool ov = false;
+        LG_TRY(check_overflow(s, &ov));
+        if (ov) {
+            if (++retries > 4) return fail(LG_E_DEVICE, "neighbour list overflow could not be resolved");
+            LG_HIP(hipMemcpyAsync(s->pos.p, s->snap_pos.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+            LG_HIP(hipMemcpyAsync(s->vel.p, s->snap_vel.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+            LG_HIP(hipMemcpyAsync(s->acc.p, s->snap_acc.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+            continue;
+        }
+        done += todo;
+    }
+    LG_HIP(hipStreamSynchronize(st));
+    return LG_OK;
+}
+
+int lg_cache_init(lg_system* s, double* energy) {
+    (void)s;
+    (void)energy;
+    return fail(LG_E_UNSUPPORTED, "Monte Carlo energy cache not built yet");
+}
+int lg_move_cost(lg_system* s, uint32_t molid, const double* newpos, double* cost) {
+    (void)s;
+    (void)molid;
+    (void)newpos;
+    (void)cost;
+    return fail(LG_E_UNSUPPORTED, "Monte Carlo energy cache not built yet");
+}
+int lg_move_commit(lg_system* s, int wrap) {
+    (void)s;
+    (void)wrap;
+    return fail(LG_E_UNSUPPORTED, "Monte Carlo energy cache not built yet");
+}
+int lg_move_discard(lg_system* s) {
+    (void)s;
+    return fail(LG_E_UNSUPPORTED, "Monte Carlo energy cache not built yet");
+}
+
+}  // extern "C"

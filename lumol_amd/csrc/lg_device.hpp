@@ -1,0 +1,156 @@
+// lg_device.hpp -- device-side parameter blocks and scalar physics for the
+// gfx950 force-and-energy kernels.
+//
+// The scalar functions restate Lumol's formulas (reference checkout paths):
+//   Lennard-Jones      lumol-core/src/energy/functions.rs:79-89
+//   Harmonic           lumol-core/src/energy/functions.rs:143-151
+//   PairInteraction    lumol-core/src/energy/pairs.rs:185-218 (r >= rc -> 0)
+//   minimum image      lumol-core/src/sys/config/cells.rs:284-300
+//   Ewald real space   lumol-core/src/energy/global/ewald.rs:387-428 (r > rc -> 0)
+//   Wolf               lumol-core/src/energy/global/wolf.rs:90-117 (r > rc -> 0)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lg {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kFrac2SqrtPi = 1.12837916709551257390;  // 2/sqrt(pi)
+// lumol-core/src/consts.rs:9,15
+constexpr double kBoltzmann = 8.31446284161522e-7;
+constexpr double kFourPiEps0 = 7.197589831304046;
+
+enum : int { POT_NONE = -1, POT_NULL = 0, POT_LJ = 1, POT_HARMONIC = 2 };
+// restrictions.rs:37-50
+enum : int { BP_NONE = 0, BP_SAME = 1, BP_ONE = 2, BP_TWO = 3, BP_THREE = 4, BP_FAR = 5 };
+enum : int { COUL_NONE = 0, COUL_EWALD = 1, COUL_WOLF = 2 };
+
+// One entry per (kind_i, kind_j), symmetric, K*K entries.
+struct PairParam {
+    int pot;           // POT_* ; POT_NONE when no interaction is set
+    int inter_excl;    // 1 when the restriction excludes inter-molecular pairs
+    double p0, p1;     // LJ {sigma, epsilon}; Harmonic {k, x0}
+    double rc;         // PairInteraction cutoff
+    double shift;      // V(rc) when shifted, else 0
+};
+
+struct CoulParam {
+    int kind;          // COUL_*
+    int inter_excl;    // restriction excludes inter-molecular pairs
+    double alpha, rc;
+    double e_const, f_const;  // Wolf energy_constant / force_constant
+};
+
+// Orthorhombic cell: lengths and reciprocal lengths.
+struct Box {
+    double L[3];
+    double inv[3];
+    int periodic;
+};
+
+// ---- geometry -----------------------------------------------------------------
+// Minimum image of d = r_i - r_j, bit-identical to
+// `d -= round(d / a) * a` (cells.rs:288-290).  The quotient is formed with the
+// reciprocal and re-done with an IEEE division only when it lies within a few
+// ulps of a half-integer, the only case where the two roundings can differ.
+#pragma clang fp contract(off)
+__device__ __forceinline__ double image1(double d, double L, double inv) {
+    double q = d * inv;
+    double n = round(q);
+    double frac = fabs(q - trunc(q));
+    if (fabs(frac - 0.5) < 1e-9) n = round(d / L);
+    return d - n * L;
+}
+
+__device__ __forceinline__ void min_image(const Box& b, double& dx, double& dy, double& dz) {
+    if (b.periodic) {
+        dx = image1(dx, b.L[0], b.inv[0]);
+        dy = image1(dy, b.L[1], b.inv[1]);
+        dz = image1(dz, b.L[2], b.inv[2]);
+    }
+}
+
+// Vector3D::norm2 order: x*x + y*y + z*z, left to right, unfused.
+__device__ __forceinline__ double norm2_ordered(double x, double y, double z) {
+    double a = x * x;
+    double b = y * y;
+    double c = z * z;
+    return (a + b) + c;
+}
+#pragma clang fp contract(on)
+
+// ---- pair potentials (FP64) ----------------------------------------------------------
+// energy and force / r  (force() returns -dV/dr; the vector force is (f/r) d).
+__device__ __forceinline__ void pot_eval(int pot, double p0, double p1, double r, double inv_r,
+                                         double& e, double& f_over_r) {
+    if (pot == POT_LJ) {
+        double x = p0 * inv_r;
+        double x2 = x * x;
+        double s6 = x2 * x2 * x2;
+        e = 4.0 * p1 * (s6 * s6 - s6);
+        f_over_r = -24.0 * p1 * (s6 - 2.0 * (s6 * s6)) * inv_r * inv_r;
+    } else if (pot == POT_HARMONIC) {
+        double dx = r - p1;
+        e = 0.5 * p0 * dx * dx;
+        f_over_r = p0 * (p1 - r) * inv_r;
+    } else {
+        e = 0.0;
+        f_over_r = 0.0;
+    }
+}
+
+// Ewald real-space pair: energy and the factor multiplying d in the force
+// vector (ewald.rs:387-428, force = force_pair(r) * rij at :490).  `excluded`
+// selects the k-space removal term (-erf) used for excluded pairs.
+__device__ __forceinline__ void ewald_real(double alpha, double qiqj, double r, double inv_r, bool excluded,
+                                           double& e, double& f_over_r) {
+    double ar = alpha * r;
+    double ex = exp(-alpha * alpha * r * r);
+    double g = alpha * kFrac2SqrtPi * ex;
+    double pref = qiqj / kFourPiEps0;
+    if (excluded) {
+        double erfv = erf(ar);
+        e = -pref * erfv * inv_r;
+        f_over_r = pref * inv_r * inv_r * (g - erfv * inv_r);
+    } else {
+        double erfcv = erfc(ar);
+        e = pref * erfcv * inv_r;
+        f_over_r = pref * inv_r * inv_r * (g + erfcv * inv_r);
+    }
+}
+
+// Wolf pair, energy and force/r (wolf.rs:90-117).
+__device__ __forceinline__ void wolf_pair(const CoulParam& c, double qiqj, double r, double inv_r, double& e,
+                                          double& f_over_r) {
+    double ar = c.alpha * r;
+    double erfcv = erfc(ar);
+    double ex = exp(-ar * ar);
+    e = qiqj * (erfcv * inv_r - c.e_const) / kFourPiEps0;
+    double factor = erfcv * inv_r * inv_r + c.alpha * kFrac2SqrtPi * ex * inv_r;
+    f_over_r = qiqj * (factor - c.f_const) * inv_r / kFourPiEps0;
+}
+
+// ---- restrictions (restrictions.rs:85-116) ------------------------------------------
+__device__ __host__ __forceinline__ void restriction_info(int restriction, double scale14, int path,
+                                                          bool& excluded, double& scaling) {
+    bool same = path != BP_NONE;
+    switch (restriction) {
+        case 0: excluded = false; break;                            // None
+        case 1: excluded = !same; break;                            // IntraMolecular
+        case 2: excluded = same; break;                             // InterMolecular
+        case 3: excluded = path == BP_ONE; break;                   // Exclude12
+        case 4: case 6: excluded = path == BP_ONE || path == BP_TWO; break;  // Exclude13, Scale14
+        case 5: excluded = path == BP_ONE || path == BP_TWO || path == BP_THREE; break;
+        default: excluded = false;
+    }
+    scaling = (restriction == 6 && path == BP_THREE) ? scale14 : 1.0;
+}
+
+// ---- wave reductions -----------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace lg

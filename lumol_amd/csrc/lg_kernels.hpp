@@ -1,0 +1,871 @@
+// lg_kernels.hpp -- gfx950 kernels of the FP64 force-and-energy path.
+//
+// Data layout in HBM (per system, N atoms, original atom order):
+//   pos   double4[N]  x, y, z, charge
+//   vel   double4[N]  vx, vy, vz, mass
+//   acc   double4[N]  ax, ay, az, -
+//   meta  int2[N]     kind, molecule id
+//   force double4[N]  fx, fy, fz, -
+// Neighbour search: a cell list (integer binning + per-cell rank sort) feeds
+// a Verlet list of inter-molecular candidates within rc + skin, stored
+// transposed (nbr[k * N + slot]) so consecutive lanes read consecutive words.
+// The list is rebuilt on the device whenever an atom has moved more than
+// skin/2 since the last build, which keeps every pair with r <= rc in the
+// list; each kernel then applies the reference's own cutoff test on the
+// FP64 distance, so the evaluated pair set is the one of the reference's
+// brute-force `for j in i+1..n` loops (lumol-core/src/sys/compute.rs:40).
+#pragma once
+
+#include "lg_device.hpp"
+
+namespace lg {
+
+// ---------------------------------------------------------------------------------
+// Neighbour-list maintenance
+// ---------------------------------------------------------------------------------
+struct Grid {
+    int n[3];        // cells per dimension (1 when fewer than 3 fit)
+    double size[3];  // cell edge
+    int ncell;
+};
+
+// Sets *flag when any atom moved more than skin/2 since the last build.
+__global__ void k_nlist_check(const double4* __restrict__ pos, const double4* __restrict__ ref, int n,
+                              double half_skin2, int* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool moved = false;
+    if (i < n) {
+        double dx = pos[i].x - ref[i].x;
+        double dy = pos[i].y - ref[i].y;
+        double dz = pos[i].z - ref[i].z;
+        moved = dx * dx + dy * dy + dz * dz > half_skin2;
+    }
+    if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+__global__ void k_cell_zero(int* __restrict__ count, int ncell, const int* __restrict__ flag) {
+    if (*flag == 0) return;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncell) count[c] = 0;
+}
+
+__device__ __forceinline__ int cell_coord(double x, double L, int nc, double csize, int periodic) {
+    if (nc == 1) return 0;
+    double w = periodic ? x - floor(x / L) * L : x;
+    int c = (int)floor(w / csize);
+    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+__global__ void k_cell_count(const double4* __restrict__ pos, int n, Grid g, Box b, int* __restrict__ cellid,
+                             int* __restrict__ slot, int* __restrict__ count, const int* __restrict__ flag) {
+    if (*flag == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 p = pos[i];
+    int cx = cell_coord(p.x, b.L[0], g.n[0], g.size[0], b.periodic);
+    int cy = cell_coord(p.y, b.L[1], g.n[1], g.size[1], b.periodic);
+    int cz = cell_coord(p.z, b.L[2], g.n[2], g.size[2], b.periodic);
+    int c = (cz * g.n[1] + cy) * g.n[0] + cx;
+    cellid[i] = c;
+    slot[i] = atomicAdd(&count[c], 1);
+}
+
+// Exclusive scan of the per-cell counts (one block, 1024 threads).
+__global__ void k_cell_scan(const int* __restrict__ count, int* __restrict__ start, int ncell,
+                            const int* __restrict__ flag) {
+    if (*flag == 0) return;
+    __shared__ int warp_sums[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < ncell; base += blockDim.x) {
+        int c = base + threadIdx.x;
+        int v = c < ncell ? count[c] : 0;
+        // inclusive scan within the wave
+        int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        int x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) warp_sums[wid] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int s = 0;
+            int nw = blockDim.x >> 6;
+            for (int w = 0; w < nw; w++) {
+                int t = warp_sums[w];
+                warp_sums[w] = s;
+                s += t;
+            }
+        }
+        __syncthreads();
+        int excl = carry + warp_sums[wid] + x - v;
+        if (c < ncell) start[c] = excl;
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) carry = excl + v;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[ncell] = carry;
+}
+
+__global__ void k_cell_scatter(const int* __restrict__ cellid, const int* __restrict__ slot,
+                               const int* __restrict__ start, int* __restrict__ tmp, int n,
+                               const int* __restrict__ flag) {
+    if (*flag == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tmp[start[cellid[i]] + slot[i]] = i;
+}
+
+// Deterministic order inside each cell: atoms by increasing index.  One block
+// per cell; an atom's rank is the number of smaller indices in its cell.
+__global__ void k_cell_rank(const int* __restrict__ tmp, const int* __restrict__ start, int* __restrict__ sorted,
+                            int ncell, const int* __restrict__ flag) {
+    if (*flag == 0) return;
+    int c = blockIdx.x;
+    if (c >= ncell) return;
+    int s = start[c], e = start[c + 1];
+    for (int a = s + threadIdx.x; a < e; a += blockDim.x) {
+        int v = tmp[a];
+        int rank = 0;
+        for (int b2 = s; b2 < e; b2++) rank += tmp[b2] < v;
+        sorted[s + rank] = v;
+    }
+}
+
+// Verlet list: slot t handles atom sorted[t]; candidates are the atoms of the
+// neighbouring cells that belong to another molecule and lie within rlist.
+__global__ void k_nlist_build(const double4* __restrict__ pos, const int2* __restrict__ meta,
+                              const int* __restrict__ sorted, const int* __restrict__ cellid,
+                              const int* __restrict__ start, Grid g, Box b, double rlist2, int n, int cap,
+                              int* __restrict__ nbr, int* __restrict__ nnbr, int* __restrict__ overflow,
+                              double4* __restrict__ ref, const int* __restrict__ flag) {
+    if (*flag == 0) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i = sorted[t];
+    double4 pi = pos[i];
+    int mi = meta[i].y;
+    int c = cellid[i];
+    int cx = c % g.n[0];
+    int cy = (c / g.n[0]) % g.n[1];
+    int cz = c / (g.n[0] * g.n[1]);
+    int rx = g.n[0] >= 3 ? 1 : 0, ry = g.n[1] >= 3 ? 1 : 0, rz = g.n[2] >= 3 ? 1 : 0;
+    int count = 0;
+    for (int dz = -rz; dz <= rz; dz++) {
+        int z = cz + dz;
+        z = z < 0 ? z + g.n[2] : (z >= g.n[2] ? z - g.n[2] : z);
+        for (int dy = -ry; dy <= ry; dy++) {
+            int y = cy + dy;
+            y = y < 0 ? y + g.n[1] : (y >= g.n[1] ? y - g.n[1] : y);
+            for (int dx = -rx; dx <= rx; dx++) {
+                int x = cx + dx;
+                x = x < 0 ? x + g.n[0] : (x >= g.n[0] ? x - g.n[0] : x);
+                int nc = (z * g.n[1] + y) * g.n[0] + x;
+                int s = start[nc], e = start[nc + 1];
+                for (int q = s; q < e; q++) {
+                    int j = sorted[q];
+                    if (meta[j].y == mi) continue;
+                    double4 pj = pos[j];
+                    double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
+                    if (b.periodic) {
+                        ddx -= rint(ddx * b.inv[0]) * b.L[0];
+                        ddy -= rint(ddy * b.inv[1]) * b.L[1];
+                        ddz -= rint(ddz * b.inv[2]) * b.L[2];
+                    }
+                    double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    if (r2 < rlist2) {
+                        if (count < cap) nbr[(size_t)count * n + t] = j;
+                        count++;
+                    }
+                }
+            }
+        }
+    }
+    if (count > cap) {
+        atomicMax(overflow, count);
+        count = cap;
+    }
+    nnbr[t] = count;
+    ref[i] = pi;
+}
+
+__global__ void k_flag_clear(int* __restrict__ flag) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
+}
+
+// ---------------------------------------------------------------------------------
+// Non-bonded pair kernel: inter-molecular pairs, Lennard-Jones family + Ewald
+// real space or Wolf, one lane per atom i over its neighbour list.
+// Forces are written for every i (full list, both directions), so no atomics
+// are needed and the per-atom sums are deterministic.  Energies and the
+// virial are accumulated for j > i only, the reference's i < j pair set.
+// ---------------------------------------------------------------------------------
+struct PairArgs {
+    const double4* pos;
+    const int2* meta;
+    const int* sorted;
+    const int* nbr;
+    const int* nnbr;
+    const PairParam* table;
+    int nkinds;
+    CoulParam coul;
+    Box box;
+    int n;
+    double4* force;
+    double* epart;  // per block: e_pairs, e_coul, w_xx, w_yy, w_zz, w_xy, w_xz, w_yz
+};
+
+constexpr int kMaxKinds = 16;
+constexpr int kPairBlock = 256;
+
+template <bool kEnergy>
+__global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
+    __shared__ PairParam tab[kMaxKinds * kMaxKinds];
+    __shared__ double red[kPairBlock / 64][8];
+    int nk2 = a.nkinds * a.nkinds;
+    for (int x = threadIdx.x; x < nk2; x += blockDim.x) tab[x] = a.table[x];
+    __syncthreads();
+
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double fx = 0, fy = 0, fz = 0;
+    double e_pair = 0, e_coul = 0;
+    double wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0;
+    int i = -1;
+    if (t < a.n) {
+        i = a.sorted[t];
+        double4 pi = a.pos[i];
+        int ki = a.meta[i].x;
+        double qi = pi.w;
+        int nn = a.nnbr[t];
+        const CoulParam& c = a.coul;
+        for (int k = 0; k < nn; k++) {
+            int j = a.nbr[(size_t)k * a.n + t];
+            double4 pj = a.pos[j];
+            int kj = a.meta[j].x;
+            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            min_image(a.box, dx, dy, dz);
+            double r2 = norm2_ordered(dx, dy, dz);
+            double r = sqrt(r2);
+            double inv_r = 1.0 / r;
+            double fs = 0.0;
+            double e_p = 0.0, e_c = 0.0;
+            const PairParam& pp = tab[ki * a.nkinds + kj];
+            if (pp.pot > POT_NULL && !pp.inter_excl && r < pp.rc) {
+                double e, f;
+                pot_eval(pp.pot, pp.p0, pp.p1, r, inv_r, e, f);
+                fs += f;
+                e_p = e - pp.shift;
+            }
+            double qiqj = qi * pj.w;
+            if (c.kind != COUL_NONE && qiqj != 0.0 && r <= c.rc) {
+                double e, f;
+                if (c.kind == COUL_EWALD) {
+                    ewald_real(c.alpha, qiqj, r, inv_r, c.inter_excl != 0, e, f);
+                    fs += f;
+                    e_c = e;
+                } else if (!c.inter_excl) {
+                    wolf_pair(c, qiqj, r, inv_r, e, f);
+                    fs += f;
+                    e_c = e;
+                }
+            }
+            fx += fs * dx;
+            fy += fs * dy;
+            fz += fs * dz;
+            if (kEnergy && j > i) {
+                e_pair += e_p;
+                e_coul += e_c;
+                wxx += fs * dx * dx;
+                wyy += fs * dy * dy;
+                wzz += fs * dz * dz;
+                wxy += fs * dx * dy;
+                wxz += fs * dx * dz;
+                wyz += fs * dy * dz;
+            }
+        }
+        a.force[i] = make_double4(fx, fy, fz, 0.0);
+    }
+    if (kEnergy) {
+        double v[8] = {e_pair, e_coul, wxx, wyy, wzz, wxy, wxz, wyz};
+        int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            double s = wave_sum(v[q]);
+            if (lane == 0) red[wid][q] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < 8) {
+            double s = 0;
+            for (int w = 0; w < kPairBlock / 64; w++) s += red[w][threadIdx.x];
+            a.epart[blockIdx.x * 8 + threadIdx.x] = s;
+        }
+    }
+}
+
+// Pair-set export: (i < j) with r < rc from the neighbour lists, using the
+// same distance code as k_pairs.
+__global__ void k_pairs_within(const double4* __restrict__ pos, const int* __restrict__ sorted,
+                               const int* __restrict__ nbr, const int* __restrict__ nnbr, int n, Box box, double rc,
+                               uint2* __restrict__ out, unsigned long long cap,
+                               unsigned long long* __restrict__ count) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i = sorted[t];
+    double4 pi = pos[i];
+    int nn = nnbr[t];
+    for (int k = 0; k < nn; k++) {
+        int j = nbr[(size_t)k * n + t];
+        if (j <= i) continue;
+        double4 pj = pos[j];
+        double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        min_image(box, dx, dy, dz);
+        double r = sqrt(norm2_ordered(dx, dy, dz));
+        if (r < rc) {
+            unsigned long long idx = atomicAdd(count, 1ull);
+            if (idx < cap) out[idx] = make_uint2((unsigned)i, (unsigned)j);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Intra-molecular kernel: one thread per molecule with >= 2 atoms.  Pairs in
+// the same molecule with their bond path (restrictions), bonds, angles and
+// dihedrals (compute.rs:60-97, energy.rs:88-138, ewald.rs:387-428 for the
+// excluded-pair correction).  The thread owns its molecule's atoms, so it
+// adds into force[] without atomics; it runs after k_pairs on the stream.
+// ---------------------------------------------------------------------------------
+struct BondedParam {
+    int pot;  // POT_* (POT_NONE: no potential)
+    double p0, p1;
+};
+
+struct MolArgs {
+    const double4* pos;
+    const int2* meta;
+    int nmol_multi;
+    const int* mol_list;       // molecules with >= 2 atoms
+    const int* mol_start;      // nmol + 1
+    const int4* ipairs;        // (a, b, path, -) per molecule, CSR by ip_start
+    const int* ip_start;
+    const int4* bonds;         // (a, b, param index, -)
+    const int* b_start;
+    const int4* angles;        // (a, b, c, param index)
+    const int* a_start;
+    const int4* dihedrals;     // (a, b, c, d)
+    const int* d_param;        // param index per dihedral
+    const int* d_start;
+    const PairParam* table;
+    int nkinds;
+    const int* pair_restr;     // restriction per kind pair
+    const double* pair_s14;    // scale14 per kind pair
+    const BondedParam* bparam;
+    const BondedParam* aparam;
+    const BondedParam* dparam;
+    CoulParam coul;
+    int coul_restr;
+    double coul_s14;
+    Box box;
+    double4* force;
+    double* epart;  // per block: pairs, coul, bonds, angles, dihedrals, w6
+};
+
+__device__ __forceinline__ void vimage(const Box& b, double& x, double& y, double& z) { min_image(b, x, y, z); }
+
+template <bool kEnergy>
+__global__ __launch_bounds__(256) void k_molecules(MolArgs a) {
+    __shared__ double red[4][11];
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (idx < a.nmol_multi) {
+        int m = a.mol_list[idx];
+        int m0 = a.mol_start[m], m1 = a.mol_start[m + 1];
+        int natm = m1 - m0;
+        // local force accumulators (molecules are small; spill to global beyond 8)
+        double lf[8][3];
+        for (int q = 0; q < 8; q++) lf[q][0] = lf[q][1] = lf[q][2] = 0.0;
+        auto addf = [&](int atom, double x, double y, double z) {
+            int l = atom - m0;
+            if (l < 8) {
+                lf[l][0] += x;
+                lf[l][1] += y;
+                lf[l][2] += z;
+            } else {
+                a.force[atom].x += x;
+                a.force[atom].y += y;
+                a.force[atom].z += z;
+            }
+        };
+        // intra-molecular pairs
+        for (int p = a.ip_start[m]; p < a.ip_start[m + 1]; p++) {
+            int4 ip = a.ipairs[p];
+            int i = ip.x, j = ip.y, path = ip.z;
+            double4 pi = a.pos[i], pj = a.pos[j];
+            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            min_image(a.box, dx, dy, dz);
+            double r = sqrt(norm2_ordered(dx, dy, dz));
+            double inv_r = 1.0 / r;
+            int ki = a.meta[i].x, kj = a.meta[j].x;
+            int tk = ki * a.nkinds + kj;
+            const PairParam& pp = a.table[tk];
+            double fs = 0.0;
+            if (pp.pot != POT_NONE) {
+                bool excl;
+                double scaling;
+                restriction_info(a.pair_restr[tk], a.pair_s14[tk], path, excl, scaling);
+                if (!excl && r < pp.rc && pp.pot != POT_NULL) {
+                    double e, f;
+                    pot_eval(pp.pot, pp.p0, pp.p1, r, inv_r, e, f);
+                    fs += scaling * f;
+                    acc[0] += scaling * (e - pp.shift);
+                }
+            }
+            double qiqj = pi.w * pj.w;
+            if (a.coul.kind != COUL_NONE && qiqj != 0.0) {
+                bool excl;
+                double scaling;
+                restriction_info(a.coul_restr, a.coul_s14, path, excl, scaling);
+                if (r <= a.coul.rc) {
+                    double e, f;
+                    if (a.coul.kind == COUL_EWALD) {
+                        ewald_real(a.coul.alpha, qiqj, r, inv_r, excl, e, f);
+                        fs += f;
+                        acc[1] += e;
+                    } else if (!excl) {
+                        wolf_pair(a.coul, qiqj, r, inv_r, e, f);
+                        fs += scaling * f;
+                        acc[1] += scaling * e;
+                    }
+                }
+            }
+            addf(i, fs * dx, fs * dy, fs * dz);
+            addf(j, -fs * dx, -fs * dy, -fs * dz);
+            acc[5] += fs * dx * dx;
+            acc[6] += fs * dy * dy;
+            acc[7] += fs * dz * dz;
+            acc[8] += fs * dx * dy;
+            acc[9] += fs * dx * dz;
+            acc[10] += fs * dy * dz;
+        }
+        // bonds (compute.rs:63-73; virial compute.rs:231-239)
+        for (int p = a.b_start[m]; p < a.b_start[m + 1]; p++) {
+            int4 bd = a.bonds[p];
+            const BondedParam& bp = a.bparam[bd.z];
+            if (bp.pot == POT_NONE) continue;
+            double4 pi = a.pos[bd.x], pj = a.pos[bd.y];
+            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            min_image(a.box, dx, dy, dz);
+            double r = sqrt(norm2_ordered(dx, dy, dz));
+            double inv_r = 1.0 / r;
+            double e, f;
+            pot_eval(bp.pot, bp.p0, bp.p1, r, inv_r, e, f);
+            addf(bd.x, f * dx, f * dy, f * dz);
+            addf(bd.y, -f * dx, -f * dy, -f * dz);
+            acc[2] += e;
+            acc[5] += f * dx * dx;
+            acc[6] += f * dy * dy;
+            acc[7] += f * dz * dz;
+            acc[8] += f * dx * dy;
+            acc[9] += f * dx * dz;
+            acc[10] += f * dy * dz;
+        }
+        // angles (cells.rs:335-359)
+        for (int p = a.a_start[m]; p < a.a_start[m + 1]; p++) {
+            int4 an = a.angles[p];
+            const BondedParam& ap = a.aparam[an.w];
+            if (ap.pot == POT_NONE) continue;
+            double4 p1 = a.pos[an.x], p2 = a.pos[an.y], p3 = a.pos[an.z];
+            double ax = p1.x - p2.x, ay = p1.y - p2.y, az = p1.z - p2.z;
+            double bx = p3.x - p2.x, by = p3.y - p2.y, bz = p3.z - p2.z;
+            vimage(a.box, ax, ay, az);
+            vimage(a.box, bx, by, bz);
+            double na = sqrt(norm2_ordered(ax, ay, az)), nb = sqrt(norm2_ordered(bx, by, bz));
+            double ux = ax / na, uy = ay / na, uz = az / na;
+            double vx = bx / nb, vy = by / nb, vz = bz / nb;
+            double cs = ux * vx + uy * vy + uz * vz;
+            double sin_inv = 1.0 / sqrt(1.0 - cs * cs);
+            double d1x = sin_inv * (cs * ux - vx) / na, d1y = sin_inv * (cs * uy - vy) / na,
+                   d1z = sin_inv * (cs * uz - vz) / na;
+            double d3x = sin_inv * (cs * vx - ux) / nb, d3y = sin_inv * (cs * vy - uy) / nb,
+                   d3z = sin_inv * (cs * vz - uz) / nb;
+            double theta = acos(cs);
+            double e, f;
+            pot_eval(ap.pot, ap.p0, ap.p1, theta, 1.0, e, f);
+            // pot_eval returns force/r with inv_r = 1 -> plain -dV/dtheta
+            addf(an.x, f * d1x, f * d1y, f * d1z);
+            addf(an.z, f * d3x, f * d3y, f * d3z);
+            addf(an.y, -f * (d1x + d3x), -f * (d1y + d3y), -f * (d1z + d3z));
+            acc[3] += e;
+        }
+        // dihedrals (cells.rs:379-411)
+        for (int p = a.d_start[m]; p < a.d_start[m + 1]; p++) {
+            int4 dh = a.dihedrals[p];
+            const BondedParam& dp = a.dparam[a.d_param[p]];
+            if (dp.pot == POT_NONE) continue;
+            double4 q1 = a.pos[dh.x], q2 = a.pos[dh.y], q3 = a.pos[dh.z], q4 = a.pos[dh.w];
+            double r12x = q2.x - q1.x, r12y = q2.y - q1.y, r12z = q2.z - q1.z;
+            double r23x = q3.x - q2.x, r23y = q3.y - q2.y, r23z = q3.z - q2.z;
+            double r34x = q4.x - q3.x, r34y = q4.y - q3.y, r34z = q4.z - q3.z;
+            vimage(a.box, r12x, r12y, r12z);
+            vimage(a.box, r23x, r23y, r23z);
+            vimage(a.box, r34x, r34y, r34z);
+            double ux = r12y * r23z - r12z * r23y, uy = r12z * r23x - r12x * r23z, uz = r12x * r23y - r12y * r23x;
+            double vx = r23y * r34z - r23z * r34y, vy = r23z * r34x - r23x * r34z, vz = r23x * r34y - r23y * r34x;
+            double u2 = ux * ux + uy * uy + uz * uz, v2 = vx * vx + vy * vy + vz * vz;
+            double r23_2 = r23x * r23x + r23y * r23y + r23z * r23z;
+            double r23n = sqrt(r23_2);
+            double d1x = (-r23n / u2) * ux, d1y = (-r23n / u2) * uy, d1z = (-r23n / u2) * uz;
+            double d4x = (r23n / v2) * vx, d4y = (r23n / v2) * vy, d4z = (r23n / v2) * vz;
+            double r23_r34 = r23x * r34x + r23y * r34y + r23z * r34z;
+            double r12_r23 = r12x * r23x + r12y * r23y + r12z * r23z;
+            double c1 = -r12_r23 / r23_2 - 1.0, c2 = r23_r34 / r23_2;
+            double c3 = -r23_r34 / r23_2 - 1.0, c4 = r12_r23 / r23_2;
+            double d2x = c1 * d1x + c2 * d4x, d2y = c1 * d1y + c2 * d4y, d2z = c1 * d1z + c2 * d4z;
+            double d3x = c3 * d4x + c4 * d1x, d3y = c3 * d4y + c4 * d1y, d3z = c3 * d4z + c4 * d1z;
+            double phi = atan2(r23n * (vx * r12x + vy * r12y + vz * r12z), ux * vx + uy * vy + uz * vz);
+            double e, f;
+            pot_eval(dp.pot, dp.p0, dp.p1, phi, 1.0, e, f);
+            addf(dh.x, f * d1x, f * d1y, f * d1z);
+            addf(dh.y, f * d2x, f * d2y, f * d2z);
+            addf(dh.z, f * d3x, f * d3y, f * d3z);
+            addf(dh.w, f * d4x, f * d4y, f * d4z);
+            acc[4] += e;
+        }
+        for (int q = 0; q < natm && q < 8; q++) {
+            int atom = m0 + q;
+            a.force[atom].x += lf[q][0];
+            a.force[atom].y += lf[q][1];
+            a.force[atom].z += lf[q][2];
+        }
+    }
+    if (kEnergy) {
+        int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        for (int q = 0; q < 11; q++) {
+            double s = wave_sum(acc[q]);
+            if (lane == 0) red[wid][q] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < 11) {
+            double s = 0;
+            for (int w = 0; w < 4; w++) s += red[w][threadIdx.x];
+            a.epart[blockIdx.x * 11 + threadIdx.x] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Ewald k-space (ewald.rs:633-733).  Per-atom, per-axis phase factors
+// e^{i n k_axis . r} for n in [-kmax, kmax] are built in LDS by the
+// reference's recursion (polar for n = 1, products for |n| >= 2, conjugates
+// for n < 0), then e^{i k.r} = (Ex * Ey) * Ez as ewald.rs:666-668.
+// ---------------------------------------------------------------------------------
+constexpr int kRhoBlock = 256;    // k-vectors per block in the rho pass
+
+// Phase-table rows: axis 0 keeps n = 0..kmax (ikx >= 0 in the half-space
+// set), axes 1 and 2 keep n = -kmax..kmax.  5*kmax + 3 rows of `width` atoms.
+__host__ __device__ __forceinline__ int phase_row(int axis, int n, int kmax) {
+    return axis == 0 ? n : (kmax + 1) + (axis - 1) * (2 * kmax + 1) + (n + kmax);
+}
+__host__ __device__ __forceinline__ int phase_rows(int kmax) { return 5 * kmax + 3; }
+
+// Builds the phase table for `cnt` atoms starting at `a0`; `width` atoms per
+// row.  One (atom, axis) recursion per thread, as ewald.rs:641-662.
+__device__ __forceinline__ void build_phase_table(const double4* __restrict__ pos, int a0, int cnt, int n_atoms,
+                                                  const Box& b, int kmax, double2* tab, int width,
+                                                  double* q_out) {
+    for (int job = threadIdx.x; job < width * 3; job += blockDim.x) {
+        int a = job % width;
+        int axis = job / width;
+        int atom = a0 + a;
+        bool live = a < cnt && atom < n_atoms;
+        double x = 0.0, q = 0.0;
+        if (live) {
+            double4 p = pos[atom];
+            x = axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
+            q = p.w;
+        }
+        // k_vector(e_axis) . r = (2 pi * inv_axis) * x  (cells.rs:224)
+        double phase = (2.0 * kPi * b.inv[axis]) * x;
+        double sn, cs;
+        sincos(phase, &sn, &cs);
+        double2 e1 = make_double2(cs, sn);
+        tab[(size_t)phase_row(axis, 0, kmax) * width + a] = make_double2(1.0, 0.0);
+        tab[(size_t)phase_row(axis, 1, kmax) * width + a] = e1;
+        if (axis != 0) tab[(size_t)phase_row(axis, -1, kmax) * width + a] = make_double2(e1.x, -e1.y);
+        double2 e = e1;
+        for (int k = 2; k <= kmax; k++) {
+            double2 nx = make_double2(e.x * e1.x - e.y * e1.y, e.x * e1.y + e.y * e1.x);
+            e = nx;
+            tab[(size_t)phase_row(axis, k, kmax) * width + a] = e;
+            if (axis != 0) tab[(size_t)phase_row(axis, -k, kmax) * width + a] = make_double2(e.x, -e.y);
+        }
+        if (axis == 0 && q_out) q_out[a] = live ? q : 0.0;
+    }
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// rho partial sums: grid (ceil(nk / kRhoBlock), nchunks), `width` atoms per chunk.
+__global__ __launch_bounds__(kRhoBlock) void k_rho_partial(const double4* __restrict__ pos, int n, Box b, int kmax,
+                                                           int width, const int4* __restrict__ kidx, int nk,
+                                                           double2* __restrict__ partial) {
+    extern __shared__ double2 smem[];
+    double2* tab = smem;
+    double* q = reinterpret_cast<double*>(tab + (size_t)phase_rows(kmax) * width);
+    int chunk = blockIdx.y;
+    int a0 = chunk * width;
+    int cnt = min(width, n - a0);
+    build_phase_table(pos, a0, cnt, n, b, kmax, tab, width, q);
+    __syncthreads();
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    int4 id = kidx[k];
+    const double2* ex = tab + (size_t)phase_row(0, id.x, kmax) * width;
+    const double2* ey = tab + (size_t)phase_row(1, id.y, kmax) * width;
+    const double2* ez = tab + (size_t)phase_row(2, id.z, kmax) * width;
+    double re = 0.0, im = 0.0;
+    for (int a = 0; a < cnt; a++) {
+        double2 phi = cmul(cmul(ex[a], ey[a]), ez[a]);
+        re += q[a] * phi.x;
+        im += q[a] * phi.y;
+    }
+    partial[(size_t)chunk * nk + k] = make_double2(re, im);
+}
+
+// rho(k) = sum over chunks in order; optional energy/virial partials per block:
+// e = factor |rho|^2, w = |rho|^2 * virial_k (6 components).
+template <bool kEnergy>
+__global__ __launch_bounds__(256) void k_rho_reduce(const double2* __restrict__ partial, int nchunks, int nk,
+                                                    double2* __restrict__ rho, const double* __restrict__ efac,
+                                                    const double* __restrict__ vfac, double* __restrict__ epart) {
+    __shared__ double red[4][7];
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (k < nk) {
+        double re = 0.0, im = 0.0;
+        for (int c = 0; c < nchunks; c++) {
+            double2 p = partial[(size_t)c * nk + k];
+            re += p.x;
+            im += p.y;
+        }
+        rho[k] = make_double2(re, im);
+        if (kEnergy) {
+            double n2 = re * re + im * im;
+            v[0] = efac[k] * n2;
+            for (int q = 0; q < 6; q++) v[1 + q] = n2 * vfac[(size_t)q * nk + k];
+        }
+    }
+    if (kEnergy) {
+        int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        for (int q = 0; q < 7; q++) {
+            double s = wave_sum(v[q]);
+            if (lane == 0) red[wid][q] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < 7) {
+            double s = 0;
+            for (int w = 0; w < 4; w++) s += red[w][threadIdx.x];
+            epart[blockIdx.x * 7 + threadIdx.x] = s;
+        }
+    }
+}
+
+// k-space forces: one lane per atom, rows of k-vectors (fixed ikx, iky;
+// consecutive ikz) read uniformly; F_i += q_i / (4 pi eps0) *
+// sum_k Im(e^{ik.r_i} conj(rho_k)) field_k  (ewald.rs:699-719).
+struct KRow {
+    int ikx, iky, kbase, count;
+};
+
+// blockDim.x atoms per block (64, or 32 when the table would not fit in LDS).
+__global__ __launch_bounds__(64) void k_kspace_force(const double4* __restrict__ pos, int n, Box b, int kmax,
+                                                     const KRow* __restrict__ rows, int nrows,
+                                                     const int4* __restrict__ kidx,
+                                                     const double2* __restrict__ rho,
+                                                     const double4* __restrict__ field,
+                                                     double4* __restrict__ force) {
+    extern __shared__ double2 smem[];
+    double2* tab = smem;
+    int width = blockDim.x;
+    int a0 = blockIdx.x * width;
+    int cnt = min(width, n - a0);
+    build_phase_table(pos, a0, cnt, n, b, kmax, tab, width, nullptr);
+    __syncthreads();
+    int a = threadIdx.x;
+    int i = a0 + a;
+    double fx = 0, fy = 0, fz = 0;
+    for (int r = 0; r < nrows; r++) {
+        KRow row = rows[r];
+        double2 exy = cmul(tab[(size_t)phase_row(0, row.ikx, kmax) * width + a],
+                           tab[(size_t)phase_row(1, row.iky, kmax) * width + a]);
+        const double2* ez = tab + (size_t)phase_row(2, 0, kmax) * width + a;
+        for (int c = 0; c < row.count; c++) {
+            int k = row.kbase + c;
+            int ikz = kidx[k].z;
+            double2 e = cmul(exy, ez[(size_t)ikz * width]);
+            double2 rk = rho[k];
+            // Im(e * conj(rho)) = e.im * rho.re - e.re * rho.im
+            double s = e.y * rk.x - e.x * rk.y;
+            double4 fk = field[k];
+            fx += s * fk.x;
+            fy += s * fk.y;
+            fz += s * fk.z;
+        }
+    }
+    if (a < cnt) {
+        double q = pos[i].w;
+        double4 f = force[i];
+        f.x += (q * fx) / kFourPiEps0;
+        f.y += (q * fy) / kFourPiEps0;
+        f.z += (q * fz) / kFourPiEps0;
+        force[i] = f;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Integrators (lumol-sim/src/md/integrators.rs), FP64, unfused as in the
+// reference so that positions follow the reference's rounding.
+// ---------------------------------------------------------------------------------
+#pragma clang fp contract(off)
+// VelocityVerlet first half (integrators.rs:47-53): optional pending
+// Berendsen scale of the previous step, then v += (dt/2) a; x += v dt.
+__global__ void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel, const double4* __restrict__ acc,
+                           int n, double dt, const double* __restrict__ lambda) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 v = vel[i];
+    if (lambda) {
+        double l = *lambda;
+        v.x = v.x * l;
+        v.y = v.y * l;
+        v.z = v.z * l;
+    }
+    double4 a = acc[i];
+    double h = 0.5 * dt;
+    v.x += h * a.x;
+    v.y += h * a.y;
+    v.z += h * a.z;
+    double4 p = pos[i];
+    p.x += v.x * dt;
+    p.y += v.y * dt;
+    p.z += v.z * dt;
+    vel[i] = v;
+    pos[i] = p;
+}
+
+// VelocityVerlet second half (integrators.rs:55-68) with the kinetic energy
+// partial sums of the block (for the thermostat).
+__global__ __launch_bounds__(256) void k_vv_second(const double4* __restrict__ force, double4* __restrict__ vel,
+                                                   double4* __restrict__ acc, int n, double dt,
+                                                   double* __restrict__ ke_part) {
+    __shared__ double red[4];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ke = 0.0;
+    if (i < n) {
+        double4 f = force[i];
+        double4 v = vel[i];
+        double m = v.w;
+        double4 a = make_double4(f.x / m, f.y / m, f.z / m, 0.0);
+        double h = 0.5 * dt;
+        v.x += h * a.x;
+        v.y += h * a.y;
+        v.z += h * a.z;
+        acc[i] = a;
+        vel[i] = v;
+        ke = 0.5 * m * (v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    if (ke_part) {
+        double s = wave_sum(ke);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) ke_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    }
+}
+
+// LeapFrog (integrators.rs:150-168), position part, after the pending
+// Berendsen scale of the previous step.
+__global__ void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel, const double4* __restrict__ acc,
+                           int n, double dt, const double* __restrict__ lambda) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 v = vel[i];
+    if (lambda) {
+        double l = *lambda;
+        v.x = v.x * l;
+        v.y = v.y * l;
+        v.z = v.z * l;
+        vel[i] = v;
+    }
+    double4 a = acc[i];
+    double dt2 = dt * dt;
+    double4 p = pos[i];
+    p.x += v.x * dt + (0.5 * a.x) * dt2;
+    p.y += v.y * dt + (0.5 * a.y) * dt2;
+    p.z += v.z * dt + (0.5 * a.z) * dt2;
+    pos[i] = p;
+}
+
+__global__ __launch_bounds__(256) void k_lf_second(const double4* __restrict__ force, double4* __restrict__ vel,
+                                                   double4* __restrict__ acc, int n, double dt,
+                                                   double* __restrict__ ke_part) {
+    __shared__ double red[4];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ke = 0.0;
+    if (i < n) {
+        double4 f = force[i];
+        double4 v = vel[i];
+        double m = v.w;
+        double4 a = acc[i];
+        double4 na = make_double4(f.x / m, f.y / m, f.z / m, 0.0);
+        v.x += (0.5 * (a.x + na.x)) * dt;
+        v.y += (0.5 * (a.y + na.y)) * dt;
+        v.z += (0.5 * (a.z + na.z)) * dt;
+        acc[i] = na;
+        vel[i] = v;
+        ke = 0.5 * m * (v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    if (ke_part) {
+        double s = wave_sum(ke);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) ke_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    }
+}
+
+// Berendsen factor (thermostats.rs:113-121) from the kinetic energy partials;
+// dof = 3N (system.rs:249-255).  One block.
+__global__ __launch_bounds__(256) void k_berendsen(const double* __restrict__ ke_part, int nparts, double dof,
+                                                   double T0, double tau, double* __restrict__ lambda) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) s += ke_part[p];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ke = red[0] + red[1] + red[2] + red[3];
+        double T = 2.0 * ke / (dof * kBoltzmann);
+        *lambda = sqrt(1.0 + (T0 / T - 1.0) / tau);
+    }
+}
+
+__global__ void k_scale_vel(double4* __restrict__ vel, int n, const double* __restrict__ lambda) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double l = *lambda;
+    double4 v = vel[i];
+    v.x = v.x * l;
+    v.y = v.y * l;
+    v.z = v.z * l;
+    vel[i] = v;
+}
+#pragma clang fp contract(on)
+
+__global__ void k_set_scalar(double* __restrict__ p, double v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+}  // namespace lg

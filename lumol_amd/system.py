@@ -1,0 +1,586 @@
+"""Host-side mirror of lumol-core's system and potential API, backed by the
+gfx950 library.
+
+Names and argument meaning follow the reference so that code written
+against lumol reads the same (paths relative to the reference checkout):
+
+  LennardJones / Harmonic / NullPotential   lumol-core/src/energy/functions.rs
+  PairInteraction (new, shifted, tail)      lumol-core/src/energy/pairs.rs:57-297
+  PairRestriction                           lumol-core/src/energy/restrictions.rs:13-33
+  Ewald / SharedEwald / Wolf                lumol-core/src/energy/global/{ewald,wolf}.rs
+  UnitCell (cubic, ortho, infinite)         lumol-core/src/sys/config/cells.rs
+  System (add_bond, set_*_potential,
+          forces, potential_energy, virial) lumol-core/src/sys/system.rs
+  VelocityVerlet / LeapFrog                 lumol-sim/src/md/integrators.rs
+  BerendsenThermostat                       lumol-sim/src/md/thermostats.rs:91-121
+  MolecularDynamics / Simulation            lumol-sim/src/md/molecular_dynamics.rs,
+                                            lumol-sim/src/simulations.rs
+
+Every evaluation runs on the GPU through the C ABI; invalid input raises
+ValueError (where the reference panics) or LumolGpuError.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _native as nat
+
+K_BOLTZMANN = 8.31446284161522e-7          # lumol-core/src/consts.rs:9
+FOUR_PI_EPSILON_0 = 7.197589831304046      # lumol-core/src/consts.rs:15
+
+# Standard atomic weights (IUPAC), the values lumol's mass table holds.
+ATOMIC_MASSES = {
+    "H": 1.008, "He": 4.002602, "Li": 6.94, "Be": 9.012182, "B": 10.81, "C": 12.011,
+    "N": 14.007, "O": 15.999, "F": 18.9984032, "Ne": 20.1797, "Na": 22.98976928,
+    "Mg": 24.305, "Al": 26.9815386, "Si": 28.085, "P": 30.973762, "S": 32.06,
+    "Cl": 35.45, "Ar": 39.948, "K": 39.0983, "Ca": 40.078, "Kr": 83.798, "Xe": 131.293,
+}
+
+
+# ---------------------------------------------------------------------------------
+# units (lumol-core/src/units.rs:30-80)
+# ---------------------------------------------------------------------------------
+class units:
+    """Conversion factors to lumol's internal units (A, fs, u, K)."""
+    U_IN_KG = 1.660538782e-27
+    AVOGADRO = 6.02214179e23
+    _energy = {"kJ/mol": 1e-7 / U_IN_KG / AVOGADRO,
+               "kcal/mol": 4.184 * 1e-7 / U_IN_KG / AVOGADRO,
+               "K": K_BOLTZMANN}
+
+    @classmethod
+    def from_(cls, value, unit):
+        if unit in cls._energy:
+            return value * cls._energy[unit]
+        if unit == "kJ/mol/A^2":
+            return value * cls._energy["kJ/mol"]
+        if unit == "kcal/mol/A^2":
+            return value * cls._energy["kcal/mol"]
+        if unit == "kJ/mol/A":
+            return value * cls._energy["kJ/mol"]
+        if unit == "kcal/mol/A":
+            return value * cls._energy["kcal/mol"]
+        if unit == "atm":
+            return value * 101325.0 * 1e-40 / cls.U_IN_KG
+        if unit == "bar":
+            return value * 1e-35 / cls.U_IN_KG
+        if unit == "deg":
+            return value * math.pi / 180.0
+        if unit in ("A", "fs", "u", "rad"):
+            return value
+        if unit == "ps":
+            return value * 1e3
+        raise ValueError(f"unknown unit {unit}")
+
+    @classmethod
+    def to(cls, value, unit):
+        return value / cls.from_(1.0, unit)
+
+
+# ---------------------------------------------------------------------------------
+# potentials
+# ---------------------------------------------------------------------------------
+class NullPotential:
+    """V = 0 (functions.rs:30-50)."""
+    kind = nat.LG_POT_NULL
+
+    def params(self):
+        return (0.0, 0.0)
+
+
+class LennardJones:
+    """V = 4 eps ((s/r)^12 - (s/r)^6) (functions.rs:52-107)."""
+    kind = nat.LG_POT_LJ
+
+    def __init__(self, sigma, epsilon):
+        self.sigma = float(sigma)
+        self.epsilon = float(epsilon)
+
+    def params(self):
+        return (self.sigma, self.epsilon)
+
+
+class Harmonic:
+    """V = k/2 (x - x0)^2 (functions.rs:109-160)."""
+    kind = nat.LG_POT_HARMONIC
+
+    def __init__(self, k, x0):
+        self.k = float(k)
+        self.x0 = float(x0)
+
+    def params(self):
+        return (self.k, self.x0)
+
+
+class PairRestriction:
+    """Restriction codes of restrictions.rs:13-33 (Scale14 carries a factor)."""
+    NONE = 0
+    INTRA_MOLECULAR = 1
+    INTER_MOLECULAR = 2
+    EXCLUDE12 = 3
+    EXCLUDE13 = 4
+    EXCLUDE14 = 5
+
+    def __init__(self, code, scaling=1.0):
+        self.code = code
+        self.scaling = scaling
+
+    @classmethod
+    def scale14(cls, scaling):
+        if not 0.0 <= scaling <= 1.0:
+            raise ValueError("scaling must be in [0, 1]")
+        return cls(6, scaling)
+
+
+def _restriction(r):
+    if isinstance(r, PairRestriction):
+        return r.code, r.scaling
+    return int(r), 1.0
+
+
+class PairInteraction:
+    """Potential + cutoff (+ shift, tail, restriction) (pairs.rs:27-297)."""
+
+    def __init__(self, potential, cutoff):
+        self.potential = potential
+        self._cutoff = float(cutoff)
+        self.shifted_ = False
+        self.tail = False
+        self.restriction_ = PairRestriction.NONE
+        self.scale14 = 1.0
+
+    @classmethod
+    def shifted(cls, potential, cutoff):
+        p = cls(potential, cutoff)
+        p.shifted_ = True
+        return p
+
+    def enable_tail_corrections(self):
+        self.tail = True
+
+    def set_restriction(self, restriction):
+        self.restriction_, self.scale14 = _restriction(restriction)
+
+    def restriction(self):
+        return self.restriction_
+
+    def cutoff(self):
+        return self._cutoff
+
+
+class Ewald:
+    """Ewald summation (ewald.rs:228-305).  alpha None selects pi / cutoff."""
+
+    def __init__(self, cutoff, kmax, alpha=None):
+        if cutoff < 0:
+            raise ValueError("the cutoff can not be negative in Ewald")
+        if alpha is not None and alpha < 0:
+            raise ValueError("alpha can not be negative in Ewald")
+        if kmax == 0:
+            raise ValueError("kmax can not be 0 in Ewald")
+        self.cutoff = float(cutoff)
+        self.kmax = int(kmax)
+        self.alpha = math.pi / cutoff if alpha is None else float(alpha)
+        self.restriction = PairRestriction.NONE
+
+    def set_restriction(self, restriction):
+        code, _ = _restriction(restriction)
+        if code == 6:
+            raise ValueError("Scaling restriction scheme using Ewald are not implemented")
+        self.restriction = code
+
+
+def SharedEwald(ewald):
+    """The reference's thread-safe wrapper (ewald.rs:847); the GPU handle is
+    already serialised, so the wrapped solver is used as is."""
+    return ewald
+
+
+class Wolf:
+    """Wolf summation (wolf.rs:53-84)."""
+
+    def __init__(self, cutoff):
+        if not cutoff > 0:
+            raise ValueError("Got a negative cutoff in Wolf summation")
+        self.cutoff = float(cutoff)
+        self.restriction = PairRestriction.NONE
+        self.scale14 = 1.0
+
+    def set_restriction(self, restriction):
+        self.restriction, self.scale14 = _restriction(restriction)
+
+
+class UnitCell:
+    """Orthorhombic or infinite cell (cells.rs:31-120)."""
+
+    def __init__(self, lengths):
+        self.lengths = tuple(float(x) for x in lengths)
+
+    @classmethod
+    def cubic(cls, length):
+        if not length > 0:
+            raise ValueError("Cell lengths must be positive")
+        return cls((length, length, length))
+
+    @classmethod
+    def ortho(cls, a, b, c):
+        if not (a > 0 and b > 0 and c > 0):
+            raise ValueError("Cell lengths must be positive")
+        return cls((a, b, c))
+
+    @classmethod
+    def infinite(cls):
+        return cls((0.0, 0.0, 0.0))
+
+    def is_infinite(self):
+        return self.lengths == (0.0, 0.0, 0.0)
+
+    def volume(self):
+        a, b, c = self.lengths
+        return a * b * c
+
+
+# ---------------------------------------------------------------------------------
+# System
+# ---------------------------------------------------------------------------------
+class System:
+    """Particles, topology and interactions; evaluation on the GPU.
+
+    Positions and velocities are host arrays until the first evaluation; after
+    that the device copy is authoritative and the properties read it back.
+    """
+
+    def __init__(self, cell=None, device=0):
+        self.cell = cell if cell is not None else UnitCell.infinite()
+        self.device = device
+        self.names = []
+        self._positions = np.zeros((0, 3))
+        self._velocities = np.zeros((0, 3))
+        self.charges = np.zeros(0)
+        self.masses = np.zeros(0)
+        self.bonds = []
+        self.kinds = {}  # name -> kind index, in order of first use
+        self.pairs = {}
+        self.bond_potentials = {}
+        self.angle_potentials = {}
+        self.dihedral_potentials = {}
+        self.coulomb = None
+        self.precision = nat.LG_PRECISION_FP64
+        self._handle = None
+        self._dirty = True
+        self._device_state = False
+
+    # -- construction ----------------------------------------------------------
+    def _kind(self, name):
+        if name not in self.kinds:
+            self.kinds[name] = len(self.kinds)
+        return self.kinds[name]
+
+    def add_particle(self, name, position=(0, 0, 0), velocity=(0, 0, 0), charge=0.0, mass=None):
+        self._pull()
+        self._kind(name)
+        self.names.append(name)
+        self._positions = np.vstack([self._positions, np.asarray(position, float).reshape(1, 3)])
+        self._velocities = np.vstack([self._velocities, np.asarray(velocity, float).reshape(1, 3)])
+        self.charges = np.append(self.charges, float(charge))
+        self.masses = np.append(self.masses, ATOMIC_MASSES.get(name, 0.0) if mass is None else float(mass))
+        self._dirty = True
+
+    @classmethod
+    def from_arrays(cls, names, positions, cell=None, charges=None, masses=None, velocities=None, device=0):
+        s = cls(cell, device)
+        for n in names:
+            s._kind(n)
+        s.names = list(names)
+        s._positions = np.array(positions, dtype=float).reshape(-1, 3)
+        n = len(s.names)
+        s._velocities = (np.zeros((n, 3)) if velocities is None
+                         else np.array(velocities, dtype=float).reshape(-1, 3))
+        s.charges = np.zeros(n) if charges is None else np.array(charges, dtype=float)
+        s.masses = (np.array([ATOMIC_MASSES.get(x, 0.0) for x in names]) if masses is None
+                    else np.array(masses, dtype=float))
+        return s
+
+    def size(self):
+        return len(self.names)
+
+    def add_bond(self, i, j):
+        if i == j or not (0 <= i < self.size() and 0 <= j < self.size()):
+            raise ValueError("invalid bond")
+        self.bonds.append((min(i, j), max(i, j)))
+        self._dirty = True
+
+    def set_pair_potential(self, names, interaction):
+        a, b = names
+        if not self.cell.is_infinite() and interaction.cutoff() > min(self.cell.lengths) / 2:
+            raise ValueError("The cutoff for this pair potential is bigger than half the cell length")
+        self.pairs[(a, b)] = interaction
+        self._kind(a)
+        self._kind(b)
+        self._dirty = True
+
+    def set_bond_potential(self, names, potential):
+        self.bond_potentials[tuple(names)] = potential
+        for n in names:
+            self._kind(n)
+        self._dirty = True
+
+    def set_angle_potential(self, names, potential):
+        self.angle_potentials[tuple(names)] = potential
+        for n in names:
+            self._kind(n)
+        self._dirty = True
+
+    def set_dihedral_potential(self, names, potential):
+        self.dihedral_potentials[tuple(names)] = potential
+        for n in names:
+            self._kind(n)
+        self._dirty = True
+
+    def set_coulomb_potential(self, potential):
+        self.coulomb = potential
+        self._dirty = True
+
+    def set_precision(self, precision):
+        self.precision = precision
+        self._dirty = True
+
+    # -- state -----------------------------------------------------------------
+    @property
+    def positions(self):
+        self._pull()
+        return self._positions
+
+    @positions.setter
+    def positions(self, value):
+        self._pull()
+        self._positions = np.array(value, dtype=float).reshape(-1, 3)
+        if self._handle is not None:
+            nat.check(nat.lib().lg_system_set_positions(self._handle, nat.dptr(self._positions)))
+
+    @property
+    def velocities(self):
+        self._pull()
+        return self._velocities
+
+    @velocities.setter
+    def velocities(self, value):
+        self._pull()
+        self._velocities = np.array(value, dtype=float).reshape(-1, 3)
+        if self._handle is not None:
+            nat.check(nat.lib().lg_system_set_velocities(self._handle, nat.dptr(self._velocities)))
+
+    def _pull(self):
+        if self._handle is not None and self._device_state:
+            L = nat.lib()
+            pos = np.zeros((self.size(), 3))
+            vel = np.zeros((self.size(), 3))
+            nat.check(L.lg_system_get_positions(self._handle, nat.dptr(pos)))
+            nat.check(L.lg_system_get_velocities(self._handle, nat.dptr(vel)))
+            self._positions, self._velocities = pos, vel
+            self._device_state = False
+
+    def particles_changed(self):
+        """Push host-side edits of charges or masses to the device."""
+        self._pull()
+        self._dirty = True
+
+    # -- device handle -------------------------------------------------------------
+    def _sync(self):
+        L = nat.lib()
+        if self._handle is not None and not self._dirty:
+            return self._handle
+        self._pull()
+        if self._handle is not None and self._handle_size != self.size():
+            L.lg_system_destroy(self._handle)
+            self._handle = None
+        if self._handle is None:
+            h = ctypes.c_void_p()
+            nat.check(L.lg_system_create(nat.context(self.device), self.size(), ctypes.byref(h)))
+            self._handle = h
+            self._handle_size = self.size()
+        h = self._handle
+        a, b, c = self.cell.lengths
+        nat.check(L.lg_system_set_cell(h, a, b, c))
+        kinds = np.array([self.kinds[n] for n in self.names], dtype=np.uint32)
+        nat.check(L.lg_system_set_kinds(h, nat.u32ptr(kinds), max(1, len(self.kinds))))
+        q = np.ascontiguousarray(self.charges, dtype=float)
+        nat.check(L.lg_system_set_charges(h, nat.dptr(q)))
+        m = np.ascontiguousarray(self.masses, dtype=float)
+        if np.all(m > 0):
+            nat.check(L.lg_system_set_masses(h, nat.dptr(m)))
+        pos = np.ascontiguousarray(self._positions)
+        vel = np.ascontiguousarray(self._velocities)
+        nat.check(L.lg_system_set_positions(h, nat.dptr(pos)))
+        nat.check(L.lg_system_set_velocities(h, nat.dptr(vel)))
+        bonds = np.array(self.bonds, dtype=np.uint32).reshape(-1, 2)
+        nat.check(L.lg_system_set_bonds(h, len(bonds), nat.u32ptr(np.ascontiguousarray(bonds))))
+        for (na, nb), inter in self.pairs.items():
+            pr = np.array(inter.potential.params(), dtype=float)
+            nat.check(L.lg_pair_set(h, self.kinds[na], self.kinds[nb], inter.potential.kind, nat.dptr(pr),
+                                    inter.cutoff(), int(inter.shifted_), int(inter.tail),
+                                    int(inter.restriction_), float(inter.scale14)))
+        for names, pot in self.bond_potentials.items():
+            pr = np.array(pot.params(), dtype=float)
+            nat.check(L.lg_bond_set(h, self.kinds[names[0]], self.kinds[names[1]], pot.kind, nat.dptr(pr)))
+        for names, pot in self.angle_potentials.items():
+            pr = np.array(pot.params(), dtype=float)
+            k = [self.kinds[x] for x in names]
+            nat.check(L.lg_angle_set(h, k[0], k[1], k[2], pot.kind, nat.dptr(pr)))
+        for names, pot in self.dihedral_potentials.items():
+            pr = np.array(pot.params(), dtype=float)
+            k = [self.kinds[x] for x in names]
+            nat.check(L.lg_dihedral_set(h, k[0], k[1], k[2], k[3], pot.kind, nat.dptr(pr)))
+        if isinstance(self.coulomb, Ewald):
+            nat.check(L.lg_coulomb_ewald(h, self.coulomb.cutoff, self.coulomb.kmax, self.coulomb.alpha,
+                                         self.coulomb.restriction))
+        elif isinstance(self.coulomb, Wolf):
+            nat.check(L.lg_coulomb_wolf(h, self.coulomb.cutoff, self.coulomb.restriction, self.coulomb.scale14))
+        else:
+            nat.check(L.lg_coulomb_clear(h))
+        nat.check(L.lg_system_set_precision(h, self.precision))
+        self._dirty = False
+        self._device_state = True
+        return h
+
+    def close(self):
+        if self._handle is not None:
+            self._pull()
+            nat.lib().lg_system_destroy(self._handle)
+            self._handle = None
+            self._dirty = True
+
+    def __del__(self):
+        try:
+            if self._handle is not None and nat._lib is not None:
+                nat._lib.lg_system_destroy(self._handle)
+        except Exception:
+            pass
+
+    # -- evaluation (System::forces / potential_energy / virial) -------------------------
+    def forces(self):
+        h = self._sync()
+        out = np.zeros((self.size(), 3))
+        nat.check(nat.lib().lg_forces(h, nat.dptr(out), 0))
+        return out
+
+    def energy_terms(self):
+        h = self._sync()
+        t = nat.EnergyTerms()
+        nat.check(nat.lib().lg_potential_energy(h, ctypes.byref(t)))
+        return t
+
+    def potential_energy(self):
+        return self.energy_terms().total
+
+    def virial(self):
+        h = self._sync()
+        out = np.zeros(9)
+        nat.check(nat.lib().lg_virial(h, nat.dptr(out)))
+        return out.reshape(3, 3)
+
+    def kinetic_energy(self):
+        h = self._sync()
+        out = ctypes.c_double()
+        nat.check(nat.lib().lg_kinetic_energy(h, ctypes.byref(out)))
+        return out.value
+
+    def total_energy(self):
+        return self.kinetic_energy() + self.potential_energy()
+
+    def temperature(self):
+        h = self._sync()
+        out = ctypes.c_double()
+        nat.check(nat.lib().lg_temperature(h, ctypes.byref(out)))
+        return out.value
+
+    def volume(self):
+        return self.cell.volume()
+
+    def pressure(self):
+        """Virial pressure (compute.rs:383-425): (dof kB T + tr W) / (3 V)."""
+        w = self.virial()
+        dof = 3 * self.size()
+        return (dof * K_BOLTZMANN * self.temperature() + np.trace(w)) / (3.0 * self.volume())
+
+    def pairs_within(self, cutoff):
+        """Inter-molecular pairs (i < j) with r < cutoff found by the device
+        neighbour search, sorted; same distance code as the force kernels."""
+        h = self._sync()
+        count = ctypes.c_uint64()
+        nat.check(nat.lib().lg_pairs_within(h, cutoff, None, 0, ctypes.byref(count)))
+        out = np.zeros((count.value, 2), dtype=np.uint32)
+        nat.check(nat.lib().lg_pairs_within(h, cutoff, nat.u32ptr(out), count.value, ctypes.byref(count)))
+        order = np.lexsort((out[:, 1], out[:, 0]))
+        return out[order]
+
+    def ewald_kvector_count(self):
+        h = self._sync()
+        out = ctypes.c_int()
+        nat.check(nat.lib().lg_ewald_kvector_count(h, ctypes.byref(out)))
+        return out.value
+
+
+# ---------------------------------------------------------------------------------
+# molecular dynamics (lumol-sim)
+# ---------------------------------------------------------------------------------
+class VelocityVerlet:
+    """integrators.rs:22-70"""
+    code = nat.LG_INTEGRATOR_VELOCITY_VERLET
+
+    def __init__(self, timestep):
+        self.timestep = float(timestep)
+
+
+class LeapFrog:
+    """integrators.rs:128-169"""
+    code = nat.LG_INTEGRATOR_LEAP_FROG
+
+    def __init__(self, timestep):
+        self.timestep = float(timestep)
+
+
+class BerendsenThermostat:
+    """thermostats.rs:91-121"""
+
+    def __init__(self, temperature, tau):
+        if temperature < 0:
+            raise ValueError("The temperature must be positive in thermostats.")
+        if tau < 1.0:
+            raise ValueError("The timestep must be larger than 1 in berendsen thermostat.")
+        self.temperature = float(temperature)
+        self.tau = float(tau)
+
+
+class MolecularDynamics:
+    """molecular_dynamics.rs:11-83: integrator + optional thermostat."""
+
+    def __init__(self, integrator):
+        self.integrator = integrator
+        self.thermostat = None
+
+    def set_thermostat(self, thermostat):
+        self.thermostat = thermostat
+
+
+class Simulation:
+    """simulations.rs:54-159: runs `nsteps` propagator steps on the device."""
+
+    def __init__(self, propagator):
+        self.propagator = propagator
+        self._setup_for = None
+
+    def setup(self, system):
+        h = system._sync()
+        md = self.propagator
+        T, tau = (md.thermostat.temperature, md.thermostat.tau) if md.thermostat else (0.0, 1.0)
+        nat.check(nat.lib().lg_md_setup(h, md.integrator.code, md.integrator.timestep, T, tau))
+        self._setup_for = (id(system), h.value)
+
+    def run(self, system, nsteps):
+        h = system._sync()
+        if self._setup_for != (id(system), h.value):
+            self.setup(system)
+        nat.check(nat.lib().lg_md_run(h, int(nsteps)))
+        system._device_state = True

@@ -1,0 +1,114 @@
+"""Shared builders for the tests: the reference's own test systems."""
+import os
+
+import numpy as np
+
+from lumol_amd import (Ewald, Harmonic, LennardJones, NullPotential, PairInteraction, PairRestriction,
+                       System, UnitCell, Wolf, units)
+from lumol_amd.workloads import read_xyz
+from reference_values import K_BOLTZMANN, NIST_SPCE_LJ
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def nist_lj_system(name, cutoff, tail):
+    """tests/nist-lj.rs:32-56: LJ eps 1 sigma 1 on ("H","H")."""
+    names, pos, cell = read_xyz(os.path.join(GOLDEN, "nist-lj", name))
+    s = System.from_arrays(names, pos, UnitCell.ortho(*cell))
+    p = PairInteraction(LennardJones(sigma=1.0, epsilon=1.0), cutoff)
+    if tail:
+        p.enable_tail_corrections()
+    s.set_pair_potential(("H", "H"), p)
+    return s
+
+
+def nist_spce_system(name, cutoff, kmax=5, alpha=None, tail=True):
+    """tests/nist-spce.rs:24-103: O-O LJ (+tail), Null O-H/H-H, Ewald InterMolecular,
+    bonds O-H for each molecule."""
+    names, pos, cell = read_xyz(os.path.join(GOLDEN, "nist-spce", name))
+    qh = NIST_SPCE_LJ["qH"]
+    charges = np.array([qh if x == "H" else -2.0 * qh for x in names])
+    s = System.from_arrays(names, pos, UnitCell.ortho(*cell), charges=charges)
+    for i in range(0, len(names), 3):
+        s.add_bond(i, i + 1)
+        s.add_bond(i, i + 2)
+    lj = PairInteraction(LennardJones(sigma=NIST_SPCE_LJ["sigma"],
+                                      epsilon=NIST_SPCE_LJ["epsilon_K"] * K_BOLTZMANN), cutoff)
+    if tail:
+        lj.enable_tail_corrections()
+    s.set_pair_potential(("O", "O"), lj)
+    s.set_pair_potential(("O", "H"), PairInteraction(NullPotential(), cutoff))
+    s.set_pair_potential(("H", "H"), PairInteraction(NullPotential(), cutoff))
+    if alpha is None:
+        alpha = 5.6 / min(cell)
+    ew = Ewald(cutoff, kmax, alpha)
+    ew.set_restriction(PairRestriction.INTER_MOLECULAR)
+    s.set_coulomb_potential(ew)
+    return s
+
+
+def read_lammps_forces(name):
+    with open(os.path.join(GOLDEN, "nist-spce", name)) as f:
+        lines = f.read().splitlines()
+    n = int(lines[0])
+    out = np.zeros((n, 3))
+    for i, line in enumerate(lines[2:2 + n]):
+        t = line.split()
+        assert int(t[0]) == i + 1
+        out[i] = [float(t[1]), float(t[2]), float(t[3])]
+    return out
+
+
+def water_molecule():
+    """lumol-core/src/energy/global/ewald.rs:975-993"""
+    from reference_values import EWALD_WATER as W
+    s = System.from_arrays(W["names"], W["positions"], UnitCell.cubic(W["cell"]), charges=W["charges"])
+    for i, j in W["bonds"]:
+        s.add_bond(i, j)
+    return s
+
+
+def random_molecular_system(n_mol=24, L=14.0, seed=7, coulomb="ewald", restriction=PairRestriction.INTER_MOLECULAR):
+    """Small flexible 3-site molecules at random positions: exercises
+    intra-molecular exclusions, bonds, angles and inter-molecular pairs."""
+    rng = np.random.default_rng(seed)
+    names, pos, bonds = [], [], []
+    grid = int(np.ceil(n_mol ** (1 / 3)))
+    sp = L / grid
+    k = 0
+    for a in range(grid):
+        for b in range(grid):
+            for c in range(grid):
+                if k >= n_mol:
+                    break
+                centre = (np.array([a, b, c]) + 0.5) * sp + rng.uniform(-0.3, 0.3, 3)
+                q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+                loc = np.array([[0, 0, 0], [1.0, 0, 0], [-0.33, 0.94, 0]]) @ q.T
+                base = len(names)
+                names += ["O", "H", "H"]
+                pos += list(loc + centre)
+                bonds += [(base, base + 1), (base, base + 2)]
+                k += 1
+    charges = np.array([-0.8476 if x == "O" else 0.4238 for x in names])
+    s = System.from_arrays(names, np.array(pos), UnitCell.cubic(L), charges=charges)
+    for i, j in bonds:
+        s.add_bond(i, j)
+    lj = PairInteraction(LennardJones(3.166, units.from_(0.65, "kJ/mol")), 6.5)
+    lj.set_restriction(restriction)
+    s.set_pair_potential(("O", "O"), lj)
+    s.set_pair_potential(("O", "H"), PairInteraction(NullPotential(), 6.5))
+    hh = PairInteraction(Harmonic(k=units.from_(79.8, "kcal/mol/A^2"), x0=1.633), 6.5)
+    hh.set_restriction(PairRestriction.INTRA_MOLECULAR)
+    s.set_pair_potential(("H", "H"), hh)
+    s.set_bond_potential(("O", "H"), Harmonic(k=units.from_(1054.2, "kcal/mol/A^2"), x0=1.0))
+    s.set_angle_potential(("H", "O", "H"), Harmonic(k=units.from_(75.9, "kcal/mol"), x0=1.9106))
+    if coulomb == "ewald":
+        ew = Ewald(6.5, 7)
+        ew.set_restriction(restriction)
+        s.set_coulomb_potential(ew)
+    elif coulomb == "wolf":
+        w = Wolf(6.5)
+        w.set_restriction(restriction)
+        s.set_coulomb_potential(w)
+    return s

@@ -1,0 +1,392 @@
+"""Pins the oracle (oracle/lumol_oracle.cpp) to the reference's own known
+answers: unit values, doc-test values, NIST LJ and SPC/E energies, LAMMPS
+forces, Ewald component energies and virials, plus invariants the
+mathematics fixes (F = -dE/dx, W = -dE/d(strain), tr W = E for Ewald point
+charges, Madelung constant, sum F = 0, brute-force pair sets, VV on a
+harmonic oscillator, time reversibility).
+
+CPU only; no GPU involved.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import lumol_oracle as lo
+import reference_values as R
+from helpers import (nist_lj_system, nist_spce_system, random_molecular_system, read_lammps_forces,
+                     water_molecule)
+from lumol_amd import (Ewald, Harmonic, LennardJones, PairInteraction, PairRestriction, System, UnitCell, Wolf,
+                       units)
+from lumol_amd.workloads import nacl
+
+
+def ulps_eq(a, b, max_ulps=4, eps=2.220446049250313e-16):
+    """approx::assert_ulps_eq! semantics (absolute epsilon OR ulps distance)."""
+    if abs(a - b) <= eps:
+        return True
+    ia = np.array([a], dtype=np.float64).view(np.int64)[0]
+    ib = np.array([b], dtype=np.float64).view(np.int64)[0]
+    if (ia < 0) != (ib < 0):
+        return False
+    return abs(int(ia) - int(ib)) <= max_ulps
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+# --- scalar functions ---------------------------------------------------------------
+def test_lj_unit_values():
+    L = lo.lib()
+    s, e = R.LJ_UNIT["sigma"], R.LJ_UNIT["epsilon"]
+    for r, v, _ in R.LJ_UNIT["energy"]:
+        assert L.lo_lj_energy(s, e, r) == v
+    for rc, v in R.LJ_UNIT["tail_energy"]:
+        assert L.lo_lj_tail_energy(s, e, rc) == v
+    for rc, v in R.LJ_UNIT["tail_virial"]:
+        assert L.lo_lj_tail_virial(s, e, rc) == v
+    for r, v, _ in R.LJ_UNIT["force"]:
+        assert ulps_eq(L.lo_lj_force(s, e, r), v)
+    # force at the minimum 2^(1/6) sigma vanishes (functions.rs:641)
+    assert abs(L.lo_lj_force(s, e, 2.0 ** (1 / 6) * s)) < 1e-15
+    d = R.LJ_DOC
+    for r, v in d["energy"]:
+        assert L.lo_lj_energy(d["sigma"], d["epsilon"], r) == v
+    for r, v in d["force"]:
+        assert L.lo_lj_force(d["sigma"], d["epsilon"], r) == v
+
+
+def test_lj_force_is_minus_derivative():
+    L = lo.lib()
+    eps = 1e-9
+    for r in (2.3, 3.0, 4.0, 6.0):
+        e0 = L.lo_lj_energy(2.0, 0.8, r)
+        e1 = L.lo_lj_energy(2.0, 0.8, r + eps)
+        assert abs((e0 - e1) / eps - L.lo_lj_force(2.0, 0.8, r)) <= 1e-6 * max(1.0, abs(L.lo_lj_force(2.0, 0.8, r)))
+
+
+def test_harmonic_and_shifted_pair():
+    L = lo.lib()
+    h = R.HARMONIC_UNIT
+    for r, v in h["energy"]:
+        assert L.lo_pair_interaction_energy(2, h["k"], h["x0"], 10.0, 0, r) == v
+    for r, v in h["force"]:
+        assert L.lo_pair_interaction_force(2, h["k"], h["x0"], 10.0, r) == v
+    sp = R.SHIFTED_PAIR
+    assert ulps_eq(L.lo_pair_interaction_energy(1, sp["sigma"], sp["epsilon"], sp["cutoff"], 1, 2.5),
+                   sp["shifted_energy_2_5"])
+    assert L.lo_pair_interaction_force(1, sp["sigma"], sp["epsilon"], sp["cutoff"], 2.5) == \
+        L.lo_lj_force(sp["sigma"], sp["epsilon"], 2.5)
+    # cutoff is r >= rc (pairs.rs:186, :213)
+    assert L.lo_pair_interaction_energy(1, 1.0, 2.0, 4.0, 1, 4.0) == 0.0
+    assert L.lo_pair_interaction_force(1, 1.0, 2.0, 4.0, 4.0) == 0.0
+    assert L.lo_pair_interaction_energy(1, 1.0, 2.0, 4.0, 1, 4.1) == 0.0
+    assert L.lo_lj_tail_energy(sp["sigma"], sp["epsilon"], sp["cutoff"]) == sp["tail_energy"]
+    assert ulps_eq(L.lo_lj_tail_virial(sp["sigma"], sp["epsilon"], sp["cutoff"]) * (1.0 / 3.0) * 3,
+                   sp["tail_virial_trace"], max_ulps=8)
+    d = R.PAIR_TAIL_DOC
+    assert L.lo_lj_tail_energy(d["sigma"], d["epsilon"], d["cutoff"]) == d["tail_energy"]
+    assert L.lo_lj_tail_virial(d["sigma"], d["epsilon"], d["cutoff"]) * (1.0 / 3.0) == d["tail_virial_diag"]
+
+
+# --- small systems from the reference's unit tests ---------------------------------------
+def test_wolf_two_ions_exact():
+    d = R.WOLF_DOC
+    s = System.from_arrays(["Na", "Cl"], [(0, 0, 0), (2, 0, 0)], UnitCell.cubic(d["cell"]), charges=[1.0, -1.0])
+    s.set_coulomb_potential(Wolf(d["cutoff"]))
+    o = lo.OracleSystem(s)
+    assert o.potential_energy() == d["energy"]
+
+
+def test_ewald_nacl_pair():
+    d = R.EWALD_NACL_PAIR
+    s = System.from_arrays(["Cl", "Na"], [(0, 0, 0), (1.5, 0, 0)], UnitCell.cubic(d["cell"]),
+                           charges=[-1.0, 1.0])
+    s.set_coulomb_potential(Ewald(d["cutoff"], d["kmax"]))
+    o = lo.OracleSystem(s)
+    assert abs(o.potential_energy() - d["energy"]) <= d["abs_tol"]
+    f = o.forces()
+    assert np.linalg.norm(f[0] + f[1]) < 1e-15  # ewald.rs:1081-1085
+    assert f[0][0] > 0 and f[1][0] < 0
+
+
+def test_ewald_water_energy_ulps():
+    s = water_molecule()
+    ew = Ewald(R.EWALD_WATER["cutoff"], R.EWALD_WATER["kmax"])
+    ew.set_restriction(PairRestriction.INTER_MOLECULAR)
+    s.set_coulomb_potential(ew)
+    o = lo.OracleSystem(s)
+    assert ulps_eq(o.potential_energy(), R.EWALD_WATER["energy"])
+
+
+def test_ewald_virial_is_energy():
+    # ewald.rs:1216-1231: tr(W) = E for point charges, rel 1e-3
+    for s in (water_molecule(),
+              System.from_arrays(["Cl", "Na"], [(0, 0, 0), (1.5, 0, 0)], UnitCell.cubic(20.0),
+                                 charges=[-1.0, 1.0])):
+        s.set_coulomb_potential(Ewald(8.0, 10))
+        o = lo.OracleSystem(s)
+        assert rel(np.trace(o.virial()), o.potential_energy()) < 1e-3
+
+
+def test_two_atom_forces_energy_virial():
+    # compute.rs:493-657: two F at 1.3 A, Harmonic k 300 kJ/mol/A^2, x0 1.2
+    d = R.TWO_ATOM
+    s = System.from_arrays(["F", "F"], d["positions"], UnitCell.cubic(d["cell"]))
+    s.set_pair_potential(("F", "F"), PairInteraction(Harmonic(k=units.from_(d["k_kj"], "kJ/mol/A^2"),
+                                                              x0=d["x0"]), 5.0))
+    o = lo.OracleSystem(s)
+    f = o.forces()
+    force = units.from_(30.0, "kJ/mol/A")
+    assert ulps_eq(f[0][0], force) and ulps_eq(f[1][0], -force)
+    assert (f[0] + f[1] == 0).all()
+    w = o.virial()
+    assert ulps_eq(w[0][0], -force * 1.3)
+
+
+def test_molecular_energy_and_virial():
+    # compute.rs:519-557, 608-612, 648-658: 4-atom chain, bonds + angles + dihedral
+    s = System.from_arrays(["F"] * 4, [(0, 0, 0), (1, 0, 0), (1, 1, 0), (2, 1, 0)], UnitCell.cubic(10.0))
+    s.add_bond(0, 1)
+    s.add_bond(1, 2)
+    s.add_bond(2, 3)
+    s.set_pair_potential(("F", "F"), PairInteraction(__import__("lumol_amd").NullPotential(), 0.0))
+    s.set_bond_potential(("F", "F"), Harmonic(k=units.from_(100.0, "kJ/mol/A^2"), x0=2.0))
+    s.set_angle_potential(("F", "F", "F"), Harmonic(k=units.from_(100.0, "kJ/mol") / units.from_(1, "deg") ** 2,
+                                                    x0=units.from_(88.0, "deg")))
+    s.set_dihedral_potential(("F", "F", "F", "F"),
+                             Harmonic(k=units.from_(100.0, "kJ/mol") / units.from_(1, "deg") ** 2,
+                                      x0=units.from_(185.0, "deg")))
+    o = lo.OracleSystem(s)
+    assert ulps_eq(o.potential_energy(), units.from_(1800.0, "kJ/mol"), max_ulps=16)
+    f = o.forces()
+    assert np.linalg.norm(f.sum(0)) ** 2 < 1e-20
+    w = o.virial()
+    unit = units.from_(100.0, "kJ/mol/A")
+    assert ulps_eq(w[0][0], 2 * unit) and ulps_eq(w[1][1], unit)
+
+
+def test_ewald_move_cost_is_energy_difference():
+    # ewald.rs:1242-1335: cost == E_new - E_old at max_relative 1e-12
+    d = R.EWALD_MOVE
+    charges = [-0.8476 if x == "O" else 0.4238 for x in d["names"]]
+    for cutoff in (8.0, 2.0):
+        s = System.from_arrays(d["names"], d["positions"], UnitCell.cubic(d["cell"]), charges=charges)
+        for i, j in d["bonds"]:
+            s.add_bond(i, j)
+        ew = Ewald(cutoff, 10)
+        ew.set_restriction(PairRestriction.INTER_MOLECULAR)
+        s.set_coulomb_potential(ew)
+        o = lo.OracleSystem(s)
+        o.cache_init()
+        e_old = o.potential_energy()
+        cost = o.move_cost(0, np.array(d["new_positions"]))
+        s2 = System.from_arrays(d["names"], d["positions"], UnitCell.cubic(d["cell"]), charges=charges)
+        for i, j in d["bonds"]:
+            s2.add_bond(i, j)
+        p = np.array(d["positions"])
+        p[:3] = d["new_positions"]
+        s2.positions = p
+        s2.set_coulomb_potential(ew)
+        e_new = lo.OracleSystem(s2).potential_energy()
+        assert rel(cost, e_new - e_old) < 1e-12
+
+
+# --- NIST reference systems -------------------------------------------------------------
+@pytest.mark.parametrize("cutoff", [3.0, 4.0])
+@pytest.mark.parametrize("name", ["lj-1.xyz", "lj-2.xyz", "lj-3.xyz", "lj-4.xyz"])
+def test_nist_lj(name, cutoff):
+    (e_ref, e_dec), (w_ref, w_dec), (t_ref, t_dec) = R.NIST_LJ[cutoff][name]
+    o = lo.OracleSystem(nist_lj_system(name, cutoff, tail=False), nthreads=4)
+    energy = o.potential_energy()
+    assert round(energy, e_dec) == e_ref
+    assert round(np.trace(o.virial()), w_dec) == w_ref
+    ot = lo.OracleSystem(nist_lj_system(name, cutoff, tail=True), nthreads=4)
+    assert round(ot.potential_energy() - energy, t_dec) == t_ref
+
+
+@pytest.mark.parametrize("cutoff", [9.0, 10.0])
+@pytest.mark.parametrize("name", ["spce-1.xyz", "spce-2.xyz", "spce-3.xyz", "spce-4.xyz"])
+def test_nist_spce_energies(name, cutoff):
+    total, pairs, tail, coul = R.NIST_SPCE_ENERGIES[cutoff][name]
+    o = lo.OracleSystem(nist_spce_system(name, cutoff), nthreads=8)
+    t = o.energy_terms()
+    kb = R.K_BOLTZMANN
+    assert rel(t["total"] / kb, total) < 1e-3
+    assert rel(t["pairs"] / kb, pairs) < 1e-3
+    assert rel(t["pairs_tail"] / kb, tail) < 1e-3
+    assert rel(t["coulomb"] / kb, coul) < 1e-3
+
+
+@pytest.mark.parametrize("cutoff", [9.0, 10.0])
+@pytest.mark.parametrize("name", ["spce-1.xyz", "spce-2.xyz", "spce-3.xyz", "spce-4.xyz"])
+def test_nist_spce_ewald_components(name, cutoff):
+    real, kspace, self_e = R.NIST_EWALD_COMPONENTS[cutoff][name]
+    o = lo.OracleSystem(nist_spce_system(name, cutoff, kmax=5), nthreads=8)
+    c = o.ewald_components()
+    kb = R.K_BOLTZMANN
+    assert rel(c["real"] / kb, real) < 1e-4
+    assert rel(c["kspace"] / kb, kspace) < (5e-3 if name == "spce-2.xyz" else 2e-3)
+    assert rel(c["self"] / kb, self_e) < 1e-4
+
+
+def _matrix_rel_ok(m, expected, tol):
+    # approx relative_eq on each entry: |a-b| <= tol * max(|a|, |b|)
+    for a, b in zip(np.ravel(m), np.ravel(expected)):
+        if abs(a - b) > tol * max(abs(a), abs(b)):
+            return False
+    return True
+
+
+@pytest.mark.parametrize("cutoff", [9.0, 10.0])
+@pytest.mark.parametrize("name", ["spce-1.xyz", "spce-2.xyz", "spce-3.xyz", "spce-4.xyz"])
+def test_nist_spce_ewald_virials(name, cutoff):
+    kmax, alpha, wreal, wk, tol = R.NIST_EWALD_VIRIALS[cutoff][name]
+    s = nist_spce_system(name, cutoff, kmax=kmax, alpha=alpha)
+    o = lo.OracleSystem(s, nthreads=8)
+    c = o.ewald_components()
+    convert = R.ATM * s.volume()
+    assert _matrix_rel_ok(c["real_virial"], convert * np.array(wreal), tol)
+    assert _matrix_rel_ok(c["kspace_virial"], convert * np.array(wk), 1e-3)
+    e = c["real"] + c["self"] + c["kspace"]
+    assert rel(np.trace(c["real_virial"] + c["kspace_virial"]), e) < 1e-3
+
+
+@pytest.mark.parametrize("cutoff", [9.0, 10.0])
+@pytest.mark.parametrize("name", ["spce-1.xyz", "spce-2.xyz", "spce-3.xyz", "spce-4.xyz"])
+def test_nist_spce_lammps_forces(name, cutoff):
+    kmax, alpha, ffile = R.NIST_SPCE_LAMMPS[cutoff][name]
+    s = nist_spce_system(name, cutoff, kmax=kmax, alpha=alpha, tail=False)
+    o = lo.OracleSystem(s, nthreads=8)
+    f = o.forces() / R.KCAL_PER_MOL
+    expected = read_lammps_forces(ffile)
+    assert f.shape == expected.shape
+    delta = np.abs((f - expected) / expected)
+    if name in ("spce-3.xyz", "spce-4.xyz"):
+        a = np.abs(expected)
+        tol = np.where(a < 1e-1, 1e-1, np.where(a < 1.0, 5e-2, 1e-2))
+        assert (delta < tol).all()
+    else:
+        assert (delta < 5e-3).all()
+
+
+# --- invariants --------------------------------------------------------------------------
+def _fd_force(o, s, atom, axis, eps=1e-6):
+    p = s.positions.copy()
+    p[atom, axis] += eps
+    o.set_positions(p)
+    ep = o.potential_energy()
+    p[atom, axis] -= 2 * eps
+    o.set_positions(p)
+    em = o.potential_energy()
+    o.set_positions(s.positions)
+    return -(ep - em) / (2 * eps)
+
+
+@pytest.mark.parametrize("coulomb", ["ewald", "wolf"])
+def test_forces_are_minus_energy_gradient(coulomb):
+    s = random_molecular_system(n_mol=8, L=13.5, coulomb=coulomb)
+    o = lo.OracleSystem(s)
+    f = o.forces()
+    for atom in (0, 1, 5, 13):
+        for axis in range(3):
+            fd = _fd_force(o, s, atom, axis)
+            assert abs(fd - f[atom, axis]) <= 2e-6 * max(1.0, np.abs(f).max())
+    assert np.abs(f.sum(0)).max() < 1e-9 * np.abs(f).max() * len(f)
+
+
+def test_virial_is_minus_strain_derivative():
+    # ewald.rs:1189-1283 recipe: W_ab = -dE/d(eps_ab) under x -> (1 + eps) x
+    s = random_molecular_system(n_mol=8, L=13.5, coulomb="ewald")
+    s.bond_potentials.clear()
+    s.angle_potentials.clear()
+    o = lo.OracleSystem(s)
+    w = o.virial()
+    eps = 1e-7
+    L0 = s.cell.lengths[0]
+    for a in range(3):
+        es = []
+        for sign in (1, -1):
+            scale = np.ones(3)
+            scale[a] += sign * eps
+            s2 = random_molecular_system(n_mol=8, L=13.5, coulomb="ewald")
+            s2.bond_potentials.clear()
+            s2.angle_potentials.clear()
+            s2.cell = UnitCell.ortho(*(L0 * scale))
+            s2.positions = s.positions * scale
+            es.append(lo.OracleSystem(s2).potential_energy())
+        fd = -(es[0] - es[1]) / (2 * eps)
+        # bonded terms are absent, so the molecules deform affinely
+        assert abs(fd - w[a][a]) <= 1e-5 * abs(w[a][a]) + 1e-9
+
+
+def test_madelung_constant_rocksalt():
+    # unjittered rocksalt, Coulomb only, converged Ewald: E = -N/2 * M / (4 pi eps0 r0)
+    s = nacl(ncell=4, jitter=0.0, cutoff=10.0)
+    s.pairs.clear()
+    L = s.cell.lengths[0]
+    s.set_coulomb_potential(Ewald(L / 2 - 1e-6, 14, alpha=6.0 / (L / 2)))
+    o = lo.OracleSystem(s, nthreads=8)
+    e = o.energy_terms()["coulomb"]
+    r0 = 5.64 / 2
+    m = -e * R.FOUR_PI_EPSILON_0 * r0 / (s.size() / 2)
+    assert abs(m - 1.747565) < 2e-6
+
+
+def test_pair_set_matches_brute_force():
+    # min-image with round-half-away-from-zero and the strict r < rc test
+    rng = np.random.default_rng(11)
+    n, L, rc = 300, 17.0, 6.0
+    pos = rng.uniform(-2 * L, 2 * L, size=(n, 3))  # unwrapped coordinates
+    s = System.from_arrays(["Ar"] * n, pos, UnitCell.cubic(L))
+    o = lo.OracleSystem(s)
+    got = {tuple(x) for x in o.pairs_within(rc)}
+    d = pos[:, None, :] - pos[None, :, :]
+    d = d - np.where(d / L >= 0, np.floor(d / L + 0.5), np.ceil(d / L - 0.5)) * L
+    r = np.sqrt((d * d).sum(-1))
+    want = {(i, j) for i in range(n) for j in range(i + 1, n) if r[i, j] < rc}
+    assert got == want and len(want) > 1000
+
+
+def test_velocity_verlet_harmonic_oscillator():
+    # Two atoms on a harmonic spring, infinite cell.  The relative coordinate
+    # u of Velocity-Verlet on a harmonic oscillator obeys the Stormer-Verlet
+    # recurrence u[n+1] = (2 - (w dt)^2) u[n] - u[n-1], w^2 = 2k/m, once the
+    # accelerations are populated (integrators.rs:40-42 starts from a = 0, so
+    # the first step only drifts).  A dropped or doubled half-kick breaks it.
+    k, m, x0, a0 = 0.02, 1.0, 1.5, 0.1
+    s = System.from_arrays(["A", "A"], [(0, 0, 0), (x0 + a0, 0, 0)], UnitCell.infinite(), masses=[m, m])
+    s.set_pair_potential(("A", "A"), PairInteraction(Harmonic(k=k, x0=x0), 100.0))
+    o = lo.OracleSystem(s)
+    dt, nsteps = 0.5, 300
+    w2 = 2 * k / m
+    o.md_reset()
+    us = []
+    for _ in range(nsteps):
+        o.md_run(0, dt, 1)
+        p = o.positions()
+        us.append(p[1, 0] - p[0, 0] - x0)
+    us = np.array(us)
+    assert us[0] == pytest.approx(a0, abs=1e-15)  # first step: v = 0, a = 0
+    resid = us[2:] - (2 - w2 * dt * dt) * us[1:-1] + us[:-2]
+    assert np.abs(resid).max() < 1e-12
+    # bounded amplitude of the discrete map
+    assert np.abs(us).max() < a0 * 1.05
+
+
+def test_velocity_verlet_time_reversible():
+    s = random_molecular_system(n_mol=8, L=13.5, coulomb="wolf")
+    rng = np.random.default_rng(2)
+    s.masses = np.where(np.array(s.names) == "O", 15.999, 1.008)
+    s.velocities = rng.normal(0, 0.002, size=(s.size(), 3))
+    o = lo.OracleSystem(s)
+    p0 = o.positions()
+    o.md_reset()
+    o.md_run(0, 0.25, 50)
+    # reverse: negate velocities; the stored accelerations are those of the
+    # final positions, which is what the reversed first half-kick needs
+    v = o.velocities()
+    lo.lib().lo_set_velocities(o.h, np.ascontiguousarray(-v).ctypes.data_as(lo.DP))
+    o.md_run(0, 0.25, 50)
+    assert np.abs(o.positions() - p0).max() < 1e-9
