@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Throughput of Lumol's force-and-energy hot path on one MI355X.
+
+Workload (default): NaCl Ewald NVT -- 4096 ions (2048 Na+, 2048 Cl-) on 8^3
+rocksalt cells, a = 5.64 A (box 45.12 A), seeded +-0.05 A jitter,
+Lorentz-Berthelot LJ + Ewald (rc 12 A, kmax 9, alpha = pi/rc), Berendsen
+1200 K (tau = 100 steps), Velocity-Verlet dt = 1 fs.  One step = one MD step
+on the device: integrator halves, neighbour-list maintenance, pair kernel
+(LJ + Ewald real space), k-space rho(k) and forces, thermostat.
+
+Prints ONE JSON line: atom-steps/s over `--steps` timed steps after
+`--warmup` untimed steps, the dominant kernel's roofline fraction (HIP
+events on the library's stream, algorithmic FLOP per launch from the
+oracle's pair-set convention), the form's force error against the FP64
+oracle, and the oracle's threaded CPU throughput on this host.
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom-steps/s, FP64 vs mixed at stated force error; p50/p99 job latency"
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic FLOP per work unit (add/sub/mul = 1, FMA = 2, div/sqrt/exp/erfc = 1),
+# counted on the reference's pair set (unordered pairs within the cutoff):
+#   fused LJ + Ewald real-space pair, forces only (the MD step computes no
+#   energy or virial): geometry 22 (d 3, min-image 12, r^2 5, sqrt 1, 1/r 1),
+#   LJ 10, Ewald real 16, force vector and +/- accumulation 10  -> 58
+#   (with energy and virial: 91, the figure of the survey's measurement table)
+#   k-space per (k-vector, atom): rho 16, force 21.
+FLOP_PAIR_LJ_EWALD_F = 58
+FLOP_KSPACE_RHO = 16
+FLOP_KSPACE_FORCE = 21
+BYTES_INTEGRATE = 128  # per atom-step, FP64 (x, v, F read + 1/m; x, v written)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(target_seconds=12.0):
+    """The oracle's threaded FP64 MD step (the reference's algorithm: O(N^2)
+    pair loops with per-thread force buffers, Ewald recomputing e^{ik.r} per
+    call) on a bounded sample of the same workload: 1 warm-up step, then 5
+    repetitions of R steps (R sized to ~target/5 s each); median."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lumol_oracle as lo
+    from lumol_amd.workloads import nacl
+
+    s = nacl()
+    cores = host_cores()
+    o = lo.OracleSystem(s, nthreads=cores)
+    o.md_reset()
+    t0 = time.perf_counter()
+    o.md_run(0, 1.0, 1, 1200.0, 100.0)
+    t1 = time.perf_counter() - t0
+    reps = 5
+    r = max(1, int(round(target_seconds / reps / max(t1, 1e-3))))
+    rates = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        o.md_run(0, 1.0, r, 1200.0, 100.0)
+        dt = time.perf_counter() - t0
+        rates.append(s.size() * r / dt)
+    return {"value": statistics.median(rates), "unit": "atom-steps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle FP64 VV+Berendsen on the same 4096-ion NaCl Ewald system: 1 warm-up step, "
+                      f"median of {reps} x {r} steps, {cores} threads"}
+
+
+def force_error(s):
+    """Force RMS / max relative error of the device form against the FP64
+    oracle on the current configuration (max over atoms with |F| > 1e-3 F_rms)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lumol_oracle as lo
+    o = lo.OracleSystem(s, nthreads=host_cores())
+    fg, fo = s.forces(), o.forces()
+    d = fg - fo
+    rms = math.sqrt((d * d).sum() / (fo * fo).sum())
+    norms = np.linalg.norm(fo, axis=1)
+    frms = math.sqrt((norms ** 2).mean())
+    mask = norms > 1e-3 * frms
+    mx = float((np.linalg.norm(d, axis=1)[mask] / norms[mask]).max())
+    eg, eo = s.potential_energy(), o.potential_energy()
+    return {"force_rms_rel": rms, "force_max_rel": mx, "energy_rel": abs(eg - eo) / abs(eo)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the GPU boxes of this build have one MI355X, and independent "
+              "systems/jobs shard across GPUs without communication (see DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from lumol_amd import BerendsenThermostat, MolecularDynamics, Simulation, VelocityVerlet
+    from lumol_amd.workloads import nacl
+
+    s = nacl()
+    natoms = s.size()
+    rc = 12.0
+    n_pairs = len(s.pairs_within(rc))
+    nk = s.ewald_kvector_count()
+    err = force_error(s)
+
+    md = MolecularDynamics(VelocityVerlet(1.0))
+    md.set_thermostat(BerendsenThermostat(1200.0, 100.0))
+    sim = Simulation(md)
+
+    # warm-up, with a profiled pass over every kernel class to find the dominant one
+    s.profile_begin(["nlist", "pairs", "molecules", "kspace_rho", "kspace_force", "integrate"])
+    sim.run(s, args.warmup)
+    warm = s.profile_end()
+    # the dominant single kernel (nlist and integrate are sequences of launches)
+    dominant = max(("pairs", "kspace_rho", "kspace_force", "molecules"), key=lambda c: warm[c][0])
+
+    # timed region: only the dominant class carries events
+    s.profile_begin([dominant])
+    t0 = time.perf_counter()
+    sim.run(s, args.steps)  # returns after the stream is synchronised
+    elapsed = time.perf_counter() - t0
+    prof = s.profile_end()
+    value = natoms * args.steps / elapsed
+
+    ms, launches = prof[dominant]
+    avg_s = ms / 1e3 / max(launches, 1)
+    if dominant == "pairs":
+        work, unit, peak, bound = n_pairs * FLOP_PAIR_LJ_EWALD_F / 1e12, "TFLOP/s", FP64_VECTOR_PEAK_TFLOPS, "valu"
+        per = f"{FLOP_PAIR_LJ_EWALD_F} FLOP x {n_pairs} pairs (r < {rc}) per launch"
+    elif dominant == "kspace_force":
+        work, unit, peak, bound = nk * natoms * FLOP_KSPACE_FORCE / 1e12, "TFLOP/s", FP64_VECTOR_PEAK_TFLOPS, "valu"
+        per = f"{FLOP_KSPACE_FORCE} FLOP x {nk} k-vectors x {natoms} atoms per launch"
+    elif dominant == "kspace_rho":
+        work, unit, peak, bound = nk * natoms * FLOP_KSPACE_RHO / 1e12, "TFLOP/s", FP64_VECTOR_PEAK_TFLOPS, "valu"
+        per = f"{FLOP_KSPACE_RHO} FLOP x {nk} k-vectors x {natoms} atoms per launch"
+    else:
+        work, unit, peak, bound = natoms * BYTES_INTEGRATE / 1e9, "GB/s", HBM_PEAK_GBS, "hbm"
+        per = f"{BYTES_INTEGRATE} B x {natoms} atoms per launch"
+    achieved = work / avg_s
+    roofline = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
+                "traffic": None, "kernel": dominant, "avg_launch_us": avg_s * 1e6, "work": per}
+
+    kernels = {c: {"ms_per_step": warm[c][0] / max(args.warmup, 1), "launches": warm[c][1]}
+               for c in warm if warm[c][1]}
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded rocksalt lattice + jitter, Maxwell-Boltzmann velocities)",
+        "config": {"workload": "NaCl Ewald NVT, 4096 ions, 8x8x8 rocksalt a=5.64 A, LJ(LB)+Ewald rc 12 A "
+                               "kmax 9, Berendsen 1200 K, VV dt 1 fs",
+                   "form": "fp64", "atoms": natoms, "pairs_within_rc": n_pairs, "kvectors": nk},
+        "error_vs_fp64_oracle": err,
+        "roofline": roofline,
+        "kernels_warmup": kernels,
+    }
+    if not args.no_cpu_baseline:
+        base = cpu_baseline(args.cpu_seconds)
+        line["cpu_baseline"] = base
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

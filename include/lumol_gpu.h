@@ -205,6 +205,28 @@ int lg_move_commit(lg_system* sys, int wrap);
 /* Rejected move: drop the staged update (monte_carlo.rs:318-321). */
 int lg_move_discard(lg_system* sys);
 
+/* ---- kernel timing ---------------------------------------------------------------
+ * No counterpart in the reference (its benches time whole calls with
+ * criterion, benches/argon.rs etc.).  Between begin and end, every launch of a kernel
+ * class in `class_mask` (bit = LG_KERNEL_*) is bracketed by HIP events on the
+ * system's stream; end synchronises and returns the summed device time and
+ * the launch count per class. */
+enum {
+    LG_KERNEL_NLIST = 0,        /* displacement check + cell list + Verlet list */
+    LG_KERNEL_PAIRS = 1,        /* k_pairs: LJ + Ewald real space / Wolf         */
+    LG_KERNEL_MOLECULES = 2,    /* k_molecules: intra pairs, bonds, angles       */
+    LG_KERNEL_KSPACE_RHO = 3,   /* k_rho_partial + k_rho_reduce                  */
+    LG_KERNEL_KSPACE_FORCE = 4, /* k_kspace_force                                */
+    LG_KERNEL_INTEGRATE = 5,    /* integrator halves + thermostat                */
+    LG_KERNEL_CLASSES = 6
+};
+typedef struct lg_profile {
+    double ms[6];
+    uint64_t launches[6];
+} lg_profile;
+int lg_profile_begin(lg_system* sys, uint32_t class_mask);
+int lg_profile_end(lg_system* sys, lg_profile* out);
+
 #ifdef __cplusplus
 }
 #endif

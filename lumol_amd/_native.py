@@ -33,8 +33,16 @@ EXPORTS = [
     "lg_coulomb_ewald", "lg_coulomb_wolf", "lg_coulomb_clear", "lg_forces",
     "lg_potential_energy", "lg_virial", "lg_kinetic_energy", "lg_temperature",
     "lg_ewald_kvector_count", "lg_pairs_within", "lg_md_setup", "lg_md_run", "lg_cache_init", "lg_move_cost",
-    "lg_move_commit", "lg_move_discard",
+    "lg_move_commit", "lg_move_discard", "lg_profile_begin", "lg_profile_end",
 ]
+
+LG_KERNEL_NLIST, LG_KERNEL_PAIRS, LG_KERNEL_MOLECULES = 0, 1, 2
+LG_KERNEL_KSPACE_RHO, LG_KERNEL_KSPACE_FORCE, LG_KERNEL_INTEGRATE = 3, 4, 5
+KERNEL_CLASSES = ["nlist", "pairs", "molecules", "kspace_rho", "kspace_force", "integrate"]
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * 6), ("launches", ctypes.c_uint64 * 6)]
 
 
 class EnergyTerms(ctypes.Structure):
@@ -103,6 +111,8 @@ def _declare(lib):
         "lg_move_cost": (I, [P, U32, DP, DP]),
         "lg_move_commit": (I, [P, I]),
         "lg_move_discard": (I, [P]),
+        "lg_profile_begin": (I, [P, U32]),
+        "lg_profile_end": (I, [P, ctypes.POINTER(Profile)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -163,12 +163,17 @@ struct lg_system {
     DevBuf<int4> ipairs, d_bonds, d_angles, d_dihedrals;
     DevBuf<BondedParam> bparam, aparam, dparam;
     // k-space
-    int nk = 0, nrows = 0, rho_width = 64, force_width = 64;
+    int nk = 0, nrows = 0, rho_width = 64, force_width = 64, ksplit = 1;
+    int tpa = 8;  // lanes per atom in k_pairs
+    DevBuf<double4> fk;  // k-space partial forces [ksplit][n]
     DevBuf<int4> kidx;
     DevBuf<KRow> rows;
     DevBuf<double> efac, vfac;
     DevBuf<double4> field;
     DevBuf<double2> rho, rho_partial;
+    DevBuf<double2> phase_tab;  // e^{i n k_axis x}, phase_rows(kmax) x ld
+    DevBuf<double> phase_q;     // charges padded to ld
+    int ld = 0;
     // energy partials
     DevBuf<double> epart_pairs, epart_mol, epart_k, ke_part, lambda;
     // MD
@@ -177,11 +182,36 @@ struct lg_system {
     bool md_ready = false;
     // snapshot for neighbour-list overflow recovery
     DevBuf<double4> snap_pos, snap_vel, snap_acc;
+    // kernel timing (lg_profile_*)
+    uint32_t prof_mask = 0;
+    std::vector<hipEvent_t> prof_events;
+    std::vector<std::pair<int, size_t>> prof_records;  // (class, first event index)
+    size_t prof_used = 0;
 };
 
 namespace {
 
 hipStream_t S(lg_system* s) { return s->ctx->stream; }
+
+// Event bracket around a kernel class; returns the record index or -1.
+long prof_start(lg_system* s, int cls) {
+    if (!(s->prof_mask & (1u << cls))) return -1;
+    while (s->prof_events.size() < s->prof_used + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        s->prof_events.push_back(e);
+    }
+    size_t idx = s->prof_used;
+    s->prof_used += 2;
+    if (hipEventRecord(s->prof_events[idx], s->ctx->stream) != hipSuccess) return -1;
+    s->prof_records.push_back({cls, idx});
+    return long(s->prof_records.size() - 1);
+}
+
+void prof_stop(lg_system* s, long rec) {
+    if (rec < 0) return;
+    (void)hipEventRecord(s->prof_events[s->prof_records[size_t(rec)].second + 1], s->ctx->stream);
+}
 
 int blocks_for(size_t n, int b) { return int((n + b - 1) / b); }
 
@@ -509,13 +539,22 @@ int prepare_kspace(lg_system* s) {
     size_t rows_bytes = size_t(phase_rows(kmax)) * sizeof(double2);
     size_t maxs = s->ctx->max_shared;
     s->rho_width = 64;
-    while (s->rho_width > 8 && rows_bytes * s->rho_width + s->rho_width * sizeof(double) > maxs) s->rho_width /= 2;
+    while (s->rho_width > 8 && rows_bytes * (s->rho_width + 1) + s->rho_width * sizeof(double) > maxs)
+        s->rho_width /= 2;
     s->force_width = 64;
-    while (s->force_width > 8 && rows_bytes * s->force_width > maxs) s->force_width /= 2;
-    if (rows_bytes * 8 > maxs) return fail(LG_E_UNSUPPORTED, "kmax too large for the k-space LDS tables");
+    while (s->force_width > 8 && rows_bytes * (s->force_width + 1) > maxs) s->force_width /= 2;
+    if (rows_bytes * 9 > maxs || kmax > 24)
+        return fail(LG_E_UNSUPPORTED, "kmax too large for the k-space LDS tables");
+    // enough waves to fill the device: split the rows when there are few atoms
+    int fwaves = blocks_for(s->n, 64);
+    s->ksplit = std::max(1, std::min(32, std::min(s->nrows, (2048 + fwaves - 1) / fwaves)));
+    LG_TRY(s->fk.ensure(size_t(s->ksplit) * s->n));
     int nchunks = blocks_for(s->n, s->rho_width);
     LG_TRY(s->rho_partial.ensure(size_t(nchunks) * s->nk));
-    LG_TRY(s->epart_k.ensure(size_t(blocks_for(s->nk, 256)) * 7));
+    s->ld = blocks_for(s->n, 64) * 64;
+    LG_TRY(s->phase_tab.ensure(size_t(phase_rows(kmax)) * s->ld));
+    LG_TRY(s->phase_q.ensure(s->ld));
+    LG_TRY(s->epart_k.ensure(size_t(blocks_for(s->nk, 64)) * 7));
     return LG_OK;
 }
 
@@ -571,11 +610,14 @@ int prepare_nlist(lg_system* s) {
     LG_TRY(s->sorted.ensure(n));
     LG_TRY(s->nbr.ensure(size_t(s->cap) * n));
     LG_TRY(s->nnbr.ensure(n));
-    LG_TRY(s->flags.ensure(2));
+    LG_TRY(s->flags.ensure(4));
     LG_TRY(s->ref.ensure(n));
-    LG_TRY(s->epart_pairs.ensure(size_t(blocks_for(n, kPairBlock)) * 8));
-    int one[2] = {1, 0};  // force a rebuild, clear overflow
-    LG_HIP(hipMemcpyAsync(s->flags.p, one, sizeof(one), hipMemcpyHostToDevice, S(s)));
+    // lanes per atom: enough waves to fill 256 CUs, fewer lanes for long lists
+    s->tpa = n <= 8192 ? 64 : n <= 32768 ? 32 : n <= 131072 ? 16 : n <= 524288 ? 8 : 4;
+    LG_TRY(s->epart_pairs.ensure(size_t(blocks_for(n * size_t(s->tpa), kPairBlock)) * 8));
+    int init[4] = {1, 0, 0, 0};  // force a rebuild; clear overflow and block counters
+    LG_HIP(hipMemcpyAsync(s->flags.p, init, sizeof(init), hipMemcpyHostToDevice, S(s)));
+    LG_HIP(hipMemsetAsync(s->count.p, 0, g.ncell * sizeof(int), S(s)));
     s->nlist_dirty = false;
     (void)rl2;
     return LG_OK;
@@ -603,33 +645,49 @@ int prepare(lg_system* s) {
 }
 
 // ---- kernel sequences --------------------------------------------------------------------
-int launch_nlist(lg_system* s, bool force_rebuild) {
+// Neighbour-list maintenance.  `check` runs the displacement test here; the
+// MD step does it inside the integrator's first half instead.  Every kernel
+// returns at once when flags[0] (rebuild) is clear; the build kernel clears
+// it again.
+int launch_nlist(lg_system* s, bool check) {
     hipStream_t st = S(s);
+    long prof = prof_start(s, LG_KERNEL_NLIST);
     int n = int(s->n);
     int* flag = s->flags.p;
-    if (force_rebuild) {
-        int one = 1;
-        LG_HIP(hipMemcpyAsync(flag, &one, sizeof(int), hipMemcpyHostToDevice, st));
-    } else {
+    if (check) {
         double hs = 0.5 * s->skin;
         k_nlist_check<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->ref.p, n, hs * hs, flag);
     }
     const Grid& g = s->grid;
-    k_cell_zero<<<blocks_for(g.ncell, 256), 256, 0, st>>>(s->count.p, g.ncell, flag);
     k_cell_count<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, n, g, s->box, s->cellid.p, s->slot.p, s->count.p,
                                                       flag);
     k_cell_scan<<<1, 1024, 0, st>>>(s->count.p, s->start.p, g.ncell, flag);
     k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(s->cellid.p, s->slot.p, s->start.p, s->tmp.p, n, flag);
     k_cell_rank<<<g.ncell, 256, 0, st>>>(s->tmp.p, s->start.p, s->sorted.p, g.ncell, flag);
-    k_nlist_build<<<blocks_for(n, 128), 128, 0, st>>>(s->pos.p, s->meta.p, s->sorted.p, s->cellid.p, s->start.p, g,
-                                                       s->box, rlist2_of(s), n, s->cap, s->nbr.p, s->nnbr.p,
-                                                       flag + 1, s->ref.p, flag);
-    k_flag_clear<<<1, 64, 0, st>>>(flag);
+    int nb = blocks_for(size_t(n) * s->tpa, 256);
+    double rl2 = rlist2_of(s);
+    switch (s->tpa) {
+#define LG_BUILD_CASE(T)                                                                                   \
+    case T:                                                                                                \
+        k_nlist_build<T><<<nb, 256, 0, st>>>(s->pos.p, s->meta.p, s->sorted.p, s->cellid.p, s->start.p,       \
+                                             s->count.p, g, s->box, rl2, n, s->cap, s->nbr.p, s->nnbr.p,     \
+                                             s->ref.p, flag);                                                 \
+        break;
+        LG_BUILD_CASE(4)
+        LG_BUILD_CASE(8)
+        LG_BUILD_CASE(16)
+        LG_BUILD_CASE(32)
+        LG_BUILD_CASE(64)
+#undef LG_BUILD_CASE
+        default:
+            return fail(LG_E_DEVICE, "invalid lanes-per-atom setting");
+    }
+    prof_stop(s, prof);
     LG_HIP(hipGetLastError());
     return LG_OK;
 }
 
-int launch_forces(lg_system* s, bool energy) {
+int launch_forces(lg_system* s, bool energy, bool combine_kspace = true) {
     hipStream_t st = S(s);
     int n = int(s->n);
     PairArgs pa;
@@ -638,6 +696,7 @@ int launch_forces(lg_system* s, bool energy) {
     pa.sorted = s->sorted.p;
     pa.nbr = s->nbr.p;
     pa.nnbr = s->nnbr.p;
+    pa.cap = s->cap;
     pa.table = s->table.p;
     pa.nkinds = int(s->nkinds);
     pa.coul = s->coul;
@@ -645,10 +704,26 @@ int launch_forces(lg_system* s, bool energy) {
     pa.n = n;
     pa.force = s->force.p;
     pa.epart = s->epart_pairs.p;
-    if (energy)
-        k_pairs<true><<<blocks_for(n, kPairBlock), kPairBlock, 0, st>>>(pa);
-    else
-        k_pairs<false><<<blocks_for(n, kPairBlock), kPairBlock, 0, st>>>(pa);
+    long prof = prof_start(s, LG_KERNEL_PAIRS);
+    int pblocks = blocks_for(size_t(n) * s->tpa, kPairBlock);
+    switch (s->tpa) {
+#define LG_PAIRS_CASE(T)                                                   \
+    case T:                                                                \
+        if (energy)                                                        \
+            k_pairs<true, T><<<pblocks, kPairBlock, 0, st>>>(pa);          \
+        else                                                               \
+            k_pairs<false, T><<<pblocks, kPairBlock, 0, st>>>(pa);         \
+        break;
+        LG_PAIRS_CASE(4)
+        LG_PAIRS_CASE(8)
+        LG_PAIRS_CASE(16)
+        LG_PAIRS_CASE(32)
+        LG_PAIRS_CASE(64)
+#undef LG_PAIRS_CASE
+        default:
+            return fail(LG_E_DEVICE, "invalid lanes-per-atom setting");
+    }
+    prof_stop(s, prof);
     if (s->nmol_multi > 0) {
         MolArgs ma;
         ma.pos = s->pos.p;
@@ -679,10 +754,12 @@ int launch_forces(lg_system* s, bool energy) {
         ma.force = s->force.p;
         ma.epart = s->epart_mol.p;
         int nb = blocks_for(s->nmol_multi, 256);
+        long pm = prof_start(s, LG_KERNEL_MOLECULES);
         if (energy)
             k_molecules<true><<<nb, 256, 0, st>>>(ma);
         else
             k_molecules<false><<<nb, 256, 0, st>>>(ma);
+        prof_stop(s, pm);
     }
     if (s->coul_kind == COUL_EWALD && s->nk > 0) {
         int kmax = s->coul_kmax;
@@ -690,19 +767,26 @@ int launch_forces(lg_system* s, bool energy) {
         int w = s->rho_width;
         int nchunks = blocks_for(n, w);
         dim3 grid(blocks_for(s->nk, kRhoBlock), nchunks);
-        size_t shm = rows_bytes * w + w * sizeof(double);
-        k_rho_partial<<<grid, kRhoBlock, shm, st>>>(s->pos.p, n, s->box, kmax, w, s->kidx.p, s->nk,
+        size_t shm = rows_bytes * (w + 1) + w * sizeof(double);
+        long pr = prof_start(s, LG_KERNEL_KSPACE_RHO);
+        k_phase<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                                                                    s->phase_tab.p, s->phase_q.p);
+        k_rho_partial<<<grid, kRhoBlock, shm, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w, s->kidx.p, s->nk,
                                                     s->rho_partial.p);
         if (energy)
-            k_rho_reduce<true><<<blocks_for(s->nk, 256), 256, 0, st>>>(s->rho_partial.p, nchunks, s->nk, s->rho.p,
-                                                                      s->efac.p, s->vfac.p, s->epart_k.p);
+            k_rho_reduce<true><<<blocks_for(s->nk, 64), 256, 0, st>>>(s->rho_partial.p, nchunks, s->nk, s->rho.p,
+                                                                     s->efac.p, s->vfac.p, s->epart_k.p);
         else
-            k_rho_reduce<false><<<blocks_for(s->nk, 256), 256, 0, st>>>(s->rho_partial.p, nchunks, s->nk, s->rho.p,
-                                                                       s->efac.p, s->vfac.p, s->epart_k.p);
-        int fw = s->force_width;
-        k_kspace_force<<<blocks_for(n, fw), fw, rows_bytes * fw, st>>>(s->pos.p, n, s->box, kmax, s->rows.p,
-                                                                       s->nrows, s->kidx.p, s->rho.p, s->field.p,
-                                                                       s->force.p);
+            k_rho_reduce<false><<<blocks_for(s->nk, 64), 256, 0, st>>>(s->rho_partial.p, nchunks, s->nk, s->rho.p,
+                                                                      s->efac.p, s->vfac.p, s->epart_k.p);
+        prof_stop(s, pr);
+        long pf = prof_start(s, LG_KERNEL_KSPACE_FORCE);
+        dim3 fgrid(blocks_for(n, 256), s->ksplit);
+        k_kspace_force<<<fgrid, 256, 0, st>>>(s->phase_tab.p, s->ld, n, kmax, s->rows.p, s->nrows, s->kidx.p,
+                                               s->rho.p, s->field.p, s->fk.p);
+        if (combine_kspace)
+            k_kspace_combine<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->fk.p, s->ksplit, n, s->force.p);
+        prof_stop(s, pf);
     }
     LG_HIP(hipGetLastError());
     return LG_OK;
@@ -716,7 +800,7 @@ int check_overflow(lg_system* s, bool* overflowed) {
     if (ov > 0) {
         s->cap = int(std::min<size_t>(s->n, size_t(ov) + size_t(ov) / 4 + 32));
         LG_TRY(s->nbr.ensure(size_t(s->cap) * s->n));
-        int z[2] = {1, 0};
+        int z[4] = {1, 0, 0, 0};
         LG_HIP(hipMemcpyAsync(s->flags.p, z, sizeof(z), hipMemcpyHostToDevice, S(s)));
     }
     return LG_OK;
@@ -726,7 +810,7 @@ int check_overflow(lg_system* s, bool* overflowed) {
 int compute(lg_system* s, bool energy) {
     LG_TRY(prepare(s));
     for (int attempt = 0; attempt < 4; attempt++) {
-        LG_TRY(launch_nlist(s, false));
+        LG_TRY(launch_nlist(s, true));
         LG_TRY(launch_forces(s, energy));
         bool ov = false;
         LG_TRY(check_overflow(s, &ov));
@@ -743,7 +827,7 @@ struct Energies {
 
 int gather_energies(lg_system* s, Energies& E) {
     hipStream_t st = S(s);
-    int nbp = blocks_for(s->n, kPairBlock);
+    int nbp = blocks_for(s->n * size_t(s->tpa), kPairBlock);
     std::vector<double> hp(size_t(nbp) * 8);
     LG_HIP(hipMemcpyAsync(hp.data(), s->epart_pairs.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     std::vector<double> hm;
@@ -753,7 +837,7 @@ int gather_energies(lg_system* s, Energies& E) {
         LG_HIP(hipMemcpyAsync(hm.data(), s->epart_mol.p, hm.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     }
     std::vector<double> hk;
-    int nbk = (s->coul_kind == COUL_EWALD && s->nk > 0) ? blocks_for(s->nk, 256) : 0;
+    int nbk = (s->coul_kind == COUL_EWALD && s->nk > 0) ? blocks_for(s->nk, 64) : 0;
     if (nbk) {
         hk.resize(size_t(nbk) * 7);
         LG_HIP(hipMemcpyAsync(hk.data(), s->epart_k.p, hk.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -935,15 +1019,19 @@ int lg_system_destroy(lg_system* s) {
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     DevBuf<int>* ibufs[] = {&s->cellid, &s->slot, &s->count, &s->start, &s->tmp, &s->sorted, &s->nbr, &s->nnbr,
+
                             &s->flags, &s->pair_restr, &s->mol_list, &s->d_mol_start, &s->ip_start, &s->b_start,
                             &s->a_start, &s->d_start, &s->d_param};
     for (auto* b : ibufs) b->release();
-    DevBuf<double4>* d4[] = {&s->pos, &s->vel, &s->acc, &s->force, &s->ref, &s->field,
+    s->phase_tab.release();
+    s->phase_q.release();
+    DevBuf<double4>* d4[] = {&s->pos, &s->vel, &s->acc, &s->force, &s->ref, &s->field, &s->fk,
                              &s->snap_pos, &s->snap_vel, &s->snap_acc};
     for (auto* b : d4) b->release();
     DevBuf<double>* d1[] = {&s->pair_s14, &s->efac, &s->vfac, &s->epart_pairs, &s->epart_mol, &s->epart_k,
                             &s->ke_part, &s->lambda};
     for (auto* b : d1) b->release();
+    for (auto e : s->prof_events) (void)hipEventDestroy(e);
     s->meta.release();
     s->table.release();
     s->ipairs.release();
@@ -1282,8 +1370,8 @@ int lg_pairs_within(lg_system* s, double cutoff, uint32_t* pairs, uint64_t capac
     LG_TRY(out.ensure(std::max<uint64_t>(capacity, 1)));
     LG_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), S(s)));
     int n = int(s->n);
-    k_pairs_within<<<blocks_for(n, 256), 256, 0, S(s)>>>(s->pos.p, s->sorted.p, s->nbr.p, s->nnbr.p, n, s->box,
-                                                         cutoff, out.p, capacity, cnt.p);
+    k_pairs_within<<<blocks_for(n, 256), 256, 0, S(s)>>>(s->pos.p, s->sorted.p, s->nbr.p, s->nnbr.p, n, s->cap,
+                                                         s->box, cutoff, out.p, capacity, cnt.p);
     LG_HIP(hipGetLastError());
     unsigned long long c = 0;
     LG_HIP(hipMemcpyAsync(&c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, S(s)));
@@ -1344,22 +1432,34 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
         LG_HIP(hipMemcpyAsync(s->snap_pos.p, s->pos.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
         LG_HIP(hipMemcpyAsync(s->snap_vel.p, s->vel.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
         LG_HIP(hipMemcpyAsync(s->snap_acc.p, s->acc.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+        bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
+        double hs = 0.5 * s->skin;
+        int nb = blocks_for(n, 256);
         for (uint64_t k = 0; k < todo; k++) {
             const double* lam = (thermo && k > 0) ? s->lambda.p : nullptr;
+            long p1 = prof_start(s, LG_KERNEL_INTEGRATE);
             if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
-                k_vv_first<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, lam);
+                k_vv_first<<<nb, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, lam, s->ref.p, hs * hs,
+                                               s->flags.p);
             else
-                k_lf_first<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, lam);
+                k_lf_first<<<nb, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, lam, s->ref.p, hs * hs,
+                                               s->flags.p);
+            prof_stop(s, p1);
             LG_TRY(launch_nlist(s, false));
-            LG_TRY(launch_forces(s, false));
+            LG_TRY(launch_forces(s, false, false));
+            long p2 = prof_start(s, LG_KERNEL_INTEGRATE);
             double* kp = thermo ? s->ke_part.p : nullptr;
+            int ks = kspace ? s->ksplit : 0;
             if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
-                k_vv_second<<<blocks_for(n, 256), 256, 0, st>>>(s->force.p, s->vel.p, s->acc.p, n, s->dt, kp);
+                k_md_second<false><<<nb, 256, 0, st>>>(s->force.p, s->pos.p, s->fk.p, ks, s->vel.p, s->acc.p, n,
+                                                       s->dt, kp);
             else
-                k_lf_second<<<blocks_for(n, 256), 256, 0, st>>>(s->force.p, s->vel.p, s->acc.p, n, s->dt, kp);
+                k_md_second<true><<<nb, 256, 0, st>>>(s->force.p, s->pos.p, s->fk.p, ks, s->vel.p, s->acc.p, n,
+                                                      s->dt, kp);
             if (thermo)
-                k_berendsen<<<1, 256, 0, st>>>(s->ke_part.p, blocks_for(n, 256), double(3 * s->n), s->berendsen_T,
-                                               s->berendsen_tau, s->lambda.p);
+                k_berendsen<<<1, 256, 0, st>>>(s->ke_part.p, nb, double(3 * s->n), s->berendsen_T, s->berendsen_tau,
+                                               s->lambda.p);
+            prof_stop(s, p2);
         }
         if (thermo) k_scale_vel<<<blocks_for(n, 256), 256, 0, st>>>(s->vel.p, n, s->lambda.p);
         LG_HIP(hipGetLastError());
@@ -1375,6 +1475,32 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
         done += todo;
     }
     LG_HIP(hipStreamSynchronize(st));
+    return LG_OK;
+}
+
+int lg_profile_begin(lg_system* s, uint32_t mask) {
+    if (!s) return fail(LG_E_INVALID, "null system");
+    s->prof_mask = mask & ((1u << LG_KERNEL_CLASSES) - 1);
+    s->prof_used = 0;
+    s->prof_records.clear();
+    return LG_OK;
+}
+
+int lg_profile_end(lg_system* s, lg_profile* out) {
+    if (!s || !out) return fail(LG_E_INVALID, "null argument");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_HIP(hipStreamSynchronize(S(s)));
+    lg_profile p{};
+    for (auto& [cls, idx] : s->prof_records) {
+        float ms = 0.0f;
+        LG_HIP(hipEventElapsedTime(&ms, s->prof_events[idx], s->prof_events[idx + 1]));
+        p.ms[cls] += double(ms);
+        p.launches[cls] += 1;
+    }
+    s->prof_mask = 0;
+    s->prof_used = 0;
+    s->prof_records.clear();
+    *out = p;
     return LG_OK;
 }
 

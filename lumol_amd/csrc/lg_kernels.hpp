@@ -8,7 +8,8 @@
 //   force double4[N]  fx, fy, fz, -
 // Neighbour search: a cell list (integer binning + per-cell rank sort) feeds
 // a Verlet list of inter-molecular candidates within rc + skin, stored
-// transposed (nbr[k * N + slot]) so consecutive lanes read consecutive words.
+// atom-major (nbr[slot * cap + k]): the lanes sharing one atom read
+// consecutive words of its row.
 // The list is rebuilt on the device whenever an atom has moved more than
 // skin/2 since the last build, which keeps every pair with r <= rc in the
 // list; each kernel then applies the reference's own cutoff test on the
@@ -41,12 +42,6 @@ __global__ void k_nlist_check(const double4* __restrict__ pos, const double4* __
         moved = dx * dx + dy * dy + dz * dz > half_skin2;
     }
     if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
-}
-
-__global__ void k_cell_zero(int* __restrict__ count, int ncell, const int* __restrict__ flag) {
-    if (*flag == 0) return;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < ncell) count[c] = 0;
 }
 
 __device__ __forceinline__ int cell_coord(double x, double L, int nc, double csize, int periodic) {
@@ -134,66 +129,95 @@ __global__ void k_cell_rank(const int* __restrict__ tmp, const int* __restrict__
     }
 }
 
-// Verlet list: slot t handles atom sorted[t]; candidates are the atoms of the
+// Verlet list: TPA lanes build the row of atom sorted[t] together.  Each
+// pass tests TPA consecutive candidates of a neighbour cell; accepted ones
+// are written in candidate order through a ballot prefix count, so the row
+// is the same whatever the lane timing.  Candidates are the atoms of the
 // neighbouring cells that belong to another molecule and lie within rlist.
-__global__ void k_nlist_build(const double4* __restrict__ pos, const int2* __restrict__ meta,
-                              const int* __restrict__ sorted, const int* __restrict__ cellid,
-                              const int* __restrict__ start, Grid g, Box b, double rlist2, int n, int cap,
-                              int* __restrict__ nbr, int* __restrict__ nnbr, int* __restrict__ overflow,
-                              double4* __restrict__ ref, const int* __restrict__ flag) {
-    if (*flag == 0) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int i = sorted[t];
-    double4 pi = pos[i];
-    int mi = meta[i].y;
-    int c = cellid[i];
-    int cx = c % g.n[0];
-    int cy = (c / g.n[0]) % g.n[1];
-    int cz = c / (g.n[0] * g.n[1]);
-    int rx = g.n[0] >= 3 ? 1 : 0, ry = g.n[1] >= 3 ? 1 : 0, rz = g.n[2] >= 3 ? 1 : 0;
-    int count = 0;
-    for (int dz = -rz; dz <= rz; dz++) {
-        int z = cz + dz;
-        z = z < 0 ? z + g.n[2] : (z >= g.n[2] ? z - g.n[2] : z);
-        for (int dy = -ry; dy <= ry; dy++) {
-            int y = cy + dy;
-            y = y < 0 ? y + g.n[1] : (y >= g.n[1] ? y - g.n[1] : y);
-            for (int dx = -rx; dx <= rx; dx++) {
-                int x = cx + dx;
-                x = x < 0 ? x + g.n[0] : (x >= g.n[0] ? x - g.n[0] : x);
-                int nc = (z * g.n[1] + y) * g.n[0] + x;
-                int s = start[nc], e = start[nc + 1];
-                for (int q = s; q < e; q++) {
-                    int j = sorted[q];
-                    if (meta[j].y == mi) continue;
-                    double4 pj = pos[j];
-                    double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
-                    if (b.periodic) {
-                        ddx -= rint(ddx * b.inv[0]) * b.L[0];
-                        ddy -= rint(ddy * b.inv[1]) * b.L[1];
-                        ddz -= rint(ddz * b.inv[2]) * b.L[2];
-                    }
-                    double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    if (r2 < rlist2) {
-                        if (count < cap) nbr[(size_t)count * n + t] = j;
-                        count++;
+// The kernel also clears the per-cell counts for the next rebuild, and its
+// last block resets the rebuild flag (flags[0]) using flags[2] as counter.
+template <int TPA>
+__global__ __launch_bounds__(256) void k_nlist_build(const double4* __restrict__ pos, const int2* __restrict__ meta,
+                                                     const int* __restrict__ sorted, const int* __restrict__ cellid,
+                                                     const int* __restrict__ start, int* __restrict__ count_zero,
+                                                     Grid g, Box b, double rlist2, int n, int cap,
+                                                     int* __restrict__ nbr, int* __restrict__ nnbr,
+                                                     double4* __restrict__ ref, int* __restrict__ flags) {
+    if (flags[0] == 0) return;
+    __shared__ int last;
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int c = gid; c < g.ncell; c += gridDim.x * blockDim.x) count_zero[c] = 0;
+    int t = gid / TPA;
+    int sub = gid % TPA;
+    int gbase = (threadIdx.x & 63) - sub;
+    if (t < n) {
+        int i = sorted[t];
+        double4 pi = pos[i];
+        int mi = meta[i].y;
+        int c = cellid[i];
+        int cx = c % g.n[0];
+        int cy = (c / g.n[0]) % g.n[1];
+        int cz = c / (g.n[0] * g.n[1]);
+        int rx = g.n[0] >= 3 ? 1 : 0, ry = g.n[1] >= 3 ? 1 : 0, rz = g.n[2] >= 3 ? 1 : 0;
+        int count = 0;
+        int* row = nbr + (size_t)t * cap;
+        for (int dz = -rz; dz <= rz; dz++) {
+            int z = cz + dz;
+            z = z < 0 ? z + g.n[2] : (z >= g.n[2] ? z - g.n[2] : z);
+            for (int dy = -ry; dy <= ry; dy++) {
+                int y = cy + dy;
+                y = y < 0 ? y + g.n[1] : (y >= g.n[1] ? y - g.n[1] : y);
+                for (int dx = -rx; dx <= rx; dx++) {
+                    int x = cx + dx;
+                    x = x < 0 ? x + g.n[0] : (x >= g.n[0] ? x - g.n[0] : x);
+                    int nc = (z * g.n[1] + y) * g.n[0] + x;
+                    int s0 = start[nc], e0 = start[nc + 1];
+                    for (int q0 = s0; q0 < e0; q0 += TPA) {
+                        int q = q0 + sub;
+                        bool accept = false;
+                        int j = 0;
+                        if (q < e0) {
+                            j = sorted[q];
+                            if (meta[j].y != mi) {
+                                double4 pj = pos[j];
+                                double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
+                                if (b.periodic) {
+                                    ddx -= rint(ddx * b.inv[0]) * b.L[0];
+                                    ddy -= rint(ddy * b.inv[1]) * b.L[1];
+                                    ddz -= rint(ddz * b.inv[2]) * b.L[2];
+                                }
+                                accept = ddx * ddx + ddy * ddy + ddz * ddz < rlist2;
+                            }
+                        }
+                        unsigned long long bal = __ballot(accept);
+                        unsigned long long gm =
+                            TPA == 64 ? bal : ((bal >> gbase) & ((1ull << (TPA & 63)) - 1ull));
+                        int slot = count + __popcll(gm & ((1ull << sub) - 1ull));
+                        if (accept && slot < cap) row[slot] = j;
+                        count += __popcll(gm);
                     }
                 }
             }
         }
+        if (sub == 0) {
+            if (count > cap) {
+                atomicMax(flags + 1, count);
+                count = cap;
+            }
+            nnbr[t] = count;
+            ref[i] = pi;
+        }
     }
-    if (count > cap) {
-        atomicMax(overflow, count);
-        count = cap;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(flags + 2, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        flags[2] = 0;
+        flags[0] = 0;
     }
-    nnbr[t] = count;
-    ref[i] = pi;
 }
 
-__global__ void k_flag_clear(int* __restrict__ flag) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
-}
 
 // ---------------------------------------------------------------------------------
 // Non-bonded pair kernel: inter-molecular pairs, Lennard-Jones family + Ewald
@@ -208,6 +232,7 @@ struct PairArgs {
     const int* sorted;
     const int* nbr;
     const int* nnbr;
+    int cap;
     const PairParam* table;
     int nkinds;
     CoulParam coul;
@@ -220,7 +245,9 @@ struct PairArgs {
 constexpr int kMaxKinds = 16;
 constexpr int kPairBlock = 256;
 
-template <bool kEnergy>
+// TPA lanes share one atom i and stride through its neighbour row; the
+// group's partial forces are combined with xor-shuffles (TPA divides 64).
+template <bool kEnergy, int TPA>
 __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
     __shared__ PairParam tab[kMaxKinds * kMaxKinds];
     __shared__ double red[kPairBlock / 64][8];
@@ -228,7 +255,9 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
     for (int x = threadIdx.x; x < nk2; x += blockDim.x) tab[x] = a.table[x];
     __syncthreads();
 
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = g / TPA;
+    int sub = g % TPA;
     double fx = 0, fy = 0, fz = 0;
     double e_pair = 0, e_coul = 0;
     double wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0;
@@ -240,8 +269,9 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
         double qi = pi.w;
         int nn = a.nnbr[t];
         const CoulParam& c = a.coul;
-        for (int k = 0; k < nn; k++) {
-            int j = a.nbr[(size_t)k * a.n + t];
+        const int* row = a.nbr + (size_t)t * a.cap;
+        for (int k = sub; k < nn; k += TPA) {
+            int j = row[k];
             double4 pj = a.pos[j];
             int kj = a.meta[j].x;
             double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -285,21 +315,27 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
                 wyz += fs * dy * dz;
             }
         }
-        a.force[i] = make_double4(fx, fy, fz, 0.0);
     }
+#pragma unroll
+    for (int off = TPA / 2; off > 0; off >>= 1) {
+        fx += __shfl_xor(fx, off, 64);
+        fy += __shfl_xor(fy, off, 64);
+        fz += __shfl_xor(fz, off, 64);
+    }
+    if (t < a.n && sub == 0) a.force[i] = make_double4(fx, fy, fz, 0.0);
     if (kEnergy) {
         double v[8] = {e_pair, e_coul, wxx, wyy, wzz, wxy, wxz, wyz};
         int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            double s = wave_sum(v[q]);
-            if (lane == 0) red[wid][q] = s;
+            double sum = wave_sum(v[q]);
+            if (lane == 0) red[wid][q] = sum;
         }
         __syncthreads();
         if (threadIdx.x < 8) {
-            double s = 0;
-            for (int w = 0; w < kPairBlock / 64; w++) s += red[w][threadIdx.x];
-            a.epart[blockIdx.x * 8 + threadIdx.x] = s;
+            double sum = 0;
+            for (int w = 0; w < kPairBlock / 64; w++) sum += red[w][threadIdx.x];
+            a.epart[blockIdx.x * 8 + threadIdx.x] = sum;
         }
     }
 }
@@ -307,8 +343,9 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
 // Pair-set export: (i < j) with r < rc from the neighbour lists, using the
 // same distance code as k_pairs.
 __global__ void k_pairs_within(const double4* __restrict__ pos, const int* __restrict__ sorted,
-                               const int* __restrict__ nbr, const int* __restrict__ nnbr, int n, Box box, double rc,
-                               uint2* __restrict__ out, unsigned long long cap,
+                               const int* __restrict__ nbr, const int* __restrict__ nnbr, int n, int cap, Box box,
+                               double rc,
+                               uint2* __restrict__ out, unsigned long long out_cap,
                                unsigned long long* __restrict__ count) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -316,7 +353,7 @@ __global__ void k_pairs_within(const double4* __restrict__ pos, const int* __res
     double4 pi = pos[i];
     int nn = nnbr[t];
     for (int k = 0; k < nn; k++) {
-        int j = nbr[(size_t)k * n + t];
+        int j = nbr[(size_t)t * cap + k];
         if (j <= i) continue;
         double4 pj = pos[j];
         double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -324,7 +361,7 @@ __global__ void k_pairs_within(const double4* __restrict__ pos, const int* __res
         double r = sqrt(norm2_ordered(dx, dy, dz));
         if (r < rc) {
             unsigned long long idx = atomicAdd(count, 1ull);
-            if (idx < cap) out[idx] = make_uint2((unsigned)i, (unsigned)j);
+            if (idx < out_cap) out[idx] = make_uint2((unsigned)i, (unsigned)j);
         }
     }
 }
@@ -555,79 +592,84 @@ __global__ __launch_bounds__(256) void k_molecules(MolArgs a) {
 }
 
 // ---------------------------------------------------------------------------------
-// Ewald k-space (ewald.rs:633-733).  Per-atom, per-axis phase factors
-// e^{i n k_axis . r} for n in [-kmax, kmax] are built in LDS by the
-// reference's recursion (polar for n = 1, products for |n| >= 2, conjugates
-// for n < 0), then e^{i k.r} = (Ex * Ey) * Ez as ewald.rs:666-668.
+// Ewald k-space (ewald.rs:633-733).  k_phase computes, once per evaluation,
+// e^{i n k_axis . r_i} for every atom and axis by the reference's recursion
+// (polar for n = 1, products for |n| >= 2, conjugates for n < 0) into a
+// table in HBM; the rho and force kernels stage the rows they use in LDS and
+// form e^{i k.r} = (Ex * Ey) * Ez as ewald.rs:666-668.
 // ---------------------------------------------------------------------------------
-constexpr int kRhoBlock = 256;    // k-vectors per block in the rho pass
+constexpr int kRhoBlock = 256;  // k-vectors per block in the rho pass
 
 // Phase-table rows: axis 0 keeps n = 0..kmax (ikx >= 0 in the half-space
-// set), axes 1 and 2 keep n = -kmax..kmax.  5*kmax + 3 rows of `width` atoms.
+// set), axes 1 and 2 keep n = -kmax..kmax: 5*kmax + 3 rows of `ld` atoms.
 __host__ __device__ __forceinline__ int phase_row(int axis, int n, int kmax) {
     return axis == 0 ? n : (kmax + 1) + (axis - 1) * (2 * kmax + 1) + (n + kmax);
 }
 __host__ __device__ __forceinline__ int phase_rows(int kmax) { return 5 * kmax + 3; }
 
-// Builds the phase table for `cnt` atoms starting at `a0`; `width` atoms per
-// row.  One (atom, axis) recursion per thread, as ewald.rs:641-662.
-__device__ __forceinline__ void build_phase_table(const double4* __restrict__ pos, int a0, int cnt, int n_atoms,
-                                                  const Box& b, int kmax, double2* tab, int width,
-                                                  double* q_out) {
-    for (int job = threadIdx.x; job < width * 3; job += blockDim.x) {
-        int a = job % width;
-        int axis = job / width;
-        int atom = a0 + a;
-        bool live = a < cnt && atom < n_atoms;
-        double x = 0.0, q = 0.0;
-        if (live) {
-            double4 p = pos[atom];
-            x = axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
-            q = p.w;
-        }
-        // k_vector(e_axis) . r = (2 pi * inv_axis) * x  (cells.rs:224)
-        double phase = (2.0 * kPi * b.inv[axis]) * x;
-        double sn, cs;
-        sincos(phase, &sn, &cs);
-        double2 e1 = make_double2(cs, sn);
-        tab[(size_t)phase_row(axis, 0, kmax) * width + a] = make_double2(1.0, 0.0);
-        tab[(size_t)phase_row(axis, 1, kmax) * width + a] = e1;
-        if (axis != 0) tab[(size_t)phase_row(axis, -1, kmax) * width + a] = make_double2(e1.x, -e1.y);
-        double2 e = e1;
-        for (int k = 2; k <= kmax; k++) {
-            double2 nx = make_double2(e.x * e1.x - e.y * e1.y, e.x * e1.y + e.y * e1.x);
-            e = nx;
-            tab[(size_t)phase_row(axis, k, kmax) * width + a] = e;
-            if (axis != 0) tab[(size_t)phase_row(axis, -k, kmax) * width + a] = make_double2(e.x, -e.y);
-        }
-        if (axis == 0 && q_out) q_out[a] = live ? q : 0.0;
-    }
-}
-
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// rho partial sums: grid (ceil(nk / kRhoBlock), nchunks), `width` atoms per chunk.
-__global__ __launch_bounds__(kRhoBlock) void k_rho_partial(const double4* __restrict__ pos, int n, Box b, int kmax,
-                                                           int width, const int4* __restrict__ kidx, int nk,
+// One thread per (atom, axis); tab[row * ld + atom], q[atom] (0 past n).
+__global__ void k_phase(const double4* __restrict__ pos, int n, int ld, Box b, int kmax, double2* __restrict__ tab,
+                        double* __restrict__ q) {
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    int axis = gid / ld;
+    int a = gid - axis * ld;
+    if (axis >= 3) return;
+    bool live = a < n;
+    double x = 0.0, qa = 0.0;
+    if (live) {
+        double4 p = pos[a];
+        x = axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
+        qa = p.w;
+    }
+    // k_vector(e_axis) . r = (2 pi * inv_axis) * x  (cells.rs:224)
+    double phase = (2.0 * kPi * b.inv[axis]) * x;
+    double sn, cs;
+    sincos(phase, &sn, &cs);
+    double2 e1 = make_double2(cs, sn);
+    tab[(size_t)phase_row(axis, 0, kmax) * ld + a] = make_double2(1.0, 0.0);
+    tab[(size_t)phase_row(axis, 1, kmax) * ld + a] = e1;
+    if (axis != 0) tab[(size_t)phase_row(axis, -1, kmax) * ld + a] = make_double2(e1.x, -e1.y);
+    double2 e = e1;
+    for (int k = 2; k <= kmax; k++) {
+        e = make_double2(e.x * e1.x - e.y * e1.y, e.x * e1.y + e.y * e1.x);
+        tab[(size_t)phase_row(axis, k, kmax) * ld + a] = e;
+        if (axis != 0) tab[(size_t)phase_row(axis, -k, kmax) * ld + a] = make_double2(e.x, -e.y);
+    }
+    if (axis == 0) q[a] = live ? qa : 0.0;
+}
+
+// rho partial sums: grid (ceil(nk / kRhoBlock), nchunks), `width` atoms per
+// chunk; the chunk's table rows are staged in LDS with a padded row stride
+// (width + 1) so lanes reading different rows hit different banks.
+__global__ __launch_bounds__(kRhoBlock) void k_rho_partial(const double2* __restrict__ tabg,
+                                                           const double* __restrict__ qg, int ld, int kmax, int width,
+                                                           const int4* __restrict__ kidx, int nk,
                                                            double2* __restrict__ partial) {
     extern __shared__ double2 smem[];
+    int R = phase_rows(kmax);
+    int stride = width + 1;
     double2* tab = smem;
-    double* q = reinterpret_cast<double*>(tab + (size_t)phase_rows(kmax) * width);
+    double* q = reinterpret_cast<double*>(tab + (size_t)R * stride);
     int chunk = blockIdx.y;
     int a0 = chunk * width;
-    int cnt = min(width, n - a0);
-    build_phase_table(pos, a0, cnt, n, b, kmax, tab, width, q);
+    for (int e = threadIdx.x; e < R * width; e += blockDim.x) {
+        int row = e / width, a = e - row * width;
+        tab[row * stride + a] = tabg[(size_t)row * ld + a0 + a];
+    }
+    for (int a = threadIdx.x; a < width; a += blockDim.x) q[a] = qg[a0 + a];
     __syncthreads();
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nk) return;
     int4 id = kidx[k];
-    const double2* ex = tab + (size_t)phase_row(0, id.x, kmax) * width;
-    const double2* ey = tab + (size_t)phase_row(1, id.y, kmax) * width;
-    const double2* ez = tab + (size_t)phase_row(2, id.z, kmax) * width;
+    const double2* ex = tab + phase_row(0, id.x, kmax) * stride;
+    const double2* ey = tab + phase_row(1, id.y, kmax) * stride;
+    const double2* ez = tab + phase_row(2, id.z, kmax) * stride;
     double re = 0.0, im = 0.0;
-    for (int a = 0; a < cnt; a++) {
+    for (int a = 0; a < width; a++) {
         double2 phi = cmul(cmul(ex[a], ey[a]), ez[a]);
         re += q[a] * phi.x;
         im += q[a] * phi.y;
@@ -635,40 +677,42 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_partial(const double4* __rest
     partial[(size_t)chunk * nk + k] = make_double2(re, im);
 }
 
-// rho(k) = sum over chunks in order; optional energy/virial partials per block:
-// e = factor |rho|^2, w = |rho|^2 * virial_k (6 components).
+// rho(k) = sum over chunks, in a fixed order: wave w of the block sums the
+// chunks c = w mod 4, then wave 0 adds the four in order.  With kEnergy the
+// block writes its energy and virial partials: e = factor |rho|^2 and
+// w = |rho|^2 * virial_k (6 components).  grid: ceil(nk / 64), block 256.
 template <bool kEnergy>
 __global__ __launch_bounds__(256) void k_rho_reduce(const double2* __restrict__ partial, int nchunks, int nk,
                                                     double2* __restrict__ rho, const double* __restrict__ efac,
                                                     const double* __restrict__ vfac, double* __restrict__ epart) {
-    __shared__ double red[4][7];
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (k < nk) {
-        double re = 0.0, im = 0.0;
-        for (int c = 0; c < nchunks; c++) {
+    __shared__ double2 part[4][64];
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int k = blockIdx.x * 64 + lane;
+    double re = 0.0, im = 0.0;
+    if (k < nk)
+        for (int c = w; c < nchunks; c += 4) {
             double2 p = partial[(size_t)c * nk + k];
             re += p.x;
             im += p.y;
         }
+    part[w][lane] = make_double2(re, im);
+    __syncthreads();
+    if (w != 0) return;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (k < nk) {
+        re = part[0][lane].x + part[1][lane].x + part[2][lane].x + part[3][lane].x;
+        im = part[0][lane].y + part[1][lane].y + part[2][lane].y + part[3][lane].y;
         rho[k] = make_double2(re, im);
         if (kEnergy) {
             double n2 = re * re + im * im;
             v[0] = efac[k] * n2;
-            for (int q = 0; q < 6; q++) v[1 + q] = n2 * vfac[(size_t)q * nk + k];
+            for (int x = 0; x < 6; x++) v[1 + x] = n2 * vfac[(size_t)x * nk + k];
         }
     }
     if (kEnergy) {
-        int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-        for (int q = 0; q < 7; q++) {
-            double s = wave_sum(v[q]);
-            if (lane == 0) red[wid][q] = s;
-        }
-        __syncthreads();
-        if (threadIdx.x < 7) {
-            double s = 0;
-            for (int w = 0; w < 4; w++) s += red[w][threadIdx.x];
-            epart[blockIdx.x * 7 + threadIdx.x] = s;
+        for (int x = 0; x < 7; x++) {
+            double sum = wave_sum(v[x]);
+            if (lane == 0) epart[blockIdx.x * 7 + x] = sum;
         }
     }
 }
@@ -680,49 +724,61 @@ struct KRow {
     int ikx, iky, kbase, count;
 };
 
-// blockDim.x atoms per block (64, or 32 when the table would not fit in LDS).
-__global__ __launch_bounds__(64) void k_kspace_force(const double4* __restrict__ pos, int n, Box b, int kmax,
-                                                     const KRow* __restrict__ rows, int nrows,
-                                                     const int4* __restrict__ kidx,
-                                                     const double2* __restrict__ rho,
-                                                     const double4* __restrict__ field,
-                                                     double4* __restrict__ force) {
-    extern __shared__ double2 smem[];
-    double2* tab = smem;
-    int width = blockDim.x;
-    int a0 = blockIdx.x * width;
-    int cnt = min(width, n - a0);
-    build_phase_table(pos, a0, cnt, n, b, kmax, tab, width, nullptr);
-    __syncthreads();
-    int a = threadIdx.x;
-    int i = a0 + a;
+// One lane per atom (256 per block); grid.y splits the rows, split g writes
+// its partial to fk[g * n + i] (combined in a fixed order by the consumer).
+// A lane reads only its own atom's column of the phase table, so the reads
+// of a wave are contiguous and come from L2; rows have consecutive ikz.
+__global__ __launch_bounds__(256) void k_kspace_force(const double2* __restrict__ tab, int ld, int n, int kmax,
+                                                      const KRow* __restrict__ rows, int nrows,
+                                                      const int4* __restrict__ kidx,
+                                                      const double2* __restrict__ rho,
+                                                      const double4* __restrict__ field,
+                                                      double4* __restrict__ fk) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int nsplit = gridDim.y;
+    int r0 = (int)(((long)nrows * blockIdx.y) / nsplit);
+    int r1 = (int)(((long)nrows * (blockIdx.y + 1)) / nsplit);
+    int a = i < ld ? i : ld - 1;
+    const double2* col = tab + a;
     double fx = 0, fy = 0, fz = 0;
-    for (int r = 0; r < nrows; r++) {
+    for (int r = r0; r < r1; r++) {
         KRow row = rows[r];
-        double2 exy = cmul(tab[(size_t)phase_row(0, row.ikx, kmax) * width + a],
-                           tab[(size_t)phase_row(1, row.iky, kmax) * width + a]);
-        const double2* ez = tab + (size_t)phase_row(2, 0, kmax) * width + a;
+        double2 exy = cmul(col[(size_t)phase_row(0, row.ikx, kmax) * ld], col[(size_t)phase_row(1, row.iky, kmax) * ld]);
+        int ikz0 = kidx[row.kbase].z;
+        const double2* ez = col + (size_t)phase_row(2, ikz0, kmax) * ld;
         for (int c = 0; c < row.count; c++) {
             int k = row.kbase + c;
-            int ikz = kidx[k].z;
-            double2 e = cmul(exy, ez[(size_t)ikz * width]);
+            double2 e = cmul(exy, ez[(size_t)c * ld]);
             double2 rk = rho[k];
             // Im(e * conj(rho)) = e.im * rho.re - e.re * rho.im
             double s = e.y * rk.x - e.x * rk.y;
-            double4 fk = field[k];
-            fx += s * fk.x;
-            fy += s * fk.y;
-            fz += s * fk.z;
+            double4 f4 = field[k];
+            fx += s * f4.x;
+            fy += s * f4.y;
+            fz += s * f4.z;
         }
     }
-    if (a < cnt) {
-        double q = pos[i].w;
-        double4 f = force[i];
-        f.x += (q * fx) / kFourPiEps0;
-        f.y += (q * fy) / kFourPiEps0;
-        f.z += (q * fz) / kFourPiEps0;
-        force[i] = f;
+    if (i < n) fk[(size_t)blockIdx.y * n + i] = make_double4(fx, fy, fz, 0.0);
+}
+
+// force[i] += q_i / (4 pi eps0) * sum_g fk[g][i]   (ewald.rs:716-719)
+__global__ void k_kspace_combine(const double4* __restrict__ pos, const double4* __restrict__ fk, int nsplit, int n,
+                                 double4* __restrict__ force) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double fx = 0, fy = 0, fz = 0;
+    for (int g = 0; g < nsplit; g++) {
+        double4 p = fk[(size_t)g * n + i];
+        fx += p.x;
+        fy += p.y;
+        fz += p.z;
     }
+    double q = pos[i].w;
+    double4 f = force[i];
+    f.x += (q * fx) / kFourPiEps0;
+    f.y += (q * fy) / kFourPiEps0;
+    f.z += (q * fz) / kFourPiEps0;
+    force[i] = f;
 }
 
 // ---------------------------------------------------------------------------------
@@ -730,124 +786,139 @@ __global__ __launch_bounds__(64) void k_kspace_force(const double4* __restrict__
 // reference so that positions follow the reference's rounding.
 // ---------------------------------------------------------------------------------
 #pragma clang fp contract(off)
-// VelocityVerlet first half (integrators.rs:47-53): optional pending
-// Berendsen scale of the previous step, then v += (dt/2) a; x += v dt.
-__global__ void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel, const double4* __restrict__ acc,
-                           int n, double dt, const double* __restrict__ lambda) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double4 v = vel[i];
-    if (lambda) {
-        double l = *lambda;
-        v.x = v.x * l;
-        v.y = v.y * l;
-        v.z = v.z * l;
-    }
-    double4 a = acc[i];
-    double h = 0.5 * dt;
-    v.x += h * a.x;
-    v.y += h * a.y;
-    v.z += h * a.z;
-    double4 p = pos[i];
-    p.x += v.x * dt;
-    p.y += v.y * dt;
-    p.z += v.z * dt;
-    vel[i] = v;
-    pos[i] = p;
+// Neighbour-list displacement test of the new position.
+__device__ __forceinline__ bool moved_beyond(double4 p, const double4* __restrict__ ref, int i, double half_skin2) {
+    double4 r = ref[i];
+    double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+    return dx * dx + dy * dy + dz * dz > half_skin2;
 }
 
-// VelocityVerlet second half (integrators.rs:55-68) with the kinetic energy
-// partial sums of the block (for the thermostat).
-__global__ __launch_bounds__(256) void k_vv_second(const double4* __restrict__ force, double4* __restrict__ vel,
-                                                   double4* __restrict__ acc, int n, double dt,
-                                                   double* __restrict__ ke_part) {
-    __shared__ double red[4];
+// VelocityVerlet first half (integrators.rs:47-53): optional pending
+// Berendsen scale of the previous step, then v += (dt/2) a; x += v dt; and
+// the displacement test of the neighbour list (sets flags[0]).
+__global__ void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel, const double4* __restrict__ acc,
+                           int n, double dt, const double* __restrict__ lambda, const double4* __restrict__ ref,
+                           double half_skin2, int* __restrict__ flags) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double ke = 0.0;
+    bool moved = false;
     if (i < n) {
-        double4 f = force[i];
         double4 v = vel[i];
-        double m = v.w;
-        double4 a = make_double4(f.x / m, f.y / m, f.z / m, 0.0);
+        if (lambda) {
+            double l = *lambda;
+            v.x = v.x * l;
+            v.y = v.y * l;
+            v.z = v.z * l;
+        }
+        double4 a = acc[i];
         double h = 0.5 * dt;
         v.x += h * a.x;
         v.y += h * a.y;
         v.z += h * a.z;
-        acc[i] = a;
+        double4 p = pos[i];
+        p.x += v.x * dt;
+        p.y += v.y * dt;
+        p.z += v.z * dt;
         vel[i] = v;
-        ke = 0.5 * m * (v.x * v.x + v.y * v.y + v.z * v.z);
+        pos[i] = p;
+        moved = moved_beyond(p, ref, i, half_skin2);
     }
-    if (ke_part) {
-        double s = wave_sum(ke);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) ke_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-    }
+    if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flags, 1);
 }
 
 // LeapFrog (integrators.rs:150-168), position part, after the pending
-// Berendsen scale of the previous step.
+// Berendsen scale of the previous step, with the displacement test.
 __global__ void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel, const double4* __restrict__ acc,
-                           int n, double dt, const double* __restrict__ lambda) {
+                           int n, double dt, const double* __restrict__ lambda, const double4* __restrict__ ref,
+                           double half_skin2, int* __restrict__ flags) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double4 v = vel[i];
-    if (lambda) {
-        double l = *lambda;
-        v.x = v.x * l;
-        v.y = v.y * l;
-        v.z = v.z * l;
-        vel[i] = v;
+    bool moved = false;
+    if (i < n) {
+        double4 v = vel[i];
+        if (lambda) {
+            double l = *lambda;
+            v.x = v.x * l;
+            v.y = v.y * l;
+            v.z = v.z * l;
+            vel[i] = v;
+        }
+        double4 a = acc[i];
+        double dt2 = dt * dt;
+        double4 p = pos[i];
+        p.x += v.x * dt + (0.5 * a.x) * dt2;
+        p.y += v.y * dt + (0.5 * a.y) * dt2;
+        p.z += v.z * dt + (0.5 * a.z) * dt2;
+        pos[i] = p;
+        moved = moved_beyond(p, ref, i, half_skin2);
     }
-    double4 a = acc[i];
-    double dt2 = dt * dt;
-    double4 p = pos[i];
-    p.x += v.x * dt + (0.5 * a.x) * dt2;
-    p.y += v.y * dt + (0.5 * a.y) * dt2;
-    p.z += v.z * dt + (0.5 * a.z) * dt2;
-    pos[i] = p;
+    if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flags, 1);
 }
 
-__global__ __launch_bounds__(256) void k_lf_second(const double4* __restrict__ force, double4* __restrict__ vel,
-                                                   double4* __restrict__ acc, int n, double dt,
-                                                   double* __restrict__ ke_part) {
+// Second half of VelocityVerlet (integrators.rs:55-68) or LeapFrog
+// (:160-167).  The total force is the pair/intra-molecular force plus the
+// k-space partials q_i / (4 pi eps0) sum_g fk[g][i] (ewald.rs:716-719).
+// With a thermostat (ke_part != null), block partials of the kinetic energy
+// are written for k_berendsen.
+template <bool kLeapFrog>
+__global__ __launch_bounds__(256) void k_md_second(const double4* __restrict__ force, const double4* __restrict__ pos,
+                                                   const double4* __restrict__ fk, int ksplit,
+                                                   double4* __restrict__ vel, double4* __restrict__ acc, int n,
+                                                   double dt, double* __restrict__ ke_part) {
     __shared__ double red[4];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ke = 0.0;
     if (i < n) {
         double4 f = force[i];
+        if (ksplit > 0) {
+            double sx = 0, sy = 0, sz = 0;
+            for (int g = 0; g < ksplit; g++) {
+                double4 p = fk[(size_t)g * n + i];
+                sx += p.x;
+                sy += p.y;
+                sz += p.z;
+            }
+            double q = pos[i].w;
+            f.x += (q * sx) / kFourPiEps0;
+            f.y += (q * sy) / kFourPiEps0;
+            f.z += (q * sz) / kFourPiEps0;
+        }
         double4 v = vel[i];
         double m = v.w;
-        double4 a = acc[i];
         double4 na = make_double4(f.x / m, f.y / m, f.z / m, 0.0);
-        v.x += (0.5 * (a.x + na.x)) * dt;
-        v.y += (0.5 * (a.y + na.y)) * dt;
-        v.z += (0.5 * (a.z + na.z)) * dt;
+        if (kLeapFrog) {
+            double4 a = acc[i];
+            v.x += (0.5 * (a.x + na.x)) * dt;
+            v.y += (0.5 * (a.y + na.y)) * dt;
+            v.z += (0.5 * (a.z + na.z)) * dt;
+        } else {
+            double h = 0.5 * dt;
+            v.x += h * na.x;
+            v.y += h * na.y;
+            v.z += h * na.z;
+        }
         acc[i] = na;
         vel[i] = v;
         ke = 0.5 * m * (v.x * v.x + v.y * v.y + v.z * v.z);
     }
-    if (ke_part) {
-        double s = wave_sum(ke);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) ke_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-    }
+    if (!ke_part) return;
+    double s = wave_sum(ke);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) ke_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// Berendsen factor (thermostats.rs:113-121) from the kinetic energy partials;
-// dof = 3N (system.rs:249-255).  One block.
+// Berendsen factor (thermostats.rs:113-121) from the kinetic energy
+// partials, dof = 3N (system.rs:249-255); one block, fixed summation order.
 __global__ __launch_bounds__(256) void k_berendsen(const double* __restrict__ ke_part, int nparts, double dof,
                                                    double T0, double tau, double* __restrict__ lambda) {
     __shared__ double red[4];
-    double s = 0.0;
-    for (int p = threadIdx.x; p < nparts; p += blockDim.x) s += ke_part[p];
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    double t = 0.0;
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) t += ke_part[p];
+    t = wave_sum(t);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double ke = red[0] + red[1] + red[2] + red[3];
-        double T = 2.0 * ke / (dof * kBoltzmann);
+        double kin = red[0] + red[1] + red[2] + red[3];
+        double T = 2.0 * kin / (dof * kBoltzmann);
         *lambda = sqrt(1.0 + (T0 / T - 1.0) / tau);
     }
 }

@@ -515,6 +515,19 @@ class System:
         order = np.lexsort((out[:, 1], out[:, 0]))
         return out[order]
 
+    def profile_begin(self, classes):
+        """Time the given kernel classes (names of _native.KERNEL_CLASSES)
+        with HIP events until profile_end()."""
+        mask = 0
+        for c in classes:
+            mask |= 1 << nat.KERNEL_CLASSES.index(c)
+        nat.check(nat.lib().lg_profile_begin(self._sync(), mask))
+
+    def profile_end(self):
+        p = nat.Profile()
+        nat.check(nat.lib().lg_profile_end(self._sync(), ctypes.byref(p)))
+        return {c: (p.ms[i], p.launches[i]) for i, c in enumerate(nat.KERNEL_CLASSES)}
+
     def ewald_kvector_count(self):
         h = self._sync()
         out = ctypes.c_int()
