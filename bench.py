@@ -98,6 +98,36 @@ def force_error(s):
     return {"force_rms_rel": rms, "force_max_rel": mx, "energy_rel": abs(eg - eo) / abs(eo)}
 
 
+def calibrate_copy(mib=256):
+    """One device-to-device hipMemcpy of `mib` MiB (the plain copy kernel the
+    FETCH_SIZE / WRITE_SIZE counters are scaled by)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (profiles/traffic_nacl_ewald_4096.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_nacl_ewald_4096.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,7 +135,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 256 MiB device copy (scale factor of the HBM byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -159,7 +194,7 @@ def main():
         per = f"{BYTES_INTEGRATE} B x {natoms} atoms per launch"
     achieved = work / avg_s
     roofline = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
-                "traffic": None, "kernel": dominant, "avg_launch_us": avg_s * 1e6, "work": per}
+                "traffic": load_traffic(dominant), "kernel": dominant, "avg_launch_us": avg_s * 1e6, "work": per}
 
     kernels = {c: {"ms_per_step": warm[c][0] / max(args.warmup, 1), "launches": warm[c][1]}
                for c in warm if warm[c][1]}
