@@ -1,18 +1,29 @@
 #!/usr/bin/env python3
 """Throughput of Lumol's force-and-energy hot path on one MI355X.
 
-Workload (default): NaCl Ewald NVT -- 4096 ions (2048 Na+, 2048 Cl-) on 8^3
-rocksalt cells, a = 5.64 A (box 45.12 A), seeded +-0.05 A jitter,
-Lorentz-Berthelot LJ + Ewald (rc 12 A, kmax 9, alpha = pi/rc), Berendsen
-1200 K (tau = 100 steps), Velocity-Verlet dt = 1 fs.  One step = one MD step
-on the device: integrator halves, neighbour-list maintenance, pair kernel
-(LJ + Ewald real space), k-space rho(k) and forces, thermostat.
+Workloads (BASELINE configurations; `--workload`):
 
-Prints ONE JSON line: atom-steps/s over `--steps` timed steps after
-`--warmup` untimed steps, the dominant kernel's roofline fraction (HIP
-events on the library's stream, algorithmic FLOP per launch from the
-oracle's pair-set convention), the form's force error against the FP64
-oracle, and the oracle's threaded CPU throughput on this host.
+* ``nacl`` (default, the metric's configuration): NaCl Ewald NVT -- 4096
+  ions (2048 Na+, 2048 Cl-) on 8^3 rocksalt cells, a = 5.64 A (box 45.12 A),
+  seeded +-0.05 A jitter, Lorentz-Berthelot LJ + Ewald (rc 12 A, kmax 9,
+  alpha = pi/rc), Berendsen 1200 K (tau = 100 steps), Velocity-Verlet
+  dt = 1 fs.
+* ``spce``: flexible SPC/E water NVE -- 3000 molecules (9000 atoms), box
+  44.8 A, Ewald rc 10 A kmax 10, inter-molecular LJ + Coulomb, harmonic
+  bonds and angles, dt = 0.5 fs.
+
+One step = one MD step on the device: integrator halves, neighbour-list
+maintenance, pair kernel (LJ + Ewald real space), intra-molecular terms,
+k-space rho(k) and forces, thermostat.  Both arithmetic forms are measured:
+``fp64`` (all FP64) and ``mixed`` (FP32 pair/erfc/trig with FP64 positions,
+phase reduction and accumulation).
+
+Prints ONE JSON line.  ``value`` is the mixed form's atom-steps/s over
+`--steps` timed steps after `--warmup` untimed steps; the FP64 form's
+figures and each form's force error against the FP64 oracle sit beside it,
+with the dominant kernel's roofline fraction (HIP events on the library's
+stream, algorithmic work per launch on the oracle's pair set) and the
+oracle's threaded CPU throughput on this host.
 """
 import argparse
 import json
@@ -26,8 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "atom-steps/s, FP64 vs mixed at stated force error; p50/p99 job latency"
-FP64_VECTOR_PEAK_TFLOPS = 78.6
-FP32_VECTOR_PEAK_TFLOPS = 157.3
+PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X vector peaks (FP64, FP32)
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic FLOP per work unit (add/sub/mul = 1, FMA = 2, div/sqrt/exp/erfc = 1),
@@ -36,11 +46,18 @@ HBM_PEAK_GBS = 8000.0
 #   energy or virial): geometry 22 (d 3, min-image 12, r^2 5, sqrt 1, 1/r 1),
 #   LJ 10, Ewald real 16, force vector and +/- accumulation 10  -> 58
 #   (with energy and virial: 91, the figure of the survey's measurement table)
-#   k-space per (k-vector, atom): rho 16, force 21.
+#   Coulomb-only pair: 58 - 10 = 48.  k-space per (k-vector, atom): rho 16, force 21.
 FLOP_PAIR_LJ_EWALD_F = 58
+FLOP_PAIR_EWALD_F = 48
 FLOP_KSPACE_RHO = 16
 FLOP_KSPACE_FORCE = 21
-BYTES_INTEGRATE = 128  # per atom-step, FP64 (x, v, F read + 1/m; x, v written)
+
+WORKLOADS = {
+    "nacl": "NaCl Ewald NVT, 4096 ions, 8x8x8 rocksalt a=5.64 A, LJ(LB)+Ewald rc 12 A kmax 9, "
+            "Berendsen 1200 K, VV dt 1 fs",
+    "spce": "flexible SPC/E water NVE, 3000 molecules (9000 atoms), box 44.8 A, LJ O-O + Ewald rc 10 A "
+            "kmax 10 (inter-molecular), harmonic O-H/H-O-H, VV dt 0.5 fs",
+}
 
 
 def host_cores():
@@ -51,51 +68,70 @@ def host_cores():
     return max(1, min(16, n))
 
 
-def cpu_baseline(target_seconds=12.0):
+def build_system(workload):
+    from lumol_amd.workloads import nacl, spce
+    return nacl() if workload == "nacl" else spce()
+
+
+def make_sim(workload):
+    from lumol_amd import BerendsenThermostat, MolecularDynamics, Simulation, VelocityVerlet
+    if workload == "nacl":
+        md = MolecularDynamics(VelocityVerlet(1.0))
+        md.set_thermostat(BerendsenThermostat(1200.0, 100.0))
+    else:
+        md = MolecularDynamics(VelocityVerlet(0.5))
+    return Simulation(md)
+
+
+def oracle_md(o, workload, nsteps):
+    if workload == "nacl":
+        o.md_run(0, 1.0, nsteps, 1200.0, 100.0)
+    else:
+        o.md_run(0, 0.5, nsteps)
+
+
+def cpu_baseline(workload, target_seconds=12.0):
     """The oracle's threaded FP64 MD step (the reference's algorithm: O(N^2)
     pair loops with per-thread force buffers, Ewald recomputing e^{ik.r} per
     call) on a bounded sample of the same workload: 1 warm-up step, then 5
     repetitions of R steps (R sized to ~target/5 s each); median."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import lumol_oracle as lo
-    from lumol_amd.workloads import nacl
 
-    s = nacl()
+    s = build_system(workload)
     cores = host_cores()
     o = lo.OracleSystem(s, nthreads=cores)
     o.md_reset()
     t0 = time.perf_counter()
-    o.md_run(0, 1.0, 1, 1200.0, 100.0)
+    oracle_md(o, workload, 1)
     t1 = time.perf_counter() - t0
     reps = 5
     r = max(1, int(round(target_seconds / reps / max(t1, 1e-3))))
     rates = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        o.md_run(0, 1.0, r, 1200.0, 100.0)
+        oracle_md(o, workload, r)
         dt = time.perf_counter() - t0
         rates.append(s.size() * r / dt)
     return {"value": statistics.median(rates), "unit": "atom-steps/s", "cores": cores, "kind": "port",
-            "sample": f"oracle FP64 VV+Berendsen on the same 4096-ion NaCl Ewald system: 1 warm-up step, "
+            "sample": f"oracle FP64 MD step on the same {s.size()}-atom system: 1 warm-up step, "
                       f"median of {reps} x {r} steps, {cores} threads"}
 
 
-def force_error(s):
-    """Force RMS / max relative error of the device form against the FP64
-    oracle on the current configuration (max over atoms with |F| > 1e-3 F_rms)."""
+def force_error(s, oracle_forces, oracle_energy):
+    """Force RMS / max relative error against the FP64 oracle (max over atoms
+    with |F| > 1e-3 F_rms), and energy relative error."""
     import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import lumol_oracle as lo
-    o = lo.OracleSystem(s, nthreads=host_cores())
-    fg, fo = s.forces(), o.forces()
+    fg = s.forces()
+    fo = oracle_forces
     d = fg - fo
     rms = math.sqrt((d * d).sum() / (fo * fo).sum())
     norms = np.linalg.norm(fo, axis=1)
     frms = math.sqrt((norms ** 2).mean())
     mask = norms > 1e-3 * frms
     mx = float((np.linalg.norm(d, axis=1)[mask] / norms[mask]).max())
-    eg, eo = s.potential_energy(), o.potential_energy()
-    return {"force_rms_rel": rms, "force_max_rel": mx, "energy_rel": abs(eg - eo) / abs(eo)}
+    eg = s.potential_energy()
+    return {"force_rms_rel": rms, "force_max_rel": mx, "energy_rel": abs(eg - oracle_energy) / abs(oracle_energy)}
 
 
 def calibrate_copy(mib=256):
@@ -116,10 +152,10 @@ def calibrate_copy(mib=256):
     print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
 
 
-def load_traffic(kernel):
+def load_traffic(workload, form, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (profiles/traffic_nacl_ewald_4096.json), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic_nacl_ewald_4096.json")
+    summary (profiles/traffic_<workload>_<form>.json), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{form}.json")
     try:
         with open(path) as f:
             data = json.load(f)
@@ -128,11 +164,65 @@ def load_traffic(kernel):
         return None
 
 
+def run_form(workload, form, steps, warmup, oracle_forces, oracle_energy):
+    from lumol_amd._native import LG_PRECISION_FP64, LG_PRECISION_MIXED
+    s = build_system(workload)
+    s.set_precision(LG_PRECISION_MIXED if form == "mixed" else LG_PRECISION_FP64)
+    natoms = s.size()
+    rc = 12.0 if workload == "nacl" else 10.0
+    n_pairs = len(s.pairs_within(rc))
+    nk = s.ewald_kvector_count()
+    err = force_error(s, oracle_forces, oracle_energy)
+    sim = make_sim(workload)
+
+    # warm-up, with a profiled pass over every kernel class to find the dominant one
+    s.profile_begin(["nlist", "pairs", "molecules", "kspace_rho", "kspace_force", "integrate"])
+    sim.run(s, warmup)
+    warm = s.profile_end()
+    dominant = max(("pairs", "kspace_rho", "kspace_force", "molecules"), key=lambda c: warm[c][0])
+
+    # timed region: only the dominant class carries events
+    s.profile_begin([dominant])
+    t0 = time.perf_counter()
+    sim.run(s, steps)  # returns after the stream is synchronised
+    elapsed = time.perf_counter() - t0
+    prof = s.profile_end()
+
+    ms, launches = prof[dominant]
+    avg_s = ms / 1e3 / max(launches, 1)
+    peak = PEAK_TFLOPS[form]
+    if dominant == "pairs":
+        # in the water model every O-O pair within rc also carries Coulomb;
+        # the pair set counted is all inter-molecular pairs within rc
+        flop = FLOP_PAIR_LJ_EWALD_F if workload == "nacl" else FLOP_PAIR_EWALD_F
+        work = n_pairs * flop / 1e12
+        per = f"{flop} FLOP x {n_pairs} pairs (r < {rc} A) per launch"
+    elif dominant == "kspace_force":
+        work = nk * natoms * FLOP_KSPACE_FORCE / 1e12
+        per = f"{FLOP_KSPACE_FORCE} FLOP x {nk} k-vectors x {natoms} atoms per launch"
+    elif dominant == "kspace_rho":
+        work = nk * natoms * FLOP_KSPACE_RHO / 1e12
+        per = f"{FLOP_KSPACE_RHO} FLOP x {nk} k-vectors x {natoms} atoms per launch"
+    else:
+        work = 0.0
+        per = "intra-molecular terms"
+    achieved = work / avg_s
+    roofline = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": load_traffic(workload, form, dominant), "kernel": dominant,
+                "avg_launch_us": avg_s * 1e6, "work": per}
+    kernels = {c: round(warm[c][0] * 1e3 / max(warmup, 1), 2) for c in warm if warm[c][1]}
+    s.close()
+    return {"value": natoms * steps / elapsed, "ms_per_step": elapsed * 1e3 / steps,
+            "error_vs_fp64_oracle": err, "roofline": roofline, "kernel_us_per_step": kernels,
+            "atoms": natoms, "pairs_within_rc": n_pairs, "kvectors": nk}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="nacl")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -149,79 +239,40 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from lumol_amd import BerendsenThermostat, MolecularDynamics, Simulation, VelocityVerlet
-    from lumol_amd.workloads import nacl
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lumol_oracle as lo
+    ref = build_system(args.workload)
+    o = lo.OracleSystem(ref, nthreads=host_cores())
+    oracle_forces, oracle_energy = o.forces(), o.potential_energy()
+    del o
 
-    s = nacl()
-    natoms = s.size()
-    rc = 12.0
-    n_pairs = len(s.pairs_within(rc))
-    nk = s.ewald_kvector_count()
-    err = force_error(s)
-
-    md = MolecularDynamics(VelocityVerlet(1.0))
-    md.set_thermostat(BerendsenThermostat(1200.0, 100.0))
-    sim = Simulation(md)
-
-    # warm-up, with a profiled pass over every kernel class to find the dominant one
-    s.profile_begin(["nlist", "pairs", "molecules", "kspace_rho", "kspace_force", "integrate"])
-    sim.run(s, args.warmup)
-    warm = s.profile_end()
-    # the dominant single kernel (nlist and integrate are sequences of launches)
-    dominant = max(("pairs", "kspace_rho", "kspace_force", "molecules"), key=lambda c: warm[c][0])
-
-    # timed region: only the dominant class carries events
-    s.profile_begin([dominant])
-    t0 = time.perf_counter()
-    sim.run(s, args.steps)  # returns after the stream is synchronised
-    elapsed = time.perf_counter() - t0
-    prof = s.profile_end()
-    value = natoms * args.steps / elapsed
-
-    ms, launches = prof[dominant]
-    avg_s = ms / 1e3 / max(launches, 1)
-    if dominant == "pairs":
-        work, unit, peak, bound = n_pairs * FLOP_PAIR_LJ_EWALD_F / 1e12, "TFLOP/s", FP64_VECTOR_PEAK_TFLOPS, "valu"
-        per = f"{FLOP_PAIR_LJ_EWALD_F} FLOP x {n_pairs} pairs (r < {rc}) per launch"
-    elif dominant == "kspace_force":
-        work, unit, peak, bound = nk * natoms * FLOP_KSPACE_FORCE / 1e12, "TFLOP/s", FP64_VECTOR_PEAK_TFLOPS, "valu"
-        per = f"{FLOP_KSPACE_FORCE} FLOP x {nk} k-vectors x {natoms} atoms per launch"
-    elif dominant == "kspace_rho":
-        work, unit, peak, bound = nk * natoms * FLOP_KSPACE_RHO / 1e12, "TFLOP/s", FP64_VECTOR_PEAK_TFLOPS, "valu"
-        per = f"{FLOP_KSPACE_RHO} FLOP x {nk} k-vectors x {natoms} atoms per launch"
-    else:
-        work, unit, peak, bound = natoms * BYTES_INTEGRATE / 1e9, "GB/s", HBM_PEAK_GBS, "hbm"
-        per = f"{BYTES_INTEGRATE} B x {natoms} atoms per launch"
-    achieved = work / avg_s
-    roofline = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
-                "traffic": load_traffic(dominant), "kernel": dominant, "avg_launch_us": avg_s * 1e6, "work": per}
-
-    kernels = {c: {"ms_per_step": warm[c][0] / max(args.warmup, 1), "launches": warm[c][1]}
-               for c in warm if warm[c][1]}
-
+    forms = {f: run_form(args.workload, f, args.steps, args.warmup, oracle_forces, oracle_energy)
+             for f in ("fp64", "mixed")}
+    m = forms["mixed"]
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": m["value"],
         "unit": "atom-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded rocksalt lattice + jitter, Maxwell-Boltzmann velocities)",
-        "config": {"workload": "NaCl Ewald NVT, 4096 ions, 8x8x8 rocksalt a=5.64 A, LJ(LB)+Ewald rc 12 A "
-                               "kmax 9, Berendsen 1200 K, VV dt 1 fs",
-                   "form": "fp64", "atoms": natoms, "pairs_within_rc": n_pairs, "kvectors": nk},
-        "error_vs_fp64_oracle": err,
-        "roofline": roofline,
-        "kernels_warmup": kernels,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded lattice/grid + jitter, Maxwell-Boltzmann velocities)",
+        "config": {"workload": WORKLOADS[args.workload], "form": "mixed", "atoms": m["atoms"],
+                   "pairs_within_rc": m["pairs_within_rc"], "kvectors": m["kvectors"]},
+        "error_vs_fp64_oracle": m["error_vs_fp64_oracle"],
+        "roofline": m["roofline"],
+        "forms": {"fp64": {k: forms["fp64"][k] for k in ("value", "ms_per_step", "error_vs_fp64_oracle",
+                                                         "roofline", "kernel_us_per_step")},
+                  "mixed": {k: m[k] for k in ("value", "ms_per_step", "kernel_us_per_step")},
+                  "mixed_over_fp64": m["value"] / forms["fp64"]["value"]},
     }
     if not args.no_cpu_baseline:
-        base = cpu_baseline(args.cpu_seconds)
-        line["cpu_baseline"] = base
+        line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

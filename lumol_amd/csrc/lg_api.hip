@@ -173,6 +173,9 @@ struct lg_system {
     DevBuf<double2> rho, rho_partial;
     DevBuf<double2> phase_tab;  // e^{i n k_axis x}, phase_rows(kmax) x ld
     DevBuf<double> phase_q;     // charges padded to ld
+    DevBuf<float2> phase_tab32;  // mixed form: FP32 phase factors
+    DevBuf<float> phase_q32;
+    DevBuf<float4> field32;      // mixed form: FP32 copy of the field factors
     int ld = 0;
     // energy partials
     DevBuf<double> epart_pairs, epart_mol, epart_k, ke_part, lambda;
@@ -534,6 +537,10 @@ int prepare_kspace(lg_system* s) {
     LG_TRY(upload(s->efac, efac, st));
     LG_TRY(upload(s->vfac, vflat, st));
     LG_TRY(upload(s->field, field, st));
+    std::vector<float4> f32(field.size());
+    for (size_t k = 0; k < field.size(); k++)
+        f32[k] = make_float4(float(field[k].x), float(field[k].y), float(field[k].z), 0.f);
+    LG_TRY(upload(s->field32, f32, st));
     LG_TRY(s->rho.ensure(s->nk));
     // LDS budget for the phase tables
     size_t rows_bytes = size_t(phase_rows(kmax)) * sizeof(double2);
@@ -554,6 +561,8 @@ int prepare_kspace(lg_system* s) {
     s->ld = blocks_for(s->n, 64) * 64;
     LG_TRY(s->phase_tab.ensure(size_t(phase_rows(kmax)) * s->ld));
     LG_TRY(s->phase_q.ensure(s->ld));
+    LG_TRY(s->phase_tab32.ensure(size_t(phase_rows(kmax)) * s->ld));
+    LG_TRY(s->phase_q32.ensure(s->ld));
     LG_TRY(s->epart_k.ensure(size_t(blocks_for(s->nk, 64)) * 7));
     return LG_OK;
 }
@@ -705,11 +714,16 @@ int launch_forces(lg_system* s, bool energy, bool combine_kspace = true) {
     pa.force = s->force.p;
     pa.epart = s->epart_pairs.p;
     long prof = prof_start(s, LG_KERNEL_PAIRS);
+    bool mixed = s->precision == LG_PRECISION_MIXED;
     int pblocks = blocks_for(size_t(n) * s->tpa, kPairBlock);
     switch (s->tpa) {
 #define LG_PAIRS_CASE(T)                                                   \
     case T:                                                                \
-        if (energy)                                                        \
+        if (mixed && energy)                                               \
+            k_pairs_mixed<true, T><<<pblocks, kPairBlock, 0, st>>>(pa);    \
+        else if (mixed)                                                    \
+            k_pairs_mixed<false, T><<<pblocks, kPairBlock, 0, st>>>(pa);   \
+        else if (energy)                                                   \
             k_pairs<true, T><<<pblocks, kPairBlock, 0, st>>>(pa);          \
         else                                                               \
             k_pairs<false, T><<<pblocks, kPairBlock, 0, st>>>(pa);         \
@@ -766,13 +780,41 @@ int launch_forces(lg_system* s, bool energy, bool combine_kspace = true) {
         size_t rows_bytes = size_t(phase_rows(kmax)) * sizeof(double2);
         int w = s->rho_width;
         int nchunks = blocks_for(n, w);
-        dim3 grid(blocks_for(s->nk, kRhoBlock), nchunks);
+        // k-vectors per thread: stage each atom chunk in as few blocks as
+        // keep ~512 blocks in flight
+        int kpt = 1;
+        while (kpt < 4 && size_t(blocks_for(s->nk, kRhoBlock * kpt * 2)) * nchunks >= 512) kpt *= 2;
+        dim3 grid(blocks_for(s->nk, kRhoBlock * kpt), nchunks);
         size_t shm = rows_bytes * (w + 1) + w * sizeof(double);
         long pr = prof_start(s, LG_KERNEL_KSPACE_RHO);
-        k_phase<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
-                                                                    s->phase_tab.p, s->phase_q.p);
-        k_rho_partial<<<grid, kRhoBlock, shm, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w, s->kidx.p, s->nk,
-                                                    s->rho_partial.p);
+        if (mixed) {
+            k_phase32<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                                                                          s->phase_tab32.p, s->phase_q32.p);
+            size_t shm32 = size_t(phase_rows(kmax)) * sizeof(float2) * (w + 1) + w * sizeof(float);
+            switch (kpt) {
+                case 1: k_rho_partial32<1><<<grid, kRhoBlock, shm32, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld,
+                                                                       kmax, w, s->kidx.p, s->nk, s->rho_partial.p);
+                    break;
+                case 2: k_rho_partial32<2><<<grid, kRhoBlock, shm32, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld,
+                                                                       kmax, w, s->kidx.p, s->nk, s->rho_partial.p);
+                    break;
+                default: k_rho_partial32<4><<<grid, kRhoBlock, shm32, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld,
+                                                                        kmax, w, s->kidx.p, s->nk, s->rho_partial.p);
+            }
+        } else {
+            k_phase<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                                                                        s->phase_tab.p, s->phase_q.p);
+            switch (kpt) {
+                case 1: k_rho_partial<1><<<grid, kRhoBlock, shm, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
+                                                                   s->kidx.p, s->nk, s->rho_partial.p);
+                    break;
+                case 2: k_rho_partial<2><<<grid, kRhoBlock, shm, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
+                                                                   s->kidx.p, s->nk, s->rho_partial.p);
+                    break;
+                default: k_rho_partial<4><<<grid, kRhoBlock, shm, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
+                                                                    s->kidx.p, s->nk, s->rho_partial.p);
+            }
+        }
         if (energy)
             k_rho_reduce<true><<<blocks_for(s->nk, 64), 256, 0, st>>>(s->rho_partial.p, nchunks, s->nk, s->rho.p,
                                                                      s->efac.p, s->vfac.p, s->epart_k.p);
@@ -782,8 +824,12 @@ int launch_forces(lg_system* s, bool energy, bool combine_kspace = true) {
         prof_stop(s, pr);
         long pf = prof_start(s, LG_KERNEL_KSPACE_FORCE);
         dim3 fgrid(blocks_for(n, 256), s->ksplit);
-        k_kspace_force<<<fgrid, 256, 0, st>>>(s->phase_tab.p, s->ld, n, kmax, s->rows.p, s->nrows, s->kidx.p,
-                                               s->rho.p, s->field.p, s->fk.p);
+        if (mixed)
+            k_kspace_force32<<<fgrid, 256, 0, st>>>(s->phase_tab32.p, s->ld, n, kmax, s->rows.p, s->nrows,
+                                                     s->kidx.p, s->rho.p, s->field32.p, s->fk.p);
+        else
+            k_kspace_force<<<fgrid, 256, 0, st>>>(s->phase_tab.p, s->ld, n, kmax, s->rows.p, s->nrows, s->kidx.p,
+                                                   s->rho.p, s->field.p, s->fk.p);
         if (combine_kspace)
             k_kspace_combine<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->fk.p, s->ksplit, n, s->force.p);
         prof_stop(s, pf);
@@ -1025,6 +1071,9 @@ int lg_system_destroy(lg_system* s) {
     for (auto* b : ibufs) b->release();
     s->phase_tab.release();
     s->phase_q.release();
+    s->phase_tab32.release();
+    s->phase_q32.release();
+    s->field32.release();
     DevBuf<double4>* d4[] = {&s->pos, &s->vel, &s->acc, &s->force, &s->ref, &s->field, &s->fk,
                              &s->snap_pos, &s->snap_vel, &s->snap_acc};
     for (auto* b : d4) b->release();
@@ -1173,7 +1222,6 @@ int lg_system_set_precision(lg_system* s, int precision) {
     if (!s) return fail(LG_E_INVALID, "null system");
     if (precision != LG_PRECISION_FP64 && precision != LG_PRECISION_MIXED)
         return fail(LG_E_INVALID, "unknown precision");
-    if (precision == LG_PRECISION_MIXED) return fail(LG_E_UNSUPPORTED, "mixed precision form not built yet");
     s->precision = precision;
     return LG_OK;
 }

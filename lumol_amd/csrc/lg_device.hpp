@@ -154,3 +154,40 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 }  // namespace lg
+
+// =====================================================================================
+// Mixed-precision helpers (FP32 arithmetic on FP64-derived inputs)
+// =====================================================================================
+namespace lg {
+
+// erfc(x) for x >= 0 from the Chebyshev fit of Numerical Recipes in C, 2nd
+// ed., section 6.2 ("erfcc"): fractional error below 1.2e-7 everywhere.
+// `ex2` = exp(-x^2) is returned for the Ewald force term.
+__device__ __forceinline__ float erfc_cheb(float x, float& ex2) {
+    float t = 1.0f / (1.0f + 0.5f * x);
+    float p = -1.26551223f +
+              t * (1.00002368f +
+                   t * (0.37409196f +
+                        t * (0.09678418f +
+                             t * (-0.18628806f +
+                                  t * (0.27886807f +
+                                       t * (-1.13520398f +
+                                            t * (1.48851587f + t * (-0.82215223f + t * 0.17087277f))))))));
+    float mx2 = -x * x;
+    ex2 = __expf(mx2);
+    return t * __expf(mx2 + p);
+}
+
+// Compensated FP32 accumulation (s, c), sum = s + c: the rounding error of
+// each addition is recovered exactly by Knuth's branch-free TwoSum
+// (TAOCP vol. 2, section 4.2.2) and collected in c.
+#pragma clang fp contract(off)
+__device__ __forceinline__ void comp_add(float& s, float& c, float v) {
+    float t = s + v;
+    float bp = t - s;
+    c += (s - (t - bp)) + (v - bp);
+    s = t;
+}
+#pragma clang fp contract(on)
+
+}  // namespace lg

@@ -340,6 +340,132 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
     }
 }
 
+// Mixed form of k_pairs: the minimum-image difference is taken in FP64 and
+// rounded to FP32; distance, LJ, erfc and the Ewald/Wolf force factor are
+// FP32 (erfc from erfc_cheb); the per-atom force is accumulated in FP32 with
+// compensated summation and stored in FP64.  Energies and virial (energy
+// mode) are accumulated in FP64.
+template <bool kEnergy, int TPA>
+__global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
+    struct P32 {
+        int pot, inter_excl;
+        float p0, p1, rc2, shift;
+    };
+    __shared__ P32 tab[kMaxKinds * kMaxKinds];
+    __shared__ double red[kPairBlock / 64][8];
+    int nk2 = a.nkinds * a.nkinds;
+    for (int x = threadIdx.x; x < nk2; x += blockDim.x) {
+        PairParam pp = a.table[x];
+        float rc = (float)pp.rc;
+        tab[x] = P32{pp.pot, pp.inter_excl, (float)pp.p0, (float)pp.p1, rc * rc, (float)pp.shift};
+    }
+    __syncthreads();
+    const float alpha = (float)a.coul.alpha;
+    const float rcc2 = (float)(a.coul.rc * a.coul.rc);
+    const float g0 = (float)(a.coul.alpha * kFrac2SqrtPi);
+    const float inv4pe = (float)(1.0 / kFourPiEps0);
+    const float we = (float)a.coul.e_const, wf = (float)a.coul.f_const;
+
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = g / TPA;
+    int sub = g % TPA;
+    float fx = 0.f, fy = 0.f, fz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    double e_pair = 0, e_coul = 0;
+    double wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0;
+    int i = -1;
+    if (t < a.n) {
+        i = a.sorted[t];
+        double4 pi = a.pos[i];
+        int ki = a.meta[i].x;
+        float qi = (float)pi.w;
+        int nn = a.nnbr[t];
+        const int* row = a.nbr + (size_t)t * a.cap;
+        for (int k = sub; k < nn; k += TPA) {
+            int j = row[k];
+            double4 pj = a.pos[j];
+            int kj = a.meta[j].x;
+            double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
+            min_image(a.box, ddx, ddy, ddz);
+            float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float inv_r = rsqrtf(r2);
+            float inv_r2 = inv_r * inv_r;
+            float fs = 0.f, e_p = 0.f, e_c = 0.f;
+            const P32& pp = tab[ki * a.nkinds + kj];
+            if (pp.pot > POT_NULL && !pp.inter_excl && r2 < pp.rc2) {
+                if (pp.pot == POT_LJ) {
+                    float s2 = pp.p0 * pp.p0 * inv_r2;
+                    float s6 = s2 * s2 * s2;
+                    fs += 24.f * pp.p1 * (2.f * s6 * s6 - s6) * inv_r2;
+                    if (kEnergy) e_p = 4.f * pp.p1 * (s6 * s6 - s6) - pp.shift;
+                } else {
+                    float r = r2 * inv_r;
+                    fs += pp.p0 * (pp.p1 - r) * inv_r;
+                    if (kEnergy) e_p = 0.5f * pp.p0 * (r - pp.p1) * (r - pp.p1) - pp.shift;
+                }
+            }
+            float qiqj = qi * (float)pj.w;
+            if (a.coul.kind != COUL_NONE && qiqj != 0.f && r2 <= rcc2) {
+                float r = r2 * inv_r;
+                float ex2;
+                float erfcv = erfc_cheb(alpha * r, ex2);
+                float pref = qiqj * inv4pe;
+                if (a.coul.kind == COUL_EWALD) {
+                    if (a.coul.inter_excl) {
+                        float erfv = 1.f - erfcv;
+                        fs += pref * inv_r2 * (g0 * ex2 - erfv * inv_r);
+                        if (kEnergy) e_c = -pref * erfv * inv_r;
+                    } else {
+                        fs += pref * inv_r2 * (g0 * ex2 + erfcv * inv_r);
+                        if (kEnergy) e_c = pref * erfcv * inv_r;
+                    }
+                } else if (!a.coul.inter_excl) {
+                    float factor = erfcv * inv_r2 + g0 * ex2 * inv_r;
+                    fs += pref * (factor - wf) * inv_r;
+                    if (kEnergy) e_c = pref * (erfcv * inv_r - we);
+                }
+            }
+            comp_add(fx, cx, fs * dx);
+            comp_add(fy, cy, fs * dy);
+            comp_add(fz, cz, fs * dz);
+            if (kEnergy && j > i) {
+                e_pair += e_p;
+                e_coul += e_c;
+                double f = fs;
+                wxx += f * ddx * ddx;
+                wyy += f * ddy * ddy;
+                wzz += f * ddz * ddz;
+                wxy += f * ddx * ddy;
+                wxz += f * ddx * ddz;
+                wyz += f * ddy * ddz;
+            }
+        }
+    }
+    double sx = (double)fx + (double)cx, sy = (double)fy + (double)cy, sz = (double)fz + (double)cz;
+#pragma unroll
+    for (int off = TPA / 2; off > 0; off >>= 1) {
+        sx += __shfl_xor(sx, off, 64);
+        sy += __shfl_xor(sy, off, 64);
+        sz += __shfl_xor(sz, off, 64);
+    }
+    if (t < a.n && sub == 0) a.force[i] = make_double4(sx, sy, sz, 0.0);
+    if (kEnergy) {
+        double v[8] = {e_pair, e_coul, wxx, wyy, wzz, wxy, wxz, wyz};
+        int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            double sum = wave_sum(v[q]);
+            if (lane == 0) red[wid][q] = sum;
+        }
+        __syncthreads();
+        if (threadIdx.x < 8) {
+            double sum = 0;
+            for (int w = 0; w < kPairBlock / 64; w++) sum += red[w][threadIdx.x];
+            a.epart[blockIdx.x * 8 + threadIdx.x] = sum;
+        }
+    }
+}
+
 // Pair-set export: (i < j) with r < rc from the neighbour lists, using the
 // same distance code as k_pairs.
 __global__ void k_pairs_within(const double4* __restrict__ pos, const int* __restrict__ sorted,
@@ -642,9 +768,11 @@ __global__ void k_phase(const double4* __restrict__ pos, int n, int ld, Box b, i
     if (axis == 0) q[a] = live ? qa : 0.0;
 }
 
-// rho partial sums: grid (ceil(nk / kRhoBlock), nchunks), `width` atoms per
-// chunk; the chunk's table rows are staged in LDS with a padded row stride
+// rho partial sums: grid (ceil(nk / (kRhoBlock * kpt)), nchunks), `width`
+// atoms per chunk, kpt k-vectors per thread (k = base + j * kRhoBlock); the
+// chunk's table rows are staged once in LDS with a padded row stride
 // (width + 1) so lanes reading different rows hit different banks.
+template <int KPT>
 __global__ __launch_bounds__(kRhoBlock) void k_rho_partial(const double2* __restrict__ tabg,
                                                            const double* __restrict__ qg, int ld, int kmax, int width,
                                                            const int4* __restrict__ kidx, int nk,
@@ -662,19 +790,35 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_partial(const double2* __rest
     }
     for (int a = threadIdx.x; a < width; a += blockDim.x) q[a] = qg[a0 + a];
     __syncthreads();
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nk) return;
-    int4 id = kidx[k];
-    const double2* ex = tab + phase_row(0, id.x, kmax) * stride;
-    const double2* ey = tab + phase_row(1, id.y, kmax) * stride;
-    const double2* ez = tab + phase_row(2, id.z, kmax) * stride;
-    double re = 0.0, im = 0.0;
-    for (int a = 0; a < width; a++) {
-        double2 phi = cmul(cmul(ex[a], ey[a]), ez[a]);
-        re += q[a] * phi.x;
-        im += q[a] * phi.y;
+    int kb = blockIdx.x * kRhoBlock * KPT + threadIdx.x;
+    const double2* ex[KPT];
+    const double2* ey[KPT];
+    const double2* ez[KPT];
+    double re[KPT], im[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+        int k = min(kb + j * kRhoBlock, nk - 1);
+        int4 id = kidx[k];
+        ex[j] = tab + phase_row(0, id.x, kmax) * stride;
+        ey[j] = tab + phase_row(1, id.y, kmax) * stride;
+        ez[j] = tab + phase_row(2, id.z, kmax) * stride;
+        re[j] = 0.0;
+        im[j] = 0.0;
     }
-    partial[(size_t)chunk * nk + k] = make_double2(re, im);
+    for (int a = 0; a < width; a++) {
+        double qa = q[a];
+#pragma unroll
+        for (int j = 0; j < KPT; j++) {
+            double2 phi = cmul(cmul(ex[j][a], ey[j][a]), ez[j][a]);
+            re[j] += qa * phi.x;
+            im[j] += qa * phi.y;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+        int k = kb + j * kRhoBlock;
+        if (k < nk) partial[(size_t)chunk * nk + k] = make_double2(re[j], im[j]);
+    }
 }
 
 // rho(k) = sum over chunks, in a fixed order: wave w of the block sums the
@@ -740,15 +884,17 @@ __global__ __launch_bounds__(256) void k_kspace_force(const double2* __restrict_
     int r1 = (int)(((long)nrows * (blockIdx.y + 1)) / nsplit);
     int a = i < ld ? i : ld - 1;
     const double2* col = tab + a;
+    // e^{i k.r} along a row: (Ex * Ey) * Ez[ikz0], then one product by
+    // e^{i kz1 z} per step of ikz
+    const double2 ez1 = col[(size_t)phase_row(2, 1, kmax) * ld];
     double fx = 0, fy = 0, fz = 0;
     for (int r = r0; r < r1; r++) {
         KRow row = rows[r];
         double2 exy = cmul(col[(size_t)phase_row(0, row.ikx, kmax) * ld], col[(size_t)phase_row(1, row.iky, kmax) * ld]);
         int ikz0 = kidx[row.kbase].z;
-        const double2* ez = col + (size_t)phase_row(2, ikz0, kmax) * ld;
+        double2 e = cmul(exy, col[(size_t)phase_row(2, ikz0, kmax) * ld]);
         for (int c = 0; c < row.count; c++) {
             int k = row.kbase + c;
-            double2 e = cmul(exy, ez[(size_t)c * ld]);
             double2 rk = rho[k];
             // Im(e * conj(rho)) = e.im * rho.re - e.re * rho.im
             double s = e.y * rk.x - e.x * rk.y;
@@ -756,10 +902,137 @@ __global__ __launch_bounds__(256) void k_kspace_force(const double2* __restrict_
             fx += s * f4.x;
             fy += s * f4.y;
             fz += s * f4.z;
+            e = cmul(e, ez1);
         }
     }
     if (i < n) fk[(size_t)blockIdx.y * n + i] = make_double4(fx, fy, fz, 0.0);
 }
+
+// ---- mixed-form k-space ----------------------------------------------------------
+// Phase factors in FP32 from phases range-reduced in FP64: for every n the
+// phase n k_axis x is formed and reduced to [-pi, pi] in FP64, then
+// sincosf.  Same row layout as k_phase.
+__global__ void k_phase32(const double4* __restrict__ pos, int n, int ld, Box b, int kmax, float2* __restrict__ tab,
+                          float* __restrict__ q) {
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    int axis = gid / ld;
+    int a = gid - axis * ld;
+    if (axis >= 3) return;
+    bool live = a < n;
+    double x = 0.0, qa = 0.0;
+    if (live) {
+        double4 p = pos[a];
+        x = axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
+        qa = p.w;
+    }
+    double phase = (2.0 * kPi * b.inv[axis]) * x;
+    const double two_pi = 2.0 * kPi, inv_two_pi = 1.0 / (2.0 * kPi);
+    tab[(size_t)phase_row(axis, 0, kmax) * ld + a] = make_float2(1.f, 0.f);
+    for (int k = 1; k <= kmax; k++) {
+        double th = k * phase;
+        th -= two_pi * rint(th * inv_two_pi);
+        float sn, cs;
+        sincosf((float)th, &sn, &cs);
+        tab[(size_t)phase_row(axis, k, kmax) * ld + a] = make_float2(cs, sn);
+        if (axis != 0) tab[(size_t)phase_row(axis, -k, kmax) * ld + a] = make_float2(cs, -sn);
+    }
+    if (axis == 0) q[a] = live ? (float)qa : 0.f;
+}
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// rho partials, mixed form: FP32 phase products, FP64 accumulation; same
+// layout as k_rho_partial.
+template <int KPT>
+__global__ __launch_bounds__(kRhoBlock) void k_rho_partial32(const float2* __restrict__ tabg,
+                                                             const float* __restrict__ qg, int ld, int kmax,
+                                                             int width, const int4* __restrict__ kidx, int nk,
+                                                             double2* __restrict__ partial) {
+    extern __shared__ float2 smem32[];
+    int R = phase_rows(kmax);
+    int stride = width + 1;
+    float2* tab = smem32;
+    float* q = reinterpret_cast<float*>(tab + (size_t)R * stride);
+    int chunk = blockIdx.y;
+    int a0 = chunk * width;
+    for (int e = threadIdx.x; e < R * width; e += blockDim.x) {
+        int row = e / width, a = e - row * width;
+        tab[row * stride + a] = tabg[(size_t)row * ld + a0 + a];
+    }
+    for (int a = threadIdx.x; a < width; a += blockDim.x) q[a] = qg[a0 + a];
+    __syncthreads();
+    int kb = blockIdx.x * kRhoBlock * KPT + threadIdx.x;
+    const float2* ex[KPT];
+    const float2* ey[KPT];
+    const float2* ez[KPT];
+    double re[KPT], im[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+        int k = min(kb + j * kRhoBlock, nk - 1);
+        int4 id = kidx[k];
+        ex[j] = tab + phase_row(0, id.x, kmax) * stride;
+        ey[j] = tab + phase_row(1, id.y, kmax) * stride;
+        ez[j] = tab + phase_row(2, id.z, kmax) * stride;
+        re[j] = 0.0;
+        im[j] = 0.0;
+    }
+    for (int a = 0; a < width; a++) {
+        float qa = q[a];
+#pragma unroll
+        for (int j = 0; j < KPT; j++) {
+            float2 phi = cmulf(cmulf(ex[j][a], ey[j][a]), ez[j][a]);
+            re[j] += (double)(qa * phi.x);
+            im[j] += (double)(qa * phi.y);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+        int k = kb + j * kRhoBlock;
+        if (k < nk) partial[(size_t)chunk * nk + k] = make_double2(re[j], im[j]);
+    }
+}
+
+// k-space forces, mixed form: FP32 products, each row summed in FP32 and
+// added to FP64 accumulators.  rho32/field32 are FP32 copies of rho/field.
+__global__ __launch_bounds__(256) void k_kspace_force32(const float2* __restrict__ tab, int ld, int n, int kmax,
+                                                        const KRow* __restrict__ rows, int nrows,
+                                                        const int4* __restrict__ kidx,
+                                                        const double2* __restrict__ rho,
+                                                        const float4* __restrict__ field32,
+                                                        double4* __restrict__ fk) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int nsplit = gridDim.y;
+    int r0 = (int)(((long)nrows * blockIdx.y) / nsplit);
+    int r1 = (int)(((long)nrows * (blockIdx.y + 1)) / nsplit);
+    int a = i < ld ? i : ld - 1;
+    const float2* col = tab + a;
+    const float2 ez1 = col[(size_t)phase_row(2, 1, kmax) * ld];
+    double fx = 0, fy = 0, fz = 0;
+    for (int r = r0; r < r1; r++) {
+        KRow row = rows[r];
+        float2 exy = cmulf(col[(size_t)phase_row(0, row.ikx, kmax) * ld], col[(size_t)phase_row(1, row.iky, kmax) * ld]);
+        int ikz0 = kidx[row.kbase].z;
+        float2 e = cmulf(exy, col[(size_t)phase_row(2, ikz0, kmax) * ld]);
+        float rx = 0.f, ry = 0.f, rz = 0.f;
+        for (int c = 0; c < row.count; c++) {
+            int k = row.kbase + c;
+            double2 rk = rho[k];
+            float s = e.y * (float)rk.x - e.x * (float)rk.y;
+            float4 f4 = field32[k];
+            rx += s * f4.x;
+            ry += s * f4.y;
+            rz += s * f4.z;
+            e = cmulf(e, ez1);
+        }
+        fx += rx;
+        fy += ry;
+        fz += rz;
+    }
+    if (i < n) fk[(size_t)blockIdx.y * n + i] = make_double4(fx, fy, fz, 0.0);
+}
+
 
 // force[i] += q_i / (4 pi eps0) * sum_g fk[g][i]   (ewald.rs:716-719)
 __global__ void k_kspace_combine(const double4* __restrict__ pos, const double4* __restrict__ fk, int nsplit, int n,
