@@ -158,9 +158,10 @@ struct lg_system {
     std::vector<PairParam> h_table;
     CoulParam coul{};
     // molecules
-    int nmol = 0, nmol_multi = 0;
-    DevBuf<int> mol_list, d_mol_start, ip_start, b_start, a_start, d_start, d_param;
-    DevBuf<int4> ipairs, d_bonds, d_angles, d_dihedrals;
+    int nmol = 0, nterms = 0;
+    DevBuf<Term> terms;          // intra-molecular terms, sorted by type
+    DevBuf<int> slot_start;      // per-atom CSR over contribution slots (n + 1)
+    DevBuf<double4> contrib;     // one force contribution per (term, atom)
     DevBuf<BondedParam> bparam, aparam, dparam;
     // k-space
     int nk = 0, nrows = 0, rho_width = 64, force_width = 64, ksplit = 1;
@@ -169,13 +170,13 @@ struct lg_system {
     DevBuf<int4> kidx;
     DevBuf<KRow> rows;
     DevBuf<double> efac, vfac;
-    DevBuf<double4> field;
+    DevBuf<KRec> krec;           // rho(k) and force factor per k
     DevBuf<double2> rho, rho_partial;
     DevBuf<double2> phase_tab;  // e^{i n k_axis x}, phase_rows(kmax) x ld
     DevBuf<double> phase_q;     // charges padded to ld
     DevBuf<float2> phase_tab32;  // mixed form: FP32 phase factors
     DevBuf<float> phase_q32;
-    DevBuf<float4> field32;      // mixed form: FP32 copy of the field factors
+    DevBuf<KRec32> krec32;       // mixed form: FP32 copy of krec
     int ld = 0;
     // energy partials
     DevBuf<double> epart_pairs, epart_mol, epart_k, ke_part, lambda;
@@ -321,15 +322,14 @@ int upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
     return LG_OK;
 }
 
+// Intra-molecular terms in type order (same-molecule pairs, bonds, angles,
+// dihedrals), each with one contribution slot per atom.  An atom's slots are
+// numbered in term order, which fixes the order in which its contributions
+// are added.
 int prepare_topology(lg_system* s) {
     std::vector<std::tuple<int, int, int>> ip;
     LG_TRY(build_topology(s, ip));
     hipStream_t st = S(s);
-    std::vector<int> mol_list;
-    std::vector<int> ip_start(s->nmol + 1, 0), b_start(s->nmol + 1, 0), a_start(s->nmol + 1, 0),
-        d_start(s->nmol + 1, 0);
-    std::vector<int4> h_ip, h_b, h_a, h_d;
-    std::vector<int> h_dparam;
     std::vector<BondedParam> h_bp, h_ap, h_dp;
     auto kind_of = [&](int a) { return s->kind[a]; };
     auto find_bond = [&](int a, int b) -> BondedParam {
@@ -359,56 +359,52 @@ int prepare_topology(lg_system* s) {
         if (it == s->dihedral_pots.end()) return {POT_NONE, 0, 0};
         return {it->second.pot, it->second.p0, it->second.p1};
     };
-    // group per molecule
-    std::vector<std::vector<int4>> per_ip(s->nmol), per_b(s->nmol), per_a(s->nmol), per_d(s->nmol);
-    std::vector<std::vector<int>> per_dp(s->nmol);
-    for (auto& [i, j, path] : ip) per_ip[s->molid[i]].push_back(make_int4(i, j, path, 0));
+    std::vector<Term> h_terms;
+    h_terms.reserve(ip.size() + s->bonds.size() + s->angles.size() + s->dihedrals.size());
+    auto term = [](int type, int param, int a0, int a1, int a2, int a3) {
+        Term t;
+        t.atoms = make_int4(a0, a1, a2, a3);
+        t.slots = make_int4(-1, -1, -1, -1);
+        t.type = type;
+        t.param = param;
+        return t;
+    };
+    for (auto& [i, j, path] : ip) h_terms.push_back(term(TERM_PAIR, path, i, j, -1, -1));
     for (auto& b : s->bonds) {
         int i = int(b.first), j = int(b.second);
         h_bp.push_back(find_bond(i, j));
-        per_b[s->molid[i]].push_back(make_int4(i, j, int(h_bp.size() - 1), 0));
+        h_terms.push_back(term(TERM_BOND, int(h_bp.size() - 1), i, j, -1, -1));
     }
     for (auto& [i, j, k] : s->angles) {
         h_ap.push_back(find_angle(i, j, k));
-        per_a[s->molid[i]].push_back(make_int4(i, j, k, int(h_ap.size() - 1)));
+        h_terms.push_back(term(TERM_ANGLE, int(h_ap.size() - 1), i, j, k, -1));
     }
     for (auto& [i, j, k, m] : s->dihedrals) {
         h_dp.push_back(find_dihedral(i, j, k, m));
-        per_d[s->molid[i]].push_back(make_int4(i, j, k, m));
-        per_dp[s->molid[i]].push_back(int(h_dp.size() - 1));
+        h_terms.push_back(term(TERM_DIHEDRAL, int(h_dp.size() - 1), i, j, k, m));
     }
-    for (int m = 0; m < s->nmol; m++) {
-        ip_start[m] = int(h_ip.size());
-        h_ip.insert(h_ip.end(), per_ip[m].begin(), per_ip[m].end());
-        b_start[m] = int(h_b.size());
-        h_b.insert(h_b.end(), per_b[m].begin(), per_b[m].end());
-        a_start[m] = int(h_a.size());
-        h_a.insert(h_a.end(), per_a[m].begin(), per_a[m].end());
-        d_start[m] = int(h_d.size());
-        h_d.insert(h_d.end(), per_d[m].begin(), per_d[m].end());
-        h_dparam.insert(h_dparam.end(), per_dp[m].begin(), per_dp[m].end());
-        if (s->mol_start[m + 1] - s->mol_start[m] >= 2) mol_list.push_back(m);
+    // slots: count per atom, prefix sum, then hand out in term order
+    std::vector<int> slot_start(s->n + 1, 0);
+    auto natoms_of = [](const Term& t) { return t.type == TERM_DIHEDRAL ? 4 : t.type == TERM_ANGLE ? 3 : 2; };
+    for (auto& t : h_terms) {
+        const int* at = &t.atoms.x;
+        for (int q = 0; q < natoms_of(t); q++) slot_start[at[q] + 1]++;
     }
-    ip_start[s->nmol] = int(h_ip.size());
-    b_start[s->nmol] = int(h_b.size());
-    a_start[s->nmol] = int(h_a.size());
-    d_start[s->nmol] = int(h_d.size());
-    s->nmol_multi = int(mol_list.size());
-    LG_TRY(upload(s->mol_list, mol_list, st));
-    LG_TRY(upload(s->d_mol_start, s->mol_start, st));
-    LG_TRY(upload(s->ip_start, ip_start, st));
-    LG_TRY(upload(s->b_start, b_start, st));
-    LG_TRY(upload(s->a_start, a_start, st));
-    LG_TRY(upload(s->d_start, d_start, st));
-    LG_TRY(upload(s->ipairs, h_ip, st));
-    LG_TRY(upload(s->d_bonds, h_b, st));
-    LG_TRY(upload(s->d_angles, h_a, st));
-    LG_TRY(upload(s->d_dihedrals, h_d, st));
-    LG_TRY(upload(s->d_param, h_dparam, st));
+    for (size_t i = 0; i < s->n; i++) slot_start[i + 1] += slot_start[i];
+    std::vector<int> next(slot_start.begin(), slot_start.end() - 1);
+    for (auto& t : h_terms) {
+        const int* at = &t.atoms.x;
+        int* sl = &t.slots.x;
+        for (int q = 0; q < natoms_of(t); q++) sl[q] = next[at[q]]++;
+    }
+    s->nterms = int(h_terms.size());
+    LG_TRY(upload(s->terms, h_terms, st));
+    LG_TRY(upload(s->slot_start, slot_start, st));
+    LG_TRY(s->contrib.ensure(std::max<size_t>(1, size_t(slot_start[s->n]))));
     LG_TRY(upload(s->bparam, h_bp, st));
     LG_TRY(upload(s->aparam, h_ap, st));
     LG_TRY(upload(s->dparam, h_dp, st));
-    LG_TRY(s->epart_mol.ensure(size_t(std::max(1, blocks_for(s->nmol_multi, 256))) * 11));
+    LG_TRY(s->epart_mol.ensure(size_t(std::max(1, blocks_for(s->nterms, 256))) * 11));
     s->meta_dirty = true;
     s->topo_dirty = false;
     return LG_OK;
@@ -494,7 +490,7 @@ int prepare_kspace(lg_system* s) {
     double four_pi_v = 4.0 * kPi / volume;
     std::vector<int4> idx;
     std::vector<double> efac;
-    std::vector<double4> field;
+    std::vector<KRec> rec;
     std::vector<double> vf[6];
     auto push = [&](int x, int y, int z) {
         double k[3];
@@ -504,7 +500,7 @@ int prepare_kspace(lg_system* s) {
         double energy = four_pi_v * std::exp(-k2 * a4) / k2;
         idx.push_back(make_int4(x, y, z, 0));
         efac.push_back(energy);
-        field.push_back(make_double4((2.0 * energy) * k[0], (2.0 * energy) * k[1], (2.0 * energy) * k[2], 0.0));
+        rec.push_back(KRec{0.0, 0.0, (2.0 * energy) * k[0], (2.0 * energy) * k[1], (2.0 * energy) * k[2], 0, 0, 0});
         double f = -2.0 * (1.0 / k2 + a4);
         // virial = energy * (I + f k (x) k); components xx yy zz xy xz yz
         vf[0].push_back(energy * (1.0 + f * (k[0] * k[0])));
@@ -536,11 +532,11 @@ int prepare_kspace(lg_system* s) {
     LG_TRY(upload(s->rows, rows, st));
     LG_TRY(upload(s->efac, efac, st));
     LG_TRY(upload(s->vfac, vflat, st));
-    LG_TRY(upload(s->field, field, st));
-    std::vector<float4> f32(field.size());
-    for (size_t k = 0; k < field.size(); k++)
-        f32[k] = make_float4(float(field[k].x), float(field[k].y), float(field[k].z), 0.f);
-    LG_TRY(upload(s->field32, f32, st));
+    LG_TRY(upload(s->krec, rec, st));
+    std::vector<KRec32> r32(rec.size());
+    for (size_t k = 0; k < rec.size(); k++)
+        r32[k] = KRec32{0.f, 0.f, float(rec[k].fx), float(rec[k].fy), float(rec[k].fz), 0.f, 0.f, 0.f};
+    LG_TRY(upload(s->krec32, r32, st));
     LG_TRY(s->rho.ensure(s->nk));
     // LDS budget for the phase tables
     size_t rows_bytes = size_t(phase_rows(kmax)) * sizeof(double2);
@@ -563,7 +559,7 @@ int prepare_kspace(lg_system* s) {
     LG_TRY(s->phase_q.ensure(s->ld));
     LG_TRY(s->phase_tab32.ensure(size_t(phase_rows(kmax)) * s->ld));
     LG_TRY(s->phase_q32.ensure(s->ld));
-    LG_TRY(s->epart_k.ensure(size_t(blocks_for(s->nk, 64)) * 7));
+    LG_TRY(s->epart_k.ensure(size_t(blocks_for(s->nk, kRhoReduceK)) * 7));
     return LG_OK;
 }
 
@@ -696,7 +692,7 @@ int launch_nlist(lg_system* s, bool check) {
     return LG_OK;
 }
 
-int launch_forces(lg_system* s, bool energy, bool combine_kspace = true) {
+int launch_forces(lg_system* s, bool energy, bool combine = true) {
     hipStream_t st = S(s);
     int n = int(s->n);
     PairArgs pa;
@@ -738,41 +734,31 @@ int launch_forces(lg_system* s, bool energy, bool combine_kspace = true) {
             return fail(LG_E_DEVICE, "invalid lanes-per-atom setting");
     }
     prof_stop(s, prof);
-    if (s->nmol_multi > 0) {
-        MolArgs ma;
-        ma.pos = s->pos.p;
-        ma.meta = s->meta.p;
-        ma.nmol_multi = s->nmol_multi;
-        ma.mol_list = s->mol_list.p;
-        ma.mol_start = s->d_mol_start.p;
-        ma.ipairs = s->ipairs.p;
-        ma.ip_start = s->ip_start.p;
-        ma.bonds = s->d_bonds.p;
-        ma.b_start = s->b_start.p;
-        ma.angles = s->d_angles.p;
-        ma.a_start = s->a_start.p;
-        ma.dihedrals = s->d_dihedrals.p;
-        ma.d_param = s->d_param.p;
-        ma.d_start = s->d_start.p;
-        ma.table = s->table.p;
-        ma.nkinds = int(s->nkinds);
-        ma.pair_restr = s->pair_restr.p;
-        ma.pair_s14 = s->pair_s14.p;
-        ma.bparam = s->bparam.p;
-        ma.aparam = s->aparam.p;
-        ma.dparam = s->dparam.p;
-        ma.coul = s->coul;
-        ma.coul_restr = s->coul_restr;
-        ma.coul_s14 = s->coul_s14;
-        ma.box = s->box;
-        ma.force = s->force.p;
-        ma.epart = s->epart_mol.p;
-        int nb = blocks_for(s->nmol_multi, 256);
+    if (s->nterms > 0) {
+        IntraArgs ia;
+        ia.pos = s->pos.p;
+        ia.meta = s->meta.p;
+        ia.nterms = s->nterms;
+        ia.terms = s->terms.p;
+        ia.table = s->table.p;
+        ia.nkinds = int(s->nkinds);
+        ia.pair_restr = s->pair_restr.p;
+        ia.pair_s14 = s->pair_s14.p;
+        ia.bparam = s->bparam.p;
+        ia.aparam = s->aparam.p;
+        ia.dparam = s->dparam.p;
+        ia.coul = s->coul;
+        ia.coul_restr = s->coul_restr;
+        ia.coul_s14 = s->coul_s14;
+        ia.box = s->box;
+        ia.contrib = s->contrib.p;
+        ia.epart = s->epart_mol.p;
+        int nb = blocks_for(s->nterms, 256);
         long pm = prof_start(s, LG_KERNEL_MOLECULES);
         if (energy)
-            k_molecules<true><<<nb, 256, 0, st>>>(ma);
+            k_intra_terms<true><<<nb, 256, 0, st>>>(ia);
         else
-            k_molecules<false><<<nb, 256, 0, st>>>(ma);
+            k_intra_terms<false><<<nb, 256, 0, st>>>(ia);
         prof_stop(s, pm);
     }
     if (s->coul_kind == COUL_EWALD && s->nk > 0) {
@@ -816,24 +802,29 @@ int launch_forces(lg_system* s, bool energy, bool combine_kspace = true) {
             }
         }
         if (energy)
-            k_rho_reduce<true><<<blocks_for(s->nk, 64), 256, 0, st>>>(s->rho_partial.p, nchunks, s->nk, s->rho.p,
-                                                                     s->efac.p, s->vfac.p, s->epart_k.p);
+            k_rho_reduce<true><<<blocks_for(s->nk, kRhoReduceK), 256, 0, st>>>(
+                s->rho_partial.p, nchunks, s->nk, s->rho.p, mixed ? nullptr : s->krec.p,
+                mixed ? s->krec32.p : nullptr, s->efac.p, s->vfac.p, s->epart_k.p);
         else
-            k_rho_reduce<false><<<blocks_for(s->nk, 64), 256, 0, st>>>(s->rho_partial.p, nchunks, s->nk, s->rho.p,
-                                                                      s->efac.p, s->vfac.p, s->epart_k.p);
+            k_rho_reduce<false><<<blocks_for(s->nk, kRhoReduceK), 256, 0, st>>>(
+                s->rho_partial.p, nchunks, s->nk, s->rho.p, mixed ? nullptr : s->krec.p,
+                mixed ? s->krec32.p : nullptr, s->efac.p, s->vfac.p, s->epart_k.p);
         prof_stop(s, pr);
         long pf = prof_start(s, LG_KERNEL_KSPACE_FORCE);
-        dim3 fgrid(blocks_for(n, 256), s->ksplit);
+        dim3 fgrid(blocks_for(n, 64), s->ksplit);
         if (mixed)
             k_kspace_force32<<<fgrid, 256, 0, st>>>(s->phase_tab32.p, s->ld, n, kmax, s->rows.p, s->nrows,
-                                                     s->kidx.p, s->rho.p, s->field32.p, s->fk.p);
+                                                     s->kidx.p, s->krec32.p, s->fk.p);
         else
             k_kspace_force<<<fgrid, 256, 0, st>>>(s->phase_tab.p, s->ld, n, kmax, s->rows.p, s->nrows, s->kidx.p,
-                                                   s->rho.p, s->field.p, s->fk.p);
-        if (combine_kspace)
-            k_kspace_combine<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->fk.p, s->ksplit, n, s->force.p);
+                                                   s->krec.p, s->fk.p);
         prof_stop(s, pf);
     }
+    bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
+    if (combine && (kspace || s->nterms > 0))
+        k_force_combine<<<blocks_for(n, 256), 256, 0, st>>>(s->force.p, s->contrib.p,
+                                                           s->nterms > 0 ? s->slot_start.p : nullptr, s->fk.p,
+                                                           kspace ? s->ksplit : 0, s->pos.p, n);
     LG_HIP(hipGetLastError());
     return LG_OK;
 }
@@ -877,13 +868,13 @@ int gather_energies(lg_system* s, Energies& E) {
     std::vector<double> hp(size_t(nbp) * 8);
     LG_HIP(hipMemcpyAsync(hp.data(), s->epart_pairs.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     std::vector<double> hm;
-    int nbm = s->nmol_multi > 0 ? blocks_for(s->nmol_multi, 256) : 0;
+    int nbm = s->nterms > 0 ? blocks_for(s->nterms, 256) : 0;
     if (nbm) {
         hm.resize(size_t(nbm) * 11);
         LG_HIP(hipMemcpyAsync(hm.data(), s->epart_mol.p, hm.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     }
     std::vector<double> hk;
-    int nbk = (s->coul_kind == COUL_EWALD && s->nk > 0) ? blocks_for(s->nk, 64) : 0;
+    int nbk = (s->coul_kind == COUL_EWALD && s->nk > 0) ? blocks_for(s->nk, kRhoReduceK) : 0;
     if (nbk) {
         hk.resize(size_t(nbk) * 7);
         LG_HIP(hipMemcpyAsync(hk.data(), s->epart_k.p, hk.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1066,15 +1057,15 @@ int lg_system_destroy(lg_system* s) {
     (void)hipStreamSynchronize(s->ctx->stream);
     DevBuf<int>* ibufs[] = {&s->cellid, &s->slot, &s->count, &s->start, &s->tmp, &s->sorted, &s->nbr, &s->nnbr,
 
-                            &s->flags, &s->pair_restr, &s->mol_list, &s->d_mol_start, &s->ip_start, &s->b_start,
-                            &s->a_start, &s->d_start, &s->d_param};
+                            &s->flags, &s->pair_restr, &s->slot_start};
     for (auto* b : ibufs) b->release();
     s->phase_tab.release();
     s->phase_q.release();
     s->phase_tab32.release();
     s->phase_q32.release();
-    s->field32.release();
-    DevBuf<double4>* d4[] = {&s->pos, &s->vel, &s->acc, &s->force, &s->ref, &s->field, &s->fk,
+    s->krec32.release();
+    s->krec.release();
+    DevBuf<double4>* d4[] = {&s->pos, &s->vel, &s->acc, &s->force, &s->ref, &s->fk,
                              &s->snap_pos, &s->snap_vel, &s->snap_acc};
     for (auto* b : d4) b->release();
     DevBuf<double>* d1[] = {&s->pair_s14, &s->efac, &s->vfac, &s->epart_pairs, &s->epart_mol, &s->epart_k,
@@ -1083,10 +1074,8 @@ int lg_system_destroy(lg_system* s) {
     for (auto e : s->prof_events) (void)hipEventDestroy(e);
     s->meta.release();
     s->table.release();
-    s->ipairs.release();
-    s->d_bonds.release();
-    s->d_angles.release();
-    s->d_dihedrals.release();
+    s->terms.release();
+    s->contrib.release();
     s->bparam.release();
     s->aparam.release();
     s->dparam.release();
@@ -1498,12 +1487,13 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
             long p2 = prof_start(s, LG_KERNEL_INTEGRATE);
             double* kp = thermo ? s->ke_part.p : nullptr;
             int ks = kspace ? s->ksplit : 0;
+            const int* sl = s->nterms > 0 ? s->slot_start.p : nullptr;
             if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
-                k_md_second<false><<<nb, 256, 0, st>>>(s->force.p, s->pos.p, s->fk.p, ks, s->vel.p, s->acc.p, n,
-                                                       s->dt, kp);
+                k_md_second<false><<<nb, 256, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
+                                                       s->vel.p, s->acc.p, n, s->dt, kp);
             else
-                k_md_second<true><<<nb, 256, 0, st>>>(s->force.p, s->pos.p, s->fk.p, ks, s->vel.p, s->acc.p, n,
-                                                      s->dt, kp);
+                k_md_second<true><<<nb, 256, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
+                                                      s->vel.p, s->acc.p, n, s->dt, kp);
             if (thermo)
                 k_berendsen<<<1, 256, 0, st>>>(s->ke_part.p, nb, double(3 * s->n), s->berendsen_T, s->berendsen_tau,
                                                s->lambda.p);

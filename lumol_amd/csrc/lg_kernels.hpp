@@ -493,36 +493,37 @@ __global__ void k_pairs_within(const double4* __restrict__ pos, const int* __res
 }
 
 // ---------------------------------------------------------------------------------
-// Intra-molecular kernel: one thread per molecule with >= 2 atoms.  Pairs in
-// the same molecule with their bond path (restrictions), bonds, angles and
-// dihedrals (compute.rs:60-97, energy.rs:88-138, ewald.rs:387-428 for the
-// excluded-pair correction).  The thread owns its molecule's atoms, so it
-// adds into force[] without atomics; it runs after k_pairs on the stream.
+// Intra-molecular terms: one thread per term (same-molecule pair with its bond
+// path, bond, angle or dihedral), compute.rs:60-97, energy.rs:88-138,
+// ewald.rs:387-428 (excluded-pair correction), cells.rs:335-411.  Each term
+// writes its force on each of its atoms to a slot reserved for that
+// (term, atom); the consumer adds an atom's slots in slot order, so the sums
+// are deterministic without atomics.  Terms are sorted by type on the host so
+// that waves do not diverge.
 // ---------------------------------------------------------------------------------
 struct BondedParam {
     int pot;  // POT_* (POT_NONE: no potential)
     double p0, p1;
 };
 
-struct MolArgs {
+enum : int { TERM_PAIR = 0, TERM_BOND = 1, TERM_ANGLE = 2, TERM_DIHEDRAL = 3 };
+
+struct Term {
+    int4 atoms;  // pair/bond: x, y; angle: x, y, z; dihedral: x, y, z, w
+    int4 slots;  // contribution slot of each atom
+    int type;    // TERM_*
+    int param;   // pair: bond path; bond/angle/dihedral: parameter index
+};
+
+struct IntraArgs {
     const double4* pos;
     const int2* meta;
-    int nmol_multi;
-    const int* mol_list;       // molecules with >= 2 atoms
-    const int* mol_start;      // nmol + 1
-    const int4* ipairs;        // (a, b, path, -) per molecule, CSR by ip_start
-    const int* ip_start;
-    const int4* bonds;         // (a, b, param index, -)
-    const int* b_start;
-    const int4* angles;        // (a, b, c, param index)
-    const int* a_start;
-    const int4* dihedrals;     // (a, b, c, d)
-    const int* d_param;        // param index per dihedral
-    const int* d_start;
+    int nterms;
+    const Term* terms;
     const PairParam* table;
     int nkinds;
-    const int* pair_restr;     // restriction per kind pair
-    const double* pair_s14;    // scale14 per kind pair
+    const int* pair_restr;  // restriction per kind pair
+    const double* pair_s14; // scale14 per kind pair
     const BondedParam* bparam;
     const BondedParam* aparam;
     const BondedParam* dparam;
@@ -530,47 +531,25 @@ struct MolArgs {
     int coul_restr;
     double coul_s14;
     Box box;
-    double4* force;
-    double* epart;  // per block: pairs, coul, bonds, angles, dihedrals, w6
+    double4* contrib;  // slot contributions
+    double* epart;     // per block: pairs, coul, bonds, angles, dihedrals, w6
 };
 
-__device__ __forceinline__ void vimage(const Box& b, double& x, double& y, double& z) { min_image(b, x, y, z); }
-
 template <bool kEnergy>
-__global__ __launch_bounds__(256) void k_molecules(MolArgs a) {
+__global__ __launch_bounds__(256) void k_intra_terms(IntraArgs a) {
     __shared__ double red[4][11];
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (idx < a.nmol_multi) {
-        int m = a.mol_list[idx];
-        int m0 = a.mol_start[m], m1 = a.mol_start[m + 1];
-        int natm = m1 - m0;
-        // local force accumulators (molecules are small; spill to global beyond 8)
-        double lf[8][3];
-        for (int q = 0; q < 8; q++) lf[q][0] = lf[q][1] = lf[q][2] = 0.0;
-        auto addf = [&](int atom, double x, double y, double z) {
-            int l = atom - m0;
-            if (l < 8) {
-                lf[l][0] += x;
-                lf[l][1] += y;
-                lf[l][2] += z;
-            } else {
-                a.force[atom].x += x;
-                a.force[atom].y += y;
-                a.force[atom].z += z;
-            }
-        };
-        // intra-molecular pairs
-        for (int p = a.ip_start[m]; p < a.ip_start[m + 1]; p++) {
-            int4 ip = a.ipairs[p];
-            int i = ip.x, j = ip.y, path = ip.z;
+    if (t < a.nterms) {
+        Term tm = a.terms[t];
+        if (tm.type == TERM_PAIR) {
+            int i = tm.atoms.x, j = tm.atoms.y, path = tm.param;
             double4 pi = a.pos[i], pj = a.pos[j];
             double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             min_image(a.box, dx, dy, dz);
             double r = sqrt(norm2_ordered(dx, dy, dz));
             double inv_r = 1.0 / r;
-            int ki = a.meta[i].x, kj = a.meta[j].x;
-            int tk = ki * a.nkinds + kj;
+            int tk = a.meta[i].x * a.nkinds + a.meta[j].x;
             const PairParam& pp = a.table[tk];
             double fs = 0.0;
             if (pp.pot != POT_NONE) {
@@ -602,119 +581,166 @@ __global__ __launch_bounds__(256) void k_molecules(MolArgs a) {
                     }
                 }
             }
-            addf(i, fs * dx, fs * dy, fs * dz);
-            addf(j, -fs * dx, -fs * dy, -fs * dz);
+            a.contrib[tm.slots.x] = make_double4(fs * dx, fs * dy, fs * dz, 0.0);
+            a.contrib[tm.slots.y] = make_double4(-fs * dx, -fs * dy, -fs * dz, 0.0);
             acc[5] += fs * dx * dx;
             acc[6] += fs * dy * dy;
             acc[7] += fs * dz * dz;
             acc[8] += fs * dx * dy;
             acc[9] += fs * dx * dz;
             acc[10] += fs * dy * dz;
-        }
-        // bonds (compute.rs:63-73; virial compute.rs:231-239)
-        for (int p = a.b_start[m]; p < a.b_start[m + 1]; p++) {
-            int4 bd = a.bonds[p];
-            const BondedParam& bp = a.bparam[bd.z];
-            if (bp.pot == POT_NONE) continue;
-            double4 pi = a.pos[bd.x], pj = a.pos[bd.y];
-            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            min_image(a.box, dx, dy, dz);
-            double r = sqrt(norm2_ordered(dx, dy, dz));
-            double inv_r = 1.0 / r;
-            double e, f;
-            pot_eval(bp.pot, bp.p0, bp.p1, r, inv_r, e, f);
-            addf(bd.x, f * dx, f * dy, f * dz);
-            addf(bd.y, -f * dx, -f * dy, -f * dz);
-            acc[2] += e;
-            acc[5] += f * dx * dx;
-            acc[6] += f * dy * dy;
-            acc[7] += f * dz * dz;
-            acc[8] += f * dx * dy;
-            acc[9] += f * dx * dz;
-            acc[10] += f * dy * dz;
-        }
-        // angles (cells.rs:335-359)
-        for (int p = a.a_start[m]; p < a.a_start[m + 1]; p++) {
-            int4 an = a.angles[p];
-            const BondedParam& ap = a.aparam[an.w];
-            if (ap.pot == POT_NONE) continue;
-            double4 p1 = a.pos[an.x], p2 = a.pos[an.y], p3 = a.pos[an.z];
-            double ax = p1.x - p2.x, ay = p1.y - p2.y, az = p1.z - p2.z;
-            double bx = p3.x - p2.x, by = p3.y - p2.y, bz = p3.z - p2.z;
-            vimage(a.box, ax, ay, az);
-            vimage(a.box, bx, by, bz);
-            double na = sqrt(norm2_ordered(ax, ay, az)), nb = sqrt(norm2_ordered(bx, by, bz));
-            double ux = ax / na, uy = ay / na, uz = az / na;
-            double vx = bx / nb, vy = by / nb, vz = bz / nb;
-            double cs = ux * vx + uy * vy + uz * vz;
-            double sin_inv = 1.0 / sqrt(1.0 - cs * cs);
-            double d1x = sin_inv * (cs * ux - vx) / na, d1y = sin_inv * (cs * uy - vy) / na,
-                   d1z = sin_inv * (cs * uz - vz) / na;
-            double d3x = sin_inv * (cs * vx - ux) / nb, d3y = sin_inv * (cs * vy - uy) / nb,
-                   d3z = sin_inv * (cs * vz - uz) / nb;
-            double theta = acos(cs);
-            double e, f;
-            pot_eval(ap.pot, ap.p0, ap.p1, theta, 1.0, e, f);
-            // pot_eval returns force/r with inv_r = 1 -> plain -dV/dtheta
-            addf(an.x, f * d1x, f * d1y, f * d1z);
-            addf(an.z, f * d3x, f * d3y, f * d3z);
-            addf(an.y, -f * (d1x + d3x), -f * (d1y + d3y), -f * (d1z + d3z));
-            acc[3] += e;
-        }
-        // dihedrals (cells.rs:379-411)
-        for (int p = a.d_start[m]; p < a.d_start[m + 1]; p++) {
-            int4 dh = a.dihedrals[p];
-            const BondedParam& dp = a.dparam[a.d_param[p]];
-            if (dp.pot == POT_NONE) continue;
-            double4 q1 = a.pos[dh.x], q2 = a.pos[dh.y], q3 = a.pos[dh.z], q4 = a.pos[dh.w];
-            double r12x = q2.x - q1.x, r12y = q2.y - q1.y, r12z = q2.z - q1.z;
-            double r23x = q3.x - q2.x, r23y = q3.y - q2.y, r23z = q3.z - q2.z;
-            double r34x = q4.x - q3.x, r34y = q4.y - q3.y, r34z = q4.z - q3.z;
-            vimage(a.box, r12x, r12y, r12z);
-            vimage(a.box, r23x, r23y, r23z);
-            vimage(a.box, r34x, r34y, r34z);
-            double ux = r12y * r23z - r12z * r23y, uy = r12z * r23x - r12x * r23z, uz = r12x * r23y - r12y * r23x;
-            double vx = r23y * r34z - r23z * r34y, vy = r23z * r34x - r23x * r34z, vz = r23x * r34y - r23y * r34x;
-            double u2 = ux * ux + uy * uy + uz * uz, v2 = vx * vx + vy * vy + vz * vz;
-            double r23_2 = r23x * r23x + r23y * r23y + r23z * r23z;
-            double r23n = sqrt(r23_2);
-            double d1x = (-r23n / u2) * ux, d1y = (-r23n / u2) * uy, d1z = (-r23n / u2) * uz;
-            double d4x = (r23n / v2) * vx, d4y = (r23n / v2) * vy, d4z = (r23n / v2) * vz;
-            double r23_r34 = r23x * r34x + r23y * r34y + r23z * r34z;
-            double r12_r23 = r12x * r23x + r12y * r23y + r12z * r23z;
-            double c1 = -r12_r23 / r23_2 - 1.0, c2 = r23_r34 / r23_2;
-            double c3 = -r23_r34 / r23_2 - 1.0, c4 = r12_r23 / r23_2;
-            double d2x = c1 * d1x + c2 * d4x, d2y = c1 * d1y + c2 * d4y, d2z = c1 * d1z + c2 * d4z;
-            double d3x = c3 * d4x + c4 * d1x, d3y = c3 * d4y + c4 * d1y, d3z = c3 * d4z + c4 * d1z;
-            double phi = atan2(r23n * (vx * r12x + vy * r12y + vz * r12z), ux * vx + uy * vy + uz * vz);
-            double e, f;
-            pot_eval(dp.pot, dp.p0, dp.p1, phi, 1.0, e, f);
-            addf(dh.x, f * d1x, f * d1y, f * d1z);
-            addf(dh.y, f * d2x, f * d2y, f * d2z);
-            addf(dh.z, f * d3x, f * d3y, f * d3z);
-            addf(dh.w, f * d4x, f * d4y, f * d4z);
-            acc[4] += e;
-        }
-        for (int q = 0; q < natm && q < 8; q++) {
-            int atom = m0 + q;
-            a.force[atom].x += lf[q][0];
-            a.force[atom].y += lf[q][1];
-            a.force[atom].z += lf[q][2];
+        } else if (tm.type == TERM_BOND) {
+            // compute.rs:63-73; virial compute.rs:231-239
+            const BondedParam& bp = a.bparam[tm.param];
+            double fx = 0, fy = 0, fz = 0;
+            if (bp.pot != POT_NONE) {
+                double4 pi = a.pos[tm.atoms.x], pj = a.pos[tm.atoms.y];
+                double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                min_image(a.box, dx, dy, dz);
+                double r = sqrt(norm2_ordered(dx, dy, dz));
+                double e, f;
+                pot_eval(bp.pot, bp.p0, bp.p1, r, 1.0 / r, e, f);
+                fx = f * dx;
+                fy = f * dy;
+                fz = f * dz;
+                acc[2] += e;
+                acc[5] += f * dx * dx;
+                acc[6] += f * dy * dy;
+                acc[7] += f * dz * dz;
+                acc[8] += f * dx * dy;
+                acc[9] += f * dx * dz;
+                acc[10] += f * dy * dz;
+            }
+            a.contrib[tm.slots.x] = make_double4(fx, fy, fz, 0.0);
+            a.contrib[tm.slots.y] = make_double4(-fx, -fy, -fz, 0.0);
+        } else if (tm.type == TERM_ANGLE) {
+            // cells.rs:335-359
+            const BondedParam& ap = a.aparam[tm.param];
+            double4 c0 = make_double4(0, 0, 0, 0), c1 = c0, c2 = c0;
+            if (ap.pot != POT_NONE) {
+                double4 p1 = a.pos[tm.atoms.x], p2 = a.pos[tm.atoms.y], p3 = a.pos[tm.atoms.z];
+                double ax = p1.x - p2.x, ay = p1.y - p2.y, az = p1.z - p2.z;
+                double bx = p3.x - p2.x, by = p3.y - p2.y, bz = p3.z - p2.z;
+                min_image(a.box, ax, ay, az);
+                min_image(a.box, bx, by, bz);
+                double na = sqrt(norm2_ordered(ax, ay, az)), nb = sqrt(norm2_ordered(bx, by, bz));
+                double ux = ax / na, uy = ay / na, uz = az / na;
+                double vx = bx / nb, vy = by / nb, vz = bz / nb;
+                double cs = ux * vx + uy * vy + uz * vz;
+                double sin_inv = 1.0 / sqrt(1.0 - cs * cs);
+                double d1x = sin_inv * (cs * ux - vx) / na, d1y = sin_inv * (cs * uy - vy) / na,
+                       d1z = sin_inv * (cs * uz - vz) / na;
+                double d3x = sin_inv * (cs * vx - ux) / nb, d3y = sin_inv * (cs * vy - uy) / nb,
+                       d3z = sin_inv * (cs * vz - uz) / nb;
+                double theta = acos(cs);
+                double e, f;
+                // with inv_r = 1 pot_eval returns the plain -dV/dtheta
+                pot_eval(ap.pot, ap.p0, ap.p1, theta, 1.0, e, f);
+                c0 = make_double4(f * d1x, f * d1y, f * d1z, 0.0);
+                c1 = make_double4(-f * (d1x + d3x), -f * (d1y + d3y), -f * (d1z + d3z), 0.0);
+                c2 = make_double4(f * d3x, f * d3y, f * d3z, 0.0);
+                acc[3] += e;
+            }
+            a.contrib[tm.slots.x] = c0;
+            a.contrib[tm.slots.y] = c1;
+            a.contrib[tm.slots.z] = c2;
+        } else {
+            // cells.rs:379-411
+            const BondedParam& dp = a.dparam[tm.param];
+            double4 c0 = make_double4(0, 0, 0, 0), c1 = c0, c2 = c0, c3 = c0;
+            if (dp.pot != POT_NONE) {
+                double4 q1 = a.pos[tm.atoms.x], q2 = a.pos[tm.atoms.y], q3 = a.pos[tm.atoms.z],
+                        q4 = a.pos[tm.atoms.w];
+                double r12x = q2.x - q1.x, r12y = q2.y - q1.y, r12z = q2.z - q1.z;
+                double r23x = q3.x - q2.x, r23y = q3.y - q2.y, r23z = q3.z - q2.z;
+                double r34x = q4.x - q3.x, r34y = q4.y - q3.y, r34z = q4.z - q3.z;
+                min_image(a.box, r12x, r12y, r12z);
+                min_image(a.box, r23x, r23y, r23z);
+                min_image(a.box, r34x, r34y, r34z);
+                double ux = r12y * r23z - r12z * r23y, uy = r12z * r23x - r12x * r23z,
+                       uz = r12x * r23y - r12y * r23x;
+                double vx = r23y * r34z - r23z * r34y, vy = r23z * r34x - r23x * r34z,
+                       vz = r23x * r34y - r23y * r34x;
+                double u2 = ux * ux + uy * uy + uz * uz, v2 = vx * vx + vy * vy + vz * vz;
+                double r23_2 = r23x * r23x + r23y * r23y + r23z * r23z;
+                double r23n = sqrt(r23_2);
+                double d1x = (-r23n / u2) * ux, d1y = (-r23n / u2) * uy, d1z = (-r23n / u2) * uz;
+                double d4x = (r23n / v2) * vx, d4y = (r23n / v2) * vy, d4z = (r23n / v2) * vz;
+                double r23_r34 = r23x * r34x + r23y * r34y + r23z * r34z;
+                double r12_r23 = r12x * r23x + r12y * r23y + r12z * r23z;
+                double k1 = -r12_r23 / r23_2 - 1.0, k2 = r23_r34 / r23_2;
+                double k3 = -r23_r34 / r23_2 - 1.0, k4 = r12_r23 / r23_2;
+                double d2x = k1 * d1x + k2 * d4x, d2y = k1 * d1y + k2 * d4y, d2z = k1 * d1z + k2 * d4z;
+                double d3x = k3 * d4x + k4 * d1x, d3y = k3 * d4y + k4 * d1y, d3z = k3 * d4z + k4 * d1z;
+                double phi = atan2(r23n * (vx * r12x + vy * r12y + vz * r12z), ux * vx + uy * vy + uz * vz);
+                double e, f;
+                pot_eval(dp.pot, dp.p0, dp.p1, phi, 1.0, e, f);
+                c0 = make_double4(f * d1x, f * d1y, f * d1z, 0.0);
+                c1 = make_double4(f * d2x, f * d2y, f * d2z, 0.0);
+                c2 = make_double4(f * d3x, f * d3y, f * d3z, 0.0);
+                c3 = make_double4(f * d4x, f * d4y, f * d4z, 0.0);
+                acc[4] += e;
+            }
+            a.contrib[tm.slots.x] = c0;
+            a.contrib[tm.slots.y] = c1;
+            a.contrib[tm.slots.z] = c2;
+            a.contrib[tm.slots.w] = c3;
         }
     }
     if (kEnergy) {
         int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
         for (int q = 0; q < 11; q++) {
-            double s = wave_sum(acc[q]);
-            if (lane == 0) red[wid][q] = s;
+            double sum = wave_sum(acc[q]);
+            if (lane == 0) red[wid][q] = sum;
         }
         __syncthreads();
         if (threadIdx.x < 11) {
-            double s = 0;
-            for (int w = 0; w < 4; w++) s += red[w][threadIdx.x];
-            a.epart[blockIdx.x * 11 + threadIdx.x] = s;
+            double sum = 0;
+            for (int w = 0; w < 4; w++) sum += red[w][threadIdx.x];
+            a.epart[blockIdx.x * 11 + threadIdx.x] = sum;
         }
     }
+}
+
+// Total force on atom i: the non-bonded pair force, the intra-molecular slot
+// contributions in slot order, and the k-space partials
+// q_i / (4 pi eps0) sum_g fk[g][i] (ewald.rs:716-719).  Shared by the MD
+// second half and k_force_combine so both paths add in the same order.
+__device__ __forceinline__ double4 total_force(int i, int n, const double4* __restrict__ force,
+                                               const double4* __restrict__ contrib, const int* __restrict__ slot_start,
+                                               const double4* __restrict__ fk, int ksplit,
+                                               const double4* __restrict__ pos) {
+    double4 f = force[i];
+    if (slot_start) {
+        for (int sl = slot_start[i]; sl < slot_start[i + 1]; sl++) {
+            double4 c = contrib[sl];
+            f.x += c.x;
+            f.y += c.y;
+            f.z += c.z;
+        }
+    }
+    if (ksplit > 0) {
+        double sx = 0, sy = 0, sz = 0;
+        for (int g = 0; g < ksplit; g++) {
+            double4 p = fk[(size_t)g * n + i];
+            sx += p.x;
+            sy += p.y;
+            sz += p.z;
+        }
+        double q = pos[i].w;
+        f.x += (q * sx) / kFourPiEps0;
+        f.y += (q * sy) / kFourPiEps0;
+        f.z += (q * sz) / kFourPiEps0;
+    }
+    return f;
+}
+
+__global__ void k_force_combine(double4* __restrict__ force, const double4* __restrict__ contrib,
+                                const int* __restrict__ slot_start, const double4* __restrict__ fk, int ksplit,
+                                const double4* __restrict__ pos, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    force[i] = total_force(i, n, force, contrib, slot_start, fk, ksplit, pos);
 }
 
 // ---------------------------------------------------------------------------------
@@ -821,32 +847,59 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_partial(const double2* __rest
     }
 }
 
-// rho(k) = sum over chunks, in a fixed order: wave w of the block sums the
-// chunks c = w mod 4, then wave 0 adds the four in order.  With kEnergy the
-// block writes its energy and virial partials: e = factor |rho|^2 and
-// w = |rho|^2 * virial_k (6 components).  grid: ceil(nk / 64), block 256.
+// Per k-vector record read by the force kernels with one scalar load:
+// rho(k) (written by k_rho_reduce) and the force factor
+// field_k = 2 * factor_k * k (ewald.rs:712-714), constant per cell.
+struct alignas(64) KRec {
+    double re, im, fx, fy, fz, pad0, pad1, pad2;
+};
+struct alignas(32) KRec32 {
+    float re, im, fx, fy, fz, pad0, pad1, pad2;
+};
+
+// rho(k) = sum over chunks, in a fixed order: thread (g, kl) of the block
+// sums the chunks c = g mod 16 for k = 16 * block + kl, then 16 threads add
+// the 16 group sums in order.  With kEnergy the block writes its energy and
+// virial partials: e = factor |rho|^2 and w = |rho|^2 * virial_k
+// (6 components).  grid: ceil(nk / kRhoReduceK), block 256.
+constexpr int kRhoReduceK = 16;
+
 template <bool kEnergy>
 __global__ __launch_bounds__(256) void k_rho_reduce(const double2* __restrict__ partial, int nchunks, int nk,
-                                                    double2* __restrict__ rho, const double* __restrict__ efac,
+                                                    double2* __restrict__ rho, KRec* __restrict__ rec,
+                                                    KRec32* __restrict__ rec32, const double* __restrict__ efac,
                                                     const double* __restrict__ vfac, double* __restrict__ epart) {
-    __shared__ double2 part[4][64];
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int k = blockIdx.x * 64 + lane;
+    constexpr int G = 256 / kRhoReduceK;
+    __shared__ double2 part[G][kRhoReduceK];
+    int g = threadIdx.x / kRhoReduceK, kl = threadIdx.x % kRhoReduceK;
+    int k = blockIdx.x * kRhoReduceK + kl;
     double re = 0.0, im = 0.0;
     if (k < nk)
-        for (int c = w; c < nchunks; c += 4) {
+        for (int c = g; c < nchunks; c += G) {
             double2 p = partial[(size_t)c * nk + k];
             re += p.x;
             im += p.y;
         }
-    part[w][lane] = make_double2(re, im);
+    part[g][kl] = make_double2(re, im);
     __syncthreads();
-    if (w != 0) return;
+    if (threadIdx.x >= 64) return;
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (k < nk) {
-        re = part[0][lane].x + part[1][lane].x + part[2][lane].x + part[3][lane].x;
-        im = part[0][lane].y + part[1][lane].y + part[2][lane].y + part[3][lane].y;
+    if (threadIdx.x < kRhoReduceK && k < nk) {
+        re = 0.0;
+        im = 0.0;
+        for (int q = 0; q < G; q++) {
+            re += part[q][kl].x;
+            im += part[q][kl].y;
+        }
         rho[k] = make_double2(re, im);
+        if (rec) {
+            rec[k].re = re;
+            rec[k].im = im;
+        }
+        if (rec32) {
+            rec32[k].re = (float)re;
+            rec32[k].im = (float)im;
+        }
         if (kEnergy) {
             double n2 = re * re + im * im;
             v[0] = efac[k] * n2;
@@ -856,7 +909,7 @@ __global__ __launch_bounds__(256) void k_rho_reduce(const double2* __restrict__ 
     if (kEnergy) {
         for (int x = 0; x < 7; x++) {
             double sum = wave_sum(v[x]);
-            if (lane == 0) epart[blockIdx.x * 7 + x] = sum;
+            if (threadIdx.x == 0) epart[blockIdx.x * 7 + x] = sum;
         }
     }
 }
@@ -868,20 +921,47 @@ struct KRow {
     int ikx, iky, kbase, count;
 };
 
-// One lane per atom (256 per block); grid.y splits the rows, split g writes
-// its partial to fk[g * n + i] (combined in a fixed order by the consumer).
-// A lane reads only its own atom's column of the phase table, so the reads
-// of a wave are contiguous and come from L2; rows have consecutive ikz.
+// Block = 64 atoms x 4 waves.  grid.y splits the rows into ksplit ranges and
+// wave w of the block takes the w-th quarter of its range; the four wave
+// partials are added in order in LDS and split g writes fk[g * n + i]
+// (combined in a fixed order by the consumer).  A lane reads only its own
+// atom's column of the phase table, so the reads of a wave are contiguous
+// and come from L2; rows have consecutive ikz.
+__device__ __forceinline__ void kspace_wave_rows(int nrows, int& r0, int& r1) {
+    // wave index made visibly uniform, so that rows[] and the k records are
+    // read with scalar loads
+    int nsplit = gridDim.y, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int s0 = (int)(((long)nrows * blockIdx.y) / nsplit);
+    int s1 = (int)(((long)nrows * (blockIdx.y + 1)) / nsplit);
+    r0 = s0 + ((s1 - s0) * w) / 4;
+    r1 = s0 + ((s1 - s0) * (w + 1)) / 4;
+}
+
+__device__ __forceinline__ void kspace_block_store(double fx, double fy, double fz, int n,
+                                                   double4* __restrict__ fk) {
+    __shared__ double red[3][4][64];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    red[0][w][lane] = fx;
+    red[1][w][lane] = fy;
+    red[2][w][lane] = fz;
+    __syncthreads();
+    int i = blockIdx.x * 64 + lane;
+    if (w == 0 && i < n) {
+        double sx = ((red[0][0][lane] + red[0][1][lane]) + red[0][2][lane]) + red[0][3][lane];
+        double sy = ((red[1][0][lane] + red[1][1][lane]) + red[1][2][lane]) + red[1][3][lane];
+        double sz = ((red[2][0][lane] + red[2][1][lane]) + red[2][2][lane]) + red[2][3][lane];
+        fk[(size_t)blockIdx.y * n + i] = make_double4(sx, sy, sz, 0.0);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_kspace_force(const double2* __restrict__ tab, int ld, int n, int kmax,
                                                       const KRow* __restrict__ rows, int nrows,
                                                       const int4* __restrict__ kidx,
-                                                      const double2* __restrict__ rho,
-                                                      const double4* __restrict__ field,
+                                                      const KRec* __restrict__ rec,
                                                       double4* __restrict__ fk) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int nsplit = gridDim.y;
-    int r0 = (int)(((long)nrows * blockIdx.y) / nsplit);
-    int r1 = (int)(((long)nrows * (blockIdx.y + 1)) / nsplit);
+    int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    int r0, r1;
+    kspace_wave_rows(nrows, r0, r1);
     int a = i < ld ? i : ld - 1;
     const double2* col = tab + a;
     // e^{i k.r} along a row: (Ex * Ey) * Ez[ikz0], then one product by
@@ -894,18 +974,16 @@ __global__ __launch_bounds__(256) void k_kspace_force(const double2* __restrict_
         int ikz0 = kidx[row.kbase].z;
         double2 e = cmul(exy, col[(size_t)phase_row(2, ikz0, kmax) * ld]);
         for (int c = 0; c < row.count; c++) {
-            int k = row.kbase + c;
-            double2 rk = rho[k];
+            const KRec& kr = rec[row.kbase + c];
             // Im(e * conj(rho)) = e.im * rho.re - e.re * rho.im
-            double s = e.y * rk.x - e.x * rk.y;
-            double4 f4 = field[k];
-            fx += s * f4.x;
-            fy += s * f4.y;
-            fz += s * f4.z;
+            double sv = e.y * kr.re - e.x * kr.im;
+            fx += sv * kr.fx;
+            fy += sv * kr.fy;
+            fz += sv * kr.fz;
             e = cmul(e, ez1);
         }
     }
-    if (i < n) fk[(size_t)blockIdx.y * n + i] = make_double4(fx, fy, fz, 0.0);
+    kspace_block_store(fx, fy, fz, n, fk);
 }
 
 // ---- mixed-form k-space ----------------------------------------------------------
@@ -994,18 +1072,16 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_partial32(const float2* __res
     }
 }
 
-// k-space forces, mixed form: FP32 products, each row summed in FP32 and
-// added to FP64 accumulators.  rho32/field32 are FP32 copies of rho/field.
+// Mixed form: FP32 phases, rho and field; each row's sum is added to the
+// FP64 accumulators.
 __global__ __launch_bounds__(256) void k_kspace_force32(const float2* __restrict__ tab, int ld, int n, int kmax,
                                                         const KRow* __restrict__ rows, int nrows,
                                                         const int4* __restrict__ kidx,
-                                                        const double2* __restrict__ rho,
-                                                        const float4* __restrict__ field32,
+                                                        const KRec32* __restrict__ rec,
                                                         double4* __restrict__ fk) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int nsplit = gridDim.y;
-    int r0 = (int)(((long)nrows * blockIdx.y) / nsplit);
-    int r1 = (int)(((long)nrows * (blockIdx.y + 1)) / nsplit);
+    int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    int r0, r1;
+    kspace_wave_rows(nrows, r0, r1);
     int a = i < ld ? i : ld - 1;
     const float2* col = tab + a;
     const float2 ez1 = col[(size_t)phase_row(2, 1, kmax) * ld];
@@ -1017,42 +1093,20 @@ __global__ __launch_bounds__(256) void k_kspace_force32(const float2* __restrict
         float2 e = cmulf(exy, col[(size_t)phase_row(2, ikz0, kmax) * ld]);
         float rx = 0.f, ry = 0.f, rz = 0.f;
         for (int c = 0; c < row.count; c++) {
-            int k = row.kbase + c;
-            double2 rk = rho[k];
-            float s = e.y * (float)rk.x - e.x * (float)rk.y;
-            float4 f4 = field32[k];
-            rx += s * f4.x;
-            ry += s * f4.y;
-            rz += s * f4.z;
+            const KRec32& kr = rec[row.kbase + c];
+            float sv = e.y * kr.re - e.x * kr.im;
+            rx += sv * kr.fx;
+            ry += sv * kr.fy;
+            rz += sv * kr.fz;
             e = cmulf(e, ez1);
         }
         fx += rx;
         fy += ry;
         fz += rz;
     }
-    if (i < n) fk[(size_t)blockIdx.y * n + i] = make_double4(fx, fy, fz, 0.0);
+    kspace_block_store(fx, fy, fz, n, fk);
 }
 
-
-// force[i] += q_i / (4 pi eps0) * sum_g fk[g][i]   (ewald.rs:716-719)
-__global__ void k_kspace_combine(const double4* __restrict__ pos, const double4* __restrict__ fk, int nsplit, int n,
-                                 double4* __restrict__ force) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double fx = 0, fy = 0, fz = 0;
-    for (int g = 0; g < nsplit; g++) {
-        double4 p = fk[(size_t)g * n + i];
-        fx += p.x;
-        fy += p.y;
-        fz += p.z;
-    }
-    double q = pos[i].w;
-    double4 f = force[i];
-    f.x += (q * fx) / kFourPiEps0;
-    f.y += (q * fy) / kFourPiEps0;
-    f.z += (q * fz) / kFourPiEps0;
-    force[i] = f;
-}
 
 // ---------------------------------------------------------------------------------
 // Integrators (lumol-sim/src/md/integrators.rs), FP64, unfused as in the
@@ -1133,6 +1187,8 @@ __global__ void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel,
 // are written for k_berendsen.
 template <bool kLeapFrog>
 __global__ __launch_bounds__(256) void k_md_second(const double4* __restrict__ force, const double4* __restrict__ pos,
+                                                   const double4* __restrict__ contrib,
+                                                   const int* __restrict__ slot_start,
                                                    const double4* __restrict__ fk, int ksplit,
                                                    double4* __restrict__ vel, double4* __restrict__ acc, int n,
                                                    double dt, double* __restrict__ ke_part) {
@@ -1140,20 +1196,7 @@ __global__ __launch_bounds__(256) void k_md_second(const double4* __restrict__ f
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ke = 0.0;
     if (i < n) {
-        double4 f = force[i];
-        if (ksplit > 0) {
-            double sx = 0, sy = 0, sz = 0;
-            for (int g = 0; g < ksplit; g++) {
-                double4 p = fk[(size_t)g * n + i];
-                sx += p.x;
-                sy += p.y;
-                sz += p.z;
-            }
-            double q = pos[i].w;
-            f.x += (q * sx) / kFourPiEps0;
-            f.y += (q * sy) / kFourPiEps0;
-            f.z += (q * sz) / kFourPiEps0;
-        }
+        double4 f = total_force(i, n, force, contrib, slot_start, fk, ksplit, pos);
         double4 v = vel[i];
         double m = v.w;
         double4 na = make_double4(f.x / m, f.y / m, f.z / m, 0.0);
