@@ -194,16 +194,40 @@ int lg_md_run(lg_system* sys, uint64_t nsteps);
 
 /* ---- Monte Carlo energy cache ---------------------------------------------------------
  * EnergyCache::init (lumol-core/src/sys/cache.rs:75-98).  Computes the
- * cached energy terms and the Ewald rho(k) cache on the device. */
+ * cached energy terms and the Ewald rho(k) cache on the device; `energy`
+ * (may be NULL) receives EnergyCache::energy (cache.rs:101-113).  Setting
+ * positions, charges, cell or potentials, or running MD, drops the cache; the
+ * calls below re-initialise it when needed. */
 int lg_cache_init(lg_system* sys, double* energy);
+/* EnergyCache::energy (cache.rs:101-113) of the cached configuration. */
+int lg_cache_energy(lg_system* sys, double* energy);
 /* EnergyCache::move_molecule_cost (cache.rs:145-220): energy change of
- * moving molecule `molid` to `newpos` (3 * molecule size doubles). */
+ * moving molecule `molid` to `newpos` (3 * molecule size doubles), pairs +
+ * coulomb (SharedEwald/Wolf GlobalCache::move_molecule_cost, ewald.rs:933-945,
+ * wolf.rs:121-165).  Stages the move.  At most 32 atoms per molecule. */
 int lg_move_cost(lg_system* sys, uint32_t molid, const double* newpos, double* cost);
 /* EnergyCache::update (cache.rs:118-127) + Translate::apply
- * (lumol-sim/src/mc/moves/translate.rs:115-127, COM wrap when wrap != 0). */
+ * (lumol-sim/src/mc/moves/translate.rs:115-127, COM wrap when wrap != 0).
+ * LG_E_STATE when no move is staged. */
 int lg_move_commit(lg_system* sys, int wrap);
 /* Rejected move: drop the staged update (monte_carlo.rs:318-321). */
 int lg_move_discard(lg_system* sys);
+/* MonteCarlo::propagate (lumol-sim/src/mc/monte_carlo.rs:289-330) with one
+ * Translate move (translate.rs:41-148) repeated for attempts attempt0 ..
+ * attempt0 + nattempts - 1, entirely on the device: uniform molecule choice,
+ * displacement U(+-delta/sqrt(3)) per axis, Metropolis on beta * cost, COM
+ * wrap on accept.  Random numbers: Philox4x32-10 with key = seed and counter
+ * (attempt, w, 0, 0), w = 0 (molecule, x), 1 (y, z), 2 (acceptance); 53-bit
+ * uniforms from word pairs.  Replaces the reference's xorshift stream (same
+ * distribution, different numbers).  `accepted` may be NULL. */
+int lg_mc_run(lg_system* sys, uint64_t seed, uint64_t attempt0, uint64_t nattempts, double beta, double delta,
+              uint64_t* accepted);
+/* lg_mc_run for `count` independent chains (distinct systems of one
+ * context) in one launch, one workgroup per chain; per-chain arguments are
+ * arrays.  No counterpart in the reference, which runs one chain per
+ * process; this is the batched form of the same propagate loop. */
+int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* seeds, const uint64_t* attempt0,
+                    const uint64_t* nattempts, const double* beta, const double* delta, uint64_t* accepted);
 
 /* ---- kernel timing ---------------------------------------------------------------
  * No counterpart in the reference (its benches time whole calls with

@@ -6,8 +6,8 @@ lumol-core / lumol-sim names on top of it.
 """
 from .system import (  # noqa: F401
     ATOMIC_MASSES, BerendsenThermostat, Ewald, FOUR_PI_EPSILON_0, Harmonic, K_BOLTZMANN,
-    LeapFrog, LennardJones, MolecularDynamics, NullPotential, PairInteraction, PairRestriction,
-    SharedEwald, Simulation, System, UnitCell, VelocityVerlet, Wolf, units,
+    LeapFrog, LennardJones, MolecularDynamics, MonteCarlo, NullPotential, PairInteraction, PairRestriction,
+    SharedEwald, Simulation, System, Translate, UnitCell, VelocityVerlet, Wolf, mc_run_batch, units,
 )
 from ._native import LumolGpuError  # noqa: F401
 

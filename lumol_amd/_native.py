@@ -33,7 +33,8 @@ EXPORTS = [
     "lg_coulomb_ewald", "lg_coulomb_wolf", "lg_coulomb_clear", "lg_forces",
     "lg_potential_energy", "lg_virial", "lg_kinetic_energy", "lg_temperature",
     "lg_ewald_kvector_count", "lg_pairs_within", "lg_md_setup", "lg_md_run", "lg_cache_init", "lg_move_cost",
-    "lg_move_commit", "lg_move_discard", "lg_profile_begin", "lg_profile_end",
+    "lg_move_commit", "lg_move_discard", "lg_cache_energy", "lg_mc_run", "lg_mc_run_batch", "lg_profile_begin",
+    "lg_profile_end",
 ]
 
 LG_KERNEL_NLIST, LG_KERNEL_PAIRS, LG_KERNEL_MOLECULES = 0, 1, 2
@@ -69,6 +70,9 @@ U32 = ctypes.c_uint32
 U32P = ctypes.POINTER(ctypes.c_uint32)
 SZ = ctypes.c_size_t
 I = ctypes.c_int
+U64 = ctypes.c_uint64
+U64P = ctypes.POINTER(ctypes.c_uint64)
+PP = ctypes.POINTER(ctypes.c_void_p)
 
 
 def _declare(lib):
@@ -111,6 +115,9 @@ def _declare(lib):
         "lg_move_cost": (I, [P, U32, DP, DP]),
         "lg_move_commit": (I, [P, I]),
         "lg_move_discard": (I, [P]),
+        "lg_cache_energy": (I, [P, DP]),
+        "lg_mc_run": (I, [P, U64, U64, U64, D, D, U64P]),
+        "lg_mc_run_batch": (I, [PP, SZ, U64P, U64P, U64P, DP, DP, U64P]),
         "lg_profile_begin": (I, [P, U32]),
         "lg_profile_end": (I, [P, ctypes.POINTER(Profile)]),
     }

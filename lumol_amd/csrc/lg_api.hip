@@ -23,6 +23,7 @@
 
 #include "../../include/lumol_gpu.h"
 #include "lg_kernels.hpp"
+#include "lg_mc.hpp"
 
 using namespace lg;
 
@@ -114,6 +115,10 @@ struct lg_context {
     int device = 0;
     hipStream_t stream = nullptr;
     size_t max_shared = 65536;
+    // Monte Carlo launches: chain descriptors and per-chain results
+    DevBuf<McChain> mc_desc;
+    DevBuf<double> mc_res;
+    DevBuf<unsigned long long> mc_cnt;
 };
 
 struct lg_system {
@@ -158,7 +163,8 @@ struct lg_system {
     std::vector<PairParam> h_table;
     CoulParam coul{};
     // molecules
-    int nmol = 0, nterms = 0;
+    int nmol = 0, nterms = 0, max_mol = 1;
+    DevBuf<int> d_mol_start;     // nmol + 1
     DevBuf<Term> terms;          // intra-molecular terms, sorted by type
     DevBuf<int> slot_start;      // per-atom CSR over contribution slots (n + 1)
     DevBuf<double4> contrib;     // one force contribution per (term, atom)
@@ -184,6 +190,19 @@ struct lg_system {
     int integrator = LG_INTEGRATOR_VELOCITY_VERLET;
     double dt = 1.0, berendsen_T = 0.0, berendsen_tau = 1.0;
     bool md_ready = false;
+    // Monte Carlo energy cache (cache.rs:22-48): cached terms, rho(k) and
+    // the staged move; cell list of the device chain
+    bool mc_ready = false, mc_staged = false, mc_cells_valid = false;
+    double c_pairs = 0, c_tail = 0, c_bonds = 0, c_angles = 0, c_coul = 0;
+    uint32_t mc_mol = 0;
+    double mc_dp = 0, mc_dc = 0;
+    DevBuf<double2> mc_rho, mc_drho;
+    DevBuf<double4> mc_newpos;
+    DevBuf<double> mc_out;
+    int mc_nc[3] = {0, 0, 0};
+    double mc_csize[3] = {0, 0, 0};
+    int mc_ncell = 0, mc_ccap = 0;
+    DevBuf<int> mc_cell_atoms, mc_cell_count, mc_atom_cell, mc_atom_slot;
     // snapshot for neighbour-list overflow recovery
     DevBuf<double4> snap_pos, snap_vel, snap_acc;
     // kernel timing (lg_profile_*)
@@ -398,6 +417,9 @@ int prepare_topology(lg_system* s) {
         for (int q = 0; q < natoms_of(t); q++) sl[q] = next[at[q]]++;
     }
     s->nterms = int(h_terms.size());
+    s->max_mol = 1;
+    for (int m = 0; m < s->nmol; m++) s->max_mol = std::max(s->max_mol, s->mol_start[m + 1] - s->mol_start[m]);
+    LG_TRY(upload(s->d_mol_start, s->mol_start, st));
     LG_TRY(upload(s->terms, h_terms, st));
     LG_TRY(upload(s->slot_start, slot_start, st));
     LG_TRY(s->contrib.ensure(std::max<size_t>(1, size_t(slot_start[s->n]))));
@@ -635,6 +657,11 @@ double rlist2_of(lg_system* s) {
 
 int prepare(lg_system* s) {
     if (s->n == 0) return fail(LG_E_INVALID, "empty system");
+    if (s->topo_dirty || s->meta_dirty || s->params_dirty || s->kspace_dirty || s->nlist_dirty) {
+        s->mc_ready = false;
+        s->mc_staged = false;
+        s->mc_cells_valid = false;
+    }
     if (s->topo_dirty) LG_TRY(prepare_topology(s));
     if (s->meta_dirty) LG_TRY(prepare_meta(s));
     if (s->params_dirty) LG_TRY(prepare_params(s));
@@ -985,6 +1012,117 @@ int set_w(lg_system* s, DevBuf<double4>& b, const std::vector<double>& w) {
     return LG_OK;
 }
 
+// ---- Monte Carlo ---------------------------------------------------------------------
+// Sets the neighbour-list rebuild flag after positions changed on the device.
+int flag_rebuild(lg_system* s) {
+    if (s->nlist_dirty || !s->flags.p) return LG_OK;
+    k_set_flag<<<1, 1, 0, S(s)>>>(s->flags.p, 1);
+    LG_HIP(hipGetLastError());
+    return LG_OK;
+}
+
+double mc_total(const lg_system* s) { return s->c_pairs + s->c_tail + s->c_bonds + s->c_angles + s->c_coul; }
+
+bool mc_ewald(const lg_system* s) { return s->coul_kind == COUL_EWALD && s->nk > 0; }
+
+size_t mc_phase_bytes(const lg_system* s) {
+    if (!mc_ewald(s)) return 0;
+    return size_t(std::min(s->max_mol, kMcMaxMol)) * 2 * size_t(phase_rows(s->coul_kmax)) * sizeof(double2);
+}
+
+// EnergyCache::init (cache.rs:75-98): the energy terms of the current
+// configuration and, with Ewald, rho(k) (ewald.rs:222).
+int mc_init(lg_system* s) {
+    lg_energy_terms t{};
+    LG_TRY(lg_potential_energy(s, &t));
+    s->c_pairs = t.pairs;
+    s->c_tail = t.pairs_tail;
+    s->c_bonds = t.bonds;
+    s->c_angles = t.angles + t.dihedrals;
+    s->c_coul = t.coulomb;
+    hipStream_t st = S(s);
+    if (mc_ewald(s)) {
+        LG_TRY(s->mc_rho.ensure(s->nk));
+        LG_TRY(s->mc_drho.ensure(s->nk));
+        LG_HIP(hipMemcpyAsync(s->mc_rho.p, s->rho.p, s->nk * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    }
+    LG_TRY(s->mc_newpos.ensure(kMcMaxMol));
+    LG_TRY(s->mc_out.ensure(8));
+    LG_HIP(hipStreamSynchronize(st));
+    s->mc_ready = true;
+    s->mc_staged = false;
+    s->mc_cells_valid = false;
+    return LG_OK;
+}
+
+// Cell list of the device chain: cells of edge >= the largest cutoff, at
+// least 3 per axis (else the all-atoms loop); capacity 4x the mean
+// occupancy + 32, or every atom when `full`.
+int mc_prepare_cells(lg_system* s, bool full) {
+    s->mc_ncell = 0;
+    s->mc_cells_valid = false;
+    double rc = max_cutoff(s);
+    if (!s->periodic || rc <= 0.0) return LG_OK;
+    int nc[3];
+    for (int d = 0; d < 3; d++) {
+        nc[d] = int(std::floor(s->L[d] / (rc * (1.0 + 1e-12))));
+        if (nc[d] < 3) return LG_OK;
+        nc[d] = std::min(nc[d], 64);
+    }
+    int ncell = nc[0] * nc[1] * nc[2];
+    size_t mean = (s->n + ncell - 1) / ncell;
+    size_t cap = full ? s->n : std::min<size_t>(s->n, 4 * mean + 32);
+    if (!full && s->mc_ccap > int(cap) && s->mc_ncell == 0) cap = size_t(s->mc_ccap);
+    for (int d = 0; d < 3; d++) {
+        s->mc_nc[d] = nc[d];
+        s->mc_csize[d] = s->L[d] / nc[d];
+    }
+    LG_TRY(s->mc_cell_atoms.ensure(size_t(ncell) * cap));
+    LG_TRY(s->mc_cell_count.ensure(ncell));
+    LG_TRY(s->mc_atom_cell.ensure(s->n));
+    LG_TRY(s->mc_atom_slot.ensure(s->n));
+    s->mc_ncell = ncell;
+    s->mc_ccap = int(cap);
+    return LG_OK;
+}
+
+McChain mc_chain(lg_system* s, bool cells) {
+    McChain c{};
+    c.pos = s->pos.p;
+    c.vel = s->vel.p;
+    c.meta = s->meta.p;
+    c.mol_start = s->d_mol_start.p;
+    c.n = int(s->n);
+    c.nmol = s->nmol;
+    c.box = s->box;
+    c.table = s->table.p;
+    c.nkinds = int(s->nkinds);
+    c.coul = s->coul;
+    if (mc_ewald(s)) {
+        c.kmax = s->coul_kmax;
+        c.nk = s->nk;
+        c.kidx = s->kidx.p;
+        c.efac = s->efac.p;
+        c.rho = s->mc_rho.p;
+        c.drho = s->mc_drho.p;
+    }
+    if (cells && s->mc_ncell > 0) {
+        for (int d = 0; d < 3; d++) {
+            c.nc[d] = s->mc_nc[d];
+            c.csize[d] = s->mc_csize[d];
+        }
+        c.ncell = s->mc_ncell;
+        c.ccap = s->mc_ccap;
+        c.cell_atoms = s->mc_cell_atoms.p;
+        c.cell_count = s->mc_cell_count.p;
+        c.atom_cell = s->mc_atom_cell.p;
+        c.atom_slot = s->mc_atom_slot.p;
+    }
+    c.wrap = 1;
+    c.out = s->mc_out.p;
+    return c;
+}
+
 }  // namespace
 
 // ===================================================================================
@@ -1020,6 +1158,9 @@ int lg_context_destroy(lg_context* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
+    ctx->mc_desc.release();
+    ctx->mc_res.release();
+    ctx->mc_cnt.release();
     delete ctx;
     return LG_OK;
 }
@@ -1057,7 +1198,8 @@ int lg_system_destroy(lg_system* s) {
     (void)hipStreamSynchronize(s->ctx->stream);
     DevBuf<int>* ibufs[] = {&s->cellid, &s->slot, &s->count, &s->start, &s->tmp, &s->sorted, &s->nbr, &s->nnbr,
 
-                            &s->flags, &s->pair_restr, &s->slot_start};
+                            &s->flags, &s->pair_restr, &s->slot_start, &s->d_mol_start,
+                            &s->mc_cell_atoms, &s->mc_cell_count, &s->mc_atom_cell, &s->mc_atom_slot};
     for (auto* b : ibufs) b->release();
     s->phase_tab.release();
     s->phase_q.release();
@@ -1076,6 +1218,10 @@ int lg_system_destroy(lg_system* s) {
     s->table.release();
     s->terms.release();
     s->contrib.release();
+    s->mc_rho.release();
+    s->mc_drho.release();
+    s->mc_newpos.release();
+    s->mc_out.release();
     s->bparam.release();
     s->aparam.release();
     s->dparam.release();
@@ -1116,6 +1262,7 @@ int lg_system_set_positions(lg_system* s, const double* xyz) {
         if (!std::isfinite(xyz[i])) return fail(LG_E_NONFINITE, "non-finite position");
     LG_HIP(hipSetDevice(s->ctx->device));
     LG_TRY(push_vec3(s->pos, s->n, xyz, s->charge, S(s)));
+    s->mc_ready = s->mc_staged = s->mc_cells_valid = false;
     if (!s->nlist_dirty && s->flags.p) {
         int one = 1;
         LG_HIP(hipMemcpy(s->flags.p, &one, sizeof(int), hipMemcpyHostToDevice));
@@ -1155,6 +1302,7 @@ int lg_system_set_charges(lg_system* s, const double* q) {
     for (size_t i = 0; i < s->n; i++)
         if (!std::isfinite(q[i])) return fail(LG_E_NONFINITE, "non-finite charge");
     s->charge.assign(q, q + s->n);
+    s->mc_ready = s->mc_staged = false;
     LG_HIP(hipSetDevice(s->ctx->device));
     return set_w(s, s->pos, s->charge);
 }
@@ -1455,6 +1603,7 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
     if (!s->md_ready) return fail(LG_E_STATE, "lg_md_setup must be called before lg_md_run");
     LG_HIP(hipSetDevice(s->ctx->device));
     LG_TRY(prepare(s));
+    s->mc_ready = s->mc_staged = s->mc_cells_valid = false;
     hipStream_t st = S(s);
     int n = int(s->n);
     bool thermo = s->berendsen_T > 0.0;
@@ -1543,25 +1692,174 @@ int lg_profile_end(lg_system* s, lg_profile* out) {
 }
 
 int lg_cache_init(lg_system* s, double* energy) {
-    (void)s;
-    (void)energy;
-    return fail(LG_E_UNSUPPORTED, "Monte Carlo energy cache not built yet");
+    if (!s) return fail(LG_E_INVALID, "null system");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(mc_init(s));
+    if (energy) *energy = mc_total(s);
+    return LG_OK;
 }
+
+int lg_cache_energy(lg_system* s, double* energy) {
+    if (!s || !energy) return fail(LG_E_INVALID, "null argument");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(prepare(s));
+    if (!s->mc_ready) LG_TRY(mc_init(s));
+    *energy = mc_total(s);
+    return LG_OK;
+}
+
 int lg_move_cost(lg_system* s, uint32_t molid, const double* newpos, double* cost) {
-    (void)s;
-    (void)molid;
-    (void)newpos;
-    (void)cost;
-    return fail(LG_E_UNSUPPORTED, "Monte Carlo energy cache not built yet");
+    if (!s || !newpos || !cost) return fail(LG_E_INVALID, "null argument");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(prepare(s));
+    if (!s->mc_ready) LG_TRY(mc_init(s));
+    if (molid >= uint32_t(s->nmol)) return fail(LG_E_INVALID, "molecule id out of range");
+    int m0 = s->mol_start[molid], m = s->mol_start[molid + 1] - m0;
+    if (m > kMcMaxMol) return fail(LG_E_UNSUPPORTED, "molecule too large for a device Monte Carlo move");
+    std::vector<double4> np(m);
+    for (int a = 0; a < m; a++) {
+        for (int d = 0; d < 3; d++)
+            if (!std::isfinite(newpos[3 * a + d])) return fail(LG_E_NONFINITE, "non-finite position");
+        np[a] = make_double4(newpos[3 * a], newpos[3 * a + 1], newpos[3 * a + 2], s->charge[m0 + a]);
+    }
+    hipStream_t st = S(s);
+    LG_HIP(hipMemcpyAsync(s->mc_newpos.p, np.data(), m * sizeof(double4), hipMemcpyHostToDevice, st));
+    McChain c = mc_chain(s, false);
+    k_mc_cost<<<1, kMcBlock, mc_phase_bytes(s), st>>>(c, int(molid), s->mc_newpos.p);
+    LG_HIP(hipGetLastError());
+    double sums[6];
+    LG_HIP(hipMemcpyAsync(sums, s->mc_out.p + 2, sizeof(sums), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipStreamSynchronize(st));
+    double dp, dc;
+    mc_terms(sums, dp, dc);
+    s->mc_staged = true;
+    s->mc_mol = molid;
+    s->mc_dp = dp;
+    s->mc_dc = dc;
+    *cost = dp + dc;
+    return LG_OK;
 }
+
 int lg_move_commit(lg_system* s, int wrap) {
-    (void)s;
-    (void)wrap;
-    return fail(LG_E_UNSUPPORTED, "Monte Carlo energy cache not built yet");
+    if (!s) return fail(LG_E_INVALID, "null system");
+    if (!s->mc_staged) return fail(LG_E_STATE, "no staged move to commit");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = S(s);
+    McChain c = mc_chain(s, false);
+    c.wrap = wrap != 0;
+    k_mc_commit<<<1, kMcBlock, 0, st>>>(c, int(s->mc_mol), s->mc_newpos.p);
+    LG_HIP(hipGetLastError());
+    s->c_pairs += s->mc_dp;
+    s->c_coul += s->mc_dc;
+    s->mc_staged = false;
+    s->mc_cells_valid = false;
+    LG_TRY(flag_rebuild(s));
+    LG_HIP(hipStreamSynchronize(st));
+    return LG_OK;
 }
+
 int lg_move_discard(lg_system* s) {
-    (void)s;
-    return fail(LG_E_UNSUPPORTED, "Monte Carlo energy cache not built yet");
+    if (!s) return fail(LG_E_INVALID, "null system");
+    s->mc_staged = false;
+    return LG_OK;
+}
+
+int lg_mc_run(lg_system* s, uint64_t seed, uint64_t attempt0, uint64_t nattempts, double beta, double delta,
+              uint64_t* accepted) {
+    if (!s) return fail(LG_E_INVALID, "null system");
+    return lg_mc_run_batch(&s, 1, &seed, &attempt0, &nattempts, &beta, &delta, accepted);
+}
+
+int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* seeds, const uint64_t* attempt0,
+                    const uint64_t* nattempts, const double* beta, const double* delta, uint64_t* accepted) {
+    if (!systems || !seeds || !attempt0 || !nattempts || !beta || !delta) return fail(LG_E_INVALID, "null argument");
+    if (count == 0) return LG_OK;
+    if (count > (size_t(1) << 20)) return fail(LG_E_INVALID, "too many chains in one batch");
+    lg_context* ctx = systems[0] ? systems[0]->ctx : nullptr;
+    if (!ctx) return fail(LG_E_INVALID, "null system");
+    for (size_t b = 0; b < count; b++) {
+        if (!systems[b]) return fail(LG_E_INVALID, "null system");
+        if (systems[b]->ctx != ctx) return fail(LG_E_INVALID, "all chains of a batch must share one context");
+        for (size_t q = 0; q < b; q++)
+            if (systems[q] == systems[b]) return fail(LG_E_INVALID, "a system appears twice in one batch");
+        if (!(beta[b] >= 0.0) || !std::isfinite(beta[b]) || !(delta[b] >= 0.0) || !std::isfinite(delta[b]))
+            return fail(LG_E_INVALID, "invalid temperature or amplitude");
+    }
+    LG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    std::vector<uint64_t> next(count), end(count), acc(count, 0);
+    size_t shmem = 0;
+    for (size_t b = 0; b < count; b++) {
+        lg_system* s = systems[b];
+        LG_TRY(prepare(s));
+        if (!s->mc_ready) LG_TRY(mc_init(s));
+        if (s->max_mol > kMcMaxMol) return fail(LG_E_UNSUPPORTED, "molecule too large for a device Monte Carlo move");
+        if (!s->mc_cells_valid) LG_TRY(mc_prepare_cells(s, false));
+        shmem = std::max(shmem, mc_phase_bytes(s));
+        next[b] = attempt0[b];
+        end[b] = attempt0[b] + nattempts[b];
+    }
+    if (shmem + sizeof(McShared) > ctx->max_shared) return fail(LG_E_UNSUPPORTED, "kmax too large for the MC LDS tables");
+    LG_TRY(ctx->mc_desc.ensure(count));
+    LG_TRY(ctx->mc_res.ensure(count * 8));
+    LG_TRY(ctx->mc_cnt.ensure(count * 2));
+    std::vector<size_t> active(count);
+    std::iota(active.begin(), active.end(), size_t(0));
+    std::vector<McChain> desc;
+    std::vector<double> res;
+    std::vector<unsigned long long> cnt;
+    for (int round = 0; !active.empty(); round++) {
+        if (round > 8) return fail(LG_E_DEVICE, "Monte Carlo cell list overflow could not be resolved");
+        desc.clear();
+        for (size_t b : active) {
+            lg_system* s = systems[b];
+            McChain c = mc_chain(s, true);
+            c.key = make_uint2(uint32_t(seeds[b]), uint32_t(seeds[b] >> 32));
+            c.attempt0 = next[b];
+            c.nattempts = end[b] - next[b];
+            c.beta = beta[b];
+            c.d = delta[b] / std::sqrt(3.0);  // translate.rs:43
+            c.wrap = 1;
+            c.rebin = s->mc_cells_valid ? 0 : 1;
+            size_t k = desc.size();
+            c.out = ctx->mc_res.p + 8 * k;
+            c.counters = ctx->mc_cnt.p + 2 * k;
+            desc.push_back(c);
+        }
+        size_t na = desc.size();
+        LG_HIP(hipMemcpyAsync(ctx->mc_desc.p, desc.data(), na * sizeof(McChain), hipMemcpyHostToDevice, st));
+        LG_HIP(hipMemsetAsync(ctx->mc_res.p, 0, na * 8 * sizeof(double), st));
+        LG_HIP(hipMemsetAsync(ctx->mc_cnt.p, 0, na * 2 * sizeof(unsigned long long), st));
+        k_mc_chains<<<unsigned(na), kMcBlock, shmem, st>>>(ctx->mc_desc.p);
+        LG_HIP(hipGetLastError());
+        res.resize(na * 8);
+        cnt.resize(na * 2);
+        LG_HIP(hipMemcpyAsync(res.data(), ctx->mc_res.p, na * 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+        LG_HIP(hipMemcpyAsync(cnt.data(), ctx->mc_cnt.p, na * 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              st));
+        LG_HIP(hipStreamSynchronize(st));
+        std::vector<size_t> again;
+        for (size_t k = 0; k < na; k++) {
+            size_t b = active[k];
+            lg_system* s = systems[b];
+            s->c_pairs += res[8 * k];
+            s->c_coul += res[8 * k + 1];
+            acc[b] += cnt[2 * k];
+            next[b] = cnt[2 * k + 1];
+            s->mc_cells_valid = true;
+            if (next[b] < end[b]) {
+                // a cell would overflow: give every cell room for all atoms
+                LG_TRY(mc_prepare_cells(s, true));
+                again.push_back(b);
+            }
+            LG_TRY(flag_rebuild(s));
+        }
+        active.swap(again);
+    }
+    LG_HIP(hipStreamSynchronize(st));
+    if (accepted)
+        for (size_t b = 0; b < count; b++) accepted[b] = acc[b];
+    return LG_OK;
 }
 
 }  // extern "C"

@@ -534,6 +534,90 @@ class System:
         nat.check(nat.lib().lg_ewald_kvector_count(h, ctypes.byref(out)))
         return out.value
 
+    # -- Monte Carlo energy cache (lumol-core/src/sys/cache.rs) -------------------------
+    def molecules(self):
+        """(start, end) index range of every molecule: the connected
+        components of the bond graph, each a contiguous range of atoms."""
+        n = self.size()
+        parent = list(range(n))
+
+        def find(x):
+            while parent[x] != x:
+                parent[x] = parent[parent[x]]
+                x = parent[x]
+            return x
+
+        for i, j in self.bonds:
+            a, b = find(i), find(j)
+            if a != b:
+                parent[max(a, b)] = min(a, b)
+        ranges = []
+        for i in range(n):
+            if find(i) == i:
+                ranges.append([i, i + 1])
+            else:
+                ranges[-1][1] = i + 1
+        return [tuple(r) for r in ranges]
+
+    def cache_init(self):
+        """EnergyCache::init (cache.rs:75-98); returns the cached energy."""
+        e = ctypes.c_double()
+        nat.check(nat.lib().lg_cache_init(self._sync(), ctypes.byref(e)))
+        return e.value
+
+    def cache_energy(self):
+        """EnergyCache::energy (cache.rs:101-113)."""
+        e = ctypes.c_double()
+        nat.check(nat.lib().lg_cache_energy(self._sync(), ctypes.byref(e)))
+        return e.value
+
+    def move_cost(self, molid, newpos):
+        """EnergyCache::move_molecule_cost (cache.rs:145-220): energy change
+        of moving molecule `molid` to `newpos`; stages the move."""
+        newpos = np.ascontiguousarray(newpos, dtype=float).reshape(-1, 3)
+        c = ctypes.c_double()
+        nat.check(nat.lib().lg_move_cost(self._sync(), int(molid), nat.dptr(newpos), ctypes.byref(c)))
+        return c.value
+
+    def move_commit(self, wrap=True):
+        """EnergyCache::update (cache.rs:118-127) and the COM wrap of
+        Translate::apply (translate.rs:115-127)."""
+        nat.check(nat.lib().lg_move_commit(self._sync(), int(bool(wrap))))
+        self._device_state = True
+
+    def move_discard(self):
+        nat.check(nat.lib().lg_move_discard(self._sync()))
+
+    def mc_run(self, seed, attempt0, nattempts, beta, delta):
+        """`nattempts` Translate attempts of MonteCarlo::propagate on the
+        device (monte_carlo.rs:289-330); returns the accepted count."""
+        acc = ctypes.c_uint64()
+        nat.check(nat.lib().lg_mc_run(self._sync(), int(seed), int(attempt0), int(nattempts), float(beta),
+                                      float(delta), ctypes.byref(acc)))
+        self._device_state = True
+        return acc.value
+
+
+def mc_run_batch(systems, seeds, attempt0, nattempts, beta, delta):
+    """lg_mc_run for independent chains in one launch; per-chain arguments are
+    sequences (or scalars, broadcast).  Returns the accepted counts."""
+    count = len(systems)
+
+    def arr(x, dtype):
+        a = np.broadcast_to(np.asarray(x, dtype=dtype), (count,))
+        return np.ascontiguousarray(a)
+
+    handles = (ctypes.c_void_p * count)(*[s._sync().value for s in systems])
+    seeds, a0, na = arr(seeds, np.uint64), arr(attempt0, np.uint64), arr(nattempts, np.uint64)
+    beta, delta = arr(beta, float), arr(delta, float)
+    acc = np.zeros(count, dtype=np.uint64)
+    u64 = lambda a: a.ctypes.data_as(nat.U64P)
+    nat.check(nat.lib().lg_mc_run_batch(handles, count, u64(seeds), u64(a0), u64(na), nat.dptr(beta),
+                                        nat.dptr(delta), u64(acc)))
+    for s in systems:
+        s._device_state = True
+    return acc
+
 
 # ---------------------------------------------------------------------------------
 # molecular dynamics (lumol-sim)
@@ -577,8 +661,43 @@ class MolecularDynamics:
         self.thermostat = thermostat
 
 
+# ---------------------------------------------------------------------------------
+# Monte Carlo (lumol-sim/src/mc)
+# ---------------------------------------------------------------------------------
+class Translate:
+    """translate.rs:22-148: displaces one random molecule by U(+-delta/sqrt(3))
+    per axis."""
+
+    def __init__(self, delta):
+        if delta <= 0:
+            raise ValueError("delta must be positive in Translate move")
+        self.delta = float(delta)
+
+
+class MonteCarlo:
+    """monte_carlo.rs:17-330: Metropolis sampling at `temperature` (K).  The
+    device supports one Translate move; random numbers come from Philox4x32-10
+    keyed by `seed` (one counter per attempt)."""
+
+    def __init__(self, temperature, seed=0):
+        if temperature < 0:
+            raise ValueError("Monte Carlo temperature must be positive")
+        self.temperature = float(temperature)
+        self.beta = 1.0 / (K_BOLTZMANN * self.temperature)
+        self.seed = int(seed)
+        self.move = None
+        self.attempts = 0
+        self.accepted = 0
+
+    def add(self, move, frequency=1.0):
+        if not isinstance(move, Translate) or self.move is not None:
+            raise NotImplementedError("the device Monte Carlo runs a single Translate move")
+        self.move = move
+
+
 class Simulation:
-    """simulations.rs:54-159: runs `nsteps` propagator steps on the device."""
+    """simulations.rs:54-159: runs `nsteps` propagator steps on the device
+    (MD steps, or Monte Carlo attempts)."""
 
     def __init__(self, propagator):
         self.propagator = propagator
@@ -587,6 +706,10 @@ class Simulation:
     def setup(self, system):
         h = system._sync()
         md = self.propagator
+        if isinstance(md, MonteCarlo):
+            system.cache_init()
+            self._setup_for = (id(system), h.value)
+            return
         T, tau = (md.thermostat.temperature, md.thermostat.tau) if md.thermostat else (0.0, 1.0)
         nat.check(nat.lib().lg_md_setup(h, md.integrator.code, md.integrator.timestep, T, tau))
         self._setup_for = (id(system), h.value)
@@ -595,5 +718,12 @@ class Simulation:
         h = system._sync()
         if self._setup_for != (id(system), h.value):
             self.setup(system)
+        p = self.propagator
+        if isinstance(p, MonteCarlo):
+            if p.move is None:
+                raise ValueError("no move added to the Monte Carlo propagator")
+            p.accepted += system.mc_run(p.seed, p.attempts, int(nsteps), p.beta, p.move.delta)
+            p.attempts += int(nsteps)
+            return
         nat.check(nat.lib().lg_md_run(h, int(nsteps)))
         system._device_state = True

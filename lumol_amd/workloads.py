@@ -8,6 +8,8 @@ same bytes:
 * ``nacl``   -- rocksalt NaCl: Lorentz-Berthelot LJ + Ewald, Berendsen NVT.
 * ``spce``   -- flexible SPC/E water on a jittered cubic grid, random
   orientations, harmonic bonds/angles, inter-molecular LJ + Ewald.
+* ``lj_mc_chain`` -- one Monte Carlo chain of LJ argon at rho* = 0.8,
+  rc = 2.5 sigma, with its temperature (T* = 1.2) and amplitude (0.3 sigma).
 
 Masses and charges follow the configurations; energies are converted from
 kJ/mol and kcal/mol to lumol's internal units (lumol-core/src/units.rs).
@@ -17,8 +19,8 @@ import math
 import numpy as np
 
 from .system import (ATOMIC_MASSES, BerendsenThermostat, Ewald, Harmonic, K_BOLTZMANN, LennardJones,
-                     MolecularDynamics, NullPotential, PairInteraction, PairRestriction, Simulation, System,
-                     UnitCell, VelocityVerlet, units)
+                     MolecularDynamics, MonteCarlo, NullPotential, PairInteraction, PairRestriction, Simulation,
+                     System, Translate, UnitCell, VelocityVerlet, units)
 
 
 def read_xyz(path):
@@ -149,3 +151,33 @@ def md_nacl_nvt(system, dt=1.0, temperature=1200.0, tau=100.0):
 
 def md_nve(dt):
     return Simulation(MolecularDynamics(VelocityVerlet(dt)))
+
+
+LJ_SIGMA = 3.405
+LJ_EPSILON = units.from_(0.997, "kJ/mol")
+
+
+def lj_mc_chain(natoms, seed=4, rho_star=0.8, t_star=1.2, rc_sigma=2.5, delta_sigma=0.3, jitter=0.05):
+    """LJ fluid for single-particle Monte Carlo: `natoms` argon atoms on
+    randomly chosen sites of a simple cubic lattice at density rho*, jittered
+    by U(+-jitter sigma), plain LJ cut at rc_sigma sigma.  Returns
+    (system, beta, delta) with beta = 1 / (kB T* eps / kB) and delta in A."""
+    rng = np.random.default_rng(seed)
+    sigma = LJ_SIGMA
+    L = (natoms / rho_star) ** (1.0 / 3.0) * sigma
+    m = int(math.ceil(natoms ** (1.0 / 3.0) - 1e-9))
+    sites = np.array([[i, j, k] for i in range(m) for j in range(m) for k in range(m)], float)
+    sites = sites[np.sort(rng.permutation(len(sites))[:natoms])]
+    pos = (sites + 0.5) * (L / m) + rng.uniform(-jitter, jitter, size=(natoms, 3)) * sigma
+    s = System.from_arrays(["Ar"] * natoms, pos, UnitCell.cubic(L), masses=np.full(natoms, 39.948))
+    s.set_pair_potential(("Ar", "Ar"), PairInteraction(LennardJones(sigma=sigma, epsilon=LJ_EPSILON),
+                                                       rc_sigma * sigma))
+    temperature = t_star * LJ_EPSILON / K_BOLTZMANN
+    beta = 1.0 / (K_BOLTZMANN * temperature)
+    return s, beta, delta_sigma * sigma
+
+
+def mc_translate(temperature, delta, seed=0):
+    mc = MonteCarlo(temperature, seed=seed)
+    mc.add(Translate(delta))
+    return Simulation(mc)
