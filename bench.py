@@ -11,6 +11,11 @@ Workloads (BASELINE configurations; `--workload`):
 * ``spce``: flexible SPC/E water NVE -- 3000 molecules (9000 atoms), box
   44.8 A, Ewald rc 10 A kmax 10, inter-molecular LJ + Coulomb, harmonic
   bonds and angles, dt = 0.5 fs.
+* ``mc``: batched Monte Carlo -- independent LJ argon chains (sizes uniform
+  in 108..4000 atoms, rho* = 0.8, T* = 1.2, rc = 2.5 sigma, translation
+  amplitude 0.3 sigma, Philox per chain), run together on the device, one
+  workgroup per chain.  MC atom-steps = attempted single-particle moves
+  (N per sweep); `--sweeps` sweeps per chain (a bounded sample of the 10k).
 
 One step = one MD step on the device: integrator halves, neighbour-list
 maintenance, pair kernel (LJ + Ewald real space), intra-molecular terms,
@@ -57,7 +62,13 @@ WORKLOADS = {
             "Berendsen 1200 K, VV dt 1 fs",
     "spce": "flexible SPC/E water NVE, 3000 molecules (9000 atoms), box 44.8 A, LJ O-O + Ewald rc 10 A "
             "kmax 10 (inter-molecular), harmonic O-H/H-O-H, VV dt 0.5 fs",
+    "mc": "batched MC: LJ argon NVT chains, sizes U{108..4000} (seed 4), rho*=0.8, T*=1.2, rc=2.5 sigma, "
+          "translation 0.3 sigma, Philox4x32-10 per chain",
 }
+
+# MC move: LJ energy at the old and the new position for every neighbour
+# within rc: 2 x 29 FLOP per neighbour (geometry 22 + LJ energy 7)
+FLOP_MC_PAIR = 29
 
 
 def host_cores():
@@ -217,6 +228,113 @@ def run_form(workload, form, steps, warmup, oracle_forces, oracle_energy):
             "atoms": natoms, "pairs_within_rc": n_pairs, "kvectors": nk}
 
 
+def mc_chain_sizes(nchains, seed=4):
+    import numpy as np
+    return np.random.default_rng(seed).integers(108, 4001, size=nchains)
+
+
+def mc_cpu_baseline(sizes, target_seconds):
+    """The oracle's Monte Carlo chain (the reference's EnergyCache with its
+    N x N pair matrix, one move at a time) on a stratified sample: one chain
+    per size decile, each on its own host thread, for as many attempts as fit
+    in ~target_seconds; attempts per second summed over threads."""
+    import threading
+
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lumol_oracle as lo
+    from lumol_amd.workloads import lj_mc_chain
+
+    cores = host_cores()
+    picks = [int(np.quantile(sizes, q)) for q in np.linspace(0.05, 0.95, min(cores, 10))]
+    chains = []
+    for k, n in enumerate(picks):
+        sysm, beta, delta = lj_mc_chain(n, seed=1000 + k)
+        o = lo.OracleSystem(sysm, nthreads=1)
+        o.cache_init()
+        chains.append((o, beta, delta, n))
+    # calibrate on a short run of the first chain; a move costs O(N)
+    o0, beta0, delta0, n0 = chains[0]
+    t0 = time.perf_counter()
+    o0.mc_run(1, 0, 400, beta0, delta0)
+    rate0 = 400 / (time.perf_counter() - t0)
+    done = [0.0] * len(chains)
+
+    def work(k):
+        o, beta, delta, n = chains[k]
+        attempts = max(200, int(rate0 * target_seconds * n0 / n))
+        t0 = time.perf_counter()
+        o.mc_run(7 + k, 1000, attempts, beta, delta)
+        done[k] = (attempts, time.perf_counter() - t0)
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(len(chains))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    total = sum(a / dt for a, dt in done)
+    return {"value": total, "unit": "atom-steps/s", "cores": len(chains), "kind": "port",
+            "sample": f"oracle MC chains of {picks} atoms (one per thread, one chain per size decile), "
+                      f"~{target_seconds:.0f} s each; attempts/s summed"}
+
+
+def bench_mc(args):
+    """Batched MC chains: every chain runs `sweeps` sweeps (N attempts each);
+    launches give each active chain `slice` attempts, chains leave the batch
+    when done."""
+    from lumol_amd import mc_run_batch
+    from lumol_amd.workloads import lj_mc_chain
+
+    sizes = mc_chain_sizes(args.chains)
+    chains = [lj_mc_chain(int(n), seed=10000 + k) for k, n in enumerate(sizes)]
+    for s, _, _ in chains:
+        s.cache_init()
+    # warm-up: a short slice on every chain
+    mc_run_batch([c[0] for c in chains], list(range(len(chains))), 0, 16, [c[1] for c in chains],
+                 [c[2] for c in chains])
+    todo = [int(n) * args.sweeps for n in sizes]
+    done = [16] * len(chains)
+    target = [16 + t for t in todo]
+    accepted = [0] * len(chains)
+    launches = 0
+    t0 = time.perf_counter()
+    while True:
+        active = [k for k in range(len(chains)) if done[k] < target[k]]
+        if not active:
+            break
+        n_att = [min(args.slice, target[k] - done[k]) for k in active]
+        acc = mc_run_batch([chains[k][0] for k in active], list(active), [done[k] for k in active], n_att,
+                           [chains[k][1] for k in active], [chains[k][2] for k in active])
+        for k, a, na in zip(active, acc, n_att):
+            accepted[k] += int(a)
+            done[k] += na
+        launches += 1
+    elapsed = time.perf_counter() - t0
+    attempts = sum(todo)
+    # drift check of the running energies against a fresh evaluation
+    worst = 0.0
+    for s, _, _ in chains[:: max(1, len(chains) // 16)]:
+        e_cache, e_new = s.cache_energy(), s.potential_energy()
+        worst = max(worst, abs(e_cache - e_new) / abs(e_new))
+    acc_ratio = sum(accepted) / attempts
+    line = {
+        "metric": METRIC, "value": attempts / elapsed, "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": args.sweeps, "warmup": 16, "ms_per_step": elapsed * 1e3 / args.sweeps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded lattice + jitter)",
+        "config": {"workload": WORKLOADS["mc"], "chains": len(chains), "sweeps_per_chain": args.sweeps,
+                   "atoms_total": int(sizes.sum()), "attempts_per_launch_per_chain": args.slice,
+                   "launches": launches},
+        "acceptance_ratio": acc_ratio,
+        "max_rel_energy_cache_drift": worst,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = mc_cpu_baseline(sizes, args.cpu_seconds)
+    for s, _, _ in chains:
+        s.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,6 +343,9 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="nacl")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--chains", type=int, default=1024, help="mc: number of chains")
+    ap.add_argument("--sweeps", type=int, default=20, help="mc: sweeps per chain")
+    ap.add_argument("--slice", type=int, default=4096, help="mc: attempts per chain per launch")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB device copy (scale factor of the HBM byte counters)")
     args = ap.parse_args()
@@ -238,6 +359,9 @@ def main():
               "systems/jobs shard across GPUs without communication (see DESIGN.md). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "mc":
+        bench_mc(args)
+        return
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import lumol_oracle as lo

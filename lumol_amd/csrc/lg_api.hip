@@ -202,7 +202,9 @@ struct lg_system {
     int mc_nc[3] = {0, 0, 0};
     double mc_csize[3] = {0, 0, 0};
     int mc_ncell = 0, mc_ccap = 0;
-    DevBuf<int> mc_cell_atoms, mc_cell_count, mc_atom_cell, mc_atom_slot;
+    DevBuf<double4> mc_cell_pos;
+    DevBuf<int2> mc_cell_ids;
+    DevBuf<int> mc_cell_count, mc_atom_cell;
     // snapshot for neighbour-list overflow recovery
     DevBuf<double4> snap_pos, snap_vel, snap_acc;
     // kernel timing (lg_profile_*)
@@ -1056,7 +1058,7 @@ int mc_init(lg_system* s) {
 }
 
 // Cell list of the device chain: cells of edge >= the largest cutoff, at
-// least 3 per axis (else the all-atoms loop); capacity 4x the mean
+// least 4 per axis (else the all-atoms loop); capacity 4x the mean
 // occupancy + 32, or every atom when `full`.
 int mc_prepare_cells(lg_system* s, bool full) {
     s->mc_ncell = 0;
@@ -1066,21 +1068,25 @@ int mc_prepare_cells(lg_system* s, bool full) {
     int nc[3];
     for (int d = 0; d < 3; d++) {
         nc[d] = int(std::floor(s->L[d] / (rc * (1.0 + 1e-12))));
-        if (nc[d] < 3) return LG_OK;
-        nc[d] = std::min(nc[d], 64);
+        if (nc[d] < 4) return LG_OK;
+    }
+    // the counts live in LDS: coarsen the largest axis until they fit
+    while (nc[0] * nc[1] * nc[2] > kMcMaxCells) {
+        int d = nc[0] >= nc[1] && nc[0] >= nc[2] ? 0 : (nc[1] >= nc[2] ? 1 : 2);
+        nc[d]--;
     }
     int ncell = nc[0] * nc[1] * nc[2];
     size_t mean = (s->n + ncell - 1) / ncell;
     size_t cap = full ? s->n : std::min<size_t>(s->n, 4 * mean + 32);
-    if (!full && s->mc_ccap > int(cap) && s->mc_ncell == 0) cap = size_t(s->mc_ccap);
+    cap = std::max(cap, size_t(std::max(s->mc_ccap, 0)));  // keep a capacity grown earlier
     for (int d = 0; d < 3; d++) {
         s->mc_nc[d] = nc[d];
         s->mc_csize[d] = s->L[d] / nc[d];
     }
-    LG_TRY(s->mc_cell_atoms.ensure(size_t(ncell) * cap));
+    LG_TRY(s->mc_cell_pos.ensure(size_t(ncell) * cap));
+    LG_TRY(s->mc_cell_ids.ensure(size_t(ncell) * cap));
     LG_TRY(s->mc_cell_count.ensure(ncell));
     LG_TRY(s->mc_atom_cell.ensure(s->n));
-    LG_TRY(s->mc_atom_slot.ensure(s->n));
     s->mc_ncell = ncell;
     s->mc_ccap = int(cap);
     return LG_OK;
@@ -1110,14 +1116,17 @@ McChain mc_chain(lg_system* s, bool cells) {
         for (int d = 0; d < 3; d++) {
             c.nc[d] = s->mc_nc[d];
             c.csize[d] = s->mc_csize[d];
+            c.inv_csize[d] = 1.0 / s->mc_csize[d];
         }
         c.ncell = s->mc_ncell;
         c.ccap = s->mc_ccap;
-        c.cell_atoms = s->mc_cell_atoms.p;
+        c.cell_pos = s->mc_cell_pos.p;
+        c.cell_ids = s->mc_cell_ids.p;
         c.cell_count = s->mc_cell_count.p;
         c.atom_cell = s->mc_atom_cell.p;
-        c.atom_slot = s->mc_atom_slot.p;
     }
+    double rc = max_cutoff(s);
+    c.r2_filter = rc * rc * (1.0 + 1e-6);
     c.wrap = 1;
     c.out = s->mc_out.p;
     return c;
@@ -1199,7 +1208,7 @@ int lg_system_destroy(lg_system* s) {
     DevBuf<int>* ibufs[] = {&s->cellid, &s->slot, &s->count, &s->start, &s->tmp, &s->sorted, &s->nbr, &s->nnbr,
 
                             &s->flags, &s->pair_restr, &s->slot_start, &s->d_mol_start,
-                            &s->mc_cell_atoms, &s->mc_cell_count, &s->mc_atom_cell, &s->mc_atom_slot};
+                            &s->mc_cell_count, &s->mc_atom_cell};
     for (auto* b : ibufs) b->release();
     s->phase_tab.release();
     s->phase_q.release();
@@ -1222,6 +1231,8 @@ int lg_system_destroy(lg_system* s) {
     s->mc_drho.release();
     s->mc_newpos.release();
     s->mc_out.release();
+    s->mc_cell_pos.release();
+    s->mc_cell_ids.release();
     s->bparam.release();
     s->aparam.release();
     s->dparam.release();
@@ -1727,7 +1738,7 @@ int lg_move_cost(lg_system* s, uint32_t molid, const double* newpos, double* cos
     McChain c = mc_chain(s, false);
     k_mc_cost<<<1, kMcBlock, mc_phase_bytes(s), st>>>(c, int(molid), s->mc_newpos.p);
     LG_HIP(hipGetLastError());
-    double sums[6];
+    double sums[4];
     LG_HIP(hipMemcpyAsync(sums, s->mc_out.p + 2, sizeof(sums), hipMemcpyDeviceToHost, st));
     LG_HIP(hipStreamSynchronize(st));
     double dp, dc;
@@ -1811,6 +1822,11 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
     for (int round = 0; !active.empty(); round++) {
         if (round > 8) return fail(LG_E_DEVICE, "Monte Carlo cell list overflow could not be resolved");
         desc.clear();
+        // chains with a Coulomb potential first: they run the kCoul kernel
+        std::stable_partition(active.begin(), active.end(),
+                              [&](size_t b) { return systems[b]->coul_kind != COUL_NONE; });
+        size_t ncoul = 0;
+        for (size_t b : active) ncoul += systems[b]->coul_kind != COUL_NONE;
         for (size_t b : active) {
             lg_system* s = systems[b];
             McChain c = mc_chain(s, true);
@@ -1821,6 +1837,11 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
             c.d = delta[b] / std::sqrt(3.0);  // translate.rs:43
             c.wrap = 1;
             c.rebin = s->mc_cells_valid ? 0 : 1;
+            // the cell list assumes a move stays within the neighbouring cells
+            if (c.ncell > 0 && !(c.d < std::min(c.csize[0], std::min(c.csize[1], c.csize[2])))) {
+                c.ncell = 0;
+                s->mc_cells_valid = false;
+            }
             size_t k = desc.size();
             c.out = ctx->mc_res.p + 8 * k;
             c.counters = ctx->mc_cnt.p + 2 * k;
@@ -1830,7 +1851,8 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
         LG_HIP(hipMemcpyAsync(ctx->mc_desc.p, desc.data(), na * sizeof(McChain), hipMemcpyHostToDevice, st));
         LG_HIP(hipMemsetAsync(ctx->mc_res.p, 0, na * 8 * sizeof(double), st));
         LG_HIP(hipMemsetAsync(ctx->mc_cnt.p, 0, na * 2 * sizeof(unsigned long long), st));
-        k_mc_chains<<<unsigned(na), kMcBlock, shmem, st>>>(ctx->mc_desc.p);
+        if (ncoul > 0) k_mc_chains<true><<<unsigned(ncoul), kMcBlock, shmem, st>>>(ctx->mc_desc.p);
+        if (na > ncoul) k_mc_chains<false><<<unsigned(na - ncoul), kMcBlock, shmem, st>>>(ctx->mc_desc.p + ncoul);
         LG_HIP(hipGetLastError());
         res.resize(na * 8);
         cnt.resize(na * 2);
@@ -1846,7 +1868,7 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
             s->c_coul += res[8 * k + 1];
             acc[b] += cnt[2 * k];
             next[b] = cnt[2 * k + 1];
-            s->mc_cells_valid = true;
+            s->mc_cells_valid = desc[k].ncell > 0;
             if (next[b] < end[b]) {
                 // a cell would overflow: give every cell room for all atoms
                 LG_TRY(mc_prepare_cells(s, true));

@@ -88,9 +88,10 @@ def run_both(s, seed, attempts, beta, delta, attempt0=0):
     return o, acc_g, acc_o
 
 
-@pytest.mark.parametrize("natoms", [108, 500])
+@pytest.mark.parametrize("natoms", [108, 500, 1000])
 def test_mc_chain_matches_oracle(natoms):
-    # 108 atoms: fewer than 3 cells per axis (all-atoms loop); 500: cell list
+    # 108 and 500 atoms: fewer than 4 cells per axis (all-atoms loop); 1000:
+    # 4 cells per axis (cell list)
     s, beta, delta = lj_mc_chain(natoms, seed=natoms)
     o, acc_g, acc_o = run_both(s, seed=1234 + natoms, attempts=3000, beta=beta, delta=delta)
     assert acc_g == acc_o and 0 < acc_g < 3000
@@ -101,8 +102,8 @@ def test_mc_chain_matches_oracle(natoms):
 
 
 def test_mc_chain_water_ewald():
-    # 3 cells per axis: molecules and Ewald through the cell-list path
-    s = spce(nmol=216, L=18.64, cutoff=6.0, kmax=5)
+    # 4 cells per axis: molecules and Ewald through the cell-list path
+    s = spce(nmol=512, L=24.86, cutoff=6.0, kmax=5)
     beta = 1.0 / (lo_kb() * 300.0)
     o, acc_g, acc_o = run_both(s, seed=77, attempts=400, beta=beta, delta=0.5)
     assert acc_g == acc_o and acc_g > 0
@@ -117,8 +118,8 @@ def lo_kb():
 
 
 def test_mc_resume_equals_single_run():
-    a, beta, delta = lj_mc_chain(300, seed=3)
-    b, _, _ = lj_mc_chain(300, seed=3)
+    a, beta, delta = lj_mc_chain(1000, seed=3)
+    b, _, _ = lj_mc_chain(1000, seed=3)
     a.mc_run(9, 0, 2000, beta, delta)
     b.mc_run(9, 0, 1000, beta, delta)
     b.mc_run(9, 1000, 1000, beta, delta)
@@ -128,7 +129,7 @@ def test_mc_resume_equals_single_run():
 
 
 def test_mc_batch_matches_single_chains():
-    sizes = [108, 300, 500, 256]
+    sizes = [108, 1000, 500, 1200]
     chains = [lj_mc_chain(n, seed=20 + n) for n in sizes]
     systems = [c[0] for c in chains]
     oracles = []
@@ -143,13 +144,14 @@ def test_mc_batch_matches_single_chains():
 
 
 def test_mc_cell_overflow_regrows():
-    # 120 atoms packed in one corner of a 30 A box with 10 A cutoff: the
-    # first binning exceeds the default cell capacity and is redone with room
-    # for every atom; the chain must still follow the oracle exactly
+    # 120 atoms packed in one corner of a 41 A box with 10 A cutoff (4 cells
+    # per axis): the first binning exceeds the default cell capacity and is
+    # redone with room for every atom; the chain must still follow the
+    # oracle exactly
     rng = np.random.default_rng(5)
     n = 120
     pos = rng.uniform(0.0, 6.0, size=(n, 3))
-    s = System.from_arrays(["Ar"] * n, pos, UnitCell.cubic(30.0), masses=np.full(n, 39.948))
+    s = System.from_arrays(["Ar"] * n, pos, UnitCell.cubic(41.0), masses=np.full(n, 39.948))
     from lumol_amd import LennardJones, PairInteraction
     s.set_pair_potential(("Ar", "Ar"), PairInteraction(LennardJones(0.4, 1e-4), 10.0))
     o, acc_g, acc_o = run_both(s, seed=5, attempts=800, beta=1.0 / (lo_kb() * 300.0), delta=1.0)
