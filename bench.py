@@ -11,6 +11,21 @@ Workloads (BASELINE configurations; `--workload`):
 * ``spce``: flexible SPC/E water NVE -- 3000 molecules (9000 atoms), box
   44.8 A, Ewald rc 10 A kmax 10, inter-molecular LJ + Coulomb, harmonic
   bonds and angles, dt = 0.5 fs.
+* ``serve``: the job server under a Poisson stream of uneven jobs -- MD
+  jobs of LJ argon (64..8192 atoms) and SPC/E Ewald water (216..12000
+  molecules) in the mixed form, with 1k..20k steps scaled by
+  `--steps-scale`, plus Monte Carlo chains as in ``mc`` with `--sweeps`
+  sweeps; arrivals at a fixed rate sized to `--serve-seconds`.  Reports
+  p50/p99 job latency (submit -> result) per class and aggregate
+  atom-steps/s.  Host-side construction of each job's system happens before
+  its submission and is not timed.
+* ``ka``: one large system -- the 1,000,000-atom Kob-Andersen 80:20 LJ
+  mixture at rho* = 1.2, cutoff 2.5 sigma_AA, NVE VV dt = 1 fs, in the mixed
+  form (FP32 forces, compensated, FP64 accumulation and integration); the
+  FP64 form is timed beside it.  The force error is stated against the FP64
+  GPU form at this size (which meets the oracle at 1e-10 on smaller KA
+  systems); the CPU baseline times the oracle's O(N^2) force evaluation on a
+  16384-atom KA system and scales it by N^2.
 * ``mc``: batched Monte Carlo -- independent LJ argon chains (sizes uniform
   in 108..4000 atoms, rho* = 0.8, T* = 1.2, rc = 2.5 sigma, translation
   amplitude 0.3 sigma, Philox per chain), run together on the device, one
@@ -64,7 +79,13 @@ WORKLOADS = {
             "kmax 10 (inter-molecular), harmonic O-H/H-O-H, VV dt 0.5 fs",
     "mc": "batched MC: LJ argon NVT chains, sizes U{108..4000} (seed 4), rho*=0.8, T*=1.2, rc=2.5 sigma, "
           "translation 0.3 sigma, Philox4x32-10 per chain",
+    "ka": "Kob-Andersen 80:20 LJ binary mixture, 1,000,000 atoms, rho*=1.2, rc=2.5 sigma_AA, NVE VV dt 1 fs",
+    "serve": "job server: Poisson stream of MD jobs (LJ argon 64..8192 atoms, SPC/E Ewald 216..12000 molecules, "
+             "mixed form) and LJ MC chains (108..4000 atoms)",
 }
+
+# LJ-only pair, forces only: geometry 22, LJ 10, force vector and +/- accumulation 10
+FLOP_PAIR_LJ_F = 42
 
 # MC move: LJ energy at the old and the new position for every neighbour
 # within rc: 2 x 29 FLOP per neighbour (geometry 22 + LJ energy 7)
@@ -335,6 +356,150 @@ def bench_mc(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_ka(args):
+    """1M-atom Kob-Andersen, mixed and FP64 forms."""
+    import numpy as np
+
+    from lumol_amd import MolecularDynamics, Simulation, VelocityVerlet
+    from lumol_amd._native import LG_PRECISION_FP64, LG_PRECISION_MIXED
+    from lumol_amd.workloads import kob_andersen
+
+    forms = {}
+    f64 = None
+    for form in ("fp64", "mixed"):
+        s = kob_andersen(args.atoms)
+        s.set_precision(LG_PRECISION_MIXED if form == "mixed" else LG_PRECISION_FP64)
+        f = s.forces()
+        if form == "fp64":
+            f64 = f
+            n_pairs = s.pair_count_within(2.5 * 3.405)
+            err = None
+        else:
+            d = f - f64
+            err = {"force_rms_rel_vs_fp64_gpu": float(np.sqrt((d * d).sum() / (f64 * f64).sum()))}
+        sim = Simulation(MolecularDynamics(VelocityVerlet(1.0)))
+        s.profile_begin(["nlist", "pairs", "integrate"])
+        sim.run(s, args.warmup)
+        warm = s.profile_end()
+        s.profile_begin(["pairs"])
+        t0 = time.perf_counter()
+        sim.run(s, args.steps)
+        elapsed = time.perf_counter() - t0
+        ms, launches = s.profile_end()["pairs"]
+        avg = ms / 1e3 / max(launches, 1)
+        achieved = n_pairs * FLOP_PAIR_LJ_F / avg / 1e12
+        peak = PEAK_TFLOPS[form]
+        forms[form] = {"value": s.size() * args.steps / elapsed, "ms_per_step": elapsed * 1e3 / args.steps,
+                       "error": err,
+                       "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                                    "frac": achieved / peak, "traffic": load_traffic("ka", form, "pairs"),
+                                    "kernel": "pairs", "avg_launch_us": avg * 1e6,
+                                    "work": f"{FLOP_PAIR_LJ_F} FLOP x {n_pairs} pairs (r < 2.5 sigma) per launch"},
+                       "kernel_us_per_step": {c: round(warm[c][0] * 1e3 / max(args.warmup, 1), 2)
+                                              for c in warm if warm[c][1]}}
+        s.close()
+    m = forms["mixed"]
+    line = {
+        "metric": METRIC, "value": m["value"], "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded lattice + jitter)",
+        "config": {"workload": WORKLOADS["ka"], "atoms": args.atoms, "form": "mixed", "pairs_within_rc": n_pairs},
+        "error_vs_fp64": m["error"], "roofline": m["roofline"],
+        "forms": {"fp64": forms["fp64"], "mixed": {k: m[k] for k in ("value", "ms_per_step", "kernel_us_per_step")},
+                  "mixed_over_fp64": m["value"] / forms["fp64"]["value"]},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = ka_cpu_baseline(args.atoms)
+    print(json.dumps(line), flush=True)
+
+
+def ka_cpu_baseline(natoms, sample=16384):
+    """The oracle's O(N^2) force evaluation of a `sample`-atom KA system
+    (median of 3), scaled by (N / sample)^2 to the full size."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lumol_oracle as lo
+    from lumol_amd.workloads import kob_andersen
+    s = kob_andersen(sample)
+    cores = host_cores()
+    o = lo.OracleSystem(s, nthreads=cores)
+    o.forces()
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.forces()
+        times.append(time.perf_counter() - t0)
+    t = statistics.median(times) * (natoms / sample) ** 2
+    return {"value": natoms / t, "unit": "atom-steps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle force evaluation of a {sample}-atom KA system (median of 3, {cores} threads), "
+                      f"scaled by (N/{sample})^2 to {natoms} atoms (extrapolated)"}
+
+
+def bench_serve(args):
+    """Poisson stream of MD and MC jobs through lumol_amd.serving.Server."""
+    import threading
+
+    import numpy as np
+
+    from lumol_amd._native import LG_PRECISION_FP64, LG_PRECISION_MIXED
+    from lumol_amd.serving import Job, Server, latency_stats
+    from lumol_amd.workloads import lj_fluid, lj_mc_chain, water_box
+
+    rng = np.random.default_rng(5)
+    precision = LG_PRECISION_MIXED if args.form == "mixed" else LG_PRECISION_FP64
+    jobs = []
+    for k in range(args.md_jobs):
+        steps = max(1, int(rng.integers(1000, 20001) * args.steps_scale))
+        if rng.random() < 0.5:
+            n = int(rng.integers(64, 8193))
+            system = lj_fluid(n, seed=100 + k)
+            jobs.append(Job("md", (lambda s=system: s), steps, md={"dt": 1.0, "precision": precision},
+                            name=f"lj-{n}"))
+        else:
+            nmol = int(rng.integers(216, 12001))
+            system = water_box(nmol, seed=100 + k)
+            jobs.append(Job("md", (lambda s=system: s), steps, md={"dt": 0.5, "precision": precision},
+                            name=f"water-{nmol}"))
+    for k, n in enumerate(mc_chain_sizes(args.chains)):
+        system, beta, delta = lj_mc_chain(int(n), seed=20000 + k)
+        jobs.append(Job("mc", (lambda s=system: s), int(n) * args.sweeps,
+                        mc={"beta": beta, "delta": delta, "seed": 7000 + k}, name=f"mc-{n}"))
+    order = rng.permutation(len(jobs))
+    jobs = [jobs[i] for i in order]
+    gaps = rng.exponential(args.serve_seconds / len(jobs), size=len(jobs))
+    with Server(md_lanes=args.md_lanes, mc_slice=args.slice) as server:
+        t0 = time.perf_counter()
+
+        def feed():
+            for j, g in zip(jobs, gaps):
+                time.sleep(g)
+                server.submit(j)
+
+        feeder = threading.Thread(target=feed)
+        feeder.start()
+        feeder.join()
+        server.drain()
+        elapsed = time.perf_counter() - t0
+        launches = server.mc_launches
+    errors = [(j.name, repr(j.error)) for j in jobs if j.error is not None]
+    md = latency_stats([j for j in jobs if j.kind == "md"])
+    mc = latency_stats([j for j in jobs if j.kind == "mc"])
+    total = latency_stats(jobs)
+    line = {
+        "metric": METRIC, "value": total["atom_steps_per_s"], "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": len(jobs), "warmup": 0, "ms_per_step": elapsed * 1e3 / len(jobs),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if args.form == "mixed" else "f64", "data": "synthetic (seeded lattices + jitter)",
+        "config": {"workload": WORKLOADS["serve"], "md_jobs": args.md_jobs, "mc_chains": args.chains,
+                   "md_steps": "U{1000..20000} x %g" % args.steps_scale, "mc_sweeps": args.sweeps,
+                   "md_lanes": args.md_lanes, "mc_slice": args.slice,
+                   "mean_interarrival_s": args.serve_seconds / len(jobs), "form": args.form},
+        "latency": {"all": total, "md": md, "mc": mc},
+        "mc_launches": launches,
+        "errors": errors[:5],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -346,6 +511,12 @@ def main():
     ap.add_argument("--chains", type=int, default=1024, help="mc: number of chains")
     ap.add_argument("--sweeps", type=int, default=20, help="mc: sweeps per chain")
     ap.add_argument("--slice", type=int, default=4096, help="mc: attempts per chain per launch")
+    ap.add_argument("--atoms", type=int, default=1_000_000, help="ka: atoms")
+    ap.add_argument("--md-jobs", type=int, default=256, help="serve: MD jobs")
+    ap.add_argument("--md-lanes", type=int, default=4, help="serve: concurrent MD jobs (HIP streams)")
+    ap.add_argument("--steps-scale", type=float, default=0.02, help="serve: factor on the 1k..20k MD steps")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="serve: span of the arrivals")
+    ap.add_argument("--form", choices=["fp64", "mixed"], default="mixed", help="serve: MD arithmetic form")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB device copy (scale factor of the HBM byte counters)")
     args = ap.parse_args()
@@ -361,6 +532,12 @@ def main():
         sys.exit(2)
     if args.workload == "mc":
         bench_mc(args)
+        return
+    if args.workload == "serve":
+        bench_serve(args)
+        return
+    if args.workload == "ka":
+        bench_ka(args)
         return
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -229,6 +229,25 @@ int lg_mc_run(lg_system* sys, uint64_t seed, uint64_t attempt0, uint64_t nattemp
 int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* seeds, const uint64_t* attempt0,
                     const uint64_t* nattempts, const double* beta, const double* delta, uint64_t* accepted);
 
+/* ---- launch tuning --------------------------------------------------------------------
+ * No counterpart in the reference (a CPU code has no launch shapes).  Knobs of
+ * the device kernels; 0 leaves the built-in size-based choice:
+ *   pair_lanes    lanes per atom in the pair and neighbour-list kernels (4..64)
+ *   kspace_split  row ranges of the k-space force kernel (1..32)
+ *   rho_kpt       k-vectors per thread of the rho(k) kernel (1, 2, 4)
+ * lumol_amd.tuning picks them per size bucket by timing the candidates.
+ * Results do not depend on them beyond summation order. */
+typedef struct lg_tuning {
+    int pair_lanes;
+    int kspace_split;
+    int rho_kpt;
+    int reserved;
+} lg_tuning;
+int lg_system_set_tuning(lg_system* sys, const lg_tuning* tuning);
+/* The values in effect (after the automatic choice); rho_kpt stays 0 when
+ * automatic (chosen per launch). */
+int lg_system_get_tuning(lg_system* sys, lg_tuning* tuning);
+
 /* ---- kernel timing ---------------------------------------------------------------
  * No counterpart in the reference (its benches time whole calls with
  * criterion, benches/argon.rs etc.).  Between begin and end, every launch of a kernel

@@ -29,7 +29,7 @@ EXPORTS = [
     "lg_system_set_positions", "lg_system_get_positions", "lg_system_set_velocities",
     "lg_system_get_velocities", "lg_system_set_masses", "lg_system_set_charges",
     "lg_system_set_kinds", "lg_system_set_bonds", "lg_system_molecule_count",
-    "lg_system_set_precision", "lg_pair_set", "lg_bond_set", "lg_angle_set", "lg_dihedral_set",
+    "lg_system_set_precision", "lg_system_set_tuning", "lg_system_get_tuning", "lg_pair_set", "lg_bond_set", "lg_angle_set", "lg_dihedral_set",
     "lg_coulomb_ewald", "lg_coulomb_wolf", "lg_coulomb_clear", "lg_forces",
     "lg_potential_energy", "lg_virial", "lg_kinetic_energy", "lg_temperature",
     "lg_ewald_kvector_count", "lg_pairs_within", "lg_md_setup", "lg_md_run", "lg_cache_init", "lg_move_cost",
@@ -44,6 +44,11 @@ KERNEL_CLASSES = ["nlist", "pairs", "molecules", "kspace_rho", "kspace_force", "
 
 class Profile(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_double * 6), ("launches", ctypes.c_uint64 * 6)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("pair_lanes", ctypes.c_int), ("kspace_split", ctypes.c_int), ("rho_kpt", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
 
 
 class EnergyTerms(ctypes.Structure):
@@ -95,6 +100,8 @@ def _declare(lib):
         "lg_system_set_bonds": (I, [P, SZ, U32P]),
         "lg_system_molecule_count": (I, [P, ctypes.POINTER(SZ)]),
         "lg_system_set_precision": (I, [P, I]),
+        "lg_system_set_tuning": (I, [P, ctypes.POINTER(Tuning)]),
+        "lg_system_get_tuning": (I, [P, ctypes.POINTER(Tuning)]),
         "lg_pair_set": (I, [P, U32, U32, I, DP, D, I, I, I, D]),
         "lg_bond_set": (I, [P, U32, U32, I, DP]),
         "lg_angle_set": (I, [P, U32, U32, U32, I, DP]),
@@ -157,16 +164,20 @@ def u32ptr(array):
 
 
 _contexts = {}
+_ctx_lock = threading.Lock()
 
 
-def context(device=0):
-    """One lg_context per device, created on first use and kept for the process."""
-    with _lock:
-        ctx = _contexts.get(device)
-    if ctx is None:
-        handle = P()
-        check(lib().lg_context_create(device, ctypes.byref(handle)))
-        with _lock:
-            _contexts[device] = handle
-        ctx = handle
+def context(device=0, slot=0):
+    """The lg_context for (device, slot), created on first use and kept for
+    the process.  Each context owns one HIP stream: systems on different
+    slots of a device run concurrently."""
+    key = (device, slot)
+    L = lib()
+    with _ctx_lock:
+        ctx = _contexts.get(key)
+        if ctx is None:
+            handle = P()
+            check(L.lg_context_create(device, ctypes.byref(handle)))
+            _contexts[key] = handle
+            ctx = handle
     return ctx

@@ -140,6 +140,7 @@ struct lg_system {
     int coul_kmax = 0, coul_restr = 0;
     double coul_s14 = 1.0;
     int precision = LG_PRECISION_FP64;
+    lg_tuning tune{};  // 0 = automatic
 
     // derived topology
     std::vector<int> molid, mol_start;
@@ -572,9 +573,11 @@ int prepare_kspace(lg_system* s) {
     while (s->force_width > 8 && rows_bytes * (s->force_width + 1) > maxs) s->force_width /= 2;
     if (rows_bytes * 9 > maxs || kmax > 24)
         return fail(LG_E_UNSUPPORTED, "kmax too large for the k-space LDS tables");
-    // enough waves to fill the device: split the rows when there are few atoms
-    int fwaves = blocks_for(s->n, 64);
-    s->ksplit = std::max(1, std::min(32, std::min(s->nrows, (2048 + fwaves - 1) / fwaves)));
+    // enough waves to fill the device (~4096: 64 atoms x 4 waves per block,
+    // grid.y row splits), with as few force partials as that allows
+    int fwaves = 4 * blocks_for(s->n, 64);
+    s->ksplit = std::max(1, std::min(32, std::min(s->nrows, (4096 + fwaves - 1) / fwaves)));
+    if (s->tune.kspace_split > 0) s->ksplit = std::min(s->tune.kspace_split, std::max(1, s->nrows));
     LG_TRY(s->fk.ensure(size_t(s->ksplit) * s->n));
     int nchunks = blocks_for(s->n, s->rho_width);
     LG_TRY(s->rho_partial.ensure(size_t(nchunks) * s->nk));
@@ -643,6 +646,7 @@ int prepare_nlist(lg_system* s) {
     LG_TRY(s->ref.ensure(n));
     // lanes per atom: enough waves to fill 256 CUs, fewer lanes for long lists
     s->tpa = n <= 8192 ? 64 : n <= 32768 ? 32 : n <= 131072 ? 16 : n <= 524288 ? 8 : 4;
+    if (s->tune.pair_lanes > 0) s->tpa = s->tune.pair_lanes;
     LG_TRY(s->epart_pairs.ensure(size_t(blocks_for(n * size_t(s->tpa), kPairBlock)) * 8));
     int init[4] = {1, 0, 0, 0};  // force a rebuild; clear overflow and block counters
     LG_HIP(hipMemcpyAsync(s->flags.p, init, sizeof(init), hipMemcpyHostToDevice, S(s)));
@@ -721,7 +725,7 @@ int launch_nlist(lg_system* s, bool check) {
     return LG_OK;
 }
 
-int launch_forces(lg_system* s, bool energy, bool combine = true) {
+int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_done = false) {
     hipStream_t st = S(s);
     int n = int(s->n);
     PairArgs pa;
@@ -799,12 +803,14 @@ int launch_forces(lg_system* s, bool energy, bool combine = true) {
         // keep ~512 blocks in flight
         int kpt = 1;
         while (kpt < 4 && size_t(blocks_for(s->nk, kRhoBlock * kpt * 2)) * nchunks >= 512) kpt *= 2;
+        if (s->tune.rho_kpt == 1 || s->tune.rho_kpt == 2 || s->tune.rho_kpt == 4) kpt = s->tune.rho_kpt;
         dim3 grid(blocks_for(s->nk, kRhoBlock * kpt), nchunks);
         size_t shm = rows_bytes * (w + 1) + w * sizeof(double);
         long pr = prof_start(s, LG_KERNEL_KSPACE_RHO);
         if (mixed) {
-            k_phase32<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
-                                                                          s->phase_tab32.p, s->phase_q32.p);
+            if (!phase_done)
+                k_phase32<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                                                                              s->phase_tab32.p, s->phase_q32.p);
             size_t shm32 = size_t(phase_rows(kmax)) * sizeof(float2) * (w + 1) + w * sizeof(float);
             switch (kpt) {
                 case 1: k_rho_partial32<1><<<grid, kRhoBlock, shm32, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld,
@@ -817,8 +823,9 @@ int launch_forces(lg_system* s, bool energy, bool combine = true) {
                                                                         kmax, w, s->kidx.p, s->nk, s->rho_partial.p);
             }
         } else {
-            k_phase<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
-                                                                        s->phase_tab.p, s->phase_q.p);
+            if (!phase_done)
+                k_phase<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                                                                            s->phase_tab.p, s->phase_q.p);
             switch (kpt) {
                 case 1: k_rho_partial<1><<<grid, kRhoBlock, shm, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
                                                                    s->kidx.p, s->nk, s->rho_partial.p);
@@ -1374,6 +1381,29 @@ int lg_system_set_precision(lg_system* s, int precision) {
     return LG_OK;
 }
 
+int lg_system_set_tuning(lg_system* s, const lg_tuning* t) {
+    if (!s || !t) return fail(LG_E_INVALID, "null argument");
+    int l = t->pair_lanes;
+    if (l != 0 && l != 4 && l != 8 && l != 16 && l != 32 && l != 64)
+        return fail(LG_E_INVALID, "pair_lanes must be 0, 4, 8, 16, 32 or 64");
+    if (t->kspace_split < 0 || t->kspace_split > 32) return fail(LG_E_INVALID, "kspace_split must be in 0..32");
+    if (t->rho_kpt != 0 && t->rho_kpt != 1 && t->rho_kpt != 2 && t->rho_kpt != 4)
+        return fail(LG_E_INVALID, "rho_kpt must be 0, 1, 2 or 4");
+    s->tune = *t;
+    s->kspace_dirty = s->nlist_dirty = true;
+    return LG_OK;
+}
+
+int lg_system_get_tuning(lg_system* s, lg_tuning* t) {
+    if (!s || !t) return fail(LG_E_INVALID, "null argument");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(prepare(s));
+    *t = s->tune;
+    t->pair_lanes = s->tpa;
+    t->kspace_split = (s->coul_kind == COUL_EWALD && s->nk > 0) ? s->ksplit : 0;
+    return LG_OK;
+}
+
 static int check_pot(int pot) {
     if (pot != LG_POT_NULL && pot != LG_POT_LJ && pot != LG_POT_HARMONIC)
         return fail(LG_E_UNSUPPORTED, "potential kind not in the GPU kernel set");
@@ -1632,18 +1662,32 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
         bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
         double hs = 0.5 * s->skin;
         int nb = blocks_for(n, 256);
+        // the first half-step also writes the phase table of the new positions
+        PhaseOut ph{};
+        if (kspace) {
+            ph.mode = s->precision == LG_PRECISION_MIXED ? 2 : 1;
+            ph.kmax = s->coul_kmax;
+            ph.ld = s->ld;
+            ph.box = s->box;
+            ph.tab = s->phase_tab.p;
+            ph.q = s->phase_q.p;
+            ph.tab32 = s->phase_tab32.p;
+            ph.q32 = s->phase_q32.p;
+        }
+        Thermo th{nullptr, nb, double(3 * s->n), s->berendsen_T, s->berendsen_tau};
         for (uint64_t k = 0; k < todo; k++) {
-            const double* lam = (thermo && k > 0) ? s->lambda.p : nullptr;
+            // the Berendsen scale of the previous step is applied by the first half
+            th.ke_part = (thermo && k > 0) ? s->ke_part.p : nullptr;
             long p1 = prof_start(s, LG_KERNEL_INTEGRATE);
             if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
-                k_vv_first<<<nb, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, lam, s->ref.p, hs * hs,
-                                               s->flags.p);
+                k_vv_first<<<nb, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
+                                               s->flags.p, ph);
             else
-                k_lf_first<<<nb, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, lam, s->ref.p, hs * hs,
-                                               s->flags.p);
+                k_lf_first<<<nb, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
+                                               s->flags.p, ph);
             prof_stop(s, p1);
             LG_TRY(launch_nlist(s, false));
-            LG_TRY(launch_forces(s, false, false));
+            LG_TRY(launch_forces(s, false, false, true));
             long p2 = prof_start(s, LG_KERNEL_INTEGRATE);
             double* kp = thermo ? s->ke_part.p : nullptr;
             int ks = kspace ? s->ksplit : 0;
@@ -1654,12 +1698,13 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
             else
                 k_md_second<true><<<nb, 256, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
                                                       s->vel.p, s->acc.p, n, s->dt, kp);
-            if (thermo)
-                k_berendsen<<<1, 256, 0, st>>>(s->ke_part.p, nb, double(3 * s->n), s->berendsen_T, s->berendsen_tau,
-                                               s->lambda.p);
             prof_stop(s, p2);
         }
-        if (thermo) k_scale_vel<<<blocks_for(n, 256), 256, 0, st>>>(s->vel.p, n, s->lambda.p);
+        if (thermo) {
+            th.ke_part = s->ke_part.p;
+            k_berendsen<<<1, 256, 0, st>>>(th, s->lambda.p);
+            k_scale_vel<<<blocks_for(n, 256), 256, 0, st>>>(s->vel.p, n, s->lambda.p);
+        }
         LG_HIP(hipGetLastError());
         bool ov = false;
         LG_TRY(check_overflow(s, &ov));

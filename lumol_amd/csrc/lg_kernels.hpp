@@ -763,6 +763,65 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// Rows of atom column `a` along `axis`: e^{i n k_axis x} by the reference's
+// recursion; k_vector(e_axis) . r = (2 pi * inv_axis) * x  (cells.rs:224).
+__device__ __forceinline__ void phase_axis64(double x, double inv, int axis, int kmax, double2* __restrict__ tab,
+                                             int ld, int a) {
+    double phase = (2.0 * kPi * inv) * x;
+    double sn, cs;
+    sincos(phase, &sn, &cs);
+    double2 e1 = make_double2(cs, sn);
+    tab[(size_t)phase_row(axis, 0, kmax) * ld + a] = make_double2(1.0, 0.0);
+    tab[(size_t)phase_row(axis, 1, kmax) * ld + a] = e1;
+    if (axis != 0) tab[(size_t)phase_row(axis, -1, kmax) * ld + a] = make_double2(e1.x, -e1.y);
+    double2 e = e1;
+    for (int k = 2; k <= kmax; k++) {
+        e = make_double2(e.x * e1.x - e.y * e1.y, e.x * e1.y + e.y * e1.x);
+        tab[(size_t)phase_row(axis, k, kmax) * ld + a] = e;
+        if (axis != 0) tab[(size_t)phase_row(axis, -k, kmax) * ld + a] = make_double2(e.x, -e.y);
+    }
+}
+
+// Mixed form: each phase n k_axis x formed and reduced to [-pi, pi] in FP64,
+// then sincosf.
+__device__ __forceinline__ void phase_axis32(double x, double inv, int axis, int kmax, float2* __restrict__ tab,
+                                             int ld, int a) {
+    double phase = (2.0 * kPi * inv) * x;
+    const double two_pi = 2.0 * kPi, inv_two_pi = 1.0 / (2.0 * kPi);
+    tab[(size_t)phase_row(axis, 0, kmax) * ld + a] = make_float2(1.f, 0.f);
+    for (int k = 1; k <= kmax; k++) {
+        double th = k * phase;
+        th -= two_pi * rint(th * inv_two_pi);
+        float sn, cs;
+        sincosf((float)th, &sn, &cs);
+        tab[(size_t)phase_row(axis, k, kmax) * ld + a] = make_float2(cs, sn);
+        if (axis != 0) tab[(size_t)phase_row(axis, -k, kmax) * ld + a] = make_float2(cs, -sn);
+    }
+}
+
+// Where the MD step's first half writes the phase table of the new
+// positions (mode 0: no k-space, 1: FP64 table, 2: FP32 table).
+struct PhaseOut {
+    int mode, kmax, ld;
+    Box box;
+    double2* tab;
+    double* q;
+    float2* tab32;
+    float* q32;
+};
+
+__device__ __forceinline__ void phase_atom(const PhaseOut& ph, int a, bool live, double4 p) {
+    double xyz[3] = {live ? p.x : 0.0, live ? p.y : 0.0, live ? p.z : 0.0};
+    if (ph.mode == 1) {
+        for (int axis = 0; axis < 3; axis++) phase_axis64(xyz[axis], ph.box.inv[axis], axis, ph.kmax, ph.tab, ph.ld, a);
+        ph.q[a] = live ? p.w : 0.0;
+    } else {
+        for (int axis = 0; axis < 3; axis++)
+            phase_axis32(xyz[axis], ph.box.inv[axis], axis, ph.kmax, ph.tab32, ph.ld, a);
+        ph.q32[a] = live ? (float)p.w : 0.f;
+    }
+}
+
 // One thread per (atom, axis); tab[row * ld + atom], q[atom] (0 past n).
 __global__ void k_phase(const double4* __restrict__ pos, int n, int ld, Box b, int kmax, double2* __restrict__ tab,
                         double* __restrict__ q) {
@@ -777,20 +836,7 @@ __global__ void k_phase(const double4* __restrict__ pos, int n, int ld, Box b, i
         x = axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
         qa = p.w;
     }
-    // k_vector(e_axis) . r = (2 pi * inv_axis) * x  (cells.rs:224)
-    double phase = (2.0 * kPi * b.inv[axis]) * x;
-    double sn, cs;
-    sincos(phase, &sn, &cs);
-    double2 e1 = make_double2(cs, sn);
-    tab[(size_t)phase_row(axis, 0, kmax) * ld + a] = make_double2(1.0, 0.0);
-    tab[(size_t)phase_row(axis, 1, kmax) * ld + a] = e1;
-    if (axis != 0) tab[(size_t)phase_row(axis, -1, kmax) * ld + a] = make_double2(e1.x, -e1.y);
-    double2 e = e1;
-    for (int k = 2; k <= kmax; k++) {
-        e = make_double2(e.x * e1.x - e.y * e1.y, e.x * e1.y + e.y * e1.x);
-        tab[(size_t)phase_row(axis, k, kmax) * ld + a] = e;
-        if (axis != 0) tab[(size_t)phase_row(axis, -k, kmax) * ld + a] = make_double2(e.x, -e.y);
-    }
+    phase_axis64(x, b.inv[axis], axis, kmax, tab, ld, a);
     if (axis == 0) q[a] = live ? qa : 0.0;
 }
 
@@ -1003,17 +1049,7 @@ __global__ void k_phase32(const double4* __restrict__ pos, int n, int ld, Box b,
         x = axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
         qa = p.w;
     }
-    double phase = (2.0 * kPi * b.inv[axis]) * x;
-    const double two_pi = 2.0 * kPi, inv_two_pi = 1.0 / (2.0 * kPi);
-    tab[(size_t)phase_row(axis, 0, kmax) * ld + a] = make_float2(1.f, 0.f);
-    for (int k = 1; k <= kmax; k++) {
-        double th = k * phase;
-        th -= two_pi * rint(th * inv_two_pi);
-        float sn, cs;
-        sincosf((float)th, &sn, &cs);
-        tab[(size_t)phase_row(axis, k, kmax) * ld + a] = make_float2(cs, sn);
-        if (axis != 0) tab[(size_t)phase_row(axis, -k, kmax) * ld + a] = make_float2(cs, -sn);
-    }
+    phase_axis32(x, b.inv[axis], axis, kmax, tab, ld, a);
     if (axis == 0) q[a] = live ? (float)qa : 0.f;
 }
 
@@ -1120,18 +1156,48 @@ __device__ __forceinline__ bool moved_beyond(double4 p, const double4* __restric
     return dx * dx + dy * dy + dz * dz > half_skin2;
 }
 
-// VelocityVerlet first half (integrators.rs:47-53): optional pending
-// Berendsen scale of the previous step, then v += (dt/2) a; x += v dt; and
-// the displacement test of the neighbour list (sets flags[0]).
-__global__ void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel, const double4* __restrict__ acc,
-                           int n, double dt, const double* __restrict__ lambda, const double4* __restrict__ ref,
-                           double half_skin2, int* __restrict__ flags) {
+// Berendsen factor (thermostats.rs:113-121) from the kinetic energy
+// partials of the previous step, dof = 3N (system.rs:249-255): the block sums
+// the partials in a fixed order (thread-strided, waves, then wave order), so
+// every block of a launch, and k_berendsen, get the same bits.
+struct Thermo {
+    const double* ke_part;  // null: no pending scale
+    int nparts;
+    double dof, T0, tau;
+};
+
+__device__ __forceinline__ double berendsen_lambda(const Thermo& th) {
+    __shared__ double red[4];
+    __shared__ double lam;
+    double t = 0.0;
+    for (int p = threadIdx.x; p < th.nparts; p += blockDim.x) t += th.ke_part[p];
+    t = wave_sum(t);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double kin = red[0] + red[1] + red[2] + red[3];
+        double T = 2.0 * kin / (th.dof * kBoltzmann);
+        lam = sqrt(1.0 + (th.T0 / T - 1.0) / th.tau);
+    }
+    __syncthreads();
+    return lam;
+}
+
+// VelocityVerlet first half (integrators.rs:47-53): the pending Berendsen
+// scale of the previous step (th.ke_part != null), then v += (dt/2) a;
+// x += v dt; the displacement test of the neighbour list (sets flags[0]);
+// and the phase table of the new position when `ph.mode` != 0.  Block of 256.
+__global__ __launch_bounds__(256) void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel,
+                                                  const double4* __restrict__ acc, int n, double dt, Thermo th,
+                                                  const double4* __restrict__ ref, double half_skin2,
+                                                  int* __restrict__ flags, PhaseOut ph) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
+    double l = th.ke_part ? berendsen_lambda(th) : 1.0;
+    double4 p = make_double4(0, 0, 0, 0);
     if (i < n) {
         double4 v = vel[i];
-        if (lambda) {
-            double l = *lambda;
+        if (th.ke_part) {
             v.x = v.x * l;
             v.y = v.y * l;
             v.z = v.z * l;
@@ -1141,7 +1207,7 @@ __global__ void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel,
         v.x += h * a.x;
         v.y += h * a.y;
         v.z += h * a.z;
-        double4 p = pos[i];
+        p = pos[i];
         p.x += v.x * dt;
         p.y += v.y * dt;
         p.z += v.z * dt;
@@ -1150,19 +1216,23 @@ __global__ void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel,
         moved = moved_beyond(p, ref, i, half_skin2);
     }
     if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flags, 1);
+    if (ph.mode && i < ph.ld) phase_atom(ph, i, i < n, p);
 }
 
 // LeapFrog (integrators.rs:150-168), position part, after the pending
-// Berendsen scale of the previous step, with the displacement test.
-__global__ void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel, const double4* __restrict__ acc,
-                           int n, double dt, const double* __restrict__ lambda, const double4* __restrict__ ref,
-                           double half_skin2, int* __restrict__ flags) {
+// Berendsen scale of the previous step, with the displacement test and the
+// phase table as k_vv_first.  Block of 256.
+__global__ __launch_bounds__(256) void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel,
+                                                  const double4* __restrict__ acc, int n, double dt, Thermo th,
+                                                  const double4* __restrict__ ref, double half_skin2,
+                                                  int* __restrict__ flags, PhaseOut ph) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
+    double l = th.ke_part ? berendsen_lambda(th) : 1.0;
+    double4 p = make_double4(0, 0, 0, 0);
     if (i < n) {
         double4 v = vel[i];
-        if (lambda) {
-            double l = *lambda;
+        if (th.ke_part) {
             v.x = v.x * l;
             v.y = v.y * l;
             v.z = v.z * l;
@@ -1170,7 +1240,7 @@ __global__ void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel,
         }
         double4 a = acc[i];
         double dt2 = dt * dt;
-        double4 p = pos[i];
+        p = pos[i];
         p.x += v.x * dt + (0.5 * a.x) * dt2;
         p.y += v.y * dt + (0.5 * a.y) * dt2;
         p.z += v.z * dt + (0.5 * a.z) * dt2;
@@ -1178,6 +1248,7 @@ __global__ void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel,
         moved = moved_beyond(p, ref, i, half_skin2);
     }
     if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flags, 1);
+    if (ph.mode && i < ph.ld) phase_atom(ph, i, i < n, p);
 }
 
 // Second half of VelocityVerlet (integrators.rs:55-68) or LeapFrog
@@ -1222,21 +1293,11 @@ __global__ __launch_bounds__(256) void k_md_second(const double4* __restrict__ f
     if (threadIdx.x == 0) ke_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// Berendsen factor (thermostats.rs:113-121) from the kinetic energy
-// partials, dof = 3N (system.rs:249-255); one block, fixed summation order.
-__global__ __launch_bounds__(256) void k_berendsen(const double* __restrict__ ke_part, int nparts, double dof,
-                                                   double T0, double tau, double* __restrict__ lambda) {
-    __shared__ double red[4];
-    double t = 0.0;
-    for (int p = threadIdx.x; p < nparts; p += blockDim.x) t += ke_part[p];
-    t = wave_sum(t);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double kin = red[0] + red[1] + red[2] + red[3];
-        double T = 2.0 * kin / (dof * kBoltzmann);
-        *lambda = sqrt(1.0 + (T0 / T - 1.0) / tau);
-    }
+// Berendsen factor of the last step of a run into *lambda (one block, same
+// sum as berendsen_lambda).
+__global__ __launch_bounds__(256) void k_berendsen(Thermo th, double* __restrict__ lambda) {
+    double l = berendsen_lambda(th);
+    if (threadIdx.x == 0) *lambda = l;
 }
 
 __global__ void k_scale_vel(double4* __restrict__ vel, int n, const double* __restrict__ lambda) {

@@ -254,6 +254,7 @@ class System:
     def __init__(self, cell=None, device=0):
         self.cell = cell if cell is not None else UnitCell.infinite()
         self.device = device
+        self.stream_slot = 0  # library context (HIP stream) of the device
         self.names = []
         self._positions = np.zeros((0, 3))
         self._velocities = np.zeros((0, 3))
@@ -346,6 +347,20 @@ class System:
         self.precision = precision
         self._dirty = True
 
+    def set_tuning(self, pair_lanes=0, kspace_split=0, rho_kpt=0):
+        """Launch knobs of the device kernels (0 = built-in choice); see
+        lumol_amd.tuning.autotune."""
+        self.tuning_ = (int(pair_lanes), int(kspace_split), int(rho_kpt))
+        if self._handle is not None and not self._dirty:
+            t = nat.Tuning(*self.tuning_, 0)
+            nat.check(nat.lib().lg_system_set_tuning(self._handle, ctypes.byref(t)))
+
+    def tuning(self):
+        """The launch knobs in effect: dict(pair_lanes, kspace_split, rho_kpt)."""
+        t = nat.Tuning()
+        nat.check(nat.lib().lg_system_get_tuning(self._sync(), ctypes.byref(t)))
+        return {"pair_lanes": t.pair_lanes, "kspace_split": t.kspace_split, "rho_kpt": t.rho_kpt}
+
     # -- state -----------------------------------------------------------------
     @property
     def positions(self):
@@ -397,7 +412,7 @@ class System:
             self._handle = None
         if self._handle is None:
             h = ctypes.c_void_p()
-            nat.check(L.lg_system_create(nat.context(self.device), self.size(), ctypes.byref(h)))
+            nat.check(L.lg_system_create(nat.context(self.device, self.stream_slot), self.size(), ctypes.byref(h)))
             self._handle = h
             self._handle_size = self.size()
         h = self._handle
@@ -440,6 +455,8 @@ class System:
         else:
             nat.check(L.lg_coulomb_clear(h))
         nat.check(L.lg_system_set_precision(h, self.precision))
+        t = nat.Tuning(*getattr(self, "tuning_", (0, 0, 0)), 0)
+        nat.check(L.lg_system_set_tuning(h, ctypes.byref(t)))
         self._dirty = False
         self._device_state = True
         return h
@@ -514,6 +531,13 @@ class System:
         nat.check(nat.lib().lg_pairs_within(h, cutoff, nat.u32ptr(out), count.value, ctypes.byref(count)))
         order = np.lexsort((out[:, 1], out[:, 0]))
         return out[order]
+
+    def pair_count_within(self, cutoff):
+        """Number of inter-molecular pairs (i < j) with r < cutoff found by the
+        device neighbour search."""
+        count = ctypes.c_uint64()
+        nat.check(nat.lib().lg_pairs_within(self._sync(), cutoff, None, 0, ctypes.byref(count)))
+        return count.value
 
     def profile_begin(self, classes):
         """Time the given kernel classes (names of _native.KERNEL_CLASSES)

@@ -1,0 +1,81 @@
+"""Per-size-bucket autotuner of the device launch knobs.
+
+The knobs (``lg_tuning``): lanes per atom of the pair and neighbour-list
+kernels, row ranges of the k-space force kernel, k-vectors per thread of the
+rho(k) kernel.  Each knob drives one kernel class, so they are tuned one
+after the other: every candidate value is timed with the library's HIP-event
+profiler over `reps` force evaluations of the system itself, and the fastest
+is kept.  Results are cached per bucket -- (Ewald or not, precision, atoms
+rounded to a power of two, k-vectors rounded to a power of two) -- so a
+server tunes once per bucket and reuses the choice for every later job of
+that bucket.  Force evaluations leave the system's state unchanged.
+"""
+import math
+import threading
+
+from . import _native as nat
+
+PAIR_LANES = (8, 16, 32, 64)
+KSPACE_SPLITS = (2, 4, 8, 16, 32)
+RHO_KPT = (1, 2, 4)
+
+_cache = {}
+_lock = threading.Lock()
+
+
+def bucket(system):
+    n = system.size()
+    ewald = system.coulomb is not None and hasattr(system.coulomb, "kmax")
+    nk = system.ewald_kvector_count() if ewald else 0
+    p2 = lambda x: 0 if x <= 0 else 1 << int(math.ceil(math.log2(x)))
+    return (ewald, int(system.precision), p2(n), p2(nk))
+
+
+def _time_class(system, cls, reps):
+    system.profile_begin([cls])
+    for _ in range(reps):
+        system.forces()
+    prof = system.profile_end()
+    ms, launches = prof[cls]
+    return ms / max(launches, 1)
+
+
+def autotune(system, reps=5, use_cache=True):
+    """Picks and applies the knobs for `system`; returns them (dict)."""
+    key = bucket(system)
+    with _lock:
+        hit = _cache.get(key) if use_cache else None
+    if hit is None:
+        system.forces()  # builds the lists and tables once
+        best = {"pair_lanes": 0, "kspace_split": 0, "rho_kpt": 0}
+        # pair kernel lanes
+        times = {}
+        for lanes in PAIR_LANES:
+            system.set_tuning(lanes, best["kspace_split"], best["rho_kpt"])
+            system.forces()
+            times[lanes] = _time_class(system, "pairs", reps)
+        best["pair_lanes"] = min(times, key=times.get)
+        if key[0]:
+            times = {}
+            for split in KSPACE_SPLITS:
+                system.set_tuning(best["pair_lanes"], split, best["rho_kpt"])
+                system.forces()
+                times[split] = _time_class(system, "kspace_force", reps)
+            best["kspace_split"] = min(times, key=times.get)
+            times = {}
+            for kpt in RHO_KPT:
+                system.set_tuning(best["pair_lanes"], best["kspace_split"], kpt)
+                system.forces()
+                times[kpt] = _time_class(system, "kspace_rho", reps)
+            best["rho_kpt"] = min(times, key=times.get)
+        hit = best
+        with _lock:
+            _cache[key] = hit
+    system.set_tuning(hit["pair_lanes"], hit["kspace_split"], hit["rho_kpt"])
+    return dict(hit)
+
+
+def cached():
+    """The choices made so far, by bucket."""
+    with _lock:
+        return dict(_cache)

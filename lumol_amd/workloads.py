@@ -8,6 +8,9 @@ same bytes:
 * ``nacl``   -- rocksalt NaCl: Lorentz-Berthelot LJ + Ewald, Berendsen NVT.
 * ``spce``   -- flexible SPC/E water on a jittered cubic grid, random
   orientations, harmonic bonds/angles, inter-molecular LJ + Ewald.
+* ``kob_andersen`` -- the Kob-Andersen 80:20 binary LJ mixture at
+  rho* = 1.2 (sigma_AA = 3.405 A, eps_AA = 0.997 kJ/mol), one cutoff
+  2.5 sigma_AA for every pair.
 * ``lj_mc_chain`` -- one Monte Carlo chain of LJ argon at rho* = 0.8,
   rc = 2.5 sigma, with its temperature (T* = 1.2) and amplitude (0.3 sigma).
 
@@ -21,6 +24,10 @@ import numpy as np
 from .system import (ATOMIC_MASSES, BerendsenThermostat, Ewald, Harmonic, K_BOLTZMANN, LennardJones,
                      MolecularDynamics, MonteCarlo, NullPotential, PairInteraction, PairRestriction, Simulation,
                      System, Translate, UnitCell, VelocityVerlet, units)
+
+
+LJ_SIGMA = 3.405
+LJ_EPSILON = units.from_(0.997, "kJ/mol")
 
 
 def read_xyz(path):
@@ -114,15 +121,14 @@ def spce(nmol=3000, L=44.8, seed=3, temperature=300.0, cutoff=10.0, kmax=10, jit
     local = np.array([[0.0, 0.0, 0.0],
                       [1.0, 0.0, 0.0],
                       [math.cos(theta), math.sin(theta), 0.0]])
-    names, pos, bonds = [], [], []
-    for c in centres:
-        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
-        xyz = local @ q.T + c
-        base = len(names)
-        names += ["O", "H", "H"]
-        pos += list(xyz)
-        bonds += [(base, base + 1), (base, base + 2)]
-    pos = np.array(pos)
+    # one random orientation per molecule (stacked QR: the same numbers as
+    # one QR per molecule in order)
+    q = np.linalg.qr(rng.normal(size=(nmol, 3, 3)))[0]
+    pos = (np.matmul(local[None], np.transpose(q, (0, 2, 1))) + centres[:, None, :]).reshape(-1, 3)
+    names = ["O", "H", "H"] * nmol
+    base = 3 * np.arange(nmol)
+    bonds = [(int(b), int(b) + 1) for b in base] + [(int(b), int(b) + 2) for b in base]
+    bonds.sort()
     n = len(names)
     charges = np.array([-0.8476 if x == "O" else 0.4238 for x in names])
     masses = np.array([ATOMIC_MASSES[x] for x in names])
@@ -143,6 +149,60 @@ def spce(nmol=3000, L=44.8, seed=3, temperature=300.0, cutoff=10.0, kmax=10, jit
     return s
 
 
+def lj_fluid(natoms, seed=1, density=500 / 28.7 ** 3, temperature=120.0, cutoff=8.5, jitter=0.1):
+    """Argon fluid of any size at the density of the 500-atom FCC system:
+    `natoms` atoms on randomly chosen sites of a simple cubic lattice with
+    U(+-jitter) A noise, shifted LJ (sigma 3.405 A, eps 0.997 kJ/mol) cut at
+    min(cutoff, L/2 - 0.01), Maxwell-Boltzmann velocities."""
+    rng = np.random.default_rng(seed)
+    L = (natoms / density) ** (1.0 / 3.0)
+    m = int(math.ceil(natoms ** (1.0 / 3.0) - 1e-9))
+    sites = np.array([[i, j, k] for i in range(m) for j in range(m) for k in range(m)], float)
+    sites = sites[np.sort(rng.permutation(len(sites))[:natoms])]
+    pos = (sites + 0.5) * (L / m) + rng.uniform(-jitter, jitter, size=(natoms, 3))
+    masses = np.full(natoms, 39.948)
+    vel = maxwell_boltzmann(masses, temperature, rng)
+    s = System.from_arrays(["Ar"] * natoms, pos, UnitCell.cubic(L), masses=masses, velocities=vel)
+    rc = min(cutoff, 0.5 * L - 0.01)
+    s.set_pair_potential(("Ar", "Ar"), PairInteraction.shifted(LennardJones(sigma=LJ_SIGMA, epsilon=LJ_EPSILON), rc))
+    return s
+
+
+def kob_andersen(natoms=1_000_000, seed=6, rho_star=1.2, t_star=1.0, rc_sigma=2.5, jitter=0.05):
+    """Kob-Andersen binary mixture (W. Kob, H. C. Andersen, PRE 51, 4626
+    (1995)): 80 % A, 20 % B; eps_AB = 1.5, eps_BB = 0.5, sigma_AB = 0.8,
+    sigma_BB = 0.88 (units of A-A), all masses 39.948.  Atoms on randomly
+    chosen sites of a simple cubic lattice at rho*, jittered by
+    U(+-jitter sigma), kinds assigned at random (seeded); Maxwell-Boltzmann
+    velocities at T*; plain LJ cut at rc_sigma sigma_AA for every pair."""
+    rng = np.random.default_rng(seed)
+    sigma, eps = LJ_SIGMA, LJ_EPSILON
+    L = (natoms / rho_star) ** (1.0 / 3.0) * sigma
+    m = int(math.ceil(natoms ** (1.0 / 3.0) - 1e-9))
+    pick = np.sort(rng.permutation(m ** 3)[:natoms])
+    sites = np.stack([pick // (m * m), (pick // m) % m, pick % m], axis=1).astype(float)
+    pos = (sites + 0.5) * (L / m) + rng.uniform(-jitter, jitter, size=(natoms, 3)) * sigma
+    kinds = np.where(rng.permutation(natoms) < int(0.8 * natoms), "A", "B").tolist()
+    masses = np.full(natoms, 39.948)
+    temperature = t_star * eps / K_BOLTZMANN
+    vel = maxwell_boltzmann(masses, temperature, rng)
+    s = System.from_arrays(kinds, pos, UnitCell.cubic(L), masses=masses, velocities=vel)
+    rc = rc_sigma * sigma
+    for (a, b), (e, sg) in {("A", "A"): (1.0, 1.0), ("A", "B"): (1.5, 0.8), ("B", "B"): (0.5, 0.88)}.items():
+        s.set_pair_potential((a, b), PairInteraction(LennardJones(sigma=sg * sigma, epsilon=e * eps), rc))
+    return s
+
+
+def water_box(nmol, seed=3, cutoff=10.0):
+    """SPC/E water of `nmol` molecules at 1.0 g/cm^3 (0.03343 molecules/A^3),
+    cutoff min(cutoff, L/2 - 0.01), kmax scaled with the box (10 at 44.8 A,
+    at least 5)."""
+    L = (nmol / 0.033428) ** (1.0 / 3.0)
+    rc = min(cutoff, 0.5 * L - 0.01)
+    kmax = max(5, int(round(10 * L / 44.8)))
+    return spce(nmol=nmol, L=L, seed=seed, cutoff=rc, kmax=kmax)
+
+
 def md_nacl_nvt(system, dt=1.0, temperature=1200.0, tau=100.0):
     md = MolecularDynamics(VelocityVerlet(dt))
     md.set_thermostat(BerendsenThermostat(temperature, tau))
@@ -152,9 +212,6 @@ def md_nacl_nvt(system, dt=1.0, temperature=1200.0, tau=100.0):
 def md_nve(dt):
     return Simulation(MolecularDynamics(VelocityVerlet(dt)))
 
-
-LJ_SIGMA = 3.405
-LJ_EPSILON = units.from_(0.997, "kJ/mol")
 
 
 def lj_mc_chain(natoms, seed=4, rho_star=0.8, t_star=1.2, rc_sigma=2.5, delta_sigma=0.3, jitter=0.05):

@@ -1,0 +1,43 @@
+"""Kob-Andersen binary LJ mixture (the 1,000,000-atom configuration's
+model) at oracle-friendly sizes, and size-independent properties at a size
+that uses the same launch shapes as 1M atoms."""
+import numpy as np
+import pytest
+
+import lumol_oracle as lo
+from lumol_amd import MolecularDynamics, Simulation, VelocityVerlet
+from lumol_amd._native import LG_PRECISION_MIXED
+from lumol_amd.workloads import kob_andersen
+
+pytestmark = pytest.mark.gpu
+
+
+def test_ka_fp64_and_mixed_parity():
+    s = kob_andersen(2000)
+    o = lo.OracleSystem(s, nthreads=8)
+    fo, eo = o.forces(), o.potential_energy()
+    f = s.forces()
+    assert np.abs(f - fo).max() / np.abs(fo).max() < 1e-10
+    assert abs(s.potential_energy() - eo) < 1e-10 * abs(eo)
+    s.set_precision(LG_PRECISION_MIXED)
+    d = s.forces() - fo
+    assert np.sqrt((d * d).sum() / (fo * fo).sum()) < 1e-5
+    assert abs(s.potential_energy() - eo) < 1e-4 * abs(eo)
+
+
+def test_ka_large_momentum_and_drift():
+    # 131072 atoms: 4 lanes per atom as at 1M atoms
+    drift = {}
+    for form in (0, LG_PRECISION_MIXED):
+        s = kob_andersen(131072, seed=8)
+        s.set_precision(form)
+        sim = Simulation(MolecularDynamics(VelocityVerlet(1.0)))
+        sim.run(s, 50)  # leave the lattice start
+        f = s.forces()
+        assert np.abs(f.sum(0)).max() < 1e-6 * np.abs(f).max() * np.sqrt(len(f))
+        e0 = s.total_energy()
+        sim.run(s, 200)
+        drift[form] = abs(s.total_energy() - e0) / abs(e0)
+        s.close()
+    assert drift[0] < 1e-3
+    assert drift[LG_PRECISION_MIXED] <= 2.0 * drift[0] + 1e-6

@@ -1,0 +1,39 @@
+"""Launch knobs and the autotuner: every setting gives the FP64 parity bar
+against the oracle, and the autotuner's choice changes forces only by
+summation order."""
+import numpy as np
+import pytest
+
+import lumol_oracle as lo
+from lumol_amd import tuning
+from lumol_amd.workloads import nacl, spce
+
+pytestmark = pytest.mark.gpu
+
+
+def frel(f, g):
+    return np.abs(f - g).max() / np.abs(g).max()
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("split", [1, 3, 32])
+def test_every_knob_setting_meets_parity(lanes, split):
+    s = nacl(ncell=4, cutoff=10.0, kmax=7)
+    fo = lo.OracleSystem(s, nthreads=8).forces()
+    s.set_tuning(pair_lanes=lanes, kspace_split=split, rho_kpt=(1, 2, 4)[lanes % 3])
+    assert frel(s.forces(), fo) < 1e-10
+    t = s.tuning()
+    assert t["pair_lanes"] == lanes and t["kspace_split"] == split
+
+
+def test_autotune_choice_and_cache():
+    s = spce(nmol=300, L=20.8, cutoff=8.0, kmax=7)
+    f0 = s.forces()
+    best = tuning.autotune(s, reps=3)
+    assert best["pair_lanes"] in tuning.PAIR_LANES
+    assert best["kspace_split"] in tuning.KSPACE_SPLITS and best["rho_kpt"] in tuning.RHO_KPT
+    assert frel(s.forces(), f0) < 1e-12
+    # a second system of the same bucket reuses the choice
+    t = spce(nmol=300, L=20.8, cutoff=8.0, kmax=7, seed=9)
+    assert tuning.autotune(t) == best
+    assert tuning.bucket(t) in tuning.cached()

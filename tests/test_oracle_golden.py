@@ -390,3 +390,22 @@ def test_velocity_verlet_time_reversible():
     lo.lib().lo_set_velocities(o.h, np.ascontiguousarray(-v).ctypes.data_as(lo.DP))
     o.md_run(0, 0.25, 50)
     assert np.abs(o.positions() - p0).max() < 1e-9
+
+
+# --- Philox4x32-10 (the MC random numbers) ------------------------------------------------
+# Known-answer vectors published with the algorithm (Salmon, Moraes, Dror,
+# Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11; Random123
+# kat_vectors, philox4x32 with 10 rounds): (counter, key) -> output.
+PHILOX_KAT = [
+    ((0x00000000, 0x00000000, 0x00000000, 0x00000000), (0x00000000, 0x00000000),
+     (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff), (0xffffffff, 0xffffffff),
+     (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, out):
+    assert tuple(int(x) for x in lo.philox4x32_10(ctr, key)) == out
