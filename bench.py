@@ -19,6 +19,10 @@ Workloads (BASELINE configurations; `--workload`):
   p50/p99 job latency (submit -> result) per class and aggregate
   atom-steps/s.  Host-side construction of each job's system happens before
   its submission and is not timed.
+* ``spce-batched``: `--batch` copies of the SPC/E system (different seeds)
+  run at once, each on its own library context (HIP stream), in the mixed
+  form; the FP64 form of one system alone is timed in the same run, and the
+  line reports the aggregate mixed-batched rate and its ratio to it.
 * ``ka``: one large system -- the 1,000,000-atom Kob-Andersen 80:20 LJ
   mixture at rho* = 1.2, cutoff 2.5 sigma_AA, NVE VV dt = 1 fs, in the mixed
   form (FP32 forces, compensated, FP64 accumulation and integration); the
@@ -79,6 +83,8 @@ WORKLOADS = {
             "kmax 10 (inter-molecular), harmonic O-H/H-O-H, VV dt 0.5 fs",
     "mc": "batched MC: LJ argon NVT chains, sizes U{108..4000} (seed 4), rho*=0.8, T*=1.2, rc=2.5 sigma, "
           "translation 0.3 sigma, Philox4x32-10 per chain",
+    "spce-batched": "flexible SPC/E water NVE, 3000 molecules (9000 atoms) per system, several systems at once, "
+                    "Ewald rc 10 A kmax 10, VV dt 0.5 fs",
     "ka": "Kob-Andersen 80:20 LJ binary mixture, 1,000,000 atoms, rho*=1.2, rc=2.5 sigma_AA, NVE VV dt 1 fs",
     "serve": "job server: Poisson stream of MD jobs (LJ argon 64..8192 atoms, SPC/E Ewald 216..12000 molecules, "
              "mixed form) and LJ MC chains (108..4000 atoms)",
@@ -356,6 +362,72 @@ def bench_mc(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_spce_batched(args):
+    """Aggregate mixed-form throughput of `batch` SPC/E systems on separate
+    streams against one FP64 system alone."""
+    import threading
+
+    from lumol_amd._native import LG_PRECISION_FP64, LG_PRECISION_MIXED
+    from lumol_amd.workloads import spce
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lumol_oracle as lo
+    ref = spce()
+    o = lo.OracleSystem(ref, nthreads=host_cores())
+    fo, eo = o.forces(), o.potential_energy()
+    del o
+
+    def make(seed, precision, slot):
+        s = spce(seed=seed)
+        s.stream_slot = slot
+        s.set_precision(precision)
+        return s
+
+    # FP64, one system alone
+    single = make(3, LG_PRECISION_FP64, 0)
+    sim = make_sim("spce")
+    sim.run(single, args.warmup)
+    t0 = time.perf_counter()
+    sim.run(single, args.steps)
+    fp64_rate = single.size() * args.steps / (time.perf_counter() - t0)
+    single.close()
+    # mixed, `batch` systems at once
+    systems = [make(3 + k, LG_PRECISION_MIXED, 1 + k) for k in range(args.batch)]
+    err = force_error(systems[0], fo, eo)
+    sims = [make_sim("spce") for _ in systems]
+    for s, sm in zip(systems, sims):
+        sm.run(s, args.warmup)
+    barrier = threading.Barrier(len(systems) + 1)
+
+    def work(s, sm):
+        barrier.wait()
+        sm.run(s, args.steps)
+
+    threads = [threading.Thread(target=work, args=(s, sm)) for s, sm in zip(systems, sims)]
+    for t in threads:
+        t.start()
+    barrier.wait()
+    t0 = time.perf_counter()
+    for t in threads:
+        t.join()
+    elapsed = time.perf_counter() - t0
+    natoms = sum(s.size() for s in systems)
+    value = natoms * args.steps / elapsed
+    for s in systems:
+        s.close()
+    line = {
+        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded grid + jitter)",
+        "config": {"workload": WORKLOADS["spce-batched"], "batch": args.batch, "form": "mixed",
+                   "atoms_per_system": systems[0].size()},
+        "error_vs_fp64_oracle": err,
+        "fp64_single_system": fp64_rate,
+        "mixed_batched_over_fp64": value / fp64_rate,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_ka(args):
     """1M-atom Kob-Andersen, mixed and FP64 forms."""
     import numpy as np
@@ -512,6 +584,7 @@ def main():
     ap.add_argument("--sweeps", type=int, default=20, help="mc: sweeps per chain")
     ap.add_argument("--slice", type=int, default=4096, help="mc: attempts per chain per launch")
     ap.add_argument("--atoms", type=int, default=1_000_000, help="ka: atoms")
+    ap.add_argument("--batch", type=int, default=4, help="spce-batched: systems at once")
     ap.add_argument("--md-jobs", type=int, default=256, help="serve: MD jobs")
     ap.add_argument("--md-lanes", type=int, default=4, help="serve: concurrent MD jobs (HIP streams)")
     ap.add_argument("--steps-scale", type=float, default=0.02, help="serve: factor on the 1k..20k MD steps")
@@ -538,6 +611,9 @@ def main():
         return
     if args.workload == "ka":
         bench_ka(args)
+        return
+    if args.workload == "spce-batched":
+        bench_spce_batched(args)
         return
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))

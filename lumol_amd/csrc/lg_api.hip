@@ -115,9 +115,8 @@ struct lg_context {
     int device = 0;
     hipStream_t stream = nullptr;
     size_t max_shared = 65536;
-    // Monte Carlo launches: chain descriptors and per-chain results
+    // Monte Carlo launches: chain descriptors and per-chain counters
     DevBuf<McChain> mc_desc;
-    DevBuf<double> mc_res;
     DevBuf<unsigned long long> mc_cnt;
 };
 
@@ -157,6 +156,7 @@ struct lg_system {
     double rlist = 0, skin = 0;
     int cap = 0;
     DevBuf<int> cellid, slot, count, start, tmp, sorted, nbr, nnbr, flags;  // flags[0] rebuild, [1] overflow
+    DevBuf<float4> cpos;  // list-build candidate records in sorted order
     // parameters
     DevBuf<PairParam> table;
     DevBuf<int> pair_restr;
@@ -609,20 +609,34 @@ int prepare_nlist(lg_system* s) {
         b.L[d] = s->periodic ? s->L[d] : 0.0;
         b.inv[d] = s->periodic ? 1.0 / s->L[d] : 0.0;
     }
+    // cells of edge >= rlist / 2 searched 2 deep when 5 fit on an axis, else
+    // >= rlist searched 1 deep when 3 fit, else one cell (nearest image); a
+    // sparse box whose half cells would outnumber 8 per atom (and 2^22) uses
+    // full cells, at most 256 per axis
     Grid& g = s->grid;
-    for (int d = 0; d < 3; d++) {
-        int nc = 1;
-        if (s->periodic && s->rlist > 0) {
-            nc = int(std::floor(s->L[d] / (s->rlist * (1.0 + 1e-12))));
-            if (nc < 3) nc = 1;
-            nc = std::min(nc, 256);
+    auto layout = [&](bool half) {
+        for (int d = 0; d < 3; d++) {
+            int nc = 1, reach = 0;
+            if (s->periodic && s->rlist > 0) {
+                int nh = int(std::floor(s->L[d] / (0.5 * s->rlist * (1.0 + 1e-12))));
+                int nf = int(std::floor(s->L[d] / (s->rlist * (1.0 + 1e-12))));
+                if (half && nh >= 5) {
+                    nc = nh;
+                    reach = 2;
+                } else if (nf >= 3) {
+                    nc = std::min(nf, 256);
+                    reach = 1;
+                }
+            }
+            g.n[d] = nc;
+            g.reach[d] = reach;
+            g.size[d] = s->periodic ? s->L[d] / nc : 0.0;
         }
-        g.n[d] = nc;
-        g.size[d] = s->periodic ? s->L[d] / nc : 0.0;
-    }
+    };
+    layout(true);
+    double cells = double(g.n[0]) * g.n[1] * g.n[2];
+    if (cells > std::min(8.0 * double(n) + 4096.0, double(1 << 22))) layout(false);
     g.ncell = g.n[0] * g.n[1] * g.n[2];
-    double rl2 = s->rlist * s->rlist * (1.0 + 1e-9);
-    if (!s->periodic || s->rlist <= 0) rl2 = INFINITY;
     // capacity from the mean density, with margin; exact sizes are checked
     // after every build (overflow flag) and grown on demand.
     size_t capacity = n;
@@ -640,6 +654,7 @@ int prepare_nlist(lg_system* s) {
     LG_TRY(s->start.ensure(g.ncell + 1));
     LG_TRY(s->tmp.ensure(n));
     LG_TRY(s->sorted.ensure(n));
+    LG_TRY(s->cpos.ensure(n));
     LG_TRY(s->nbr.ensure(size_t(s->cap) * n));
     LG_TRY(s->nnbr.ensure(n));
     LG_TRY(s->flags.ensure(4));
@@ -652,14 +667,9 @@ int prepare_nlist(lg_system* s) {
     LG_HIP(hipMemcpyAsync(s->flags.p, init, sizeof(init), hipMemcpyHostToDevice, S(s)));
     LG_HIP(hipMemsetAsync(s->count.p, 0, g.ncell * sizeof(int), S(s)));
     s->nlist_dirty = false;
-    (void)rl2;
     return LG_OK;
 }
 
-double rlist2_of(lg_system* s) {
-    if (!s->periodic || s->rlist <= 0) return INFINITY;
-    return s->rlist * s->rlist * (1.0 + 1e-9);
-}
 
 int prepare(lg_system* s) {
     if (s->n == 0) return fail(LG_E_INVALID, "empty system");
@@ -701,15 +711,17 @@ int launch_nlist(lg_system* s, bool check) {
                                                       flag);
     k_cell_scan<<<1, 1024, 0, st>>>(s->count.p, s->start.p, g.ncell, flag);
     k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(s->cellid.p, s->slot.p, s->start.p, s->tmp.p, n, flag);
-    k_cell_rank<<<g.ncell, 256, 0, st>>>(s->tmp.p, s->start.p, s->sorted.p, g.ncell, flag);
+    k_cell_rank<<<g.ncell, 256, 0, st>>>(s->tmp.p, s->start.p, s->sorted.p, s->cpos.p, s->pos.p, s->meta.p,
+                                         s->box, g.ncell, flag);
     int nb = blocks_for(size_t(n) * s->tpa, 256);
-    double rl2 = rlist2_of(s);
+    // FP32 candidate test with a 1e-4 relative margin on rlist
+    float rl2f = (!s->periodic || s->rlist <= 0) ? INFINITY : float(s->rlist * (1.0 + 1e-4) * s->rlist * (1.0 + 1e-4));
     switch (s->tpa) {
 #define LG_BUILD_CASE(T)                                                                                   \
     case T:                                                                                                \
-        k_nlist_build<T><<<nb, 256, 0, st>>>(s->pos.p, s->meta.p, s->sorted.p, s->cellid.p, s->start.p,       \
-                                             s->count.p, g, s->box, rl2, n, s->cap, s->nbr.p, s->nnbr.p,     \
-                                             s->ref.p, flag);                                                 \
+        k_nlist_build<T><<<nb, 256, 0, st>>>(s->cpos.p, s->sorted.p, s->cellid.p, s->start.p, s->count.p, g,  \
+                                             s->box, rl2f, n, s->cap, s->nbr.p, s->nnbr.p, s->pos.p, s->ref.p,  \
+                                             flag);                                                           \
         break;
         LG_BUILD_CASE(4)
         LG_BUILD_CASE(8)
@@ -1030,7 +1042,15 @@ int flag_rebuild(lg_system* s) {
     return LG_OK;
 }
 
-double mc_total(const lg_system* s) { return s->c_pairs + s->c_tail + s->c_bonds + s->c_angles + s->c_coul; }
+// EnergyCache::energy: the pair and coulomb terms live on the device (updated
+// by every accepted move), the others are unchanged by translations.
+int mc_total(lg_system* s, double* e) {
+    double pc[2];
+    LG_HIP(hipMemcpyAsync(pc, s->mc_out.p, sizeof(pc), hipMemcpyDeviceToHost, S(s)));
+    LG_HIP(hipStreamSynchronize(S(s)));
+    *e = pc[0] + s->c_tail + s->c_bonds + s->c_angles + pc[1];
+    return LG_OK;
+}
 
 bool mc_ewald(const lg_system* s) { return s->coul_kind == COUL_EWALD && s->nk > 0; }
 
@@ -1057,6 +1077,8 @@ int mc_init(lg_system* s) {
     }
     LG_TRY(s->mc_newpos.ensure(kMcMaxMol));
     LG_TRY(s->mc_out.ensure(8));
+    double pc[2] = {s->c_pairs, s->c_coul};
+    LG_HIP(hipMemcpyAsync(s->mc_out.p, pc, sizeof(pc), hipMemcpyHostToDevice, st));
     LG_HIP(hipStreamSynchronize(st));
     s->mc_ready = true;
     s->mc_staged = false;
@@ -1175,7 +1197,6 @@ int lg_context_destroy(lg_context* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
     ctx->mc_desc.release();
-    ctx->mc_res.release();
     ctx->mc_cnt.release();
     delete ctx;
     return LG_OK;
@@ -1239,6 +1260,7 @@ int lg_system_destroy(lg_system* s) {
     s->mc_newpos.release();
     s->mc_out.release();
     s->mc_cell_pos.release();
+    s->cpos.release();
     s->mc_cell_ids.release();
     s->bparam.release();
     s->aparam.release();
@@ -1751,7 +1773,7 @@ int lg_cache_init(lg_system* s, double* energy) {
     if (!s) return fail(LG_E_INVALID, "null system");
     LG_HIP(hipSetDevice(s->ctx->device));
     LG_TRY(mc_init(s));
-    if (energy) *energy = mc_total(s);
+    if (energy) LG_TRY(mc_total(s, energy));
     return LG_OK;
 }
 
@@ -1760,8 +1782,7 @@ int lg_cache_energy(lg_system* s, double* energy) {
     LG_HIP(hipSetDevice(s->ctx->device));
     LG_TRY(prepare(s));
     if (!s->mc_ready) LG_TRY(mc_init(s));
-    *energy = mc_total(s);
-    return LG_OK;
+    return mc_total(s, energy);
 }
 
 int lg_move_cost(lg_system* s, uint32_t molid, const double* newpos, double* cost) {
@@ -1803,10 +1824,8 @@ int lg_move_commit(lg_system* s, int wrap) {
     hipStream_t st = S(s);
     McChain c = mc_chain(s, false);
     c.wrap = wrap != 0;
-    k_mc_commit<<<1, kMcBlock, 0, st>>>(c, int(s->mc_mol), s->mc_newpos.p);
+    k_mc_commit<<<1, kMcBlock, 0, st>>>(c, int(s->mc_mol), s->mc_newpos.p, s->mc_dp, s->mc_dc);
     LG_HIP(hipGetLastError());
-    s->c_pairs += s->mc_dp;
-    s->c_coul += s->mc_dc;
     s->mc_staged = false;
     s->mc_cells_valid = false;
     LG_TRY(flag_rebuild(s));
@@ -1857,12 +1876,10 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
     }
     if (shmem + sizeof(McShared) > ctx->max_shared) return fail(LG_E_UNSUPPORTED, "kmax too large for the MC LDS tables");
     LG_TRY(ctx->mc_desc.ensure(count));
-    LG_TRY(ctx->mc_res.ensure(count * 8));
     LG_TRY(ctx->mc_cnt.ensure(count * 2));
     std::vector<size_t> active(count);
     std::iota(active.begin(), active.end(), size_t(0));
     std::vector<McChain> desc;
-    std::vector<double> res;
     std::vector<unsigned long long> cnt;
     for (int round = 0; !active.empty(); round++) {
         if (round > 8) return fail(LG_E_DEVICE, "Monte Carlo cell list overflow could not be resolved");
@@ -1888,20 +1905,16 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
                 s->mc_cells_valid = false;
             }
             size_t k = desc.size();
-            c.out = ctx->mc_res.p + 8 * k;
             c.counters = ctx->mc_cnt.p + 2 * k;
             desc.push_back(c);
         }
         size_t na = desc.size();
         LG_HIP(hipMemcpyAsync(ctx->mc_desc.p, desc.data(), na * sizeof(McChain), hipMemcpyHostToDevice, st));
-        LG_HIP(hipMemsetAsync(ctx->mc_res.p, 0, na * 8 * sizeof(double), st));
         LG_HIP(hipMemsetAsync(ctx->mc_cnt.p, 0, na * 2 * sizeof(unsigned long long), st));
         if (ncoul > 0) k_mc_chains<true><<<unsigned(ncoul), kMcBlock, shmem, st>>>(ctx->mc_desc.p);
         if (na > ncoul) k_mc_chains<false><<<unsigned(na - ncoul), kMcBlock, shmem, st>>>(ctx->mc_desc.p + ncoul);
         LG_HIP(hipGetLastError());
-        res.resize(na * 8);
         cnt.resize(na * 2);
-        LG_HIP(hipMemcpyAsync(res.data(), ctx->mc_res.p, na * 8 * sizeof(double), hipMemcpyDeviceToHost, st));
         LG_HIP(hipMemcpyAsync(cnt.data(), ctx->mc_cnt.p, na * 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                               st));
         LG_HIP(hipStreamSynchronize(st));
@@ -1909,8 +1922,6 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
         for (size_t k = 0; k < na; k++) {
             size_t b = active[k];
             lg_system* s = systems[b];
-            s->c_pairs += res[8 * k];
-            s->c_coul += res[8 * k + 1];
             acc[b] += cnt[2 * k];
             next[b] = cnt[2 * k + 1];
             s->mc_cells_valid = desc[k].ncell > 0;

@@ -25,8 +25,9 @@ namespace lg {
 // Neighbour-list maintenance
 // ---------------------------------------------------------------------------------
 struct Grid {
-    int n[3];        // cells per dimension (1 when fewer than 3 fit)
-    double size[3];  // cell edge
+    int n[3];        // cells per dimension
+    double size[3];  // cell edge: >= rlist (reach 1), >= rlist / 2 (reach 2), or L (1 cell, reach 0)
+    int reach[3];    // neighbour cells searched on each side
     int ncell;
 };
 
@@ -44,9 +45,14 @@ __global__ void k_nlist_check(const double4* __restrict__ pos, const double4* __
     if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
 }
 
+// Coordinate wrapped into [0, L] (periodic) as the cell binning sees it.
+__device__ __forceinline__ double wrap_coord(double x, double L, int periodic) {
+    return periodic ? x - floor(x / L) * L : x;
+}
+
 __device__ __forceinline__ int cell_coord(double x, double L, int nc, double csize, int periodic) {
     if (nc == 1) return 0;
-    double w = periodic ? x - floor(x / L) * L : x;
+    double w = wrap_coord(x, L, periodic);
     int c = (int)floor(w / csize);
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
@@ -115,8 +121,11 @@ __global__ void k_cell_scatter(const int* __restrict__ cellid, const int* __rest
 
 // Deterministic order inside each cell: atoms by increasing index.  One block
 // per cell; an atom's rank is the number of smaller indices in its cell.
+// Also writes the candidate record of the list build in sorted order:
+// FP32 wrapped position (the binning's own wrap) and the molecule id.
 __global__ void k_cell_rank(const int* __restrict__ tmp, const int* __restrict__ start, int* __restrict__ sorted,
-                            int ncell, const int* __restrict__ flag) {
+                            float4* __restrict__ cpos, const double4* __restrict__ pos,
+                            const int2* __restrict__ meta, Box b, int ncell, const int* __restrict__ flag) {
     if (*flag == 0) return;
     int c = blockIdx.x;
     if (c >= ncell) return;
@@ -126,23 +135,35 @@ __global__ void k_cell_rank(const int* __restrict__ tmp, const int* __restrict__
         int rank = 0;
         for (int b2 = s; b2 < e; b2++) rank += tmp[b2] < v;
         sorted[s + rank] = v;
+        double4 p = pos[v];
+        cpos[s + rank] = make_float4((float)wrap_coord(p.x, b.L[0], b.periodic),
+                                     (float)wrap_coord(p.y, b.L[1], b.periodic),
+                                     (float)wrap_coord(p.z, b.L[2], b.periodic), __int_as_float(meta[v].y));
     }
 }
 
-// Verlet list: TPA lanes build the row of atom sorted[t] together.  Each
-// pass tests TPA consecutive candidates of a neighbour cell; accepted ones
-// are written in candidate order through a ballot prefix count, so the row
-// is the same whatever the lane timing.  Candidates are the atoms of the
-// neighbouring cells that belong to another molecule and lie within rlist.
-// The kernel also clears the per-cell counts for the next rebuild, and its
-// last block resets the rebuild flag (flags[0]) using flags[2] as counter.
+// Verlet list: TPA lanes build the row of atom sorted[t] together.  For each
+// (dz, dy) of the neighbour cells, the cells along x form at most three runs
+// of consecutive cell indices (split where the periodic wrap is crossed), and
+// a run's atoms are consecutive in sorted order, so each pass reads TPA
+// consecutive candidate records (FP32 wrapped position + molecule id, one
+// coalesced 16-byte load).  The image of a candidate is its cell's periodic
+// shift (reach 1 or 2), or the nearest image on axes with one cell.  A
+// candidate is kept when it belongs to another molecule and its FP32
+// distance is below rlist (1 + 1e-4): FP32 rounding of the wrapped
+// coordinates moves distances by far less than that margin, so the list
+// holds every pair within rlist; the force kernels apply the exact cutoff.
+// Accepted ones are written in candidate order through a ballot prefix
+// count, so the row is the same whatever the lane timing.  The kernel also
+// clears the per-cell counts for the next rebuild, and its last block resets
+// the rebuild flag (flags[0]) using flags[2] as counter.
 template <int TPA>
-__global__ __launch_bounds__(256) void k_nlist_build(const double4* __restrict__ pos, const int2* __restrict__ meta,
-                                                     const int* __restrict__ sorted, const int* __restrict__ cellid,
-                                                     const int* __restrict__ start, int* __restrict__ count_zero,
-                                                     Grid g, Box b, double rlist2, int n, int cap,
-                                                     int* __restrict__ nbr, int* __restrict__ nnbr,
-                                                     double4* __restrict__ ref, int* __restrict__ flags) {
+__global__ __launch_bounds__(256) void k_nlist_build(const float4* __restrict__ cpos, const int* __restrict__ sorted,
+                                                     const int* __restrict__ cellid, const int* __restrict__ start,
+                                                     int* __restrict__ count_zero, Grid g, Box b, float rl2f,
+                                                     int n, int cap, int* __restrict__ nbr, int* __restrict__ nnbr,
+                                                     const double4* __restrict__ pos, double4* __restrict__ ref,
+                                                     int* __restrict__ flags) {
     if (flags[0] == 0) return;
     __shared__ int last;
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -152,48 +173,82 @@ __global__ __launch_bounds__(256) void k_nlist_build(const double4* __restrict__
     int gbase = (threadIdx.x & 63) - sub;
     if (t < n) {
         int i = sorted[t];
-        double4 pi = pos[i];
-        int mi = meta[i].y;
+        float4 pi = cpos[t];
+        int mi = __float_as_int(pi.w);
         int c = cellid[i];
         int cx = c % g.n[0];
         int cy = (c / g.n[0]) % g.n[1];
         int cz = c / (g.n[0] * g.n[1]);
-        int rx = g.n[0] >= 3 ? 1 : 0, ry = g.n[1] >= 3 ? 1 : 0, rz = g.n[2] >= 3 ? 1 : 0;
+        const float Lx = (float)b.L[0], Ly = (float)b.L[1], Lz = (float)b.L[2];
+        const float ix = (float)b.inv[0], iy = (float)b.inv[1], iz = (float)b.inv[2];
+        // nearest image on axes searched as a single cell (periodic only)
+        const bool nx1 = b.periodic && g.reach[0] == 0, ny1 = b.periodic && g.reach[1] == 0,
+                   nz1 = b.periodic && g.reach[2] == 0;
         int count = 0;
         int* row = nbr + (size_t)t * cap;
-        for (int dz = -rz; dz <= rz; dz++) {
+        for (int dz = -g.reach[2]; dz <= g.reach[2]; dz++) {
             int z = cz + dz;
-            z = z < 0 ? z + g.n[2] : (z >= g.n[2] ? z - g.n[2] : z);
-            for (int dy = -ry; dy <= ry; dy++) {
+            float sz = 0.f;
+            if (z < 0) {
+                z += g.n[2];
+                sz = -Lz;
+            } else if (z >= g.n[2]) {
+                z -= g.n[2];
+                sz = Lz;
+            }
+            float ddz0 = pi.z - sz;
+            for (int dy = -g.reach[1]; dy <= g.reach[1]; dy++) {
                 int y = cy + dy;
-                y = y < 0 ? y + g.n[1] : (y >= g.n[1] ? y - g.n[1] : y);
-                for (int dx = -rx; dx <= rx; dx++) {
-                    int x = cx + dx;
-                    x = x < 0 ? x + g.n[0] : (x >= g.n[0] ? x - g.n[0] : x);
-                    int nc = (z * g.n[1] + y) * g.n[0] + x;
-                    int s0 = start[nc], e0 = start[nc + 1];
+                float sy = 0.f;
+                if (y < 0) {
+                    y += g.n[1];
+                    sy = -Ly;
+                } else if (y >= g.n[1]) {
+                    y -= g.n[1];
+                    sy = Ly;
+                }
+                float ddy0 = pi.y - sy;
+                int rowc = (z * g.n[1] + y) * g.n[0];
+                int x0 = cx - g.reach[0], x1 = cx + g.reach[0];
+                // runs: [x0, -1] shifted by -L, [0, nx-1] within, [nx, x1] shifted by +L
+                for (int run = 0; run < 3; run++) {
+                    int xa, xb;
+                    float sx;
+                    if (run == 0) {
+                        xa = x0 + g.n[0];
+                        xb = g.n[0] - 1;
+                        sx = -Lx;
+                        if (x0 >= 0) continue;
+                    } else if (run == 1) {
+                        xa = x0 < 0 ? 0 : x0;
+                        xb = x1 >= g.n[0] ? g.n[0] - 1 : x1;
+                        sx = 0.f;
+                    } else {
+                        xa = 0;
+                        xb = x1 - g.n[0];
+                        sx = Lx;
+                        if (x1 < g.n[0]) continue;
+                    }
+                    float ddx0 = pi.x - sx;
+                    int s0 = start[rowc + xa], e0 = start[rowc + xb + 1];
                     for (int q0 = s0; q0 < e0; q0 += TPA) {
                         int q = q0 + sub;
                         bool accept = false;
-                        int j = 0;
                         if (q < e0) {
-                            j = sorted[q];
-                            if (meta[j].y != mi) {
-                                double4 pj = pos[j];
-                                double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
-                                if (b.periodic) {
-                                    ddx -= rint(ddx * b.inv[0]) * b.L[0];
-                                    ddy -= rint(ddy * b.inv[1]) * b.L[1];
-                                    ddz -= rint(ddz * b.inv[2]) * b.L[2];
-                                }
-                                accept = ddx * ddx + ddy * ddy + ddz * ddz < rlist2;
+                            float4 cj = cpos[q];
+                            if (__float_as_int(cj.w) != mi) {
+                                float ddx = ddx0 - cj.x, ddy = ddy0 - cj.y, ddz = ddz0 - cj.z;
+                                if (nx1) ddx -= rintf(ddx * ix) * Lx;
+                                if (ny1) ddy -= rintf(ddy * iy) * Ly;
+                                if (nz1) ddz -= rintf(ddz * iz) * Lz;
+                                accept = ddx * ddx + ddy * ddy + ddz * ddz < rl2f;
                             }
                         }
                         unsigned long long bal = __ballot(accept);
                         unsigned long long gm =
                             TPA == 64 ? bal : ((bal >> gbase) & ((1ull << (TPA & 63)) - 1ull));
                         int slot = count + __popcll(gm & ((1ull << sub) - 1ull));
-                        if (accept && slot < cap) row[slot] = j;
+                        if (accept && slot < cap) row[slot] = sorted[q];
                         count += __popcll(gm);
                     }
                 }
@@ -205,7 +260,7 @@ __global__ __launch_bounds__(256) void k_nlist_build(const double4* __restrict__
                 count = cap;
             }
             nnbr[t] = count;
-            ref[i] = pi;
+            ref[i] = pos[i];
         }
     }
     __threadfence();

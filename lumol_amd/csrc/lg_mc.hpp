@@ -98,9 +98,11 @@ struct McChain {
     double beta, d;         // 1 / kT, per-axis half width delta / sqrt(3)
     int wrap;
     int rebin;              // rebuild the cell list before the first attempt
-    // results: out[0] delta pairs, out[1] delta coulomb, out[2..5] the four
-    // sums of the last cost; counters[0] accepted, counters[1] the attempt at
-    // which the run stopped (attempt0 + nattempts when complete)
+    // out[0], out[1]: the cached pair and coulomb energies, updated by every
+    // accepted move in attempt order (so a run gives the same bits however it
+    // is split into launches); out[2..5]: the four sums of the last cost.
+    // counters[0] accepted, counters[1] the attempt at which the run stopped
+    // (attempt0 + nattempts when complete)
     double* out;
     unsigned long long* counters;
 };
@@ -428,8 +430,9 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_cost(McChain c, int mol, const 
 }
 
 // Applies the staged move: positions (with the centre-of-mass wrap when
-// c.wrap) and rho += drho (one block).
-__global__ __launch_bounds__(kMcBlock) void k_mc_commit(McChain c, int mol, const double4* newpos) {
+// c.wrap), rho += drho and the cached energies += (dp, dc) (one block).
+__global__ __launch_bounds__(kMcBlock) void k_mc_commit(McChain c, int mol, const double4* newpos, double dp,
+                                                       double dc) {
     __shared__ McShared sh;
     int m0 = c.mol_start[mol], m = c.mol_start[mol + 1] - m0;
     if (threadIdx.x < m) sh.newp[threadIdx.x] = newpos[threadIdx.x];
@@ -439,6 +442,10 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_commit(McChain c, int mol, cons
     if (threadIdx.x < m) {
         double4 p = sh.newp[threadIdx.x];
         c.pos[m0 + threadIdx.x] = make_double4(p.x, p.y, p.z, c.pos[m0 + threadIdx.x].w);
+    }
+    if (threadIdx.x == 0) {
+        c.out[0] += dp;
+        c.out[1] += dc;
     }
     for (int k = threadIdx.x; k < c.nk; k += kMcBlock) {
         double2 r = c.rho[k], d = c.drho[k];
@@ -540,7 +547,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_chains(const McChain* __restric
         }
     }
     __syncthreads();
-    double d_pairs_total = 0.0, d_coul_total = 0.0;
+    double e_pairs = c.out[0], e_coul = c.out[1];  // running cached energies (used by thread 0)
     unsigned long long accepted = 0;
     unsigned long long t = c.attempt0, end = c.attempt0 + c.nattempts;
     if (!binned) end = t;
@@ -608,8 +615,8 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_chains(const McChain* __restric
                         if (cell != mc_cell_of(c, sh.oldp[a]) && cnt[cell] + m > c.ccap) ok = 0;
                     }
                 if (ok) {
-                    d_pairs_total += dp;
-                    d_coul_total += dc;
+                    e_pairs += dp;
+                    e_coul += dc;
                     accepted++;
 #pragma unroll 1
                     for (int a = 0; a < m; a++) {
@@ -639,8 +646,8 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_chains(const McChain* __restric
     if (cells && binned)
         for (int x = tid; x < c.ncell; x += kMcBlock) c.cell_count[x] = cnt[x];
     if (tid == 0) {
-        c.out[0] += d_pairs_total;
-        c.out[1] += d_coul_total;
+        c.out[0] = e_pairs;
+        c.out[1] = e_coul;
         c.counters[0] += accepted;
         c.counters[1] = t;
     }

@@ -26,8 +26,12 @@ def test_ka_fp64_and_mixed_parity():
 
 
 def test_ka_large_momentum_and_drift():
-    # 131072 atoms: 4 lanes per atom as at 1M atoms
-    drift = {}
+    # 131072 atoms: 4 lanes per atom as at 1M atoms.  The KA potential is
+    # plain LJ cut at 2.5 sigma (unshifted), so pairs crossing the cutoff
+    # change the total energy by V(rc) each: energy is not a constant of the
+    # model here, and the bound is relative to the FP64 form (mixed drift
+    # within 2x the FP64 drift, same final energy to 1e-4).
+    drift, final = {}, {}
     for form in (0, LG_PRECISION_MIXED):
         s = kob_andersen(131072, seed=8)
         s.set_precision(form)
@@ -37,7 +41,8 @@ def test_ka_large_momentum_and_drift():
         assert np.abs(f.sum(0)).max() < 1e-6 * np.abs(f).max() * np.sqrt(len(f))
         e0 = s.total_energy()
         sim.run(s, 200)
-        drift[form] = abs(s.total_energy() - e0) / abs(e0)
+        final[form] = s.total_energy()
+        drift[form] = abs(final[form] - e0) / abs(e0)
         s.close()
-    assert drift[0] < 1e-3
     assert drift[LG_PRECISION_MIXED] <= 2.0 * drift[0] + 1e-6
+    assert abs(final[LG_PRECISION_MIXED] - final[0]) <= 1e-4 * abs(final[0])
