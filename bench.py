@@ -211,6 +211,9 @@ def run_form(workload, form, steps, warmup, oracle_forces, oracle_energy):
     n_pairs = len(s.pairs_within(rc))
     nk = s.ewald_kvector_count()
     err = force_error(s, oracle_forces, oracle_energy)
+    # launch knobs for this size bucket (lumol_amd.tuning), chosen before timing
+    from lumol_amd import tuning
+    knobs = tuning.autotune(s, reps=5)
     sim = make_sim(workload)
 
     # warm-up, with a profiled pass over every kernel class to find the dominant one
@@ -252,7 +255,7 @@ def run_form(workload, form, steps, warmup, oracle_forces, oracle_energy):
     s.close()
     return {"value": natoms * steps / elapsed, "ms_per_step": elapsed * 1e3 / steps,
             "error_vs_fp64_oracle": err, "roofline": roofline, "kernel_us_per_step": kernels,
-            "atoms": natoms, "pairs_within_rc": n_pairs, "kvectors": nk}
+            "atoms": natoms, "pairs_within_rc": n_pairs, "kvectors": nk, "tuning": knobs}
 
 
 def mc_chain_sizes(nchains, seed=4):
@@ -444,7 +447,6 @@ def bench_ka(args):
         f = s.forces()
         if form == "fp64":
             f64 = f
-            n_pairs = s.pair_count_within(2.5 * 3.405)
             err = None
         else:
             d = f - f64
@@ -453,6 +455,9 @@ def bench_ka(args):
         s.profile_begin(["nlist", "pairs", "integrate"])
         sim.run(s, args.warmup)
         warm = s.profile_end()
+        if form == "fp64":
+            # pairs of the configuration being simulated (after the lattice start)
+            n_pairs = s.pair_count_within(2.5 * 3.405)
         s.profile_begin(["pairs"])
         t0 = time.perf_counter()
         sim.run(s, args.steps)
@@ -644,8 +649,8 @@ def main():
         "error_vs_fp64_oracle": m["error_vs_fp64_oracle"],
         "roofline": m["roofline"],
         "forms": {"fp64": {k: forms["fp64"][k] for k in ("value", "ms_per_step", "error_vs_fp64_oracle",
-                                                         "roofline", "kernel_us_per_step")},
-                  "mixed": {k: m[k] for k in ("value", "ms_per_step", "kernel_us_per_step")},
+                                                         "roofline", "kernel_us_per_step", "tuning")},
+                  "mixed": {k: m[k] for k in ("value", "ms_per_step", "kernel_us_per_step", "tuning")},
                   "mixed_over_fp64": m["value"] / forms["fp64"]["value"]},
     }
     if not args.no_cpu_baseline:

@@ -546,7 +546,7 @@ int prepare_kspace(lg_system* s) {
     for (int k = 0; k < s->nk; k++) {
         if (rows.empty() || rows.back().ikx != idx[k].x || rows.back().iky != idx[k].y ||
             idx[k - 1].z + 1 != idx[k].z)
-            rows.push_back(KRow{idx[k].x, idx[k].y, k, 0});
+            rows.push_back(KRow{short(idx[k].x), short(idx[k].y), short(idx[k].z), 0, k, 0});
         rows.back().count++;
     }
     s->nrows = int(rows.size());
@@ -816,7 +816,8 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         int kpt = 1;
         while (kpt < 4 && size_t(blocks_for(s->nk, kRhoBlock * kpt * 2)) * nchunks >= 512) kpt *= 2;
         if (s->tune.rho_kpt == 1 || s->tune.rho_kpt == 2 || s->tune.rho_kpt == 4) kpt = s->tune.rho_kpt;
-        dim3 grid(blocks_for(s->nk, kRhoBlock * kpt), nchunks);
+        int nkb = blocks_for(s->nk, kRhoBlock * kpt);
+        unsigned grid = xcd_grid(nkb, nchunks);
         size_t shm = rows_bytes * (w + 1) + w * sizeof(double);
         long pr = prof_start(s, LG_KERNEL_KSPACE_RHO);
         if (mixed) {
@@ -826,13 +827,13 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
             size_t shm32 = size_t(phase_rows(kmax)) * sizeof(float2) * (w + 1) + w * sizeof(float);
             switch (kpt) {
                 case 1: k_rho_partial32<1><<<grid, kRhoBlock, shm32, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld,
-                                                                       kmax, w, s->kidx.p, s->nk, s->rho_partial.p);
+                                                                       kmax, w, s->kidx.p, s->nk, s->rho_partial.p, nkb, nchunks);
                     break;
                 case 2: k_rho_partial32<2><<<grid, kRhoBlock, shm32, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld,
-                                                                       kmax, w, s->kidx.p, s->nk, s->rho_partial.p);
+                                                                       kmax, w, s->kidx.p, s->nk, s->rho_partial.p, nkb, nchunks);
                     break;
                 default: k_rho_partial32<4><<<grid, kRhoBlock, shm32, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld,
-                                                                        kmax, w, s->kidx.p, s->nk, s->rho_partial.p);
+                                                                        kmax, w, s->kidx.p, s->nk, s->rho_partial.p, nkb, nchunks);
             }
         } else {
             if (!phase_done)
@@ -840,13 +841,13 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
                                                                             s->phase_tab.p, s->phase_q.p);
             switch (kpt) {
                 case 1: k_rho_partial<1><<<grid, kRhoBlock, shm, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
-                                                                   s->kidx.p, s->nk, s->rho_partial.p);
+                                                                   s->kidx.p, s->nk, s->rho_partial.p, nkb, nchunks);
                     break;
                 case 2: k_rho_partial<2><<<grid, kRhoBlock, shm, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
-                                                                   s->kidx.p, s->nk, s->rho_partial.p);
+                                                                   s->kidx.p, s->nk, s->rho_partial.p, nkb, nchunks);
                     break;
                 default: k_rho_partial<4><<<grid, kRhoBlock, shm, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
-                                                                    s->kidx.p, s->nk, s->rho_partial.p);
+                                                                    s->kidx.p, s->nk, s->rho_partial.p, nkb, nchunks);
             }
         }
         if (energy)
@@ -859,13 +860,14 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
                 mixed ? s->krec32.p : nullptr, s->efac.p, s->vfac.p, s->epart_k.p);
         prof_stop(s, pr);
         long pf = prof_start(s, LG_KERNEL_KSPACE_FORCE);
-        dim3 fgrid(blocks_for(n, 64), s->ksplit);
+        int nab = blocks_for(n, 64);
+        unsigned fgrid = xcd_grid(s->ksplit, nab);
         if (mixed)
             k_kspace_force32<<<fgrid, 256, 0, st>>>(s->phase_tab32.p, s->ld, n, kmax, s->rows.p, s->nrows,
-                                                     s->kidx.p, s->krec32.p, s->fk.p);
+                                                     s->kidx.p, s->krec32.p, s->ksplit, nab, s->fk.p);
         else
             k_kspace_force<<<fgrid, 256, 0, st>>>(s->phase_tab.p, s->ld, n, kmax, s->rows.p, s->nrows, s->kidx.p,
-                                                   s->krec.p, s->fk.p);
+                                                   s->krec.p, s->ksplit, nab, s->fk.p);
         prof_stop(s, pf);
     }
     bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;

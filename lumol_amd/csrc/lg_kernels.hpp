@@ -818,6 +818,21 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// XCD-aware coordinates of a logical 2-D grid (gx, gy) launched as a 1-D
+// grid of gx * round_up(gy, 8) blocks.  The dispatcher hands block L to XCD
+// L mod 8, and each XCD has its own L2; logical row `by` is given only ids
+// with L mod 8 == by mod 8, so all blocks that read row by's data (an atom
+// chunk of the phase table) share one L2 and the chunk is fetched from HBM
+// once.  False for the padding blocks (whole block; call before any barrier).
+__device__ __forceinline__ bool xcd_block(int gx, int gy, int& bx, int& by) {
+    int L = blockIdx.x;
+    int q = L >> 3;
+    by = (q / gx) * 8 + (L & 7);
+    bx = q - (q / gx) * gx;
+    return by < gy;
+}
+__host__ __forceinline__ unsigned xcd_grid(int gx, int gy) { return unsigned(gx) * unsigned((gy + 7) / 8 * 8); }
+
 // Rows of atom column `a` along `axis`: e^{i n k_axis x} by the reference's
 // recursion; k_vector(e_axis) . r = (2 pi * inv_axis) * x  (cells.rs:224).
 __device__ __forceinline__ void phase_axis64(double x, double inv, int axis, int kmax, double2* __restrict__ tab,
@@ -903,13 +918,14 @@ template <int KPT>
 __global__ __launch_bounds__(kRhoBlock) void k_rho_partial(const double2* __restrict__ tabg,
                                                            const double* __restrict__ qg, int ld, int kmax, int width,
                                                            const int4* __restrict__ kidx, int nk,
-                                                           double2* __restrict__ partial) {
+                                                           double2* __restrict__ partial, int nkb, int nchunks) {
     extern __shared__ double2 smem[];
+    int bx, chunk;
+    if (!xcd_block(nkb, nchunks, bx, chunk)) return;
     int R = phase_rows(kmax);
     int stride = width + 1;
     double2* tab = smem;
     double* q = reinterpret_cast<double*>(tab + (size_t)R * stride);
-    int chunk = blockIdx.y;
     int a0 = chunk * width;
     for (int e = threadIdx.x; e < R * width; e += blockDim.x) {
         int row = e / width, a = e - row * width;
@@ -917,7 +933,7 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_partial(const double2* __rest
     }
     for (int a = threadIdx.x; a < width; a += blockDim.x) q[a] = qg[a0 + a];
     __syncthreads();
-    int kb = blockIdx.x * kRhoBlock * KPT + threadIdx.x;
+    int kb = bx * kRhoBlock * KPT + threadIdx.x;
     const double2* ex[KPT];
     const double2* ey[KPT];
     const double2* ez[KPT];
@@ -1019,7 +1035,8 @@ __global__ __launch_bounds__(256) void k_rho_reduce(const double2* __restrict__ 
 // consecutive ikz) read uniformly; F_i += q_i / (4 pi eps0) *
 // sum_k Im(e^{ik.r_i} conj(rho_k)) field_k  (ewald.rs:699-719).
 struct KRow {
-    int ikx, iky, kbase, count;
+    short ikx, iky, ikz0, count;  // (ikx, iky), first ikz, k-vectors in the row
+    int kbase, pad;               // index of the row's first k-vector
 };
 
 // Block = 64 atoms x 4 waves.  grid.y splits the rows into ksplit ranges and
@@ -1028,17 +1045,17 @@ struct KRow {
 // (combined in a fixed order by the consumer).  A lane reads only its own
 // atom's column of the phase table, so the reads of a wave are contiguous
 // and come from L2; rows have consecutive ikz.
-__device__ __forceinline__ void kspace_wave_rows(int nrows, int& r0, int& r1) {
+__device__ __forceinline__ void kspace_wave_rows(int nrows, int split, int nsplit, int& r0, int& r1) {
     // wave index made visibly uniform, so that rows[] and the k records are
     // read with scalar loads
-    int nsplit = gridDim.y, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    int s0 = (int)(((long)nrows * blockIdx.y) / nsplit);
-    int s1 = (int)(((long)nrows * (blockIdx.y + 1)) / nsplit);
+    int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int s0 = (int)(((long)nrows * split) / nsplit);
+    int s1 = (int)(((long)nrows * (split + 1)) / nsplit);
     r0 = s0 + ((s1 - s0) * w) / 4;
     r1 = s0 + ((s1 - s0) * (w + 1)) / 4;
 }
 
-__device__ __forceinline__ void kspace_block_store(double fx, double fy, double fz, int n,
+__device__ __forceinline__ void kspace_block_store(double fx, double fy, double fz, int n, int ablock, int split,
                                                    double4* __restrict__ fk) {
     __shared__ double red[3][4][64];
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1046,45 +1063,61 @@ __device__ __forceinline__ void kspace_block_store(double fx, double fy, double 
     red[1][w][lane] = fy;
     red[2][w][lane] = fz;
     __syncthreads();
-    int i = blockIdx.x * 64 + lane;
+    int i = ablock * 64 + lane;
     if (w == 0 && i < n) {
         double sx = ((red[0][0][lane] + red[0][1][lane]) + red[0][2][lane]) + red[0][3][lane];
         double sy = ((red[1][0][lane] + red[1][1][lane]) + red[1][2][lane]) + red[1][3][lane];
         double sz = ((red[2][0][lane] + red[2][1][lane]) + red[2][2][lane]) + red[2][3][lane];
-        fk[(size_t)blockIdx.y * n + i] = make_double4(sx, sy, sz, 0.0);
+        fk[(size_t)split * n + i] = make_double4(sx, sy, sz, 0.0);
     }
 }
 
 __global__ __launch_bounds__(256) void k_kspace_force(const double2* __restrict__ tab, int ld, int n, int kmax,
                                                       const KRow* __restrict__ rows, int nrows,
                                                       const int4* __restrict__ kidx,
-                                                      const KRec* __restrict__ rec,
+                                                      const KRec* __restrict__ rec, int nsplit, int nab,
                                                       double4* __restrict__ fk) {
-    int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    int split, ablock;  // all splits of an atom block on one XCD
+    if (!xcd_block(nsplit, nab, split, ablock)) return;
+    int i = ablock * 64 + (threadIdx.x & 63);
     int r0, r1;
-    kspace_wave_rows(nrows, r0, r1);
+    kspace_wave_rows(nrows, split, nsplit, r0, r1);
     int a = i < ld ? i : ld - 1;
     const double2* col = tab + a;
     // e^{i k.r} along a row: (Ex * Ey) * Ez[ikz0], then one product by
     // e^{i kz1 z} per step of ikz
     const double2 ez1 = col[(size_t)phase_row(2, 1, kmax) * ld];
     double fx = 0, fy = 0, fz = 0;
+    // software pipeline: the next row's header and phase columns, and the
+    // next k record, are loaded while the current ones are used
+    KRow nrow = rows[r0 < r1 ? r0 : 0];
+    double2 pex = col[(size_t)phase_row(0, nrow.ikx, kmax) * ld];
+    double2 pey = col[(size_t)phase_row(1, nrow.iky, kmax) * ld];
+    double2 pez = col[(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
     for (int r = r0; r < r1; r++) {
-        KRow row = rows[r];
-        double2 exy = cmul(col[(size_t)phase_row(0, row.ikx, kmax) * ld], col[(size_t)phase_row(1, row.iky, kmax) * ld]);
-        int ikz0 = kidx[row.kbase].z;
-        double2 e = cmul(exy, col[(size_t)phase_row(2, ikz0, kmax) * ld]);
+        KRow row = nrow;
+        double2 ex = pex, ey = pey, ez0 = pez;
+        if (r + 1 < r1) {
+            nrow = rows[r + 1];
+            pex = col[(size_t)phase_row(0, nrow.ikx, kmax) * ld];
+            pey = col[(size_t)phase_row(1, nrow.iky, kmax) * ld];
+            pez = col[(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
+        }
+        double2 e = cmul(cmul(ex, ey), ez0);
+        const KRec* rp = rec + row.kbase;
+        KRec cur = rp[0];
         for (int c = 0; c < row.count; c++) {
-            const KRec& kr = rec[row.kbase + c];
+            KRec nxt = rp[c + 1 < row.count ? c + 1 : c];
             // Im(e * conj(rho)) = e.im * rho.re - e.re * rho.im
-            double sv = e.y * kr.re - e.x * kr.im;
-            fx += sv * kr.fx;
-            fy += sv * kr.fy;
-            fz += sv * kr.fz;
+            double sv = e.y * cur.re - e.x * cur.im;
+            fx += sv * cur.fx;
+            fy += sv * cur.fy;
+            fz += sv * cur.fz;
             e = cmul(e, ez1);
+            cur = nxt;
         }
     }
-    kspace_block_store(fx, fy, fz, n, fk);
+    kspace_block_store(fx, fy, fz, n, ablock, split, fk);
 }
 
 // ---- mixed-form k-space ----------------------------------------------------------
@@ -1118,13 +1151,14 @@ template <int KPT>
 __global__ __launch_bounds__(kRhoBlock) void k_rho_partial32(const float2* __restrict__ tabg,
                                                              const float* __restrict__ qg, int ld, int kmax,
                                                              int width, const int4* __restrict__ kidx, int nk,
-                                                             double2* __restrict__ partial) {
+                                                             double2* __restrict__ partial, int nkb, int nchunks) {
     extern __shared__ float2 smem32[];
     int R = phase_rows(kmax);
     int stride = width + 1;
     float2* tab = smem32;
     float* q = reinterpret_cast<float*>(tab + (size_t)R * stride);
-    int chunk = blockIdx.y;
+    int bx, chunk;
+    if (!xcd_block(nkb, nchunks, bx, chunk)) return;
     int a0 = chunk * width;
     for (int e = threadIdx.x; e < R * width; e += blockDim.x) {
         int row = e / width, a = e - row * width;
@@ -1132,7 +1166,7 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_partial32(const float2* __res
     }
     for (int a = threadIdx.x; a < width; a += blockDim.x) q[a] = qg[a0 + a];
     __syncthreads();
-    int kb = blockIdx.x * kRhoBlock * KPT + threadIdx.x;
+    int kb = bx * kRhoBlock * KPT + threadIdx.x;
     const float2* ex[KPT];
     const float2* ey[KPT];
     const float2* ez[KPT];
@@ -1168,34 +1202,48 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_partial32(const float2* __res
 __global__ __launch_bounds__(256) void k_kspace_force32(const float2* __restrict__ tab, int ld, int n, int kmax,
                                                         const KRow* __restrict__ rows, int nrows,
                                                         const int4* __restrict__ kidx,
-                                                        const KRec32* __restrict__ rec,
+                                                        const KRec32* __restrict__ rec, int nsplit, int nab,
                                                         double4* __restrict__ fk) {
-    int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    int split, ablock;  // all splits of an atom block on one XCD
+    if (!xcd_block(nsplit, nab, split, ablock)) return;
+    int i = ablock * 64 + (threadIdx.x & 63);
     int r0, r1;
-    kspace_wave_rows(nrows, r0, r1);
+    kspace_wave_rows(nrows, split, nsplit, r0, r1);
     int a = i < ld ? i : ld - 1;
     const float2* col = tab + a;
     const float2 ez1 = col[(size_t)phase_row(2, 1, kmax) * ld];
     double fx = 0, fy = 0, fz = 0;
+    KRow nrow = rows[r0 < r1 ? r0 : 0];
+    float2 pex = col[(size_t)phase_row(0, nrow.ikx, kmax) * ld];
+    float2 pey = col[(size_t)phase_row(1, nrow.iky, kmax) * ld];
+    float2 pez = col[(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
     for (int r = r0; r < r1; r++) {
-        KRow row = rows[r];
-        float2 exy = cmulf(col[(size_t)phase_row(0, row.ikx, kmax) * ld], col[(size_t)phase_row(1, row.iky, kmax) * ld]);
-        int ikz0 = kidx[row.kbase].z;
-        float2 e = cmulf(exy, col[(size_t)phase_row(2, ikz0, kmax) * ld]);
+        KRow row = nrow;
+        float2 ex = pex, ey = pey, ez0 = pez;
+        if (r + 1 < r1) {
+            nrow = rows[r + 1];
+            pex = col[(size_t)phase_row(0, nrow.ikx, kmax) * ld];
+            pey = col[(size_t)phase_row(1, nrow.iky, kmax) * ld];
+            pez = col[(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
+        }
+        float2 e = cmulf(cmulf(ex, ey), ez0);
         float rx = 0.f, ry = 0.f, rz = 0.f;
+        const KRec32* rp = rec + row.kbase;
+        KRec32 cur = rp[0];
         for (int c = 0; c < row.count; c++) {
-            const KRec32& kr = rec[row.kbase + c];
-            float sv = e.y * kr.re - e.x * kr.im;
-            rx += sv * kr.fx;
-            ry += sv * kr.fy;
-            rz += sv * kr.fz;
+            KRec32 nxt = rp[c + 1 < row.count ? c + 1 : c];
+            float sv = e.y * cur.re - e.x * cur.im;
+            rx += sv * cur.fx;
+            ry += sv * cur.fy;
+            rz += sv * cur.fz;
             e = cmulf(e, ez1);
+            cur = nxt;
         }
         fx += rx;
         fy += ry;
         fz += rz;
     }
-    kspace_block_store(fx, fy, fz, n, fk);
+    kspace_block_store(fx, fy, fz, n, ablock, split, fk);
 }
 
 
