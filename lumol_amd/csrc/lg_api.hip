@@ -871,10 +871,14 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         prof_stop(s, pf);
     }
     bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
-    if (combine && (kspace || s->nterms > 0))
+    if (combine && (kspace || s->nterms > 0)) {
+        // timed with the k-space forces: it adds their partials
+        long pc = prof_start(s, kspace ? LG_KERNEL_KSPACE_FORCE : LG_KERNEL_MOLECULES);
         k_force_combine<<<blocks_for(n, 256), 256, 0, st>>>(s->force.p, s->contrib.p,
                                                            s->nterms > 0 ? s->slot_start.p : nullptr, s->fk.p,
                                                            kspace ? s->ksplit : 0, s->pos.p, n);
+        prof_stop(s, pc);
+    }
     LG_HIP(hipGetLastError());
     return LG_OK;
 }

@@ -186,73 +186,108 @@ __global__ __launch_bounds__(256) void k_nlist_build(const float4* __restrict__ 
                    nz1 = b.periodic && g.reach[2] == 0;
         int count = 0;
         int* row = nbr + (size_t)t * cap;
-        for (int dz = -g.reach[2]; dz <= g.reach[2]; dz++) {
-            int z = cz + dz;
-            float sz = 0.f;
+        // The neighbour runs, in (dz, dy, run) order: run 0 = the cells past
+        // the low x edge (shifted by -L), 1 = the cells inside, 2 = past the
+        // high edge (+L).  A run's bounds are loaded one run ahead and a
+        // pass's candidate record one pass ahead, so the dependent loads of
+        // consecutive runs and passes overlap.
+        const int ey = 2 * g.reach[1] + 1, nseg = (2 * g.reach[2] + 1) * ey * 3;
+        const int x0 = cx - g.reach[0], x1 = cx + g.reach[0];
+        struct Run {
+            int s0, e0;
+            float sx, sy, sz;
+        };
+        // next run with cells at or after index k (nseg when none)
+        auto next_run = [&](int k) {
+            while (k < nseg) {
+                int run = k % 3;
+                if ((run == 0 && x0 < 0) || run == 1 || (run == 2 && x1 >= g.n[0])) return k;
+                k++;
+            }
+            return k;
+        };
+        auto run_bounds = [&](int k) {
+            int run = k % 3, dyi = (k / 3) % ey, dzi = k / (3 * ey);
+            int z = cz - g.reach[2] + dzi, y = cy - g.reach[1] + dyi;
+            Run r;
+            r.sz = 0.f;
             if (z < 0) {
                 z += g.n[2];
-                sz = -Lz;
+                r.sz = -Lz;
             } else if (z >= g.n[2]) {
                 z -= g.n[2];
-                sz = Lz;
+                r.sz = Lz;
             }
-            float ddz0 = pi.z - sz;
-            for (int dy = -g.reach[1]; dy <= g.reach[1]; dy++) {
-                int y = cy + dy;
-                float sy = 0.f;
-                if (y < 0) {
-                    y += g.n[1];
-                    sy = -Ly;
-                } else if (y >= g.n[1]) {
-                    y -= g.n[1];
-                    sy = Ly;
-                }
-                float ddy0 = pi.y - sy;
-                int rowc = (z * g.n[1] + y) * g.n[0];
-                int x0 = cx - g.reach[0], x1 = cx + g.reach[0];
-                // runs: [x0, -1] shifted by -L, [0, nx-1] within, [nx, x1] shifted by +L
-                for (int run = 0; run < 3; run++) {
-                    int xa, xb;
-                    float sx;
-                    if (run == 0) {
-                        xa = x0 + g.n[0];
-                        xb = g.n[0] - 1;
-                        sx = -Lx;
-                        if (x0 >= 0) continue;
-                    } else if (run == 1) {
-                        xa = x0 < 0 ? 0 : x0;
-                        xb = x1 >= g.n[0] ? g.n[0] - 1 : x1;
-                        sx = 0.f;
-                    } else {
-                        xa = 0;
-                        xb = x1 - g.n[0];
-                        sx = Lx;
-                        if (x1 < g.n[0]) continue;
-                    }
-                    float ddx0 = pi.x - sx;
-                    int s0 = start[rowc + xa], e0 = start[rowc + xb + 1];
-                    for (int q0 = s0; q0 < e0; q0 += TPA) {
-                        int q = q0 + sub;
-                        bool accept = false;
-                        if (q < e0) {
-                            float4 cj = cpos[q];
-                            if (__float_as_int(cj.w) != mi) {
-                                float ddx = ddx0 - cj.x, ddy = ddy0 - cj.y, ddz = ddz0 - cj.z;
-                                if (nx1) ddx -= rintf(ddx * ix) * Lx;
-                                if (ny1) ddy -= rintf(ddy * iy) * Ly;
-                                if (nz1) ddz -= rintf(ddz * iz) * Lz;
-                                accept = ddx * ddx + ddy * ddy + ddz * ddz < rl2f;
-                            }
-                        }
-                        unsigned long long bal = __ballot(accept);
-                        unsigned long long gm =
-                            TPA == 64 ? bal : ((bal >> gbase) & ((1ull << (TPA & 63)) - 1ull));
-                        int slot = count + __popcll(gm & ((1ull << sub) - 1ull));
-                        if (accept && slot < cap) row[slot] = sorted[q];
-                        count += __popcll(gm);
-                    }
-                }
+            r.sy = 0.f;
+            if (y < 0) {
+                y += g.n[1];
+                r.sy = -Ly;
+            } else if (y >= g.n[1]) {
+                y -= g.n[1];
+                r.sy = Ly;
             }
+            int xa, xb;
+            if (run == 0) {
+                xa = x0 + g.n[0];
+                xb = g.n[0] - 1;
+                r.sx = -Lx;
+            } else if (run == 1) {
+                xa = x0 < 0 ? 0 : x0;
+                xb = x1 >= g.n[0] ? g.n[0] - 1 : x1;
+                r.sx = 0.f;
+            } else {
+                xa = 0;
+                xb = x1 - g.n[0];
+                r.sx = Lx;
+            }
+            int rowc = (z * g.n[1] + y) * g.n[0];
+            r.s0 = start[rowc + xa];
+            r.e0 = start[rowc + xb + 1];
+            return r;
+        };
+        int k = next_run(0);
+        Run cur = run_bounds(k);  // k < nseg always: run 1 exists for every (dz, dy)
+        // loads are unconditional, with indices clamped into range, so that
+        // the compiler can leave them in flight across the work in between
+        while (k < nseg) {
+            int kn = next_run(k + 1);
+            Run nxt = run_bounds(kn < nseg ? kn : k);
+            float ddx0 = pi.x - cur.sx, ddy0 = pi.y - cur.sy, ddz0 = pi.z - cur.sz;
+            // one pass: TPA consecutive candidates, kept in candidate order by
+            // a ballot prefix count
+            auto pass = [&](int q, float4 cj, int jj) {
+                bool accept = false;
+                if (q < cur.e0 && __float_as_int(cj.w) != mi) {
+                    float ddx = ddx0 - cj.x, ddy = ddy0 - cj.y, ddz = ddz0 - cj.z;
+                    if (nx1) ddx -= rintf(ddx * ix) * Lx;
+                    if (ny1) ddy -= rintf(ddy * iy) * Ly;
+                    if (nz1) ddz -= rintf(ddz * iz) * Lz;
+                    accept = ddx * ddx + ddy * ddy + ddz * ddz < rl2f;
+                }
+                unsigned long long bal = __ballot(accept);
+                unsigned long long gm = TPA == 64 ? bal : ((bal >> gbase) & ((1ull << (TPA & 63)) - 1ull));
+                int slot = count + __popcll(gm & ((1ull << sub) - 1ull));
+                if (accept && slot < cap) row[slot] = jj;
+                count += __popcll(gm);
+            };
+            // two passes per iteration on alternating registers, each
+            // buffer reloaded right after its use, so that a candidate load
+            // stays in flight across a whole pass
+            auto clampq = [&](int q) { return q < n ? q : n - 1; };
+            int q = cur.s0 + sub;
+            float4 cA = cpos[clampq(q)];
+            int jA = sorted[clampq(q)];
+            for (int q0 = cur.s0; q0 < cur.e0; q0 += 2 * TPA) {
+                float4 cB = cpos[clampq(q + TPA)];
+                int jB = sorted[clampq(q + TPA)];
+                pass(q, cA, jA);
+                cA = cpos[clampq(q + 2 * TPA)];
+                jA = sorted[clampq(q + 2 * TPA)];
+                if (q0 + TPA < cur.e0) pass(q + TPA, cB, jB);
+                q += 2 * TPA;
+            }
+            k = kn;
+            cur = nxt;
         }
         if (sub == 0) {
             if (count > cap) {

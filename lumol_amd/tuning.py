@@ -2,10 +2,11 @@
 
 The knobs (``lg_tuning``): lanes per atom of the pair and neighbour-list
 kernels, row ranges of the k-space force kernel, k-vectors per thread of the
-rho(k) kernel.  Each knob drives one kernel class, so they are tuned one
-after the other: every candidate value is timed with the library's HIP-event
-profiler over `reps` force evaluations of the system itself, and the fastest
-is kept.  Results are cached per bucket -- (Ewald or not, precision, atoms
+rho(k) kernel.  They are tuned one after the other: every candidate value is
+timed with the library's HIP-event profiler over `reps` force evaluations of
+the system itself -- the summed device time of all kernel classes, so that a
+choice that speeds one kernel and slows another (more k-space partials to
+add) is seen as such -- and the fastest is kept.  Results are cached per bucket -- (Ewald or not, precision, atoms
 rounded to a power of two, k-vectors rounded to a power of two) -- so a
 server tunes once per bucket and reuses the choice for every later job of
 that bucket.  Force evaluations leave the system's state unchanged.
@@ -31,13 +32,13 @@ def bucket(system):
     return (ewald, int(system.precision), p2(n), p2(nk))
 
 
-def _time_class(system, cls, reps):
-    system.profile_begin([cls])
+def _time_all(system, reps):
+    """Device time (ms) of one force evaluation, summed over kernel classes."""
+    system.profile_begin(list(nat.KERNEL_CLASSES))
     for _ in range(reps):
         system.forces()
     prof = system.profile_end()
-    ms, launches = prof[cls]
-    return ms / max(launches, 1)
+    return sum(ms for ms, _ in prof.values()) / reps
 
 
 def autotune(system, reps=5, use_cache=True):
@@ -53,20 +54,20 @@ def autotune(system, reps=5, use_cache=True):
         for lanes in PAIR_LANES:
             system.set_tuning(lanes, best["kspace_split"], best["rho_kpt"])
             system.forces()
-            times[lanes] = _time_class(system, "pairs", reps)
+            times[lanes] = _time_all(system, reps)
         best["pair_lanes"] = min(times, key=times.get)
         if key[0]:
             times = {}
             for split in KSPACE_SPLITS:
                 system.set_tuning(best["pair_lanes"], split, best["rho_kpt"])
                 system.forces()
-                times[split] = _time_class(system, "kspace_force", reps)
+                times[split] = _time_all(system, reps)
             best["kspace_split"] = min(times, key=times.get)
             times = {}
             for kpt in RHO_KPT:
                 system.set_tuning(best["pair_lanes"], best["kspace_split"], kpt)
                 system.forces()
-                times[kpt] = _time_class(system, "kspace_rho", reps)
+                times[kpt] = _time_all(system, reps)
             best["rho_kpt"] = min(times, key=times.get)
         hit = best
         with _lock:
