@@ -235,13 +235,16 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
  *   pair_lanes    lanes per atom in the pair and neighbour-list kernels (4..64)
  *   kspace_split  row ranges of the k-space force kernel (1..32)
  *   rho_kpt       k-vectors per thread of the rho(k) kernel (1, 2, 4)
+ *   nlist_build   neighbour-list build: 1 lanes per atom, 2 one workgroup per
+ *                 cell with candidates staged in LDS (default: cells when
+ *                 there are >= 128 cells and >= 8 atoms per cell)
  * lumol_amd.tuning picks them per size bucket by timing the candidates.
  * Results do not depend on them beyond summation order. */
 typedef struct lg_tuning {
     int pair_lanes;
     int kspace_split;
     int rho_kpt;
-    int reserved;
+    int nlist_build;
 } lg_tuning;
 int lg_system_set_tuning(lg_system* sys, const lg_tuning* tuning);
 /* The values in effect (after the automatic choice); rho_kpt stays 0 when

@@ -697,6 +697,16 @@ int prepare(lg_system* s) {
 // MD step does it inside the integrator's first half instead.  Every kernel
 // returns at once when flags[0] (rebuild) is clear; the build kernel clears
 // it again.
+// Which list-build kernel runs: one workgroup per cell (k_nlist_cells) when
+// there are enough cells to fill the device and enough atoms per cell to keep
+// its waves busy, else TPA lanes per atom (k_nlist_build); the tuning knob
+// nlist_build overrides (1 lanes per atom, 2 cells).
+static bool cell_build(const lg_system* s) {
+    if (s->tune.nlist_build == 1) return false;
+    if (s->tune.nlist_build == 2) return true;
+    return s->grid.ncell >= 128 && s->n >= size_t(8) * s->grid.ncell;
+}
+
 int launch_nlist(lg_system* s, bool check) {
     hipStream_t st = S(s);
     long prof = prof_start(s, LG_KERNEL_NLIST);
@@ -716,6 +726,13 @@ int launch_nlist(lg_system* s, bool check) {
     int nb = blocks_for(size_t(n) * s->tpa, 256);
     // FP32 candidate test with a 1e-4 relative margin on rlist
     float rl2f = (!s->periodic || s->rlist <= 0) ? INFINITY : float(s->rlist * (1.0 + 1e-4) * s->rlist * (1.0 + 1e-4));
+    if (cell_build(s)) {
+        k_nlist_cells<<<g.ncell, kCellBlock, 0, st>>>(s->cpos.p, s->sorted.p, s->start.p, s->count.p, g, s->box, rl2f,
+                                                      s->cap, s->nbr.p, s->nnbr.p, s->pos.p, s->ref.p, flag);
+        prof_stop(s, prof);
+        LG_HIP(hipGetLastError());
+        return LG_OK;
+    }
     switch (s->tpa) {
 #define LG_BUILD_CASE(T)                                                                                   \
     case T:                                                                                                \
@@ -1417,6 +1434,7 @@ int lg_system_set_tuning(lg_system* s, const lg_tuning* t) {
     if (t->kspace_split < 0 || t->kspace_split > 32) return fail(LG_E_INVALID, "kspace_split must be in 0..32");
     if (t->rho_kpt != 0 && t->rho_kpt != 1 && t->rho_kpt != 2 && t->rho_kpt != 4)
         return fail(LG_E_INVALID, "rho_kpt must be 0, 1, 2 or 4");
+    if (t->nlist_build < 0 || t->nlist_build > 2) return fail(LG_E_INVALID, "nlist_build must be 0, 1 or 2");
     s->tune = *t;
     s->kspace_dirty = s->nlist_dirty = true;
     return LG_OK;
@@ -1429,6 +1447,7 @@ int lg_system_get_tuning(lg_system* s, lg_tuning* t) {
     *t = s->tune;
     t->pair_lanes = s->tpa;
     t->kspace_split = (s->coul_kind == COUL_EWALD && s->nk > 0) ? s->ksplit : 0;
+    t->nlist_build = cell_build(s) ? 2 : 1;
     return LG_OK;
 }
 

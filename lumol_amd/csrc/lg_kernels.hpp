@@ -308,6 +308,201 @@ __global__ __launch_bounds__(256) void k_nlist_build(const float4* __restrict__ 
     }
 }
 
+// Verlet list, one workgroup per home cell.  The candidate runs of the cell
+// (the same (dz, dy, run) segments, in the same order, as k_nlist_build) are
+// flattened into one index range; the workgroup stages them in LDS a tile at
+// a time, every thread issuing kCellTile / 256 independent loads, with the
+// cell's periodic shift already added.  Each wave then scans the tile for the
+// home atoms it owns (one atom per wave at a time, 64 candidates a pass).
+// Rows come out in the same candidate order as k_nlist_build's, so the
+// force sums over them are the same; the acceptance test differs only in
+// where the shift is rounded, well inside the 1e-4 margin on rlist.  The
+// global loads of a whole tile are in flight together, where the per-atom
+// kernel waits on one dependent load per pass.
+constexpr int kCellBlock = 256;
+constexpr int kCellTile = 2048;  // candidates staged per tile (40 KB of LDS)
+constexpr int kCellHome = 64;    // home atoms per chunk
+constexpr int kCellMaxSeg = 128; // (2 reach + 1)^2 rows x 3 runs <= 75
+
+__global__ __launch_bounds__(kCellBlock) void k_nlist_cells(const float4* __restrict__ cpos,
+                                                            const int* __restrict__ sorted,
+                                                            const int* __restrict__ start,
+                                                            int* __restrict__ count_zero, Grid g, Box b, float rl2f,
+                                                            int cap, int* __restrict__ nbr, int* __restrict__ nnbr,
+                                                            const double4* __restrict__ pos,
+                                                            double4* __restrict__ ref, int* __restrict__ flags) {
+    if (flags[0] == 0) return;
+    __shared__ float4 tile[kCellTile];
+    __shared__ int tidx[kCellTile];
+    __shared__ int seg_base[kCellMaxSeg + 1];
+    __shared__ int seg_s0[kCellMaxSeg];
+    __shared__ float seg_sh[kCellMaxSeg][3];
+    __shared__ float4 home[kCellHome];
+    __shared__ int home_cnt[kCellHome];
+    __shared__ int last;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int c = blockIdx.x * kCellBlock + tid; c < g.ncell; c += gridDim.x * kCellBlock) count_zero[c] = 0;
+    const float Lx = (float)b.L[0], Ly = (float)b.L[1], Lz = (float)b.L[2];
+    const float ix = (float)b.inv[0], iy = (float)b.inv[1], iz = (float)b.inv[2];
+    const bool nx1 = b.periodic && g.reach[0] == 0, ny1 = b.periodic && g.reach[1] == 0,
+               nz1 = b.periodic && g.reach[2] == 0;
+    const int ey = 2 * g.reach[1] + 1, nseg = (2 * g.reach[2] + 1) * ey * 3;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int c = blockIdx.x; c < g.ncell; c += gridDim.x) {
+        const int hs = start[c], H = start[c + 1] - hs;
+        if (H == 0) continue;  // uniform over the workgroup
+        const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
+        const int x0 = cx - g.reach[0], x1 = cx + g.reach[0];
+        // segment table: wave 0, lane l owns segments 2l and 2l + 1
+        if (wave == 0) {
+            int len[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                int k = 2 * lane + u;
+                len[u] = 0;
+                int run = k % 3;
+                bool exists = k < nseg && (run == 1 || (run == 0 && x0 < 0) || (run == 2 && x1 >= g.n[0]));
+                if (exists) {
+                    int dyi = (k / 3) % ey, dzi = k / (3 * ey);
+                    int z = cz - g.reach[2] + dzi, y = cy - g.reach[1] + dyi;
+                    float sz = 0.f, sy = 0.f, sx = 0.f;
+                    if (z < 0) {
+                        z += g.n[2];
+                        sz = -Lz;
+                    } else if (z >= g.n[2]) {
+                        z -= g.n[2];
+                        sz = Lz;
+                    }
+                    if (y < 0) {
+                        y += g.n[1];
+                        sy = -Ly;
+                    } else if (y >= g.n[1]) {
+                        y -= g.n[1];
+                        sy = Ly;
+                    }
+                    int xa, xb;
+                    if (run == 0) {
+                        xa = x0 + g.n[0];
+                        xb = g.n[0] - 1;
+                        sx = -Lx;
+                    } else if (run == 1) {
+                        xa = x0 < 0 ? 0 : x0;
+                        xb = x1 >= g.n[0] ? g.n[0] - 1 : x1;
+                    } else {
+                        xa = 0;
+                        xb = x1 - g.n[0];
+                        sx = Lx;
+                    }
+                    int rowc = (z * g.n[1] + y) * g.n[0];
+                    int s0 = start[rowc + xa];
+                    len[u] = start[rowc + xb + 1] - s0;
+                    seg_s0[k] = s0;
+                    seg_sh[k][0] = sx;
+                    seg_sh[k][1] = sy;
+                    seg_sh[k][2] = sz;
+                }
+            }
+            int pair = len[0] + len[1], incl = pair;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                int y = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += y;
+            }
+            seg_base[2 * lane] = incl - pair;
+            seg_base[2 * lane + 1] = incl - pair + len[0];
+            if (lane == 63) seg_base[kCellMaxSeg] = incl;
+        }
+        __syncthreads();
+        const int M = seg_base[kCellMaxSeg];
+        for (int h0 = 0; h0 < H; h0 += kCellHome) {
+            const int hn = min(kCellHome, H - h0);
+            if (tid < hn) {
+                home[tid] = cpos[hs + h0 + tid];
+                home_cnt[tid] = 0;
+            }
+            for (int m0 = 0; m0 < M; m0 += kCellTile) {
+                const int tl = min(kCellTile, M - m0);
+                constexpr int R = kCellTile / kCellBlock;
+                int qv[R], kv[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    int m = m0 + tid + r * kCellBlock;
+                    int mm = m < M ? m : M - 1;
+                    int k = 0;
+#pragma unroll
+                    for (int step = kCellMaxSeg / 2; step > 0; step >>= 1)
+                        if (k + step < nseg && seg_base[k + step] <= mm) k += step;
+                    kv[r] = k;
+                    qv[r] = seg_s0[k] + (mm - seg_base[k]);
+                }
+                float4 cv[R];
+                int jv[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    cv[r] = cpos[qv[r]];
+                    jv[r] = sorted[qv[r]];
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    int e = tid + r * kCellBlock;
+                    if (e < tl) {
+                        int k = kv[r];
+                        tile[e] = make_float4(cv[r].x + seg_sh[k][0], cv[r].y + seg_sh[k][1],
+                                              cv[r].z + seg_sh[k][2], cv[r].w);
+                        tidx[e] = jv[r];
+                    }
+                }
+                __syncthreads();
+                for (int h = wave; h < hn; h += kCellBlock / 64) {
+                    const float4 pi = home[h];
+                    const int mi = __float_as_int(pi.w);
+                    int cnt = home_cnt[h];
+                    int* row = nbr + (size_t)(hs + h0 + h) * cap;
+#pragma unroll 2
+                    for (int p = 0; p < tl; p += 64) {
+                        int e = p + lane;
+                        float4 cj = tile[e < tl ? e : 0];
+                        bool accept = false;
+                        if (e < tl && __float_as_int(cj.w) != mi) {
+                            float ddx = pi.x - cj.x, ddy = pi.y - cj.y, ddz = pi.z - cj.z;
+                            if (nx1) ddx -= rintf(ddx * ix) * Lx;
+                            if (ny1) ddy -= rintf(ddy * iy) * Ly;
+                            if (nz1) ddz -= rintf(ddz * iz) * Lz;
+                            accept = ddx * ddx + ddy * ddy + ddz * ddz < rl2f;
+                        }
+                        unsigned long long bal = __ballot(accept);
+                        int slot = cnt + __popcll(bal & below);
+                        if (accept && slot < cap) row[slot] = tidx[e];
+                        cnt += __popcll(bal);
+                    }
+                    if (lane == 0) home_cnt[h] = cnt;
+                }
+                __syncthreads();
+            }
+            if (tid < hn) {
+                int t = hs + h0 + tid;
+                int cnt = home_cnt[tid];
+                if (cnt > cap) {
+                    atomicMax(flags + 1, cnt);
+                    cnt = cap;
+                }
+                nnbr[t] = cnt;
+                int i = sorted[t];
+                ref[i] = pos[i];
+            }
+            __syncthreads();
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) last = atomicAdd(flags + 2, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (last && tid == 0) {
+        flags[2] = 0;
+        flags[0] = 0;
+    }
+}
+
 
 // ---------------------------------------------------------------------------------
 // Non-bonded pair kernel: inter-molecular pairs, Lennard-Jones family + Ewald
