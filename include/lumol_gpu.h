@@ -234,7 +234,7 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
  * the device kernels; 0 leaves the built-in size-based choice:
  *   pair_lanes    lanes per atom in the pair and neighbour-list kernels (4..64)
  *   kspace_split  row ranges of the k-space force kernel (1..32)
- *   rho_kpt       k-vectors per thread of the rho(k) kernel (1, 2, 4)
+ *   rho_width     atoms per block of the rho(k) kernel (64, 128, 256, 512)
  *   nlist_build   neighbour-list build: 1 lanes per atom, 2 one workgroup per
  *                 cell with candidates staged in LDS (default: cells when
  *                 there are >= 128 cells and >= 8 atoms per cell)
@@ -243,12 +243,12 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
 typedef struct lg_tuning {
     int pair_lanes;
     int kspace_split;
-    int rho_kpt;
+    int rho_width;
     int nlist_build;
 } lg_tuning;
 int lg_system_set_tuning(lg_system* sys, const lg_tuning* tuning);
-/* The values in effect (after the automatic choice); rho_kpt stays 0 when
- * automatic (chosen per launch). */
+/* The values in effect (after the automatic choice); kspace_split and
+ * rho_width are 0 for systems without Ewald k-space. */
 int lg_system_get_tuning(lg_system* sys, lg_tuning* tuning);
 
 /* ---- kernel timing ---------------------------------------------------------------

@@ -47,7 +47,7 @@ class Profile(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("pair_lanes", ctypes.c_int), ("kspace_split", ctypes.c_int), ("rho_kpt", ctypes.c_int),
+    _fields_ = [("pair_lanes", ctypes.c_int), ("kspace_split", ctypes.c_int), ("rho_width", ctypes.c_int),
                 ("nlist_build", ctypes.c_int)]
 
 

@@ -171,11 +171,14 @@ struct lg_system {
     DevBuf<double4> contrib;     // one force contribution per (term, atom)
     DevBuf<BondedParam> bparam, aparam, dparam;
     // k-space
-    int nk = 0, nrows = 0, rho_width = 64, force_width = 64, ksplit = 1;
+    int nk = 0, nrows = 0, nunits = 0, ksplit = 1;
     int tpa = 8;  // lanes per atom in k_pairs
     DevBuf<double4> fk;  // k-space partial forces [ksplit][n]
     DevBuf<int4> kidx;
     DevBuf<KRow> rows;
+    DevBuf<KUnit> units;         // rho pass work units (up to 4 k-vectors of a row)
+    DevBuf<int> kslot;           // k -> slot of the rho partials (j * nunits + unit)
+    double kg[3] = {0, 0, 0};    // 2 pi / L per axis
     DevBuf<double> efac, vfac;
     DevBuf<KRec> krec;           // rho(k) and force factor per k
     DevBuf<double2> rho, rho_partial;
@@ -525,7 +528,7 @@ int prepare_kspace(lg_system* s) {
         double energy = four_pi_v * std::exp(-k2 * a4) / k2;
         idx.push_back(make_int4(x, y, z, 0));
         efac.push_back(energy);
-        rec.push_back(KRec{0.0, 0.0, (2.0 * energy) * k[0], (2.0 * energy) * k[1], (2.0 * energy) * k[2], 0, 0, 0});
+        rec.push_back(KRec{0.0, 0.0, 2.0 * energy, (2.0 * energy) * double(z)});
         double f = -2.0 * (1.0 / k2 + a4);
         // virial = energy * (I + f k (x) k); components xx yy zz xy xz yz
         vf[0].push_back(energy * (1.0 + f * (k[0] * k[0])));
@@ -550,38 +553,40 @@ int prepare_kspace(lg_system* s) {
         rows.back().count++;
     }
     s->nrows = int(rows.size());
+    std::vector<KUnit> units;
+    for (const KRow& r : rows)
+        for (int j = 0; j < r.count; j += 4)
+            units.push_back(KUnit{r.ikx, r.iky, short(r.ikz0 + j), short(std::min(4, r.count - j)), r.kbase + j, 0});
+    s->nunits = int(units.size());
+    std::vector<int> kslot(s->nk);
+    for (int u = 0; u < s->nunits; u++)
+        for (int j = 0; j < units[u].nv; j++) kslot[units[u].kb + j] = j * s->nunits + u;
+    for (int d = 0; d < 3; d++) s->kg[d] = 2.0 * kPi * inv[d];
     std::vector<double> vflat;
     for (int q = 0; q < 6; q++) vflat.insert(vflat.end(), vf[q].begin(), vf[q].end());
     hipStream_t st = S(s);
     LG_TRY(upload(s->kidx, idx, st));
     LG_TRY(upload(s->rows, rows, st));
+    LG_TRY(upload(s->units, units, st));
+    LG_TRY(upload(s->kslot, kslot, st));
     LG_TRY(upload(s->efac, efac, st));
     LG_TRY(upload(s->vfac, vflat, st));
     LG_TRY(upload(s->krec, rec, st));
     std::vector<KRec32> r32(rec.size());
     for (size_t k = 0; k < rec.size(); k++)
-        r32[k] = KRec32{0.f, 0.f, float(rec[k].fx), float(rec[k].fy), float(rec[k].fz), 0.f, 0.f, 0.f};
+        r32[k] = KRec32{0.f, 0.f, float(rec[k].g), float(rec[k].gz)};
     LG_TRY(upload(s->krec32, r32, st));
     LG_TRY(s->rho.ensure(s->nk));
-    // LDS budget for the phase tables
-    size_t rows_bytes = size_t(phase_rows(kmax)) * sizeof(double2);
-    size_t maxs = s->ctx->max_shared;
-    s->rho_width = 64;
-    while (s->rho_width > 8 && rows_bytes * (s->rho_width + 1) + s->rho_width * sizeof(double) > maxs)
-        s->rho_width /= 2;
-    s->force_width = 64;
-    while (s->force_width > 8 && rows_bytes * (s->force_width + 1) > maxs) s->force_width /= 2;
-    if (rows_bytes * 9 > maxs || kmax > 24)
-        return fail(LG_E_UNSUPPORTED, "kmax too large for the k-space LDS tables");
+    if (kmax > 1000) return fail(LG_E_UNSUPPORTED, "kmax too large");
     // enough waves to fill the device (~4096: 64 atoms x 4 waves per block,
     // grid.y row splits), with as few force partials as that allows
     int fwaves = 4 * blocks_for(s->n, 64);
     s->ksplit = std::max(1, std::min(32, std::min(s->nrows, (4096 + fwaves - 1) / fwaves)));
     if (s->tune.kspace_split > 0) s->ksplit = std::min(s->tune.kspace_split, std::max(1, s->nrows));
     LG_TRY(s->fk.ensure(size_t(s->ksplit) * s->n));
-    int nchunks = blocks_for(s->n, s->rho_width);
-    LG_TRY(s->rho_partial.ensure(size_t(nchunks) * s->nk));
     s->ld = blocks_for(s->n, 64) * 64;
+    // rho partials for the narrowest atom chunk (64)
+    LG_TRY(s->rho_partial.ensure(size_t(blocks_for(s->ld, 64)) * 4 * s->nunits));
     LG_TRY(s->phase_tab.ensure(size_t(phase_rows(kmax)) * s->ld));
     LG_TRY(s->phase_q.ensure(s->ld));
     LG_TRY(s->phase_tab32.ensure(size_t(phase_rows(kmax)) * s->ld));
@@ -707,6 +712,17 @@ static bool cell_build(const lg_system* s) {
     return s->grid.ncell >= 128 && s->n >= size_t(8) * s->grid.ncell;
 }
 
+// Atoms per block of the rho pass: the narrowest chunk (64..512) that keeps
+// the grid at or below ~2048 blocks, so that per-block work stays
+// substantial while the partials stay few; the tuning knob overrides.
+static int rho_chunk(const lg_system* s) {
+    if (s->tune.rho_width > 0) return s->tune.rho_width;
+    int nub = blocks_for(s->nunits, 64);
+    int w = 64;
+    while (w < 512 && size_t(nub) * size_t(blocks_for(s->ld, w)) > 2048) w *= 2;
+    return w;
+}
+
 int launch_nlist(lg_system* s, bool check) {
     hipStream_t st = S(s);
     long prof = prof_start(s, LG_KERNEL_NLIST);
@@ -825,66 +841,45 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     }
     if (s->coul_kind == COUL_EWALD && s->nk > 0) {
         int kmax = s->coul_kmax;
-        size_t rows_bytes = size_t(phase_rows(kmax)) * sizeof(double2);
-        int w = s->rho_width;
-        int nchunks = blocks_for(n, w);
-        // k-vectors per thread: stage each atom chunk in as few blocks as
-        // keep ~512 blocks in flight
-        int kpt = 1;
-        while (kpt < 4 && size_t(blocks_for(s->nk, kRhoBlock * kpt * 2)) * nchunks >= 512) kpt *= 2;
-        if (s->tune.rho_kpt == 1 || s->tune.rho_kpt == 2 || s->tune.rho_kpt == 4) kpt = s->tune.rho_kpt;
-        int nkb = blocks_for(s->nk, kRhoBlock * kpt);
-        unsigned grid = xcd_grid(nkb, nchunks);
-        size_t shm = rows_bytes * (w + 1) + w * sizeof(double);
+        int nub = blocks_for(s->nunits, 64);
+        int w = rho_chunk(s);
+        int nchunks = blocks_for(s->ld, w);
+        int nslot = 4 * s->nunits;
+        unsigned grid = xcd_grid(nub, nchunks);
         long pr = prof_start(s, LG_KERNEL_KSPACE_RHO);
         if (mixed) {
             if (!phase_done)
                 k_phase32<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
                                                                               s->phase_tab32.p, s->phase_q32.p);
-            size_t shm32 = size_t(phase_rows(kmax)) * sizeof(float2) * (w + 1) + w * sizeof(float);
-            switch (kpt) {
-                case 1: k_rho_partial32<1><<<grid, kRhoBlock, shm32, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld,
-                                                                       kmax, w, s->kidx.p, s->nk, s->rho_partial.p, nkb, nchunks);
-                    break;
-                case 2: k_rho_partial32<2><<<grid, kRhoBlock, shm32, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld,
-                                                                       kmax, w, s->kidx.p, s->nk, s->rho_partial.p, nkb, nchunks);
-                    break;
-                default: k_rho_partial32<4><<<grid, kRhoBlock, shm32, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld,
-                                                                        kmax, w, s->kidx.p, s->nk, s->rho_partial.p, nkb, nchunks);
-            }
+            k_rho_rows<float, float2><<<grid, 256, 0, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld, kmax, w,
+                                                            s->units.p, s->nunits, s->rho_partial.p, nub, nchunks);
         } else {
             if (!phase_done)
                 k_phase<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
                                                                             s->phase_tab.p, s->phase_q.p);
-            switch (kpt) {
-                case 1: k_rho_partial<1><<<grid, kRhoBlock, shm, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
-                                                                   s->kidx.p, s->nk, s->rho_partial.p, nkb, nchunks);
-                    break;
-                case 2: k_rho_partial<2><<<grid, kRhoBlock, shm, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
-                                                                   s->kidx.p, s->nk, s->rho_partial.p, nkb, nchunks);
-                    break;
-                default: k_rho_partial<4><<<grid, kRhoBlock, shm, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
-                                                                    s->kidx.p, s->nk, s->rho_partial.p, nkb, nchunks);
-            }
+            k_rho_rows<double, double2><<<grid, 256, 0, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
+                                                              s->units.p, s->nunits, s->rho_partial.p, nub, nchunks);
         }
         if (energy)
             k_rho_reduce<true><<<blocks_for(s->nk, kRhoReduceK), 256, 0, st>>>(
-                s->rho_partial.p, nchunks, s->nk, s->rho.p, mixed ? nullptr : s->krec.p,
+                s->rho_partial.p, nchunks, nslot, s->kslot.p, s->nk, s->rho.p, mixed ? nullptr : s->krec.p,
                 mixed ? s->krec32.p : nullptr, s->efac.p, s->vfac.p, s->epart_k.p);
         else
             k_rho_reduce<false><<<blocks_for(s->nk, kRhoReduceK), 256, 0, st>>>(
-                s->rho_partial.p, nchunks, s->nk, s->rho.p, mixed ? nullptr : s->krec.p,
+                s->rho_partial.p, nchunks, nslot, s->kslot.p, s->nk, s->rho.p, mixed ? nullptr : s->krec.p,
                 mixed ? s->krec32.p : nullptr, s->efac.p, s->vfac.p, s->epart_k.p);
         prof_stop(s, pr);
         long pf = prof_start(s, LG_KERNEL_KSPACE_FORCE);
         int nab = blocks_for(n, 64);
         unsigned fgrid = xcd_grid(s->ksplit, nab);
         if (mixed)
-            k_kspace_force32<<<fgrid, 256, 0, st>>>(s->phase_tab32.p, s->ld, n, kmax, s->rows.p, s->nrows,
-                                                     s->kidx.p, s->krec32.p, s->ksplit, nab, s->fk.p);
+            k_kspace_force<float, float2, KRec32><<<fgrid, 256, 0, st>>>(
+                s->phase_tab32.p, s->ld, n, kmax, s->rows.p, s->nrows, s->krec32.p, s->kg[0], s->kg[1], s->kg[2],
+                s->ksplit, nab, s->fk.p);
         else
-            k_kspace_force<<<fgrid, 256, 0, st>>>(s->phase_tab.p, s->ld, n, kmax, s->rows.p, s->nrows, s->kidx.p,
-                                                   s->krec.p, s->ksplit, nab, s->fk.p);
+            k_kspace_force<double, double2, KRec><<<fgrid, 256, 0, st>>>(
+                s->phase_tab.p, s->ld, n, kmax, s->rows.p, s->nrows, s->krec.p, s->kg[0], s->kg[1], s->kg[2],
+                s->ksplit, nab, s->fk.p);
         prof_stop(s, pf);
     }
     bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
@@ -1432,8 +1427,8 @@ int lg_system_set_tuning(lg_system* s, const lg_tuning* t) {
     if (l != 0 && l != 4 && l != 8 && l != 16 && l != 32 && l != 64)
         return fail(LG_E_INVALID, "pair_lanes must be 0, 4, 8, 16, 32 or 64");
     if (t->kspace_split < 0 || t->kspace_split > 32) return fail(LG_E_INVALID, "kspace_split must be in 0..32");
-    if (t->rho_kpt != 0 && t->rho_kpt != 1 && t->rho_kpt != 2 && t->rho_kpt != 4)
-        return fail(LG_E_INVALID, "rho_kpt must be 0, 1, 2 or 4");
+    if (t->rho_width != 0 && t->rho_width != 64 && t->rho_width != 128 && t->rho_width != 256 && t->rho_width != 512)
+        return fail(LG_E_INVALID, "rho_width must be 0, 64, 128, 256 or 512");
     if (t->nlist_build < 0 || t->nlist_build > 2) return fail(LG_E_INVALID, "nlist_build must be 0, 1 or 2");
     s->tune = *t;
     s->kspace_dirty = s->nlist_dirty = true;
@@ -1448,6 +1443,7 @@ int lg_system_get_tuning(lg_system* s, lg_tuning* t) {
     t->pair_lanes = s->tpa;
     t->kspace_split = (s->coul_kind == COUL_EWALD && s->nk > 0) ? s->ksplit : 0;
     t->nlist_build = cell_build(s) ? 2 : 1;
+    t->rho_width = (s->coul_kind == COUL_EWALD && s->nk > 0) ? rho_chunk(s) : 0;
     return LG_OK;
 }
 

@@ -1035,7 +1035,6 @@ __global__ void k_force_combine(double4* __restrict__ force, const double4* __re
 // table in HBM; the rho and force kernels stage the rows they use in LDS and
 // form e^{i k.r} = (Ex * Ey) * Ez as ewald.rs:666-668.
 // ---------------------------------------------------------------------------------
-constexpr int kRhoBlock = 256;  // k-vectors per block in the rho pass
 
 // Phase-table rows: axis 0 keeps n = 0..kmax (ikx >= 0 in the half-space
 // set), axes 1 and 2 keep n = -kmax..kmax: 5*kmax + 3 rows of `ld` atoms.
@@ -1140,68 +1139,88 @@ __global__ void k_phase(const double4* __restrict__ pos, int n, int ld, Box b, i
     if (axis == 0) q[a] = live ? qa : 0.0;
 }
 
-// rho partial sums: grid (ceil(nk / (kRhoBlock * kpt)), nchunks), `width`
-// atoms per chunk, kpt k-vectors per thread (k = base + j * kRhoBlock); the
-// chunk's table rows are staged once in LDS with a padded row stride
-// (width + 1) so lanes reading different rows hit different banks.
-template <int KPT>
-__global__ __launch_bounds__(kRhoBlock) void k_rho_partial(const double2* __restrict__ tabg,
-                                                           const double* __restrict__ qg, int ld, int kmax, int width,
-                                                           const int4* __restrict__ kidx, int nk,
-                                                           double2* __restrict__ partial, int nkb, int nchunks) {
-    extern __shared__ double2 smem[];
-    int bx, chunk;
-    if (!xcd_block(nkb, nchunks, bx, chunk)) return;
-    int R = phase_rows(kmax);
-    int stride = width + 1;
-    double2* tab = smem;
-    double* q = reinterpret_cast<double*>(tab + (size_t)R * stride);
-    int a0 = chunk * width;
-    for (int e = threadIdx.x; e < R * width; e += blockDim.x) {
-        int row = e / width, a = e - row * width;
-        tab[row * stride + a] = tabg[(size_t)row * ld + a0 + a];
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// rho partial sums by row units.  A unit is up to 4 consecutive k-vectors of
+// one row (fixed ikx, iky; consecutive ikz): per atom, the lane forms
+// q e^{i k0.r} = q (Ex * Ey) * Ez[ikz0] and steps to the next ikz by one
+// product with Ez[1], 7.5 operations and one table read per (atom, k).
+// Block = 64 units (one per lane) x 4 waves; the block's atom chunk of
+// `width` atoms is split in four and each wave sums its quarter; the four
+// wave sums are added in order in LDS and wave j writes k-vector j of every
+// unit: partial[chunk * 4 * nunits + j * nunits + unit] (slots with j past
+// the unit's count are not written).  Table rows are read straight from L2:
+// a wave's lanes read few distinct rows at one atom column.  The table is
+// padded with zero charges to `ld` atoms.  Grid: xcd_grid(nub, nchunks),
+// so all unit blocks of an atom chunk share one L2.  Mixed form (T = float):
+// each wave's quarter is summed in FP32, the rest in FP64.
+struct KUnit {
+    short ikx, iky, kz0, nv;  // row, first ikz, k-vectors in the unit (1..4)
+    int kb, pad;              // index of the unit's first k-vector
+};
+
+template <typename T, typename T2>
+__global__ __launch_bounds__(256) void k_rho_rows(const T2* __restrict__ tab, const T* __restrict__ q, int ld,
+                                                  int kmax, int width, const KUnit* __restrict__ units, int nunits,
+                                                  double2* __restrict__ partial, int nub, int nchunks) {
+    __shared__ double2 red[4][4][64];
+    int ub, chunk;
+    if (!xcd_block(nub, nchunks, ub, chunk)) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int u = ub * 64 + lane;
+    const KUnit un = units[u < nunits ? u : nunits - 1];
+    const T2* cx = tab + (size_t)phase_row(0, un.ikx, kmax) * ld;
+    const T2* cy = tab + (size_t)phase_row(1, un.iky, kmax) * ld;
+    const T2* cz = tab + (size_t)phase_row(2, un.kz0, kmax) * ld;
+    const T2* c1 = tab + (size_t)phase_row(2, 1, kmax) * ld;
+    const int per = width / 4;
+    const int a0 = chunk * width + w * per;
+    const int a1 = min(a0 + per, ld);
+    T r0 = 0, i0 = 0, r1 = 0, i1 = 0, r2 = 0, i2 = 0, r3 = 0, i3 = 0;
+#pragma unroll 4
+    for (int a = a0; a < a1; a++) {
+        const T qa = q[a];
+        const T2 e1 = c1[a];
+        T2 e = cmul(cmul(cx[a], cy[a]), cz[a]);
+        e.x *= qa;
+        e.y *= qa;
+        r0 += e.x;
+        i0 += e.y;
+        e = cmul(e, e1);
+        r1 += e.x;
+        i1 += e.y;
+        e = cmul(e, e1);
+        r2 += e.x;
+        i2 += e.y;
+        e = cmul(e, e1);
+        r3 += e.x;
+        i3 += e.y;
     }
-    for (int a = threadIdx.x; a < width; a += blockDim.x) q[a] = qg[a0 + a];
+    red[w][0][lane] = make_double2(r0, i0);
+    red[w][1][lane] = make_double2(r1, i1);
+    red[w][2][lane] = make_double2(r2, i2);
+    red[w][3][lane] = make_double2(r3, i3);
     __syncthreads();
-    int kb = bx * kRhoBlock * KPT + threadIdx.x;
-    const double2* ex[KPT];
-    const double2* ey[KPT];
-    const double2* ez[KPT];
-    double re[KPT], im[KPT];
-#pragma unroll
-    for (int j = 0; j < KPT; j++) {
-        int k = min(kb + j * kRhoBlock, nk - 1);
-        int4 id = kidx[k];
-        ex[j] = tab + phase_row(0, id.x, kmax) * stride;
-        ey[j] = tab + phase_row(1, id.y, kmax) * stride;
-        ez[j] = tab + phase_row(2, id.z, kmax) * stride;
-        re[j] = 0.0;
-        im[j] = 0.0;
-    }
-    for (int a = 0; a < width; a++) {
-        double qa = q[a];
-#pragma unroll
-        for (int j = 0; j < KPT; j++) {
-            double2 phi = cmul(cmul(ex[j][a], ey[j][a]), ez[j][a]);
-            re[j] += qa * phi.x;
-            im[j] += qa * phi.y;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < KPT; j++) {
-        int k = kb + j * kRhoBlock;
-        if (k < nk) partial[(size_t)chunk * nk + k] = make_double2(re[j], im[j]);
+    if (u < nunits && w < un.nv) {
+        double2 s0 = red[0][w][lane], s1 = red[1][w][lane], s2 = red[2][w][lane], s3 = red[3][w][lane];
+        partial[(size_t)chunk * 4 * nunits + (size_t)w * nunits + u] =
+            make_double2(((s0.x + s1.x) + s2.x) + s3.x, ((s0.y + s1.y) + s2.y) + s3.y);
     }
 }
 
-// Per k-vector record read by the force kernels with one scalar load:
-// rho(k) (written by k_rho_reduce) and the force factor
-// field_k = 2 * factor_k * k (ewald.rs:712-714), constant per cell.
-struct alignas(64) KRec {
-    double re, im, fx, fy, fz, pad0, pad1, pad2;
+// Per k-vector record of the force kernels: rho(k) (written by
+// k_rho_reduce) and g = 2 factor_k, gz = g * ikz.  The reference's force
+// factor field_k = 2 factor_k k (ewald.rs:712-714) has k = (gx ikx, gy iky,
+// gz' ikz) with gx = 2 pi / Lx..., so along a row only the z component
+// varies: the kernels sum A = sum_k s_k g and B = sum_k s_k gz and apply
+// (gx ikx A, gy iky A, gz' B) once per row.
+struct alignas(32) KRec {
+    double re, im, g, gz;
 };
-struct alignas(32) KRec32 {
-    float re, im, fx, fy, fz, pad0, pad1, pad2;
+struct alignas(16) KRec32 {
+    float re, im, g, gz;
 };
 
 // rho(k) = sum over chunks, in a fixed order: thread (g, kl) of the block
@@ -1212,7 +1231,8 @@ struct alignas(32) KRec32 {
 constexpr int kRhoReduceK = 16;
 
 template <bool kEnergy>
-__global__ __launch_bounds__(256) void k_rho_reduce(const double2* __restrict__ partial, int nchunks, int nk,
+__global__ __launch_bounds__(256) void k_rho_reduce(const double2* __restrict__ partial, int nchunks, int nslot,
+                                                    const int* __restrict__ kslot, int nk,
                                                     double2* __restrict__ rho, KRec* __restrict__ rec,
                                                     KRec32* __restrict__ rec32, const double* __restrict__ efac,
                                                     const double* __restrict__ vfac, double* __restrict__ epart) {
@@ -1221,12 +1241,14 @@ __global__ __launch_bounds__(256) void k_rho_reduce(const double2* __restrict__ 
     int g = threadIdx.x / kRhoReduceK, kl = threadIdx.x % kRhoReduceK;
     int k = blockIdx.x * kRhoReduceK + kl;
     double re = 0.0, im = 0.0;
-    if (k < nk)
+    if (k < nk) {
+        const int slot = kslot[k];
         for (int c = g; c < nchunks; c += G) {
-            double2 p = partial[(size_t)c * nk + k];
+            double2 p = partial[(size_t)c * nslot + slot];
             re += p.x;
             im += p.y;
         }
+    }
     part[g][kl] = make_double2(re, im);
     __syncthreads();
     if (threadIdx.x >= 64) return;
@@ -1302,50 +1324,70 @@ __device__ __forceinline__ void kspace_block_store(double fx, double fy, double 
     }
 }
 
-__global__ __launch_bounds__(256) void k_kspace_force(const double2* __restrict__ tab, int ld, int n, int kmax,
+// One step along a row: s = Im(e conj(rho)), A += s g, B += s gz, e *= Ez[1].
+template <typename T, typename T2, typename R>
+__device__ __forceinline__ void kforce_step(const R& r, T2& e, T2 ez1, T& A, T& B) {
+    T sv = e.y * r.re - e.x * r.im;
+    A += sv * r.g;
+    B += sv * r.gz;
+    e = cmul(e, ez1);
+}
+
+// T = double: FP64 form; T = float: mixed form (FP32 phases, rho and
+// factors; each row's sums are added to FP64 accumulators).  Records are
+// read four at a time with one batch of scalar loads, so that a wave waits
+// once per four k-vectors.
+template <typename T, typename T2, typename R>
+__global__ __launch_bounds__(256) void k_kspace_force(const T2* __restrict__ tab, int ld, int n, int kmax,
                                                       const KRow* __restrict__ rows, int nrows,
-                                                      const int4* __restrict__ kidx,
-                                                      const KRec* __restrict__ rec, int nsplit, int nab,
-                                                      double4* __restrict__ fk) {
+                                                      const R* __restrict__ rec, double gx, double gy, double gz,
+                                                      int nsplit, int nab, double4* __restrict__ fk) {
     int split, ablock;  // all splits of an atom block on one XCD
     if (!xcd_block(nsplit, nab, split, ablock)) return;
     int i = ablock * 64 + (threadIdx.x & 63);
     int r0, r1;
     kspace_wave_rows(nrows, split, nsplit, r0, r1);
     int a = i < ld ? i : ld - 1;
-    const double2* col = tab + a;
+    const T2* col = tab + a;
     // e^{i k.r} along a row: (Ex * Ey) * Ez[ikz0], then one product by
     // e^{i kz1 z} per step of ikz
-    const double2 ez1 = col[(size_t)phase_row(2, 1, kmax) * ld];
+    const T2 ez1 = col[(size_t)phase_row(2, 1, kmax) * ld];
+    const T tgx = (T)gx, tgy = (T)gy, tgz = (T)gz;
     double fx = 0, fy = 0, fz = 0;
-    // software pipeline: the next row's header and phase columns, and the
-    // next k record, are loaded while the current ones are used
+    // software pipeline: the next row's header and phase columns are loaded
+    // while the current row is summed
     KRow nrow = rows[r0 < r1 ? r0 : 0];
-    double2 pex = col[(size_t)phase_row(0, nrow.ikx, kmax) * ld];
-    double2 pey = col[(size_t)phase_row(1, nrow.iky, kmax) * ld];
-    double2 pez = col[(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
+    T2 pex = col[(size_t)phase_row(0, nrow.ikx, kmax) * ld];
+    T2 pey = col[(size_t)phase_row(1, nrow.iky, kmax) * ld];
+    T2 pez = col[(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
     for (int r = r0; r < r1; r++) {
         KRow row = nrow;
-        double2 ex = pex, ey = pey, ez0 = pez;
+        T2 ex = pex, ey = pey, ez0 = pez;
         if (r + 1 < r1) {
             nrow = rows[r + 1];
             pex = col[(size_t)phase_row(0, nrow.ikx, kmax) * ld];
             pey = col[(size_t)phase_row(1, nrow.iky, kmax) * ld];
             pez = col[(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
         }
-        double2 e = cmul(cmul(ex, ey), ez0);
-        const KRec* rp = rec + row.kbase;
-        KRec cur = rp[0];
-        for (int c = 0; c < row.count; c++) {
-            KRec nxt = rp[c + 1 < row.count ? c + 1 : c];
-            // Im(e * conj(rho)) = e.im * rho.re - e.re * rho.im
-            double sv = e.y * cur.re - e.x * cur.im;
-            fx += sv * cur.fx;
-            fy += sv * cur.fy;
-            fz += sv * cur.fz;
-            e = cmul(e, ez1);
-            cur = nxt;
+        T2 e = cmul(cmul(ex, ey), ez0);
+        T A = 0, B = 0;
+        const R* rp = rec + row.kbase;
+        const int cnt = row.count;
+        int c = 0;
+        for (; c + 4 <= cnt; c += 4) {
+            const R q0 = rp[c], q1 = rp[c + 1], q2 = rp[c + 2], q3 = rp[c + 3];
+            kforce_step(q0, e, ez1, A, B);
+            kforce_step(q1, e, ez1, A, B);
+            kforce_step(q2, e, ez1, A, B);
+            kforce_step(q3, e, ez1, A, B);
         }
+        for (; c < cnt; c++) {
+            const R q0 = rp[c];
+            kforce_step(q0, e, ez1, A, B);
+        }
+        fx += (double)(A * (tgx * (T)row.ikx));
+        fy += (double)(A * (tgy * (T)row.iky));
+        fz += (double)(B * tgz);
     }
     kspace_block_store(fx, fy, fz, n, ablock, split, fk);
 }
@@ -1370,112 +1412,6 @@ __global__ void k_phase32(const double4* __restrict__ pos, int n, int ld, Box b,
     phase_axis32(x, b.inv[axis], axis, kmax, tab, ld, a);
     if (axis == 0) q[a] = live ? (float)qa : 0.f;
 }
-
-__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-// rho partials, mixed form: FP32 phase products, FP64 accumulation; same
-// layout as k_rho_partial.
-template <int KPT>
-__global__ __launch_bounds__(kRhoBlock) void k_rho_partial32(const float2* __restrict__ tabg,
-                                                             const float* __restrict__ qg, int ld, int kmax,
-                                                             int width, const int4* __restrict__ kidx, int nk,
-                                                             double2* __restrict__ partial, int nkb, int nchunks) {
-    extern __shared__ float2 smem32[];
-    int R = phase_rows(kmax);
-    int stride = width + 1;
-    float2* tab = smem32;
-    float* q = reinterpret_cast<float*>(tab + (size_t)R * stride);
-    int bx, chunk;
-    if (!xcd_block(nkb, nchunks, bx, chunk)) return;
-    int a0 = chunk * width;
-    for (int e = threadIdx.x; e < R * width; e += blockDim.x) {
-        int row = e / width, a = e - row * width;
-        tab[row * stride + a] = tabg[(size_t)row * ld + a0 + a];
-    }
-    for (int a = threadIdx.x; a < width; a += blockDim.x) q[a] = qg[a0 + a];
-    __syncthreads();
-    int kb = bx * kRhoBlock * KPT + threadIdx.x;
-    const float2* ex[KPT];
-    const float2* ey[KPT];
-    const float2* ez[KPT];
-    double re[KPT], im[KPT];
-#pragma unroll
-    for (int j = 0; j < KPT; j++) {
-        int k = min(kb + j * kRhoBlock, nk - 1);
-        int4 id = kidx[k];
-        ex[j] = tab + phase_row(0, id.x, kmax) * stride;
-        ey[j] = tab + phase_row(1, id.y, kmax) * stride;
-        ez[j] = tab + phase_row(2, id.z, kmax) * stride;
-        re[j] = 0.0;
-        im[j] = 0.0;
-    }
-    for (int a = 0; a < width; a++) {
-        float qa = q[a];
-#pragma unroll
-        for (int j = 0; j < KPT; j++) {
-            float2 phi = cmulf(cmulf(ex[j][a], ey[j][a]), ez[j][a]);
-            re[j] += (double)(qa * phi.x);
-            im[j] += (double)(qa * phi.y);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < KPT; j++) {
-        int k = kb + j * kRhoBlock;
-        if (k < nk) partial[(size_t)chunk * nk + k] = make_double2(re[j], im[j]);
-    }
-}
-
-// Mixed form: FP32 phases, rho and field; each row's sum is added to the
-// FP64 accumulators.
-__global__ __launch_bounds__(256) void k_kspace_force32(const float2* __restrict__ tab, int ld, int n, int kmax,
-                                                        const KRow* __restrict__ rows, int nrows,
-                                                        const int4* __restrict__ kidx,
-                                                        const KRec32* __restrict__ rec, int nsplit, int nab,
-                                                        double4* __restrict__ fk) {
-    int split, ablock;  // all splits of an atom block on one XCD
-    if (!xcd_block(nsplit, nab, split, ablock)) return;
-    int i = ablock * 64 + (threadIdx.x & 63);
-    int r0, r1;
-    kspace_wave_rows(nrows, split, nsplit, r0, r1);
-    int a = i < ld ? i : ld - 1;
-    const float2* col = tab + a;
-    const float2 ez1 = col[(size_t)phase_row(2, 1, kmax) * ld];
-    double fx = 0, fy = 0, fz = 0;
-    KRow nrow = rows[r0 < r1 ? r0 : 0];
-    float2 pex = col[(size_t)phase_row(0, nrow.ikx, kmax) * ld];
-    float2 pey = col[(size_t)phase_row(1, nrow.iky, kmax) * ld];
-    float2 pez = col[(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
-    for (int r = r0; r < r1; r++) {
-        KRow row = nrow;
-        float2 ex = pex, ey = pey, ez0 = pez;
-        if (r + 1 < r1) {
-            nrow = rows[r + 1];
-            pex = col[(size_t)phase_row(0, nrow.ikx, kmax) * ld];
-            pey = col[(size_t)phase_row(1, nrow.iky, kmax) * ld];
-            pez = col[(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
-        }
-        float2 e = cmulf(cmulf(ex, ey), ez0);
-        float rx = 0.f, ry = 0.f, rz = 0.f;
-        const KRec32* rp = rec + row.kbase;
-        KRec32 cur = rp[0];
-        for (int c = 0; c < row.count; c++) {
-            KRec32 nxt = rp[c + 1 < row.count ? c + 1 : c];
-            float sv = e.y * cur.re - e.x * cur.im;
-            rx += sv * cur.fx;
-            ry += sv * cur.fy;
-            rz += sv * cur.fz;
-            e = cmulf(e, ez1);
-            cur = nxt;
-        }
-        fx += rx;
-        fy += ry;
-        fz += rz;
-    }
-    kspace_block_store(fx, fy, fz, n, ablock, split, fk);
-}
-
 
 // ---------------------------------------------------------------------------------
 // Integrators (lumol-sim/src/md/integrators.rs), FP64, unfused as in the

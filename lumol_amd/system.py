@@ -347,21 +347,21 @@ class System:
         self.precision = precision
         self._dirty = True
 
-    def set_tuning(self, pair_lanes=0, kspace_split=0, rho_kpt=0, nlist_build=0):
+    def set_tuning(self, pair_lanes=0, kspace_split=0, rho_width=0, nlist_build=0):
         """Launch knobs of the device kernels (0 = built-in choice); see
-        lumol_amd.tuning.autotune.  nlist_build: 1 lanes per atom, 2 one
-        workgroup per cell."""
-        self.tuning_ = (int(pair_lanes), int(kspace_split), int(rho_kpt), int(nlist_build))
+        lumol_amd.tuning.autotune.  rho_width: atoms per block of the rho(k)
+        kernel; nlist_build: 1 lanes per atom, 2 one workgroup per cell."""
+        self.tuning_ = (int(pair_lanes), int(kspace_split), int(rho_width), int(nlist_build))
         if self._handle is not None and not self._dirty:
             t = nat.Tuning(*self.tuning_)
             nat.check(nat.lib().lg_system_set_tuning(self._handle, ctypes.byref(t)))
 
     def tuning(self):
-        """The launch knobs in effect: dict(pair_lanes, kspace_split, rho_kpt,
+        """The launch knobs in effect: dict(pair_lanes, kspace_split, rho_width,
         nlist_build)."""
         t = nat.Tuning()
         nat.check(nat.lib().lg_system_get_tuning(self._sync(), ctypes.byref(t)))
-        return {"pair_lanes": t.pair_lanes, "kspace_split": t.kspace_split, "rho_kpt": t.rho_kpt,
+        return {"pair_lanes": t.pair_lanes, "kspace_split": t.kspace_split, "rho_width": t.rho_width,
                 "nlist_build": t.nlist_build}
 
     # -- state -----------------------------------------------------------------

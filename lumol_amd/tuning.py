@@ -1,7 +1,7 @@
 """Per-size-bucket autotuner of the device launch knobs.
 
 The knobs (``lg_tuning``): lanes per atom of the pair and neighbour-list
-kernels, row ranges of the k-space force kernel, k-vectors per thread of the
+kernels, row ranges of the k-space force kernel, atoms per block of the
 rho(k) kernel.  They are tuned one after the other: every candidate value is
 timed with the library's HIP-event profiler over `reps` force evaluations of
 the system itself -- the summed device time of all kernel classes, so that a
@@ -18,7 +18,7 @@ from . import _native as nat
 
 PAIR_LANES = (8, 16, 32, 64)
 KSPACE_SPLITS = (2, 4, 8, 16, 32)
-RHO_KPT = (1, 2, 4)
+RHO_WIDTHS = (64, 128, 256)
 
 _cache = {}
 _lock = threading.Lock()
@@ -48,31 +48,31 @@ def autotune(system, reps=5, use_cache=True):
         hit = _cache.get(key) if use_cache else None
     if hit is None:
         system.forces()  # builds the lists and tables once
-        best = {"pair_lanes": 0, "kspace_split": 0, "rho_kpt": 0}
+        best = {"pair_lanes": 0, "kspace_split": 0, "rho_width": 0}
         # pair kernel lanes
         times = {}
         for lanes in PAIR_LANES:
-            system.set_tuning(lanes, best["kspace_split"], best["rho_kpt"])
+            system.set_tuning(lanes, best["kspace_split"], best["rho_width"])
             system.forces()
             times[lanes] = _time_all(system, reps)
         best["pair_lanes"] = min(times, key=times.get)
         if key[0]:
             times = {}
             for split in KSPACE_SPLITS:
-                system.set_tuning(best["pair_lanes"], split, best["rho_kpt"])
+                system.set_tuning(best["pair_lanes"], split, best["rho_width"])
                 system.forces()
                 times[split] = _time_all(system, reps)
             best["kspace_split"] = min(times, key=times.get)
             times = {}
-            for kpt in RHO_KPT:
-                system.set_tuning(best["pair_lanes"], best["kspace_split"], kpt)
+            for width in RHO_WIDTHS:
+                system.set_tuning(best["pair_lanes"], best["kspace_split"], width)
                 system.forces()
-                times[kpt] = _time_all(system, reps)
-            best["rho_kpt"] = min(times, key=times.get)
+                times[width] = _time_all(system, reps)
+            best["rho_width"] = min(times, key=times.get)
         hit = best
         with _lock:
             _cache[key] = hit
-    system.set_tuning(hit["pair_lanes"], hit["kspace_split"], hit["rho_kpt"])
+    system.set_tuning(hit["pair_lanes"], hit["kspace_split"], hit["rho_width"])
     return dict(hit)
 
 
