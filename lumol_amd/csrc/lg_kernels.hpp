@@ -31,9 +31,15 @@ struct Grid {
     int ncell;
 };
 
-// Sets *flag when any atom moved more than skin/2 since the last build.
+// Rebuild protocol: every displacement test carries a generation number
+// `gen` (one per force evaluation, increasing) and raises *flag to it when
+// an atom moved more than skin/2 since the last build; the list kernels of
+// that evaluation rebuild when *flag == gen.  The flag never needs clearing,
+// so no kernel has to signal the end of a rebuild (a grid-wide last-block
+// handshake costs an L2 write-back per block on a multi-XCD device).
+// Forcing a rebuild = storing the next generation.
 __global__ void k_nlist_check(const double4* __restrict__ pos, const double4* __restrict__ ref, int n,
-                              double half_skin2, int* __restrict__ flag) {
+                              double half_skin2, int gen, int* __restrict__ flag) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
     if (i < n) {
@@ -42,7 +48,7 @@ __global__ void k_nlist_check(const double4* __restrict__ pos, const double4* __
         double dz = pos[i].z - ref[i].z;
         moved = dx * dx + dy * dy + dz * dz > half_skin2;
     }
-    if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+    if (__any(moved) && (threadIdx.x & 63) == 0) atomicMax(flag, gen);
 }
 
 // Coordinate wrapped into [0, L] (periodic) as the cell binning sees it.
@@ -58,8 +64,9 @@ __device__ __forceinline__ int cell_coord(double x, double L, int nc, double csi
 }
 
 __global__ void k_cell_count(const double4* __restrict__ pos, int n, Grid g, Box b, int* __restrict__ cellid,
-                             int* __restrict__ slot, int* __restrict__ count, const int* __restrict__ flag) {
-    if (*flag == 0) return;
+                             int* __restrict__ slot, int* __restrict__ count, int gen,
+                             const int* __restrict__ flag) {
+    if (*flag != gen) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 p = pos[i];
@@ -72,9 +79,9 @@ __global__ void k_cell_count(const double4* __restrict__ pos, int n, Grid g, Box
 }
 
 // Exclusive scan of the per-cell counts (one block, 1024 threads).
-__global__ void k_cell_scan(const int* __restrict__ count, int* __restrict__ start, int ncell,
+__global__ void k_cell_scan(const int* __restrict__ count, int* __restrict__ start, int ncell, int gen,
                             const int* __restrict__ flag) {
-    if (*flag == 0) return;
+    if (*flag != gen) return;
     __shared__ int warp_sums[16];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
@@ -111,9 +118,9 @@ __global__ void k_cell_scan(const int* __restrict__ count, int* __restrict__ sta
 }
 
 __global__ void k_cell_scatter(const int* __restrict__ cellid, const int* __restrict__ slot,
-                               const int* __restrict__ start, int* __restrict__ tmp, int n,
+                               const int* __restrict__ start, int* __restrict__ tmp, int n, int gen,
                                const int* __restrict__ flag) {
-    if (*flag == 0) return;
+    if (*flag != gen) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     tmp[start[cellid[i]] + slot[i]] = i;
@@ -125,8 +132,9 @@ __global__ void k_cell_scatter(const int* __restrict__ cellid, const int* __rest
 // FP32 wrapped position (the binning's own wrap) and the molecule id.
 __global__ void k_cell_rank(const int* __restrict__ tmp, const int* __restrict__ start, int* __restrict__ sorted,
                             float4* __restrict__ cpos, const double4* __restrict__ pos,
-                            const int2* __restrict__ meta, Box b, int ncell, const int* __restrict__ flag) {
-    if (*flag == 0) return;
+                            const int2* __restrict__ meta, Box b, int ncell, int gen,
+                            const int* __restrict__ flag) {
+    if (*flag != gen) return;
     int c = blockIdx.x;
     if (c >= ncell) return;
     int s = start[c], e = start[c + 1];
@@ -155,17 +163,15 @@ __global__ void k_cell_rank(const int* __restrict__ tmp, const int* __restrict__
 // holds every pair within rlist; the force kernels apply the exact cutoff.
 // Accepted ones are written in candidate order through a ballot prefix
 // count, so the row is the same whatever the lane timing.  The kernel also
-// clears the per-cell counts for the next rebuild, and its last block resets
-// the rebuild flag (flags[0]) using flags[2] as counter.
+// clears the per-cell counts for the next rebuild.
 template <int TPA>
 __global__ __launch_bounds__(256) void k_nlist_build(const float4* __restrict__ cpos, const int* __restrict__ sorted,
                                                      const int* __restrict__ cellid, const int* __restrict__ start,
                                                      int* __restrict__ count_zero, Grid g, Box b, float rl2f,
                                                      int n, int cap, int* __restrict__ nbr, int* __restrict__ nnbr,
                                                      const double4* __restrict__ pos, double4* __restrict__ ref,
-                                                     int* __restrict__ flags) {
-    if (flags[0] == 0) return;
-    __shared__ int last;
+                                                     int gen, int* __restrict__ flags) {
+    if (flags[0] != gen) return;
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
     for (int c = gid; c < g.ncell; c += gridDim.x * blockDim.x) count_zero[c] = 0;
     int t = gid / TPA;
@@ -298,14 +304,6 @@ __global__ __launch_bounds__(256) void k_nlist_build(const float4* __restrict__ 
             ref[i] = pos[i];
         }
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(flags + 2, 1) == (int)gridDim.x - 1;
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-        flags[2] = 0;
-        flags[0] = 0;
-    }
 }
 
 // Verlet list, one workgroup per home cell.  The candidate runs of the cell
@@ -318,20 +316,25 @@ __global__ __launch_bounds__(256) void k_nlist_build(const float4* __restrict__ 
 // force sums over them are the same; the acceptance test differs only in
 // where the shift is rounded, well inside the 1e-4 margin on rlist.  The
 // global loads of a whole tile are in flight together, where the per-atom
-// kernel waits on one dependent load per pass.
-constexpr int kCellBlock = 256;
+// kernel waits on one dependent load per pass; in the scan, a wave tests
+// each candidate record (one LDS read) against two home atoms at once,
+// without branches.
+constexpr int kCellBlock = 512;
 constexpr int kCellTile = 2048;  // candidates staged per tile (40 KB of LDS)
 constexpr int kCellHome = 64;    // home atoms per chunk
 constexpr int kCellMaxSeg = 128; // (2 reach + 1)^2 rows x 3 runs <= 75
 
+// kMinImage: some axis has a single cell (nearest image along it).
+template <bool kMinImage>
 __global__ __launch_bounds__(kCellBlock) void k_nlist_cells(const float4* __restrict__ cpos,
                                                             const int* __restrict__ sorted,
                                                             const int* __restrict__ start,
                                                             int* __restrict__ count_zero, Grid g, Box b, float rl2f,
                                                             int cap, int* __restrict__ nbr, int* __restrict__ nnbr,
                                                             const double4* __restrict__ pos,
-                                                            double4* __restrict__ ref, int* __restrict__ flags) {
-    if (flags[0] == 0) return;
+                                                            double4* __restrict__ ref, int gen,
+                                                            int* __restrict__ flags) {
+    if (flags[0] != gen) return;
     __shared__ float4 tile[kCellTile];
     __shared__ int tidx[kCellTile];
     __shared__ int seg_base[kCellMaxSeg + 1];
@@ -339,7 +342,6 @@ __global__ __launch_bounds__(kCellBlock) void k_nlist_cells(const float4* __rest
     __shared__ float seg_sh[kCellMaxSeg][3];
     __shared__ float4 home[kCellHome];
     __shared__ int home_cnt[kCellHome];
-    __shared__ int last;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int c = blockIdx.x * kCellBlock + tid; c < g.ncell; c += gridDim.x * kCellBlock) count_zero[c] = 0;
     const float Lx = (float)b.L[0], Ly = (float)b.L[1], Lz = (float)b.L[2];
@@ -424,14 +426,15 @@ __global__ __launch_bounds__(kCellBlock) void k_nlist_cells(const float4* __rest
                 const int tl = min(kCellTile, M - m0);
                 constexpr int R = kCellTile / kCellBlock;
                 int qv[R], kv[R];
+                // largest k with seg_base[k] <= mm: entries past the last
+                // segment hold M > mm, so no bound test is needed
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     int m = m0 + tid + r * kCellBlock;
                     int mm = m < M ? m : M - 1;
                     int k = 0;
 #pragma unroll
-                    for (int step = kCellMaxSeg / 2; step > 0; step >>= 1)
-                        if (k + step < nseg && seg_base[k + step] <= mm) k += step;
+                    for (int step = kCellMaxSeg / 2; step > 0; step >>= 1) k = seg_base[k + step] <= mm ? k + step : k;
                     kv[r] = k;
                     qv[r] = seg_s0[k] + (mm - seg_base[k]);
                 }
@@ -453,29 +456,44 @@ __global__ __launch_bounds__(kCellBlock) void k_nlist_cells(const float4* __rest
                     }
                 }
                 __syncthreads();
-                for (int h = wave; h < hn; h += kCellBlock / 64) {
-                    const float4 pi = home[h];
-                    const int mi = __float_as_int(pi.w);
-                    int cnt = home_cnt[h];
-                    int* row = nbr + (size_t)(hs + h0 + h) * cap;
+                constexpr int NW = kCellBlock / 64;
+                auto accept = [&](float4 pi, float4 cj, bool live) {
+                    float ddx = pi.x - cj.x, ddy = pi.y - cj.y, ddz = pi.z - cj.z;
+                    if (kMinImage) {
+                        if (nx1) ddx -= rintf(ddx * ix) * Lx;
+                        if (ny1) ddy -= rintf(ddy * iy) * Ly;
+                        if (nz1) ddz -= rintf(ddz * iz) * Lz;
+                    }
+                    float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    return live & (__float_as_int(cj.w) != __float_as_int(pi.w)) & (d2 < rl2f);
+                };
+                // home atoms h and h + NW of this wave together (the second
+                // one repeats the first when absent, and is not written)
+                for (int h = wave; h < hn; h += 2 * NW) {
+                    const bool two = h + NW < hn;
+                    const int hb = two ? h + NW : h;
+                    const float4 pa = home[h], pb = home[hb];
+                    int ca = home_cnt[h], cb = home_cnt[hb];
+                    int* rowa = nbr + (size_t)(hs + h0 + h) * cap;
+                    int* rowb = nbr + (size_t)(hs + h0 + hb) * cap;
 #pragma unroll 2
                     for (int p = 0; p < tl; p += 64) {
-                        int e = p + lane;
-                        float4 cj = tile[e < tl ? e : 0];
-                        bool accept = false;
-                        if (e < tl && __float_as_int(cj.w) != mi) {
-                            float ddx = pi.x - cj.x, ddy = pi.y - cj.y, ddz = pi.z - cj.z;
-                            if (nx1) ddx -= rintf(ddx * ix) * Lx;
-                            if (ny1) ddy -= rintf(ddy * iy) * Ly;
-                            if (nz1) ddz -= rintf(ddz * iz) * Lz;
-                            accept = ddx * ddx + ddy * ddy + ddz * ddz < rl2f;
-                        }
-                        unsigned long long bal = __ballot(accept);
-                        int slot = cnt + __popcll(bal & below);
-                        if (accept && slot < cap) row[slot] = tidx[e];
-                        cnt += __popcll(bal);
+                        const int e = p + lane;  // < kCellTile: tl <= kCellTile, a multiple of 64
+                        const float4 cj = tile[e];
+                        const int jj = tidx[e];
+                        const bool live = e < tl;
+                        const bool aa = accept(pa, cj, live), ab = accept(pb, cj, live) & two;
+                        const unsigned long long ba = __ballot(aa), bb = __ballot(ab);
+                        const int sa = ca + __popcll(ba & below), sb = cb + __popcll(bb & below);
+                        if (aa && sa < cap) rowa[sa] = jj;
+                        if (ab && sb < cap) rowb[sb] = jj;
+                        ca += __popcll(ba);
+                        cb += __popcll(bb);
                     }
-                    if (lane == 0) home_cnt[h] = cnt;
+                    if (lane == 0) {
+                        home_cnt[h] = ca;
+                        if (two) home_cnt[hb] = cb;
+                    }
                 }
                 __syncthreads();
             }
@@ -492,14 +510,6 @@ __global__ __launch_bounds__(kCellBlock) void k_nlist_cells(const float4* __rest
             }
             __syncthreads();
         }
-    }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) last = atomicAdd(flags + 2, 1) == (int)gridDim.x - 1;
-    __syncthreads();
-    if (last && tid == 0) {
-        flags[2] = 0;
-        flags[0] = 0;
     }
 }
 
@@ -1454,12 +1464,13 @@ __device__ __forceinline__ double berendsen_lambda(const Thermo& th) {
 
 // VelocityVerlet first half (integrators.rs:47-53): the pending Berendsen
 // scale of the previous step (th.ke_part != null), then v += (dt/2) a;
-// x += v dt; the displacement test of the neighbour list (sets flags[0]);
+// x += v dt; the displacement test of the neighbour list (raises flags[0]
+// to gen);
 // and the phase table of the new position when `ph.mode` != 0.  Block of 256.
 __global__ __launch_bounds__(256) void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel,
                                                   const double4* __restrict__ acc, int n, double dt, Thermo th,
                                                   const double4* __restrict__ ref, double half_skin2,
-                                                  int* __restrict__ flags, PhaseOut ph) {
+                                                  int gen, int* __restrict__ flags, PhaseOut ph) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
     double l = th.ke_part ? berendsen_lambda(th) : 1.0;
@@ -1484,7 +1495,7 @@ __global__ __launch_bounds__(256) void k_vv_first(double4* __restrict__ pos, dou
         pos[i] = p;
         moved = moved_beyond(p, ref, i, half_skin2);
     }
-    if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flags, 1);
+    if (__any(moved) && (threadIdx.x & 63) == 0) atomicMax(flags, gen);
     if (ph.mode && i < ph.ld) phase_atom(ph, i, i < n, p);
 }
 
@@ -1494,7 +1505,7 @@ __global__ __launch_bounds__(256) void k_vv_first(double4* __restrict__ pos, dou
 __global__ __launch_bounds__(256) void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel,
                                                   const double4* __restrict__ acc, int n, double dt, Thermo th,
                                                   const double4* __restrict__ ref, double half_skin2,
-                                                  int* __restrict__ flags, PhaseOut ph) {
+                                                  int gen, int* __restrict__ flags, PhaseOut ph) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
     double l = th.ke_part ? berendsen_lambda(th) : 1.0;
@@ -1516,7 +1527,7 @@ __global__ __launch_bounds__(256) void k_lf_first(double4* __restrict__ pos, dou
         pos[i] = p;
         moved = moved_beyond(p, ref, i, half_skin2);
     }
-    if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flags, 1);
+    if (__any(moved) && (threadIdx.x & 63) == 0) atomicMax(flags, gen);
     if (ph.mode && i < ph.ld) phase_atom(ph, i, i < n, p);
 }
 
