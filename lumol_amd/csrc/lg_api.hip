@@ -1259,7 +1259,7 @@ int lg_system_create(lg_context* ctx, size_t natoms, lg_system** out) {
     int r;
     if ((r = s->pos.ensure(natoms)) || (r = s->vel.ensure(natoms)) || (r = s->acc.ensure(natoms)) ||
         (r = s->force.ensure(natoms)) || (r = s->lambda.ensure(1)) ||
-        (r = s->ke_part.ensure(size_t(blocks_for(natoms, 256))))) {
+        (r = s->ke_part.ensure(size_t(blocks_for(natoms, 64))))) {
         delete s;
         return r;
     }
@@ -1727,8 +1727,9 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
         LG_HIP(hipMemcpyAsync(s->snap_acc.p, s->acc.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
         bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
         double hs = 0.5 * s->skin;
-        int nb = blocks_for(n, 256);
+        int nb = blocks_for(n, 64);  // second half: one wave per block, one KE partial each
         // the first half-step also writes the phase table of the new positions
+        // (64 atoms per block, ld atoms with the padding)
         PhaseOut ph{};
         if (kspace) {
             ph.mode = s->precision == LG_PRECISION_MIXED ? 2 : 1;
@@ -1741,16 +1742,17 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
             ph.q32 = s->phase_q32.p;
         }
         Thermo th{nullptr, nb, double(3 * s->n), s->berendsen_T, s->berendsen_tau};
+        int nbf = blocks_for(std::max(n, kspace ? s->ld : 0), 64);
         for (uint64_t k = 0; k < todo; k++) {
             // the Berendsen scale of the previous step is applied by the first half
             th.ke_part = (thermo && k > 0) ? s->ke_part.p : nullptr;
             const int gen = next_gen(s);
             long p1 = prof_start(s, LG_KERNEL_INTEGRATE);
             if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
-                k_vv_first<<<nb, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
+                k_vv_first<<<nbf, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
                                                gen, s->flags.p, ph);
             else
-                k_lf_first<<<nb, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
+                k_lf_first<<<nbf, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
                                                gen, s->flags.p, ph);
             prof_stop(s, p1);
             LG_TRY(launch_nlist(s, false, gen));
@@ -1760,10 +1762,10 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
             int ks = kspace ? s->ksplit : 0;
             const int* sl = s->nterms > 0 ? s->slot_start.p : nullptr;
             if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
-                k_md_second<false><<<nb, 256, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
+                k_md_second<false><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
                                                        s->vel.p, s->acc.p, n, s->dt, kp);
             else
-                k_md_second<true><<<nb, 256, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
+                k_md_second<true><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
                                                       s->vel.p, s->acc.p, n, s->dt, kp);
             prof_stop(s, p2);
         }

@@ -1462,88 +1462,128 @@ __device__ __forceinline__ double berendsen_lambda(const Thermo& th) {
     return lam;
 }
 
+// Phase-table part of a first half-step: role r (1..3) of the block writes
+// the rows of axis r - 1 for the block's 64 new positions, staged in LDS by
+// role 0 (charges with axis 0).  Padding atoms (n <= i < ld) get x = 0 and
+// q = 0, as k_phase.
+__device__ __forceinline__ void phase_roles(const PhaseOut& ph, int i, int n, int role, int lane,
+                                            const double (&px)[4][64]) {
+    if (role == 0 || i >= ph.ld) return;
+    const int axis = role - 1;
+    const double x = i < n ? px[axis][lane] : 0.0;
+    if (ph.mode == 1) {
+        phase_axis64(x, ph.box.inv[axis], axis, ph.kmax, ph.tab, ph.ld, i);
+        if (axis == 0) ph.q[i] = i < n ? px[3][lane] : 0.0;
+    } else {
+        phase_axis32(x, ph.box.inv[axis], axis, ph.kmax, ph.tab32, ph.ld, i);
+        if (axis == 0) ph.q32[i] = i < n ? (float)px[3][lane] : 0.f;
+    }
+}
+
 // VelocityVerlet first half (integrators.rs:47-53): the pending Berendsen
 // scale of the previous step (th.ke_part != null), then v += (dt/2) a;
 // x += v dt; the displacement test of the neighbour list (raises flags[0]
-// to gen);
-// and the phase table of the new position when `ph.mode` != 0.  Block of 256.
+// to gen); and the phase table of the new position when `ph.mode` != 0.
+// Block = 64 atoms x 4 roles: role 0 integrates, roles 1-3 write the phase
+// rows of one axis each, so that the trigonometry of an atom runs on three
+// waves.  Grid: ceil(max(n, ld) / 64).
 __global__ __launch_bounds__(256) void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel,
                                                   const double4* __restrict__ acc, int n, double dt, Thermo th,
                                                   const double4* __restrict__ ref, double half_skin2,
                                                   int gen, int* __restrict__ flags, PhaseOut ph) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool moved = false;
+    __shared__ double px[4][64];
+    const int lane = threadIdx.x & 63, role = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
     double l = th.ke_part ? berendsen_lambda(th) : 1.0;
-    double4 p = make_double4(0, 0, 0, 0);
-    if (i < n) {
-        double4 v = vel[i];
-        if (th.ke_part) {
-            v.x = v.x * l;
-            v.y = v.y * l;
-            v.z = v.z * l;
+    if (role == 0) {
+        bool moved = false;
+        double4 p = make_double4(0, 0, 0, 0);
+        if (i < n) {
+            double4 v = vel[i];
+            if (th.ke_part) {
+                v.x = v.x * l;
+                v.y = v.y * l;
+                v.z = v.z * l;
+            }
+            double4 a = acc[i];
+            double h = 0.5 * dt;
+            v.x += h * a.x;
+            v.y += h * a.y;
+            v.z += h * a.z;
+            p = pos[i];
+            p.x += v.x * dt;
+            p.y += v.y * dt;
+            p.z += v.z * dt;
+            vel[i] = v;
+            pos[i] = p;
+            moved = moved_beyond(p, ref, i, half_skin2);
         }
-        double4 a = acc[i];
-        double h = 0.5 * dt;
-        v.x += h * a.x;
-        v.y += h * a.y;
-        v.z += h * a.z;
-        p = pos[i];
-        p.x += v.x * dt;
-        p.y += v.y * dt;
-        p.z += v.z * dt;
-        vel[i] = v;
-        pos[i] = p;
-        moved = moved_beyond(p, ref, i, half_skin2);
+        if (__any(moved) && lane == 0) atomicMax(flags, gen);
+        px[0][lane] = p.x;
+        px[1][lane] = p.y;
+        px[2][lane] = p.z;
+        px[3][lane] = p.w;
     }
-    if (__any(moved) && (threadIdx.x & 63) == 0) atomicMax(flags, gen);
-    if (ph.mode && i < ph.ld) phase_atom(ph, i, i < n, p);
+    if (!ph.mode) return;  // uniform: a kernel argument
+    __syncthreads();
+    phase_roles(ph, i, n, role, lane, px);
 }
 
 // LeapFrog (integrators.rs:150-168), position part, after the pending
 // Berendsen scale of the previous step, with the displacement test and the
-// phase table as k_vv_first.  Block of 256.
+// phase table as k_vv_first (same block layout).
 __global__ __launch_bounds__(256) void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel,
                                                   const double4* __restrict__ acc, int n, double dt, Thermo th,
                                                   const double4* __restrict__ ref, double half_skin2,
                                                   int gen, int* __restrict__ flags, PhaseOut ph) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool moved = false;
+    __shared__ double px[4][64];
+    const int lane = threadIdx.x & 63, role = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
     double l = th.ke_part ? berendsen_lambda(th) : 1.0;
-    double4 p = make_double4(0, 0, 0, 0);
-    if (i < n) {
-        double4 v = vel[i];
-        if (th.ke_part) {
-            v.x = v.x * l;
-            v.y = v.y * l;
-            v.z = v.z * l;
-            vel[i] = v;
+    if (role == 0) {
+        bool moved = false;
+        double4 p = make_double4(0, 0, 0, 0);
+        if (i < n) {
+            double4 v = vel[i];
+            if (th.ke_part) {
+                v.x = v.x * l;
+                v.y = v.y * l;
+                v.z = v.z * l;
+                vel[i] = v;
+            }
+            double4 a = acc[i];
+            double dt2 = dt * dt;
+            p = pos[i];
+            p.x += v.x * dt + (0.5 * a.x) * dt2;
+            p.y += v.y * dt + (0.5 * a.y) * dt2;
+            p.z += v.z * dt + (0.5 * a.z) * dt2;
+            pos[i] = p;
+            moved = moved_beyond(p, ref, i, half_skin2);
         }
-        double4 a = acc[i];
-        double dt2 = dt * dt;
-        p = pos[i];
-        p.x += v.x * dt + (0.5 * a.x) * dt2;
-        p.y += v.y * dt + (0.5 * a.y) * dt2;
-        p.z += v.z * dt + (0.5 * a.z) * dt2;
-        pos[i] = p;
-        moved = moved_beyond(p, ref, i, half_skin2);
+        if (__any(moved) && lane == 0) atomicMax(flags, gen);
+        px[0][lane] = p.x;
+        px[1][lane] = p.y;
+        px[2][lane] = p.z;
+        px[3][lane] = p.w;
     }
-    if (__any(moved) && (threadIdx.x & 63) == 0) atomicMax(flags, gen);
-    if (ph.mode && i < ph.ld) phase_atom(ph, i, i < n, p);
+    if (!ph.mode) return;
+    __syncthreads();
+    phase_roles(ph, i, n, role, lane, px);
 }
 
 // Second half of VelocityVerlet (integrators.rs:55-68) or LeapFrog
 // (:160-167).  The total force is the pair/intra-molecular force plus the
 // k-space partials q_i / (4 pi eps0) sum_g fk[g][i] (ewald.rs:716-719).
 // With a thermostat (ke_part != null), block partials of the kinetic energy
-// are written for k_berendsen.
+// are written for k_berendsen.  Blocks of 64 atoms (one wave), so that the
+// loads of a small system spread over many CUs.
 template <bool kLeapFrog>
-__global__ __launch_bounds__(256) void k_md_second(const double4* __restrict__ force, const double4* __restrict__ pos,
+__global__ __launch_bounds__(64) void k_md_second(const double4* __restrict__ force, const double4* __restrict__ pos,
                                                    const double4* __restrict__ contrib,
                                                    const int* __restrict__ slot_start,
                                                    const double4* __restrict__ fk, int ksplit,
                                                    double4* __restrict__ vel, double4* __restrict__ acc, int n,
                                                    double dt, double* __restrict__ ke_part) {
-    __shared__ double red[4];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ke = 0.0;
     if (i < n) {
@@ -1568,9 +1608,7 @@ __global__ __launch_bounds__(256) void k_md_second(const double4* __restrict__ f
     }
     if (!ke_part) return;
     double s = wave_sum(ke);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) ke_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) ke_part[blockIdx.x] = s;
 }
 
 // Berendsen factor of the last step of a run into *lambda (one block, same
