@@ -816,27 +816,36 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     long prof = prof_start(s, LG_KERNEL_PAIRS);
     bool mixed = s->precision == LG_PRECISION_MIXED;
     int pblocks = blocks_for(size_t(n) * s->tpa, kPairBlock);
-    switch (s->tpa) {
-#define LG_PAIRS_CASE(T)                                                   \
-    case T:                                                                \
-        if (mixed && energy)                                               \
-            k_pairs_mixed<true, T><<<pblocks, kPairBlock, 0, st>>>(pa);    \
-        else if (mixed)                                                    \
-            k_pairs_mixed<false, T><<<pblocks, kPairBlock, 0, st>>>(pa);   \
-        else if (energy)                                                   \
-            k_pairs<true, T><<<pblocks, kPairBlock, 0, st>>>(pa);          \
-        else                                                               \
-            k_pairs<false, T><<<pblocks, kPairBlock, 0, st>>>(pa);         \
+    const int ck = coulomb_variant(s->coul);
+#define LG_PAIRS_CK(T, CK)                                                      \
+    if (ck == CK) {                                                             \
+        if (mixed && energy)                                                    \
+            k_pairs_mixed<true, T, CK><<<pblocks, kPairBlock, 0, st>>>(pa);     \
+        else if (mixed)                                                         \
+            k_pairs_mixed<false, T, CK><<<pblocks, kPairBlock, 0, st>>>(pa);    \
+        else if (energy)                                                        \
+            k_pairs<true, T, CK><<<pblocks, kPairBlock, 0, st>>>(pa);           \
+        else                                                                    \
+            k_pairs<false, T, CK><<<pblocks, kPairBlock, 0, st>>>(pa);          \
+    }
+#define LG_PAIRS_CASE(T)                 \
+    case T:                              \
+        LG_PAIRS_CK(T, CK_NONE)          \
+        LG_PAIRS_CK(T, CK_EWALD)         \
+        LG_PAIRS_CK(T, CK_EWALD_ERF)     \
+        LG_PAIRS_CK(T, CK_WOLF)          \
         break;
+    switch (s->tpa) {
         LG_PAIRS_CASE(4)
         LG_PAIRS_CASE(8)
         LG_PAIRS_CASE(16)
         LG_PAIRS_CASE(32)
         LG_PAIRS_CASE(64)
-#undef LG_PAIRS_CASE
         default:
             return fail(LG_E_DEVICE, "invalid lanes-per-atom setting");
     }
+#undef LG_PAIRS_CASE
+#undef LG_PAIRS_CK
     prof_stop(s, prof);
     if (s->nterms > 0) {
         IntraArgs ia;

@@ -540,9 +540,22 @@ struct PairArgs {
 constexpr int kMaxKinds = 16;
 constexpr int kPairBlock = 256;
 
+// Coulomb variant of a pair kernel instance, fixed per system: none, Ewald
+// real space (erfc), Ewald for inter-molecular pairs that the restriction
+// excludes (the -erf removal term only), Wolf.  A Wolf system whose
+// restriction excludes the pairs uses CK_NONE.  One variant per instance
+// keeps the registers of the unused paths out of the kernel.
+enum : int { CK_NONE = 0, CK_EWALD = 1, CK_EWALD_ERF = 2, CK_WOLF = 3 };
+
+__host__ __forceinline__ int coulomb_variant(const CoulParam& c) {
+    if (c.kind == COUL_EWALD) return c.inter_excl ? CK_EWALD_ERF : CK_EWALD;
+    if (c.kind == COUL_WOLF) return c.inter_excl ? CK_NONE : CK_WOLF;
+    return CK_NONE;
+}
+
 // TPA lanes share one atom i and stride through its neighbour row; the
 // group's partial forces are combined with xor-shuffles (TPA divides 64).
-template <bool kEnergy, int TPA>
+template <bool kEnergy, int TPA, int CK>
 __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
     __shared__ PairParam tab[kMaxKinds * kMaxKinds];
     __shared__ double red[kPairBlock / 64][8];
@@ -584,17 +597,14 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
                 e_p = e - pp.shift;
             }
             double qiqj = qi * pj.w;
-            if (c.kind != COUL_NONE && qiqj != 0.0 && r <= c.rc) {
+            if (CK != CK_NONE && qiqj != 0.0 && r <= c.rc) {
                 double e, f;
-                if (c.kind == COUL_EWALD) {
-                    ewald_real(c.alpha, qiqj, r, inv_r, c.inter_excl != 0, e, f);
-                    fs += f;
-                    e_c = e;
-                } else if (!c.inter_excl) {
+                if (CK == CK_WOLF)
                     wolf_pair(c, qiqj, r, inv_r, e, f);
-                    fs += f;
-                    e_c = e;
-                }
+                else
+                    ewald_real(c.alpha, qiqj, r, inv_r, CK == CK_EWALD_ERF, e, f);
+                fs += f;
+                e_c = e;
             }
             fx += fs * dx;
             fy += fs * dy;
@@ -640,7 +650,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
 // FP32 (erfc from erfc_cheb); the per-atom force is accumulated in FP32 with
 // compensated summation and stored in FP64.  Energies and virial (energy
 // mode) are accumulated in FP64.
-template <bool kEnergy, int TPA>
+template <bool kEnergy, int TPA, int CK>
 __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
     struct P32 {
         int pot, inter_excl;
@@ -700,21 +710,19 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
                 }
             }
             float qiqj = qi * (float)pj.w;
-            if (a.coul.kind != COUL_NONE && qiqj != 0.f && r2 <= rcc2) {
+            if (CK != CK_NONE && qiqj != 0.f && r2 <= rcc2) {
                 float r = r2 * inv_r;
                 float ex2;
                 float erfcv = erfc_cheb(alpha * r, ex2);
                 float pref = qiqj * inv4pe;
-                if (a.coul.kind == COUL_EWALD) {
-                    if (a.coul.inter_excl) {
-                        float erfv = 1.f - erfcv;
-                        fs += pref * inv_r2 * (g0 * ex2 - erfv * inv_r);
-                        if (kEnergy) e_c = -pref * erfv * inv_r;
-                    } else {
-                        fs += pref * inv_r2 * (g0 * ex2 + erfcv * inv_r);
-                        if (kEnergy) e_c = pref * erfcv * inv_r;
-                    }
-                } else if (!a.coul.inter_excl) {
+                if (CK == CK_EWALD_ERF) {
+                    float erfv = 1.f - erfcv;
+                    fs += pref * inv_r2 * (g0 * ex2 - erfv * inv_r);
+                    if (kEnergy) e_c = -pref * erfv * inv_r;
+                } else if (CK == CK_EWALD) {
+                    fs += pref * inv_r2 * (g0 * ex2 + erfcv * inv_r);
+                    if (kEnergy) e_c = pref * erfcv * inv_r;
+                } else {
                     float factor = erfcv * inv_r2 + g0 * ex2 * inv_r;
                     fs += pref * (factor - wf) * inv_r;
                     if (kEnergy) e_c = pref * (erfcv * inv_r - we);
