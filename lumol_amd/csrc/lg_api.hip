@@ -752,12 +752,20 @@ int launch_nlist(lg_system* s, bool check, int gen) {
         k_nlist_check<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->ref.p, n, hs * hs, gen, flag);
     }
     const Grid& g = s->grid;
-    k_cell_count<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, n, g, s->box, s->cellid.p, s->slot.p, s->count.p,
-                                                      gen, flag);
-    k_cell_scan<<<1, 1024, 0, st>>>(s->count.p, s->start.p, g.ncell, gen, flag);
-    k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(s->cellid.p, s->slot.p, s->start.p, s->tmp.p, n, gen, flag);
-    k_cell_rank<<<g.ncell, 256, 0, st>>>(s->tmp.p, s->start.p, s->sorted.p, s->cpos.p, s->pos.p, s->meta.p,
-                                         s->box, g.ncell, gen, flag);
+    // small systems: the whole cell list in one workgroup
+    size_t small_lds = size_t(8) * g.ncell + 4 + size_t(8) * n;
+    if (g.ncell < 65536 && n <= 32768 && small_lds <= s->ctx->max_shared) {
+        k_cell_small<<<1, 1024, small_lds, st>>>(s->pos.p, s->meta.p, n, g, s->box, s->cellid.p, s->start.p,
+                                                 s->sorted.p, s->cpos.p, gen, flag);
+    } else {
+        k_cell_count<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, n, g, s->box, s->cellid.p, s->slot.p,
+                                                          s->count.p, gen, flag);
+        k_cell_scan<<<1, 1024, 0, st>>>(s->count.p, s->start.p, g.ncell, gen, flag);
+        k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(s->cellid.p, s->slot.p, s->start.p, s->tmp.p, n, gen,
+                                                            flag);
+        k_cell_rank<<<g.ncell, 256, 0, st>>>(s->tmp.p, s->start.p, s->sorted.p, s->cpos.p, s->pos.p, s->meta.p,
+                                             s->box, g.ncell, gen, flag);
+    }
     int nb = blocks_for(size_t(n) * s->tpa, 256);
     // FP32 candidate test with a 1e-4 relative margin on rlist
     float rl2f = (!s->periodic || s->rlist <= 0) ? INFINITY : float(s->rlist * (1.0 + 1e-4) * s->rlist * (1.0 + 1e-4));

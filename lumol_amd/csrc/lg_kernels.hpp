@@ -150,6 +150,95 @@ __global__ void k_cell_rank(const int* __restrict__ tmp, const int* __restrict__
     }
 }
 
+// Cell list of a small system in one workgroup of 1024 threads: the counts,
+// offsets, scatter and per-cell rank order of the four-kernel chain above
+// (k_cell_count .. k_cell_rank), with the counters, the slot table and the
+// atoms' cells in LDS (dynamic: 8 ncell + 4 + 8 n bytes).  Same outputs:
+// cellid[n], start[ncell + 1], sorted[n], cpos[n]; one launch instead of
+// four, which is also what a step without a rebuild pays.  ncell < 65536.
+__global__ __launch_bounds__(1024) void k_cell_small(const double4* __restrict__ pos, const int2* __restrict__ meta,
+                                                     int n, Grid g, Box b, int* __restrict__ cellid,
+                                                     int* __restrict__ start, int* __restrict__ sorted,
+                                                     float4* __restrict__ cpos, int gen, const int* __restrict__ flag) {
+    if (*flag != gen) return;
+    extern __shared__ int smem_cells[];
+    const int nc = g.ncell;
+    int* cnt = smem_cells;                                         // [nc]: counts, then slot counters
+    int* off = cnt + nc;                                           // [nc + 1]
+    int* tmp = off + nc + 1;                                       // [n]: atom of each slot
+    unsigned short* slot_cell = reinterpret_cast<unsigned short*>(tmp + n);  // [n]
+    unsigned short* atom_cell = slot_cell + n;                               // [n]
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (int c = tid; c < nc; c += 1024) cnt[c] = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += 1024) {
+        double4 p = pos[i];
+        int cx = cell_coord(p.x, b.L[0], g.n[0], g.size[0], b.periodic);
+        int cy = cell_coord(p.y, b.L[1], g.n[1], g.size[1], b.periodic);
+        int cz = cell_coord(p.z, b.L[2], g.n[2], g.size[2], b.periodic);
+        int c = (cz * g.n[1] + cy) * g.n[0] + cx;
+        cellid[i] = c;
+        atom_cell[i] = (unsigned short)c;
+        atomicAdd(&cnt[c], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the counts: `per` consecutive cells per thread
+    const int per = (nc + 1023) / 1024;
+    const int c0 = tid * per, c1 = min(c0 + per, nc);
+    int local = 0;
+    for (int c = c0; c < c1; c++) local += cnt[c];
+    int x = local;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int w = 0; w < 16; w++) {
+            int t = wsum[w];
+            wsum[w] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+    int run = wsum[wid] + x - local;
+    for (int c = c0; c < c1; c++) {
+        int k = cnt[c];
+        off[c] = run;
+        start[c] = run;
+        cnt[c] = run;
+        run += k;
+    }
+    if (tid == 0) {
+        off[nc] = n;
+        start[nc] = n;
+    }
+    __syncthreads();
+    // scatter in arbitrary order within a cell ...
+    for (int i = tid; i < n; i += 1024) {
+        int c = atom_cell[i];
+        int sl = atomicAdd(&cnt[c], 1);
+        tmp[sl] = i;
+        slot_cell[sl] = (unsigned short)c;
+    }
+    __syncthreads();
+    // ... made deterministic: atoms by increasing index within each cell
+    for (int sl = tid; sl < n; sl += 1024) {
+        int v = tmp[sl], c = slot_cell[sl];
+        int o0 = off[c], o1 = off[c + 1];
+        int rank = 0;
+        for (int q = o0; q < o1; q++) rank += tmp[q] < v;
+        sorted[o0 + rank] = v;
+        double4 p = pos[v];
+        cpos[o0 + rank] = make_float4((float)wrap_coord(p.x, b.L[0], b.periodic),
+                                      (float)wrap_coord(p.y, b.L[1], b.periodic),
+                                      (float)wrap_coord(p.z, b.L[2], b.periodic), __int_as_float(meta[v].y));
+    }
+}
+
 // Verlet list: TPA lanes build the row of atom sorted[t] together.  For each
 // (dz, dy) of the neighbour cells, the cells along x form at most three runs
 // of consecutive cell indices (split where the periodic wrap is crossed), and
