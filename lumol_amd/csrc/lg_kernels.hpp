@@ -667,10 +667,16 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
         int nn = a.nnbr[t];
         const CoulParam& c = a.coul;
         const int* row = a.nbr + (size_t)t * a.cap;
-        for (int k = sub; k < nn; k += TPA) {
-            int j = row[k];
-            double4 pj = a.pos[j];
-            int kj = a.meta[j].x;
+        // Software pipeline over the row: the row entries two passes ahead
+        // and the next pass's neighbour records are loaded while the
+        // current pair is evaluated, unrolled by two on alternating
+        // registers (A, B).  Entries past the row end read a clamped index
+        // and stand for atom i itself, never evaluated.
+        auto jat = [&](int k) {
+            int v = row[min(k, max(nn - 1, 0))];
+            return k < nn ? v : i;
+        };
+        auto pair = [&](int j, const double4& pj, int kj) {
             double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             min_image(a.box, dx, dy, dz);
             double r2 = norm2_ordered(dx, dy, dz);
@@ -708,6 +714,21 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
                 wxz += fs * dx * dz;
                 wyz += fs * dy * dz;
             }
+        };
+        int k = sub;
+        int jA = jat(k), jB = jat(k + TPA);
+        double4 pA = a.pos[jA], pB = a.pos[jB];
+        int kA = a.meta[jA].x, kB = a.meta[jB].x;
+        for (; k < nn; k += 2 * TPA) {
+            const int jA2 = jat(k + 2 * TPA), jB2 = jat(k + 3 * TPA);
+            pair(jA, pA, kA);
+            pA = a.pos[jA2];
+            kA = a.meta[jA2].x;
+            if (k + TPA < nn) pair(jB, pB, kB);
+            pB = a.pos[jB2];
+            kB = a.meta[jB2].x;
+            jA = jA2;
+            jB = jB2;
         }
     }
 #pragma unroll
@@ -774,10 +795,16 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
         float qi = (float)pi.w;
         int nn = a.nnbr[t];
         const int* row = a.nbr + (size_t)t * a.cap;
-        for (int k = sub; k < nn; k += TPA) {
-            int j = row[k];
-            double4 pj = a.pos[j];
-            int kj = a.meta[j].x;
+        // Software pipeline over the row: the row entries two passes ahead
+        // and the next pass's neighbour records are loaded while the
+        // current pair is evaluated, unrolled by two on alternating
+        // registers (A, B).  Entries past the row end read a clamped index
+        // and stand for atom i itself, never evaluated.
+        auto jat = [&](int k) {
+            int v = row[min(k, max(nn - 1, 0))];
+            return k < nn ? v : i;
+        };
+        auto pair = [&](int j, const double4& pj, int kj) {
             double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
             min_image(a.box, ddx, ddy, ddz);
             float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
@@ -831,6 +858,21 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
                 wxz += f * ddx * ddz;
                 wyz += f * ddy * ddz;
             }
+        };
+        int k = sub;
+        int jA = jat(k), jB = jat(k + TPA);
+        double4 pA = a.pos[jA], pB = a.pos[jB];
+        int kA = a.meta[jA].x, kB = a.meta[jB].x;
+        for (; k < nn; k += 2 * TPA) {
+            const int jA2 = jat(k + 2 * TPA), jB2 = jat(k + 3 * TPA);
+            pair(jA, pA, kA);
+            pA = a.pos[jA2];
+            kA = a.meta[jA2].x;
+            if (k + TPA < nn) pair(jB, pB, kB);
+            pB = a.pos[jB2];
+            kB = a.meta[jB2].x;
+            jA = jA2;
+            jB = jB2;
         }
     }
     double sx = (double)fx + (double)cx, sy = (double)fy + (double)cy, sz = (double)fz + (double)cz;
