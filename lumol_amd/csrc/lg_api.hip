@@ -154,7 +154,6 @@ struct lg_system {
     Grid grid{};
     Box box{};
     double rlist = 0, skin = 0;
-    int nl_gen = 0;  // generation of the last displacement test (rebuild protocol of k_nlist_check)
     int cap = 0;
     DevBuf<int> cellid, slot, count, start, tmp, sorted, nbr, nnbr, flags;  // flags[0] rebuild, [1] overflow
     DevBuf<float4> cpos;  // list-build candidate records in sorted order
@@ -669,8 +668,7 @@ int prepare_nlist(lg_system* s) {
     s->tpa = n <= 8192 ? 64 : n <= 32768 ? 32 : n <= 131072 ? 16 : n <= 524288 ? 8 : 4;
     if (s->tune.pair_lanes > 0) s->tpa = s->tune.pair_lanes;
     LG_TRY(s->epart_pairs.ensure(size_t(blocks_for(n * size_t(s->tpa), kPairBlock)) * 8));
-    s->nl_gen = 0;
-    int init[4] = {1, 0, 0, 0};  // force a rebuild (generation 1 is next); clear overflow
+    int init[4] = {1, 0, 0, 0};  // force a rebuild; clear overflow
     LG_HIP(hipMemcpyAsync(s->flags.p, init, sizeof(init), hipMemcpyHostToDevice, S(s)));
     LG_HIP(hipMemsetAsync(s->count.p, 0, g.ncell * sizeof(int), S(s)));
     s->nlist_dirty = false;
@@ -725,46 +723,36 @@ static int rho_chunk(const lg_system* s) {
     return w;
 }
 
-// Generation of the next displacement test; wraps long before int overflow
-// (the restart at 1 forces one rebuild).
-static int next_gen(lg_system* s) {
-    if (s->nl_gen >= (1 << 30)) {
-        s->nl_gen = 0;
-        k_set_flag<<<1, 1, 0, S(s)>>>(s->flags.p, 1);
-    }
-    return ++s->nl_gen;
-}
-
-// The next displacement test rebuilds whatever it finds.
+// The next force evaluation rebuilds the list whatever the displacements.
 static int force_rebuild(lg_system* s) {
-    k_set_flag<<<1, 1, 0, S(s)>>>(s->flags.p, s->nl_gen + 1);
+    k_set_flag<<<1, 1, 0, S(s)>>>(s->flags.p, 1);
     LG_HIP(hipGetLastError());
     return LG_OK;
 }
 
-int launch_nlist(lg_system* s, bool check, int gen) {
+int launch_nlist(lg_system* s, bool check) {
     hipStream_t st = S(s);
     long prof = prof_start(s, LG_KERNEL_NLIST);
     int n = int(s->n);
     int* flag = s->flags.p;
     if (check) {
         double hs = 0.5 * s->skin;
-        k_nlist_check<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->ref.p, n, hs * hs, gen, flag);
+        k_nlist_check<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->ref.p, n, hs * hs, flag);
     }
     const Grid& g = s->grid;
     // small systems: the whole cell list in one workgroup
     size_t small_lds = size_t(8) * g.ncell + 4 + size_t(8) * n;
     if (g.ncell < 65536 && n <= 32768 && small_lds <= s->ctx->max_shared) {
         k_cell_small<<<1, 1024, small_lds, st>>>(s->pos.p, s->meta.p, n, g, s->box, s->cellid.p, s->start.p,
-                                                 s->sorted.p, s->cpos.p, gen, flag);
+                                                 s->sorted.p, s->cpos.p, flag);
     } else {
         k_cell_count<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, n, g, s->box, s->cellid.p, s->slot.p,
-                                                          s->count.p, gen, flag);
-        k_cell_scan<<<1, 1024, 0, st>>>(s->count.p, s->start.p, g.ncell, gen, flag);
-        k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(s->cellid.p, s->slot.p, s->start.p, s->tmp.p, n, gen,
+                                                          s->count.p, flag);
+        k_cell_scan<<<1, 1024, 0, st>>>(s->count.p, s->start.p, g.ncell, flag);
+        k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(s->cellid.p, s->slot.p, s->start.p, s->tmp.p, n,
                                                             flag);
         k_cell_rank<<<g.ncell, 256, 0, st>>>(s->tmp.p, s->start.p, s->sorted.p, s->cpos.p, s->pos.p, s->meta.p,
-                                             s->box, g.ncell, gen, flag);
+                                             s->box, g.ncell, flag);
     }
     int nb = blocks_for(size_t(n) * s->tpa, 256);
     // FP32 candidate test with a 1e-4 relative margin on rlist
@@ -774,11 +762,11 @@ int launch_nlist(lg_system* s, bool check, int gen) {
         if (nearest)
             k_nlist_cells<true><<<g.ncell, kCellBlock, 0, st>>>(s->cpos.p, s->sorted.p, s->start.p, s->count.p, g,
                                                                 s->box, rl2f, s->cap, s->nbr.p, s->nnbr.p, s->pos.p,
-                                                                s->ref.p, gen, flag);
+                                                                s->ref.p, flag);
         else
             k_nlist_cells<false><<<g.ncell, kCellBlock, 0, st>>>(s->cpos.p, s->sorted.p, s->start.p, s->count.p, g,
                                                                  s->box, rl2f, s->cap, s->nbr.p, s->nnbr.p, s->pos.p,
-                                                                 s->ref.p, gen, flag);
+                                                                 s->ref.p, flag);
         prof_stop(s, prof);
         LG_HIP(hipGetLastError());
         return LG_OK;
@@ -788,7 +776,7 @@ int launch_nlist(lg_system* s, bool check, int gen) {
     case T:                                                                                                \
         k_nlist_build<T><<<nb, 256, 0, st>>>(s->cpos.p, s->sorted.p, s->cellid.p, s->start.p, s->count.p, g,  \
                                              s->box, rl2f, n, s->cap, s->nbr.p, s->nnbr.p, s->pos.p, s->ref.p,  \
-                                             gen, flag);                                                      \
+                                             flag);                                                           \
         break;
         LG_BUILD_CASE(4)
         LG_BUILD_CASE(8)
@@ -821,6 +809,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     pa.n = n;
     pa.force = s->force.p;
     pa.epart = s->epart_pairs.p;
+    pa.rebuild = s->flags.p;
     long prof = prof_start(s, LG_KERNEL_PAIRS);
     bool mixed = s->precision == LG_PRECISION_MIXED;
     int pblocks = blocks_for(size_t(n) * s->tpa, kPairBlock);
@@ -958,7 +947,7 @@ int check_overflow(lg_system* s, bool* overflowed) {
 int compute(lg_system* s, bool energy) {
     LG_TRY(prepare(s));
     for (int attempt = 0; attempt < 4; attempt++) {
-        LG_TRY(launch_nlist(s, true, next_gen(s)));
+        LG_TRY(launch_nlist(s, true));
         LG_TRY(launch_forces(s, energy));
         bool ov = false;
         LG_TRY(check_overflow(s, &ov));
@@ -1763,16 +1752,15 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
         for (uint64_t k = 0; k < todo; k++) {
             // the Berendsen scale of the previous step is applied by the first half
             th.ke_part = (thermo && k > 0) ? s->ke_part.p : nullptr;
-            const int gen = next_gen(s);
             long p1 = prof_start(s, LG_KERNEL_INTEGRATE);
             if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
                 k_vv_first<<<nbf, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
-                                               gen, s->flags.p, ph);
+                                               s->flags.p, ph);
             else
                 k_lf_first<<<nbf, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
-                                               gen, s->flags.p, ph);
+                                               s->flags.p, ph);
             prof_stop(s, p1);
-            LG_TRY(launch_nlist(s, false, gen));
+            LG_TRY(launch_nlist(s, false));
             LG_TRY(launch_forces(s, false, false, true));
             long p2 = prof_start(s, LG_KERNEL_INTEGRATE);
             double* kp = thermo ? s->ke_part.p : nullptr;

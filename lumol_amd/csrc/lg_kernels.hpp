@@ -31,15 +31,16 @@ struct Grid {
     int ncell;
 };
 
-// Rebuild protocol: every displacement test carries a generation number
-// `gen` (one per force evaluation, increasing) and raises *flag to it when
-// an atom moved more than skin/2 since the last build; the list kernels of
-// that evaluation rebuild when *flag == gen.  The flag never needs clearing,
-// so no kernel has to signal the end of a rebuild (a grid-wide last-block
-// handshake costs an L2 write-back per block on a multi-XCD device).
-// Forcing a rebuild = storing the next generation.
+// Rebuild protocol: a displacement test (here, or in the MD first half-step)
+// sets *flag when an atom moved more than skin/2 since the last build; the
+// list kernels that follow rebuild when it is set, and the pair kernel,
+// which always runs after them and before the next test, clears it.  Stream
+// order does all the signalling: no kernel has to detect the end of a
+// rebuild (a grid-wide last-block handshake costs an L2 write-back per block
+// on a multi-XCD device), and a step captured in a HIP graph needs no
+// per-step arguments.
 __global__ void k_nlist_check(const double4* __restrict__ pos, const double4* __restrict__ ref, int n,
-                              double half_skin2, int gen, int* __restrict__ flag) {
+                              double half_skin2, int* __restrict__ flag) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
     if (i < n) {
@@ -48,7 +49,7 @@ __global__ void k_nlist_check(const double4* __restrict__ pos, const double4* __
         double dz = pos[i].z - ref[i].z;
         moved = dx * dx + dy * dy + dz * dz > half_skin2;
     }
-    if (__any(moved) && (threadIdx.x & 63) == 0) atomicMax(flag, gen);
+    if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
 }
 
 // Coordinate wrapped into [0, L] (periodic) as the cell binning sees it.
@@ -64,9 +65,8 @@ __device__ __forceinline__ int cell_coord(double x, double L, int nc, double csi
 }
 
 __global__ void k_cell_count(const double4* __restrict__ pos, int n, Grid g, Box b, int* __restrict__ cellid,
-                             int* __restrict__ slot, int* __restrict__ count, int gen,
-                             const int* __restrict__ flag) {
-    if (*flag != gen) return;
+                             int* __restrict__ slot, int* __restrict__ count, const int* __restrict__ flag) {
+    if (*flag == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 p = pos[i];
@@ -79,9 +79,8 @@ __global__ void k_cell_count(const double4* __restrict__ pos, int n, Grid g, Box
 }
 
 // Exclusive scan of the per-cell counts (one block, 1024 threads).
-__global__ void k_cell_scan(const int* __restrict__ count, int* __restrict__ start, int ncell, int gen,
-                            const int* __restrict__ flag) {
-    if (*flag != gen) return;
+__global__ void k_cell_scan(const int* __restrict__ count, int* __restrict__ start, int ncell, const int* __restrict__ flag) {
+    if (*flag == 0) return;
     __shared__ int warp_sums[16];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
@@ -118,9 +117,8 @@ __global__ void k_cell_scan(const int* __restrict__ count, int* __restrict__ sta
 }
 
 __global__ void k_cell_scatter(const int* __restrict__ cellid, const int* __restrict__ slot,
-                               const int* __restrict__ start, int* __restrict__ tmp, int n, int gen,
-                               const int* __restrict__ flag) {
-    if (*flag != gen) return;
+                               const int* __restrict__ start, int* __restrict__ tmp, int n, const int* __restrict__ flag) {
+    if (*flag == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     tmp[start[cellid[i]] + slot[i]] = i;
@@ -132,9 +130,8 @@ __global__ void k_cell_scatter(const int* __restrict__ cellid, const int* __rest
 // FP32 wrapped position (the binning's own wrap) and the molecule id.
 __global__ void k_cell_rank(const int* __restrict__ tmp, const int* __restrict__ start, int* __restrict__ sorted,
                             float4* __restrict__ cpos, const double4* __restrict__ pos,
-                            const int2* __restrict__ meta, Box b, int ncell, int gen,
-                            const int* __restrict__ flag) {
-    if (*flag != gen) return;
+                            const int2* __restrict__ meta, Box b, int ncell, const int* __restrict__ flag) {
+    if (*flag == 0) return;
     int c = blockIdx.x;
     if (c >= ncell) return;
     int s = start[c], e = start[c + 1];
@@ -159,8 +156,8 @@ __global__ void k_cell_rank(const int* __restrict__ tmp, const int* __restrict__
 __global__ __launch_bounds__(1024) void k_cell_small(const double4* __restrict__ pos, const int2* __restrict__ meta,
                                                      int n, Grid g, Box b, int* __restrict__ cellid,
                                                      int* __restrict__ start, int* __restrict__ sorted,
-                                                     float4* __restrict__ cpos, int gen, const int* __restrict__ flag) {
-    if (*flag != gen) return;
+                                                     float4* __restrict__ cpos, const int* __restrict__ flag) {
+    if (*flag == 0) return;
     extern __shared__ int smem_cells[];
     const int nc = g.ncell;
     int* cnt = smem_cells;                                         // [nc]: counts, then slot counters
@@ -259,8 +256,8 @@ __global__ __launch_bounds__(256) void k_nlist_build(const float4* __restrict__ 
                                                      int* __restrict__ count_zero, Grid g, Box b, float rl2f,
                                                      int n, int cap, int* __restrict__ nbr, int* __restrict__ nnbr,
                                                      const double4* __restrict__ pos, double4* __restrict__ ref,
-                                                     int gen, int* __restrict__ flags) {
-    if (flags[0] != gen) return;
+                                                     int* __restrict__ flags) {
+    if (flags[0] == 0) return;
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
     for (int c = gid; c < g.ncell; c += gridDim.x * blockDim.x) count_zero[c] = 0;
     int t = gid / TPA;
@@ -421,9 +418,9 @@ __global__ __launch_bounds__(kCellBlock) void k_nlist_cells(const float4* __rest
                                                             int* __restrict__ count_zero, Grid g, Box b, float rl2f,
                                                             int cap, int* __restrict__ nbr, int* __restrict__ nnbr,
                                                             const double4* __restrict__ pos,
-                                                            double4* __restrict__ ref, int gen,
+                                                            double4* __restrict__ ref,
                                                             int* __restrict__ flags) {
-    if (flags[0] != gen) return;
+    if (flags[0] == 0) return;
     __shared__ float4 tile[kCellTile];
     __shared__ int tidx[kCellTile];
     __shared__ int seg_base[kCellMaxSeg + 1];
@@ -624,6 +621,7 @@ struct PairArgs {
     int n;
     double4* force;
     double* epart;  // per block: e_pairs, e_coul, w_xx, w_yy, w_zz, w_xy, w_xz, w_yz
+    int* rebuild;   // neighbour-list rebuild flag, cleared here (see k_nlist_check)
 };
 
 constexpr int kMaxKinds = 16;
@@ -648,6 +646,7 @@ template <bool kEnergy, int TPA, int CK>
 __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
     __shared__ PairParam tab[kMaxKinds * kMaxKinds];
     __shared__ double red[kPairBlock / 64][8];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.rebuild) *a.rebuild = 0;
     int nk2 = a.nkinds * a.nkinds;
     for (int x = threadIdx.x; x < nk2; x += blockDim.x) tab[x] = a.table[x];
     __syncthreads();
@@ -768,6 +767,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
     };
     __shared__ P32 tab[kMaxKinds * kMaxKinds];
     __shared__ double red[kPairBlock / 64][8];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.rebuild) *a.rebuild = 0;
     int nk2 = a.nkinds * a.nkinds;
     for (int x = threadIdx.x; x < nk2; x += blockDim.x) {
         PairParam pp = a.table[x];
@@ -1621,15 +1621,14 @@ __device__ __forceinline__ void phase_roles(const PhaseOut& ph, int i, int n, in
 
 // VelocityVerlet first half (integrators.rs:47-53): the pending Berendsen
 // scale of the previous step (th.ke_part != null), then v += (dt/2) a;
-// x += v dt; the displacement test of the neighbour list (raises flags[0]
-// to gen); and the phase table of the new position when `ph.mode` != 0.
+// x += v dt; the displacement test of the neighbour list (sets flags[0]); and the phase table of the new position when `ph.mode` != 0.
 // Block = 64 atoms x 4 roles: role 0 integrates, roles 1-3 write the phase
 // rows of one axis each, so that the trigonometry of an atom runs on three
 // waves.  Grid: ceil(max(n, ld) / 64).
 __global__ __launch_bounds__(256) void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel,
                                                   const double4* __restrict__ acc, int n, double dt, Thermo th,
                                                   const double4* __restrict__ ref, double half_skin2,
-                                                  int gen, int* __restrict__ flags, PhaseOut ph) {
+                                                  int* __restrict__ flags, PhaseOut ph) {
     __shared__ double px[4][64];
     const int lane = threadIdx.x & 63, role = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
@@ -1657,7 +1656,7 @@ __global__ __launch_bounds__(256) void k_vv_first(double4* __restrict__ pos, dou
             pos[i] = p;
             moved = moved_beyond(p, ref, i, half_skin2);
         }
-        if (__any(moved) && lane == 0) atomicMax(flags, gen);
+        if (__any(moved) && lane == 0) atomicOr(flags, 1);
         px[0][lane] = p.x;
         px[1][lane] = p.y;
         px[2][lane] = p.z;
@@ -1674,7 +1673,7 @@ __global__ __launch_bounds__(256) void k_vv_first(double4* __restrict__ pos, dou
 __global__ __launch_bounds__(256) void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel,
                                                   const double4* __restrict__ acc, int n, double dt, Thermo th,
                                                   const double4* __restrict__ ref, double half_skin2,
-                                                  int gen, int* __restrict__ flags, PhaseOut ph) {
+                                                  int* __restrict__ flags, PhaseOut ph) {
     __shared__ double px[4][64];
     const int lane = threadIdx.x & 63, role = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
@@ -1699,7 +1698,7 @@ __global__ __launch_bounds__(256) void k_lf_first(double4* __restrict__ pos, dou
             pos[i] = p;
             moved = moved_beyond(p, ref, i, half_skin2);
         }
-        if (__any(moved) && lane == 0) atomicMax(flags, gen);
+        if (__any(moved) && lane == 0) atomicOr(flags, 1);
         px[0][lane] = p.x;
         px[1][lane] = p.y;
         px[2][lane] = p.z;
