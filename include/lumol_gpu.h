@@ -238,6 +238,11 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
  *   nlist_build   neighbour-list build: 1 lanes per atom, 2 one workgroup per
  *                 cell with candidates staged in LDS (default: cells when
  *                 there are >= 128 cells and >= 8 atoms per cell)
+ *   md_graph      lg_md_run: 1 (default) launches every kernel; 2 captures a
+ *                 chunk of steps in a HIP graph once and replays it (eager
+ *                 launches while a kernel profile is active).  On ROCm 7.0
+ *                 the replay measured 4-18% slower than eager launches, so
+ *                 it is off by default.
  * lumol_amd.tuning picks them per size bucket by timing the candidates.
  * Results do not depend on them beyond summation order. */
 typedef struct lg_tuning {
@@ -245,6 +250,7 @@ typedef struct lg_tuning {
     int kspace_split;
     int rho_width;
     int nlist_build;
+    int md_graph;
 } lg_tuning;
 int lg_system_set_tuning(lg_system* sys, const lg_tuning* tuning);
 /* The values in effect (after the automatic choice); kspace_split and

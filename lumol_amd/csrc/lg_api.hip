@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -48,12 +49,18 @@ int fail(int code, const std::string& msg) {
         if (r_ != LG_OK) return r_; \
     } while (0)
 
+// Bumped whenever any device buffer is (re)allocated or freed: a captured
+// HIP graph holds buffer addresses, so it is replayed only while this and
+// the system's configuration epoch are unchanged.
+static std::atomic<uint64_t> g_alloc_epoch{0};
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t cap = 0;
     int ensure(size_t count) {
         if (count <= cap && p) return LG_OK;
+        g_alloc_epoch.fetch_add(1);
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
@@ -64,7 +71,10 @@ struct DevBuf {
         return LG_OK;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            g_alloc_epoch.fetch_add(1);
+            (void)hipFree(p);
+        }
         p = nullptr;
         cap = 0;
     }
@@ -211,6 +221,10 @@ struct lg_system {
     DevBuf<int> mc_cell_count, mc_atom_cell;
     // snapshot for neighbour-list overflow recovery
     DevBuf<double4> snap_pos, snap_vel, snap_acc;
+    // MD chunk captured as a HIP graph, with what it was captured for
+    hipGraphExec_t md_exec = nullptr;
+    uint64_t md_key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t cfg_epoch = 0;  // bumped by prepare() whenever anything was re-derived
     // kernel timing (lg_profile_*)
     uint32_t prof_mask = 0;
     std::vector<hipEvent_t> prof_events;
@@ -679,6 +693,7 @@ int prepare_nlist(lg_system* s) {
 int prepare(lg_system* s) {
     if (s->n == 0) return fail(LG_E_INVALID, "empty system");
     if (s->topo_dirty || s->meta_dirty || s->params_dirty || s->kspace_dirty || s->nlist_dirty) {
+        s->cfg_epoch++;
         s->mc_ready = false;
         s->mc_staged = false;
         s->mc_cells_valid = false;
@@ -1301,6 +1316,8 @@ int lg_system_destroy(lg_system* s) {
                             &s->ke_part, &s->lambda};
     for (auto* b : d1) b->release();
     for (auto e : s->prof_events) (void)hipEventDestroy(e);
+    if (s->md_exec) (void)hipGraphExecDestroy(s->md_exec);
+    s->md_exec = nullptr;
     s->meta.release();
     s->table.release();
     s->terms.release();
@@ -1459,6 +1476,7 @@ int lg_system_set_tuning(lg_system* s, const lg_tuning* t) {
     if (t->rho_width != 0 && t->rho_width != 64 && t->rho_width != 128 && t->rho_width != 256 && t->rho_width != 512)
         return fail(LG_E_INVALID, "rho_width must be 0, 64, 128, 256 or 512");
     if (t->nlist_build < 0 || t->nlist_build > 2) return fail(LG_E_INVALID, "nlist_build must be 0, 1 or 2");
+    if (t->md_graph < 0 || t->md_graph > 2) return fail(LG_E_INVALID, "md_graph must be 0, 1 or 2");
     s->tune = *t;
     s->kspace_dirty = s->nlist_dirty = true;
     return LG_OK;
@@ -1472,6 +1490,7 @@ int lg_system_get_tuning(lg_system* s, lg_tuning* t) {
     t->pair_lanes = s->tpa;
     t->kspace_split = (s->coul_kind == COUL_EWALD && s->nk > 0) ? s->ksplit : 0;
     t->nlist_build = cell_build(s) ? 2 : 1;
+    t->md_graph = s->tune.md_graph == 2 ? 2 : 1;
     t->rho_width = (s->coul_kind == COUL_EWALD && s->nk > 0) ? rho_chunk(s) : 0;
     return LG_OK;
 }
@@ -1709,6 +1728,71 @@ int lg_md_setup(lg_system* s, int integrator, double timestep, double bT, double
     return LG_OK;
 }
 
+// The kernels of `todo` MD steps and, with a thermostat, the Berendsen
+// scale at the end: launched eagerly, or captured into a HIP graph by
+// lg_md_run.  No allocation, copy, memset or synchronisation in between.
+static int md_chunk(lg_system* s, uint64_t todo) {
+    hipStream_t st = S(s);
+    const int n = int(s->n);
+    const bool thermo = s->berendsen_T > 0.0;
+    const bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
+    double hs = 0.5 * s->skin;
+    int nb = blocks_for(n, 64);  // second half: one wave per block, one KE partial each
+    // the first half-step also writes the phase table of the new positions
+    // (64 atoms per block, ld atoms with the padding)
+    PhaseOut ph{};
+    if (kspace) {
+        ph.mode = s->precision == LG_PRECISION_MIXED ? 2 : 1;
+        ph.kmax = s->coul_kmax;
+        ph.ld = s->ld;
+        ph.box = s->box;
+        ph.tab = s->phase_tab.p;
+        ph.q = s->phase_q.p;
+        ph.tab32 = s->phase_tab32.p;
+        ph.q32 = s->phase_q32.p;
+    }
+    Thermo th{nullptr, nb, double(3 * s->n), s->berendsen_T, s->berendsen_tau};
+    int nbf = blocks_for(std::max(n, kspace ? s->ld : 0), 64);
+    for (uint64_t k = 0; k < todo; k++) {
+        // the Berendsen scale of the previous step is applied by the first half
+        th.ke_part = (thermo && k > 0) ? s->ke_part.p : nullptr;
+        long p1 = prof_start(s, LG_KERNEL_INTEGRATE);
+        if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
+            k_vv_first<<<nbf, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
+                                           s->flags.p, ph);
+        else
+            k_lf_first<<<nbf, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
+                                           s->flags.p, ph);
+        prof_stop(s, p1);
+        LG_TRY(launch_nlist(s, false));
+        LG_TRY(launch_forces(s, false, false, true));
+        long p2 = prof_start(s, LG_KERNEL_INTEGRATE);
+        double* kp = thermo ? s->ke_part.p : nullptr;
+        int ks = kspace ? s->ksplit : 0;
+        const int* sl = s->nterms > 0 ? s->slot_start.p : nullptr;
+        if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
+            k_md_second<false><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
+                                                   s->vel.p, s->acc.p, n, s->dt, kp);
+        else
+            k_md_second<true><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
+                                                  s->vel.p, s->acc.p, n, s->dt, kp);
+        prof_stop(s, p2);
+    }
+    if (thermo) {
+        th.ke_part = s->ke_part.p;
+        k_berendsen<<<1, 256, 0, st>>>(th, s->lambda.p);
+        k_scale_vel<<<blocks_for(n, 256), 256, 0, st>>>(s->vel.p, n, s->lambda.p);
+    }
+    LG_HIP(hipGetLastError());
+    return LG_OK;
+}
+
+static uint64_t bits_of(double x) {
+    uint64_t u;
+    std::memcpy(&u, &x, sizeof(u));
+    return u;
+}
+
 // Steps in chunks; a chunk whose neighbour list overflowed is replayed from a
 // snapshot with a larger list.
 int lg_md_run(lg_system* s, uint64_t nsteps) {
@@ -1718,8 +1802,6 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
     LG_TRY(prepare(s));
     s->mc_ready = s->mc_staged = s->mc_cells_valid = false;
     hipStream_t st = S(s);
-    int n = int(s->n);
-    bool thermo = s->berendsen_T > 0.0;
     const uint64_t chunk = 200;
     LG_TRY(s->snap_pos.ensure(s->n));
     LG_TRY(s->snap_vel.ensure(s->n));
@@ -1731,53 +1813,36 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
         LG_HIP(hipMemcpyAsync(s->snap_pos.p, s->pos.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
         LG_HIP(hipMemcpyAsync(s->snap_vel.p, s->vel.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
         LG_HIP(hipMemcpyAsync(s->snap_acc.p, s->acc.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
-        bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
-        double hs = 0.5 * s->skin;
-        int nb = blocks_for(n, 64);  // second half: one wave per block, one KE partial each
-        // the first half-step also writes the phase table of the new positions
-        // (64 atoms per block, ld atoms with the padding)
-        PhaseOut ph{};
-        if (kspace) {
-            ph.mode = s->precision == LG_PRECISION_MIXED ? 2 : 1;
-            ph.kmax = s->coul_kmax;
-            ph.ld = s->ld;
-            ph.box = s->box;
-            ph.tab = s->phase_tab.p;
-            ph.q = s->phase_q.p;
-            ph.tab32 = s->phase_tab32.p;
-            ph.q32 = s->phase_q32.p;
-        }
-        Thermo th{nullptr, nb, double(3 * s->n), s->berendsen_T, s->berendsen_tau};
-        int nbf = blocks_for(std::max(n, kspace ? s->ld : 0), 64);
-        for (uint64_t k = 0; k < todo; k++) {
-            // the Berendsen scale of the previous step is applied by the first half
-            th.ke_part = (thermo && k > 0) ? s->ke_part.p : nullptr;
-            long p1 = prof_start(s, LG_KERNEL_INTEGRATE);
-            if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
-                k_vv_first<<<nbf, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
-                                               s->flags.p, ph);
-            else
-                k_lf_first<<<nbf, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
-                                               s->flags.p, ph);
-            prof_stop(s, p1);
-            LG_TRY(launch_nlist(s, false));
-            LG_TRY(launch_forces(s, false, false, true));
-            long p2 = prof_start(s, LG_KERNEL_INTEGRATE);
-            double* kp = thermo ? s->ke_part.p : nullptr;
-            int ks = kspace ? s->ksplit : 0;
-            const int* sl = s->nterms > 0 ? s->slot_start.p : nullptr;
-            if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
-                k_md_second<false><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
-                                                       s->vel.p, s->acc.p, n, s->dt, kp);
-            else
-                k_md_second<true><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
-                                                      s->vel.p, s->acc.p, n, s->dt, kp);
-            prof_stop(s, p2);
-        }
-        if (thermo) {
-            th.ke_part = s->ke_part.p;
-            k_berendsen<<<1, 256, 0, st>>>(th, s->lambda.p);
-            k_scale_vel<<<blocks_for(n, 256), 256, 0, st>>>(s->vel.p, n, s->lambda.p);
+        const bool graph = s->tune.md_graph == 2 && s->prof_mask == 0;
+        if (graph) {
+            // captured once per (chunk length, configuration, buffer set,
+            // integrator settings) and replayed
+            const uint64_t key[8] = {todo, s->cfg_epoch, g_alloc_epoch.load(), uint64_t(s->integrator),
+                                     uint64_t(s->precision), bits_of(s->dt), bits_of(s->berendsen_T),
+                                     bits_of(s->berendsen_tau)};
+            if (!s->md_exec || !std::equal(key, key + 8, s->md_key)) {
+                if (s->md_exec) (void)hipGraphExecDestroy(s->md_exec);
+                s->md_exec = nullptr;
+                hipGraph_t g = nullptr;
+                LG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+                int rc = md_chunk(s, todo);
+                hipError_t e = hipStreamEndCapture(st, &g);
+                if (rc != LG_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (e != hipSuccess) return fail(LG_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+                e = hipGraphInstantiate(&s->md_exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) {
+                    s->md_exec = nullptr;
+                    return fail(LG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+                }
+                std::copy(key, key + 8, s->md_key);
+            }
+            LG_HIP(hipGraphLaunch(s->md_exec, st));
+        } else {
+            LG_TRY(md_chunk(s, todo));
         }
         LG_HIP(hipGetLastError());
         bool ov = false;

@@ -44,6 +44,7 @@ def _time_all(system, reps):
 def autotune(system, reps=5, use_cache=True):
     """Picks and applies the knobs for `system`; returns them (dict)."""
     key = bucket(system)
+    keep = tuple(getattr(system, "tuning_", (0, 0, 0, 0, 0))[3:])  # nlist_build, md_graph: left as set
     with _lock:
         hit = _cache.get(key) if use_cache else None
     if hit is None:
@@ -52,27 +53,27 @@ def autotune(system, reps=5, use_cache=True):
         # pair kernel lanes
         times = {}
         for lanes in PAIR_LANES:
-            system.set_tuning(lanes, best["kspace_split"], best["rho_width"])
+            system.set_tuning(lanes, best["kspace_split"], best["rho_width"], *keep)
             system.forces()
             times[lanes] = _time_all(system, reps)
         best["pair_lanes"] = min(times, key=times.get)
         if key[0]:
             times = {}
             for split in KSPACE_SPLITS:
-                system.set_tuning(best["pair_lanes"], split, best["rho_width"])
+                system.set_tuning(best["pair_lanes"], split, best["rho_width"], *keep)
                 system.forces()
                 times[split] = _time_all(system, reps)
             best["kspace_split"] = min(times, key=times.get)
             times = {}
             for width in RHO_WIDTHS:
-                system.set_tuning(best["pair_lanes"], best["kspace_split"], width)
+                system.set_tuning(best["pair_lanes"], best["kspace_split"], width, *keep)
                 system.forces()
                 times[width] = _time_all(system, reps)
             best["rho_width"] = min(times, key=times.get)
         hit = best
         with _lock:
             _cache[key] = hit
-    system.set_tuning(hit["pair_lanes"], hit["kspace_split"], hit["rho_width"])
+    system.set_tuning(hit["pair_lanes"], hit["kspace_split"], hit["rho_width"], *keep)
     return dict(hit)
 
 
