@@ -390,6 +390,7 @@ def bench_spce_batched(args):
         s = spce(seed=seed)
         s.stream_slot = slot
         s.set_precision(precision)
+        s.set_tuning(md_graph=args.md_graph)
         return s
 
     # FP64, one system alone
@@ -428,7 +429,7 @@ def bench_spce_batched(args):
         "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded grid + jitter)",
-        "config": {"workload": WORKLOADS["spce-batched"], "batch": args.batch, "form": "mixed",
+        "config": {"workload": WORKLOADS["spce-batched"], "batch": args.batch, "form": "mixed", "md_graph": args.md_graph,
                    "atoms_per_system": systems[0].size()},
         "error_vs_fp64_oracle": err,
         "fp64_single_system": fp64_rate,
@@ -598,6 +599,8 @@ def main():
     ap.add_argument("--slice", type=int, default=4096, help="mc: attempts per chain per launch")
     ap.add_argument("--atoms", type=int, default=1_000_000, help="ka: atoms")
     ap.add_argument("--batch", type=int, default=4, help="spce-batched: systems at once")
+    ap.add_argument("--md-graph", type=int, default=0, choices=[0, 1, 2],
+                    help="spce-batched: 2 replays MD chunks from HIP graphs, 1 launches every kernel")
     ap.add_argument("--md-jobs", type=int, default=256, help="serve: MD jobs")
     ap.add_argument("--md-lanes", type=int, default=4, help="serve: concurrent MD jobs (HIP streams)")
     ap.add_argument("--steps-scale", type=float, default=0.02, help="serve: factor on the 1k..20k MD steps")

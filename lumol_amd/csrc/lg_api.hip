@@ -825,20 +825,43 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     pa.force = s->force.p;
     pa.epart = s->epart_pairs.p;
     pa.rebuild = s->flags.p;
+    // the intra-molecular terms run as the last blocks of the same launch
+    IntraArgs& ia = pa.intra;
+    ia = IntraArgs{};
+    ia.pos = s->pos.p;
+    ia.meta = s->meta.p;
+    ia.nterms = s->nterms;
+    ia.terms = s->terms.p;
+    ia.table = s->table.p;
+    ia.nkinds = int(s->nkinds);
+    ia.pair_restr = s->pair_restr.p;
+    ia.pair_s14 = s->pair_s14.p;
+    ia.bparam = s->bparam.p;
+    ia.aparam = s->aparam.p;
+    ia.dparam = s->dparam.p;
+    ia.coul = s->coul;
+    ia.coul_restr = s->coul_restr;
+    ia.coul_s14 = s->coul_s14;
+    ia.box = s->box;
+    ia.contrib = s->contrib.p;
+    ia.epart = s->epart_mol.p;
+    const int iblocks = s->nterms > 0 ? blocks_for(s->nterms, 256) : 0;
     long prof = prof_start(s, LG_KERNEL_PAIRS);
     bool mixed = s->precision == LG_PRECISION_MIXED;
-    int pblocks = blocks_for(size_t(n) * s->tpa, kPairBlock);
+    const int pblocks = blocks_for(size_t(n) * s->tpa, kPairBlock);
+    pa.pair_blocks = pblocks;
+    const int nblocks = pblocks + iblocks;
     const int ck = coulomb_variant(s->coul);
 #define LG_PAIRS_CK(T, CK)                                                      \
     if (ck == CK) {                                                             \
         if (mixed && energy)                                                    \
-            k_pairs_mixed<true, T, CK><<<pblocks, kPairBlock, 0, st>>>(pa);     \
+            k_pairs_mixed<true, T, CK><<<nblocks, kPairBlock, 0, st>>>(pa);     \
         else if (mixed)                                                         \
-            k_pairs_mixed<false, T, CK><<<pblocks, kPairBlock, 0, st>>>(pa);    \
+            k_pairs_mixed<false, T, CK><<<nblocks, kPairBlock, 0, st>>>(pa);    \
         else if (energy)                                                        \
-            k_pairs<true, T, CK><<<pblocks, kPairBlock, 0, st>>>(pa);           \
+            k_pairs<true, T, CK><<<nblocks, kPairBlock, 0, st>>>(pa);           \
         else                                                                    \
-            k_pairs<false, T, CK><<<pblocks, kPairBlock, 0, st>>>(pa);          \
+            k_pairs<false, T, CK><<<nblocks, kPairBlock, 0, st>>>(pa);          \
     }
 #define LG_PAIRS_CASE(T)                 \
     case T:                              \
@@ -859,33 +882,6 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
 #undef LG_PAIRS_CASE
 #undef LG_PAIRS_CK
     prof_stop(s, prof);
-    if (s->nterms > 0) {
-        IntraArgs ia;
-        ia.pos = s->pos.p;
-        ia.meta = s->meta.p;
-        ia.nterms = s->nterms;
-        ia.terms = s->terms.p;
-        ia.table = s->table.p;
-        ia.nkinds = int(s->nkinds);
-        ia.pair_restr = s->pair_restr.p;
-        ia.pair_s14 = s->pair_s14.p;
-        ia.bparam = s->bparam.p;
-        ia.aparam = s->aparam.p;
-        ia.dparam = s->dparam.p;
-        ia.coul = s->coul;
-        ia.coul_restr = s->coul_restr;
-        ia.coul_s14 = s->coul_s14;
-        ia.box = s->box;
-        ia.contrib = s->contrib.p;
-        ia.epart = s->epart_mol.p;
-        int nb = blocks_for(s->nterms, 256);
-        long pm = prof_start(s, LG_KERNEL_MOLECULES);
-        if (energy)
-            k_intra_terms<true><<<nb, 256, 0, st>>>(ia);
-        else
-            k_intra_terms<false><<<nb, 256, 0, st>>>(ia);
-        prof_stop(s, pm);
-    }
     if (s->coul_kind == COUL_EWALD && s->nk > 0) {
         int kmax = s->coul_kmax;
         int nub = blocks_for(s->nunits, 64);

@@ -601,6 +601,223 @@ __global__ __launch_bounds__(kCellBlock) void k_nlist_cells(const float4* __rest
 
 
 // ---------------------------------------------------------------------------------
+// Intra-molecular terms: one thread per term (same-molecule pair with its bond
+// path, bond, angle or dihedral), compute.rs:60-97, energy.rs:88-138,
+// ewald.rs:387-428 (excluded-pair correction), cells.rs:335-411.  Each term
+// writes its force on each of its atoms to a slot reserved for that
+// (term, atom); the consumer adds an atom's slots in slot order, so the sums
+// are deterministic without atomics.  Terms are sorted by type on the host so
+// that waves do not diverge.
+// ---------------------------------------------------------------------------------
+struct BondedParam {
+    int pot;  // POT_* (POT_NONE: no potential)
+    double p0, p1;
+};
+
+enum : int { TERM_PAIR = 0, TERM_BOND = 1, TERM_ANGLE = 2, TERM_DIHEDRAL = 3 };
+
+struct Term {
+    int4 atoms;  // pair/bond: x, y; angle: x, y, z; dihedral: x, y, z, w
+    int4 slots;  // contribution slot of each atom
+    int type;    // TERM_*
+    int param;   // pair: bond path; bond/angle/dihedral: parameter index
+};
+
+struct IntraArgs {
+    const double4* pos;
+    const int2* meta;
+    int nterms;
+    const Term* terms;
+    const PairParam* table;
+    int nkinds;
+    const int* pair_restr;  // restriction per kind pair
+    const double* pair_s14; // scale14 per kind pair
+    const BondedParam* bparam;
+    const BondedParam* aparam;
+    const BondedParam* dparam;
+    CoulParam coul;
+    int coul_restr;
+    double coul_s14;
+    Box box;
+    double4* contrib;  // slot contributions
+    double* epart;     // per block: pairs, coul, bonds, angles, dihedrals, w6
+};
+
+// One block (256 threads) of intra-molecular terms, block index `blk`: run as
+// its own kernel or as the extra blocks of a pair-kernel launch.
+template <bool kEnergy>
+__device__ __forceinline__ void intra_block(const IntraArgs& a, int blk) {
+    __shared__ double red[4][11];
+    int t = blk * blockDim.x + threadIdx.x;
+    double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (t < a.nterms) {
+        Term tm = a.terms[t];
+        if (tm.type == TERM_PAIR) {
+            int i = tm.atoms.x, j = tm.atoms.y, path = tm.param;
+            double4 pi = a.pos[i], pj = a.pos[j];
+            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            min_image(a.box, dx, dy, dz);
+            double r = sqrt(norm2_ordered(dx, dy, dz));
+            double inv_r = 1.0 / r;
+            int tk = a.meta[i].x * a.nkinds + a.meta[j].x;
+            const PairParam& pp = a.table[tk];
+            double fs = 0.0;
+            if (pp.pot != POT_NONE) {
+                bool excl;
+                double scaling;
+                restriction_info(a.pair_restr[tk], a.pair_s14[tk], path, excl, scaling);
+                if (!excl && r < pp.rc && pp.pot != POT_NULL) {
+                    double e, f;
+                    pot_eval(pp.pot, pp.p0, pp.p1, r, inv_r, e, f);
+                    fs += scaling * f;
+                    acc[0] += scaling * (e - pp.shift);
+                }
+            }
+            double qiqj = pi.w * pj.w;
+            if (a.coul.kind != COUL_NONE && qiqj != 0.0) {
+                bool excl;
+                double scaling;
+                restriction_info(a.coul_restr, a.coul_s14, path, excl, scaling);
+                if (r <= a.coul.rc) {
+                    double e, f;
+                    if (a.coul.kind == COUL_EWALD) {
+                        ewald_real(a.coul.alpha, qiqj, r, inv_r, excl, e, f);
+                        fs += f;
+                        acc[1] += e;
+                    } else if (!excl) {
+                        wolf_pair(a.coul, qiqj, r, inv_r, e, f);
+                        fs += scaling * f;
+                        acc[1] += scaling * e;
+                    }
+                }
+            }
+            a.contrib[tm.slots.x] = make_double4(fs * dx, fs * dy, fs * dz, 0.0);
+            a.contrib[tm.slots.y] = make_double4(-fs * dx, -fs * dy, -fs * dz, 0.0);
+            acc[5] += fs * dx * dx;
+            acc[6] += fs * dy * dy;
+            acc[7] += fs * dz * dz;
+            acc[8] += fs * dx * dy;
+            acc[9] += fs * dx * dz;
+            acc[10] += fs * dy * dz;
+        } else if (tm.type == TERM_BOND) {
+            // compute.rs:63-73; virial compute.rs:231-239
+            const BondedParam& bp = a.bparam[tm.param];
+            double fx = 0, fy = 0, fz = 0;
+            if (bp.pot != POT_NONE) {
+                double4 pi = a.pos[tm.atoms.x], pj = a.pos[tm.atoms.y];
+                double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                min_image(a.box, dx, dy, dz);
+                double r = sqrt(norm2_ordered(dx, dy, dz));
+                double e, f;
+                pot_eval(bp.pot, bp.p0, bp.p1, r, 1.0 / r, e, f);
+                fx = f * dx;
+                fy = f * dy;
+                fz = f * dz;
+                acc[2] += e;
+                acc[5] += f * dx * dx;
+                acc[6] += f * dy * dy;
+                acc[7] += f * dz * dz;
+                acc[8] += f * dx * dy;
+                acc[9] += f * dx * dz;
+                acc[10] += f * dy * dz;
+            }
+            a.contrib[tm.slots.x] = make_double4(fx, fy, fz, 0.0);
+            a.contrib[tm.slots.y] = make_double4(-fx, -fy, -fz, 0.0);
+        } else if (tm.type == TERM_ANGLE) {
+            // cells.rs:335-359
+            const BondedParam& ap = a.aparam[tm.param];
+            double4 c0 = make_double4(0, 0, 0, 0), c1 = c0, c2 = c0;
+            if (ap.pot != POT_NONE) {
+                double4 p1 = a.pos[tm.atoms.x], p2 = a.pos[tm.atoms.y], p3 = a.pos[tm.atoms.z];
+                double ax = p1.x - p2.x, ay = p1.y - p2.y, az = p1.z - p2.z;
+                double bx = p3.x - p2.x, by = p3.y - p2.y, bz = p3.z - p2.z;
+                min_image(a.box, ax, ay, az);
+                min_image(a.box, bx, by, bz);
+                double na = sqrt(norm2_ordered(ax, ay, az)), nb = sqrt(norm2_ordered(bx, by, bz));
+                double ux = ax / na, uy = ay / na, uz = az / na;
+                double vx = bx / nb, vy = by / nb, vz = bz / nb;
+                double cs = ux * vx + uy * vy + uz * vz;
+                double sin_inv = 1.0 / sqrt(1.0 - cs * cs);
+                double d1x = sin_inv * (cs * ux - vx) / na, d1y = sin_inv * (cs * uy - vy) / na,
+                       d1z = sin_inv * (cs * uz - vz) / na;
+                double d3x = sin_inv * (cs * vx - ux) / nb, d3y = sin_inv * (cs * vy - uy) / nb,
+                       d3z = sin_inv * (cs * vz - uz) / nb;
+                double theta = acos(cs);
+                double e, f;
+                // with inv_r = 1 pot_eval returns the plain -dV/dtheta
+                pot_eval(ap.pot, ap.p0, ap.p1, theta, 1.0, e, f);
+                c0 = make_double4(f * d1x, f * d1y, f * d1z, 0.0);
+                c1 = make_double4(-f * (d1x + d3x), -f * (d1y + d3y), -f * (d1z + d3z), 0.0);
+                c2 = make_double4(f * d3x, f * d3y, f * d3z, 0.0);
+                acc[3] += e;
+            }
+            a.contrib[tm.slots.x] = c0;
+            a.contrib[tm.slots.y] = c1;
+            a.contrib[tm.slots.z] = c2;
+        } else {
+            // cells.rs:379-411
+            const BondedParam& dp = a.dparam[tm.param];
+            double4 c0 = make_double4(0, 0, 0, 0), c1 = c0, c2 = c0, c3 = c0;
+            if (dp.pot != POT_NONE) {
+                double4 q1 = a.pos[tm.atoms.x], q2 = a.pos[tm.atoms.y], q3 = a.pos[tm.atoms.z],
+                        q4 = a.pos[tm.atoms.w];
+                double r12x = q2.x - q1.x, r12y = q2.y - q1.y, r12z = q2.z - q1.z;
+                double r23x = q3.x - q2.x, r23y = q3.y - q2.y, r23z = q3.z - q2.z;
+                double r34x = q4.x - q3.x, r34y = q4.y - q3.y, r34z = q4.z - q3.z;
+                min_image(a.box, r12x, r12y, r12z);
+                min_image(a.box, r23x, r23y, r23z);
+                min_image(a.box, r34x, r34y, r34z);
+                double ux = r12y * r23z - r12z * r23y, uy = r12z * r23x - r12x * r23z,
+                       uz = r12x * r23y - r12y * r23x;
+                double vx = r23y * r34z - r23z * r34y, vy = r23z * r34x - r23x * r34z,
+                       vz = r23x * r34y - r23y * r34x;
+                double u2 = ux * ux + uy * uy + uz * uz, v2 = vx * vx + vy * vy + vz * vz;
+                double r23_2 = r23x * r23x + r23y * r23y + r23z * r23z;
+                double r23n = sqrt(r23_2);
+                double d1x = (-r23n / u2) * ux, d1y = (-r23n / u2) * uy, d1z = (-r23n / u2) * uz;
+                double d4x = (r23n / v2) * vx, d4y = (r23n / v2) * vy, d4z = (r23n / v2) * vz;
+                double r23_r34 = r23x * r34x + r23y * r34y + r23z * r34z;
+                double r12_r23 = r12x * r23x + r12y * r23y + r12z * r23z;
+                double k1 = -r12_r23 / r23_2 - 1.0, k2 = r23_r34 / r23_2;
+                double k3 = -r23_r34 / r23_2 - 1.0, k4 = r12_r23 / r23_2;
+                double d2x = k1 * d1x + k2 * d4x, d2y = k1 * d1y + k2 * d4y, d2z = k1 * d1z + k2 * d4z;
+                double d3x = k3 * d4x + k4 * d1x, d3y = k3 * d4y + k4 * d1y, d3z = k3 * d4z + k4 * d1z;
+                double phi = atan2(r23n * (vx * r12x + vy * r12y + vz * r12z), ux * vx + uy * vy + uz * vz);
+                double e, f;
+                pot_eval(dp.pot, dp.p0, dp.p1, phi, 1.0, e, f);
+                c0 = make_double4(f * d1x, f * d1y, f * d1z, 0.0);
+                c1 = make_double4(f * d2x, f * d2y, f * d2z, 0.0);
+                c2 = make_double4(f * d3x, f * d3y, f * d3z, 0.0);
+                c3 = make_double4(f * d4x, f * d4y, f * d4z, 0.0);
+                acc[4] += e;
+            }
+            a.contrib[tm.slots.x] = c0;
+            a.contrib[tm.slots.y] = c1;
+            a.contrib[tm.slots.z] = c2;
+            a.contrib[tm.slots.w] = c3;
+        }
+    }
+    if (kEnergy) {
+        int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        for (int q = 0; q < 11; q++) {
+            double sum = wave_sum(acc[q]);
+            if (lane == 0) red[wid][q] = sum;
+        }
+        __syncthreads();
+        if (threadIdx.x < 11) {
+            double sum = 0;
+            for (int w = 0; w < 4; w++) sum += red[w][threadIdx.x];
+            a.epart[blk * 11 + threadIdx.x] = sum;
+        }
+    }
+}
+
+template <bool kEnergy>
+__global__ __launch_bounds__(256) void k_intra_terms(IntraArgs a) {
+    intra_block<kEnergy>(a, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------------
 // Non-bonded pair kernel: inter-molecular pairs, Lennard-Jones family + Ewald
 // real space or Wolf, one lane per atom i over its neighbour list.
 // Forces are written for every i (full list, both directions), so no atomics
@@ -622,6 +839,8 @@ struct PairArgs {
     double4* force;
     double* epart;  // per block: e_pairs, e_coul, w_xx, w_yy, w_zz, w_xy, w_xz, w_yz
     int* rebuild;   // neighbour-list rebuild flag, cleared here (see k_nlist_check)
+    int pair_blocks;  // blocks of pairs; the launch's further blocks run the intra-molecular terms
+    IntraArgs intra;
 };
 
 constexpr int kMaxKinds = 16;
@@ -646,6 +865,10 @@ template <bool kEnergy, int TPA, int CK>
 __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
     __shared__ PairParam tab[kMaxKinds * kMaxKinds];
     __shared__ double red[kPairBlock / 64][8];
+    if ((int)blockIdx.x >= a.pair_blocks) {  // uniform over the block
+        intra_block<kEnergy>(a.intra, blockIdx.x - a.pair_blocks);
+        return;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.rebuild) *a.rebuild = 0;
     int nk2 = a.nkinds * a.nkinds;
     for (int x = threadIdx.x; x < nk2; x += blockDim.x) tab[x] = a.table[x];
@@ -767,6 +990,10 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
     };
     __shared__ P32 tab[kMaxKinds * kMaxKinds];
     __shared__ double red[kPairBlock / 64][8];
+    if ((int)blockIdx.x >= a.pair_blocks) {  // uniform over the block
+        intra_block<kEnergy>(a.intra, blockIdx.x - a.pair_blocks);
+        return;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.rebuild) *a.rebuild = 0;
     int nk2 = a.nkinds * a.nkinds;
     for (int x = threadIdx.x; x < nk2; x += blockDim.x) {
@@ -926,215 +1153,6 @@ __global__ void k_pairs_within(const double4* __restrict__ pos, const int* __res
     }
 }
 
-// ---------------------------------------------------------------------------------
-// Intra-molecular terms: one thread per term (same-molecule pair with its bond
-// path, bond, angle or dihedral), compute.rs:60-97, energy.rs:88-138,
-// ewald.rs:387-428 (excluded-pair correction), cells.rs:335-411.  Each term
-// writes its force on each of its atoms to a slot reserved for that
-// (term, atom); the consumer adds an atom's slots in slot order, so the sums
-// are deterministic without atomics.  Terms are sorted by type on the host so
-// that waves do not diverge.
-// ---------------------------------------------------------------------------------
-struct BondedParam {
-    int pot;  // POT_* (POT_NONE: no potential)
-    double p0, p1;
-};
-
-enum : int { TERM_PAIR = 0, TERM_BOND = 1, TERM_ANGLE = 2, TERM_DIHEDRAL = 3 };
-
-struct Term {
-    int4 atoms;  // pair/bond: x, y; angle: x, y, z; dihedral: x, y, z, w
-    int4 slots;  // contribution slot of each atom
-    int type;    // TERM_*
-    int param;   // pair: bond path; bond/angle/dihedral: parameter index
-};
-
-struct IntraArgs {
-    const double4* pos;
-    const int2* meta;
-    int nterms;
-    const Term* terms;
-    const PairParam* table;
-    int nkinds;
-    const int* pair_restr;  // restriction per kind pair
-    const double* pair_s14; // scale14 per kind pair
-    const BondedParam* bparam;
-    const BondedParam* aparam;
-    const BondedParam* dparam;
-    CoulParam coul;
-    int coul_restr;
-    double coul_s14;
-    Box box;
-    double4* contrib;  // slot contributions
-    double* epart;     // per block: pairs, coul, bonds, angles, dihedrals, w6
-};
-
-template <bool kEnergy>
-__global__ __launch_bounds__(256) void k_intra_terms(IntraArgs a) {
-    __shared__ double red[4][11];
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (t < a.nterms) {
-        Term tm = a.terms[t];
-        if (tm.type == TERM_PAIR) {
-            int i = tm.atoms.x, j = tm.atoms.y, path = tm.param;
-            double4 pi = a.pos[i], pj = a.pos[j];
-            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            min_image(a.box, dx, dy, dz);
-            double r = sqrt(norm2_ordered(dx, dy, dz));
-            double inv_r = 1.0 / r;
-            int tk = a.meta[i].x * a.nkinds + a.meta[j].x;
-            const PairParam& pp = a.table[tk];
-            double fs = 0.0;
-            if (pp.pot != POT_NONE) {
-                bool excl;
-                double scaling;
-                restriction_info(a.pair_restr[tk], a.pair_s14[tk], path, excl, scaling);
-                if (!excl && r < pp.rc && pp.pot != POT_NULL) {
-                    double e, f;
-                    pot_eval(pp.pot, pp.p0, pp.p1, r, inv_r, e, f);
-                    fs += scaling * f;
-                    acc[0] += scaling * (e - pp.shift);
-                }
-            }
-            double qiqj = pi.w * pj.w;
-            if (a.coul.kind != COUL_NONE && qiqj != 0.0) {
-                bool excl;
-                double scaling;
-                restriction_info(a.coul_restr, a.coul_s14, path, excl, scaling);
-                if (r <= a.coul.rc) {
-                    double e, f;
-                    if (a.coul.kind == COUL_EWALD) {
-                        ewald_real(a.coul.alpha, qiqj, r, inv_r, excl, e, f);
-                        fs += f;
-                        acc[1] += e;
-                    } else if (!excl) {
-                        wolf_pair(a.coul, qiqj, r, inv_r, e, f);
-                        fs += scaling * f;
-                        acc[1] += scaling * e;
-                    }
-                }
-            }
-            a.contrib[tm.slots.x] = make_double4(fs * dx, fs * dy, fs * dz, 0.0);
-            a.contrib[tm.slots.y] = make_double4(-fs * dx, -fs * dy, -fs * dz, 0.0);
-            acc[5] += fs * dx * dx;
-            acc[6] += fs * dy * dy;
-            acc[7] += fs * dz * dz;
-            acc[8] += fs * dx * dy;
-            acc[9] += fs * dx * dz;
-            acc[10] += fs * dy * dz;
-        } else if (tm.type == TERM_BOND) {
-            // compute.rs:63-73; virial compute.rs:231-239
-            const BondedParam& bp = a.bparam[tm.param];
-            double fx = 0, fy = 0, fz = 0;
-            if (bp.pot != POT_NONE) {
-                double4 pi = a.pos[tm.atoms.x], pj = a.pos[tm.atoms.y];
-                double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                min_image(a.box, dx, dy, dz);
-                double r = sqrt(norm2_ordered(dx, dy, dz));
-                double e, f;
-                pot_eval(bp.pot, bp.p0, bp.p1, r, 1.0 / r, e, f);
-                fx = f * dx;
-                fy = f * dy;
-                fz = f * dz;
-                acc[2] += e;
-                acc[5] += f * dx * dx;
-                acc[6] += f * dy * dy;
-                acc[7] += f * dz * dz;
-                acc[8] += f * dx * dy;
-                acc[9] += f * dx * dz;
-                acc[10] += f * dy * dz;
-            }
-            a.contrib[tm.slots.x] = make_double4(fx, fy, fz, 0.0);
-            a.contrib[tm.slots.y] = make_double4(-fx, -fy, -fz, 0.0);
-        } else if (tm.type == TERM_ANGLE) {
-            // cells.rs:335-359
-            const BondedParam& ap = a.aparam[tm.param];
-            double4 c0 = make_double4(0, 0, 0, 0), c1 = c0, c2 = c0;
-            if (ap.pot != POT_NONE) {
-                double4 p1 = a.pos[tm.atoms.x], p2 = a.pos[tm.atoms.y], p3 = a.pos[tm.atoms.z];
-                double ax = p1.x - p2.x, ay = p1.y - p2.y, az = p1.z - p2.z;
-                double bx = p3.x - p2.x, by = p3.y - p2.y, bz = p3.z - p2.z;
-                min_image(a.box, ax, ay, az);
-                min_image(a.box, bx, by, bz);
-                double na = sqrt(norm2_ordered(ax, ay, az)), nb = sqrt(norm2_ordered(bx, by, bz));
-                double ux = ax / na, uy = ay / na, uz = az / na;
-                double vx = bx / nb, vy = by / nb, vz = bz / nb;
-                double cs = ux * vx + uy * vy + uz * vz;
-                double sin_inv = 1.0 / sqrt(1.0 - cs * cs);
-                double d1x = sin_inv * (cs * ux - vx) / na, d1y = sin_inv * (cs * uy - vy) / na,
-                       d1z = sin_inv * (cs * uz - vz) / na;
-                double d3x = sin_inv * (cs * vx - ux) / nb, d3y = sin_inv * (cs * vy - uy) / nb,
-                       d3z = sin_inv * (cs * vz - uz) / nb;
-                double theta = acos(cs);
-                double e, f;
-                // with inv_r = 1 pot_eval returns the plain -dV/dtheta
-                pot_eval(ap.pot, ap.p0, ap.p1, theta, 1.0, e, f);
-                c0 = make_double4(f * d1x, f * d1y, f * d1z, 0.0);
-                c1 = make_double4(-f * (d1x + d3x), -f * (d1y + d3y), -f * (d1z + d3z), 0.0);
-                c2 = make_double4(f * d3x, f * d3y, f * d3z, 0.0);
-                acc[3] += e;
-            }
-            a.contrib[tm.slots.x] = c0;
-            a.contrib[tm.slots.y] = c1;
-            a.contrib[tm.slots.z] = c2;
-        } else {
-            // cells.rs:379-411
-            const BondedParam& dp = a.dparam[tm.param];
-            double4 c0 = make_double4(0, 0, 0, 0), c1 = c0, c2 = c0, c3 = c0;
-            if (dp.pot != POT_NONE) {
-                double4 q1 = a.pos[tm.atoms.x], q2 = a.pos[tm.atoms.y], q3 = a.pos[tm.atoms.z],
-                        q4 = a.pos[tm.atoms.w];
-                double r12x = q2.x - q1.x, r12y = q2.y - q1.y, r12z = q2.z - q1.z;
-                double r23x = q3.x - q2.x, r23y = q3.y - q2.y, r23z = q3.z - q2.z;
-                double r34x = q4.x - q3.x, r34y = q4.y - q3.y, r34z = q4.z - q3.z;
-                min_image(a.box, r12x, r12y, r12z);
-                min_image(a.box, r23x, r23y, r23z);
-                min_image(a.box, r34x, r34y, r34z);
-                double ux = r12y * r23z - r12z * r23y, uy = r12z * r23x - r12x * r23z,
-                       uz = r12x * r23y - r12y * r23x;
-                double vx = r23y * r34z - r23z * r34y, vy = r23z * r34x - r23x * r34z,
-                       vz = r23x * r34y - r23y * r34x;
-                double u2 = ux * ux + uy * uy + uz * uz, v2 = vx * vx + vy * vy + vz * vz;
-                double r23_2 = r23x * r23x + r23y * r23y + r23z * r23z;
-                double r23n = sqrt(r23_2);
-                double d1x = (-r23n / u2) * ux, d1y = (-r23n / u2) * uy, d1z = (-r23n / u2) * uz;
-                double d4x = (r23n / v2) * vx, d4y = (r23n / v2) * vy, d4z = (r23n / v2) * vz;
-                double r23_r34 = r23x * r34x + r23y * r34y + r23z * r34z;
-                double r12_r23 = r12x * r23x + r12y * r23y + r12z * r23z;
-                double k1 = -r12_r23 / r23_2 - 1.0, k2 = r23_r34 / r23_2;
-                double k3 = -r23_r34 / r23_2 - 1.0, k4 = r12_r23 / r23_2;
-                double d2x = k1 * d1x + k2 * d4x, d2y = k1 * d1y + k2 * d4y, d2z = k1 * d1z + k2 * d4z;
-                double d3x = k3 * d4x + k4 * d1x, d3y = k3 * d4y + k4 * d1y, d3z = k3 * d4z + k4 * d1z;
-                double phi = atan2(r23n * (vx * r12x + vy * r12y + vz * r12z), ux * vx + uy * vy + uz * vz);
-                double e, f;
-                pot_eval(dp.pot, dp.p0, dp.p1, phi, 1.0, e, f);
-                c0 = make_double4(f * d1x, f * d1y, f * d1z, 0.0);
-                c1 = make_double4(f * d2x, f * d2y, f * d2z, 0.0);
-                c2 = make_double4(f * d3x, f * d3y, f * d3z, 0.0);
-                c3 = make_double4(f * d4x, f * d4y, f * d4z, 0.0);
-                acc[4] += e;
-            }
-            a.contrib[tm.slots.x] = c0;
-            a.contrib[tm.slots.y] = c1;
-            a.contrib[tm.slots.z] = c2;
-            a.contrib[tm.slots.w] = c3;
-        }
-    }
-    if (kEnergy) {
-        int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-        for (int q = 0; q < 11; q++) {
-            double sum = wave_sum(acc[q]);
-            if (lane == 0) red[wid][q] = sum;
-        }
-        __syncthreads();
-        if (threadIdx.x < 11) {
-            double sum = 0;
-            for (int w = 0; w < 4; w++) sum += red[w][threadIdx.x];
-            a.epart[blockIdx.x * 11 + threadIdx.x] = sum;
-        }
-    }
-}
 
 // Total force on atom i: the non-bonded pair force, the intra-molecular slot
 // contributions in slot order, and the k-space partials
