@@ -810,6 +810,32 @@ int launch_nlist(lg_system* s, bool check) {
 int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_done = false) {
     hipStream_t st = S(s);
     int n = int(s->n);
+    bool mixed = s->precision == LG_PRECISION_MIXED;
+    // rho(k) partials first: their reduction runs as the last blocks of the
+    // pair launch, and the k-space forces after it
+    int nchunks = 0;
+    if (s->coul_kind == COUL_EWALD && s->nk > 0) {
+        int kmax = s->coul_kmax;
+        int nub = blocks_for(s->nunits, 64);
+        int w = rho_chunk(s);
+        nchunks = blocks_for(s->ld, w);
+        unsigned grid = xcd_grid(nub, nchunks);
+        long pr = prof_start(s, LG_KERNEL_KSPACE_RHO);
+        if (mixed) {
+            if (!phase_done)
+                k_phase32<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                                                                              s->phase_tab32.p, s->phase_q32.p);
+            k_rho_rows<float, float2><<<grid, 256, 0, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld, kmax, w,
+                                                            s->units.p, s->nunits, s->rho_partial.p, nub, nchunks);
+        } else {
+            if (!phase_done)
+                k_phase<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                                                                            s->phase_tab.p, s->phase_q.p);
+            k_rho_rows<double, double2><<<grid, 256, 0, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
+                                                              s->units.p, s->nunits, s->rho_partial.p, nub, nchunks);
+        }
+        prof_stop(s, pr);
+    }
     PairArgs pa;
     pa.pos = s->pos.p;
     pa.meta = s->meta.p;
@@ -846,11 +872,30 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     ia.contrib = s->contrib.p;
     ia.epart = s->epart_mol.p;
     const int iblocks = s->nterms > 0 ? blocks_for(s->nterms, 256) : 0;
+    // rho(k) reduction blocks
+    const bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
+    RhoReduceArgs& rr = pa.rr;
+    rr = RhoReduceArgs{};
+    int rblocks = 0;
+    if (kspace) {
+        rr.partial = s->rho_partial.p;
+        rr.nchunks = nchunks;
+        rr.nslot = 4 * s->nunits;
+        rr.kslot = s->kslot.p;
+        rr.nk = s->nk;
+        rr.rho = s->rho.p;
+        rr.rec = mixed ? nullptr : s->krec.p;
+        rr.rec32 = mixed ? s->krec32.p : nullptr;
+        rr.efac = s->efac.p;
+        rr.vfac = s->vfac.p;
+        rr.epart = s->epart_k.p;
+        rblocks = blocks_for(s->nk, kRhoReduceK);
+    }
     long prof = prof_start(s, LG_KERNEL_PAIRS);
-    bool mixed = s->precision == LG_PRECISION_MIXED;
     const int pblocks = blocks_for(size_t(n) * s->tpa, kPairBlock);
     pa.pair_blocks = pblocks;
-    const int nblocks = pblocks + iblocks;
+    pa.intra_blocks = iblocks;
+    const int nblocks = pblocks + iblocks + rblocks;
     const int ck = coulomb_variant(s->coul);
 #define LG_PAIRS_CK(T, CK)                                                      \
     if (ck == CK) {                                                             \
@@ -884,34 +929,6 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     prof_stop(s, prof);
     if (s->coul_kind == COUL_EWALD && s->nk > 0) {
         int kmax = s->coul_kmax;
-        int nub = blocks_for(s->nunits, 64);
-        int w = rho_chunk(s);
-        int nchunks = blocks_for(s->ld, w);
-        int nslot = 4 * s->nunits;
-        unsigned grid = xcd_grid(nub, nchunks);
-        long pr = prof_start(s, LG_KERNEL_KSPACE_RHO);
-        if (mixed) {
-            if (!phase_done)
-                k_phase32<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
-                                                                              s->phase_tab32.p, s->phase_q32.p);
-            k_rho_rows<float, float2><<<grid, 256, 0, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld, kmax, w,
-                                                            s->units.p, s->nunits, s->rho_partial.p, nub, nchunks);
-        } else {
-            if (!phase_done)
-                k_phase<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
-                                                                            s->phase_tab.p, s->phase_q.p);
-            k_rho_rows<double, double2><<<grid, 256, 0, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
-                                                              s->units.p, s->nunits, s->rho_partial.p, nub, nchunks);
-        }
-        if (energy)
-            k_rho_reduce<true><<<blocks_for(s->nk, kRhoReduceK), 256, 0, st>>>(
-                s->rho_partial.p, nchunks, nslot, s->kslot.p, s->nk, s->rho.p, mixed ? nullptr : s->krec.p,
-                mixed ? s->krec32.p : nullptr, s->efac.p, s->vfac.p, s->epart_k.p);
-        else
-            k_rho_reduce<false><<<blocks_for(s->nk, kRhoReduceK), 256, 0, st>>>(
-                s->rho_partial.p, nchunks, nslot, s->kslot.p, s->nk, s->rho.p, mixed ? nullptr : s->krec.p,
-                mixed ? s->krec32.p : nullptr, s->efac.p, s->vfac.p, s->epart_k.p);
-        prof_stop(s, pr);
         long pf = prof_start(s, LG_KERNEL_KSPACE_FORCE);
         int nab = blocks_for(n, 64);
         unsigned fgrid = xcd_grid(s->ksplit, nab);
@@ -925,7 +942,6 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
                 s->ksplit, nab, s->fk.p);
         prof_stop(s, pf);
     }
-    bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
     if (combine && (kspace || s->nterms > 0)) {
         // timed with the k-space forces: it adds their partials
         long pc = prof_start(s, kspace ? LG_KERNEL_KSPACE_FORCE : LG_KERNEL_MOLECULES);

@@ -817,6 +817,97 @@ __global__ __launch_bounds__(256) void k_intra_terms(IntraArgs a) {
     intra_block<kEnergy>(a, blockIdx.x);
 }
 
+// Per k-vector record of the force kernels: rho(k) (written by
+// k_rho_reduce) and g = 2 factor_k, gz = g * ikz.  The reference's force
+// factor field_k = 2 factor_k k (ewald.rs:712-714) has k = (gx ikx, gy iky,
+// gz' ikz) with gx = 2 pi / Lx..., so along a row only the z component
+// varies: the kernels sum A = sum_k s_k g and B = sum_k s_k gz and apply
+// (gx ikx A, gy iky A, gz' B) once per row.
+struct alignas(32) KRec {
+    double re, im, g, gz;
+};
+struct alignas(16) KRec32 {
+    float re, im, g, gz;
+};
+
+// rho(k) = sum over chunks, in a fixed order: thread (g, kl) of the block
+// sums the chunks c = g mod 16 for k = 16 * block + kl, then 16 threads add
+// the 16 group sums in order.  With kEnergy the block writes its energy and
+// virial partials: e = factor |rho|^2 and w = |rho|^2 * virial_k
+// (6 components).  grid: ceil(nk / kRhoReduceK), block 256.
+constexpr int kRhoReduceK = 16;
+
+struct RhoReduceArgs {
+    const double2* partial;
+    int nchunks, nslot;
+    const int* kslot;
+    int nk;
+    double2* rho;
+    KRec* rec;
+    KRec32* rec32;
+    const double* efac;
+    const double* vfac;
+    double* epart;
+};
+
+// One block (256 threads) of the reduction, block index `blk`: run as its
+// own kernel or as the last blocks of a pair-kernel launch.
+template <bool kEnergy>
+__device__ __forceinline__ void rho_reduce_block(const RhoReduceArgs& a, int blk) {
+    const double2* __restrict__ partial = a.partial;
+    const int nchunks = a.nchunks, nslot = a.nslot, nk = a.nk;
+    constexpr int G = 256 / kRhoReduceK;
+    __shared__ double2 part[G][kRhoReduceK];
+    int g = threadIdx.x / kRhoReduceK, kl = threadIdx.x % kRhoReduceK;
+    int k = blk * kRhoReduceK + kl;
+    double re = 0.0, im = 0.0;
+    if (k < nk) {
+        const int slot = a.kslot[k];
+        for (int c = g; c < nchunks; c += G) {
+            double2 p = partial[(size_t)c * nslot + slot];
+            re += p.x;
+            im += p.y;
+        }
+    }
+    part[g][kl] = make_double2(re, im);
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (threadIdx.x < kRhoReduceK && k < nk) {
+        re = 0.0;
+        im = 0.0;
+        for (int q = 0; q < G; q++) {
+            re += part[q][kl].x;
+            im += part[q][kl].y;
+        }
+        a.rho[k] = make_double2(re, im);
+        if (a.rec) {
+            a.rec[k].re = re;
+            a.rec[k].im = im;
+        }
+        if (a.rec32) {
+            a.rec32[k].re = (float)re;
+            a.rec32[k].im = (float)im;
+        }
+        if (kEnergy) {
+            double n2 = re * re + im * im;
+            v[0] = a.efac[k] * n2;
+            for (int x = 0; x < 6; x++) v[1 + x] = n2 * a.vfac[(size_t)x * nk + k];
+        }
+    }
+    if (kEnergy) {
+        for (int x = 0; x < 7; x++) {
+            double sum = wave_sum(v[x]);
+            if (threadIdx.x == 0) a.epart[blk * 7 + x] = sum;
+        }
+    }
+}
+
+template <bool kEnergy>
+__global__ __launch_bounds__(256) void k_rho_reduce(RhoReduceArgs a) {
+    rho_reduce_block<kEnergy>(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------------
 // Non-bonded pair kernel: inter-molecular pairs, Lennard-Jones family + Ewald
 // real space or Wolf, one lane per atom i over its neighbour list.
@@ -839,8 +930,11 @@ struct PairArgs {
     double4* force;
     double* epart;  // per block: e_pairs, e_coul, w_xx, w_yy, w_zz, w_xy, w_xz, w_yz
     int* rebuild;   // neighbour-list rebuild flag, cleared here (see k_nlist_check)
-    int pair_blocks;  // blocks of pairs; the launch's further blocks run the intra-molecular terms
+    // blocks [0, pair_blocks) evaluate pairs; the next intra_blocks run the
+    // intra-molecular terms and the rest the rho(k) reduction
+    int pair_blocks, intra_blocks;
     IntraArgs intra;
+    RhoReduceArgs rr;
 };
 
 constexpr int kMaxKinds = 16;
@@ -866,7 +960,11 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
     __shared__ PairParam tab[kMaxKinds * kMaxKinds];
     __shared__ double red[kPairBlock / 64][8];
     if ((int)blockIdx.x >= a.pair_blocks) {  // uniform over the block
-        intra_block<kEnergy>(a.intra, blockIdx.x - a.pair_blocks);
+        const int b = blockIdx.x - a.pair_blocks;
+        if (b < a.intra_blocks)
+            intra_block<kEnergy>(a.intra, b);
+        else
+            rho_reduce_block<kEnergy>(a.rr, b - a.intra_blocks);
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.rebuild) *a.rebuild = 0;
@@ -991,7 +1089,11 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
     __shared__ P32 tab[kMaxKinds * kMaxKinds];
     __shared__ double red[kPairBlock / 64][8];
     if ((int)blockIdx.x >= a.pair_blocks) {  // uniform over the block
-        intra_block<kEnergy>(a.intra, blockIdx.x - a.pair_blocks);
+        const int b = blockIdx.x - a.pair_blocks;
+        if (b < a.intra_blocks)
+            intra_block<kEnergy>(a.intra, b);
+        else
+            rho_reduce_block<kEnergy>(a.rr, b - a.intra_blocks);
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.rebuild) *a.rebuild = 0;
@@ -1377,78 +1479,6 @@ __global__ __launch_bounds__(256) void k_rho_rows(const T2* __restrict__ tab, co
     }
 }
 
-// Per k-vector record of the force kernels: rho(k) (written by
-// k_rho_reduce) and g = 2 factor_k, gz = g * ikz.  The reference's force
-// factor field_k = 2 factor_k k (ewald.rs:712-714) has k = (gx ikx, gy iky,
-// gz' ikz) with gx = 2 pi / Lx..., so along a row only the z component
-// varies: the kernels sum A = sum_k s_k g and B = sum_k s_k gz and apply
-// (gx ikx A, gy iky A, gz' B) once per row.
-struct alignas(32) KRec {
-    double re, im, g, gz;
-};
-struct alignas(16) KRec32 {
-    float re, im, g, gz;
-};
-
-// rho(k) = sum over chunks, in a fixed order: thread (g, kl) of the block
-// sums the chunks c = g mod 16 for k = 16 * block + kl, then 16 threads add
-// the 16 group sums in order.  With kEnergy the block writes its energy and
-// virial partials: e = factor |rho|^2 and w = |rho|^2 * virial_k
-// (6 components).  grid: ceil(nk / kRhoReduceK), block 256.
-constexpr int kRhoReduceK = 16;
-
-template <bool kEnergy>
-__global__ __launch_bounds__(256) void k_rho_reduce(const double2* __restrict__ partial, int nchunks, int nslot,
-                                                    const int* __restrict__ kslot, int nk,
-                                                    double2* __restrict__ rho, KRec* __restrict__ rec,
-                                                    KRec32* __restrict__ rec32, const double* __restrict__ efac,
-                                                    const double* __restrict__ vfac, double* __restrict__ epart) {
-    constexpr int G = 256 / kRhoReduceK;
-    __shared__ double2 part[G][kRhoReduceK];
-    int g = threadIdx.x / kRhoReduceK, kl = threadIdx.x % kRhoReduceK;
-    int k = blockIdx.x * kRhoReduceK + kl;
-    double re = 0.0, im = 0.0;
-    if (k < nk) {
-        const int slot = kslot[k];
-        for (int c = g; c < nchunks; c += G) {
-            double2 p = partial[(size_t)c * nslot + slot];
-            re += p.x;
-            im += p.y;
-        }
-    }
-    part[g][kl] = make_double2(re, im);
-    __syncthreads();
-    if (threadIdx.x >= 64) return;
-    double v[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (threadIdx.x < kRhoReduceK && k < nk) {
-        re = 0.0;
-        im = 0.0;
-        for (int q = 0; q < G; q++) {
-            re += part[q][kl].x;
-            im += part[q][kl].y;
-        }
-        rho[k] = make_double2(re, im);
-        if (rec) {
-            rec[k].re = re;
-            rec[k].im = im;
-        }
-        if (rec32) {
-            rec32[k].re = (float)re;
-            rec32[k].im = (float)im;
-        }
-        if (kEnergy) {
-            double n2 = re * re + im * im;
-            v[0] = efac[k] * n2;
-            for (int x = 0; x < 6; x++) v[1 + x] = n2 * vfac[(size_t)x * nk + k];
-        }
-    }
-    if (kEnergy) {
-        for (int x = 0; x < 7; x++) {
-            double sum = wave_sum(v[x]);
-            if (threadIdx.x == 0) epart[blockIdx.x * 7 + x] = sum;
-        }
-    }
-}
 
 // k-space forces: one lane per atom, rows of k-vectors (fixed ikx, iky;
 // consecutive ikz) read uniformly; F_i += q_i / (4 pi eps0) *
