@@ -221,6 +221,9 @@ struct lg_system {
     DevBuf<int> mc_cell_count, mc_atom_cell;
     // snapshot for neighbour-list overflow recovery
     DevBuf<double4> snap_pos, snap_vel, snap_acc;
+    // Verlet build deferred by launch_nlist into the rho(k) launch (k_cells_rho)
+    bool cells_pending = false, cells_nearest = false;
+    CellsArgs cells_args{};
     // MD chunk captured as a HIP graph, with what it was captured for
     hipGraphExec_t md_exec = nullptr;
     uint64_t md_key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -773,16 +776,23 @@ int launch_nlist(lg_system* s, bool check) {
     // FP32 candidate test with a 1e-4 relative margin on rlist
     float rl2f = (!s->periodic || s->rlist <= 0) ? INFINITY : float(s->rlist * (1.0 + 1e-4) * s->rlist * (1.0 + 1e-4));
     if (cell_build(s)) {
-        bool nearest = s->periodic && (g.reach[0] == 0 || g.reach[1] == 0 || g.reach[2] == 0);
-        if (nearest)
-            k_nlist_cells<true><<<g.ncell, kCellBlock, 0, st>>>(s->cpos.p, s->sorted.p, s->start.p, s->count.p, g,
-                                                                s->box, rl2f, s->cap, s->nbr.p, s->nnbr.p, s->pos.p,
-                                                                s->ref.p, flag);
-        else
-            k_nlist_cells<false><<<g.ncell, kCellBlock, 0, st>>>(s->cpos.p, s->sorted.p, s->start.p, s->count.p, g,
-                                                                 s->box, rl2f, s->cap, s->nbr.p, s->nnbr.p, s->pos.p,
-                                                                 s->ref.p, flag);
+        CellsArgs ca{s->cpos.p, s->sorted.p, s->start.p, s->count.p, g, s->box, rl2f, s->cap,
+                     s->nbr.p, s->nnbr.p, s->pos.p, s->ref.p, flag};
+        const bool nearest = s->periodic && (g.reach[0] == 0 || g.reach[1] == 0 || g.reach[2] == 0);
         prof_stop(s, prof);
+        if (s->coul_kind == COUL_EWALD && s->nk > 0) {
+            // launched by launch_forces together with the rho(k) partials
+            s->cells_pending = true;
+            s->cells_nearest = nearest;
+            s->cells_args = ca;
+            return LG_OK;
+        }
+        long p2 = prof_start(s, LG_KERNEL_NLIST);
+        if (nearest)
+            k_nlist_cells<true><<<g.ncell, kCellBlock, 0, st>>>(ca);
+        else
+            k_nlist_cells<false><<<g.ncell, kCellBlock, 0, st>>>(ca);
+        prof_stop(s, p2);
         LG_HIP(hipGetLastError());
         return LG_OK;
     }
@@ -821,20 +831,46 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         nchunks = blocks_for(s->ld, w);
         unsigned grid = xcd_grid(nub, nchunks);
         long pr = prof_start(s, LG_KERNEL_KSPACE_RHO);
+        // with a deferred Verlet build: one launch of k_cells_rho, the list
+        // blocks first (cells rounded up to 8), then the rho blocks
+        const bool cells = s->cells_pending;
+        const int ncb = cells ? (s->grid.ncell + 7) / 8 * 8 : 0;
+        const CellsArgs& ca = s->cells_args;
+        const bool near = s->cells_nearest;
         if (mixed) {
             if (!phase_done)
                 k_phase32<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
                                                                               s->phase_tab32.p, s->phase_q32.p);
-            k_rho_rows<float, float2><<<grid, 256, 0, st>>>(s->phase_tab32.p, s->phase_q32.p, s->ld, kmax, w,
-                                                            s->units.p, s->nunits, s->rho_partial.p, nub, nchunks);
+            RhoRowsArgs<float, float2> ra{s->phase_tab32.p, s->phase_q32.p, s->ld, kmax, w, s->units.p,
+                                          s->nunits, s->rho_partial.p, nub, nchunks};
+            if (!cells)
+                k_rho_rows<float, float2><<<grid, 512, 0, st>>>(ra);
+            else if (near)
+                k_cells_rho<true, float, float2><<<ncb + grid, kCellBlock, 0, st>>>(ca, ra, ncb);
+            else
+                k_cells_rho<false, float, float2><<<ncb + grid, kCellBlock, 0, st>>>(ca, ra, ncb);
         } else {
             if (!phase_done)
                 k_phase<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
                                                                             s->phase_tab.p, s->phase_q.p);
-            k_rho_rows<double, double2><<<grid, 256, 0, st>>>(s->phase_tab.p, s->phase_q.p, s->ld, kmax, w,
-                                                              s->units.p, s->nunits, s->rho_partial.p, nub, nchunks);
+            RhoRowsArgs<double, double2> ra{s->phase_tab.p, s->phase_q.p, s->ld, kmax, w, s->units.p,
+                                            s->nunits, s->rho_partial.p, nub, nchunks};
+            if (!cells)
+                k_rho_rows<double, double2><<<grid, 512, 0, st>>>(ra);
+            else if (near)
+                k_cells_rho<true, double, double2><<<ncb + grid, kCellBlock, 0, st>>>(ca, ra, ncb);
+            else
+                k_cells_rho<false, double, double2><<<ncb + grid, kCellBlock, 0, st>>>(ca, ra, ncb);
         }
+        s->cells_pending = false;
         prof_stop(s, pr);
+    }
+    if (s->cells_pending) {  // deferred build without k-space (not expected): launch it alone
+        if (s->cells_nearest)
+            k_nlist_cells<true><<<s->grid.ncell, kCellBlock, 0, st>>>(s->cells_args);
+        else
+            k_nlist_cells<false><<<s->grid.ncell, kCellBlock, 0, st>>>(s->cells_args);
+        s->cells_pending = false;
     }
     PairArgs pa;
     pa.pos = s->pos.p;

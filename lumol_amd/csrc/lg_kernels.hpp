@@ -410,16 +410,39 @@ constexpr int kCellTile = 2048;  // candidates staged per tile (40 KB of LDS)
 constexpr int kCellHome = 64;    // home atoms per chunk
 constexpr int kCellMaxSeg = 128; // (2 reach + 1)^2 rows x 3 runs <= 75
 
-// kMinImage: some axis has a single cell (nearest image along it).
+struct CellsArgs {
+    const float4* cpos;
+    const int* sorted;
+    const int* start;
+    int* count_zero;
+    Grid g;
+    Box b;
+    float rl2f;
+    int cap;
+    int* nbr;
+    int* nnbr;
+    const double4* pos;
+    double4* ref;
+    int* flags;
+};
+
+// kMinImage: some axis has a single cell (nearest image along it).  Block
+// `blk` of `nblk` (cells blk, blk + nblk, ...).
 template <bool kMinImage>
-__global__ __launch_bounds__(kCellBlock) void k_nlist_cells(const float4* __restrict__ cpos,
-                                                            const int* __restrict__ sorted,
-                                                            const int* __restrict__ start,
-                                                            int* __restrict__ count_zero, Grid g, Box b, float rl2f,
-                                                            int cap, int* __restrict__ nbr, int* __restrict__ nnbr,
-                                                            const double4* __restrict__ pos,
-                                                            double4* __restrict__ ref,
-                                                            int* __restrict__ flags) {
+__device__ __forceinline__ void nlist_cells_block(const CellsArgs& args, int blk, int nblk) {
+    const float4* __restrict__ cpos = args.cpos;
+    const int* __restrict__ sorted = args.sorted;
+    const int* __restrict__ start = args.start;
+    int* __restrict__ count_zero = args.count_zero;
+    const Grid& g = args.g;
+    const Box& b = args.b;
+    const float rl2f = args.rl2f;
+    const int cap = args.cap;
+    int* __restrict__ nbr = args.nbr;
+    int* __restrict__ nnbr = args.nnbr;
+    const double4* __restrict__ pos = args.pos;
+    double4* __restrict__ ref = args.ref;
+    int* __restrict__ flags = args.flags;
     if (flags[0] == 0) return;
     __shared__ float4 tile[kCellTile];
     __shared__ int tidx[kCellTile];
@@ -429,14 +452,14 @@ __global__ __launch_bounds__(kCellBlock) void k_nlist_cells(const float4* __rest
     __shared__ float4 home[kCellHome];
     __shared__ int home_cnt[kCellHome];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int c = blockIdx.x * kCellBlock + tid; c < g.ncell; c += gridDim.x * kCellBlock) count_zero[c] = 0;
+    for (int c = blk * kCellBlock + tid; c < g.ncell; c += nblk * kCellBlock) count_zero[c] = 0;
     const float Lx = (float)b.L[0], Ly = (float)b.L[1], Lz = (float)b.L[2];
     const float ix = (float)b.inv[0], iy = (float)b.inv[1], iz = (float)b.inv[2];
     const bool nx1 = b.periodic && g.reach[0] == 0, ny1 = b.periodic && g.reach[1] == 0,
                nz1 = b.periodic && g.reach[2] == 0;
     const int ey = 2 * g.reach[1] + 1, nseg = (2 * g.reach[2] + 1) * ey * 3;
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (int c = blockIdx.x; c < g.ncell; c += gridDim.x) {
+    for (int c = blk; c < g.ncell; c += nblk) {
         const int hs = start[c], H = start[c + 1] - hs;
         if (H == 0) continue;  // uniform over the workgroup
         const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
@@ -598,6 +621,12 @@ __global__ __launch_bounds__(kCellBlock) void k_nlist_cells(const float4* __rest
         }
     }
 }
+
+template <bool kMinImage>
+__global__ __launch_bounds__(kCellBlock) void k_nlist_cells(CellsArgs ca) {
+    nlist_cells_block<kMinImage>(ca, blockIdx.x, gridDim.x);
+}
+
 
 
 // ---------------------------------------------------------------------------------
@@ -1322,12 +1351,14 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 // with L mod 8 == by mod 8, so all blocks that read row by's data (an atom
 // chunk of the phase table) share one L2 and the chunk is fetched from HBM
 // once.  False for the padding blocks (whole block; call before any barrier).
-__device__ __forceinline__ bool xcd_block(int gx, int gy, int& bx, int& by) {
-    int L = blockIdx.x;
+__device__ __forceinline__ bool xcd_block_at(int L, int gx, int gy, int& bx, int& by) {
     int q = L >> 3;
     by = (q / gx) * 8 + (L & 7);
     bx = q - (q / gx) * gx;
     return by < gy;
+}
+__device__ __forceinline__ bool xcd_block(int gx, int gy, int& bx, int& by) {
+    return xcd_block_at(blockIdx.x, gx, gy, bx, by);
 }
 __host__ __forceinline__ unsigned xcd_grid(int gx, int gy) { return unsigned(gx) * unsigned((gy + 7) / 8 * 8); }
 
@@ -1416,8 +1447,8 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 // one row (fixed ikx, iky; consecutive ikz): per atom, the lane forms
 // q e^{i k0.r} = q (Ex * Ey) * Ez[ikz0] and steps to the next ikz by one
 // product with Ez[1], 7.5 operations and one table read per (atom, k).
-// Block = 64 units (one per lane) x 4 waves; the block's atom chunk of
-// `width` atoms is split in four and each wave sums its quarter; the four
+// Block = 64 units (one per lane) x 8 waves; the block's atom chunk of
+// `width` atoms is split in eight and each wave sums its eighth; the eight
 // wave sums are added in order in LDS and wave j writes k-vector j of every
 // unit: partial[chunk * 4 * nunits + j * nunits + unit] (slots with j past
 // the unit's count are not written).  Table rows are read straight from L2:
@@ -1431,12 +1462,28 @@ struct KUnit {
 };
 
 template <typename T, typename T2>
-__global__ __launch_bounds__(256) void k_rho_rows(const T2* __restrict__ tab, const T* __restrict__ q, int ld,
-                                                  int kmax, int width, const KUnit* __restrict__ units, int nunits,
-                                                  double2* __restrict__ partial, int nub, int nchunks) {
-    __shared__ double2 red[4][4][64];
+struct RhoRowsArgs {
+    const T2* tab;
+    const T* q;
+    int ld, kmax, width;
+    const KUnit* units;
+    int nunits;
+    double2* partial;
+    int nub, nchunks;
+};
+
+// NW waves per block (8 in k_rho_rows and k_cells_rho), logical
+// block index L of the xcd_grid(nub, nchunks) layout.
+template <typename T, typename T2, int NW>
+__device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int L) {
+    const T2* __restrict__ tab = ra.tab;
+    const T* __restrict__ q = ra.q;
+    const int ld = ra.ld, kmax = ra.kmax, width = ra.width, nunits = ra.nunits;
+    const KUnit* __restrict__ units = ra.units;
+    double2* __restrict__ partial = ra.partial;
+    __shared__ double2 red[NW][4][64];
     int ub, chunk;
-    if (!xcd_block(nub, nchunks, ub, chunk)) return;
+    if (!xcd_block_at(L, ra.nub, ra.nchunks, ub, chunk)) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int u = ub * 64 + lane;
     const KUnit un = units[u < nunits ? u : nunits - 1];
@@ -1444,7 +1491,7 @@ __global__ __launch_bounds__(256) void k_rho_rows(const T2* __restrict__ tab, co
     const T2* cy = tab + (size_t)phase_row(1, un.iky, kmax) * ld;
     const T2* cz = tab + (size_t)phase_row(2, un.kz0, kmax) * ld;
     const T2* c1 = tab + (size_t)phase_row(2, 1, kmax) * ld;
-    const int per = width / 4;
+    const int per = width / NW;
     const int a0 = chunk * width + w * per;
     const int a1 = min(a0 + per, ld);
     T r0 = 0, i0 = 0, r1 = 0, i1 = 0, r2 = 0, i2 = 0, r3 = 0, i3 = 0;
@@ -1473,10 +1520,34 @@ __global__ __launch_bounds__(256) void k_rho_rows(const T2* __restrict__ tab, co
     red[w][3][lane] = make_double2(r3, i3);
     __syncthreads();
     if (u < nunits && w < un.nv) {
-        double2 s0 = red[0][w][lane], s1 = red[1][w][lane], s2 = red[2][w][lane], s3 = red[3][w][lane];
-        partial[(size_t)chunk * 4 * nunits + (size_t)w * nunits + u] =
-            make_double2(((s0.x + s1.x) + s2.x) + s3.x, ((s0.y + s1.y) + s2.y) + s3.y);
+        double2 sum = red[0][w][lane];
+#pragma unroll
+        for (int v = 1; v < NW; v++) {
+            sum.x += red[v][w][lane].x;
+            sum.y += red[v][w][lane].y;
+        }
+        partial[(size_t)chunk * 4 * nunits + (size_t)w * nunits + u] = sum;
     }
+}
+
+// Same 8-wave blocks as the rho part of k_cells_rho, so that the partial
+// sums do not depend on which launch computed them.
+template <typename T, typename T2>
+__global__ __launch_bounds__(512) void k_rho_rows(RhoRowsArgs<T, T2> ra) {
+    rho_rows_block<T, T2, 8>(ra, blockIdx.x);
+}
+
+// One launch for two independent passes of a force evaluation: blocks
+// [0, ncb) build the Verlet list (ncb = cells rounded up to 8, so that the
+// rho blocks that follow keep the XCD layout of xcd_grid), the rest compute
+// the rho(k) partials with 8 waves per block.  Both read only what the
+// first half-step and the cell list wrote.
+template <bool kMinImage, typename T, typename T2>
+__global__ __launch_bounds__(kCellBlock) void k_cells_rho(CellsArgs ca, RhoRowsArgs<T, T2> ra, int ncb) {
+    if ((int)blockIdx.x < ncb)
+        nlist_cells_block<kMinImage>(ca, blockIdx.x, ncb);
+    else
+        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb);
 }
 
 

@@ -18,7 +18,7 @@ from . import _native as nat
 
 PAIR_LANES = (8, 16, 32, 64)
 KSPACE_SPLITS = (2, 4, 8, 16, 32)
-RHO_WIDTHS = (64, 128, 256)
+RHO_WIDTHS = (64, 128, 256, 512)
 
 _cache = {}
 _lock = threading.Lock()
