@@ -1801,15 +1801,16 @@ static int md_chunk(lg_system* s, uint64_t todo) {
     }
     Thermo th{nullptr, nb, double(3 * s->n), s->berendsen_T, s->berendsen_tau};
     int nbf = blocks_for(std::max(n, kspace ? s->ld : 0), 64);
+    const int tbf = ph.mode == 2 ? 512 : 256;  // waves per first-half block: see phase_roles
     for (uint64_t k = 0; k < todo; k++) {
         // the Berendsen scale of the previous step is applied by the first half
         th.ke_part = (thermo && k > 0) ? s->ke_part.p : nullptr;
         long p1 = prof_start(s, LG_KERNEL_INTEGRATE);
         if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
-            k_vv_first<<<nbf, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
+            k_vv_first<<<nbf, tbf, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
                                            s->flags.p, ph);
         else
-            k_lf_first<<<nbf, 256, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
+            k_lf_first<<<nbf, tbf, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
                                            s->flags.p, ph);
         prof_stop(s, p1);
         LG_TRY(launch_nlist(s, false));

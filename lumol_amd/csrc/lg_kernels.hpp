@@ -1384,11 +1384,12 @@ __device__ __forceinline__ void phase_axis64(double x, double inv, int axis, int
 // Mixed form: each phase n k_axis x formed and reduced to [-pi, pi] in FP64,
 // then sincosf.
 __device__ __forceinline__ void phase_axis32(double x, double inv, int axis, int kmax, float2* __restrict__ tab,
-                                             int ld, int a) {
+                                             int ld, int a, int k0 = 1, int k1 = -1) {
     double phase = (2.0 * kPi * inv) * x;
     const double two_pi = 2.0 * kPi, inv_two_pi = 1.0 / (2.0 * kPi);
-    tab[(size_t)phase_row(axis, 0, kmax) * ld + a] = make_float2(1.f, 0.f);
-    for (int k = 1; k <= kmax; k++) {
+    if (k1 < 0) k1 = kmax;
+    if (k0 == 1) tab[(size_t)phase_row(axis, 0, kmax) * ld + a] = make_float2(1.f, 0.f);
+    for (int k = k0; k <= k1; k++) {
         double th = k * phase;
         th -= two_pi * rint(th * inv_two_pi);
         float sn, cs;
@@ -1703,13 +1704,16 @@ struct Thermo {
     double dof, T0, tau;
 };
 
+// The sum runs over the first 256 threads whatever the block size, so that
+// every kernel computes the same lambda bit for bit.
 __device__ __forceinline__ double berendsen_lambda(const Thermo& th) {
     __shared__ double red[4];
     __shared__ double lam;
     double t = 0.0;
-    for (int p = threadIdx.x; p < th.nparts; p += blockDim.x) t += th.ke_part[p];
+    if (threadIdx.x < 256)
+        for (int p = threadIdx.x; p < th.nparts; p += 256) t += th.ke_part[p];
     t = wave_sum(t);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+    if ((threadIdx.x & 63) == 0 && threadIdx.x < 256) red[threadIdx.x >> 6] = t;
     __syncthreads();
     if (threadIdx.x == 0) {
         double kin = red[0] + red[1] + red[2] + red[3];
@@ -1724,27 +1728,39 @@ __device__ __forceinline__ double berendsen_lambda(const Thermo& th) {
 // the rows of axis r - 1 for the block's 64 new positions, staged in LDS by
 // role 0 (charges with axis 0).  Padding atoms (n <= i < ld) get x = 0 and
 // q = 0, as k_phase.
+// FP64 table: roles 1-3, one axis each (the recursion runs along k).
+// FP32 table (every k reduced and evaluated on its own): with 8 waves per
+// block, roles 1-6 write (axis, half of the k range), role 7 idles.
 __device__ __forceinline__ void phase_roles(const PhaseOut& ph, int i, int n, int role, int lane,
                                             const double (&px)[4][64]) {
     if (role == 0 || i >= ph.ld) return;
-    const int axis = role - 1;
-    const double x = i < n ? px[axis][lane] : 0.0;
     if (ph.mode == 1) {
+        if (role > 3) return;
+        const int axis = role - 1;
+        const double x = i < n ? px[axis][lane] : 0.0;
         phase_axis64(x, ph.box.inv[axis], axis, ph.kmax, ph.tab, ph.ld, i);
         if (axis == 0) ph.q[i] = i < n ? px[3][lane] : 0.0;
     } else {
-        phase_axis32(x, ph.box.inv[axis], axis, ph.kmax, ph.tab32, ph.ld, i);
-        if (axis == 0) ph.q32[i] = i < n ? (float)px[3][lane] : 0.f;
+        const int nsp = ((int)blockDim.x / 64 - 1) / 3;  // 1 (4 waves) or 2 (8 waves)
+        const int r = role - 1;
+        if (r >= 3 * nsp) return;
+        const int axis = r % 3, part = r / 3;
+        const int k0 = part * ph.kmax / nsp + 1, k1 = (part + 1) * ph.kmax / nsp;
+        const double x = i < n ? px[axis][lane] : 0.0;
+        phase_axis32(x, ph.box.inv[axis], axis, ph.kmax, ph.tab32, ph.ld, i, k0, k1);
+        if (axis == 0 && part == 0) ph.q32[i] = i < n ? (float)px[3][lane] : 0.f;
     }
 }
 
 // VelocityVerlet first half (integrators.rs:47-53): the pending Berendsen
 // scale of the previous step (th.ke_part != null), then v += (dt/2) a;
-// x += v dt; the displacement test of the neighbour list (sets flags[0]); and the phase table of the new position when `ph.mode` != 0.
-// Block = 64 atoms x 4 roles: role 0 integrates, roles 1-3 write the phase
-// rows of one axis each, so that the trigonometry of an atom runs on three
-// waves.  Grid: ceil(max(n, ld) / 64).
-__global__ __launch_bounds__(256) void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel,
+// x += v dt; the displacement test of the neighbour list (sets flags[0]);
+// and the phase table of the new position when `ph.mode` != 0.
+// Block = 64 atoms x 4 roles (FP64 table) or 8 roles (FP32 table): role 0
+// integrates, the others write the phase rows (phase_roles), so that the
+// trigonometry of an atom runs on three or six waves.  Grid:
+// ceil(max(n, ld) / 64).
+__global__ __launch_bounds__(512) void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel,
                                                   const double4* __restrict__ acc, int n, double dt, Thermo th,
                                                   const double4* __restrict__ ref, double half_skin2,
                                                   int* __restrict__ flags, PhaseOut ph) {
@@ -1789,7 +1805,7 @@ __global__ __launch_bounds__(256) void k_vv_first(double4* __restrict__ pos, dou
 // LeapFrog (integrators.rs:150-168), position part, after the pending
 // Berendsen scale of the previous step, with the displacement test and the
 // phase table as k_vv_first (same block layout).
-__global__ __launch_bounds__(256) void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel,
+__global__ __launch_bounds__(512) void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel,
                                                   const double4* __restrict__ acc, int n, double dt, Thermo th,
                                                   const double4* __restrict__ ref, double half_skin2,
                                                   int* __restrict__ flags, PhaseOut ph) {
