@@ -653,6 +653,7 @@ int prepare_nlist(lg_system* s) {
             g.n[d] = nc;
             g.reach[d] = reach;
             g.size[d] = s->periodic ? s->L[d] / nc : 0.0;
+            g.inv_size[d] = s->periodic ? 1.0 / g.size[d] : 0.0;
         }
     };
     layout(true);

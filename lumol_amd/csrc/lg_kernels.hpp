@@ -25,9 +25,10 @@ namespace lg {
 // Neighbour-list maintenance
 // ---------------------------------------------------------------------------------
 struct Grid {
-    int n[3];        // cells per dimension
-    double size[3];  // cell edge: >= rlist (reach 1), >= rlist / 2 (reach 2), or L (1 cell, reach 0)
-    int reach[3];    // neighbour cells searched on each side
+    int n[3];           // cells per dimension
+    double size[3];     // cell edge: >= rlist (reach 1), >= rlist / 2 (reach 2), or L (1 cell, reach 0)
+    double inv_size[3]; // 1 / size
+    int reach[3];       // neighbour cells searched on each side
     int ncell;
 };
 
@@ -52,15 +53,20 @@ __global__ void k_nlist_check(const double4* __restrict__ pos, const double4* __
     if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
 }
 
-// Coordinate wrapped into [0, L] (periodic) as the cell binning sees it.
-__device__ __forceinline__ double wrap_coord(double x, double L, int periodic) {
-    return periodic ? x - floor(x / L) * L : x;
+// Coordinate wrapped into [0, L) (periodic) as the cell binning sees it,
+// and its cell index.  Multiplications by the inverses instead of
+// divisions: the results move by a few ulps at most, far inside the 1e-12
+// relative margin of the cell edges over rlist (and the 1e-4 margin of the
+// FP32 candidate test), and the binning runs at FP64 multiply rate (the
+// single-workgroup cell list of a small system bins every atom on one CU).
+__device__ __forceinline__ double wrap_coord(double x, double L, double invL, int periodic) {
+    return periodic ? x - floor(x * invL) * L : x;
 }
 
-__device__ __forceinline__ int cell_coord(double x, double L, int nc, double csize, int periodic) {
+__device__ __forceinline__ int cell_coord(double x, double L, double invL, int nc, double inv_csize, int periodic) {
     if (nc == 1) return 0;
-    double w = wrap_coord(x, L, periodic);
-    int c = (int)floor(w / csize);
+    double w = wrap_coord(x, L, invL, periodic);
+    int c = (int)floor(w * inv_csize);
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
@@ -70,9 +76,9 @@ __global__ void k_cell_count(const double4* __restrict__ pos, int n, Grid g, Box
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 p = pos[i];
-    int cx = cell_coord(p.x, b.L[0], g.n[0], g.size[0], b.periodic);
-    int cy = cell_coord(p.y, b.L[1], g.n[1], g.size[1], b.periodic);
-    int cz = cell_coord(p.z, b.L[2], g.n[2], g.size[2], b.periodic);
+    int cx = cell_coord(p.x, b.L[0], b.inv[0], g.n[0], g.inv_size[0], b.periodic);
+    int cy = cell_coord(p.y, b.L[1], b.inv[1], g.n[1], g.inv_size[1], b.periodic);
+    int cz = cell_coord(p.z, b.L[2], b.inv[2], g.n[2], g.inv_size[2], b.periodic);
     int c = (cz * g.n[1] + cy) * g.n[0] + cx;
     cellid[i] = c;
     slot[i] = atomicAdd(&count[c], 1);
@@ -141,9 +147,9 @@ __global__ void k_cell_rank(const int* __restrict__ tmp, const int* __restrict__
         for (int b2 = s; b2 < e; b2++) rank += tmp[b2] < v;
         sorted[s + rank] = v;
         double4 p = pos[v];
-        cpos[s + rank] = make_float4((float)wrap_coord(p.x, b.L[0], b.periodic),
-                                     (float)wrap_coord(p.y, b.L[1], b.periodic),
-                                     (float)wrap_coord(p.z, b.L[2], b.periodic), __int_as_float(meta[v].y));
+        cpos[s + rank] = make_float4((float)wrap_coord(p.x, b.L[0], b.inv[0], b.periodic),
+                                     (float)wrap_coord(p.y, b.L[1], b.inv[1], b.periodic),
+                                     (float)wrap_coord(p.z, b.L[2], b.inv[2], b.periodic), __int_as_float(meta[v].y));
     }
 }
 
@@ -169,15 +175,27 @@ __global__ __launch_bounds__(1024) void k_cell_small(const double4* __restrict__
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     for (int c = tid; c < nc; c += 1024) cnt[c] = 0;
     __syncthreads();
-    for (int i = tid; i < n; i += 1024) {
-        double4 p = pos[i];
-        int cx = cell_coord(p.x, b.L[0], g.n[0], g.size[0], b.periodic);
-        int cy = cell_coord(p.y, b.L[1], g.n[1], g.size[1], b.periodic);
-        int cz = cell_coord(p.z, b.L[2], g.n[2], g.size[2], b.periodic);
-        int c = (cz * g.n[1] + cy) * g.n[0] + cx;
-        cellid[i] = c;
-        atom_cell[i] = (unsigned short)c;
-        atomicAdd(&cnt[c], 1);
+    // four atoms per thread per pass, their loads issued together
+    for (int i0 = tid; i0 < n; i0 += 4 * 1024) {
+        double4 pv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int i = i0 + u * 1024;
+            pv[u] = pos[i < n ? i : n - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int i = i0 + u * 1024;
+            if (i < n) {
+                int cx = cell_coord(pv[u].x, b.L[0], b.inv[0], g.n[0], g.inv_size[0], b.periodic);
+                int cy = cell_coord(pv[u].y, b.L[1], b.inv[1], g.n[1], g.inv_size[1], b.periodic);
+                int cz = cell_coord(pv[u].z, b.L[2], b.inv[2], g.n[2], g.inv_size[2], b.periodic);
+                int c = (cz * g.n[1] + cy) * g.n[0] + cx;
+                cellid[i] = c;
+                atom_cell[i] = (unsigned short)c;
+                atomicAdd(&cnt[c], 1);
+            }
+        }
     }
     __syncthreads();
     // exclusive scan of the counts: `per` consecutive cells per thread
@@ -223,16 +241,39 @@ __global__ __launch_bounds__(1024) void k_cell_small(const double4* __restrict__
     }
     __syncthreads();
     // ... made deterministic: atoms by increasing index within each cell
-    for (int sl = tid; sl < n; sl += 1024) {
-        int v = tmp[sl], c = slot_cell[sl];
-        int o0 = off[c], o1 = off[c + 1];
-        int rank = 0;
-        for (int q = o0; q < o1; q++) rank += tmp[q] < v;
-        sorted[o0 + rank] = v;
-        double4 p = pos[v];
-        cpos[o0 + rank] = make_float4((float)wrap_coord(p.x, b.L[0], b.periodic),
-                                      (float)wrap_coord(p.y, b.L[1], b.periodic),
-                                      (float)wrap_coord(p.z, b.L[2], b.periodic), __int_as_float(meta[v].y));
+    for (int s0 = tid; s0 < n; s0 += 4 * 1024) {
+        int vv[4], dst[4];
+        double4 pv[4];
+        int mv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int sl = s0 + u * 1024;
+            int v = 0, d = 0;
+            if (sl < n) {
+                v = tmp[sl];
+                int c = slot_cell[sl];
+                int o0 = off[c], o1 = off[c + 1];
+                int rank = 0;
+                for (int q = o0; q < o1; q++) rank += tmp[q] < v;
+                d = o0 + rank;
+            }
+            vv[u] = v;
+            dst[u] = d;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            pv[u] = pos[vv[u]];
+            mv[u] = meta[vv[u]].y;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (s0 + u * 1024 < n) {
+                sorted[dst[u]] = vv[u];
+                cpos[dst[u]] = make_float4((float)wrap_coord(pv[u].x, b.L[0], b.inv[0], b.periodic),
+                                           (float)wrap_coord(pv[u].y, b.L[1], b.inv[1], b.periodic),
+                                           (float)wrap_coord(pv[u].z, b.L[2], b.inv[2], b.periodic), __int_as_float(mv[u]));
+            }
+        }
     }
 }
 
