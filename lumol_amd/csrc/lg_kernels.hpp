@@ -1205,7 +1205,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
         };
         auto pair = [&](int j, const double4& pj, int kj) {
             double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
-            min_image(a.box, ddx, ddy, ddz);
+            min_image_fast(a.box, ddx, ddy, ddz);
             float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
             float r2 = dx * dx + dy * dy + dz * dz;
             float inv_r = rsqrtf(r2);
