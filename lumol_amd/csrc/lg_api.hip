@@ -967,14 +967,15 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     if (s->coul_kind == COUL_EWALD && s->nk > 0) {
         int kmax = s->coul_kmax;
         long pf = prof_start(s, LG_KERNEL_KSPACE_FORCE);
-        int nab = blocks_for(n, 64);
+        constexpr int kApl = 1;  // atoms per lane (2 measured slower at 4k-9k atoms)
+        int nab = blocks_for(n, 64 * kApl);
         unsigned fgrid = xcd_grid(s->ksplit, nab);
         if (mixed)
-            k_kspace_force<float, float2, KRec32><<<fgrid, 256, 0, st>>>(
+            k_kspace_force<float, float2, KRec32, kApl><<<fgrid, 256, 0, st>>>(
                 s->phase_tab32.p, s->ld, n, kmax, s->rows.p, s->nrows, s->krec32.p, s->kg[0], s->kg[1], s->kg[2],
                 s->ksplit, nab, s->fk.p);
         else
-            k_kspace_force<double, double2, KRec><<<fgrid, 256, 0, st>>>(
+            k_kspace_force<double, double2, KRec, kApl><<<fgrid, 256, 0, st>>>(
                 s->phase_tab.p, s->ld, n, kmax, s->rows.p, s->nrows, s->krec.p, s->kg[0], s->kg[1], s->kg[2],
                 s->ksplit, nab, s->fk.p);
         prof_stop(s, pf);

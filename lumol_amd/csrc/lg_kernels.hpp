@@ -1646,60 +1646,91 @@ __device__ __forceinline__ void kforce_step(const R& r, T2& e, T2 ez1, T& A, T& 
 // T = double: FP64 form; T = float: mixed form (FP32 phases, rho and
 // factors; each row's sums are added to FP64 accumulators).  Records are
 // read four at a time with one batch of scalar loads, so that a wave waits
-// once per four k-vectors.
-template <typename T, typename T2, typename R>
+// once per four k-vectors.  APL atoms per lane (block = 64 APL atoms): the
+// row headers, records and their waits are shared by APL independent
+// recursions.
+template <typename T, typename T2, typename R, int APL>
 __global__ __launch_bounds__(256) void k_kspace_force(const T2* __restrict__ tab, int ld, int n, int kmax,
                                                       const KRow* __restrict__ rows, int nrows,
                                                       const R* __restrict__ rec, double gx, double gy, double gz,
                                                       int nsplit, int nab, double4* __restrict__ fk) {
     int split, ablock;  // all splits of an atom block on one XCD
     if (!xcd_block(nsplit, nab, split, ablock)) return;
-    int i = ablock * 64 + (threadIdx.x & 63);
+    const int lane = threadIdx.x & 63;
     int r0, r1;
     kspace_wave_rows(nrows, split, nsplit, r0, r1);
-    int a = i < ld ? i : ld - 1;
-    const T2* col = tab + a;
-    // e^{i k.r} along a row: (Ex * Ey) * Ez[ikz0], then one product by
-    // e^{i kz1 z} per step of ikz
-    const T2 ez1 = col[(size_t)phase_row(2, 1, kmax) * ld];
+    const T2* col[APL];
+    T2 ez1[APL];
+#pragma unroll
+    for (int u = 0; u < APL; u++) {
+        int i = (ablock * APL + u) * 64 + lane;
+        col[u] = tab + (i < ld ? i : ld - 1);
+        // e^{i k.r} along a row: (Ex * Ey) * Ez[ikz0], then one product by
+        // e^{i kz1 z} per step of ikz
+        ez1[u] = col[u][(size_t)phase_row(2, 1, kmax) * ld];
+    }
     const T tgx = (T)gx, tgy = (T)gy, tgz = (T)gz;
-    double fx = 0, fy = 0, fz = 0;
+    double fx[APL], fy[APL], fz[APL];
     // software pipeline: the next row's header and phase columns are loaded
     // while the current row is summed
     KRow nrow = rows[r0 < r1 ? r0 : 0];
-    T2 pex = col[(size_t)phase_row(0, nrow.ikx, kmax) * ld];
-    T2 pey = col[(size_t)phase_row(1, nrow.iky, kmax) * ld];
-    T2 pez = col[(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
+    T2 pex[APL], pey[APL], pez[APL];
+#pragma unroll
+    for (int u = 0; u < APL; u++) {
+        fx[u] = fy[u] = fz[u] = 0.0;
+        pex[u] = col[u][(size_t)phase_row(0, nrow.ikx, kmax) * ld];
+        pey[u] = col[u][(size_t)phase_row(1, nrow.iky, kmax) * ld];
+        pez[u] = col[u][(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
+    }
     for (int r = r0; r < r1; r++) {
         KRow row = nrow;
-        T2 ex = pex, ey = pey, ez0 = pez;
+        T2 e[APL];
+        T A[APL], B[APL];
+#pragma unroll
+        for (int u = 0; u < APL; u++) {
+            e[u] = cmul(cmul(pex[u], pey[u]), pez[u]);
+            A[u] = 0;
+            B[u] = 0;
+        }
         if (r + 1 < r1) {
             nrow = rows[r + 1];
-            pex = col[(size_t)phase_row(0, nrow.ikx, kmax) * ld];
-            pey = col[(size_t)phase_row(1, nrow.iky, kmax) * ld];
-            pez = col[(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
+#pragma unroll
+            for (int u = 0; u < APL; u++) {
+                pex[u] = col[u][(size_t)phase_row(0, nrow.ikx, kmax) * ld];
+                pey[u] = col[u][(size_t)phase_row(1, nrow.iky, kmax) * ld];
+                pez[u] = col[u][(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
+            }
         }
-        T2 e = cmul(cmul(ex, ey), ez0);
-        T A = 0, B = 0;
         const R* rp = rec + row.kbase;
         const int cnt = row.count;
         int c = 0;
         for (; c + 4 <= cnt; c += 4) {
             const R q0 = rp[c], q1 = rp[c + 1], q2 = rp[c + 2], q3 = rp[c + 3];
-            kforce_step(q0, e, ez1, A, B);
-            kforce_step(q1, e, ez1, A, B);
-            kforce_step(q2, e, ez1, A, B);
-            kforce_step(q3, e, ez1, A, B);
+#pragma unroll
+            for (int u = 0; u < APL; u++) {
+                kforce_step(q0, e[u], ez1[u], A[u], B[u]);
+                kforce_step(q1, e[u], ez1[u], A[u], B[u]);
+                kforce_step(q2, e[u], ez1[u], A[u], B[u]);
+                kforce_step(q3, e[u], ez1[u], A[u], B[u]);
+            }
         }
         for (; c < cnt; c++) {
             const R q0 = rp[c];
-            kforce_step(q0, e, ez1, A, B);
+#pragma unroll
+            for (int u = 0; u < APL; u++) kforce_step(q0, e[u], ez1[u], A[u], B[u]);
         }
-        fx += (double)(A * (tgx * (T)row.ikx));
-        fy += (double)(A * (tgy * (T)row.iky));
-        fz += (double)(B * tgz);
+#pragma unroll
+        for (int u = 0; u < APL; u++) {
+            fx[u] += (double)(A[u] * (tgx * (T)row.ikx));
+            fy[u] += (double)(A[u] * (tgy * (T)row.iky));
+            fz[u] += (double)(B[u] * tgz);
+        }
     }
-    kspace_block_store(fx, fy, fz, n, ablock, split, fk);
+#pragma unroll
+    for (int u = 0; u < APL; u++) {
+        if (u > 0) __syncthreads();  // the LDS of the previous store is read before it is reused
+        kspace_block_store(fx[u], fy[u], fz[u], n, ablock * APL + u, split, fk);
+    }
 }
 
 // ---- mixed-form k-space ----------------------------------------------------------
