@@ -254,6 +254,7 @@ __global__ __launch_bounds__(1024) void k_cell_small(const double4* __restrict__
                 int c = slot_cell[sl];
                 int o0 = off[c], o1 = off[c + 1];
                 int rank = 0;
+#pragma unroll 8
                 for (int q = o0; q < o1; q++) rank += tmp[q] < v;
                 d = o0 + rank;
             }
@@ -451,6 +452,62 @@ constexpr int kCellTile = 2048;  // candidates staged per tile (40 KB of LDS)
 constexpr int kCellHome = 64;    // home atoms per chunk
 constexpr int kCellMaxSeg = 128; // (2 reach + 1)^2 rows x 3 runs <= 75
 
+// The scan of a staged candidate tile against the home atoms of a workgroup
+// (both Verlet builds): each wave tests every staged record against two home
+// atoms at once, branch-free, and appends the accepted ones to the home
+// atoms' rows (home_row: row index) in candidate order with ballot prefix
+// counts; home_cnt carries the row lengths across tiles.
+template <bool kMinImage>
+__device__ __forceinline__ void tile_scan(const float4* tile, const int* tidx, int tl, const float4* home,
+                                          const int* home_row, int* home_cnt, int hn, int* __restrict__ nbr, int cap,
+                                          float rl2f, const Box& b, const Grid& g) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const float Lx = (float)b.L[0], Ly = (float)b.L[1], Lz = (float)b.L[2];
+    const float ix = (float)b.inv[0], iy = (float)b.inv[1], iz = (float)b.inv[2];
+    const bool nx1 = b.periodic && g.reach[0] == 0, ny1 = b.periodic && g.reach[1] == 0,
+               nz1 = b.periodic && g.reach[2] == 0;
+    constexpr int NW = kCellBlock / 64;
+    auto accept = [&](float4 pi, float4 cj, bool live) {
+        float ddx = pi.x - cj.x, ddy = pi.y - cj.y, ddz = pi.z - cj.z;
+        if (kMinImage) {
+            if (nx1) ddx -= rintf(ddx * ix) * Lx;
+            if (ny1) ddy -= rintf(ddy * iy) * Ly;
+            if (nz1) ddz -= rintf(ddz * iz) * Lz;
+        }
+        float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        return live & (__float_as_int(cj.w) != __float_as_int(pi.w)) & (d2 < rl2f);
+    };
+    // home atoms h and h + NW of this wave together (the second
+    // one repeats the first when absent, and is not written)
+    for (int h = wave; h < hn; h += 2 * NW) {
+        const bool two = h + NW < hn;
+        const int hb = two ? h + NW : h;
+        const float4 pa = home[h], pb = home[hb];
+        int ca = home_cnt[h], cb = home_cnt[hb];
+        int* rowa = nbr + (size_t)home_row[h] * cap;
+        int* rowb = nbr + (size_t)home_row[hb] * cap;
+#pragma unroll 2
+        for (int p = 0; p < tl; p += 64) {
+            const int e = p + lane;  // < kCellTile: tl <= kCellTile, a multiple of 64
+            const float4 cj = tile[e];
+            const int jj = tidx[e];
+            const bool live = e < tl;
+            const bool aa = accept(pa, cj, live), ab = accept(pb, cj, live) & two;
+            const unsigned long long ba = __ballot(aa), bb = __ballot(ab);
+            const int sa = ca + __popcll(ba & below), sb = cb + __popcll(bb & below);
+            if (aa && sa < cap) rowa[sa] = jj;
+            if (ab && sb < cap) rowb[sb] = jj;
+            ca += __popcll(ba);
+            cb += __popcll(bb);
+        }
+        if (lane == 0) {
+            home_cnt[h] = ca;
+            if (two) home_cnt[hb] = cb;
+        }
+    }
+}
+
 struct CellsArgs {
     const float4* cpos;
     const int* sorted;
@@ -491,15 +548,12 @@ __device__ __forceinline__ void nlist_cells_block(const CellsArgs& args, int blk
     __shared__ int seg_s0[kCellMaxSeg];
     __shared__ float seg_sh[kCellMaxSeg][3];
     __shared__ float4 home[kCellHome];
+    __shared__ int home_row[kCellHome];
     __shared__ int home_cnt[kCellHome];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int c = blk * kCellBlock + tid; c < g.ncell; c += nblk * kCellBlock) count_zero[c] = 0;
     const float Lx = (float)b.L[0], Ly = (float)b.L[1], Lz = (float)b.L[2];
-    const float ix = (float)b.inv[0], iy = (float)b.inv[1], iz = (float)b.inv[2];
-    const bool nx1 = b.periodic && g.reach[0] == 0, ny1 = b.periodic && g.reach[1] == 0,
-               nz1 = b.periodic && g.reach[2] == 0;
     const int ey = 2 * g.reach[1] + 1, nseg = (2 * g.reach[2] + 1) * ey * 3;
-    const unsigned long long below = (1ull << lane) - 1ull;
     for (int c = blk; c < g.ncell; c += nblk) {
         const int hs = start[c], H = start[c + 1] - hs;
         if (H == 0) continue;  // uniform over the workgroup
@@ -570,6 +624,7 @@ __device__ __forceinline__ void nlist_cells_block(const CellsArgs& args, int blk
             const int hn = min(kCellHome, H - h0);
             if (tid < hn) {
                 home[tid] = cpos[hs + h0 + tid];
+                home_row[tid] = hs + h0 + tid;
                 home_cnt[tid] = 0;
             }
             for (int m0 = 0; m0 < M; m0 += kCellTile) {
@@ -606,45 +661,7 @@ __device__ __forceinline__ void nlist_cells_block(const CellsArgs& args, int blk
                     }
                 }
                 __syncthreads();
-                constexpr int NW = kCellBlock / 64;
-                auto accept = [&](float4 pi, float4 cj, bool live) {
-                    float ddx = pi.x - cj.x, ddy = pi.y - cj.y, ddz = pi.z - cj.z;
-                    if (kMinImage) {
-                        if (nx1) ddx -= rintf(ddx * ix) * Lx;
-                        if (ny1) ddy -= rintf(ddy * iy) * Ly;
-                        if (nz1) ddz -= rintf(ddz * iz) * Lz;
-                    }
-                    float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    return live & (__float_as_int(cj.w) != __float_as_int(pi.w)) & (d2 < rl2f);
-                };
-                // home atoms h and h + NW of this wave together (the second
-                // one repeats the first when absent, and is not written)
-                for (int h = wave; h < hn; h += 2 * NW) {
-                    const bool two = h + NW < hn;
-                    const int hb = two ? h + NW : h;
-                    const float4 pa = home[h], pb = home[hb];
-                    int ca = home_cnt[h], cb = home_cnt[hb];
-                    int* rowa = nbr + (size_t)(hs + h0 + h) * cap;
-                    int* rowb = nbr + (size_t)(hs + h0 + hb) * cap;
-#pragma unroll 2
-                    for (int p = 0; p < tl; p += 64) {
-                        const int e = p + lane;  // < kCellTile: tl <= kCellTile, a multiple of 64
-                        const float4 cj = tile[e];
-                        const int jj = tidx[e];
-                        const bool live = e < tl;
-                        const bool aa = accept(pa, cj, live), ab = accept(pb, cj, live) & two;
-                        const unsigned long long ba = __ballot(aa), bb = __ballot(ab);
-                        const int sa = ca + __popcll(ba & below), sb = cb + __popcll(bb & below);
-                        if (aa && sa < cap) rowa[sa] = jj;
-                        if (ab && sb < cap) rowb[sb] = jj;
-                        ca += __popcll(ba);
-                        cb += __popcll(bb);
-                    }
-                    if (lane == 0) {
-                        home_cnt[h] = ca;
-                        if (two) home_cnt[hb] = cb;
-                    }
-                }
+                tile_scan<kMinImage>(tile, tidx, tl, home, home_row, home_cnt, hn, nbr, cap, rl2f, b, g);
                 __syncthreads();
             }
             if (tid < hn) {
