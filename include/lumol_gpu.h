@@ -234,7 +234,8 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
  * the device kernels; 0 leaves the built-in size-based choice:
  *   pair_lanes    lanes per atom in the pair and neighbour-list kernels (4..64)
  *   kspace_split  row ranges of the k-space force kernel (1..32)
- *   rho_width     atoms per block of the rho(k) kernel (64, 128, 256, 512)
+ *   rho_width     atoms per block of the row form of the rho(k) kernel (64,
+ *                 128, 256, 512)
  *   nlist_build   neighbour-list build: 1 lanes per atom, 2 one workgroup per
  *                 cell with candidates staged in LDS (default: cells when
  *                 there are >= 128 cells and >= 8 atoms per cell)
@@ -243,6 +244,10 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
  *                 launches while a kernel profile is active).  On ROCm 7.0
  *                 the replay measured 4-18% slower than eager launches, so
  *                 it is off by default.
+ *   rho_kind      rho(k) kernel: 1 row form (one lane per run of four
+ *                 k-vectors, a loop over atoms; the default), 2 matrix-core
+ *                 form (rho as a complex product over atoms on v_mfma
+ *                 16x16x4; kmax <= 16, else the row form).
  * lumol_amd.tuning picks them per size bucket by timing the candidates.
  * Results do not depend on them beyond summation order. */
 typedef struct lg_tuning {
@@ -251,6 +256,7 @@ typedef struct lg_tuning {
     int rho_width;
     int nlist_build;
     int md_graph;
+    int rho_kind;
 } lg_tuning;
 int lg_system_set_tuning(lg_system* sys, const lg_tuning* tuning);
 /* The values in effect (after the automatic choice); kspace_split and

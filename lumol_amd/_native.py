@@ -48,7 +48,7 @@ class Profile(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("pair_lanes", ctypes.c_int), ("kspace_split", ctypes.c_int), ("rho_width", ctypes.c_int),
-                ("nlist_build", ctypes.c_int), ("md_graph", ctypes.c_int)]
+                ("nlist_build", ctypes.c_int), ("md_graph", ctypes.c_int), ("rho_kind", ctypes.c_int)]
 
 
 class EnergyTerms(ctypes.Structure):

@@ -188,6 +188,9 @@ struct lg_system {
     DevBuf<KRow> rows;
     DevBuf<KUnit> units;         // rho pass work units (up to 4 k-vectors of a row)
     DevBuf<int> kslot;           // k -> slot of the rho partials (j * nunits + unit)
+    // matrix-core rho: column tiles of (iky, ikz), output -> k map, identity slots
+    int mfma_nt = 0;
+    DevBuf<int> mcols, mkmap, kslot_id;
     double kg[3] = {0, 0, 0};    // 2 pi / L per axis
     DevBuf<double> efac, vfac;
     DevBuf<KRec> krec;           // rho(k) and force factor per k
@@ -509,6 +512,44 @@ int prepare_params(lg_system* s) {
 }
 
 // ewald.rs:130-183 and :353-378 for an orthorhombic cell
+// Launch shape of the matrix-core rho pass: atoms per stage and tiles per
+// wave, the first candidate with at least 256 blocks (one per CU) -- wider
+// stages and more tiles per wave share more staging and operand work --
+// then stages merged per block so that the grid stays near 1024-2048 blocks
+// and the partials (chunks x k-vectors) few.
+struct RhoShape {
+    int width, wlog, cpb, ngroups, nchunks;
+};
+// dynamic LDS of a matrix-core rho launch: its own, or the Verlet build's
+// when the two share the launch and the buffer
+static size_t rho_mfma_bytes(int width, int kmax, bool mixed) {
+    const size_t r = mixed ? rho_mfma_lds<float>(width, kmax) : rho_mfma_lds<double>(width, kmax);
+    return std::max(r, sizeof(CellsLds));
+}
+// Launch shape of the matrix-core rho pass: the widest stage (atoms staged
+// per block: wider shares more staging per k) that still gives at least 256
+// blocks (one per CU) and fits the LDS, then stages merged per block so that
+// the grid stays near 1024-2048 blocks and the partials (chunks x k-vectors)
+// few.
+static RhoShape rho_mfma_shape(const lg_system* s, bool mixed) {
+    // widths 16..128 (FP32) or 16..64 (FP64): the kernel's trips take four
+    // atoms of each of its four sub-waves
+    const int wl0 = mixed ? 7 : 6, wlmin = 4;
+    RhoShape sh{};
+    sh.ngroups = (s->mfma_nt + 2 * kRhoMfmaTpw - 1) / (2 * kRhoMfmaTpw);
+    sh.wlog = wlmin;
+    for (int wl = wl0; wl >= wlmin; wl--) {
+        if (rho_mfma_bytes(1 << wl, s->coul_kmax, mixed) > s->ctx->max_shared) continue;
+        sh.wlog = wl;
+        if (size_t(blocks_for(s->ld, 1 << wl)) * sh.ngroups >= 256) break;
+    }
+    sh.width = 1 << sh.wlog;
+    const int stages = blocks_for(s->ld, sh.width);
+    sh.cpb = std::max(1, int(size_t(stages) * sh.ngroups / 1024));
+    sh.nchunks = (stages + sh.cpb - 1) / sh.cpb;
+    return sh;
+}
+
 int prepare_kspace(lg_system* s) {
     s->kspace_dirty = false;
     s->nk = 0;
@@ -579,6 +620,26 @@ int prepare_kspace(lg_system* s) {
     for (int u = 0; u < s->nunits; u++)
         for (int j = 0; j < units[u].nv; j++) kslot[units[u].kb + j] = j * s->nunits + u;
     for (int d = 0; d < 3; d++) s->kg[d] = 2.0 * kPi * inv[d];
+    // matrix-core rho: distinct (iky, ikz) columns in tiles of 16; rows ikx
+    // 0..15 cover the set when kmax <= 16
+    std::vector<int> mcols, mkmap, kid(s->nk);
+    s->mfma_nt = 0;
+    if (kmax <= 16) {
+        std::vector<std::pair<int, int>> cl;
+        for (int k = 0; k < s->nk; k++) cl.push_back({idx[k].y, idx[k].z});
+        std::sort(cl.begin(), cl.end());
+        cl.erase(std::unique(cl.begin(), cl.end()), cl.end());
+        const int nt = int((cl.size() + 15) / 16);
+        mcols.assign(size_t(nt) * 16, (kmax << 16) | kmax);
+        for (size_t j = 0; j < cl.size(); j++) mcols[j] = ((cl[j].first + kmax) << 16) | (cl[j].second + kmax);
+        mkmap.assign(size_t(nt) * 256, -1);
+        for (int k = 0; k < s->nk; k++) {
+            int j = int(std::lower_bound(cl.begin(), cl.end(), std::make_pair(idx[k].y, idx[k].z)) - cl.begin());
+            mkmap[size_t(j / 16) * 256 + idx[k].x * 16 + j % 16] = k;
+        }
+        s->mfma_nt = nt;
+    }
+    for (int k = 0; k < s->nk; k++) kid[k] = k;
     std::vector<double> vflat;
     for (int q = 0; q < 6; q++) vflat.insert(vflat.end(), vf[q].begin(), vf[q].end());
     hipStream_t st = S(s);
@@ -586,6 +647,11 @@ int prepare_kspace(lg_system* s) {
     LG_TRY(upload(s->rows, rows, st));
     LG_TRY(upload(s->units, units, st));
     LG_TRY(upload(s->kslot, kslot, st));
+    if (s->mfma_nt > 0) {
+        LG_TRY(upload(s->mcols, mcols, st));
+        LG_TRY(upload(s->mkmap, mkmap, st));
+        LG_TRY(upload(s->kslot_id, kid, st));
+    }
     LG_TRY(upload(s->efac, efac, st));
     LG_TRY(upload(s->vfac, vflat, st));
     LG_TRY(upload(s->krec, rec, st));
@@ -602,8 +668,12 @@ int prepare_kspace(lg_system* s) {
     if (s->tune.kspace_split > 0) s->ksplit = std::min(s->tune.kspace_split, std::max(1, s->nrows));
     LG_TRY(s->fk.ensure(size_t(s->ksplit) * s->n));
     s->ld = blocks_for(s->n, 64) * 64;
-    // rho partials for the narrowest atom chunk (64)
-    LG_TRY(s->rho_partial.ensure(size_t(blocks_for(s->ld, 64)) * 4 * s->nunits));
+    // rho partials: the row form's narrowest atom chunk (64) or the
+    // matrix-core form's chunks, whichever needs more
+    size_t npart = size_t(blocks_for(s->ld, 64)) * 4 * s->nunits;
+    if (s->mfma_nt > 0)
+        for (int m = 0; m < 2; m++) npart = std::max(npart, size_t(rho_mfma_shape(s, m == 1).nchunks) * s->nk);
+    LG_TRY(s->rho_partial.ensure(npart));
     LG_TRY(s->phase_tab.ensure(size_t(phase_rows(kmax)) * s->ld));
     LG_TRY(s->phase_q.ensure(s->ld));
     LG_TRY(s->phase_tab32.ensure(size_t(phase_rows(kmax)) * s->ld));
@@ -742,6 +812,16 @@ static int rho_chunk(const lg_system* s) {
     return w;
 }
 
+// The matrix-core rho form when asked for (rho_kind 2; the row form is the
+// default: it measured as fast or faster on the bench systems), kmax <= 16
+// and its LDS (the larger of its own and the Verlet build's, which share a
+// launch) within the device limit.
+static bool rho_use_mfma(const lg_system* s, bool mixed) {
+    if (s->mfma_nt <= 0 || s->tune.rho_kind != 2) return false;
+    const RhoShape sh = rho_mfma_shape(s, mixed);
+    return rho_mfma_bytes(sh.width, s->coul_kmax, mixed) <= s->ctx->max_shared;
+}
+
 // The next force evaluation rebuilds the list whatever the displacements.
 static int force_rebuild(lg_system* s) {
     k_set_flag<<<1, 1, 0, S(s)>>>(s->flags.p, 1);
@@ -825,6 +905,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     // rho(k) partials first: their reduction runs as the last blocks of the
     // pair launch, and the k-space forces after it
     int nchunks = 0;
+    const bool use_mfma = s->coul_kind == COUL_EWALD && s->nk > 0 && rho_use_mfma(s, mixed);
     if (s->coul_kind == COUL_EWALD && s->nk > 0) {
         int kmax = s->coul_kmax;
         int nub = blocks_for(s->nunits, 64);
@@ -838,7 +919,40 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         const int ncb = cells ? (s->grid.ncell + 7) / 8 * 8 : 0;
         const CellsArgs& ca = s->cells_args;
         const bool near = s->cells_nearest;
-        if (mixed) {
+        if (use_mfma) {
+            const RhoShape sh = rho_mfma_shape(s, mixed);
+            nchunks = sh.nchunks;
+            const unsigned mgrid = xcd_grid(sh.ngroups, sh.nchunks);
+            if (mixed) {
+                if (!phase_done)
+                    k_phase32<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                                                                                  s->phase_tab32.p, s->phase_q32.p);
+                RhoMfmaArgs<float, float2> ra{s->pos.p, n, s->box, s->ld, kmax, sh.wlog, sh.cpb,
+                                              s->mfma_nt, sh.ngroups, sh.nchunks, s->mcols.p, s->mkmap.p, s->nk,
+                                              s->rho_partial.p};
+                const size_t lds = rho_mfma_bytes(sh.width, kmax, true);
+                if (!cells)
+                    k_rho_mfma<float, float2><<<mgrid, 512, lds, st>>>(ra);
+                else if (near)
+                    k_cells_rho_mfma<true, float, float2><<<ncb + mgrid, kCellBlock, lds, st>>>(ca, ra, ncb);
+                else
+                    k_cells_rho_mfma<false, float, float2><<<ncb + mgrid, kCellBlock, lds, st>>>(ca, ra, ncb);
+            } else {
+                if (!phase_done)
+                    k_phase<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                                                                                s->phase_tab.p, s->phase_q.p);
+                RhoMfmaArgs<double, double2> ra{s->pos.p, n, s->box, s->ld, kmax, sh.wlog, sh.cpb,
+                                                s->mfma_nt, sh.ngroups, sh.nchunks, s->mcols.p, s->mkmap.p, s->nk,
+                                                s->rho_partial.p};
+                const size_t lds = rho_mfma_bytes(sh.width, kmax, false);
+                if (!cells)
+                    k_rho_mfma<double, double2><<<mgrid, 512, lds, st>>>(ra);
+                else if (near)
+                    k_cells_rho_mfma<true, double, double2><<<ncb + mgrid, kCellBlock, lds, st>>>(ca, ra, ncb);
+                else
+                    k_cells_rho_mfma<false, double, double2><<<ncb + mgrid, kCellBlock, lds, st>>>(ca, ra, ncb);
+            }
+        } else if (mixed) {
             if (!phase_done)
                 k_phase32<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
                                                                               s->phase_tab32.p, s->phase_q32.p);
@@ -917,8 +1031,8 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     if (kspace) {
         rr.partial = s->rho_partial.p;
         rr.nchunks = nchunks;
-        rr.nslot = 4 * s->nunits;
-        rr.kslot = s->kslot.p;
+        rr.nslot = use_mfma ? s->nk : 4 * s->nunits;
+        rr.kslot = use_mfma ? s->kslot_id.p : s->kslot.p;
         rr.nk = s->nk;
         rr.rho = s->rho.p;
         rr.rec = mixed ? nullptr : s->krec.p;
@@ -1351,8 +1465,9 @@ int lg_system_destroy(lg_system* s) {
     DevBuf<int>* ibufs[] = {&s->cellid, &s->slot, &s->count, &s->start, &s->tmp, &s->sorted, &s->nbr, &s->nnbr,
 
                             &s->flags, &s->pair_restr, &s->slot_start, &s->d_mol_start,
-                            &s->mc_cell_count, &s->mc_atom_cell};
+                            &s->mc_cell_count, &s->mc_atom_cell, &s->kslot, &s->mcols, &s->mkmap, &s->kslot_id};
     for (auto* b : ibufs) b->release();
+    s->units.release();
     s->phase_tab.release();
     s->phase_q.release();
     s->phase_tab32.release();
@@ -1527,6 +1642,7 @@ int lg_system_set_tuning(lg_system* s, const lg_tuning* t) {
         return fail(LG_E_INVALID, "rho_width must be 0, 64, 128, 256 or 512");
     if (t->nlist_build < 0 || t->nlist_build > 2) return fail(LG_E_INVALID, "nlist_build must be 0, 1 or 2");
     if (t->md_graph < 0 || t->md_graph > 2) return fail(LG_E_INVALID, "md_graph must be 0, 1 or 2");
+    if (t->rho_kind < 0 || t->rho_kind > 2) return fail(LG_E_INVALID, "rho_kind must be 0, 1 or 2");
     s->tune = *t;
     s->kspace_dirty = s->nlist_dirty = true;
     return LG_OK;
@@ -1542,6 +1658,9 @@ int lg_system_get_tuning(lg_system* s, lg_tuning* t) {
     t->nlist_build = cell_build(s) ? 2 : 1;
     t->md_graph = s->tune.md_graph == 2 ? 2 : 1;
     t->rho_width = (s->coul_kind == COUL_EWALD && s->nk > 0) ? rho_chunk(s) : 0;
+    t->rho_kind = (s->coul_kind == COUL_EWALD && s->nk > 0)
+                      ? (rho_use_mfma(s, s->precision == LG_PRECISION_MIXED) ? 2 : 1)
+                      : 0;
     return LG_OK;
 }
 

@@ -487,7 +487,6 @@ __device__ __forceinline__ void tile_scan(const float4* tile, const int* tidx, i
         int ca = home_cnt[h], cb = home_cnt[hb];
         int* rowa = nbr + (size_t)home_row[h] * cap;
         int* rowb = nbr + (size_t)home_row[hb] * cap;
-#pragma unroll 2
         for (int p = 0; p < tl; p += 64) {
             const int e = p + lane;  // < kCellTile: tl <= kCellTile, a multiple of 64
             const float4 cj = tile[e];
@@ -526,8 +525,21 @@ struct CellsArgs {
 
 // kMinImage: some axis has a single cell (nearest image along it).  Block
 // `blk` of `nblk` (cells blk, blk + nblk, ...).
+// LDS of one block of the cell build (about 44 KB); a launch that shares its
+// blocks with another pass hands both the same buffer.
+struct CellsLds {
+    float4 tile[kCellTile];
+    float4 home[kCellHome];
+    int tidx[kCellTile];
+    int seg_base[kCellMaxSeg + 1];
+    int seg_s0[kCellMaxSeg];
+    float seg_sh[kCellMaxSeg][3];
+    int home_row[kCellHome];
+    int home_cnt[kCellHome];
+};
+
 template <bool kMinImage>
-__device__ __forceinline__ void nlist_cells_block(const CellsArgs& args, int blk, int nblk) {
+__device__ __forceinline__ void nlist_cells_block(const CellsArgs& args, int blk, int nblk, CellsLds& lds) {
     const float4* __restrict__ cpos = args.cpos;
     const int* __restrict__ sorted = args.sorted;
     const int* __restrict__ start = args.start;
@@ -542,14 +554,14 @@ __device__ __forceinline__ void nlist_cells_block(const CellsArgs& args, int blk
     double4* __restrict__ ref = args.ref;
     int* __restrict__ flags = args.flags;
     if (flags[0] == 0) return;
-    __shared__ float4 tile[kCellTile];
-    __shared__ int tidx[kCellTile];
-    __shared__ int seg_base[kCellMaxSeg + 1];
-    __shared__ int seg_s0[kCellMaxSeg];
-    __shared__ float seg_sh[kCellMaxSeg][3];
-    __shared__ float4 home[kCellHome];
-    __shared__ int home_row[kCellHome];
-    __shared__ int home_cnt[kCellHome];
+    float4* tile = lds.tile;
+    int* tidx = lds.tidx;
+    int* seg_base = lds.seg_base;
+    int* seg_s0 = lds.seg_s0;
+    float(*seg_sh)[3] = lds.seg_sh;
+    float4* home = lds.home;
+    int* home_row = lds.home_row;
+    int* home_cnt = lds.home_cnt;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int c = blk * kCellBlock + tid; c < g.ncell; c += nblk * kCellBlock) count_zero[c] = 0;
     const float Lx = (float)b.L[0], Ly = (float)b.L[1], Lz = (float)b.L[2];
@@ -682,7 +694,8 @@ __device__ __forceinline__ void nlist_cells_block(const CellsArgs& args, int blk
 
 template <bool kMinImage>
 __global__ __launch_bounds__(kCellBlock) void k_nlist_cells(CellsArgs ca) {
-    nlist_cells_block<kMinImage>(ca, blockIdx.x, gridDim.x);
+    __shared__ CellsLds lds;
+    nlist_cells_block<kMinImage>(ca, blockIdx.x, gridDim.x, lds);
 }
 
 
@@ -1441,19 +1454,24 @@ __device__ __forceinline__ void phase_axis64(double x, double inv, int axis, int
 
 // Mixed form: each phase n k_axis x formed and reduced to [-pi, pi] in FP64,
 // then sincosf.
+__device__ __forceinline__ float2 phase32_n(double phase, int k) {
+    const double two_pi = 2.0 * kPi, inv_two_pi = 1.0 / (2.0 * kPi);
+    double th = k * phase;
+    th -= two_pi * rint(th * inv_two_pi);
+    float sn, cs;
+    sincosf((float)th, &sn, &cs);
+    return make_float2(cs, sn);
+}
+
 __device__ __forceinline__ void phase_axis32(double x, double inv, int axis, int kmax, float2* __restrict__ tab,
                                              int ld, int a, int k0 = 1, int k1 = -1) {
     double phase = (2.0 * kPi * inv) * x;
-    const double two_pi = 2.0 * kPi, inv_two_pi = 1.0 / (2.0 * kPi);
     if (k1 < 0) k1 = kmax;
     if (k0 == 1) tab[(size_t)phase_row(axis, 0, kmax) * ld + a] = make_float2(1.f, 0.f);
     for (int k = k0; k <= k1; k++) {
-        double th = k * phase;
-        th -= two_pi * rint(th * inv_two_pi);
-        float sn, cs;
-        sincosf((float)th, &sn, &cs);
-        tab[(size_t)phase_row(axis, k, kmax) * ld + a] = make_float2(cs, sn);
-        if (axis != 0) tab[(size_t)phase_row(axis, -k, kmax) * ld + a] = make_float2(cs, -sn);
+        const float2 e = phase32_n(phase, k);
+        tab[(size_t)phase_row(axis, k, kmax) * ld + a] = e;
+        if (axis != 0) tab[(size_t)phase_row(axis, -k, kmax) * ld + a] = make_float2(e.x, -e.y);
     }
 }
 
@@ -1603,12 +1621,235 @@ __global__ __launch_bounds__(512) void k_rho_rows(RhoRowsArgs<T, T2> ra) {
 // first half-step and the cell list wrote.
 template <bool kMinImage, typename T, typename T2>
 __global__ __launch_bounds__(kCellBlock) void k_cells_rho(CellsArgs ca, RhoRowsArgs<T, T2> ra, int ncb) {
+    __shared__ CellsLds lds;
     if ((int)blockIdx.x < ncb)
-        nlist_cells_block<kMinImage>(ca, blockIdx.x, ncb);
+        nlist_cells_block<kMinImage>(ca, blockIdx.x, ncb, lds);
     else
         rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb);
 }
 
+
+// ---- rho(k) on the matrix cores ---------------------------------------------------
+// With e^{ik.r} = Ex[ikx] * Eyz[(iky, ikz)] and Eyz = q Ey Ez, the structure
+// factor is a complex matrix product over atoms:
+//     rho[ikx][col] = sum_i Ex[i][ikx] * Eyz[i][col],   col = (iky, ikz)
+// (the same triple product as ewald.rs:664-673, grouped differently).  Rows
+// ikx = 0..15 (kmax <= 16), columns = the distinct (iky, ikz) of the k set in
+// tiles of 16; the reduction index runs over (atom, re/im), so one
+// v_mfma 16x16x4 adds two atoms to a 16 x 16 tile of Re(rho) and another to
+// Im(rho):
+//     Re: A = (Ex.re, -Ex.im), B = (Eyz.re, Eyz.im)
+//     Im: A = (Ex.im,  Ex.re), B = the same
+// The reduction over atoms -- across lanes in a lane-per-atom layout --
+// happens inside the matrix core.  Block = 8 waves on `width` atoms whose
+// phase factors it forms from the positions into LDS as
+// [atom][X 0..15 | Ey -kmax..kmax | Ez -kmax..kmax] plus the charges
+// (applied to the A operand); wave
+// (sub = w & 3, half = w >> 2) sums atoms [sub * width/4, +width/4) of each
+// stage for 3 tiles; a block covers 6 tiles (group) and `cpb`
+// consecutive stages (chunk).  The four sub-waves' tiles are added in order
+// in FP64 and written to partial[chunk * nk + k] for the valid k only
+// (kmap); the fixed-order chunk reduction of rho_reduce_block follows.
+typedef float rho_f4 __attribute__((ext_vector_type(4)));
+typedef double rho_d4 __attribute__((ext_vector_type(4)));
+template <typename T> struct MfmaAcc;
+template <> struct MfmaAcc<float> { typedef rho_f4 type; };
+template <> struct MfmaAcc<double> { typedef rho_d4 type; };
+__device__ __forceinline__ rho_f4 mfma16x16x4(float a, float b, rho_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ rho_d4 mfma16x16x4(double a, double b, rho_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+// Both take A[i][k] from lane i + 16 k and B[k][j] from lane j + 16 k; output
+// r of lane l is D[row][l % 16] with row = 4 (l / 16) + r for the FP32 form
+// and l / 16 + 4 r for the FP64 form (measured on gfx950).
+__device__ __forceinline__ int mfma16_row(float, int l16, int r) { return 4 * l16 + r; }
+__device__ __forceinline__ int mfma16_row(double, int l16, int r) { return l16 + 4 * r; }
+
+constexpr int kRhoMfmaTpw = 3;  // tiles per wave (a block: 2 x 3 tiles)
+
+template <typename T, typename T2>
+struct RhoMfmaArgs {
+    const double4* pos;  // positions, charge in w
+    int n;
+    Box box;
+    int ld, kmax;
+    int wlog;     // log2 of the atoms per stage: 4..7 (a multiple of 16)
+    int cpb;      // stages per block
+    int nt;       // column tiles
+    int ngroups;  // tile groups (blocks per chunk)
+    int nchunks;
+    const int* cols;  // nt * 16: (iky + kmax) << 16 | (ikz + kmax)
+    const int* kmap;  // nt * 256: k of output (ikx, col), or -1
+    int nk;
+    double2* partial;
+};
+
+// LDS entries per staged atom: X (16) + Ey (2 kmax + 1) + Ez (2 kmax + 1),
+// plus one so that the stride is odd and the stores of one row (consecutive
+// atoms, 8 or 16 bytes each) fall in distinct banks.
+__host__ __device__ inline int rho_mfma_stride(int kmax) {
+    return 16 + 2 * (2 * kmax + 1) + 1;
+}
+// LDS bytes of one block: the staged rows and charges, or the sub-wave
+// tiles, whichever is larger (the second reuses the first).
+template <typename T>
+__host__ __device__ inline size_t rho_mfma_lds(int width, int kmax) {
+    size_t stage = size_t(width) * rho_mfma_stride(kmax) * 2 * sizeof(T) + size_t(width) * sizeof(T);
+    size_t red = size_t(2) * 3 * kRhoMfmaTpw * 2 * 4 * 64 * sizeof(T);
+    return stage > red ? stage : red;
+}
+
+template <typename T, typename T2>
+__device__ __forceinline__ void rho_mfma_block(const RhoMfmaArgs<T, T2>& ra, int L, T2* __restrict__ smem) {
+    typedef typename MfmaAcc<T>::type acc_t;
+    int grp, chunk;
+    if (!xcd_block_at(L, ra.ngroups, ra.nchunks, grp, chunk)) return;
+    const int kmax = ra.kmax, wl = ra.wlog, W = 1 << wl, nt = ra.nt;
+    constexpr int tpw = kRhoMfmaTpw;
+    const int ny = 2 * kmax + 1, S = rho_mfma_stride(kmax);
+    T* qs = reinterpret_cast<T*>(smem + W * S);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int sub = w & 3, half = w >> 2;
+    const int kk = lane >> 4, c = lane & 15, comp = kk & 1;
+    const int t0 = grp * 2 * tpw + half * tpw;
+    int offy[kRhoMfmaTpw], offz[kRhoMfmaTpw];
+#pragma unroll
+    for (int t = 0; t < kRhoMfmaTpw; t++) {
+        const int tt = t0 + t;
+        // tiles past the last compute on a stand-in column and are not stored
+        const int ci = tt < nt ? ra.cols[tt * 16 + c] : ((kmax << 16) | kmax);
+        offy[t] = 16 + (ci >> 16);
+        offz[t] = 16 + ny + (ci & 0xffff);
+    }
+    // the k-vector of each output, fetched now so that its latency is hidden
+    int kmp[kRhoMfmaTpw][4];
+#pragma unroll
+    for (int t = 0; t < kRhoMfmaTpw; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            kmp[t][r] = (sub == 0 && t0 + t < nt) ? ra.kmap[(t0 + t) * 256 + mfma16_row(T(0), kk, r) * 16 + c] : -1;
+    acc_t are[kRhoMfmaTpw], aim[kRhoMfmaTpw];
+#pragma unroll
+    for (int t = 0; t < kRhoMfmaTpw; t++) {
+        are[t] = acc_t{0, 0, 0, 0};
+        aim[t] = acc_t{0, 0, 0, 0};
+    }
+    const int per = W >> 2;
+    // staging: thread (axis, atom) forms the atom's phase factors along one
+    // axis by the reference's recursion in FP64 (phase_axis64; the mixed form
+    // rounds each factor to FP32 once), and the x-axis thread its charge;
+    // atoms past n are zero
+    for (int stg = 0; stg < ra.cpb; stg++) {
+        const int a0 = (chunk * ra.cpb + stg) * W;
+        if (stg > 0) __syncthreads();  // the previous stage is read before it is replaced
+        if ((int)threadIdx.x < 3 * W) {
+            const int a = threadIdx.x & (W - 1), axis = threadIdx.x >> wl, ga = a0 + a;
+            double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (ga < ra.n) p = ra.pos[ga];
+            const double x = axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
+            const double phase = (2.0 * kPi * ra.box.inv[axis]) * x;
+            T2* row = smem + a * S + (axis == 0 ? 0 : (axis == 1 ? 16 + kmax : 16 + ny + kmax));
+            T2 one;
+            one.x = 1;
+            one.y = 0;
+            row[0] = one;
+            double sn, cs;
+            sincos(phase, &sn, &cs);
+            double ex = cs, ey = sn;
+            // x rows ikx = 0..kmax-1 (the half-space set), zero up to 15
+            const int top = axis == 0 ? kmax - 1 : kmax;
+            for (int k = 1; k <= top; k++) {
+                if (k > 1) {
+                    const double rx = ex * cs - ey * sn, ry = ex * sn + ey * cs;
+                    ex = rx;
+                    ey = ry;
+                }
+                T2 e, cj;
+                e.x = cj.x = (T)ex;
+                e.y = (T)ey;
+                cj.y = -e.y;
+                row[k] = e;
+                if (axis != 0) row[-k] = cj;
+            }
+            if (axis == 0) {
+                T2 zero;
+                zero.x = zero.y = 0;
+                for (int k = kmax > 1 ? kmax : 1; k < 16; k++) row[k] = zero;
+                qs[a] = (T)p.w;
+            }
+        }
+        __syncthreads();
+        const int abase = sub * per + (kk >> 1);
+        const T2* at0 = smem + abase * S;
+        // per = width / 4 is a multiple of 4: two atom pairs per trip
+        for (int p = 0; p < per; p += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; u += 2) {
+                const T2* at = at0 + (p + u) * S;
+                const T2 x = at[c];
+                const T qa = qs[abase + p + u];
+                const T ar = qa * (comp ? -x.y : x.x);
+                const T ai = qa * (comp ? x.x : x.y);
+#pragma unroll
+                for (int t = 0; t < kRhoMfmaTpw; t++) {
+                    const T2 y = at[offy[t]], z = at[offz[t]];
+                    const T b = comp ? (y.x * z.y + y.y * z.x) : (y.x * z.x - y.y * z.y);
+                    are[t] = mfma16x16x4(ar, b, are[t]);
+                    aim[t] = mfma16x16x4(ai, b, aim[t]);
+                }
+            }
+        }
+    }
+    __syncthreads();  // staging reads done: the LDS now holds sub-wave tiles
+    T* red = reinterpret_cast<T*>(smem);
+    auto slot = [&](int s, int t, int part, int r) { return ((((half * 3 + s) * kRhoMfmaTpw + t) * 2 + part) * 4 + r) * 64 + lane; };
+    if (sub > 0) {
+#pragma unroll
+        for (int t = 0; t < kRhoMfmaTpw; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                red[slot(sub - 1, t, 0, r)] = are[t][r];
+                red[slot(sub - 1, t, 1, r)] = aim[t][r];
+            }
+    }
+    __syncthreads();
+    if (sub == 0) {
+#pragma unroll
+        for (int t = 0; t < kRhoMfmaTpw; t++) {
+            const int tt = t0 + t;
+            if (tt >= nt) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int k = kmp[t][r];
+                double sr = (double)are[t][r], si = (double)aim[t][r];
+                for (int s2 = 0; s2 < 3; s2++) {
+                    sr += (double)red[slot(s2, t, 0, r)];
+                    si += (double)red[slot(s2, t, 1, r)];
+                }
+                if (k >= 0) ra.partial[(size_t)chunk * ra.nk + k] = make_double2(sr, si);
+            }
+        }
+    }
+}
+
+template <typename T, typename T2>
+__global__ __launch_bounds__(512) void k_rho_mfma(RhoMfmaArgs<T, T2> ra) {
+    extern __shared__ __align__(16) unsigned char rho_smem[];
+    rho_mfma_block<T, T2>(ra, blockIdx.x, reinterpret_cast<T2*>(rho_smem));
+}
+
+// k_cells_rho with the matrix-core rho blocks; both parts use the one
+// dynamic LDS buffer (max of sizeof(CellsLds) and rho_mfma_lds).
+template <bool kMinImage, typename T, typename T2>
+__global__ __launch_bounds__(kCellBlock) void k_cells_rho_mfma(CellsArgs ca, RhoMfmaArgs<T, T2> ra, int ncb) {
+    extern __shared__ __align__(16) unsigned char rho_smem[];
+    if ((int)blockIdx.x < ncb)
+        nlist_cells_block<kMinImage>(ca, blockIdx.x, ncb, *reinterpret_cast<CellsLds*>(rho_smem));
+    else
+        rho_mfma_block<T, T2>(ra, blockIdx.x - ncb, reinterpret_cast<T2*>(rho_smem));
+}
 
 // k-space forces: one lane per atom, rows of k-vectors (fixed ikx, iky;
 // consecutive ikz) read uniformly; F_i += q_i / (4 pi eps0) *
@@ -1628,8 +1869,8 @@ __device__ __forceinline__ void kspace_wave_rows(int nrows, int split, int nspli
     // wave index made visibly uniform, so that rows[] and the k records are
     // read with scalar loads
     int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    int s0 = (int)(((long)nrows * split) / nsplit);
-    int s1 = (int)(((long)nrows * (split + 1)) / nsplit);
+    const int s0 = (int)(((long)nrows * split) / nsplit);
+    const int s1 = (int)(((long)nrows * (split + 1)) / nsplit);
     r0 = s0 + ((s1 - s0) * w) / 4;
     r1 = s0 + ((s1 - s0) * (w + 1)) / 4;
 }
@@ -1663,9 +1904,9 @@ __device__ __forceinline__ void kforce_step(const R& r, T2& e, T2 ez1, T& A, T& 
 // T = double: FP64 form; T = float: mixed form (FP32 phases, rho and
 // factors; each row's sums are added to FP64 accumulators).  Records are
 // read four at a time with one batch of scalar loads, so that a wave waits
-// once per four k-vectors.  APL atoms per lane (block = 64 APL atoms): the
-// row headers, records and their waits are shared by APL independent
-// recursions.
+// once per four k-vectors (a copy of the block's records in LDS measured no
+// faster).  APL atoms per lane (block = 64 APL atoms): the row headers,
+// records and their waits are shared by APL independent recursions.
 template <typename T, typename T2, typename R, int APL>
 __global__ __launch_bounds__(256) void k_kspace_force(const T2* __restrict__ tab, int ld, int n, int kmax,
                                                       const KRow* __restrict__ rows, int nrows,

@@ -347,23 +347,25 @@ class System:
         self.precision = precision
         self._dirty = True
 
-    def set_tuning(self, pair_lanes=0, kspace_split=0, rho_width=0, nlist_build=0, md_graph=0):
+    def set_tuning(self, pair_lanes=0, kspace_split=0, rho_width=0, nlist_build=0, md_graph=0, rho_kind=0):
         """Launch knobs of the device kernels (0 = built-in choice); see
-        lumol_amd.tuning.autotune.  rho_width: atoms per block of the rho(k)
-        kernel; nlist_build: 1 lanes per atom, 2 one workgroup per cell;
-        md_graph: 1 eager launches (default), 2 HIP-graph replay of MD chunks."""
-        self.tuning_ = (int(pair_lanes), int(kspace_split), int(rho_width), int(nlist_build), int(md_graph))
+        lumol_amd.tuning.autotune.  rho_width: atoms per block of the row form
+        of the rho(k) kernel; nlist_build: 1 lanes per atom, 2 one workgroup
+        per cell; md_graph: 1 eager launches (default), 2 HIP-graph replay of
+        MD chunks; rho_kind: 1 row form, 2 matrix-core form (kmax <= 16)."""
+        self.tuning_ = (int(pair_lanes), int(kspace_split), int(rho_width), int(nlist_build), int(md_graph),
+                        int(rho_kind))
         if self._handle is not None and not self._dirty:
             t = nat.Tuning(*self.tuning_)
             nat.check(nat.lib().lg_system_set_tuning(self._handle, ctypes.byref(t)))
 
     def tuning(self):
         """The launch knobs in effect: dict(pair_lanes, kspace_split, rho_width,
-        nlist_build, md_graph)."""
+        nlist_build, md_graph, rho_kind)."""
         t = nat.Tuning()
         nat.check(nat.lib().lg_system_get_tuning(self._sync(), ctypes.byref(t)))
         return {"pair_lanes": t.pair_lanes, "kspace_split": t.kspace_split, "rho_width": t.rho_width,
-                "nlist_build": t.nlist_build, "md_graph": t.md_graph}
+                "nlist_build": t.nlist_build, "md_graph": t.md_graph, "rho_kind": t.rho_kind}
 
     # -- state -----------------------------------------------------------------
     @property
@@ -459,7 +461,7 @@ class System:
         else:
             nat.check(L.lg_coulomb_clear(h))
         nat.check(L.lg_system_set_precision(h, self.precision))
-        t = nat.Tuning(*getattr(self, "tuning_", (0, 0, 0, 0, 0)))
+        t = nat.Tuning(*getattr(self, "tuning_", (0, 0, 0, 0, 0, 0)))
         nat.check(L.lg_system_set_tuning(h, ctypes.byref(t)))
         self._dirty = False
         self._device_state = True

@@ -1,8 +1,8 @@
 """Per-size-bucket autotuner of the device launch knobs.
 
 The knobs (``lg_tuning``): lanes per atom of the pair and neighbour-list
-kernels, row ranges of the k-space force kernel, atoms per block of the
-rho(k) kernel.  They are tuned one after the other: every candidate value is
+kernels, row ranges of the k-space force kernel, the form of the rho(k)
+kernel (rows, with its atoms per block, or the matrix cores).  They are tuned one after the other: every candidate value is
 timed with the library's HIP-event profiler over `reps` force evaluations of
 the system itself -- the summed device time of all kernel classes, so that a
 choice that speeds one kernel and slows another (more k-space partials to
@@ -44,36 +44,44 @@ def _time_all(system, reps):
 def autotune(system, reps=5, use_cache=True):
     """Picks and applies the knobs for `system`; returns them (dict)."""
     key = bucket(system)
-    keep = tuple(getattr(system, "tuning_", (0, 0, 0, 0, 0))[3:])  # nlist_build, md_graph: left as set
+    keep = tuple(getattr(system, "tuning_", (0, 0, 0, 0, 0, 0))[3:5])  # nlist_build, md_graph: left as set
+
+    def apply(b):
+        system.set_tuning(b["pair_lanes"], b["kspace_split"], b["rho_width"], *keep, rho_kind=b["rho_kind"])
+
     with _lock:
         hit = _cache.get(key) if use_cache else None
     if hit is None:
         system.forces()  # builds the lists and tables once
-        best = {"pair_lanes": 0, "kspace_split": 0, "rho_width": 0}
+        best = {"pair_lanes": 0, "kspace_split": 0, "rho_width": 0, "rho_kind": 0}
+
+        def pick(field, values, **extra):
+            times = {}
+            for v in values:
+                trial = dict(best, **{field: v}, **extra)
+                apply(trial)
+                system.forces()
+                times[v] = _time_all(system, reps)
+            return min(times, key=times.get), times
+
         # pair kernel lanes
-        times = {}
-        for lanes in PAIR_LANES:
-            system.set_tuning(lanes, best["kspace_split"], best["rho_width"], *keep)
-            system.forces()
-            times[lanes] = _time_all(system, reps)
-        best["pair_lanes"] = min(times, key=times.get)
+        best["pair_lanes"], _ = pick("pair_lanes", PAIR_LANES)
         if key[0]:
-            times = {}
-            for split in KSPACE_SPLITS:
-                system.set_tuning(best["pair_lanes"], split, best["rho_width"], *keep)
-                system.forces()
-                times[split] = _time_all(system, reps)
-            best["kspace_split"] = min(times, key=times.get)
-            times = {}
-            for width in RHO_WIDTHS:
-                system.set_tuning(best["pair_lanes"], best["kspace_split"], width, *keep)
-                system.forces()
-                times[width] = _time_all(system, reps)
-            best["rho_width"] = min(times, key=times.get)
+            best["kspace_split"], _ = pick("kspace_split", KSPACE_SPLITS)
+            # rho(k): each width of the row form, then the matrix-core form
+            width, times = pick("rho_width", RHO_WIDTHS, rho_kind=1)
+            trial = dict(best, rho_kind=2)
+            apply(trial)
+            system.forces()
+            t_mfma = _time_all(system, reps) if system.tuning()["rho_kind"] == 2 else float("inf")
+            if t_mfma < times[width]:
+                best["rho_kind"] = 2
+            else:
+                best["rho_kind"], best["rho_width"] = 1, width
         hit = best
         with _lock:
             _cache[key] = hit
-    system.set_tuning(hit["pair_lanes"], hit["kspace_split"], hit["rho_width"], *keep)
+    apply(hit)
     return dict(hit)
 
 
