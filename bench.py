@@ -616,7 +616,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the GPU boxes of this build have one MI355X, and independent "
-              "systems/jobs shard across GPUs without communication (see DESIGN.md). Run with --gpus 1.",
+              "systems/jobs shard across GPUs without communication. Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "mc":

@@ -143,6 +143,19 @@ def lib():
                 raise RuntimeError(
                     f"{LIB_PATH} is missing: build it with lumol_amd.build.build() "
                     "(python __graft_entry__.py build)")
+            # Each library context owns one HIP stream, and contexts run
+            # concurrently (batched systems, serving lanes) only on distinct
+            # hardware queues: with HIP's default of 4 per process a fifth
+            # stream shares a queue and serialises behind another.  A lower
+            # setting is raised to 8 (4 SPC/E systems at once: 2.15x -> 2.65x
+            # the FP64 single-system rate); it takes effect if the HIP runtime
+            # is not initialised yet.
+            try:
+                queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
+            except ValueError:
+                queues = 0
+            if queues < 8:
+                os.environ["GPU_MAX_HW_QUEUES"] = "8"
             handle = ctypes.CDLL(LIB_PATH)
             _declare(handle)
             _lib = handle
