@@ -225,8 +225,9 @@ struct lg_system {
     // snapshot for neighbour-list overflow recovery
     DevBuf<double4> snap_pos, snap_vel, snap_acc;
     // Verlet build deferred by launch_nlist into the rho(k) launch (k_cells_rho)
-    bool cells_pending = false, cells_nearest = false;
+    bool cells_pending = false, cells_nearest = false, direct_pending = false;
     CellsArgs cells_args{};
+    DirectArgs direct_args{};
     // MD chunk captured as a HIP graph, with what it was captured for
     hipGraphExec_t md_exec = nullptr;
     uint64_t md_key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -795,11 +796,30 @@ int prepare(lg_system* s) {
 // there are enough cells to fill the device and enough atoms per cell to keep
 // its waves busy, else TPA lanes per atom (k_nlist_build); the tuning knob
 // nlist_build overrides (1 lanes per atom, 2 cells).
-static bool cell_build(const lg_system* s) {
-    if (s->tune.nlist_build == 1) return false;
-    if (s->tune.nlist_build == 2) return true;
+// Verlet build: 1 lanes per atom over the cell list, 2 one workgroup per
+// cell over the cell list, 3 one workgroup per cell over the atoms' cell
+// tags (no cell list; up to 32768 atoms and 1023 cells per axis).  Default:
+// the per-cell builds when there are >= 128 cells and >= 8 atoms per cell,
+// the direct one where it applies.
+static bool direct_fits(const lg_system* s) {
+    return s->n <= 32768 && s->grid.n[0] <= 1023 && s->grid.n[1] <= 1023 && s->grid.n[2] <= 1023;
+}
+static bool cells_dense(const lg_system* s) {
     return s->grid.ncell >= 128 && s->n >= size_t(8) * s->grid.ncell;
 }
+static bool direct_build(const lg_system* s) {
+    if (!direct_fits(s)) return false;
+    if (s->tune.nlist_build == 3) return true;
+    return s->tune.nlist_build == 0 && cells_dense(s);
+}
+static bool cell_build(const lg_system* s) {
+    if (s->tune.nlist_build == 1) return false;
+    if (s->tune.nlist_build == 2 || s->tune.nlist_build == 3) return true;
+    return cells_dense(s);
+}
+// Where the displacement tests write the atoms' cell tags: only for the
+// direct build.
+static CellTag cell_tags(lg_system* s) { return CellTag{direct_build(s) ? s->cellid.p : nullptr, s->grid, s->box}; }
 
 // Atoms per block of the rho pass: the narrowest chunk (64..512) that keeps
 // the grid at or below ~2048 blocks, so that per-block work stays
@@ -834,11 +854,37 @@ int launch_nlist(lg_system* s, bool check) {
     long prof = prof_start(s, LG_KERNEL_NLIST);
     int n = int(s->n);
     int* flag = s->flags.p;
+    const Grid& g = s->grid;
+    const bool direct = direct_build(s);
     if (check) {
         double hs = 0.5 * s->skin;
-        k_nlist_check<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->ref.p, n, hs * hs, flag);
+        k_nlist_check<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->ref.p, n, hs * hs, flag, cell_tags(s));
     }
-    const Grid& g = s->grid;
+    if (direct) {
+        // the cell tags of the current positions are written (here or by
+        // the MD first half-step); no cell list
+        const float rl2 = (!s->periodic || s->rlist <= 0) ? INFINITY
+                                                           : float(s->rlist * (1.0 + 1e-4) * s->rlist * (1.0 + 1e-4));
+        DirectArgs da{s->pos.p, s->meta.p, s->cellid.p, n, g, s->box, rl2, s->cap,
+                      s->nbr.p, s->nnbr.p, s->ref.p, s->sorted.p, flag};
+        const bool nearest = s->periodic && (g.reach[0] == 0 || g.reach[1] == 0 || g.reach[2] == 0);
+        prof_stop(s, prof);
+        if (s->coul_kind == COUL_EWALD && s->nk > 0) {
+            // launched by launch_forces together with the rho(k) partials
+            s->direct_pending = true;
+            s->cells_nearest = nearest;
+            s->direct_args = da;
+            return LG_OK;
+        }
+        long p2 = prof_start(s, LG_KERNEL_NLIST);
+        if (nearest)
+            k_nlist_direct<true><<<g.ncell, kCellBlock, 0, st>>>(da);
+        else
+            k_nlist_direct<false><<<g.ncell, kCellBlock, 0, st>>>(da);
+        prof_stop(s, p2);
+        LG_HIP(hipGetLastError());
+        return LG_OK;
+    }
     // small systems: the whole cell list in one workgroup
     size_t small_lds = size_t(8) * g.ncell + 4 + size_t(8) * n;
     if (g.ncell < 65536 && n <= 32768 && small_lds <= s->ctx->max_shared) {
@@ -915,11 +961,20 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         long pr = prof_start(s, LG_KERNEL_KSPACE_RHO);
         // with a deferred Verlet build: one launch of k_cells_rho, the list
         // blocks first (cells rounded up to 8), then the rho blocks
-        const bool cells = s->cells_pending;
+        const bool direct = s->direct_pending;
+        const bool cells = s->cells_pending || direct;
         const int ncb = cells ? (s->grid.ncell + 7) / 8 * 8 : 0;
         const CellsArgs& ca = s->cells_args;
+        const DirectArgs& da = s->direct_args;
         const bool near = s->cells_nearest;
         if (use_mfma) {
+            // the matrix-core rho shares its launch with the cell-list build
+            // only; a direct build runs first on its own
+            if (direct && near)
+                k_nlist_direct<true><<<s->grid.ncell, kCellBlock, 0, st>>>(da);
+            else if (direct)
+                k_nlist_direct<false><<<s->grid.ncell, kCellBlock, 0, st>>>(da);
+            const bool cells = s->cells_pending;
             const RhoShape sh = rho_mfma_shape(s, mixed);
             nchunks = sh.nchunks;
             const unsigned mgrid = xcd_grid(sh.ngroups, sh.nchunks);
@@ -960,6 +1015,10 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
                                           s->nunits, s->rho_partial.p, nub, nchunks};
             if (!cells)
                 k_rho_rows<float, float2><<<grid, 512, 0, st>>>(ra);
+            else if (direct && near)
+                k_direct_rho<true, float, float2><<<ncb + grid, kCellBlock, 0, st>>>(da, ra, ncb);
+            else if (direct)
+                k_direct_rho<false, float, float2><<<ncb + grid, kCellBlock, 0, st>>>(da, ra, ncb);
             else if (near)
                 k_cells_rho<true, float, float2><<<ncb + grid, kCellBlock, 0, st>>>(ca, ra, ncb);
             else
@@ -972,12 +1031,17 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
                                             s->nunits, s->rho_partial.p, nub, nchunks};
             if (!cells)
                 k_rho_rows<double, double2><<<grid, 512, 0, st>>>(ra);
+            else if (direct && near)
+                k_direct_rho<true, double, double2><<<ncb + grid, kCellBlock, 0, st>>>(da, ra, ncb);
+            else if (direct)
+                k_direct_rho<false, double, double2><<<ncb + grid, kCellBlock, 0, st>>>(da, ra, ncb);
             else if (near)
                 k_cells_rho<true, double, double2><<<ncb + grid, kCellBlock, 0, st>>>(ca, ra, ncb);
             else
                 k_cells_rho<false, double, double2><<<ncb + grid, kCellBlock, 0, st>>>(ca, ra, ncb);
         }
         s->cells_pending = false;
+        s->direct_pending = false;
         prof_stop(s, pr);
     }
     if (s->cells_pending) {  // deferred build without k-space (not expected): launch it alone
@@ -986,6 +1050,13 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         else
             k_nlist_cells<false><<<s->grid.ncell, kCellBlock, 0, st>>>(s->cells_args);
         s->cells_pending = false;
+    }
+    if (s->direct_pending) {
+        if (s->cells_nearest)
+            k_nlist_direct<true><<<s->grid.ncell, kCellBlock, 0, st>>>(s->direct_args);
+        else
+            k_nlist_direct<false><<<s->grid.ncell, kCellBlock, 0, st>>>(s->direct_args);
+        s->direct_pending = false;
     }
     PairArgs pa;
     pa.pos = s->pos.p;
@@ -1640,7 +1711,7 @@ int lg_system_set_tuning(lg_system* s, const lg_tuning* t) {
     if (t->kspace_split < 0 || t->kspace_split > 32) return fail(LG_E_INVALID, "kspace_split must be in 0..32");
     if (t->rho_width != 0 && t->rho_width != 64 && t->rho_width != 128 && t->rho_width != 256 && t->rho_width != 512)
         return fail(LG_E_INVALID, "rho_width must be 0, 64, 128, 256 or 512");
-    if (t->nlist_build < 0 || t->nlist_build > 2) return fail(LG_E_INVALID, "nlist_build must be 0, 1 or 2");
+    if (t->nlist_build < 0 || t->nlist_build > 3) return fail(LG_E_INVALID, "nlist_build must be 0, 1, 2 or 3");
     if (t->md_graph < 0 || t->md_graph > 2) return fail(LG_E_INVALID, "md_graph must be 0, 1 or 2");
     if (t->rho_kind < 0 || t->rho_kind > 2) return fail(LG_E_INVALID, "rho_kind must be 0, 1 or 2");
     s->tune = *t;
@@ -1655,7 +1726,7 @@ int lg_system_get_tuning(lg_system* s, lg_tuning* t) {
     *t = s->tune;
     t->pair_lanes = s->tpa;
     t->kspace_split = (s->coul_kind == COUL_EWALD && s->nk > 0) ? s->ksplit : 0;
-    t->nlist_build = cell_build(s) ? 2 : 1;
+    t->nlist_build = direct_build(s) ? 3 : (cell_build(s) ? 2 : 1);
     t->md_graph = s->tune.md_graph == 2 ? 2 : 1;
     t->rho_width = (s->coul_kind == COUL_EWALD && s->nk > 0) ? rho_chunk(s) : 0;
     t->rho_kind = (s->coul_kind == COUL_EWALD && s->nk > 0)
@@ -1923,16 +1994,17 @@ static int md_chunk(lg_system* s, uint64_t todo) {
     Thermo th{nullptr, nb, double(3 * s->n), s->berendsen_T, s->berendsen_tau};
     int nbf = blocks_for(std::max(n, kspace ? s->ld : 0), 64);
     const int tbf = ph.mode == 2 ? 512 : 256;  // waves per first-half block: see phase_roles
+    const CellTag tags = cell_tags(s);
     for (uint64_t k = 0; k < todo; k++) {
         // the Berendsen scale of the previous step is applied by the first half
         th.ke_part = (thermo && k > 0) ? s->ke_part.p : nullptr;
         long p1 = prof_start(s, LG_KERNEL_INTEGRATE);
         if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
             k_vv_first<<<nbf, tbf, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
-                                           s->flags.p, ph);
+                                           s->flags.p, ph, tags);
         else
             k_lf_first<<<nbf, tbf, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
-                                           s->flags.p, ph);
+                                           s->flags.p, ph, tags);
         prof_stop(s, p1);
         LG_TRY(launch_nlist(s, false));
         LG_TRY(launch_forces(s, false, false, true));

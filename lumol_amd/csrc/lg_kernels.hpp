@@ -40,19 +40,6 @@ struct Grid {
 // rebuild (a grid-wide last-block handshake costs an L2 write-back per block
 // on a multi-XCD device), and a step captured in a HIP graph needs no
 // per-step arguments.
-__global__ void k_nlist_check(const double4* __restrict__ pos, const double4* __restrict__ ref, int n,
-                              double half_skin2, int* __restrict__ flag) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool moved = false;
-    if (i < n) {
-        double dx = pos[i].x - ref[i].x;
-        double dy = pos[i].y - ref[i].y;
-        double dz = pos[i].z - ref[i].z;
-        moved = dx * dx + dy * dy + dz * dz > half_skin2;
-    }
-    if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
-}
-
 // Coordinate wrapped into [0, L) (periodic) as the cell binning sees it,
 // and its cell index.  Multiplications by the inverses instead of
 // divisions: the results move by a few ulps at most, far inside the 1e-12
@@ -68,6 +55,36 @@ __device__ __forceinline__ int cell_coord(double x, double L, double invL, int n
     double w = wrap_coord(x, L, invL, periodic);
     int c = (int)floor(w * inv_csize);
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+// Cell coordinates of every atom for the direct Verlet build, written by the
+// kernels that see each new position (the displacement test, the first half
+// of an MD step): cx | cy << 10 | cz << 20 (cells per axis <= 256).  Off
+// when tag is null.
+struct CellTag {
+    int* tag;
+    Grid g;
+    Box b;
+};
+__device__ __forceinline__ int cell_tag(const CellTag& t, double4 p) {
+    const int cx = cell_coord(p.x, t.b.L[0], t.b.inv[0], t.g.n[0], t.g.inv_size[0], t.b.periodic);
+    const int cy = cell_coord(p.y, t.b.L[1], t.b.inv[1], t.g.n[1], t.g.inv_size[1], t.b.periodic);
+    const int cz = cell_coord(p.z, t.b.L[2], t.b.inv[2], t.g.n[2], t.g.inv_size[2], t.b.periodic);
+    return cx | (cy << 10) | (cz << 20);
+}
+
+__global__ void k_nlist_check(const double4* __restrict__ pos, const double4* __restrict__ ref, int n,
+                              double half_skin2, int* __restrict__ flag, CellTag tg) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool moved = false;
+    if (tg.tag && i < n) tg.tag[i] = cell_tag(tg, pos[i]);
+    if (i < n) {
+        double dx = pos[i].x - ref[i].x;
+        double dy = pos[i].y - ref[i].y;
+        double dz = pos[i].z - ref[i].z;
+        moved = dx * dx + dy * dy + dz * dz > half_skin2;
+    }
+    if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
 }
 
 __global__ void k_cell_count(const double4* __restrict__ pos, int n, Grid g, Box b, int* __restrict__ cellid,
@@ -696,6 +713,216 @@ template <bool kMinImage>
 __global__ __launch_bounds__(kCellBlock) void k_nlist_cells(CellsArgs ca) {
     __shared__ CellsLds lds;
     nlist_cells_block<kMinImage>(ca, blockIdx.x, gridDim.x, lds);
+}
+
+// ---- direct Verlet build (systems up to 32k atoms) --------------------------------
+// The same rows as the cell build without a sorted cell list: workgroup c
+// reads every atom's cell tag (CellTag, written when the position was) and
+// takes, in atom-index order, the atoms of its own cell as home atoms and
+// those of the cells within reach as candidates, with the periodic shift of
+// the image they stand for (a cell index past the reach on one side is a
+// neighbour on the other).  Candidates go to an LDS list (windows of
+// kDirectCand, re-read past that), then in tiles through tile_scan; rows are
+// indexed by atom (sorted[i] = i), so no launch has to sort anything and a
+// step without a rebuild pays no cell-list launch.
+constexpr int kDirectCand = 4096;
+constexpr int kDirectHome = 256;
+constexpr int kDirectBatch = 8;  // scan rounds (of one atom per thread) per pair of barriers
+
+struct DirectArgs {
+    const double4* pos;
+    const int2* meta;
+    const int* tag;
+    int n;
+    Grid g;
+    Box b;
+    float rl2f;
+    int cap;
+    int* nbr;
+    int* nnbr;
+    double4* ref;
+    int* sorted;
+    int* flags;
+};
+
+struct DirectLds {
+    float4 tile[kCellTile];
+    float4 home[kDirectHome];
+    int tidx[kCellTile];
+    int cand[kDirectCand];  // atom | shift code << 24 (2 bits per axis: 0, +L, -L)
+    int home_row[kDirectHome];
+    int home_cnt[kDirectHome];
+    int wc[kDirectBatch][kCellBlock / 64], wh[kDirectBatch][kCellBlock / 64];
+};
+
+__device__ __forceinline__ float direct_shift(int code, float L) { return code == 1 ? L : (code == 2 ? -L : 0.f); }
+
+template <bool kMinImage>
+__device__ __forceinline__ void nlist_direct_block(const DirectArgs& a, int blk, int nblk, DirectLds& lds) {
+    if (a.flags[0] == 0) return;
+    constexpr int NW = kCellBlock / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const Grid& g = a.g;
+    const Box& b = a.b;
+    const int n = a.n;
+    const float Lx = (float)b.L[0], Ly = (float)b.L[1], Lz = (float)b.L[2];
+    for (int c = blk; c < g.ncell; c += nblk) {
+        const int hx = c % g.n[0], hy = (c / g.n[0]) % g.n[1], hz = c / (g.n[0] * g.n[1]);
+        int hwin = 0, H = 0;
+        do {
+            int cwin = 0, M = 0;
+            do {
+                // scan: block-wide ranks of candidates and home atoms in index
+                // order, kDirectBatch rounds per pair of barriers
+                int basec = 0, baseh = 0;
+                for (int i0 = 0; i0 < n; i0 += kDirectBatch * kCellBlock) {
+                    int tv[kDirectBatch];
+#pragma unroll
+                    for (int u = 0; u < kDirectBatch; u++) {
+                        const int i = i0 + u * kCellBlock + tid;
+                        tv[u] = a.tag[i < n ? i : n - 1];
+                    }
+                    unsigned long long bc[kDirectBatch], bh[kDirectBatch];
+                    int code[kDirectBatch];
+#pragma unroll
+                    for (int u = 0; u < kDirectBatch; u++) {
+                        const int i = i0 + u * kCellBlock + tid;
+                        const int t = tv[u];
+                        int dx = (t & 1023) - hx, dy = ((t >> 10) & 1023) - hy, dz = (t >> 20) - hz;
+                        int cd = 0;
+                        if (dx > g.reach[0]) {
+                            dx -= g.n[0];
+                            cd |= 2;
+                        } else if (dx < -g.reach[0]) {
+                            dx += g.n[0];
+                            cd |= 1;
+                        }
+                        if (dy > g.reach[1]) {
+                            dy -= g.n[1];
+                            cd |= 2 << 2;
+                        } else if (dy < -g.reach[1]) {
+                            dy += g.n[1];
+                            cd |= 1 << 2;
+                        }
+                        if (dz > g.reach[2]) {
+                            dz -= g.n[2];
+                            cd |= 2 << 4;
+                        } else if (dz < -g.reach[2]) {
+                            dz += g.n[2];
+                            cd |= 1 << 4;
+                        }
+                        const bool live = i < n;
+                        const bool cand =
+                            live && abs(dx) <= g.reach[0] && abs(dy) <= g.reach[1] && abs(dz) <= g.reach[2];
+                        const bool home = live && (dx | dy | dz | cd) == 0;
+                        code[u] = cd;
+                        bc[u] = __ballot(cand);
+                        bh[u] = __ballot(home);
+                    }
+                    if (lane == 0)
+#pragma unroll
+                        for (int u = 0; u < kDirectBatch; u++) {
+                            lds.wc[u][wave] = __popcll(bc[u]);
+                            lds.wh[u][wave] = __popcll(bh[u]);
+                        }
+                    __syncthreads();
+#pragma unroll
+                    for (int u = 0; u < kDirectBatch; u++) {
+                        const int i = i0 + u * kCellBlock + tid;
+                        int offc = 0, offh = 0, totc = 0, toth = 0;
+#pragma unroll
+                        for (int w = 0; w < NW; w++) {
+                            const int xc = lds.wc[u][w], xh = lds.wh[u][w];
+                            if (w < wave) {
+                                offc += xc;
+                                offh += xh;
+                            }
+                            totc += xc;
+                            toth += xh;
+                        }
+                        const bool cand = (bc[u] >> lane) & 1ull, home = (bh[u] >> lane) & 1ull;
+                        const int rc = basec + offc + __popcll(bc[u] & below) - cwin;
+                        if (cand && rc >= 0 && rc < kDirectCand) lds.cand[rc] = i | (code[u] << 24);
+                        const int rh = baseh + offh + __popcll(bh[u] & below) - hwin;
+                        if (cwin == 0 && home && rh >= 0 && rh < kDirectHome) lds.home_row[rh] = i;
+                        basec += totc;
+                        baseh += toth;
+                    }
+                    __syncthreads();  // wc / wh are rewritten by the next batch
+                }
+                M = basec;
+                H = baseh;
+                const int hn = min(kDirectHome, H - hwin);
+                if (hn <= 0) break;  // uniform: an empty cell
+                if (cwin == 0)
+                    for (int h = tid; h < hn; h += kCellBlock) {
+                        const int i = lds.home_row[h];
+                        const double4 p = a.pos[i];
+                        lds.home[h] = make_float4((float)wrap_coord(p.x, b.L[0], b.inv[0], b.periodic),
+                                                  (float)wrap_coord(p.y, b.L[1], b.inv[1], b.periodic),
+                                                  (float)wrap_coord(p.z, b.L[2], b.inv[2], b.periodic),
+                                                  __int_as_float(a.meta[i].y));
+                        lds.home_cnt[h] = 0;
+                    }
+                const int mw = min(kDirectCand, M - cwin);
+                for (int t0 = 0; t0 < mw; t0 += kCellTile) {
+                    const int tl = min(kCellTile, mw - t0);
+                    __syncthreads();  // the list and homes are written; the previous tile is scanned
+                    constexpr int R = kCellTile / kCellBlock;
+                    double4 pv[R];
+                    int2 mv[R];
+                    int vv[R];
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const int e = tid + r * kCellBlock;
+                        vv[r] = e < tl ? lds.cand[t0 + e] : lds.cand[t0];
+                        const int i = vv[r] & 0xffffff;
+                        pv[r] = a.pos[i];
+                        mv[r] = a.meta[i];
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const int e = tid + r * kCellBlock;
+                        if (e < tl) {
+                            const int code = vv[r] >> 24;
+                            lds.tile[e] = make_float4(
+                                (float)wrap_coord(pv[r].x, b.L[0], b.inv[0], b.periodic) + direct_shift(code & 3, Lx),
+                                (float)wrap_coord(pv[r].y, b.L[1], b.inv[1], b.periodic) + direct_shift((code >> 2) & 3, Ly),
+                                (float)wrap_coord(pv[r].z, b.L[2], b.inv[2], b.periodic) + direct_shift(code >> 4, Lz),
+                                __int_as_float(mv[r].y));
+                            lds.tidx[e] = vv[r] & 0xffffff;
+                        }
+                    }
+                    __syncthreads();
+                    tile_scan<kMinImage>(lds.tile, lds.tidx, tl, lds.home, lds.home_row, lds.home_cnt, hn, a.nbr, a.cap,
+                                         a.rl2f, b, g);
+                }
+                __syncthreads();
+                cwin += kDirectCand;
+            } while (cwin < M);
+            const int hn = min(kDirectHome, H - hwin);
+            for (int h = tid; h < hn; h += kCellBlock) {
+                const int i = lds.home_row[h];
+                int cnt = lds.home_cnt[h];
+                if (cnt > a.cap) {
+                    atomicMax(a.flags + 1, cnt);
+                    cnt = a.cap;
+                }
+                a.nnbr[i] = cnt;
+                a.sorted[i] = i;
+                a.ref[i] = a.pos[i];
+            }
+            __syncthreads();
+            hwin += kDirectHome;
+        } while (hwin < H);
+    }
+}
+
+template <bool kMinImage>
+__global__ __launch_bounds__(kCellBlock) void k_nlist_direct(DirectArgs da) {
+    __shared__ DirectLds lds;
+    nlist_direct_block<kMinImage>(da, blockIdx.x, gridDim.x, lds);
 }
 
 
@@ -1552,13 +1779,12 @@ struct RhoRowsArgs {
 // NW waves per block (8 in k_rho_rows and k_cells_rho), logical
 // block index L of the xcd_grid(nub, nchunks) layout.
 template <typename T, typename T2, int NW>
-__device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int L) {
+__device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int L, double2 (*red)[4][64]) {
     const T2* __restrict__ tab = ra.tab;
     const T* __restrict__ q = ra.q;
     const int ld = ra.ld, kmax = ra.kmax, width = ra.width, nunits = ra.nunits;
     const KUnit* __restrict__ units = ra.units;
     double2* __restrict__ partial = ra.partial;
-    __shared__ double2 red[NW][4][64];
     int ub, chunk;
     if (!xcd_block_at(L, ra.nub, ra.nchunks, ub, chunk)) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1611,7 +1837,8 @@ __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int
 // sums do not depend on which launch computed them.
 template <typename T, typename T2>
 __global__ __launch_bounds__(512) void k_rho_rows(RhoRowsArgs<T, T2> ra) {
-    rho_rows_block<T, T2, 8>(ra, blockIdx.x);
+    __shared__ double2 red[8][4][64];
+    rho_rows_block<T, T2, 8>(ra, blockIdx.x, red);
 }
 
 // One launch for two independent passes of a force evaluation: blocks
@@ -1621,11 +1848,23 @@ __global__ __launch_bounds__(512) void k_rho_rows(RhoRowsArgs<T, T2> ra) {
 // first half-step and the cell list wrote.
 template <bool kMinImage, typename T, typename T2>
 __global__ __launch_bounds__(kCellBlock) void k_cells_rho(CellsArgs ca, RhoRowsArgs<T, T2> ra, int ncb) {
-    __shared__ CellsLds lds;
+    constexpr size_t kRed = sizeof(double2) * (kCellBlock / 64) * 4 * 64;
+    __shared__ __align__(16) unsigned char buf[sizeof(CellsLds) > kRed ? sizeof(CellsLds) : kRed];
     if ((int)blockIdx.x < ncb)
-        nlist_cells_block<kMinImage>(ca, blockIdx.x, ncb, lds);
+        nlist_cells_block<kMinImage>(ca, blockIdx.x, ncb, *reinterpret_cast<CellsLds*>(buf));
     else
-        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb);
+        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb, reinterpret_cast<double2(*)[4][64]>(buf));
+}
+
+// The same with the direct Verlet build in the first ncb blocks.
+template <bool kMinImage, typename T, typename T2>
+__global__ __launch_bounds__(kCellBlock) void k_direct_rho(DirectArgs da, RhoRowsArgs<T, T2> ra, int ncb) {
+    constexpr size_t kRed = sizeof(double2) * (kCellBlock / 64) * 4 * 64;
+    __shared__ __align__(16) unsigned char buf[sizeof(DirectLds) > kRed ? sizeof(DirectLds) : kRed];
+    if ((int)blockIdx.x < ncb)
+        nlist_direct_block<kMinImage>(da, blockIdx.x, ncb, *reinterpret_cast<DirectLds*>(buf));
+    else
+        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb, reinterpret_cast<double2(*)[4][64]>(buf));
 }
 
 
@@ -2093,7 +2332,7 @@ __device__ __forceinline__ void phase_roles(const PhaseOut& ph, int i, int n, in
 __global__ __launch_bounds__(512) void k_vv_first(double4* __restrict__ pos, double4* __restrict__ vel,
                                                   const double4* __restrict__ acc, int n, double dt, Thermo th,
                                                   const double4* __restrict__ ref, double half_skin2,
-                                                  int* __restrict__ flags, PhaseOut ph) {
+                                                  int* __restrict__ flags, PhaseOut ph, CellTag tg) {
     __shared__ double px[4][64];
     const int lane = threadIdx.x & 63, role = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
@@ -2120,6 +2359,7 @@ __global__ __launch_bounds__(512) void k_vv_first(double4* __restrict__ pos, dou
             vel[i] = v;
             pos[i] = p;
             moved = moved_beyond(p, ref, i, half_skin2);
+            if (tg.tag) tg.tag[i] = cell_tag(tg, p);
         }
         if (__any(moved) && lane == 0) atomicOr(flags, 1);
         px[0][lane] = p.x;
@@ -2138,7 +2378,7 @@ __global__ __launch_bounds__(512) void k_vv_first(double4* __restrict__ pos, dou
 __global__ __launch_bounds__(512) void k_lf_first(double4* __restrict__ pos, double4* __restrict__ vel,
                                                   const double4* __restrict__ acc, int n, double dt, Thermo th,
                                                   const double4* __restrict__ ref, double half_skin2,
-                                                  int* __restrict__ flags, PhaseOut ph) {
+                                                  int* __restrict__ flags, PhaseOut ph, CellTag tg) {
     __shared__ double px[4][64];
     const int lane = threadIdx.x & 63, role = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
@@ -2162,6 +2402,7 @@ __global__ __launch_bounds__(512) void k_lf_first(double4* __restrict__ pos, dou
             p.z += v.z * dt + (0.5 * a.z) * dt2;
             pos[i] = p;
             moved = moved_beyond(p, ref, i, half_skin2);
+            if (tg.tag) tg.tag[i] = cell_tag(tg, p);
         }
         if (__any(moved) && lane == 0) atomicOr(flags, 1);
         px[0][lane] = p.x;

@@ -1,7 +1,10 @@
-"""The two neighbour-list builds (lanes per atom; one workgroup per cell with
-candidates staged in LDS) give rows with the same pairs within the cutoff in
-the same order, so forces, energies and trajectories are bit-identical; the
-cell build also meets the FP64 parity bar against the oracle by itself."""
+"""The neighbour-list builds: lanes per atom (1) and one workgroup per cell
+(2) over the sorted cell list give rows with the same pairs within the
+cutoff in the same order, so forces, energies and trajectories are
+bit-identical; the direct build (3: one workgroup per cell over the atoms'
+cell tags, rows in atom order) gives the same pairs in another order, so
+the same forces to summation order.  Every build meets the FP64 parity bar
+against the oracle by itself."""
 import numpy as np
 import pytest
 
@@ -64,6 +67,48 @@ def test_builds_give_identical_forces(case):
     assert np.abs(f2 - fo).max() <= 1e-10 * np.abs(fo).max()
 
 
+@pytest.mark.parametrize("case", list(CASES))
+def test_direct_build_matches(case):
+    s2, f2, e2, v2 = evaluate(CASES[case], 2)
+    s3, f3, e3, v3 = evaluate(CASES[case], 3)
+    assert np.abs(f3 - f2).max() <= 1e-12 * np.abs(f2).max()
+    assert abs(e3 - e2) <= 1e-12 * abs(e2)
+    assert np.abs(v3 - v2).max() <= 1e-12 * np.abs(v2).max()
+    fo = lo.OracleSystem(s3, nthreads=8).forces()
+    assert np.abs(f3 - fo).max() <= 1e-10 * np.abs(fo).max()
+
+
+@pytest.mark.parametrize("case", ["water-ewald", "lj-fluid"])
+def test_direct_build_trajectory(case):
+    # 60 steps: the direct build's list is rebuilt from the tags the first
+    # half-steps write, and stays within summation-order noise of build 2
+    out = []
+    for build in (2, 3):
+        s = CASES[case]()
+        s.set_tuning(nlist_build=build)
+        Simulation(MolecularDynamics(VelocityVerlet(0.5 if case == "water-ewald" else 2.0))).run(s, 60)
+        out.append(s.positions.copy())
+    assert np.abs(out[0] - out[1]).max() < 1e-8
+
+
+def test_direct_build_mixed_and_positions_reset():
+    # mixed form, then new positions from the host: the next evaluation's
+    # displacement test writes the tags the rebuild reads
+    from lumol_amd._native import LG_PRECISION_MIXED
+    s = spce(nmol=1000, L=31.07, cutoff=7.0, kmax=6)
+    s.set_precision(LG_PRECISION_MIXED)
+    s.set_tuning(nlist_build=3)
+    s.forces()
+    rng = np.random.default_rng(4)
+    pos = s.positions + rng.uniform(-0.5, 0.5, size=s.positions.shape)
+    s.positions = pos
+    f = s.forces()
+    assert s.tuning()["nlist_build"] == 3
+    fo = lo.OracleSystem(s, nthreads=8).forces()
+    rms = np.sqrt(((f - fo) ** 2).sum() / (fo ** 2).sum())
+    assert rms < 1e-5
+
+
 @pytest.mark.parametrize("case", ["water-ewald", "lj-fluid"])
 def test_builds_give_identical_trajectories(case):
     out = []
@@ -78,6 +123,10 @@ def test_builds_give_identical_trajectories(case):
 
 def test_default_choice():
     big = spce(nmol=3000, L=44.8, cutoff=10.0, kmax=4)  # 512 half cells, 17 atoms each
-    assert big.tuning()["nlist_build"] == 2
+    assert big.tuning()["nlist_build"] == 3
     small = nacl(ncell=3, cutoff=8.0, kmax=5)
     assert small.tuning()["nlist_build"] == 1
+    # past 32768 atoms the per-cell build keeps the cell list
+    huge = lj_fluid(40000)
+    huge.set_tuning(nlist_build=3)
+    assert huge.tuning()["nlist_build"] == 2
