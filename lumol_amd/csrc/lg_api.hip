@@ -1995,30 +1995,47 @@ static int md_chunk(lg_system* s, uint64_t todo) {
     int nbf = blocks_for(std::max(n, kspace ? s->ld : 0), 64);
     const int tbf = ph.mode == 2 ? 512 : 256;  // waves per first-half block: see phase_roles
     const CellTag tags = cell_tags(s);
+    const bool lf = s->integrator != LG_INTEGRATOR_VELOCITY_VERLET;
+    const int ks = kspace ? s->ksplit : 0;
+    const int* sl = s->nterms > 0 ? s->slot_start.p : nullptr;
+    // without a thermostat the second half of a step and the first half of
+    // the next share a launch (k_md_fused); the chunk's first and last
+    // halves run alone
+    bool first_pending = true;
     for (uint64_t k = 0; k < todo; k++) {
-        // the Berendsen scale of the previous step is applied by the first half
-        th.ke_part = (thermo && k > 0) ? s->ke_part.p : nullptr;
-        long p1 = prof_start(s, LG_KERNEL_INTEGRATE);
-        if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
-            k_vv_first<<<nbf, tbf, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
-                                           s->flags.p, ph, tags);
-        else
-            k_lf_first<<<nbf, tbf, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
-                                           s->flags.p, ph, tags);
-        prof_stop(s, p1);
+        if (first_pending) {
+            // the Berendsen scale of the previous step is applied by the first half
+            th.ke_part = (thermo && k > 0) ? s->ke_part.p : nullptr;
+            long p1 = prof_start(s, LG_KERNEL_INTEGRATE);
+            if (!lf)
+                k_vv_first<<<nbf, tbf, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
+                                               s->flags.p, ph, tags);
+            else
+                k_lf_first<<<nbf, tbf, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
+                                               s->flags.p, ph, tags);
+            prof_stop(s, p1);
+        }
         LG_TRY(launch_nlist(s, false));
         LG_TRY(launch_forces(s, false, false, true));
         long p2 = prof_start(s, LG_KERNEL_INTEGRATE);
-        double* kp = thermo ? s->ke_part.p : nullptr;
-        int ks = kspace ? s->ksplit : 0;
-        const int* sl = s->nterms > 0 ? s->slot_start.p : nullptr;
-        if (s->integrator == LG_INTEGRATOR_VELOCITY_VERLET)
-            k_md_second<false><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
-                                                   s->vel.p, s->acc.p, n, s->dt, kp);
-        else
-            k_md_second<true><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
-                                                  s->vel.p, s->acc.p, n, s->dt, kp);
+        const bool fuse = !thermo && k + 1 < todo;
+        if (fuse && !lf)
+            k_md_fused<false><<<nbf, tbf, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks, s->vel.p,
+                                                   s->acc.p, n, s->dt, s->ref.p, hs * hs, s->flags.p, ph, tags);
+        else if (fuse)
+            k_md_fused<true><<<nbf, tbf, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks, s->vel.p,
+                                                  s->acc.p, n, s->dt, s->ref.p, hs * hs, s->flags.p, ph, tags);
+        else {
+            double* kp = thermo ? s->ke_part.p : nullptr;
+            if (!lf)
+                k_md_second<false><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
+                                                       s->vel.p, s->acc.p, n, s->dt, kp);
+            else
+                k_md_second<true><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
+                                                      s->vel.p, s->acc.p, n, s->dt, kp);
+        }
         prof_stop(s, p2);
+        first_pending = !fuse;
     }
     if (thermo) {
         th.ke_part = s->ke_part.p;

@@ -2455,6 +2455,77 @@ __global__ __launch_bounds__(64) void k_md_second(const double4* __restrict__ fo
     if (threadIdx.x == 0) ke_part[blockIdx.x] = s;
 }
 
+// The second half of step k and the first half of step k + 1 in one launch
+// (inside lg_md_run, without a thermostat: the first half then needs no
+// global kinetic energy).  The arithmetic of k_md_second followed by
+// k_vv_first / k_lf_first, in the same order, on the values those kernels
+// pass through memory: the same bits, one launch fewer per step.  Block
+// layout of the first-half kernels (64 atoms; the other waves write the
+// phase table).
+template <bool kLeapFrog>
+__global__ __launch_bounds__(512) void k_md_fused(const double4* __restrict__ force, double4* __restrict__ pos,
+                                                  const double4* __restrict__ contrib,
+                                                  const int* __restrict__ slot_start,
+                                                  const double4* __restrict__ fk, int ksplit,
+                                                  double4* __restrict__ vel, double4* __restrict__ acc, int n,
+                                                  double dt, const double4* __restrict__ ref, double half_skin2,
+                                                  int* __restrict__ flags, PhaseOut ph, CellTag tg) {
+    __shared__ double px[4][64];
+    const int lane = threadIdx.x & 63, role = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    if (role == 0) {
+        bool moved = false;
+        double4 p = make_double4(0, 0, 0, 0);
+        if (i < n) {
+            // second half of step k
+            double4 f = total_force(i, n, force, contrib, slot_start, fk, ksplit, pos);
+            double4 v = vel[i];
+            double m = v.w;
+            double4 na = make_double4(f.x / m, f.y / m, f.z / m, 0.0);
+            if (kLeapFrog) {
+                double4 a = acc[i];
+                v.x += (0.5 * (a.x + na.x)) * dt;
+                v.y += (0.5 * (a.y + na.y)) * dt;
+                v.z += (0.5 * (a.z + na.z)) * dt;
+            } else {
+                double h = 0.5 * dt;
+                v.x += h * na.x;
+                v.y += h * na.y;
+                v.z += h * na.z;
+            }
+            acc[i] = na;
+            // first half of step k + 1
+            p = pos[i];
+            if (kLeapFrog) {
+                double dt2 = dt * dt;
+                p.x += v.x * dt + (0.5 * na.x) * dt2;
+                p.y += v.y * dt + (0.5 * na.y) * dt2;
+                p.z += v.z * dt + (0.5 * na.z) * dt2;
+            } else {
+                double h = 0.5 * dt;
+                v.x += h * na.x;
+                v.y += h * na.y;
+                v.z += h * na.z;
+                p.x += v.x * dt;
+                p.y += v.y * dt;
+                p.z += v.z * dt;
+            }
+            vel[i] = v;
+            pos[i] = p;
+            moved = moved_beyond(p, ref, i, half_skin2);
+            if (tg.tag) tg.tag[i] = cell_tag(tg, p);
+        }
+        if (__any(moved) && lane == 0) atomicOr(flags, 1);
+        px[0][lane] = p.x;
+        px[1][lane] = p.y;
+        px[2][lane] = p.z;
+        px[3][lane] = p.w;
+    }
+    if (!ph.mode) return;  // uniform: a kernel argument
+    __syncthreads();
+    phase_roles(ph, i, n, role, lane, px);
+}
+
 // Berendsen factor of the last step of a run into *lambda (one block, same
 // sum as berendsen_lambda).
 __global__ __launch_bounds__(256) void k_berendsen(Thermo th, double* __restrict__ lambda) {
