@@ -191,16 +191,4 @@ __device__ __forceinline__ float erfc_cheb(float x, float& ex2) {
     return t * __expf(mx2 + p);
 }
 
-// Compensated FP32 accumulation (s, c), sum = s + c: the rounding error of
-// each addition is recovered exactly by Knuth's branch-free TwoSum
-// (TAOCP vol. 2, section 4.2.2) and collected in c.
-#pragma clang fp contract(off)
-__device__ __forceinline__ void comp_add(float& s, float& c, float v) {
-    float t = s + v;
-    float bp = t - s;
-    c += (s - (t - bp)) + (v - bp);
-    s = t;
-}
-#pragma clang fp contract(on)
-
 }  // namespace lg

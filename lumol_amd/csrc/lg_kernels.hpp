@@ -1404,9 +1404,10 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
 
 // Mixed form of k_pairs: the minimum-image difference is taken in FP64 and
 // rounded to FP32; distance, LJ, erfc and the Ewald/Wolf force factor are
-// FP32 (erfc from erfc_cheb); the per-atom force is accumulated in FP32 with
-// compensated summation and stored in FP64.  Energies and virial (energy
-// mode) are accumulated in FP64.
+// FP32 (erfc from erfc_cheb); the FP32 pair forces are accumulated in FP64
+// (two operations per component, against seven for a compensated FP32 sum,
+// and no less accurate).  Energies and virial (energy mode) are accumulated
+// in FP64.
 template <bool kEnergy, int TPA, int CK>
 __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
     struct P32 {
@@ -1440,7 +1441,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     int t = g / TPA;
     int sub = g % TPA;
-    float fx = 0.f, fy = 0.f, fz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
     double e_pair = 0, e_coul = 0;
     double wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0;
     int i = -1;
@@ -1500,9 +1501,9 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
                     if (kEnergy) e_c = pref * (erfcv * inv_r - we);
                 }
             }
-            comp_add(fx, cx, fs * dx);
-            comp_add(fy, cy, fs * dy);
-            comp_add(fz, cz, fs * dz);
+            fx += (double)(fs * dx);
+            fy += (double)(fs * dy);
+            fz += (double)(fs * dz);
             if (kEnergy && j > i) {
                 e_pair += e_p;
                 e_coul += e_c;
@@ -1531,7 +1532,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
             jB = jB2;
         }
     }
-    double sx = (double)fx + (double)cx, sy = (double)fy + (double)cy, sz = (double)fz + (double)cz;
+    double sx = fx, sy = fy, sz = fz;
 #pragma unroll
     for (int off = TPA / 2; off > 0; off >>= 1) {
         sx += __shfl_xor(sx, off, 64);
