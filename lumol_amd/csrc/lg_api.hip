@@ -903,7 +903,7 @@ int launch_nlist(lg_system* s, bool check) {
     // FP32 candidate test with a 1e-4 relative margin on rlist
     float rl2f = (!s->periodic || s->rlist <= 0) ? INFINITY : float(s->rlist * (1.0 + 1e-4) * s->rlist * (1.0 + 1e-4));
     if (cell_build(s)) {
-        CellsArgs ca{s->cpos.p, s->sorted.p, s->start.p, s->count.p, g, s->box, rl2f, s->cap,
+        CellsArgs ca{s->meta.p, s->cpos.p, s->sorted.p, s->start.p, s->count.p, g, s->box, rl2f, s->cap,
                      s->nbr.p, s->nnbr.p, s->pos.p, s->ref.p, flag};
         const bool nearest = s->periodic && (g.reach[0] == 0 || g.reach[1] == 0 || g.reach[2] == 0);
         prof_stop(s, prof);
@@ -928,7 +928,7 @@ int launch_nlist(lg_system* s, bool check) {
     case T:                                                                                                \
         k_nlist_build<T><<<nb, 256, 0, st>>>(s->cpos.p, s->sorted.p, s->cellid.p, s->start.p, s->count.p, g,  \
                                              s->box, rl2f, n, s->cap, s->nbr.p, s->nnbr.p, s->pos.p, s->ref.p,  \
-                                             flag);                                                           \
+                                             flag, s->meta.p);                                                \
         break;
         LG_BUILD_CASE(4)
         LG_BUILD_CASE(8)
@@ -1504,7 +1504,9 @@ int lg_context_destroy(lg_context* ctx) {
 
 int lg_system_create(lg_context* ctx, size_t natoms, lg_system** out) {
     if (!ctx || !out) return fail(LG_E_INVALID, "null argument");
-    if (natoms == 0 || natoms > (size_t(1) << 30)) return fail(LG_E_INVALID, "invalid number of atoms");
+    if (natoms == 0) return fail(LG_E_INVALID, "invalid number of atoms");
+    // Verlet-list entries hold the atom index in 24 bits (nbr_entry)
+    if (natoms > (size_t(1) << 24)) return fail(LG_E_UNSUPPORTED, "more than 2^24 atoms in one system");
     LG_HIP(hipSetDevice(ctx->device));
     lg_system* s = new lg_system();
     s->ctx = ctx;

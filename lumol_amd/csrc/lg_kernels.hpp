@@ -32,6 +32,12 @@ struct Grid {
     int ncell;
 };
 
+// A Verlet-list entry: the neighbour's atom index in bits 0-23 and its kind
+// in bits 24-30, so that the pair kernels need no per-pair load of the kind.
+__host__ __device__ __forceinline__ int nbr_entry(int j, int kind) { return j | (kind << 24); }
+__host__ __device__ __forceinline__ int nbr_atom(int e) { return e & 0xffffff; }
+__host__ __device__ __forceinline__ int nbr_kind(int e) { return e >> 24; }
+
 // Rebuild protocol: a displacement test (here, or in the MD first half-step)
 // sets *flag when an atom moved more than skin/2 since the last build; the
 // list kernels that follow rebuild when it is set, and the pair kernel,
@@ -315,7 +321,7 @@ __global__ __launch_bounds__(256) void k_nlist_build(const float4* __restrict__ 
                                                      int* __restrict__ count_zero, Grid g, Box b, float rl2f,
                                                      int n, int cap, int* __restrict__ nbr, int* __restrict__ nnbr,
                                                      const double4* __restrict__ pos, double4* __restrict__ ref,
-                                                     int* __restrict__ flags) {
+                                                     int* __restrict__ flags, const int2* __restrict__ meta) {
     if (flags[0] == 0) return;
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
     for (int c = gid; c < g.ncell; c += gridDim.x * blockDim.x) count_zero[c] = 0;
@@ -418,7 +424,7 @@ __global__ __launch_bounds__(256) void k_nlist_build(const float4* __restrict__ 
                 unsigned long long bal = __ballot(accept);
                 unsigned long long gm = TPA == 64 ? bal : ((bal >> gbase) & ((1ull << (TPA & 63)) - 1ull));
                 int slot = count + __popcll(gm & ((1ull << sub) - 1ull));
-                if (accept && slot < cap) row[slot] = jj;
+                if (accept && slot < cap) row[slot] = nbr_entry(jj, meta[jj].x);
                 count += __popcll(gm);
             };
             // two passes per iteration on alternating registers, each
@@ -525,6 +531,7 @@ __device__ __forceinline__ void tile_scan(const float4* tile, const int* tidx, i
 }
 
 struct CellsArgs {
+    const int2* meta;
     const float4* cpos;
     const int* sorted;
     const int* start;
@@ -673,12 +680,14 @@ __device__ __forceinline__ void nlist_cells_block(const CellsArgs& args, int blk
                     qv[r] = seg_s0[k] + (mm - seg_base[k]);
                 }
                 float4 cv[R];
-                int jv[R];
+                int jv[R], kv2[R];
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     cv[r] = cpos[qv[r]];
                     jv[r] = sorted[qv[r]];
                 }
+#pragma unroll
+                for (int r = 0; r < R; r++) kv2[r] = args.meta[jv[r]].x;
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     int e = tid + r * kCellBlock;
@@ -686,7 +695,7 @@ __device__ __forceinline__ void nlist_cells_block(const CellsArgs& args, int blk
                         int k = kv[r];
                         tile[e] = make_float4(cv[r].x + seg_sh[k][0], cv[r].y + seg_sh[k][1],
                                               cv[r].z + seg_sh[k][2], cv[r].w);
-                        tidx[e] = jv[r];
+                        tidx[e] = nbr_entry(jv[r], kv2[r]);
                     }
                 }
                 __syncthreads();
@@ -891,7 +900,7 @@ __device__ __forceinline__ void nlist_direct_block(const DirectArgs& a, int blk,
                                 (float)wrap_coord(pv[r].y, b.L[1], b.inv[1], b.periodic) + direct_shift((code >> 2) & 3, Ly),
                                 (float)wrap_coord(pv[r].z, b.L[2], b.inv[2], b.periodic) + direct_shift(code >> 4, Lz),
                                 __int_as_float(mv[r].y));
-                            lds.tidx[e] = vv[r] & 0xffffff;
+                            lds.tidx[e] = nbr_entry(vv[r] & 0xffffff, mv[r].x);
                         }
                     }
                     __syncthreads();
@@ -1315,13 +1324,14 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
         const CoulParam& c = a.coul;
         const int* row = a.nbr + (size_t)t * a.cap;
         // Software pipeline over the row: the row entries two passes ahead
-        // and the next pass's neighbour records are loaded while the
+        // and the next pass's neighbour positions are loaded while the
         // current pair is evaluated, unrolled by two on alternating
-        // registers (A, B).  Entries past the row end read a clamped index
-        // and stand for atom i itself, never evaluated.
+        // registers (A, B); an entry carries the neighbour's kind.  Entries
+        // past the row end read a clamped index and stand for atom i
+        // itself, never evaluated.
         auto jat = [&](int k) {
             int v = row[min(k, max(nn - 1, 0))];
-            return k < nn ? v : i;
+            return k < nn ? v : nbr_entry(i, ki);
         };
         auto pair = [&](int j, const double4& pj, int kj) {
             double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -1363,19 +1373,16 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
             }
         };
         int k = sub;
-        int jA = jat(k), jB = jat(k + TPA);
-        double4 pA = a.pos[jA], pB = a.pos[jB];
-        int kA = a.meta[jA].x, kB = a.meta[jB].x;
+        int eA = jat(k), eB = jat(k + TPA);  // list entries: atom and kind
+        double4 pA = a.pos[nbr_atom(eA)], pB = a.pos[nbr_atom(eB)];
         for (; k < nn; k += 2 * TPA) {
-            const int jA2 = jat(k + 2 * TPA), jB2 = jat(k + 3 * TPA);
-            pair(jA, pA, kA);
-            pA = a.pos[jA2];
-            kA = a.meta[jA2].x;
-            if (k + TPA < nn) pair(jB, pB, kB);
-            pB = a.pos[jB2];
-            kB = a.meta[jB2].x;
-            jA = jA2;
-            jB = jB2;
+            const int eA2 = jat(k + 2 * TPA), eB2 = jat(k + 3 * TPA);
+            pair(nbr_atom(eA), pA, nbr_kind(eA));
+            pA = a.pos[nbr_atom(eA2)];
+            if (k + TPA < nn) pair(nbr_atom(eB), pB, nbr_kind(eB));
+            pB = a.pos[nbr_atom(eB2)];
+            eA = eA2;
+            eB = eB2;
         }
     }
 #pragma unroll
@@ -1453,13 +1460,14 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
         int nn = a.nnbr[t];
         const int* row = a.nbr + (size_t)t * a.cap;
         // Software pipeline over the row: the row entries two passes ahead
-        // and the next pass's neighbour records are loaded while the
+        // and the next pass's neighbour positions are loaded while the
         // current pair is evaluated, unrolled by two on alternating
-        // registers (A, B).  Entries past the row end read a clamped index
-        // and stand for atom i itself, never evaluated.
+        // registers (A, B); an entry carries the neighbour's kind.  Entries
+        // past the row end read a clamped index and stand for atom i
+        // itself, never evaluated.
         auto jat = [&](int k) {
             int v = row[min(k, max(nn - 1, 0))];
-            return k < nn ? v : i;
+            return k < nn ? v : nbr_entry(i, ki);
         };
         auto pair = [&](int j, const double4& pj, int kj) {
             double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
@@ -1517,19 +1525,16 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
             }
         };
         int k = sub;
-        int jA = jat(k), jB = jat(k + TPA);
-        double4 pA = a.pos[jA], pB = a.pos[jB];
-        int kA = a.meta[jA].x, kB = a.meta[jB].x;
+        int eA = jat(k), eB = jat(k + TPA);  // list entries: atom and kind
+        double4 pA = a.pos[nbr_atom(eA)], pB = a.pos[nbr_atom(eB)];
         for (; k < nn; k += 2 * TPA) {
-            const int jA2 = jat(k + 2 * TPA), jB2 = jat(k + 3 * TPA);
-            pair(jA, pA, kA);
-            pA = a.pos[jA2];
-            kA = a.meta[jA2].x;
-            if (k + TPA < nn) pair(jB, pB, kB);
-            pB = a.pos[jB2];
-            kB = a.meta[jB2].x;
-            jA = jA2;
-            jB = jB2;
+            const int eA2 = jat(k + 2 * TPA), eB2 = jat(k + 3 * TPA);
+            pair(nbr_atom(eA), pA, nbr_kind(eA));
+            pA = a.pos[nbr_atom(eA2)];
+            if (k + TPA < nn) pair(nbr_atom(eB), pB, nbr_kind(eB));
+            pB = a.pos[nbr_atom(eB2)];
+            eA = eA2;
+            eB = eB2;
         }
     }
     double sx = fx, sy = fy, sz = fz;
@@ -1570,7 +1575,7 @@ __global__ void k_pairs_within(const double4* __restrict__ pos, const int* __res
     double4 pi = pos[i];
     int nn = nnbr[t];
     for (int k = 0; k < nn; k++) {
-        int j = nbr[(size_t)t * cap + k];
+        int j = nbr_atom(nbr[(size_t)t * cap + k]);
         if (j <= i) continue;
         double4 pj = pos[j];
         double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
