@@ -1801,7 +1801,9 @@ __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int
     const T2* cz = tab + (size_t)phase_row(2, un.kz0, kmax) * ld;
     const T2* c1 = tab + (size_t)phase_row(2, 1, kmax) * ld;
     const int per = width / NW;
-    const int a0 = chunk * width + w * per;
+    // the wave's atom range made visibly uniform: the charge and Ez[1] of
+    // an atom are then scalar loads, three vector loads per atom remain
+    const int a0 = __builtin_amdgcn_readfirstlane(chunk * width + w * per);
     const int a1 = min(a0 + per, ld);
     T r0 = 0, i0 = 0, r1 = 0, i1 = 0, r2 = 0, i2 = 0, r3 = 0, i3 = 0;
 #pragma unroll 4
