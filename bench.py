@@ -213,7 +213,7 @@ def run_form(workload, form, steps, warmup, oracle_forces, oracle_energy):
     err = force_error(s, oracle_forces, oracle_energy)
     # launch knobs for this size bucket (lumol_amd.tuning), chosen before timing
     from lumol_amd import tuning
-    knobs = tuning.autotune(s, reps=5)
+    knobs = tuning.autotune(s, reps=10)
     sim = make_sim(workload)
 
     # warm-up, with a profiled pass over every kernel class to find the dominant one

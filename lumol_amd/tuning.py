@@ -2,14 +2,17 @@
 
 The knobs (``lg_tuning``): lanes per atom of the pair and neighbour-list
 kernels, row ranges of the k-space force kernel, the form of the rho(k)
-kernel (rows, with its atoms per block, or the matrix cores).  They are tuned one after the other: every candidate value is
-timed with the library's HIP-event profiler over `reps` force evaluations of
-the system itself -- the summed device time of all kernel classes, so that a
-choice that speeds one kernel and slows another (more k-space partials to
-add) is seen as such -- and the fastest is kept.  Results are cached per bucket -- (Ewald or not, precision, atoms
-rounded to a power of two, k-vectors rounded to a power of two) -- so a
-server tunes once per bucket and reuses the choice for every later job of
-that bucket.  Force evaluations leave the system's state unchanged.
+kernel (rows, with its atoms per block, or the matrix cores).  They are
+tuned one after the other: every candidate value is timed with the
+library's HIP-event profiler over `reps` force evaluations of the system
+itself, in `rounds` interleaved passes of which the fastest counts -- the
+summed device time of all kernel classes, so that a choice that speeds one
+kernel and slows another (more k-space partials to add) is seen as such --
+and the fastest candidate is kept.  Results are cached per bucket -- (Ewald
+or not, precision, atoms rounded to a power of two, k-vectors rounded to a
+power of two) -- so a server tunes once per bucket and reuses the choice for
+every later job of that bucket.  Force evaluations leave the system's state
+unchanged.
 """
 import math
 import threading
@@ -41,7 +44,7 @@ def _time_all(system, reps):
     return sum(ms for ms, _ in prof.values()) / reps
 
 
-def autotune(system, reps=5, use_cache=True):
+def autotune(system, reps=5, use_cache=True, rounds=3):
     """Picks and applies the knobs for `system`; returns them (dict)."""
     key = bucket(system)
     keep = tuple(getattr(system, "tuning_", (0, 0, 0, 0, 0, 0))[3:5])  # nlist_build, md_graph: left as set
@@ -56,12 +59,16 @@ def autotune(system, reps=5, use_cache=True):
         best = {"pair_lanes": 0, "kspace_split": 0, "rho_width": 0, "rho_kind": 0}
 
         def pick(field, values, **extra):
-            times = {}
-            for v in values:
-                trial = dict(best, **{field: v}, **extra)
-                apply(trial)
-                system.forces()
-                times[v] = _time_all(system, reps)
+            # each candidate timed in `rounds` interleaved passes, the
+            # fastest pass kept: a single short timing flips the choice
+            # between runs on small systems
+            times = {v: float("inf") for v in values}
+            for _ in range(rounds):
+                for v in values:
+                    trial = dict(best, **{field: v}, **extra)
+                    apply(trial)
+                    system.forces()
+                    times[v] = min(times[v], _time_all(system, reps))
             return min(times, key=times.get), times
 
         # pair kernel lanes
@@ -73,7 +80,10 @@ def autotune(system, reps=5, use_cache=True):
             trial = dict(best, rho_kind=2)
             apply(trial)
             system.forces()
-            t_mfma = _time_all(system, reps) if system.tuning()["rho_kind"] == 2 else float("inf")
+            t_mfma = float("inf")
+            if system.tuning()["rho_kind"] == 2:
+                for _ in range(rounds):
+                    t_mfma = min(t_mfma, _time_all(system, reps))
             if t_mfma < times[width]:
                 best["rho_kind"] = 2
             else:
