@@ -896,8 +896,8 @@ int launch_nlist(lg_system* s, bool check) {
         k_cell_scan<<<1, 1024, 0, st>>>(s->count.p, s->start.p, g.ncell, flag);
         k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(s->cellid.p, s->slot.p, s->start.p, s->tmp.p, n,
                                                             flag);
-        k_cell_rank<<<g.ncell, 256, 0, st>>>(s->tmp.p, s->start.p, s->sorted.p, s->cpos.p, s->pos.p, s->meta.p,
-                                             s->box, g.ncell, flag);
+        k_cell_rank<<<std::min(blocks_for(g.ncell, 4), kRankBlocks), 256, 0, st>>>(
+            s->tmp.p, s->start.p, s->sorted.p, s->cpos.p, s->pos.p, s->meta.p, s->box, g.ncell, flag);
     }
     int nb = blocks_for(size_t(n) * s->tpa, 256);
     // FP32 candidate test with a 1e-4 relative margin on rlist

@@ -46,3 +46,19 @@ def test_ka_large_momentum_and_drift():
         s.close()
     assert drift[LG_PRECISION_MIXED] <= 2.0 * drift[0] + 1e-6
     assert abs(final[LG_PRECISION_MIXED] - final[0]) <= 1e-4 * abs(final[0])
+
+
+@pytest.mark.parametrize("build", [1, 2])
+def test_ka_cell_list_path_parity(build):
+    # 40000 atoms: past the single-workgroup cell list and the direct build,
+    # so the four-kernel cell list (count, scan, scatter, rank) feeds the
+    # Verlet build; both builds give the same rows and meet the oracle
+    s = kob_andersen(40000, seed=5)
+    s.set_tuning(nlist_build=build)
+    assert s.tuning()["nlist_build"] == build
+    f = s.forces()
+    fo = lo.OracleSystem(s, nthreads=8).forces()
+    assert np.abs(f - fo).max() / np.abs(fo).max() < 1e-10
+    t = kob_andersen(40000, seed=5)
+    t.set_tuning(nlist_build=3 - build)
+    assert np.array_equal(t.forces(), f)
