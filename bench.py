@@ -202,6 +202,9 @@ def load_traffic(workload, form, kernel):
         return None
 
 
+TUNING_CACHE = None  # --tuning-cache
+
+
 def run_form(workload, form, steps, warmup, oracle_forces, oracle_energy):
     from lumol_amd._native import LG_PRECISION_FP64, LG_PRECISION_MIXED
     s = build_system(workload)
@@ -214,6 +217,8 @@ def run_form(workload, form, steps, warmup, oracle_forces, oracle_energy):
     # launch knobs for this size bucket (lumol_amd.tuning), chosen before timing
     from lumol_amd import tuning
     knobs = tuning.autotune(s, reps=10)
+    if TUNING_CACHE:
+        tuning.save_cache(TUNING_CACHE)
     sim = make_sim(workload)
 
     # warm-up, with a profiled pass over every kernel class to find the dominant one
@@ -606,9 +611,18 @@ def main():
     ap.add_argument("--steps-scale", type=float, default=0.02, help="serve: factor on the 1k..20k MD steps")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serve: span of the arrivals")
     ap.add_argument("--form", choices=["fp64", "mixed"], default="mixed", help="serve: MD arithmetic form")
+    ap.add_argument("--tuning-cache", default=None,
+                    help="JSON file of autotuner choices: read if it exists, written after tuning (so that a "
+                         "profiled rerun launches the same kernel variants as the timed run)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB device copy (scale factor of the HBM byte counters)")
     args = ap.parse_args()
+    if args.tuning_cache:
+        global TUNING_CACHE
+        TUNING_CACHE = args.tuning_cache
+        if os.path.exists(args.tuning_cache):
+            from lumol_amd import tuning
+            tuning.load_cache(args.tuning_cache)
     if args.calibrate_copy:
         calibrate_copy()
         return

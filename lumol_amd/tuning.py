@@ -99,3 +99,25 @@ def cached():
     """The choices made so far, by bucket."""
     with _lock:
         return dict(_cache)
+
+
+def save_cache(path):
+    """Writes the choices made so far to a JSON file (a list of
+    [bucket, knobs] pairs)."""
+    import json
+    with _lock:
+        items = [[list(k), v] for k, v in _cache.items()]
+    with open(path, "w") as f:
+        json.dump(items, f)
+
+
+def load_cache(path):
+    """Adds the choices of a file written by save_cache, so that another
+    process (a profiled rerun, another server) uses the same knobs for the
+    same buckets instead of timing them again."""
+    import json
+    with open(path) as f:
+        items = json.load(f)
+    with _lock:
+        for k, v in items:
+            _cache[tuple(k)] = dict(v)
