@@ -191,6 +191,11 @@ struct lg_system {
     // matrix-core rho: column tiles of (iky, ikz), output -> k map, identity slots
     int mfma_nt = 0;
     DevBuf<int> mcols, mkmap, kslot_id;
+    // erfcx tables of the real-space erfc (empty: library erfc)
+    DevBuf<double> ecx64;
+    DevBuf<float> ecx32;
+    double ecx_inv_h = 0.0;
+    bool ecx_ok = false;
     double kg[3] = {0, 0, 0};    // 2 pi / L per axis
     DevBuf<double> efac, vfac;
     DevBuf<KRec> krec;           // rho(k) and force factor per k
@@ -467,6 +472,58 @@ int prepare_meta(lg_system* s) {
     return LG_OK;
 }
 
+// erfcx(x) = exp(x^2) erfc(x) on [0, xmax] as kErfcxPieces polynomials of
+// degree `deg` in t in [-1, 1] per piece: the piece's Chebyshev interpolant
+// (deg + 1 Chebyshev nodes, long double), expanded in powers of t.  Returns
+// the largest relative error against erfcl over 64 points per piece.
+static long double erfcx_ld(long double x) { return erfcl(x) * expl(x * x); }
+template <typename T>
+static double fit_erfcx(double xmax, int deg, std::vector<T>& out) {
+    const int M = kErfcxPieces, D = deg;
+    const long double pi = 3.141592653589793238462643383279502884L;
+    const long double h = (long double)xmax / M;
+    out.assign(size_t(M) * (D + 1), T(0));
+    double worst = 0.0;
+    std::vector<long double> f(D + 1), c(D + 1), mono(D + 1), tprev(D + 1), tcur(D + 1), tnext(D + 1);
+    for (int k = 0; k < M; k++) {
+        const long double mid = (k + 0.5L) * h, half = 0.5L * h;
+        for (int j = 0; j <= D; j++) f[j] = erfcx_ld(mid + half * cosl(pi * (j + 0.5L) / (D + 1)));
+        for (int m = 0; m <= D; m++) {
+            long double acc = 0;
+            for (int j = 0; j <= D; j++) acc += f[j] * cosl(m * pi * (j + 0.5L) / (D + 1));
+            c[m] = acc * 2.0L / (D + 1);
+        }
+        c[0] *= 0.5L;
+        // sum c_m T_m(t) in powers of t (T_0 = 1, T_1 = t, T_{m+1} = 2t T_m - T_{m-1})
+        std::fill(mono.begin(), mono.end(), 0.0L);
+        std::fill(tprev.begin(), tprev.end(), 0.0L);
+        std::fill(tcur.begin(), tcur.end(), 0.0L);
+        tprev[0] = 1.0L;
+        mono[0] += c[0];
+        if (D >= 1) {
+            tcur[1] = 1.0L;
+            mono[1] += c[1];
+        }
+        for (int m = 1; m < D; m++) {
+            std::fill(tnext.begin(), tnext.end(), 0.0L);
+            for (int q = 0; q < D; q++) tnext[q + 1] += 2.0L * tcur[q];
+            for (int q = 0; q <= D; q++) tnext[q] -= tprev[q];
+            for (int q = 0; q <= D; q++) mono[q] += c[m + 1] * tnext[q];
+            tprev = tcur;
+            tcur = tnext;
+        }
+        for (int q = 0; q <= D; q++) out[size_t(k) * (D + 1) + q] = T(mono[q]);
+        for (int z = 0; z < 64; z++) {
+            const long double t = -1.0L + 2.0L * (z + 0.5L) / 64.0L;
+            long double p = (long double)out[size_t(k) * (D + 1) + D];
+            for (int q = D - 1; q >= 0; q--) p = p * t + (long double)out[size_t(k) * (D + 1) + q];
+            const long double ref = erfcx_ld(mid + half * t);
+            worst = std::max(worst, double(fabsl(p - ref) / ref));
+        }
+    }
+    return worst;
+}
+
 int prepare_params(lg_system* s) {
     uint32_t K = s->nkinds;
     if (K > uint32_t(kMaxKinds)) return fail(LG_E_UNSUPPORTED, "more than 16 particle kinds");
@@ -507,6 +564,22 @@ int prepare_params(lg_system* s) {
         c.f_const = std::erfc(ac) / (s->coul_rc * s->coul_rc) + kFrac2SqrtPi * c.alpha * std::exp(-ac2) / s->coul_rc;
     }
     s->coul = c;
+    // erfcx tables for the pair kernels' real-space erfc (Ewald and Wolf);
+    // kept only if the fit meets 1e-14 (FP64) and 1e-7 (FP32) relative
+    s->ecx_ok = false;
+    if ((s->coul_kind == COUL_EWALD || s->coul_kind == COUL_WOLF) && c.alpha > 0.0 && c.rc > 0.0) {
+        const double xmax = c.alpha * c.rc * (1.0 + 1e-9);
+        std::vector<double> t64;
+        std::vector<float> t32;
+        const double e64 = fit_erfcx(xmax, kErfcxDeg64, t64);
+        const double e32 = fit_erfcx(xmax, kErfcxDeg32, t32);
+        if (xmax < 27.0 && e64 < 1e-14 && e32 < 1e-7) {
+            LG_TRY(upload(s->ecx64, t64, S(s)));
+            LG_TRY(upload(s->ecx32, t32, S(s)));
+            s->ecx_inv_h = kErfcxPieces / xmax;
+            s->ecx_ok = true;
+        }
+    }
     s->params_dirty = false;
     s->nlist_dirty = true;
     return LG_OK;
@@ -1058,7 +1131,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
             k_nlist_direct<false><<<s->grid.ncell, kCellBlock, 0, st>>>(s->direct_args);
         s->direct_pending = false;
     }
-    PairArgs pa;
+    PairArgs pa{};
     pa.pos = s->pos.p;
     pa.meta = s->meta.p;
     pa.sorted = s->sorted.p;
@@ -1073,6 +1146,9 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     pa.force = s->force.p;
     pa.epart = s->epart_pairs.p;
     pa.rebuild = s->flags.p;
+    pa.ecx64 = s->ecx_ok ? s->ecx64.p : nullptr;
+    pa.ecx32 = s->ecx_ok ? s->ecx32.p : nullptr;
+    pa.ecx_inv_h = s->ecx_inv_h;
     // the intra-molecular terms run as the last blocks of the same launch
     IntraArgs& ia = pa.intra;
     ia = IntraArgs{};
@@ -1541,6 +1617,8 @@ int lg_system_destroy(lg_system* s) {
                             &s->mc_cell_count, &s->mc_atom_cell, &s->kslot, &s->mcols, &s->mkmap, &s->kslot_id};
     for (auto* b : ibufs) b->release();
     s->units.release();
+    s->ecx64.release();
+    s->ecx32.release();
     s->phase_tab.release();
     s->phase_q.release();
     s->phase_tab32.release();

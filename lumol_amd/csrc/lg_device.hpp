@@ -173,6 +173,37 @@ __device__ __forceinline__ double wave_sum(double v) {
 // =====================================================================================
 namespace lg {
 
+// erfc(x) = exp(-x^2) erfcx(x) with erfcx from a piecewise polynomial table
+// over [0, alpha rc]: kErfcxPieces equal pieces, each a polynomial in
+// t in [-1, 1] (the piece's Chebyshev interpolant at degree kErfcxDeg64 or
+// kErfcxDeg32, fitted on the host in long double against erfcl and checked
+// there).  exp(-x^2) is the factor the Ewald/Wolf force needs anyway, so
+// erfc costs a few FMAs instead of a second exponential and a rational
+// approximation.
+constexpr int kErfcxPieces = 256;
+constexpr int kErfcxDeg64 = 5;
+constexpr int kErfcxDeg32 = 3;
+__device__ __forceinline__ double erfcx_tab(const double* __restrict__ c, double x, double inv_h) {
+    const double u = x * inv_h;
+    const int k = min(max((int)u, 0), kErfcxPieces - 1);
+    const double t = 2.0 * (u - (double)k) - 1.0;
+    const double* q = c + k * (kErfcxDeg64 + 1);
+    double p = q[kErfcxDeg64];
+#pragma unroll
+    for (int d = kErfcxDeg64 - 1; d >= 0; d--) p = p * t + q[d];
+    return p;
+}
+__device__ __forceinline__ float erfcx_tab(const float* __restrict__ c, float x, float inv_h) {
+    const float u = x * inv_h;
+    const int k = min(max((int)u, 0), kErfcxPieces - 1);
+    const float t = 2.0f * (u - (float)k) - 1.0f;
+    const float* q = c + k * (kErfcxDeg32 + 1);
+    float p = q[kErfcxDeg32];
+#pragma unroll
+    for (int d = kErfcxDeg32 - 1; d >= 0; d--) p = p * t + q[d];
+    return p;
+}
+
 // erfc(x) for x >= 0 from the Chebyshev fit of Numerical Recipes in C, 2nd
 // ed., section 6.2 ("erfcc"): fractional error below 1.2e-7 everywhere.
 // `ex2` = exp(-x^2) is returned for the Ewald force term.

@@ -1294,6 +1294,11 @@ struct PairArgs {
     int pair_blocks, intra_blocks;
     IntraArgs intra;
     RhoReduceArgs rr;
+    // erfcx tables of the Ewald / Wolf real-space erfc (erfcx_tab), or null
+    // for the library erfc
+    const double* ecx64;
+    const float* ecx32;
+    double ecx_inv_h;
 };
 
 constexpr int kMaxKinds = 16;
@@ -1318,6 +1323,8 @@ template <bool kEnergy, int TPA, int CK>
 __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
     __shared__ PairParam tab[kMaxKinds * kMaxKinds];
     __shared__ double red[kPairBlock / 64][8];
+    constexpr bool kTab = CK == CK_EWALD || CK == CK_WOLF;
+    __shared__ double ecx[kTab ? kErfcxPieces * (kErfcxDeg64 + 1) : 1];
     if ((int)blockIdx.x >= a.pair_blocks) {  // uniform over the block
         const int b = blockIdx.x - a.pair_blocks;
         if (b < a.intra_blocks)
@@ -1329,6 +1336,9 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.rebuild) *a.rebuild = 0;
     int nk2 = a.nkinds * a.nkinds;
     for (int x = threadIdx.x; x < nk2; x += blockDim.x) tab[x] = a.table[x];
+    const bool use_tab = kTab && a.ecx64 != nullptr;  // uniform
+    if (use_tab)
+        for (int x = threadIdx.x; x < kErfcxPieces * (kErfcxDeg64 + 1); x += blockDim.x) ecx[x] = a.ecx64[x];
     __syncthreads();
 
     int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1374,10 +1384,28 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs(PairArgs a) {
             double qiqj = qi * pj.w;
             if (CK != CK_NONE && qiqj != 0.0 && r <= c.rc) {
                 double e, f;
-                if (CK == CK_WOLF)
+                if (kTab && use_tab) {
+                    // ewald.rs:387-428 / wolf.rs:90-117 with erfc = exp(-x^2) erfcx(x)
+                    const double ar = c.alpha * r;
+                    if (CK == CK_WOLF) {
+                        const double ex = exp(-ar * ar);
+                        const double erfcv = ex * erfcx_tab(ecx, ar, a.ecx_inv_h);
+                        e = qiqj * (erfcv * inv_r - c.e_const) / kFourPiEps0;
+                        const double factor = erfcv * inv_r * inv_r + c.alpha * kFrac2SqrtPi * ex * inv_r;
+                        f = qiqj * (factor - c.f_const) * inv_r / kFourPiEps0;
+                    } else {
+                        const double ex = exp(-c.alpha * c.alpha * r * r);
+                        const double erfcv = ex * erfcx_tab(ecx, ar, a.ecx_inv_h);
+                        const double g = c.alpha * kFrac2SqrtPi * ex;
+                        const double pref = qiqj / kFourPiEps0;
+                        e = pref * erfcv * inv_r;
+                        f = pref * inv_r * inv_r * (g + erfcv * inv_r);
+                    }
+                } else if (CK == CK_WOLF) {
                     wolf_pair(c, qiqj, r, inv_r, e, f);
-                else
+                } else {
                     ewald_real(c.alpha, qiqj, r, inv_r, CK == CK_EWALD_ERF, e, f);
+                }
                 fs += f;
                 e_c = e;
             }
@@ -1446,6 +1474,8 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
     };
     __shared__ P32 tab[kMaxKinds * kMaxKinds];
     __shared__ double red[kPairBlock / 64][8];
+    constexpr bool kTab = CK == CK_EWALD || CK == CK_WOLF;
+    __shared__ float ecx[kTab ? kErfcxPieces * (kErfcxDeg32 + 1) : 1];
     if ((int)blockIdx.x >= a.pair_blocks) {  // uniform over the block
         const int b = blockIdx.x - a.pair_blocks;
         if (b < a.intra_blocks)
@@ -1461,7 +1491,11 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
         float rc = (float)pp.rc;
         tab[x] = P32{pp.pot, pp.inter_excl, (float)pp.p0, (float)pp.p1, rc * rc, (float)pp.shift};
     }
+    const bool use_tab = kTab && a.ecx32 != nullptr;  // uniform
+    if (use_tab)
+        for (int x = threadIdx.x; x < kErfcxPieces * (kErfcxDeg32 + 1); x += blockDim.x) ecx[x] = a.ecx32[x];
     __syncthreads();
+    const float ecx_inv_h = (float)a.ecx_inv_h;
     const float alpha = (float)a.coul.alpha;
     const float rcc2 = (float)(a.coul.rc * a.coul.rc);
     const float g0 = (float)(a.coul.alpha * kFrac2SqrtPi);
@@ -1516,8 +1550,14 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
             float qiqj = qi * (float)pj.w;
             if (CK != CK_NONE && qiqj != 0.f && r2 <= rcc2) {
                 float r = r2 * inv_r;
-                float ex2;
-                float erfcv = erfc_cheb(alpha * r, ex2);
+                float ex2, erfcv;
+                if (kTab && use_tab) {
+                    const float x = alpha * r;
+                    ex2 = __expf(-x * x);
+                    erfcv = ex2 * erfcx_tab(ecx, x, ecx_inv_h);
+                } else {
+                    erfcv = erfc_cheb(alpha * r, ex2);
+                }
                 float pref = qiqj * inv4pe;
                 if (CK == CK_EWALD_ERF) {
                     float erfv = 1.f - erfcv;
