@@ -216,7 +216,7 @@ def run_form(workload, form, steps, warmup, oracle_forces, oracle_energy):
     err = force_error(s, oracle_forces, oracle_energy)
     # launch knobs for this size bucket (lumol_amd.tuning), chosen before timing
     from lumol_amd import tuning
-    knobs = tuning.autotune(s, reps=10)
+    knobs = tuning.autotune(s, reps=10, rounds=5)
     if TUNING_CACHE:
         tuning.save_cache(TUNING_CACHE)
     sim = make_sim(workload)

@@ -6,9 +6,9 @@ kernel (rows, with its atoms per block, or the matrix cores).  They are
 tuned one after the other: every candidate value is timed with the
 library's HIP-event profiler over `reps` force evaluations of the system
 itself, in `rounds` interleaved passes of which the fastest counts -- the
-summed device time of all kernel classes, so that a choice that speeds one
-kernel and slows another (more k-space partials to add) is seen as such --
-and the fastest candidate is kept.  Results are cached per bucket -- (Ewald
+summed device time of the kernel classes the knob changes (the rho chunk
+width, for example, changes both the rho launch and the pair launch that
+reduces its partials) -- and the fastest candidate is kept.  Results are cached per bucket -- (Ewald
 or not, precision, atoms rounded to a power of two, k-vectors rounded to a
 power of two) -- so a server tunes once per bucket and reuses the choice for
 every later job of that bucket.  Force evaluations leave the system's state
@@ -35,13 +35,24 @@ def bucket(system):
     return (ewald, int(system.precision), p2(n), p2(nk))
 
 
-def _time_all(system, reps):
-    """Device time (ms) of one force evaluation, summed over kernel classes."""
-    system.profile_begin(list(nat.KERNEL_CLASSES))
+def _time_all(system, reps, classes=None):
+    """Device time (ms) of one force evaluation, summed over `classes` (the
+    kernel classes a knob changes; all by default)."""
+    system.profile_begin(list(classes or nat.KERNEL_CLASSES))
     for _ in range(reps):
         system.forces()
     prof = system.profile_end()
     return sum(ms for ms, _ in prof.values()) / reps
+
+
+# the kernel classes each knob changes: timing only those keeps the noise of
+# the other launches out of a choice between close candidates
+_KNOB_CLASSES = {
+    "pair_lanes": ("nlist", "pairs"),
+    "kspace_split": ("kspace_force",),
+    "rho_width": ("kspace_rho", "pairs"),  # the rho reduction runs in the pair launch
+    "rho_kind": ("kspace_rho", "pairs"),
+}
 
 
 def autotune(system, reps=5, use_cache=True, rounds=3):
@@ -68,7 +79,7 @@ def autotune(system, reps=5, use_cache=True, rounds=3):
                     trial = dict(best, **{field: v}, **extra)
                     apply(trial)
                     system.forces()
-                    times[v] = min(times[v], _time_all(system, reps))
+                    times[v] = min(times[v], _time_all(system, reps, _KNOB_CLASSES[field]))
             return min(times, key=times.get), times
 
         # pair kernel lanes
@@ -83,7 +94,7 @@ def autotune(system, reps=5, use_cache=True, rounds=3):
             t_mfma = float("inf")
             if system.tuning()["rho_kind"] == 2:
                 for _ in range(rounds):
-                    t_mfma = min(t_mfma, _time_all(system, reps))
+                    t_mfma = min(t_mfma, _time_all(system, reps, _KNOB_CLASSES["rho_kind"]))
             if t_mfma < times[width]:
                 best["rho_kind"] = 2
             else:
