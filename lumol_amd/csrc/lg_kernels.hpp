@@ -1664,7 +1664,11 @@ __device__ __forceinline__ double4 total_force(int i, int n, const double4* __re
                                                const double4* __restrict__ fk, int ksplit,
                                                const double4* __restrict__ pos) {
     double4 f = force[i];
+    // loops unrolled so that their loads are in flight together (a split
+    // at a time cost one L2 round trip per split); the additions keep
+    // their order
     if (slot_start) {
+#pragma unroll 4
         for (int sl = slot_start[i]; sl < slot_start[i + 1]; sl++) {
             double4 c = contrib[sl];
             f.x += c.x;
@@ -1674,6 +1678,7 @@ __device__ __forceinline__ double4 total_force(int i, int n, const double4* __re
     }
     if (ksplit > 0) {
         double sx = 0, sy = 0, sz = 0;
+#pragma unroll 8
         for (int g = 0; g < ksplit; g++) {
             double4 p = fk[(size_t)g * n + i];
             sx += p.x;
