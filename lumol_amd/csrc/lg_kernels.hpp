@@ -1222,7 +1222,8 @@ __device__ __forceinline__ void rho_reduce_block(const RhoReduceArgs& a, int blk
     double re = 0.0, im = 0.0;
     if (k < nk) {
         const int slot = a.kslot[k];
-        for (int c = g; c < nchunks; c += G) {
+#pragma unroll 4
+        for (int c = g; c < nchunks; c += G) {  // loads in flight together, sums in order
             double2 p = partial[(size_t)c * nslot + slot];
             re += p.x;
             im += p.y;
