@@ -893,6 +893,10 @@ static bool cell_build(const lg_system* s) {
 // Where the displacement tests write the atoms' cell tags: only for the
 // direct build.
 static CellTag cell_tags(lg_system* s) { return CellTag{direct_build(s) ? s->cellid.p : nullptr, s->grid, s->box}; }
+// Workgroups per cell of the direct build: about 256 workgroups in all (one
+// per CU of the MI355X), at most 8 per cell.
+static int direct_split(const lg_system* s) { return std::max(1, std::min(8, 256 / std::max(1, s->grid.ncell))); }
+static int direct_blocks(const lg_system* s) { return s->grid.ncell * direct_split(s); }
 
 // Atoms per block of the rho pass: the narrowest chunk (64..512) that keeps
 // the grid at or below ~2048 blocks, so that per-block work stays
@@ -939,7 +943,7 @@ int launch_nlist(lg_system* s, bool check) {
         const float rl2 = (!s->periodic || s->rlist <= 0) ? INFINITY
                                                            : float(s->rlist * (1.0 + 1e-4) * s->rlist * (1.0 + 1e-4));
         DirectArgs da{s->pos.p, s->meta.p, s->cellid.p, n, g, s->box, rl2, s->cap,
-                      s->nbr.p, s->nnbr.p, s->ref.p, s->sorted.p, flag};
+                      s->nbr.p, s->nnbr.p, s->ref.p, s->sorted.p, flag, direct_split(s)};
         const bool nearest = s->periodic && (g.reach[0] == 0 || g.reach[1] == 0 || g.reach[2] == 0);
         prof_stop(s, prof);
         if (s->coul_kind == COUL_EWALD && s->nk > 0) {
@@ -951,9 +955,9 @@ int launch_nlist(lg_system* s, bool check) {
         }
         long p2 = prof_start(s, LG_KERNEL_NLIST);
         if (nearest)
-            k_nlist_direct<true><<<g.ncell, kCellBlock, 0, st>>>(da);
+            k_nlist_direct<true><<<direct_blocks(s), kCellBlock, 0, st>>>(da);
         else
-            k_nlist_direct<false><<<g.ncell, kCellBlock, 0, st>>>(da);
+            k_nlist_direct<false><<<direct_blocks(s), kCellBlock, 0, st>>>(da);
         prof_stop(s, p2);
         LG_HIP(hipGetLastError());
         return LG_OK;
@@ -1036,7 +1040,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         // blocks first (cells rounded up to 8), then the rho blocks
         const bool direct = s->direct_pending;
         const bool cells = s->cells_pending || direct;
-        const int ncb = cells ? (s->grid.ncell + 7) / 8 * 8 : 0;
+        const int ncb = cells ? ((direct ? direct_blocks(s) : s->grid.ncell) + 7) / 8 * 8 : 0;
         const CellsArgs& ca = s->cells_args;
         const DirectArgs& da = s->direct_args;
         const bool near = s->cells_nearest;
@@ -1044,9 +1048,9 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
             // the matrix-core rho shares its launch with the cell-list build
             // only; a direct build runs first on its own
             if (direct && near)
-                k_nlist_direct<true><<<s->grid.ncell, kCellBlock, 0, st>>>(da);
+                k_nlist_direct<true><<<direct_blocks(s), kCellBlock, 0, st>>>(da);
             else if (direct)
-                k_nlist_direct<false><<<s->grid.ncell, kCellBlock, 0, st>>>(da);
+                k_nlist_direct<false><<<direct_blocks(s), kCellBlock, 0, st>>>(da);
             const bool cells = s->cells_pending;
             const RhoShape sh = rho_mfma_shape(s, mixed);
             nchunks = sh.nchunks;
@@ -1126,9 +1130,9 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     }
     if (s->direct_pending) {
         if (s->cells_nearest)
-            k_nlist_direct<true><<<s->grid.ncell, kCellBlock, 0, st>>>(s->direct_args);
+            k_nlist_direct<true><<<direct_blocks(s), kCellBlock, 0, st>>>(s->direct_args);
         else
-            k_nlist_direct<false><<<s->grid.ncell, kCellBlock, 0, st>>>(s->direct_args);
+            k_nlist_direct<false><<<direct_blocks(s), kCellBlock, 0, st>>>(s->direct_args);
         s->direct_pending = false;
     }
     PairArgs pa{};

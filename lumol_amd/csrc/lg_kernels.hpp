@@ -756,7 +756,11 @@ __global__ __launch_bounds__(kCellBlock) void k_nlist_cells(CellsArgs ca) {
 // neighbour on the other).  Candidates go to an LDS list (windows of
 // kDirectCand, re-read past that), then in tiles through tile_scan; rows are
 // indexed by atom (sorted[i] = i), so no launch has to sort anything and a
-// step without a rebuild pays no cell-list launch.
+// step without a rebuild pays no cell-list launch.  A cell's home atoms are
+// dealt to `split` workgroups (home rank r to slice r % split), each scanning
+// the whole candidate list for its slice, so that a grid of few cells still
+// fills the device; every row is still built by one workgroup, in candidate
+// order, so the rows do not depend on the split.
 constexpr int kDirectCand = 4096;
 constexpr int kDirectHome = 256;
 constexpr int kDirectBatch = 8;  // scan rounds (of one atom per thread) per pair of barriers
@@ -775,6 +779,7 @@ struct DirectArgs {
     double4* ref;
     int* sorted;
     int* flags;
+    int split;  // workgroups per cell (1..8)
 };
 
 struct DirectLds {
@@ -799,9 +804,11 @@ __device__ __forceinline__ void nlist_direct_block(const DirectArgs& a, int blk,
     const Box& b = a.b;
     const int n = a.n;
     const float Lx = (float)b.L[0], Ly = (float)b.L[1], Lz = (float)b.L[2];
-    for (int c = blk; c < g.ncell; c += nblk) {
+    const int split = a.split;
+    for (int cs = blk; cs < g.ncell * split; cs += nblk) {
+        const int c = cs / split, sl = cs - c * split;
         const int hx = c % g.n[0], hy = (c / g.n[0]) % g.n[1], hz = c / (g.n[0] * g.n[1]);
-        int hwin = 0, H = 0;
+        int hwin = 0, Hs = 0;
         do {
             int cwin = 0, M = 0;
             do {
@@ -876,17 +883,19 @@ __device__ __forceinline__ void nlist_direct_block(const DirectArgs& a, int blk,
                         const bool cand = (bc[u] >> lane) & 1ull, home = (bh[u] >> lane) & 1ull;
                         const int rc = basec + offc + __popcll(bc[u] & below) - cwin;
                         if (cand && rc >= 0 && rc < kDirectCand) lds.cand[rc] = i | (code[u] << 24);
-                        const int rh = baseh + offh + __popcll(bh[u] & below) - hwin;
-                        if (cwin == 0 && home && rh >= 0 && rh < kDirectHome) lds.home_row[rh] = i;
+                        const int rg = baseh + offh + __popcll(bh[u] & below);  // rank in the cell
+                        const int rh = rg / split - hwin;                       // rank in the slice window
+                        if (cwin == 0 && home && rg - (rg / split) * split == sl && rh >= 0 && rh < kDirectHome)
+                            lds.home_row[rh] = i;
                         basec += totc;
                         baseh += toth;
                     }
                     __syncthreads();  // wc / wh are rewritten by the next batch
                 }
                 M = basec;
-                H = baseh;
-                const int hn = min(kDirectHome, H - hwin);
-                if (hn <= 0) break;  // uniform: an empty cell
+                Hs = (baseh - sl + split - 1) / split;  // home ranks r < baseh with r % split == sl
+                const int hn = min(kDirectHome, Hs - hwin);
+                if (hn <= 0) break;  // uniform: an empty cell or slice
                 if (cwin == 0)
                     for (int h = tid; h < hn; h += kCellBlock) {
                         const int i = lds.home_row[h];
@@ -933,7 +942,7 @@ __device__ __forceinline__ void nlist_direct_block(const DirectArgs& a, int blk,
                 __syncthreads();
                 cwin += kDirectCand;
             } while (cwin < M);
-            const int hn = min(kDirectHome, H - hwin);
+            const int hn = min(kDirectHome, Hs - hwin);
             for (int h = tid; h < hn; h += kCellBlock) {
                 const int i = lds.home_row[h];
                 int cnt = lds.home_cnt[h];
@@ -947,7 +956,7 @@ __device__ __forceinline__ void nlist_direct_block(const DirectArgs& a, int blk,
             }
             __syncthreads();
             hwin += kDirectHome;
-        } while (hwin < H);
+        } while (hwin < Hs);
     }
 }
 
