@@ -1,9 +1,10 @@
 """The neighbour-list builds: lanes per atom (1) and one workgroup per cell
 (2) over the sorted cell list give rows with the same pairs within the
 cutoff in the same order, so forces, energies and trajectories are
-bit-identical; the direct build (3: one workgroup per cell over the atoms'
-cell tags, rows in atom order) gives the same pairs in another order, so
-the same forces to summation order.  Every build meets the FP64 parity bar
+bit-identical; the direct build (3: workgroups per cell over the atoms'
+cell tags, rows in atom order; a box with fewer than 256 cells deals each
+cell's home atoms to up to 8 workgroups) gives the same pairs in another
+order, so the same forces to summation order.  Every build meets the FP64 parity bar
 against the oracle by itself."""
 import numpy as np
 import pytest
@@ -26,6 +27,19 @@ def slab():
     return s
 
 
+def thin_slab():
+    # 10.5 x 10.5 x 8 A, rc 3.6: 5 x 5 half cells and one cell along z
+    # (nearest image), 25 cells, so the direct build deals each cell's home
+    # atoms to 8 workgroups
+    rng = np.random.default_rng(14)
+    g = np.stack(np.meshgrid(np.arange(6), np.arange(6), np.arange(4), indexing="ij"), -1).reshape(-1, 3)
+    pos = (g + 0.5) * [1.75, 1.75, 2.0] + rng.uniform(-0.3, 0.3, size=g.shape)
+    n = len(pos)
+    s = System.from_arrays(["Ar"] * n, pos, UnitCell.ortho(10.5, 10.5, 8.0), masses=np.full(n, 39.948))
+    s.set_pair_potential(("Ar", "Ar"), PairInteraction(LennardJones(1.6, 1e-3), 3.6))
+    return s
+
+
 def clustered():
     # half the atoms in a 9 A corner cube: rows far longer than the mean
     # density predicts, so the first build overflows and the list regrows
@@ -42,6 +56,9 @@ CASES = {
     "nacl-full-cells": lambda: nacl(ncell=4, cutoff=10.0, kmax=7),
     "lj-fluid": lambda: lj_fluid(4000),
     "slab-nearest-image": slab,
+    # 125 half cells: two workgroups per cell in the direct build
+    "water-125-cells": lambda: spce(nmol=500, L=24.7, cutoff=8.0, kmax=6),
+    "thin-slab-25-cells": thin_slab,
     "clustered-overflow": clustered,
 }
 
