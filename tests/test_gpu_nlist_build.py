@@ -95,7 +95,7 @@ def test_direct_build_matches(case):
     assert np.abs(f3 - fo).max() <= 1e-10 * np.abs(fo).max()
 
 
-@pytest.mark.parametrize("case", ["water-ewald", "lj-fluid"])
+@pytest.mark.parametrize("case", ["water-ewald", "water-125-cells", "lj-fluid"])
 def test_direct_build_trajectory(case):
     # 60 steps: the direct build's list is rebuilt from the tags the first
     # half-steps write, and stays within summation-order noise of build 2
@@ -103,7 +103,7 @@ def test_direct_build_trajectory(case):
     for build in (2, 3):
         s = CASES[case]()
         s.set_tuning(nlist_build=build)
-        Simulation(MolecularDynamics(VelocityVerlet(0.5 if case == "water-ewald" else 2.0))).run(s, 60)
+        Simulation(MolecularDynamics(VelocityVerlet(0.5 if case.startswith("water") else 2.0))).run(s, 60)
         out.append(s.positions.copy())
     assert np.abs(out[0] - out[1]).max() < 1e-8
 
