@@ -237,8 +237,12 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
  *   rho_width     atoms per block of the row form of the rho(k) kernel (64,
  *                 128, 256, 512)
  *   nlist_build   neighbour-list build: 1 lanes per atom, 2 one workgroup per
- *                 cell with candidates staged in LDS (default: cells when
- *                 there are >= 128 cells and >= 8 atoms per cell)
+ *                 cell with candidates staged in LDS, 3 direct: workgroups
+ *                 per cell over the atoms' cell tags, no cell list (up to
+ *                 32768 atoms; a box of fewer than 256 cells deals each
+ *                 cell's home atoms to up to 8 workgroups).  Default: 3 when
+ *                 there are >= 128 cells and >= 8 atoms per cell and it
+ *                 applies, else 2 there, else 1
  *   md_graph      lg_md_run: 1 (default) launches every kernel; 2 captures a
  *                 chunk of steps in a HIP graph once and replays it (eager
  *                 launches while a kernel profile is active).  On ROCm 7.0

@@ -870,8 +870,8 @@ int prepare(lg_system* s) {
 // its waves busy, else TPA lanes per atom (k_nlist_build); the tuning knob
 // nlist_build overrides (1 lanes per atom, 2 cells).
 // Verlet build: 1 lanes per atom over the cell list, 2 one workgroup per
-// cell over the cell list, 3 one workgroup per cell over the atoms' cell
-// tags (no cell list; up to 32768 atoms and 1023 cells per axis).  Default:
+// cell over the cell list, 3 workgroups per cell (direct_split) over the
+// atoms' cell tags (no cell list; up to 32768 atoms and 1023 cells per axis).  Default:
 // the per-cell builds when there are >= 128 cells and >= 8 atoms per cell,
 // the direct one where it applies.
 static bool direct_fits(const lg_system* s) {
