@@ -351,7 +351,8 @@ class System:
         """Launch knobs of the device kernels (0 = built-in choice); see
         lumol_amd.tuning.autotune.  rho_width: atoms per block of the row form
         of the rho(k) kernel; nlist_build: 1 lanes per atom, 2 one workgroup
-        per cell; md_graph: 1 eager launches (default), 2 HIP-graph replay of
+        per cell over the cell list, 3 direct (workgroups per cell over the
+        atoms' cell tags, up to 32768 atoms); md_graph: 1 eager launches (default), 2 HIP-graph replay of
         MD chunks; rho_kind: 1 row form, 2 matrix-core form (kmax <= 16)."""
         self.tuning_ = (int(pair_lanes), int(kspace_split), int(rho_width), int(nlist_build), int(md_graph),
                         int(rho_kind))
