@@ -398,8 +398,13 @@ def bench_spce_batched(args):
         s.set_tuning(md_graph=args.md_graph)
         return s
 
-    # FP64, one system alone
+    from lumol_amd import tuning
+
+    # FP64, one system alone, with the knobs the autotuner picks for it
     single = make(3, LG_PRECISION_FP64, 0)
+    knobs64 = tuning.autotune(single, reps=10, rounds=5)
+    if TUNING_CACHE:
+        tuning.save_cache(TUNING_CACHE)
     sim = make_sim("spce")
     sim.run(single, args.warmup)
     t0 = time.perf_counter()
@@ -408,24 +413,39 @@ def bench_spce_batched(args):
     single.close()
     # mixed, `batch` systems at once
     systems = [make(3 + k, LG_PRECISION_MIXED, 1 + k) for k in range(args.batch)]
-    err = force_error(systems[0], fo, eo)
     sims = [make_sim("spce") for _ in systems]
     for s, sm in zip(systems, sims):
         sm.run(s, args.warmup)
-    barrier = threading.Barrier(len(systems) + 1)
 
-    def work(s, sm):
+    def concurrent(steps, warm=0):
+        # every system advances `steps` MD steps on its own host thread and
+        # stream; the wall time from the common start to the last finish
+        barrier = threading.Barrier(len(systems) + 1)
+
+        def work(s, sm):
+            if warm:
+                sm.run(s, warm)
+            barrier.wait()
+            sm.run(s, steps)
+
+        threads = [threading.Thread(target=work, args=(s, sm)) for s, sm in zip(systems, sims)]
+        for t in threads:
+            t.start()
         barrier.wait()
-        sm.run(s, args.steps)
+        t0 = time.perf_counter()
+        for t in threads:
+            t.join()
+        return time.perf_counter() - t0
 
-    threads = [threading.Thread(target=work, args=(s, sm)) for s, sm in zip(systems, sims)]
-    for t in threads:
-        t.start()
-    barrier.wait()
-    t0 = time.perf_counter()
-    for t in threads:
-        t.join()
-    elapsed = time.perf_counter() - t0
+    # one set of knobs for the batch, timed on the batch itself
+    knobs = tuning.autotune_concurrent(systems, lambda: concurrent(200, warm=20))
+    # the stated error, of the first system's initial state with these knobs
+    chk = make(3, LG_PRECISION_MIXED, 1)
+    chk.set_tuning(knobs["pair_lanes"], knobs["kspace_split"], knobs["rho_width"], 0, args.md_graph)
+    err = force_error(chk, fo, eo)
+    chk.close()
+    concurrent(args.warmup)
+    elapsed = concurrent(args.steps)
     natoms = sum(s.size() for s in systems)
     value = natoms * args.steps / elapsed
     for s in systems:
@@ -438,6 +458,7 @@ def bench_spce_batched(args):
                    "atoms_per_system": systems[0].size()},
         "error_vs_fp64_oracle": err,
         "fp64_single_system": fp64_rate,
+        "tuning": {"fp64_single_system": knobs64, "mixed_batched": knobs},
         "mixed_batched_over_fp64": value / fp64_rate,
     }
     print(json.dumps(line), flush=True)

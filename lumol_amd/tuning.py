@@ -106,6 +106,36 @@ def autotune(system, reps=5, use_cache=True, rounds=3):
     return dict(hit)
 
 
+def autotune_concurrent(systems, run, rounds=2):
+    """Picks and applies one set of knobs for systems that run at the same
+    time, each on its own stream: the knobs fastest for one system alone are
+    not those for several sharing the device (four SPC/E systems: wider rho
+    blocks).  `run()` advances every system by the same fixed work,
+    concurrently, and returns its wall time.  The knobs are chosen in turn
+    (pair lanes, then the k-space split and rho width), every candidate timed
+    in `rounds` interleaved passes and the fastest pass kept.  Returns the
+    knobs (dict)."""
+    keep = [tuple(getattr(s, "tuning_", (0, 0, 0, 0, 0, 0))[3:5]) for s in systems]
+    best = {"pair_lanes": 0, "kspace_split": 0, "rho_width": 0}
+
+    def apply(b):
+        for s, k in zip(systems, keep):
+            s.set_tuning(b["pair_lanes"], b["kspace_split"], b["rho_width"], *k)
+
+    fields = [("pair_lanes", PAIR_LANES)]
+    if bucket(systems[0])[0]:
+        fields += [("kspace_split", KSPACE_SPLITS), ("rho_width", RHO_WIDTHS)]
+    for field, values in fields:
+        times = {v: math.inf for v in values}
+        for _ in range(rounds):
+            for v in values:
+                apply(dict(best, **{field: v}))
+                times[v] = min(times[v], run())
+        best[field] = min(times, key=times.get)
+    apply(best)
+    return dict(best)
+
+
 def cached():
     """The choices made so far, by bucket."""
     with _lock:
