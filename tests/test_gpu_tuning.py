@@ -40,3 +40,43 @@ def test_autotune_choice_and_cache():
     t = spce(nmol=300, L=20.8, cutoff=8.0, kmax=7, seed=9)
     assert tuning.autotune(t) == best
     assert tuning.bucket(t) in tuning.cached()
+
+
+def test_autotune_concurrent_applies_one_choice():
+    # two water systems on their own streams: one set of knobs for both,
+    # chosen by timing them together; forces change only by summation order
+    import threading
+    import time
+
+    from lumol_amd import MolecularDynamics, Simulation, VelocityVerlet
+    systems = [spce(nmol=300, L=20.8, cutoff=8.0, kmax=7, seed=s) for s in (21, 22)]
+    for k, s in enumerate(systems):
+        s.stream_slot = 1 + k
+    f0 = [s.forces() for s in systems]
+    calls = []
+
+    def run():
+        def work(s):
+            Simulation(MolecularDynamics(VelocityVerlet(0.5))).run(s, 5)
+        ts = [threading.Thread(target=work, args=(s,)) for s in systems]
+        t0 = time.perf_counter()
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        calls.append(1)
+        return time.perf_counter() - t0
+
+    best = tuning.autotune_concurrent(systems, run, rounds=1)
+    assert len(calls) == len(tuning.PAIR_LANES) + len(tuning.KSPACE_SPLITS) + len(tuning.RHO_WIDTHS)
+    assert best["pair_lanes"] in tuning.PAIR_LANES and best["kspace_split"] in tuning.KSPACE_SPLITS
+    assert best["rho_width"] in tuning.RHO_WIDTHS
+    for s in systems:
+        t = s.tuning()
+        assert (t["pair_lanes"], t["kspace_split"], t["rho_width"]) == (
+            best["pair_lanes"], best["kspace_split"], best["rho_width"])
+    # fresh copies of the initial states with the chosen knobs
+    for seed, f in zip((21, 22), f0):
+        s = spce(nmol=300, L=20.8, cutoff=8.0, kmax=7, seed=seed)
+        s.set_tuning(pair_lanes=best["pair_lanes"], kspace_split=best["kspace_split"], rho_width=best["rho_width"])
+        assert frel(s.forces(), f) < 1e-12
