@@ -1708,8 +1708,10 @@ uint64_t lo_pairs_within(lo_system* h, double rc, uint32_t* out, uint64_t capaci
     return c;
 }
 
-// MD: integrator 0 = Velocity-Verlet, 1 = LeapFrog.  berendsen_T <= 0 disables
-// the thermostat.  The integrator starts from zero accelerations on the first
+// MD: integrator 0 = Velocity-Verlet, 1 = LeapFrog.  berendsen_T >= 0 applies
+// BerendsenThermostat(berendsen_T, tau) after each step (T = 0 is allowed, as
+// thermostats.rs:103 asserts temperature >= 0); a negative or NaN value
+// disables it.  The integrator starts from zero accelerations on the first
 // call after lo_md_reset (integrators.rs:40-42, :146-148).
 int lo_md_reset(lo_system* h) {
     System& s = S(h);
@@ -1724,7 +1726,7 @@ int lo_md_run(lo_system* h, int integrator, double dt, uint64_t nsteps, double b
             md_step_vv(s, dt);
         else
             md_step_leapfrog(s, dt);
-        if (berendsen_T > 0.0) berendsen(s, berendsen_T, tau);
+        if (berendsen_T >= 0.0) berendsen(s, berendsen_T, tau);
     }
     return 0;
 }

@@ -212,7 +212,8 @@ class OracleSystem:
     def md_reset(self):
         _chk(lib().lo_md_reset(self.h))
 
-    def md_run(self, integrator, dt, nsteps, berendsen_T=0.0, tau=1.0):
+    def md_run(self, integrator, dt, nsteps, berendsen_T=-1.0, tau=1.0):
+        """berendsen_T >= 0 applies the Berendsen thermostat after each step."""
         _chk(lib().lo_md_run(self.h, integrator, dt, nsteps, berendsen_T, tau))
 
     def cache_init(self):
