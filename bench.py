@@ -554,23 +554,13 @@ def bench_serve(args):
 
     from lumol_amd._native import LG_PRECISION_FP64, LG_PRECISION_MIXED
     from lumol_amd.serving import Job, Server, latency_stats
-    from lumol_amd.workloads import lj_fluid, lj_mc_chain, water_box
+    from lumol_amd.workloads import lj_mc_chain, serving_stream
 
     rng = np.random.default_rng(5)
     precision = LG_PRECISION_MIXED if args.form == "mixed" else LG_PRECISION_FP64
-    jobs = []
-    for k in range(args.md_jobs):
-        steps = max(1, int(rng.integers(1000, 20001) * args.steps_scale))
-        if rng.random() < 0.5:
-            n = int(rng.integers(64, 8193))
-            system = lj_fluid(n, seed=100 + k)
-            jobs.append(Job("md", (lambda s=system: s), steps, md={"dt": 1.0, "precision": precision},
-                            name=f"lj-{n}"))
-        else:
-            nmol = int(rng.integers(216, 12001))
-            system = water_box(nmol, seed=100 + k)
-            jobs.append(Job("md", (lambda s=system: s), steps, md={"dt": 0.5, "precision": precision},
-                            name=f"water-{nmol}"))
+    jobs = [Job("md", (lambda s=system: s), steps, md=opts, name=name)
+            for name, system, steps, opts in serving_stream(args.md_jobs, args.steps_scale, precision=precision,
+                                                            rng=rng)]
     for k, n in enumerate(mc_chain_sizes(args.chains)):
         system, beta, delta = lj_mc_chain(int(n), seed=20000 + k)
         jobs.append(Job("mc", (lambda s=system: s), int(n) * args.sweeps,

@@ -152,7 +152,10 @@ int lg_dihedral_set(lg_system* sys, uint32_t kind_i, uint32_t kind_j, uint32_t k
  * (ewald.rs:279).  `restriction` as CoulombicPotential::set_restriction
  * (global/mod.rs:197-202); Scale14 is rejected as ewald.rs:388 asserts. */
 int lg_coulomb_ewald(lg_system* sys, double cutoff, int kmax, double alpha, int restriction);
-/* Wolf::new(cutoff) (wolf.rs:68-84). */
+/* Wolf::new(cutoff) (wolf.rs:68-84).
+ * Both return LG_E_INVALID when the cutoff is above half the smallest
+ * length of a periodic cell (system.rs:157-167 panics there); evaluations
+ * re-check it, since the cell may change after the potential is set. */
 int lg_coulomb_wolf(lg_system* sys, double cutoff, int restriction, double scale14);
 int lg_coulomb_clear(lg_system* sys);
 
@@ -185,11 +188,18 @@ int lg_ewald_kvector_count(lg_system* sys, int* out);
  * molecular_dynamics.rs:11-83, simulations.rs:69-97).  `integrator` is
  * VelocityVerlet (integrators.rs:22-70) or LeapFrog (:128-169); the first
  * step starts from zero accelerations as both setups do (:40-42, :146-148).
- * berendsen_temperature > 0 enables BerendsenThermostat::new(T, tau)
- * (thermostats.rs:91-121), applied after each step. */
+ * berendsen_temperature >= 0 enables BerendsenThermostat::new(T, tau)
+ * (thermostats.rs:91-121; T = 0 is valid there, :103), applied after each
+ * step; a negative or NaN temperature means no thermostat. */
 int lg_md_setup(lg_system* sys, int integrator, double timestep, double berendsen_temperature,
                 double berendsen_tau);
-/* Runs `nsteps` steps on the device (positions/velocities stay in HBM). */
+/* Runs `nsteps` steps on the device (positions/velocities stay in HBM).
+ * Returns LG_E_NONFINITE when a position, velocity or acceleration became
+ * non-finite (checked on the device every step, reported after each chunk of
+ * at most 200 steps; the state of the end of that chunk stays on the
+ * device).  The reference's Simulation::run only warns, every 10 000 steps,
+ * about coordinates above 1e6 A or velocities above 1000 A/fs
+ * (lumol-sim/src/simulations.rs:94-96, 129-155). */
 int lg_md_run(lg_system* sys, uint64_t nsteps);
 
 /* ---- Monte Carlo energy cache ---------------------------------------------------------

@@ -7,6 +7,7 @@ opened, every call raises.  The library is built in-tree by
 import ctypes
 import os
 import threading
+import warnings
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblumol_gpu.so")
@@ -134,6 +135,15 @@ def _declare(lib):
         fn.argtypes = args
 
 
+def _hip_loaded():
+    """True when the HIP runtime library is already mapped in this process."""
+    try:
+        with open("/proc/self/maps") as f:
+            return "libamdhip64" in f.read()
+    except OSError:
+        return False
+
+
 def lib():
     """The loaded library (raises if it has not been built)."""
     global _lib
@@ -150,12 +160,18 @@ def lib():
             # setting is raised to 8 (4 SPC/E systems at once: 2.15x -> 2.65x
             # the FP64 single-system rate); it takes effect if the HIP runtime
             # is not initialised yet.
-            try:
-                queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
-            except ValueError:
-                queues = 0
-            if queues < 8:
-                os.environ["GPU_MAX_HW_QUEUES"] = "8"
+            # LUMOL_GPU_KEEP_HW_QUEUES=1 keeps the caller's setting as it is.
+            if os.environ.get("LUMOL_GPU_KEEP_HW_QUEUES", "0") != "1":
+                try:
+                    queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
+                except ValueError:
+                    queues = 0
+                if queues < 8:
+                    if _hip_loaded():
+                        warnings.warn("the HIP runtime was loaded before the lumol_gpu library: "
+                                      "GPU_MAX_HW_QUEUES=8 may not take effect, and concurrent "
+                                      "contexts may share hardware queues", RuntimeWarning, stacklevel=3)
+                    os.environ["GPU_MAX_HW_QUEUES"] = "8"
             handle = ctypes.CDLL(LIB_PATH)
             _declare(handle)
             _lib = handle

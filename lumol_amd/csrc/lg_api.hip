@@ -211,6 +211,7 @@ struct lg_system {
     // MD
     int integrator = LG_INTEGRATOR_VELOCITY_VERLET;
     double dt = 1.0, berendsen_T = 0.0, berendsen_tau = 1.0;
+    bool thermostat = false;  // BerendsenThermostat(berendsen_T, berendsen_tau) after each step
     bool md_ready = false;
     // Monte Carlo energy cache (cache.rs:22-48): cached terms, rho(k) and
     // the staged move; cell list of the device chain
@@ -855,6 +856,9 @@ int prepare(lg_system* s) {
         for (auto& [k, p] : s->pairs)
             if (p.rc > half)
                 return fail(LG_E_INVALID, "The cutoff for this pair potential is bigger than half the cell length");
+        // the cell may have changed after the coulomb potential was set
+        if (s->coul_kind != COUL_NONE && s->coul_rc > half)
+            return fail(LG_E_INVALID, "The cutoff for the coulomb potential is bigger than half the cell length");
     }
     if (s->nlist_dirty) LG_TRY(prepare_nlist(s));
     return LG_OK;
@@ -1257,11 +1261,15 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     return LG_OK;
 }
 
-int check_overflow(lg_system* s, bool* overflowed) {
-    int ov = 0;
-    LG_HIP(hipMemcpyAsync(&ov, s->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, S(s)));
+// Reads the list-overflow flag (flags[1]) and, when `nonfinite` is given, the
+// non-finite-state flag of the integrator (flags[2]).
+int check_overflow(lg_system* s, bool* overflowed, bool* nonfinite = nullptr) {
+    int fl[2] = {0, 0};
+    LG_HIP(hipMemcpyAsync(fl, s->flags.p + 1, sizeof(fl), hipMemcpyDeviceToHost, S(s)));
     LG_HIP(hipStreamSynchronize(S(s)));
+    const int ov = fl[0];
     *overflowed = ov > 0;
+    if (nonfinite) *nonfinite = fl[1] != 0;
     if (ov > 0) {
         s->cap = int(std::min<size_t>(s->n, size_t(ov) + size_t(ov) / 4 + 32));
         LG_TRY(s->nbr.ensure(size_t(s->cap) * s->n));
@@ -1880,10 +1888,19 @@ int lg_dihedral_set(lg_system* s, uint32_t ki, uint32_t kj, uint32_t kk, uint32_
     return LG_OK;
 }
 
+// System::set_coulomb_potential (system.rs:157-167): the cutoff must not
+// exceed half the smallest cell length of a periodic cell.
+static int check_coulomb_cutoff(const lg_system* s, double cutoff) {
+    if (s->periodic && cutoff > 0.5 * std::min(s->L[0], std::min(s->L[1], s->L[2])))
+        return fail(LG_E_INVALID, "The cutoff for the coulomb potential is bigger than half the cell length");
+    return LG_OK;
+}
+
 int lg_coulomb_ewald(lg_system* s, double cutoff, int kmax, double alpha, int restriction) {
     if (!s) return fail(LG_E_INVALID, "null system");
     if (std::isnan(alpha)) alpha = kPi / cutoff;  // ewald.rs:279
     if (!(cutoff >= 0.0)) return fail(LG_E_INVALID, "the cutoff can not be negative in Ewald");
+    LG_TRY(check_coulomb_cutoff(s, cutoff));
     if (alpha < 0.0) return fail(LG_E_INVALID, "alpha can not be negative in Ewald");
     if (kmax <= 0) return fail(LG_E_INVALID, "kmax can not be 0 in Ewald");
     if (restriction == LG_RESTRICT_SCALE14)
@@ -1902,6 +1919,7 @@ int lg_coulomb_ewald(lg_system* s, double cutoff, int kmax, double alpha, int re
 int lg_coulomb_wolf(lg_system* s, double cutoff, int restriction, double scale14) {
     if (!s) return fail(LG_E_INVALID, "null system");
     if (!(cutoff > 0.0)) return fail(LG_E_INVALID, "Got a negative cutoff in Wolf summation");
+    LG_TRY(check_coulomb_cutoff(s, cutoff));
     if (restriction < 0 || restriction > 6) return fail(LG_E_INVALID, "unknown restriction");
     s->coul_kind = COUL_WOLF;
     s->coul_rc = cutoff;
@@ -2038,12 +2056,17 @@ int lg_md_setup(lg_system* s, int integrator, double timestep, double bT, double
     if (integrator != LG_INTEGRATOR_VELOCITY_VERLET && integrator != LG_INTEGRATOR_LEAP_FROG)
         return fail(LG_E_UNSUPPORTED, "unknown integrator");
     if (!(timestep > 0.0)) return fail(LG_E_INVALID, "timestep must be positive");
-    if (bT > 0.0 && !(btau >= 1.0))
+    // thermostats.rs:103-104: temperature >= 0 (0 included) and tau >= 1;
+    // a negative or NaN temperature means no thermostat
+    const bool thermo = bT >= 0.0;
+    if (thermo && !std::isfinite(bT)) return fail(LG_E_INVALID, "The temperature must be finite in thermostats.");
+    if (thermo && !(btau >= 1.0))
         return fail(LG_E_INVALID, "The timestep must be larger than 1 in berendsen thermostat.");
     s->integrator = integrator;
     s->dt = timestep;
-    s->berendsen_T = bT > 0.0 ? bT : 0.0;
-    s->berendsen_tau = btau;
+    s->thermostat = thermo;
+    s->berendsen_T = thermo ? bT : 0.0;
+    s->berendsen_tau = thermo ? btau : 1.0;
     LG_HIP(hipSetDevice(s->ctx->device));
     // Integrator::setup: accelerations start at zero (integrators.rs:40-42)
     LG_HIP(hipMemsetAsync(s->acc.p, 0, s->n * sizeof(double4), S(s)));
@@ -2058,7 +2081,7 @@ int lg_md_setup(lg_system* s, int integrator, double timestep, double bT, double
 static int md_chunk(lg_system* s, uint64_t todo) {
     hipStream_t st = S(s);
     const int n = int(s->n);
-    const bool thermo = s->berendsen_T > 0.0;
+    const bool thermo = s->thermostat;
     const bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
     double hs = 0.5 * s->skin;
     int nb = blocks_for(n, 64);  // second half: one wave per block, one KE partial each
@@ -2113,10 +2136,10 @@ static int md_chunk(lg_system* s, uint64_t todo) {
             double* kp = thermo ? s->ke_part.p : nullptr;
             if (!lf)
                 k_md_second<false><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
-                                                       s->vel.p, s->acc.p, n, s->dt, kp);
+                                                       s->vel.p, s->acc.p, n, s->dt, kp, s->flags.p);
             else
                 k_md_second<true><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
-                                                      s->vel.p, s->acc.p, n, s->dt, kp);
+                                                      s->vel.p, s->acc.p, n, s->dt, kp, s->flags.p);
         }
         prof_stop(s, p2);
         first_pending = !fuse;
@@ -2151,6 +2174,7 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
     LG_TRY(s->snap_acc.ensure(s->n));
     uint64_t done = 0;
     int retries = 0;
+    LG_HIP(hipMemsetAsync(s->flags.p + 2, 0, sizeof(int), st));
     while (done < nsteps) {
         uint64_t todo = std::min<uint64_t>(chunk, nsteps - done);
         LG_HIP(hipMemcpyAsync(s->snap_pos.p, s->pos.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
@@ -2161,7 +2185,8 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
             // captured once per (chunk length, configuration, buffer set,
             // integrator settings) and replayed
             const uint64_t key[8] = {todo, s->cfg_epoch, g_alloc_epoch.load(), uint64_t(s->integrator),
-                                     uint64_t(s->precision), bits_of(s->dt), bits_of(s->berendsen_T),
+                                     uint64_t(s->precision), bits_of(s->dt),
+                                     s->thermostat ? bits_of(s->berendsen_T) : ~uint64_t(0),
                                      bits_of(s->berendsen_tau)};
             if (!s->md_exec || !std::equal(key, key + 8, s->md_key)) {
                 if (s->md_exec) (void)hipGraphExecDestroy(s->md_exec);
@@ -2188,15 +2213,21 @@ int lg_md_run(lg_system* s, uint64_t nsteps) {
             LG_TRY(md_chunk(s, todo));
         }
         LG_HIP(hipGetLastError());
-        bool ov = false;
-        LG_TRY(check_overflow(s, &ov));
+        bool ov = false, bad = false;
+        LG_TRY(check_overflow(s, &ov, &bad));
         if (ov) {
             if (++retries > 4) return fail(LG_E_DEVICE, "neighbour list overflow could not be resolved");
             LG_HIP(hipMemcpyAsync(s->pos.p, s->snap_pos.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
             LG_HIP(hipMemcpyAsync(s->vel.p, s->snap_vel.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
             LG_HIP(hipMemcpyAsync(s->acc.p, s->snap_acc.p, s->n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+            LG_HIP(hipMemsetAsync(s->flags.p + 2, 0, sizeof(int), st));
             continue;
         }
+        // a stronger form of the reference's periodic sanity_check
+        // (simulations.rs:94-96, 129-155), which only warns: a run whose
+        // positions or velocities are no longer finite fails, with the
+        // state of the end of the chunk left on the device
+        if (bad) return fail(LG_E_NONFINITE, "positions or velocities became non-finite during the MD run");
         done += todo;
     }
     LG_HIP(hipStreamSynchronize(st));

@@ -183,9 +183,11 @@ namespace lg {
 constexpr int kErfcxPieces = 256;
 constexpr int kErfcxDeg64 = 5;
 constexpr int kErfcxDeg32 = 3;
-__device__ __forceinline__ double erfcx_tab(const double* __restrict__ c, double x, double inv_h) {
+// The piece index is clamped in floating point before the conversion (NaN
+// lands in piece 0), so that a non-finite distance reads inside the table.
+__host__ __device__ __forceinline__ double erfcx_tab(const double* __restrict__ c, double x, double inv_h) {
     const double u = x * inv_h;
-    const int k = min(max((int)u, 0), kErfcxPieces - 1);
+    const int k = (int)fmin(fmax(u, 0.0), (double)(kErfcxPieces - 1));
     const double t = 2.0 * (u - (double)k) - 1.0;
     const double* q = c + k * (kErfcxDeg64 + 1);
     double p = q[kErfcxDeg64];
@@ -193,9 +195,9 @@ __device__ __forceinline__ double erfcx_tab(const double* __restrict__ c, double
     for (int d = kErfcxDeg64 - 1; d >= 0; d--) p = p * t + q[d];
     return p;
 }
-__device__ __forceinline__ float erfcx_tab(const float* __restrict__ c, float x, float inv_h) {
+__host__ __device__ __forceinline__ float erfcx_tab(const float* __restrict__ c, float x, float inv_h) {
     const float u = x * inv_h;
-    const int k = min(max((int)u, 0), kErfcxPieces - 1);
+    const int k = (int)fminf(fmaxf(u, 0.f), (float)(kErfcxPieces - 1));
     const float t = 2.0f * (u - (float)k) - 1.0f;
     const float* q = c + k * (kErfcxDeg32 + 1);
     float p = q[kErfcxDeg32];

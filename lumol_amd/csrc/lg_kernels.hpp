@@ -52,15 +52,23 @@ __host__ __device__ __forceinline__ int nbr_kind(int e) { return e >> 24; }
 // relative margin of the cell edges over rlist (and the 1e-4 margin of the
 // FP32 candidate test), and the binning runs at FP64 multiply rate (the
 // single-workgroup cell list of a small system bins every atom on one CU).
-__device__ __forceinline__ double wrap_coord(double x, double L, double invL, int periodic) {
+__host__ __device__ __forceinline__ double wrap_coord(double x, double L, double invL, int periodic) {
     return periodic ? x - floor(x * invL) * L : x;
 }
 
-__device__ __forceinline__ int cell_coord(double x, double L, double invL, int nc, double inv_csize, int periodic) {
+// The cell index is clamped in floating point before the conversion, so that
+// any input -- NaN, infinities, coordinates far outside the box -- gives an
+// index in [0, nc) (NaN compares false and lands in cell 0): a simulation that
+// blows up keeps every list kernel in bounds.
+__host__ __device__ __forceinline__ int clamp_cell(double f, int nc) {
+    return f >= 1.0 ? (f < (double)nc ? (int)f : nc - 1) : 0;
+}
+
+__host__ __device__ __forceinline__ int cell_coord(double x, double L, double invL, int nc, double inv_csize,
+                                                   int periodic) {
     if (nc == 1) return 0;
     double w = wrap_coord(x, L, invL, periodic);
-    int c = (int)floor(w * inv_csize);
-    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+    return clamp_cell(floor(w * inv_csize), nc);
 }
 
 // Cell coordinates of every atom for the direct Verlet build, written by the
@@ -72,7 +80,7 @@ struct CellTag {
     Grid g;
     Box b;
 };
-__device__ __forceinline__ int cell_tag(const CellTag& t, double4 p) {
+__host__ __device__ __forceinline__ int cell_tag(const CellTag& t, double4 p) {
     const int cx = cell_coord(p.x, t.b.L[0], t.b.inv[0], t.g.n[0], t.g.inv_size[0], t.b.periodic);
     const int cy = cell_coord(p.y, t.b.L[1], t.b.inv[1], t.g.n[1], t.g.inv_size[1], t.b.periodic);
     const int cz = cell_coord(p.z, t.b.L[2], t.b.inv[2], t.g.n[2], t.g.inv_size[2], t.b.periodic);
@@ -2352,6 +2360,16 @@ __device__ __forceinline__ bool moved_beyond(double4 p, const double4* __restric
     return dx * dx + dy * dy + dz * dz > half_skin2;
 }
 
+// Non-finite state of an atom after an integration step: flags[2] is set
+// (wave-level OR, one vector atomic per wave) so that the host can fail the
+// run instead of returning NaN positions as a result.
+__device__ __forceinline__ bool nonfinite3(double x, double y, double z) {
+    return !isfinite(x) || !isfinite(y) || !isfinite(z);
+}
+__device__ __forceinline__ void flag_nonfinite(bool bad, int* __restrict__ flags) {
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flags + 2, 1);
+}
+
 // Berendsen factor (thermostats.rs:113-121) from the kinetic energy
 // partials of the previous step, dof = 3N (system.rs:249-255): the block sums
 // the partials in a fixed order (thread-strided, waves, then wave order), so
@@ -2427,7 +2445,7 @@ __global__ __launch_bounds__(512) void k_vv_first(double4* __restrict__ pos, dou
     const int i = blockIdx.x * 64 + lane;
     double l = th.ke_part ? berendsen_lambda(th) : 1.0;
     if (role == 0) {
-        bool moved = false;
+        bool moved = false, bad = false;
         double4 p = make_double4(0, 0, 0, 0);
         if (i < n) {
             double4 v = vel[i];
@@ -2448,9 +2466,11 @@ __global__ __launch_bounds__(512) void k_vv_first(double4* __restrict__ pos, dou
             vel[i] = v;
             pos[i] = p;
             moved = moved_beyond(p, ref, i, half_skin2);
+            bad = nonfinite3(p.x, p.y, p.z) || nonfinite3(v.x, v.y, v.z);
             if (tg.tag) tg.tag[i] = cell_tag(tg, p);
         }
         if (__any(moved) && lane == 0) atomicOr(flags, 1);
+        flag_nonfinite(bad, flags);
         px[0][lane] = p.x;
         px[1][lane] = p.y;
         px[2][lane] = p.z;
@@ -2473,7 +2493,7 @@ __global__ __launch_bounds__(512) void k_lf_first(double4* __restrict__ pos, dou
     const int i = blockIdx.x * 64 + lane;
     double l = th.ke_part ? berendsen_lambda(th) : 1.0;
     if (role == 0) {
-        bool moved = false;
+        bool moved = false, bad = false;
         double4 p = make_double4(0, 0, 0, 0);
         if (i < n) {
             double4 v = vel[i];
@@ -2491,9 +2511,11 @@ __global__ __launch_bounds__(512) void k_lf_first(double4* __restrict__ pos, dou
             p.z += v.z * dt + (0.5 * a.z) * dt2;
             pos[i] = p;
             moved = moved_beyond(p, ref, i, half_skin2);
+            bad = nonfinite3(p.x, p.y, p.z) || nonfinite3(v.x, v.y, v.z);
             if (tg.tag) tg.tag[i] = cell_tag(tg, p);
         }
         if (__any(moved) && lane == 0) atomicOr(flags, 1);
+        flag_nonfinite(bad, flags);
         px[0][lane] = p.x;
         px[1][lane] = p.y;
         px[2][lane] = p.z;
@@ -2516,9 +2538,11 @@ __global__ __launch_bounds__(64) void k_md_second(const double4* __restrict__ fo
                                                    const int* __restrict__ slot_start,
                                                    const double4* __restrict__ fk, int ksplit,
                                                    double4* __restrict__ vel, double4* __restrict__ acc, int n,
-                                                   double dt, double* __restrict__ ke_part) {
+                                                   double dt, double* __restrict__ ke_part,
+                                                   int* __restrict__ flags) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ke = 0.0;
+    bool bad = false;
     if (i < n) {
         double4 f = total_force(i, n, force, contrib, slot_start, fk, ksplit, pos);
         double4 v = vel[i];
@@ -2538,7 +2562,9 @@ __global__ __launch_bounds__(64) void k_md_second(const double4* __restrict__ fo
         acc[i] = na;
         vel[i] = v;
         ke = 0.5 * m * (v.x * v.x + v.y * v.y + v.z * v.z);
+        bad = nonfinite3(v.x, v.y, v.z) || nonfinite3(na.x, na.y, na.z);
     }
+    flag_nonfinite(bad, flags);
     if (!ke_part) return;
     double s = wave_sum(ke);
     if (threadIdx.x == 0) ke_part[blockIdx.x] = s;
@@ -2563,7 +2589,7 @@ __global__ __launch_bounds__(512) void k_md_fused(const double4* __restrict__ fo
     const int lane = threadIdx.x & 63, role = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
     if (role == 0) {
-        bool moved = false;
+        bool moved = false, bad = false;
         double4 p = make_double4(0, 0, 0, 0);
         if (i < n) {
             // second half of step k
@@ -2602,9 +2628,11 @@ __global__ __launch_bounds__(512) void k_md_fused(const double4* __restrict__ fo
             vel[i] = v;
             pos[i] = p;
             moved = moved_beyond(p, ref, i, half_skin2);
+            bad = nonfinite3(p.x, p.y, p.z) || nonfinite3(v.x, v.y, v.z);
             if (tg.tag) tg.tag[i] = cell_tag(tg, p);
         }
         if (__any(moved) && lane == 0) atomicOr(flags, 1);
+        flag_nonfinite(bad, flags);
         px[0][lane] = p.x;
         px[1][lane] = p.y;
         px[2][lane] = p.z;

@@ -178,10 +178,9 @@ __device__ __forceinline__ void mc_divmod34(int v, int e, int& q, int& r) {
 // into [0, L) and scaled by the inverse cell edge.  Only self-consistency
 // matters (binning and lookups use this one function); the cells are
 // larger than the cutoff by a relative 1e-12, far above the rounding.
-__device__ __forceinline__ int mc_axis_cell(double x, double L, double invL, double inv_cs, int nc) {
+__host__ __device__ __forceinline__ int mc_axis_cell(double x, double L, double invL, double inv_cs, int nc) {
     double w = x - floor(x * invL) * L;
-    int k = (int)floor(w * inv_cs);
-    return k < 0 ? 0 : (k >= nc ? nc - 1 : k);
+    return clamp_cell(floor(w * inv_cs), nc);
 }
 
 __device__ __forceinline__ int3 mc_cell3(const McChain& c, double4 p) {

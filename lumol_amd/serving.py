@@ -102,7 +102,10 @@ def device_bytes(system, kind):
 
 
 def device_free_bytes(device=0):
-    """Free and total device memory (hipMemGetInfo)."""
+    """Free and total device memory (hipMemGetInfo).  The library is loaded
+    first, so that its loader sets the hardware-queue count before this call
+    initialises the HIP runtime."""
+    nat.lib()
     hip = ctypes.CDLL("libamdhip64.so")
     free, total = ctypes.c_size_t(), ctypes.c_size_t()
     hip.hipSetDevice(device)
@@ -192,7 +195,33 @@ class Server:
     def _fail(job, err):
         job.error = err
         job.finished = time.perf_counter()
+        Server._close_system(job)
         job._done.set()
+
+    @staticmethod
+    def _close_system(job):
+        s, job._system = job._system, None
+        if s is not None:
+            try:
+                s.close()
+            except Exception:  # noqa: BLE001 - the job already carries its error
+                pass
+
+    def _head(self, queue, kind):
+        """Caller holds self._cv.  The job at the head of `queue` with its
+        host-side System built and its footprint estimated; a job whose
+        build() or estimate raises is removed and failed (returns None)."""
+        cand = queue[0]
+        if cand._system is None:
+            try:
+                cand._system = cand.build()
+                cand._bytes = device_bytes(cand._system, kind)
+            except Exception as err:  # noqa: BLE001 - reported on the job
+                queue.popleft()
+                cand._bytes = 0
+                self._fail(cand, err)
+                return None
+        return cand
 
     # -- MD lanes -------------------------------------------------------------------------
     def _md_loop(self, lane):
@@ -202,10 +231,9 @@ class Server:
                 job = None
                 while job is None:
                     if self._md_queue:
-                        cand = self._md_queue[0]
-                        if cand._system is None:
-                            cand._system = cand.build()
-                            cand._bytes = device_bytes(cand._system, "md")
+                        cand = self._head(self._md_queue, "md")
+                        if cand is None:
+                            continue
                         if self._reserve(cand._bytes):
                             job = self._md_queue.popleft()
                             break
@@ -232,10 +260,11 @@ class Server:
         prop = MolecularDynamics(integ)
         if md.get("thermostat"):
             prop.set_thermostat(BerendsenThermostat(*md["thermostat"]))
-        Simulation(prop).run(s, job.steps)
-        job.result = {"positions": s.positions.copy(), "velocities": s.velocities.copy()}
-        s.close()
-        job._system = None
+        try:
+            Simulation(prop).run(s, job.steps)
+            job.result = {"positions": s.positions.copy(), "velocities": s.velocities.copy()}
+        finally:
+            self._close_system(job)
         job.finished = time.perf_counter()
         job._done.set()
 
@@ -246,10 +275,9 @@ class Server:
             with self._cv:
                 while True:
                     while self._mc_queue and len(active) < self.mc_max_chains:
-                        cand = self._mc_queue[0]
-                        if cand._system is None:
-                            cand._system = cand.build()
-                            cand._bytes = device_bytes(cand._system, "mc")
+                        cand = self._head(self._mc_queue, "mc")
+                        if cand is None:
+                            continue
                         if not self._reserve(cand._bytes):
                             break
                         active.append(self._mc_queue.popleft())
@@ -281,17 +309,11 @@ class Server:
             if not active:
                 continue
             n_att = [min(self.mc_slice, j.steps - j._done_attempts) for j in active]
-            try:
-                acc = mc_run_batch([j._system for j in active], [j.mc.get("seed", 0) for j in active],
-                                   [j._done_attempts for j in active], n_att, [j.mc["beta"] for j in active],
-                                   [j.mc["delta"] for j in active])
-            except Exception as err:  # noqa: BLE001
-                for j in active:
-                    self._fail(j, err)
-                    self._release(j._bytes)
-                active = []
-                continue
-            self.mc_launches += 1
+            acc = self._mc_launch(active, n_att)
+            ok = [a is not None for a in acc]
+            active = [j for j, k in zip(active, ok) if k]
+            n_att = [na for na, k in zip(n_att, ok) if k]
+            acc = [a for a in acc if a is not None]
             still = []
             for j, a, na in zip(active, acc, n_att):
                 j._accepted += int(a)
@@ -308,6 +330,34 @@ class Server:
                 else:
                     still.append(j)
             active = still
+
+
+    def _mc_launch(self, active, n_att):
+        """One batched launch of the chains `active`; the accepted counts, or
+        None for a chain that failed.  A failed batch does not fail every
+        chain: the library validates every chain before it launches, so a
+        failure leaves the chains where they were, and each chain is then
+        run on its own so that only the offending ones fail."""
+        def run(jobs, natt):
+            return mc_run_batch([j._system for j in jobs], [j.mc.get("seed", 0) for j in jobs],
+                                [j._done_attempts for j in jobs], natt, [j.mc["beta"] for j in jobs],
+                                [j.mc["delta"] for j in jobs])
+        try:
+            out = [int(a) for a in run(active, n_att)]
+            self.mc_launches += 1
+            return out
+        except Exception:  # noqa: BLE001 - narrowed down below
+            pass
+        out = []
+        for j, na in zip(active, n_att):
+            try:
+                out.append(int(run([j], [na])[0]))
+                self.mc_launches += 1
+            except Exception as err:  # noqa: BLE001 - reported on the job
+                self._fail(j, err)
+                self._release(j._bytes)
+                out.append(None)
+        return out
 
 
 Job._system = None

@@ -340,6 +340,10 @@ class System:
         self._dirty = True
 
     def set_coulomb_potential(self, potential):
+        # system.rs:157-167
+        if (potential is not None and not self.cell.is_infinite()
+                and potential.cutoff > min(self.cell.lengths) / 2):
+            raise ValueError("The cutoff for the coulomb potential is bigger than half the cell length")
         self.coulomb = potential
         self._dirty = True
 
@@ -741,7 +745,7 @@ class Simulation:
             system.cache_init()
             self._setup_for = (id(system), h.value)
             return
-        T, tau = (md.thermostat.temperature, md.thermostat.tau) if md.thermostat else (0.0, 1.0)
+        T, tau = (md.thermostat.temperature, md.thermostat.tau) if md.thermostat else (-1.0, 1.0)
         nat.check(nat.lib().lg_md_setup(h, md.integrator.code, md.integrator.timestep, T, tau))
         self._setup_for = (id(system), h.value)
 

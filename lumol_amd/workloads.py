@@ -238,3 +238,34 @@ def mc_translate(temperature, delta, seed=0):
     mc = MonteCarlo(temperature, seed=seed)
     mc.add(Translate(delta))
     return Simulation(mc)
+
+
+def serving_stream(md_jobs=256, steps_scale=0.02, seed=5, precision=None, fp64_every=0, rng=None):
+    """The MD requests of the mixed serving stream: `md_jobs` jobs, each LJ
+    argon of U{64..8192} atoms (dt 1 fs) or SPC/E water of U{216..12000}
+    molecules (dt 0.5 fs) with equal odds, running U{1000..20000} steps
+    scaled by `steps_scale`.  Returns (name, system, steps, md options)
+    tuples in submission order; every `fp64_every`-th job (0: none) runs the
+    FP64 form, the others `precision`.  Draws from `rng` when given (a
+    caller that goes on drawing from the same stream), else from a
+    generator seeded with `seed`."""
+    rng = np.random.default_rng(seed) if rng is None else rng
+    out = []
+    for k in range(md_jobs):
+        steps = max(1, int(rng.integers(1000, 20001) * steps_scale))
+        form = precision
+        if fp64_every and k % fp64_every == 0:
+            form = 0
+        if rng.random() < 0.5:
+            n = int(rng.integers(64, 8193))
+            opts = {"dt": 1.0}
+            if form is not None:
+                opts["precision"] = form
+            out.append((f"lj-{n}", lj_fluid(n, seed=100 + k), steps, opts))
+        else:
+            nmol = int(rng.integers(216, 12001))
+            opts = {"dt": 0.5}
+            if form is not None:
+                opts["precision"] = form
+            out.append((f"water-{nmol}", water_box(nmol, seed=100 + k), steps, opts))
+    return out

@@ -112,3 +112,71 @@ def random_molecular_system(n_mol=24, L=14.0, seed=7, coulomb="ewald", restricti
         w.set_restriction(restriction)
         s.set_coulomb_potential(w)
     return s
+
+
+def butane_system(n_mol=27, L=22.0, seed=11, restriction=PairRestriction.NONE, coulomb="wolf",
+                  dihedral=True):
+    """United-atom butane-like chains CA-CB-CB-CA (bonds 1.54 A, angles near
+    114 deg, random torsions) at random orientations on a jittered grid:
+    every bond path (1-2, 1-3, 1-4) occurs within a molecule, so each
+    PairRestriction (restrictions.rs:85-116) excludes or scales a different
+    set of pairs.  Harmonic bonds, angles and dihedrals; LJ on every kind
+    pair and Wolf (or no Coulomb) under the same restriction."""
+    rng = np.random.default_rng(seed)
+    names, pos, bonds, charges = [], [], [], []
+    grid = int(np.ceil(n_mol ** (1 / 3)))
+    sp = L / grid
+    k = 0
+    for a in range(grid):
+        for b in range(grid):
+            for c in range(grid):
+                if k >= n_mol:
+                    break
+                centre = (np.array([a, b, c]) + 0.5) * sp + rng.uniform(-0.4, 0.4, 3)
+                theta = np.radians(rng.uniform(105, 120))
+                phi = rng.uniform(-np.pi, np.pi)
+                # internal coordinates -> cartesian: first three in a plane,
+                # the fourth rotated by the torsion phi about the middle bond
+                p0 = np.array([0.0, 0.0, 0.0])
+                p1 = np.array([1.54, 0.0, 0.0])
+                p2 = p1 + 1.54 * np.array([-np.cos(theta), np.sin(theta), 0.0])
+                bvec = (p2 - p1) / np.linalg.norm(p2 - p1)
+                u = (p1 - p0) - np.dot(p1 - p0, bvec) * bvec
+                u /= np.linalg.norm(u)
+                w = np.cross(bvec, u)
+                d = np.cos(np.pi - theta) * bvec + np.sin(np.pi - theta) * (np.cos(phi) * u + np.sin(phi) * w)
+                p3 = p2 + 1.54 * d
+                loc = np.array([p0, p1, p2, p3])
+                loc -= loc.mean(0)
+                q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+                base = len(names)
+                names += ["CA", "CB", "CB", "CA"]
+                pos += list(loc @ q.T + centre)
+                charges += [-0.25, 0.25, 0.25, -0.25]
+                bonds += [(base, base + 1), (base + 1, base + 2), (base + 2, base + 3)]
+                k += 1
+    s = System.from_arrays(names, np.array(pos), UnitCell.cubic(L), charges=np.array(charges),
+                           masses=[15.035 if x == "CA" else 14.027 for x in names])
+    for i, j in bonds:
+        s.add_bond(i, j)
+    rc = min(7.5, L / 2)
+    for pair, (sig, eps) in {("CA", "CA"): (3.75, 0.815), ("CA", "CB"): (3.85, 0.555),
+                             ("CB", "CB"): (3.95, 0.382)}.items():
+        lj = PairInteraction(LennardJones(sig, units.from_(eps, "kJ/mol")), rc)
+        lj.set_restriction(restriction)
+        s.set_pair_potential(pair, lj)
+    s.set_bond_potential(("CA", "CB"), Harmonic(k=units.from_(2000.0, "kJ/mol/A^2"), x0=1.54))
+    s.set_bond_potential(("CB", "CB"), Harmonic(k=units.from_(2000.0, "kJ/mol/A^2"), x0=1.54))
+    s.set_angle_potential(("CA", "CB", "CB"), Harmonic(k=units.from_(500.0, "kJ/mol"), x0=np.radians(114.0)))
+    if dihedral:
+        s.set_dihedral_potential(("CA", "CB", "CB", "CA"),
+                                 Harmonic(k=units.from_(40.0, "kJ/mol"), x0=np.radians(180.0)))
+    if coulomb == "wolf":
+        w = Wolf(rc)
+        w.set_restriction(restriction)
+        s.set_coulomb_potential(w)
+    elif coulomb == "ewald":
+        ew = Ewald(rc, 6)
+        ew.set_restriction(restriction)
+        s.set_coulomb_potential(ew)
+    return s

@@ -67,3 +67,10 @@ def test_host_mirror_validation():
         BerendsenThermostat(300.0, 0.5)
     with pytest.raises(ValueError):
         UnitCell.cubic(-1.0)
+    # system.rs:157-167: the coulomb cutoff must fit half the cell too
+    with pytest.raises(ValueError):
+        s.set_coulomb_potential(Ewald(6.0, 5))
+    with pytest.raises(ValueError):
+        s.set_coulomb_potential(Wolf(5.5))
+    s.set_coulomb_potential(Wolf(5.0))
+    BerendsenThermostat(0.0, 10.0)  # thermostats.rs:103 allows T = 0

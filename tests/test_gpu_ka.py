@@ -62,3 +62,34 @@ def test_ka_cell_list_path_parity(build):
     t = kob_andersen(40000, seed=5)
     t.set_tuning(nlist_build=3 - build)
     assert np.array_equal(t.forces(), f)
+
+
+def test_ka_one_million_atoms_properties():
+    # the configuration's own size: 1,000,000 atoms.  Size-independent
+    # properties: total force ~ 0 (Newton's third law over the full list),
+    # the mixed form's forces against the FP64 form's (RMS < 1e-5), and the
+    # energy change over 100 steps of the mixed form within twice the FP64
+    # form's (plus 1e-6), both starting from the same equilibrated state.
+    s = kob_andersen(1_000_000)
+    sim = Simulation(MolecularDynamics(VelocityVerlet(1.0)))
+    sim.run(s, 20)  # leave the lattice start
+    pos, vel = s.positions.copy(), s.velocities.copy()
+    f64 = s.forces()
+    assert np.abs(f64.sum(0)).max() < 1e-6 * np.abs(f64).max() * np.sqrt(len(f64))
+    change = {}
+    for form in (0, LG_PRECISION_MIXED):
+        s.set_precision(form)
+        s.positions, s.velocities = pos, vel
+        if form:
+            d = s.forces() - f64
+            assert np.sqrt((d * d).sum() / (f64 * f64).sum()) < 1e-5
+        e0 = s.total_energy()
+        run = Simulation(MolecularDynamics(VelocityVerlet(1.0)))
+        run.run(s, 100)
+        e1 = s.total_energy()
+        assert np.isfinite(e1)
+        change[form] = abs(e1 - e0) / abs(e0)
+    s.close()
+    print(f"KA 1M: relative energy change over 100 steps fp64 {change[0]:.2e}, "
+          f"mixed {change[LG_PRECISION_MIXED]:.2e}")
+    assert change[LG_PRECISION_MIXED] <= 2.0 * change[0] + 1e-6

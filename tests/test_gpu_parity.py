@@ -159,7 +159,7 @@ def test_pair_set_bit_exact(builder):
 
 
 # --- molecular dynamics -------------------------------------------------------------------------
-def md_compare(s, sim, oracle_integrator, dt, nsteps, bT=0.0, tau=1.0, tol=1e-8):
+def md_compare(s, sim, oracle_integrator, dt, nsteps, bT=-1.0, tau=1.0, tol=1e-8):
     o = lo.OracleSystem(s, nthreads=16)
     o.md_reset()
     sim.run(s, nsteps)

@@ -108,3 +108,35 @@ def test_memory_admission_serialises_but_completes():
     spans = sorted((j.started, j.finished) for j in jobs)
     for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
         assert b0 >= a1 - 1e-3
+
+
+def _blowup():
+    # two atoms on top of each other: non-finite forces on the first step
+    from lumol_amd import LennardJones, PairInteraction, System, UnitCell, units
+    s = System.from_arrays(["Ar"] * 3, [(5.0, 5.0, 5.0), (5.0, 5.0, 5.0), (9.0, 9.0, 9.0)], UnitCell.cubic(20.0))
+    s.set_pair_potential(("Ar", "Ar"), PairInteraction(LennardJones(3.4, units.from_(1.0, "kJ/mol")), 8.0))
+    return s
+
+
+def test_failing_jobs_do_not_disturb_the_mix():
+    from lumol_amd._native import LG_E_NONFINITE, LumolGpuError
+
+    def bad_build():
+        raise ValueError("the spec cannot be built")
+
+    good = md_jobs()[:3] + mc_jobs()[:3]
+    bad = [Job("md", _blowup, 100, md={"dt": 1.0}, name="blows-up"),
+           Job("md", bad_build, 100, md={"dt": 1.0}, name="bad-spec"),
+           Job("mc", lambda: lj_mc_chain(200, seed=77)[0], 500, mc={"beta": float("nan"), "delta": 0.3, "seed": 1},
+               name="mc-bad-beta")]
+    jobs = good[:2] + bad[:1] + good[2:4] + bad[1:] + good[4:]
+    with Server(md_lanes=2, mc_slice=300) as server:
+        for j in jobs:
+            server.submit(j)
+        server.drain(timeout=300)
+    assert isinstance(bad[0].error, LumolGpuError) and bad[0].error.code == LG_E_NONFINITE
+    assert isinstance(bad[1].error, ValueError)
+    assert isinstance(bad[2].error, LumolGpuError)
+    for j in good:
+        assert j.error is None, (j.name, j.error)
+        same(j.result, run_alone(j))
