@@ -1,16 +1,22 @@
 #!/usr/bin/env python3
 """Throughput of Lumol's force-and-energy hot path on one MI355X.
 
-Workloads (BASELINE configurations; `--workload`):
+Workloads (`--workload`):
 
-* ``nacl`` (default, the metric's configuration): NaCl Ewald NVT -- 4096
-  ions (2048 Na+, 2048 Cl-) on 8^3 rocksalt cells, a = 5.64 A (box 45.12 A),
-  seeded +-0.05 A jitter, Lorentz-Berthelot LJ + Ewald (rc 12 A, kmax 9,
-  alpha = pi/rc), Berendsen 1200 K (tau = 100 steps), Velocity-Verlet
-  dt = 1 fs.
-* ``spce``: flexible SPC/E water NVE -- 3000 molecules (9000 atoms), box
-  44.8 A, Ewald rc 10 A kmax 10, inter-molecular LJ + Coulomb, harmonic
-  bonds and angles, dt = 0.5 fs.
+* ``spce`` (default, the headline configuration): flexible SPC/E water NVE
+  -- 3000 molecules (9000 atoms), box 44.8 A, Ewald rc 10 A kmax 10,
+  inter-molecular LJ + Coulomb, harmonic bonds and angles, Velocity-Verlet
+  dt = 0.5 fs, 2000 timed steps.  ``value`` is the mixed form's aggregate
+  atom-steps/s over `--batch` systems (seeds 3, 4, ...) advancing together;
+  the line carries the FP64 and mixed single-system rates, every form's
+  error against the FP64 oracle (force RMS/max, energy, virial) and NVE
+  drift in kJ/mol/atom/ns over 2000 steps, the ratio mixed-batched / FP64,
+  the dominant kernel's roofline, the serving stream's p50/p99 latency at a
+  stated step scale and the oracle's CPU throughput on this host.
+* ``nacl``: NaCl Ewald NVT -- 4096 ions (2048 Na+, 2048 Cl-) on 8^3
+  rocksalt cells, a = 5.64 A (box 45.12 A), seeded +-0.05 A jitter,
+  Lorentz-Berthelot LJ + Ewald (rc 12 A, kmax 9, alpha = pi/rc), Berendsen
+  1200 K (tau = 100 steps), Velocity-Verlet dt = 1 fs; FP64 and mixed forms.
 * ``serve``: the job server under a Poisson stream of uneven jobs -- MD
   jobs of LJ argon (64..8192 atoms) and SPC/E Ewald water (216..12000
   molecules) in the mixed form, with 1k..20k steps scaled by
@@ -42,12 +48,12 @@ k-space rho(k) and forces, thermostat.  Both arithmetic forms are measured:
 ``fp64`` (all FP64) and ``mixed`` (FP32 pair/erfc/trig with FP64 positions,
 phase reduction and accumulation).
 
-Prints ONE JSON line.  ``value`` is the mixed form's atom-steps/s over
-`--steps` timed steps after `--warmup` untimed steps; the FP64 form's
-figures and each form's force error against the FP64 oracle sit beside it,
-with the dominant kernel's roofline fraction (HIP events on the library's
-stream, algorithmic work per launch on the oracle's pair set) and the
-oracle's threaded CPU throughput on this host.
+Prints ONE JSON line.  Rates are over `--steps` timed steps after
+`--warmup` untimed steps, with inputs resident in HBM.  The roofline names
+the kernel class with the largest device time per step in a pass of the
+same length right after the timed region (HIP events on the library's
+stream around every launch), with algorithmic work per launch counted on the
+oracle's pair set.
 """
 import argparse
 import json
@@ -80,7 +86,7 @@ WORKLOADS = {
     "nacl": "NaCl Ewald NVT, 4096 ions, 8x8x8 rocksalt a=5.64 A, LJ(LB)+Ewald rc 12 A kmax 9, "
             "Berendsen 1200 K, VV dt 1 fs",
     "spce": "flexible SPC/E water NVE, 3000 molecules (9000 atoms), box 44.8 A, LJ O-O + Ewald rc 10 A "
-            "kmax 10 (inter-molecular), harmonic O-H/H-O-H, VV dt 0.5 fs",
+            "kmax 10 (inter-molecular), harmonic O-H/H-O-H, VV dt 0.5 fs, 2000 steps",
     "mc": "batched MC: LJ argon NVT chains, sizes U{108..4000} (seed 4), rho*=0.8, T*=1.2, rc=2.5 sigma, "
           "translation 0.3 sigma, Philox4x32-10 per chain",
     "spce-batched": "flexible SPC/E water NVE, 3000 molecules (9000 atoms) per system, several systems at once, "
@@ -92,6 +98,9 @@ WORKLOADS = {
 
 # LJ-only pair, forces only: geometry 22, LJ 10, force vector and +/- accumulation 10
 FLOP_PAIR_LJ_F = 42
+# Velocity-Verlet, algorithmic HBM bytes per atom-step (FP64): read x, v, F
+# (72) and 1/m (8), write x and v (48)
+BYTES_INTEGRATE = 128
 
 # MC move: LJ energy at the old and the new position for every neighbour
 # within rc: 2 x 29 FLOP per neighbour (geometry 22 + LJ energy 7)
@@ -99,11 +108,24 @@ FLOP_MC_PAIR = 29
 
 
 def host_cores():
+    """The host cores this job owns: the cores it may run on, limited to the
+    host's CPU share per GPU when the environment states one
+    (OMP_NUM_THREADS; 16 of a GPU box's cores belong to its one GPU)."""
     try:
         n = len(os.sched_getaffinity(0))
     except Exception:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        share = 0
+    if share > 0:
+        n = min(n, share)
+    return max(1, n)
+
+
+def nproc():
+    return os.cpu_count() or 1
 
 
 def build_system(workload):
@@ -152,13 +174,16 @@ def cpu_baseline(workload, target_seconds=12.0):
         dt = time.perf_counter() - t0
         rates.append(s.size() * r / dt)
     return {"value": statistics.median(rates), "unit": "atom-steps/s", "cores": cores, "kind": "port",
+            "nproc": nproc(),
             "sample": f"oracle FP64 MD step on the same {s.size()}-atom system: 1 warm-up step, "
-                      f"median of {reps} x {r} steps, {cores} threads"}
+                      f"median of {reps} x {r} steps, {cores} threads (the host's CPU share for this GPU; "
+                      f"nproc {nproc()})"}
 
 
-def force_error(s, oracle_forces, oracle_energy):
+def force_error(s, oracle_forces, oracle_energy, oracle_virial=None):
     """Force RMS / max relative error against the FP64 oracle (max over atoms
-    with |F| > 1e-3 F_rms), and energy relative error."""
+    with |F| > 1e-3 F_rms), energy relative error and, given the oracle's
+    virial, the virial tensor's relative error (Frobenius)."""
     import numpy as np
     fg = s.forces()
     fo = oracle_forces
@@ -169,7 +194,35 @@ def force_error(s, oracle_forces, oracle_energy):
     mask = norms > 1e-3 * frms
     mx = float((np.linalg.norm(d, axis=1)[mask] / norms[mask]).max())
     eg = s.potential_energy()
-    return {"force_rms_rel": rms, "force_max_rel": mx, "energy_rel": abs(eg - oracle_energy) / abs(oracle_energy)}
+    out = {"force_rms_rel": rms, "force_max_rel": mx, "energy_rel": abs(eg - oracle_energy) / abs(oracle_energy)}
+    if oracle_virial is not None:
+        wg = s.virial()
+        out["virial_rel"] = float(np.linalg.norm(wg - oracle_virial) / np.linalg.norm(oracle_virial))
+    return out
+
+
+def nve_drift(make, precision, nsteps=2000, every=100, dt=0.5):
+    """NVE drift of a fresh system from make(): least-squares slope of the
+    total energy per atom over `nsteps` Velocity-Verlet steps, in
+    kJ/mol/atom/ns, with its standard error."""
+    import numpy as np
+    from lumol_amd import MolecularDynamics, Simulation, VelocityVerlet, units
+    s = make()
+    s.set_precision(precision)
+    sim = Simulation(MolecularDynamics(VelocityVerlet(dt)))
+    t, e = [], []
+    for k in range(nsteps // every + 1):
+        t.append(k * every * dt * 1e-6)
+        e.append(units.to(s.total_energy(), "kJ/mol") / s.size())
+        if k < nsteps // every:
+            sim.run(s, every)
+    s.close()
+    t, e = np.array(t), np.array(e)
+    A = np.vstack([t, np.ones_like(t)]).T
+    coef, *_ = np.linalg.lstsq(A, e, rcond=None)
+    resid = e - A @ coef
+    se = math.sqrt((resid ** 2).sum() / (len(t) - 2) / ((t - t.mean()) ** 2).sum())
+    return {"kj_per_mol_atom_ns": float(coef[0]), "stderr": se, "steps": nsteps, "dt_fs": dt}
 
 
 def calibrate_copy(mib=256):
@@ -205,67 +258,80 @@ def load_traffic(workload, form, kernel):
 TUNING_CACHE = None  # --tuning-cache
 
 
-def run_form(workload, form, steps, warmup, oracle_forces, oracle_energy):
-    from lumol_amd._native import LG_PRECISION_FP64, LG_PRECISION_MIXED
+ROOFLINE_CLASSES = ("pairs", "kspace_rho", "kspace_force", "integrate")
+
+
+def class_work(cls, workload, form, natoms, n_pairs, nk, batch=1):
+    """(bound, algorithmic work per launch, peak, unit, description) of a
+    kernel class: VALU FLOP for the pair and k-space kernels (FP32 peak in
+    the mixed form, FP64 otherwise), HBM bytes for the integrator."""
+    peak = PEAK_TFLOPS[form]
+    if cls == "pairs":
+        # water: every O-O pair within rc also carries Coulomb; the pair set
+        # counted is all inter-molecular pairs within rc (oracle's set)
+        flop = FLOP_PAIR_LJ_EWALD_F if workload == "nacl" else FLOP_PAIR_EWALD_F
+        return ("valu", batch * n_pairs * flop / 1e12, peak, "TFLOP/s",
+                f"{flop} FLOP x {n_pairs} pairs within rc (forces only) x {batch} system(s) per launch")
+    if cls == "kspace_force":
+        return ("valu", batch * nk * natoms * FLOP_KSPACE_FORCE / 1e12, peak, "TFLOP/s",
+                f"{FLOP_KSPACE_FORCE} FLOP x {nk} k-vectors x {natoms} atoms x {batch} system(s) per launch")
+    if cls == "kspace_rho":
+        return ("valu", batch * nk * natoms * FLOP_KSPACE_RHO / 1e12, peak, "TFLOP/s",
+                f"{FLOP_KSPACE_RHO} FLOP x {nk} k-vectors x {natoms} atoms x {batch} system(s) per launch "
+                f"(the launch also rebuilds the Verlet list when due)")
+    return ("hbm", batch * natoms * BYTES_INTEGRATE / 1e9, HBM_PEAK_GBS, "GB/s",
+            f"{BYTES_INTEGRATE} B x {natoms} atoms x {batch} system(s) per launch")
+
+
+def roofline_of(prof, steps, workload, form, natoms, n_pairs, nk, batch=1, timing=""):
+    """Dominant kernel class of a profiled pass (largest device time per
+    step) and its roofline: algorithmic work per launch / average launch
+    duration."""
+    per_step = {c: prof[c][0] * 1e3 / steps for c in prof if prof[c][1]}
+    dom = max((c for c in ROOFLINE_CLASSES if c in per_step), key=lambda c: per_step[c])
+    ms, launches = prof[dom]
+    avg_s = ms / 1e3 / launches
+    bound, work, peak, unit, per = class_work(dom, workload, form, natoms, n_pairs, nk, batch)
+    achieved = work / avg_s
+    return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
+            "traffic": load_traffic(workload, form, dom), "kernel": dom, "avg_launch_us": avg_s * 1e6,
+            "launches": launches, "work": per, "timing": timing}, {c: round(v, 2) for c, v in per_step.items()}
+
+
+def run_form(workload, form, steps, warmup, oracle_forces, oracle_energy, oracle_virial=None):
+    from lumol_amd._native import KERNEL_CLASSES, LG_PRECISION_FP64, LG_PRECISION_MIXED
     s = build_system(workload)
     s.set_precision(LG_PRECISION_MIXED if form == "mixed" else LG_PRECISION_FP64)
     natoms = s.size()
     rc = 12.0 if workload == "nacl" else 10.0
     n_pairs = len(s.pairs_within(rc))
     nk = s.ewald_kvector_count()
-    err = force_error(s, oracle_forces, oracle_energy)
+    err = force_error(s, oracle_forces, oracle_energy, oracle_virial)
     # launch knobs for this size bucket (lumol_amd.tuning), chosen before timing
     from lumol_amd import tuning
     knobs = tuning.autotune(s, reps=10, rounds=5)
     if TUNING_CACHE:
         tuning.save_cache(TUNING_CACHE)
     sim = make_sim(workload)
-
-    # warm-up, with a profiled pass over every kernel class to find the dominant one
-    s.profile_begin(["nlist", "pairs", "molecules", "kspace_rho", "kspace_force", "integrate"])
     sim.run(s, warmup)
-    warm = s.profile_end()
-    dominant = max(("pairs", "kspace_rho", "kspace_force", "molecules"), key=lambda c: warm[c][0])
-
     # timed region: no instrumentation (an event pair per launch costs ~10%
     # of the step rate of these small systems)
     t0 = time.perf_counter()
     sim.run(s, steps)  # returns after the stream is synchronised
     elapsed = time.perf_counter() - t0
-    # the same number of steps again with HIP events around the dominant
-    # class, for its average launch duration
-    s.profile_begin([dominant])
+    # the same number of steps again with HIP events around every launch:
+    # per-step device time of each kernel class, the dominant class and its
+    # average launch duration
+    s.profile_begin(list(KERNEL_CLASSES))
     sim.run(s, steps)
     prof = s.profile_end()
-
-    ms, launches = prof[dominant]
-    avg_s = ms / 1e3 / max(launches, 1)
-    peak = PEAK_TFLOPS[form]
-    if dominant == "pairs":
-        # in the water model every O-O pair within rc also carries Coulomb;
-        # the pair set counted is all inter-molecular pairs within rc
-        flop = FLOP_PAIR_LJ_EWALD_F if workload == "nacl" else FLOP_PAIR_EWALD_F
-        work = n_pairs * flop / 1e12
-        per = f"{flop} FLOP x {n_pairs} pairs (r < {rc} A) per launch"
-    elif dominant == "kspace_force":
-        work = nk * natoms * FLOP_KSPACE_FORCE / 1e12
-        per = f"{FLOP_KSPACE_FORCE} FLOP x {nk} k-vectors x {natoms} atoms per launch"
-    elif dominant == "kspace_rho":
-        work = nk * natoms * FLOP_KSPACE_RHO / 1e12
-        per = f"{FLOP_KSPACE_RHO} FLOP x {nk} k-vectors x {natoms} atoms per launch"
-    else:
-        work = 0.0
-        per = "intra-molecular terms"
-    achieved = work / avg_s
-    roofline = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                "traffic": load_traffic(workload, form, dominant), "kernel": dominant,
-                "avg_launch_us": avg_s * 1e6, "work": per,
-                "timing": f"HIP events on the library stream around every launch of the class, over a second "
-                          f"pass of the same {steps} steps right after the timed region"}
-    kernels = {c: round(warm[c][0] * 1e3 / max(warmup, 1), 2) for c in warm if warm[c][1]}
+    roofline, per_step = roofline_of(
+        prof, steps, workload, form, natoms, n_pairs, nk,
+        timing=f"HIP events on the library stream around every launch, over a pass of the same {steps} steps "
+               f"right after the timed region")
     s.close()
     return {"value": natoms * steps / elapsed, "ms_per_step": elapsed * 1e3 / steps,
-            "error_vs_fp64_oracle": err, "roofline": roofline, "kernel_us_per_step": kernels,
+            "error_vs_fp64_oracle": err, "roofline": roofline, "kernel_us_per_step": per_step,
             "atoms": natoms, "pairs_within_rc": n_pairs, "kvectors": nk, "tuning": knobs}
 
 
@@ -376,49 +442,39 @@ def bench_mc(args):
     print(json.dumps(line), flush=True)
 
 
-def bench_spce_batched(args):
-    """Aggregate mixed-form throughput of `batch` SPC/E systems on separate
-    streams against one FP64 system alone."""
-    import threading
-
-    from lumol_amd._native import LG_PRECISION_FP64, LG_PRECISION_MIXED
-    from lumol_amd.workloads import spce
-
+def oracle_reference(workload):
+    """The FP64 oracle's forces, energy and virial of the workload's initial
+    state (the error reference of every form)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import lumol_oracle as lo
-    ref = spce()
-    o = lo.OracleSystem(ref, nthreads=host_cores())
-    fo, eo = o.forces(), o.potential_energy()
-    del o
+    o = lo.OracleSystem(build_system(workload), nthreads=host_cores())
+    return o.forces(), o.potential_energy(), o.virial()
 
-    def make(seed, precision, slot):
-        s = spce(seed=seed)
-        s.stream_slot = slot
-        s.set_precision(precision)
-        s.set_tuning(md_graph=args.md_graph)
-        return s
+
+def run_batched(batch, steps, warmup, ref, md_graph=0):
+    """Aggregate mixed-form throughput of `batch` SPC/E systems (seeds 3,
+    4, ...) advancing together, each on its own library context (HIP
+    stream), with one set of launch knobs tuned on the batch itself."""
+    import threading
 
     from lumol_amd import tuning
+    from lumol_amd._native import LG_PRECISION_MIXED
+    from lumol_amd.workloads import spce
 
-    # FP64, one system alone, with the knobs the autotuner picks for it
-    single = make(3, LG_PRECISION_FP64, 0)
-    knobs64 = tuning.autotune(single, reps=10, rounds=5)
-    if TUNING_CACHE:
-        tuning.save_cache(TUNING_CACHE)
-    sim = make_sim("spce")
-    sim.run(single, args.warmup)
-    t0 = time.perf_counter()
-    sim.run(single, args.steps)
-    fp64_rate = single.size() * args.steps / (time.perf_counter() - t0)
-    single.close()
-    # mixed, `batch` systems at once
-    systems = [make(3 + k, LG_PRECISION_MIXED, 1 + k) for k in range(args.batch)]
+    def make(seed, slot):
+        s = spce(seed=seed)
+        s.stream_slot = slot
+        s.set_precision(LG_PRECISION_MIXED)
+        s.set_tuning(md_graph=md_graph)
+        return s
+
+    systems = [make(3 + k, 1 + k) for k in range(batch)]
     sims = [make_sim("spce") for _ in systems]
     for s, sm in zip(systems, sims):
-        sm.run(s, args.warmup)
+        sm.run(s, warmup)
 
-    def concurrent(steps, warm=0):
-        # every system advances `steps` MD steps on its own host thread and
+    def concurrent(nsteps, warm=0):
+        # every system advances `nsteps` MD steps on its own host thread and
         # stream; the wall time from the common start to the last finish
         barrier = threading.Barrier(len(systems) + 1)
 
@@ -426,7 +482,7 @@ def bench_spce_batched(args):
             if warm:
                 sm.run(s, warm)
             barrier.wait()
-            sm.run(s, steps)
+            sm.run(s, nsteps)
 
         threads = [threading.Thread(target=work, args=(s, sm)) for s, sm in zip(systems, sims)]
         for t in threads:
@@ -437,30 +493,82 @@ def bench_spce_batched(args):
             t.join()
         return time.perf_counter() - t0
 
-    # one set of knobs for the batch, timed on the batch itself
     knobs = tuning.autotune_concurrent(systems, lambda: concurrent(200, warm=20))
-    # the stated error, of the first system's initial state with these knobs
-    chk = make(3, LG_PRECISION_MIXED, 1)
-    chk.set_tuning(knobs["pair_lanes"], knobs["kspace_split"], knobs["rho_width"], 0, args.md_graph)
-    err = force_error(chk, fo, eo)
+    # the stated error: the first system's initial state with these knobs
+    chk = make(3, 1)
+    chk.set_tuning(knobs["pair_lanes"], knobs["kspace_split"], knobs["rho_width"], 0, md_graph)
+    err = force_error(chk, *ref)
     chk.close()
-    concurrent(args.warmup)
-    elapsed = concurrent(args.steps)
+    concurrent(warmup)
+    elapsed = concurrent(steps)
     natoms = sum(s.size() for s in systems)
-    value = natoms * args.steps / elapsed
     for s in systems:
         s.close()
+    return {"value": natoms * steps / elapsed, "ms_per_step": elapsed * 1e3 / steps, "batch": batch,
+            "atoms": natoms, "error_vs_fp64_oracle": err, "tuning": knobs,
+            "how": f"{batch} systems (seeds 3..{2 + batch}), one host thread and HIP stream each, "
+                   f"wall time from a common start to the last finish"}
+
+
+def bench_spce_batched(args):
+    """Aggregate mixed-form throughput of `batch` SPC/E systems against one
+    FP64 system alone."""
+    ref = oracle_reference("spce")
+    single = run_form("spce", "fp64", args.steps, args.warmup, *ref)
+    bat = run_batched(args.batch, args.steps, args.warmup, ref, args.md_graph)
     line = {
-        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+        "metric": METRIC, "value": bat["value"], "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": bat["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded grid + jitter)",
-        "config": {"workload": WORKLOADS["spce-batched"], "batch": args.batch, "form": "mixed", "md_graph": args.md_graph,
-                   "atoms_per_system": systems[0].size()},
-        "error_vs_fp64_oracle": err,
-        "fp64_single_system": fp64_rate,
-        "tuning": {"fp64_single_system": knobs64, "mixed_batched": knobs},
-        "mixed_batched_over_fp64": value / fp64_rate,
+        "config": {"workload": WORKLOADS["spce-batched"], "batch": args.batch, "form": "mixed",
+                   "md_graph": args.md_graph},
+        "error_vs_fp64_oracle": bat["error_vs_fp64_oracle"],
+        "fp64_single_system": single["value"],
+        "tuning": {"fp64_single_system": single["tuning"], "mixed_batched": bat["tuning"]},
+        "mixed_batched_over_fp64": bat["value"] / single["value"],
     }
+    print(json.dumps(line), flush=True)
+
+
+def bench_c3(args):
+    """The headline measurement: flexible SPC/E water, 3000 molecules (9000
+    atoms), Ewald.  `value` is the mixed form's aggregate rate over
+    `--batch` systems advancing together; FP64 and mixed single-system rates,
+    each form's error against the FP64 oracle and NVE drift, the dominant
+    kernel's roofline, the serving stream's p50/p99 and the CPU baseline sit
+    beside it."""
+    from lumol_amd._native import LG_PRECISION_FP64, LG_PRECISION_MIXED
+    from lumol_amd.workloads import spce
+    ref = oracle_reference("spce")
+    fp64 = run_form("spce", "fp64", args.steps, args.warmup, *ref)
+    mixed = run_form("spce", "mixed", args.steps, args.warmup, *ref)
+    bat = run_batched(args.batch, args.steps, args.warmup, ref, args.md_graph)
+    drift = {"fp64": nve_drift(spce, LG_PRECISION_FP64), "mixed": nve_drift(spce, LG_PRECISION_MIXED)}
+    keep = ("value", "ms_per_step", "error_vs_fp64_oracle", "roofline", "kernel_us_per_step", "tuning")
+    line = {
+        "metric": METRIC, "value": bat["value"], "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": bat["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded jittered grid, random orientations, Maxwell-Boltzmann velocities)",
+        "config": {"workload": WORKLOADS["spce"], "form": "mixed, batched", "batch": args.batch,
+                   "atoms_per_system": mixed["atoms"], "pairs_within_rc": mixed["pairs_within_rc"],
+                   "kvectors": mixed["kvectors"]},
+        "error_vs_fp64_oracle": bat["error_vs_fp64_oracle"],
+        "roofline": mixed["roofline"],
+        "forms": {
+            "fp64_single": dict({k: fp64[k] for k in keep}, nve_drift=drift["fp64"]),
+            "mixed_single": dict({k: mixed[k] for k in keep}, nve_drift=drift["mixed"]),
+            "mixed_batched": bat,
+            "mixed_batched_over_fp64": bat["value"] / fp64["value"],
+            "mixed_single_over_fp64": mixed["value"] / fp64["value"],
+        },
+        "roofline_note": "dominant kernel of the mixed single-system run (the kernel measured alone on the device)",
+    }
+    if not args.no_serve:
+        line["serve"] = run_serve(args.serve_jobs, args.serve_scale, args.serve_seconds, 0, 0, args.md_lanes,
+                                  args.slice, "mixed")
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline("spce", args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 
@@ -546,8 +654,11 @@ def ka_cpu_baseline(natoms, sample=16384):
                       f"scaled by (N/{sample})^2 to {natoms} atoms (extrapolated)"}
 
 
-def bench_serve(args):
-    """Poisson stream of MD and MC jobs through lumol_amd.serving.Server."""
+def run_serve(md_jobs, steps_scale, seconds, chains, sweeps, md_lanes, mc_slice, form):
+    """Poisson stream of MD and MC jobs through lumol_amd.serving.Server:
+    p50/p99 job latency (submit -> result) per class and aggregate
+    atom-steps/s.  Host-side construction of each job's system happens
+    before its submission and is not timed."""
     import threading
 
     import numpy as np
@@ -557,18 +668,17 @@ def bench_serve(args):
     from lumol_amd.workloads import lj_mc_chain, serving_stream
 
     rng = np.random.default_rng(5)
-    precision = LG_PRECISION_MIXED if args.form == "mixed" else LG_PRECISION_FP64
+    precision = LG_PRECISION_MIXED if form == "mixed" else LG_PRECISION_FP64
     jobs = [Job("md", (lambda s=system: s), steps, md=opts, name=name)
-            for name, system, steps, opts in serving_stream(args.md_jobs, args.steps_scale, precision=precision,
-                                                            rng=rng)]
-    for k, n in enumerate(mc_chain_sizes(args.chains)):
+            for name, system, steps, opts in serving_stream(md_jobs, steps_scale, precision=precision, rng=rng)]
+    for k, n in enumerate(mc_chain_sizes(chains)):
         system, beta, delta = lj_mc_chain(int(n), seed=20000 + k)
-        jobs.append(Job("mc", (lambda s=system: s), int(n) * args.sweeps,
+        jobs.append(Job("mc", (lambda s=system: s), int(n) * sweeps,
                         mc={"beta": beta, "delta": delta, "seed": 7000 + k}, name=f"mc-{n}"))
     order = rng.permutation(len(jobs))
     jobs = [jobs[i] for i in order]
-    gaps = rng.exponential(args.serve_seconds / len(jobs), size=len(jobs))
-    with Server(md_lanes=args.md_lanes, mc_slice=args.slice) as server:
+    gaps = rng.exponential(seconds / len(jobs), size=len(jobs))
+    with Server(md_lanes=md_lanes, mc_slice=mc_slice) as server:
         t0 = time.perf_counter()
 
         def feed():
@@ -583,21 +693,30 @@ def bench_serve(args):
         elapsed = time.perf_counter() - t0
         launches = server.mc_launches
     errors = [(j.name, repr(j.error)) for j in jobs if j.error is not None]
-    md = latency_stats([j for j in jobs if j.kind == "md"])
-    mc = latency_stats([j for j in jobs if j.kind == "mc"])
-    total = latency_stats(jobs)
+    return {
+        "md_jobs": md_jobs, "mc_chains": chains, "md_steps": "U{1000..20000} x %g" % steps_scale,
+        "mc_sweeps": sweeps, "md_lanes": md_lanes, "mc_slice": mc_slice, "form": form,
+        "mean_interarrival_s": seconds / len(jobs), "elapsed_s": elapsed,
+        "latency": {"all": latency_stats(jobs), "md": latency_stats([j for j in jobs if j.kind == "md"]),
+                    "mc": latency_stats([j for j in jobs if j.kind == "mc"])},
+        "mc_launches": launches, "errors": errors[:5],
+    }
+
+
+def bench_serve(args):
+    r = run_serve(args.md_jobs, args.steps_scale, args.serve_seconds, args.chains, args.sweeps, args.md_lanes,
+                  args.slice, args.form)
+    total = r["latency"]["all"]
+    njobs = args.md_jobs + args.chains
     line = {
         "metric": METRIC, "value": total["atom_steps_per_s"], "unit": "atom-steps/s", "n_gpus": 1,
-        "steps": len(jobs), "warmup": 0, "ms_per_step": elapsed * 1e3 / len(jobs),
+        "steps": njobs, "warmup": 0, "ms_per_step": r["elapsed_s"] * 1e3 / njobs,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if args.form == "mixed" else "f64", "data": "synthetic (seeded lattices + jitter)",
-        "config": {"workload": WORKLOADS["serve"], "md_jobs": args.md_jobs, "mc_chains": args.chains,
-                   "md_steps": "U{1000..20000} x %g" % args.steps_scale, "mc_sweeps": args.sweeps,
-                   "md_lanes": args.md_lanes, "mc_slice": args.slice,
-                   "mean_interarrival_s": args.serve_seconds / len(jobs), "form": args.form},
-        "latency": {"all": total, "md": md, "mc": mc},
-        "mc_launches": launches,
-        "errors": errors[:5],
+        "config": {"workload": WORKLOADS["serve"], **{k: r[k] for k in ("md_jobs", "mc_chains", "md_steps",
+                                                                        "mc_sweeps", "md_lanes", "mc_slice",
+                                                                        "mean_interarrival_s", "form")}},
+        "latency": r["latency"], "mc_launches": r["mc_launches"], "errors": r["errors"],
     }
     print(json.dumps(line), flush=True)
 
@@ -605,9 +724,9 @@ def bench_serve(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="nacl")
+    ap.add_argument("--steps", type=int, default=2000, help="timed MD steps (the SPC/E configuration's run)")
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="spce")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--chains", type=int, default=1024, help="mc: number of chains")
@@ -622,6 +741,10 @@ def main():
     ap.add_argument("--steps-scale", type=float, default=0.02, help="serve: factor on the 1k..20k MD steps")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serve: span of the arrivals")
     ap.add_argument("--form", choices=["fp64", "mixed"], default="mixed", help="serve: MD arithmetic form")
+    ap.add_argument("--no-serve", action="store_true", help="spce: leave out the serving stream")
+    ap.add_argument("--serve-jobs", type=int, default=256, help="spce: MD jobs of the serving stream")
+    ap.add_argument("--serve-scale", type=float, default=0.005,
+                    help="spce: factor on the serving stream's 1k..20k MD steps")
     ap.add_argument("--tuning-cache", default=None,
                     help="JSON file of autotuner choices: read if it exists, written after tuning (so that a "
                          "profiled rerun launches the same kernel variants as the timed run)")
@@ -656,16 +779,12 @@ def main():
     if args.workload == "spce-batched":
         bench_spce_batched(args)
         return
+    if args.workload == "spce":
+        bench_c3(args)
+        return
 
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import lumol_oracle as lo
-    ref = build_system(args.workload)
-    o = lo.OracleSystem(ref, nthreads=host_cores())
-    oracle_forces, oracle_energy = o.forces(), o.potential_energy()
-    del o
-
-    forms = {f: run_form(args.workload, f, args.steps, args.warmup, oracle_forces, oracle_energy)
-             for f in ("fp64", "mixed")}
+    ref = oracle_reference(args.workload)
+    forms = {f: run_form(args.workload, f, args.steps, args.warmup, *ref) for f in ("fp64", "mixed")}
     m = forms["mixed"]
     line = {
         "metric": METRIC,
