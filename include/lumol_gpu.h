@@ -45,10 +45,35 @@ enum {
 /* ---- closed set of GPU potential kinds ----------------------------------
  * Replaces the open `Potential`/`PairPotential` trait objects
  * (lumol-core/src/energy/mod.rs:98-163); kernels cannot call host closures.
- * params: LJ {sigma, epsilon} (functions.rs:72-107);
- *         HARMONIC {k, x0} (functions.rs:128-160); NULL {} (functions.rs:30-50).
+ * `params` of each kind, in the reference's argument order, and where the
+ * reference implements it (P pair, B bond, A angle, D dihedral):
+ *   NULL {}                                functions.rs:30-50     P B A D
+ *   LJ {sigma, epsilon}                    functions.rs:52-107    P
+ *   HARMONIC {k, x0}                       functions.rs:109-160   P B A D
+ *   BUCKINGHAM {A, C, rho}                 functions.rs:260-327   P
+ *   BORN (Born-Mayer-Huggins)
+ *        {A, C, D, sigma, rho}             functions.rs:330-395   P
+ *   MORSE {A, x0, depth}                   functions.rs:397-437   P B A D
+ *   GAUSSIAN {a, b}, b > 0                 functions.rs:439-499   P
+ *   MIE {sigma, epsilon, n, m}, m < n      functions.rs:501-592   P
+ *   COSINE_HARMONIC {k, x0}                functions.rs:162-210     A D
+ *   TORSION {k, delta, n}, n integer >= 0  functions.rs:212-258       D
+ * A kind used where the reference has no such trait implementation returns
+ * LG_E_UNSUPPORTED; parameters the reference's constructors reject
+ * (Gaussian::new, Mie::new assert) return LG_E_INVALID.
  */
-enum { LG_POT_NULL = 0, LG_POT_LJ = 1, LG_POT_HARMONIC = 2 };
+enum {
+    LG_POT_NULL = 0,
+    LG_POT_LJ = 1,
+    LG_POT_HARMONIC = 2,
+    LG_POT_BUCKINGHAM = 3,
+    LG_POT_BORN = 4,
+    LG_POT_MORSE = 5,
+    LG_POT_GAUSSIAN = 6,
+    LG_POT_MIE = 7,
+    LG_POT_COSINE_HARMONIC = 8,
+    LG_POT_TORSION = 9
+};
 
 /* PairRestriction (lumol-core/src/energy/restrictions.rs:13-33). */
 enum {
@@ -139,6 +164,15 @@ int lg_system_set_precision(lg_system* sys, int precision);
  * length (system.rs:123-129 panics there). */
 int lg_pair_set(lg_system* sys, uint32_t kind_i, uint32_t kind_j, int pot_kind, const double* params,
                 double cutoff, int shifted, int tail, int restriction, double scale14);
+/* lg_pair_set with the potential evaluated through
+ * TableComputation::new(potential, table_size, table_max)
+ * (lumol-core/src/energy/computations.rs:96-117): energy and force are
+ * tabulated at r = i * table_max / table_size and interpolated linearly,
+ * zero from the last bin on (:122-146); tails stay those of the potential
+ * (:148-175) and a shift is the table's energy at the cutoff. */
+int lg_pair_set_table(lg_system* sys, uint32_t kind_i, uint32_t kind_j, int pot_kind, const double* params,
+                      uint64_t table_size, double table_max, double cutoff, int shifted, int tail,
+                      int restriction, double scale14);
 /* System::set_bond_potential / set_angle_potential / set_dihedral_potential
  * (system.rs:133-170). */
 int lg_bond_set(lg_system* sys, uint32_t kind_i, uint32_t kind_j, int pot_kind, const double* params);
@@ -170,6 +204,14 @@ int lg_potential_energy(lg_system* sys, lg_energy_terms* out);
 /* System::virial with the atomic definition (system.rs:286,
  * compute.rs:195-255), row-major 3x3. */
 int lg_virial(lg_system* sys, double* virial /* 9 */);
+/* MolecularVirial (compute.rs:278-364), row-major 3x3: the pair virials
+ * between molecules weighted by (r_ab . r_ij) / r_ab^2 (r_ij between the
+ * centres of mass), the tail corrections, no bond/angle/dihedral term, and
+ * for Ewald the real-space pairs between molecules plus the k-space atomic
+ * virial minus sum_i F_i (x) (r_i - com) (ewald.rs:533-570, 736-753; Wolf
+ * wolf.rs:285-324).  The virial of System::virial when the simulated degrees
+ * of freedom are molecules (compute.rs:372-381). */
+int lg_molecular_virial(lg_system* sys, double* virial /* 9 */);
 /* KineticEnergy / Temperature (compute.rs:133-172; dof = 3N, system.rs:249). */
 int lg_kinetic_energy(lg_system* sys, double* out);
 int lg_temperature(lg_system* sys, double* out);
@@ -242,7 +284,9 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
 /* ---- launch tuning --------------------------------------------------------------------
  * No counterpart in the reference (a CPU code has no launch shapes).  Knobs of
  * the device kernels; 0 leaves the built-in size-based choice:
- *   pair_lanes    lanes per atom in the pair and neighbour-list kernels (4..64)
+ *   pair_lanes    lanes per atom in the pair and neighbour-list kernels (4..64;
+ *                 16 for systems with a pair potential other than LJ,
+ *                 Harmonic and Null, or a table)
  *   kspace_split  row ranges of the k-space force kernel (1..32)
  *   rho_width     atoms per block of the row form of the rho(k) kernel (64,
  *                 128, 256, 512)

@@ -20,6 +20,8 @@ LG_E_STATE = 4
 LG_E_NONFINITE = 5
 
 LG_POT_NULL, LG_POT_LJ, LG_POT_HARMONIC = 0, 1, 2
+LG_POT_BUCKINGHAM, LG_POT_BORN, LG_POT_MORSE, LG_POT_GAUSSIAN, LG_POT_MIE = 3, 4, 5, 6, 7
+LG_POT_COSINE_HARMONIC, LG_POT_TORSION = 8, 9
 LG_PRECISION_FP64, LG_PRECISION_MIXED = 0, 1
 LG_INTEGRATOR_VELOCITY_VERLET, LG_INTEGRATOR_LEAP_FROG = 0, 1
 
@@ -30,9 +32,9 @@ EXPORTS = [
     "lg_system_set_positions", "lg_system_get_positions", "lg_system_set_velocities",
     "lg_system_get_velocities", "lg_system_set_masses", "lg_system_set_charges",
     "lg_system_set_kinds", "lg_system_set_bonds", "lg_system_molecule_count",
-    "lg_system_set_precision", "lg_system_set_tuning", "lg_system_get_tuning", "lg_pair_set", "lg_bond_set", "lg_angle_set", "lg_dihedral_set",
+    "lg_system_set_precision", "lg_system_set_tuning", "lg_system_get_tuning", "lg_pair_set", "lg_pair_set_table", "lg_bond_set", "lg_angle_set", "lg_dihedral_set",
     "lg_coulomb_ewald", "lg_coulomb_wolf", "lg_coulomb_clear", "lg_forces",
-    "lg_potential_energy", "lg_virial", "lg_kinetic_energy", "lg_temperature",
+    "lg_potential_energy", "lg_virial", "lg_molecular_virial", "lg_kinetic_energy", "lg_temperature",
     "lg_ewald_kvector_count", "lg_pairs_within", "lg_md_setup", "lg_md_run", "lg_cache_init", "lg_move_cost",
     "lg_move_commit", "lg_move_discard", "lg_cache_energy", "lg_mc_run", "lg_mc_run_batch", "lg_profile_begin",
     "lg_profile_end",
@@ -104,6 +106,7 @@ def _declare(lib):
         "lg_system_set_tuning": (I, [P, ctypes.POINTER(Tuning)]),
         "lg_system_get_tuning": (I, [P, ctypes.POINTER(Tuning)]),
         "lg_pair_set": (I, [P, U32, U32, I, DP, D, I, I, I, D]),
+        "lg_pair_set_table": (I, [P, U32, U32, I, DP, U64, D, D, I, I, I, D]),
         "lg_bond_set": (I, [P, U32, U32, I, DP]),
         "lg_angle_set": (I, [P, U32, U32, U32, I, DP]),
         "lg_dihedral_set": (I, [P, U32, U32, U32, U32, I, DP]),
@@ -113,6 +116,7 @@ def _declare(lib):
         "lg_forces": (I, [P, DP, I]),
         "lg_potential_energy": (I, [P, ctypes.POINTER(EnergyTerms)]),
         "lg_virial": (I, [P, DP]),
+        "lg_molecular_virial": (I, [P, DP]),
         "lg_kinetic_energy": (I, [P, DP]),
         "lg_temperature": (I, [P, DP]),
         "lg_ewald_kvector_count": (I, [P, ctypes.POINTER(I)]),

@@ -80,43 +80,215 @@ struct DevBuf {
     }
 };
 
+// A potential as the reference's structs hold it (parameters in field
+// order; see PairParam / BondedParam), plus an optional TableComputation.
+struct PotSpec {
+    int pot = LG_POT_NULL;
+    double p[5] = {0, 0, 0, 0, 0};
+    uint64_t tab_n = 0;  // TableComputation size (0: evaluated directly)
+    double tab_max = 0;
+};
 struct PairSpec {
-    int pot;
-    double p0, p1, rc;
+    PotSpec v;
+    double rc;
     int shifted, tail, restriction;
     double scale14;
 };
-struct BondedSpec {
-    int pot;
-    double p0, p1;
-};
+using BondedSpec = PotSpec;
 
-double host_pot_energy(int pot, double p0, double p1, double r) {
-    if (pot == LG_POT_LJ) {
-        double x = p0 / r;
-        double x2 = x * x;
-        double x4 = x2 * x2;
-        double s6 = x2 * x4;
-        return 4.0 * p1 * (s6 * s6 - s6);
-    }
-    if (pot == LG_POT_HARMONIC) {
-        double dx = r - p1;
-        return 0.5 * p0 * dx * dx;
+// Host evaluation of the potentials in the reference's operation order,
+// without fused multiply-adds (Rust fuses none): the shift V(rc), the
+// TableComputation tables and the tail corrections are computed here.
+#pragma clang fp contract(off)
+// functions.rs:79-89, 135-145, 198-207, 244-256, 286-300, 353-367, 414-424,
+// 474-482, 552-566
+double host_direct(const PotSpec& q, double r, bool force) {
+    const double* p = q.p;
+    switch (q.pot) {
+        case LG_POT_LJ: {
+            double x = p[0] / r;
+            double x2 = x * x;
+            double x4 = x2 * x2;
+            double s6 = x2 * x4;
+            return force ? -24.0 * p[1] * (s6 - 2.0 * (s6 * s6)) / r : 4.0 * p[1] * (s6 * s6 - s6);
+        }
+        case LG_POT_HARMONIC: {
+            if (force) return p[0] * (p[1] - r);
+            double dx = r - p[1];
+            return 0.5 * p[0] * dx * dx;
+        }
+        case LG_POT_BUCKINGHAM: {
+            double r3 = r * r * r;
+            double ex = std::exp(-r / p[2]);
+            if (force) return p[0] / p[2] * ex - 6.0 * p[1] / (r3 * r3 * r);
+            return p[0] * ex - p[1] / (r3 * r3);
+        }
+        case LG_POT_BORN: {
+            double r2 = r * r;
+            double ex = std::exp((p[3] - r) / p[4]);
+            if (force) {
+                double r7 = r2 * r2 * r2 * r;
+                return p[0] / p[4] * ex - 6.0 * p[1] / r7 + 8.0 * p[2] / (r7 * r2);
+            }
+            double r6 = r2 * r2 * r2;
+            return p[0] * ex - p[1] / r6 + p[2] / (r6 * r2);
+        }
+        case LG_POT_MORSE: {
+            double ex = std::exp((p[1] - r) * p[0]);
+            if (force) return 2.0 * p[2] * (1.0 - ex * ex) * p[0];
+            double m = 1.0 - ex;
+            return p[2] * m * m;
+        }
+        case LG_POT_GAUSSIAN: {
+            double e = -p[0] * std::exp(-p[1] * r * r);
+            return force ? 2.0 * p[1] * r * e : e;
+        }
+        case LG_POT_MIE: {
+            double sr = p[0] / r;
+            double rep = std::pow(sr, p[1]), att = std::pow(sr, p[2]);
+            return force ? p[3] * (p[1] * rep - p[2] * att) / r : p[3] * (rep - att);
+        }
+        case LG_POT_COSINE_HARMONIC: {
+            double dr = std::cos(r) - p[1];
+            return force ? p[0] * dr * std::sin(r) : 0.5 * p[0] * dr * dr;
+        }
+        case LG_POT_TORSION: {
+            double x = p[2] * r - p[1];
+            return force ? p[0] * p[2] * std::sin(x) : p[0] * (1.0 + std::cos(x));
+        }
     }
     return 0.0;
 }
-// functions.rs:92-106 (LJ only; Null/Harmonic tails are 0)
-double host_tail_energy(const PairSpec& p) {
-    if (!p.tail || p.pot != LG_POT_LJ) return 0.0;
-    double s3 = p.p0 * p.p0 * p.p0, rc3 = p.rc * p.rc * p.rc;
-    double s9 = s3 * s3 * s3, rc9 = rc3 * rc3 * rc3;
-    return 4.0 / 3.0 * p.p1 * s3 * (1.0 / 3.0 * s9 / rc9 - s3 / rc3);
+// computations.rs:96-117: entries at r = i * max / size
+void host_table(const PotSpec& q, std::vector<double>& e, std::vector<double>& f) {
+    const double delta = q.tab_max / double(q.tab_n);
+    e.resize(q.tab_n);
+    f.resize(q.tab_n);
+    for (uint64_t i = 0; i < q.tab_n; i++) {
+        double r = double(i) * delta;
+        e[i] = host_direct(q, r, false);
+        f[i] = host_direct(q, r, true);
+    }
 }
-double host_tail_virial(const PairSpec& p) {
-    if (!p.tail || p.pot != LG_POT_LJ) return 0.0;
-    double s3 = p.p0 * p.p0 * p.p0, rc3 = p.rc * p.rc * p.rc;
-    double s9 = s3 * s3 * s3, rc9 = rc3 * rc3 * rc3;
-    return 8.0 * p.p1 * s3 * (2.0 / 3.0 * s9 / rc9 - s3 / rc3);
+// computations.rs:122-134 (energy of the table, for the shift)
+double host_energy(const PotSpec& q, double r) {
+    if (q.tab_n == 0) return host_direct(q, r, false);
+    const double delta = q.tab_max / double(q.tab_n);
+    const double u = std::floor(r / delta);
+    const double bin = u > 0.0 ? u : 0.0;
+    if (!(bin < double(q.tab_n - 1))) return 0.0;
+    const uint64_t b = uint64_t(bin);
+    double e0 = host_direct(q, double(b) * delta, false), e1 = host_direct(q, double(b + 1) * delta, false);
+    double dx = r - bin * delta;
+    return e0 + dx * ((e1 - e0) / delta);
+}
+// Tails: functions.rs:92-106 (LJ), :302-327 (Buckingham, including its
+// `+ 8.0`), :369-395 (BornMayerHuggins), :484-499 (Gaussian), :568-592
+// (Mie); zero for Null, Harmonic and Morse.  A table keeps the tails of the
+// potential it wraps (computations.rs:148-175).
+double host_tail(const PotSpec& q, double rc, bool virial) {
+    const double* p = q.p;
+    switch (q.pot) {
+        case LG_POT_LJ: {
+            double s3 = p[0] * p[0] * p[0], rc3 = rc * rc * rc;
+            double s9 = s3 * s3 * s3, rc9 = rc3 * rc3 * rc3;
+            if (virial) return 8.0 * p[1] * s3 * (2.0 / 3.0 * s9 / rc9 - s3 / rc3);
+            return 4.0 / 3.0 * p[1] * s3 * (1.0 / 3.0 * s9 / rc9 - s3 / rc3);
+        }
+        case LG_POT_BUCKINGHAM: case LG_POT_BORN: {
+            const bool born = q.pot == LG_POT_BORN;
+            const double rho = born ? p[4] : p[2];
+            double rc2 = rc * rc, rc3 = rc2 * rc;
+            double ex = born ? std::exp((p[3] - rc) / rho) : std::exp(-rc / rho);
+            if (virial) {
+                double factor = rc3 + 3.0 * rc2 * rho + 6.0 * rc * rho * rho + 6.0 * rho * rho * rho;
+                double w = p[0] * ex * factor - 20.0 * p[1] / rc3;
+                return born ? w + 8.0 * p[2] / (5.0 * rc2 * rc3) : w + 8.0;
+            }
+            double factor = rc2 - 2.0 * rc * rho + 2.0 * rho * rho;
+            double e = p[0] * rho * ex * factor - p[1] / (3.0 * rc3);
+            return born ? e + p[2] / (5.0 * rc2 * rc3) : e;
+        }
+        case LG_POT_GAUSSIAN: {
+            double erc = host_direct(q, rc, false);
+            if (virial)
+                return 3.0 * std::sqrt(kPi) * p[0] * std::erfc(std::sqrt(p[1]) * rc) / (4.0 * std::pow(p[1], 3.0 / 2.0)) -
+                       erc * rc * (2.0 * p[1] * rc * rc + 3.0) / (2.0 * p[1]);
+            return erc * rc / (2.0 * p[1]) -
+                   p[0] * std::sqrt(kPi) * std::erfc(std::sqrt(p[1]) * rc) / (4.0 * std::pow(p[1], 3.0 / 2.0));
+        }
+        case LG_POT_MIE: {
+            if (p[2] <= 3.0) return 0.0;
+            double sr = p[0] / rc, n3 = p[1] - 3.0, m3 = p[2] - 3.0;
+            double rep = std::pow(sr, n3), att = std::pow(sr, m3);
+            if (virial) return p[3] * (p[0] * p[0] * p[0]) * (rep * p[1] / n3 - att * p[2] / m3);
+            return p[3] * (p[0] * p[0] * p[0]) * (rep / n3 - att / m3);
+        }
+    }
+    return 0.0;
+}
+#pragma clang fp contract(on)
+
+double host_tail_energy(const PairSpec& p) { return p.tail ? host_tail(p.v, p.rc, false) : 0.0; }
+double host_tail_virial(const PairSpec& p) { return p.tail ? host_tail(p.v, p.rc, true) : 0.0; }
+
+// Number of parameters of each kind at the C ABI, and where each kind may be
+// used (the reference's trait impls: PairPotential, BondPotential,
+// AnglePotential, DihedralPotential; functions.rs, computations.rs).
+enum : int { ROLE_PAIR = 1, ROLE_BOND = 2, ROLE_ANGLE = 4, ROLE_DIHEDRAL = 8 };
+struct PotInfo {
+    int nparams, roles;
+    const char* name;
+};
+PotInfo pot_info(int pot) {
+    switch (pot) {
+        case LG_POT_NULL: return {0, ROLE_PAIR | ROLE_BOND | ROLE_ANGLE | ROLE_DIHEDRAL, "NullPotential"};
+        case LG_POT_LJ: return {2, ROLE_PAIR, "LennardJones"};
+        case LG_POT_HARMONIC: return {2, ROLE_PAIR | ROLE_BOND | ROLE_ANGLE | ROLE_DIHEDRAL, "Harmonic"};
+        case LG_POT_BUCKINGHAM: return {3, ROLE_PAIR, "Buckingham"};
+        case LG_POT_BORN: return {5, ROLE_PAIR, "BornMayerHuggins"};
+        case LG_POT_MORSE: return {3, ROLE_PAIR | ROLE_BOND | ROLE_ANGLE | ROLE_DIHEDRAL, "Morse"};
+        case LG_POT_GAUSSIAN: return {2, ROLE_PAIR, "Gaussian"};
+        case LG_POT_MIE: return {4, ROLE_PAIR, "Mie"};
+        case LG_POT_COSINE_HARMONIC: return {2, ROLE_ANGLE | ROLE_DIHEDRAL, "CosineHarmonic"};
+        case LG_POT_TORSION: return {3, ROLE_DIHEDRAL, "Torsion"};
+    }
+    return {-1, 0, "unknown"};
+}
+
+// The C ABI's parameters -> PotSpec, validated as the reference's
+// constructors assert: Gaussian::new b > 0 (functions.rs:465), Mie::new
+// m < n (:544, prefac as there), Torsion's integer multiplicity, and
+// CosineHarmonic::new storing cos(x0) (:189-196).
+int make_pot(int pot, const double* params, int role, PotSpec& out) {
+    PotInfo info = pot_info(pot);
+    if (info.nparams < 0) return fail(LG_E_UNSUPPORTED, "unknown potential kind");
+    if (!(info.roles & role)) {
+        const char* what = role == ROLE_PAIR ? "a pair" : role == ROLE_BOND ? "a bond"
+                         : role == ROLE_ANGLE ? "an angle" : "a dihedral";
+        return fail(LG_E_UNSUPPORTED, std::string(info.name) + " is not " + what + " potential");
+    }
+    if (info.nparams > 0 && !params) return fail(LG_E_INVALID, "missing potential parameters");
+    out = PotSpec{};
+    out.pot = pot;
+    for (int k = 0; k < info.nparams; k++) {
+        if (!std::isfinite(params[k])) return fail(LG_E_INVALID, "potential parameters must be finite");
+        out.p[k] = params[k];
+    }
+    if (pot == LG_POT_GAUSSIAN && !(out.p[1] > 0.0))
+        return fail(LG_E_INVALID, "\"b\" has to be positive in Gaussian potential");
+    if (pot == LG_POT_MIE) {
+        double sigma = params[0], eps = params[1], n = params[2], m = params[3];
+        if (!(m < n)) return fail(LG_E_INVALID, "The repulsive exponent n has to be larger than the attractive exponent m");
+        out.p[0] = sigma;
+        out.p[1] = n;
+        out.p[2] = m;
+        out.p[3] = n / (n - m) * std::pow(n / m, m / (n - m)) * eps;
+    }
+    if (pot == LG_POT_COSINE_HARMONIC) out.p[1] = std::cos(params[1]);
+    if (pot == LG_POT_TORSION && !(params[2] >= 0.0 && params[2] == std::floor(params[2])))
+        return fail(LG_E_INVALID, "the multiplicity of a torsion must be a non-negative integer");
+    return LG_OK;
 }
 
 }  // namespace
@@ -169,6 +341,8 @@ struct lg_system {
     DevBuf<float4> cpos;  // list-build candidate records in sorted order
     // parameters
     DevBuf<PairParam> table;
+    DevBuf<double> ptabs;  // TableComputation tables: energies then forces per kind pair
+    bool gen_pairs = false;  // a pair potential other than LJ/Harmonic/Null, or a table
     DevBuf<int> pair_restr;
     DevBuf<double> pair_s14;
     std::vector<PairParam> h_table;
@@ -208,6 +382,9 @@ struct lg_system {
     int ld = 0;
     // energy partials
     DevBuf<double> epart_pairs, epart_mol, epart_k, ke_part, lambda;
+    // molecular virial: centres of mass and per-block partial sums
+    DevBuf<double4> mol_com;
+    DevBuf<double> molvir_part;
     // MD
     int integrator = LG_INTEGRATOR_VELOCITY_VERLET;
     double dt = 1.0, berendsen_T = 0.0, berendsen_tau = 1.0;
@@ -384,18 +561,22 @@ int prepare_topology(lg_system* s) {
     hipStream_t st = S(s);
     std::vector<BondedParam> h_bp, h_ap, h_dp;
     auto kind_of = [&](int a) { return s->kind[a]; };
+    auto bonded = [](const BondedSpec& b) {
+        return BondedParam{b.pot, 0, 0, b.p[0], b.p[1], b.p[2], b.p[3], b.p[4], 0.0};
+    };
+    const BondedParam none{POT_NONE, 0, 0, 0, 0, 0, 0, 0, 0};
     auto find_bond = [&](int a, int b) -> BondedParam {
         uint32_t x = kind_of(a), y = kind_of(b);
         auto it = s->bond_pots.find({std::min(x, y), std::max(x, y)});
-        if (it == s->bond_pots.end()) return {POT_NONE, 0, 0};
-        return {it->second.pot, it->second.p0, it->second.p1};
+        if (it == s->bond_pots.end()) return none;
+        return bonded(it->second);
     };
     auto find_angle = [&](int a, int b, int c) -> BondedParam {
         uint32_t x = kind_of(a), y = kind_of(b), z = kind_of(c);
         if (x > z) std::swap(x, z);
         auto it = s->angle_pots.find({x, y, z});
-        if (it == s->angle_pots.end()) return {POT_NONE, 0, 0};
-        return {it->second.pot, it->second.p0, it->second.p1};
+        if (it == s->angle_pots.end()) return none;
+        return bonded(it->second);
     };
     auto dkey = [](uint32_t i, uint32_t j, uint32_t k, uint32_t m) {
         uint32_t ij = std::max(i, j), km = std::max(k, m);
@@ -408,8 +589,8 @@ int prepare_topology(lg_system* s) {
     };
     auto find_dihedral = [&](int a, int b, int c, int d) -> BondedParam {
         auto it = s->dihedral_pots.find(dkey(kind_of(a), kind_of(b), kind_of(c), kind_of(d)));
-        if (it == s->dihedral_pots.end()) return {POT_NONE, 0, 0};
-        return {it->second.pot, it->second.p0, it->second.p1};
+        if (it == s->dihedral_pots.end()) return none;
+        return bonded(it->second);
     };
     std::vector<Term> h_terms;
     h_terms.reserve(ip.size() + s->bonds.size() + s->angles.size() + s->dihedrals.size());
@@ -528,25 +709,45 @@ static double fit_erfcx(double xmax, int deg, std::vector<T>& out) {
 int prepare_params(lg_system* s) {
     uint32_t K = s->nkinds;
     if (K > uint32_t(kMaxKinds)) return fail(LG_E_UNSUPPORTED, "more than 16 particle kinds");
-    s->h_table.assign(size_t(K) * K, PairParam{POT_NONE, 0, 0, 0, 0, 0});
+    s->h_table.assign(size_t(K) * K, PairParam{POT_NONE, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0});
     std::vector<int> restr(size_t(K) * K, 0);
     std::vector<double> s14(size_t(K) * K, 1.0);
+    std::vector<double> tabs;
+    s->gen_pairs = false;
     for (auto& [key, p] : s->pairs) {
+        int tab_off = 0;
+        if (p.v.tab_n > 0) {
+            std::vector<double> te, tf;
+            host_table(p.v, te, tf);
+            tab_off = int(tabs.size());
+            tabs.insert(tabs.end(), te.begin(), te.end());
+            tabs.insert(tabs.end(), tf.begin(), tf.end());
+        }
+        if (!pot_is_fast(p.v.pot, int(p.v.tab_n))) s->gen_pairs = true;
         for (int o = 0; o < 2; o++) {
             uint32_t a = o ? key.second : key.first, b = o ? key.first : key.second;
             if (a >= K || b >= K) return fail(LG_E_INVALID, "pair potential kind out of range");
             PairParam& pp = s->h_table[size_t(a) * K + b];
-            pp.pot = p.pot;
+            pp.pot = p.v.pot;
             pp.inter_excl = p.restriction == LG_RESTRICT_INTRA ? 1 : 0;
-            pp.p0 = p.p0;
-            pp.p1 = p.p1;
+            pp.p0 = p.v.p[0];
+            pp.p1 = p.v.p[1];
+            pp.p2 = p.v.p[2];
+            pp.p3 = p.v.p[3];
+            pp.p4 = p.v.p[4];
             pp.rc = p.rc;
-            // pairs.rs:86-95: shift = V(cutoff) of the bare potential
-            pp.shift = p.shifted ? host_pot_energy(p.pot, p.p0, p.p1, p.rc) : 0.0;
+            pp.tab_n = int(p.v.tab_n);
+            pp.tab_off = tab_off;
+            pp.tab_delta = p.v.tab_n > 0 ? p.v.tab_max / double(p.v.tab_n) : 0.0;
+            // pairs.rs:86-95: shift = V(cutoff) of the potential (of its
+            // table when tabulated)
+            pp.shift = p.shifted ? host_energy(p.v, p.rc) : 0.0;
             restr[size_t(a) * K + b] = p.restriction;
             s14[size_t(a) * K + b] = p.scale14;
         }
     }
+    if (tabs.empty()) tabs.push_back(0.0);
+    LG_TRY(upload(s->ptabs, tabs, S(s)));
     LG_TRY(upload(s->table, s->h_table, S(s)));
     LG_TRY(upload(s->pair_restr, restr, S(s)));
     LG_TRY(upload(s->pair_s14, s14, S(s)));
@@ -760,7 +961,7 @@ int prepare_kspace(lg_system* s) {
 double max_cutoff(lg_system* s) {
     double rc = 0.0;
     for (auto& [k, p] : s->pairs)
-        if (p.pot != LG_POT_NULL) rc = std::max(rc, p.rc);
+        if (p.v.pot != LG_POT_NULL) rc = std::max(rc, p.rc);
     if (s->coul_kind != COUL_NONE) rc = std::max(rc, s->coul_rc);
     return rc;
 }
@@ -830,6 +1031,7 @@ int prepare_nlist(lg_system* s) {
     // lanes per atom: enough waves to fill 256 CUs, fewer lanes for long lists
     s->tpa = n <= 8192 ? 64 : n <= 32768 ? 32 : n <= 131072 ? 16 : n <= 524288 ? 8 : 4;
     if (s->tune.pair_lanes > 0) s->tpa = s->tune.pair_lanes;
+    if (s->gen_pairs) s->tpa = 16;  // the general pair kernel has one instance
     LG_TRY(s->epart_pairs.ensure(size_t(blocks_for(n * size_t(s->tpa), kPairBlock)) * 8));
     int init[4] = {1, 0, 0, 0};  // force a rebuild; clear overflow
     LG_HIP(hipMemcpyAsync(s->flags.p, init, sizeof(init), hipMemcpyHostToDevice, S(s)));
@@ -1147,6 +1349,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     pa.nnbr = s->nnbr.p;
     pa.cap = s->cap;
     pa.table = s->table.p;
+    pa.tabs = s->ptabs.p;
     pa.nkinds = int(s->nkinds);
     pa.coul = s->coul;
     pa.box = s->box;
@@ -1165,6 +1368,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     ia.nterms = s->nterms;
     ia.terms = s->terms.p;
     ia.table = s->table.p;
+    ia.tabs = s->ptabs.p;
     ia.nkinds = int(s->nkinds);
     ia.pair_restr = s->pair_restr.p;
     ia.pair_s14 = s->pair_s14.p;
@@ -1221,7 +1425,21 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         LG_PAIRS_CK(T, CK_EWALD_ERF)     \
         LG_PAIRS_CK(T, CK_WOLF)          \
         break;
-    switch (s->tpa) {
+    if (s->gen_pairs) {
+        // any pair potential kind or a table: the general FP64 instance, in
+        // both forms
+        if (energy) {
+            if (ck == CK_NONE) k_pairs<true, 16, CK_NONE, true><<<nblocks, kPairBlock, 0, st>>>(pa);
+            if (ck == CK_EWALD) k_pairs<true, 16, CK_EWALD, true><<<nblocks, kPairBlock, 0, st>>>(pa);
+            if (ck == CK_EWALD_ERF) k_pairs<true, 16, CK_EWALD_ERF, true><<<nblocks, kPairBlock, 0, st>>>(pa);
+            if (ck == CK_WOLF) k_pairs<true, 16, CK_WOLF, true><<<nblocks, kPairBlock, 0, st>>>(pa);
+        } else {
+            if (ck == CK_NONE) k_pairs<false, 16, CK_NONE, true><<<nblocks, kPairBlock, 0, st>>>(pa);
+            if (ck == CK_EWALD) k_pairs<false, 16, CK_EWALD, true><<<nblocks, kPairBlock, 0, st>>>(pa);
+            if (ck == CK_EWALD_ERF) k_pairs<false, 16, CK_EWALD_ERF, true><<<nblocks, kPairBlock, 0, st>>>(pa);
+            if (ck == CK_WOLF) k_pairs<false, 16, CK_WOLF, true><<<nblocks, kPairBlock, 0, st>>>(pa);
+        }
+    } else switch (s->tpa) {
         LG_PAIRS_CASE(4)
         LG_PAIRS_CASE(8)
         LG_PAIRS_CASE(16)
@@ -1519,6 +1737,7 @@ McChain mc_chain(lg_system* s, bool cells) {
     c.nmol = s->nmol;
     c.box = s->box;
     c.table = s->table.p;
+    c.tabs = s->ptabs.p;
     c.nkinds = int(s->nkinds);
     c.coul = s->coul;
     if (mc_ewald(s)) {
@@ -1827,18 +2046,9 @@ int lg_system_get_tuning(lg_system* s, lg_tuning* t) {
     return LG_OK;
 }
 
-static int check_pot(int pot) {
-    if (pot != LG_POT_NULL && pot != LG_POT_LJ && pot != LG_POT_HARMONIC)
-        return fail(LG_E_UNSUPPORTED, "potential kind not in the GPU kernel set");
-    return LG_OK;
-}
-
-int lg_pair_set(lg_system* s, uint32_t ki, uint32_t kj, int pot, const double* params, double cutoff, int shifted,
-                int tail, int restriction, double scale14) {
-    if (!s) return fail(LG_E_INVALID, "null system");
-    LG_TRY(check_pot(pot));
+static int set_pair(lg_system* s, uint32_t ki, uint32_t kj, const PotSpec& v, double cutoff, int shifted, int tail,
+                    int restriction, double scale14) {
     if (ki >= s->nkinds || kj >= s->nkinds) return fail(LG_E_INVALID, "kind index out of range");
-    if (pot != LG_POT_NULL && !params) return fail(LG_E_INVALID, "missing potential parameters");
     if (restriction < 0 || restriction > 6) return fail(LG_E_INVALID, "unknown restriction");
     if (!(cutoff >= 0.0) || !std::isfinite(cutoff)) return fail(LG_E_INVALID, "invalid cutoff");
     if (s->periodic) {
@@ -1846,35 +2056,57 @@ int lg_pair_set(lg_system* s, uint32_t ki, uint32_t kj, int pot, const double* p
         if (cutoff > half)
             return fail(LG_E_INVALID, "The cutoff for this pair potential is bigger than half the cell length");
     }
-    PairSpec p{pot, params ? params[0] : 0.0, params ? params[1] : 0.0, cutoff, shifted, tail, restriction, scale14};
+    PairSpec p{v, cutoff, shifted, tail, restriction, scale14};
     s->pairs[{std::min(ki, kj), std::max(ki, kj)}] = p;
     s->params_dirty = true;
     s->nlist_dirty = true;
     return LG_OK;
 }
 
+int lg_pair_set(lg_system* s, uint32_t ki, uint32_t kj, int pot, const double* params, double cutoff, int shifted,
+                int tail, int restriction, double scale14) {
+    if (!s) return fail(LG_E_INVALID, "null system");
+    PotSpec v;
+    LG_TRY(make_pot(pot, params, ROLE_PAIR, v));
+    return set_pair(s, ki, kj, v, cutoff, shifted, tail, restriction, scale14);
+}
+
+int lg_pair_set_table(lg_system* s, uint32_t ki, uint32_t kj, int pot, const double* params, uint64_t table_size,
+                      double table_max, double cutoff, int shifted, int tail, int restriction, double scale14) {
+    if (!s) return fail(LG_E_INVALID, "null system");
+    PotSpec v;
+    LG_TRY(make_pot(pot, params, ROLE_PAIR, v));
+    if (table_size < 1 || table_size > (uint64_t(1) << 24))
+        return fail(LG_E_INVALID, "table size must be in [1, 2^24]");
+    if (!(table_max > 0.0) || !std::isfinite(table_max)) return fail(LG_E_INVALID, "invalid table maximum");
+    v.tab_n = table_size;
+    v.tab_max = table_max;
+    return set_pair(s, ki, kj, v, cutoff, shifted, tail, restriction, scale14);
+}
+
 int lg_bond_set(lg_system* s, uint32_t ki, uint32_t kj, int pot, const double* params) {
-    if (!s || !params) return fail(LG_E_INVALID, "null argument");
-    LG_TRY(check_pot(pot));
-    s->bond_pots[{std::min(ki, kj), std::max(ki, kj)}] = BondedSpec{pot, params[0], params[1]};
+    if (!s) return fail(LG_E_INVALID, "null argument");
+    BondedSpec v;
+    LG_TRY(make_pot(pot, params, ROLE_BOND, v));
+    s->bond_pots[{std::min(ki, kj), std::max(ki, kj)}] = v;
     s->topo_dirty = true;
     return LG_OK;
 }
 
 int lg_angle_set(lg_system* s, uint32_t ki, uint32_t kj, uint32_t kk, int pot, const double* params) {
-    if (!s || !params) return fail(LG_E_INVALID, "null argument");
-    LG_TRY(check_pot(pot));
-    if (pot == LG_POT_LJ) return fail(LG_E_UNSUPPORTED, "Lennard-Jones is not an angle potential");
-    s->angle_pots[{std::min(ki, kk), kj, std::max(ki, kk)}] = BondedSpec{pot, params[0], params[1]};
+    if (!s) return fail(LG_E_INVALID, "null argument");
+    BondedSpec v;
+    LG_TRY(make_pot(pot, params, ROLE_ANGLE, v));
+    s->angle_pots[{std::min(ki, kk), kj, std::max(ki, kk)}] = v;
     s->topo_dirty = true;
     return LG_OK;
 }
 
 int lg_dihedral_set(lg_system* s, uint32_t ki, uint32_t kj, uint32_t kk, uint32_t km, int pot,
                     const double* params) {
-    if (!s || !params) return fail(LG_E_INVALID, "null argument");
-    LG_TRY(check_pot(pot));
-    if (pot == LG_POT_LJ) return fail(LG_E_UNSUPPORTED, "Lennard-Jones is not a dihedral potential");
+    if (!s) return fail(LG_E_INVALID, "null argument");
+    BondedSpec v;
+    LG_TRY(make_pot(pot, params, ROLE_DIHEDRAL, v));
     uint32_t ij = std::max(ki, kj), kmm = std::max(kk, km);
     std::tuple<uint32_t, uint32_t, uint32_t, uint32_t> key;
     if (ij == kmm)
@@ -1883,7 +2115,7 @@ int lg_dihedral_set(lg_system* s, uint32_t ki, uint32_t kj, uint32_t kk, uint32_
         key = std::make_tuple(ki, kj, kk, km);
     else
         key = std::make_tuple(km, kk, kj, ki);
-    s->dihedral_pots[key] = BondedSpec{pot, params[0], params[1]};
+    s->dihedral_pots[key] = v;
     s->topo_dirty = true;
     return LG_OK;
 }
@@ -1993,6 +2225,69 @@ int lg_virial(lg_system* s, double* w9) {
     w[1] += w_tail;
     w[2] += w_tail;
     double m[9] = {w[0], w[3], w[4], w[3], w[1], w[5], w[4], w[5], w[2]};
+    std::memcpy(w9, m, sizeof(m));
+    return LG_OK;
+}
+
+// MolecularVirial (compute.rs:278-364): pairs between molecules weighted by
+// (r_ab . r_ij) / r_ab^2, the tails as the atomic virial, no bond, angle or
+// dihedral term, and for Ewald the real-space pairs between molecules plus
+// the atomic k-space virial minus sum_i F_i (x) (r_i - com) (ewald.rs:736-753).
+int lg_molecular_virial(lg_system* s, double* w9) {
+    if (!s || !w9) return fail(LG_E_INVALID, "null argument");
+    if (!s->periodic) return fail(LG_E_INVALID, "Can not compute virial for infinite cell");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(compute(s, true));
+    Energies E;
+    LG_TRY(gather_energies(s, E));
+    hipStream_t st = S(s);
+    const int n = int(s->n);
+    LG_TRY(s->mol_com.ensure(size_t(s->nmol)));
+    k_mol_com<<<blocks_for(s->nmol, 256), 256, 0, st>>>(s->pos.p, s->vel.p, s->d_mol_start.p, s->nmol,
+                                                        s->mol_com.p);
+    const int nbp = blocks_for(n, 256);
+    const bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
+    LG_TRY(s->molvir_part.ensure(size_t(nbp) * (6 + (kspace ? 9 : 0))));
+    MolVirialArgs a{};
+    a.pos = s->pos.p;
+    a.meta = s->meta.p;
+    a.sorted = s->sorted.p;
+    a.nbr = s->nbr.p;
+    a.nnbr = s->nnbr.p;
+    a.cap = s->cap;
+    a.n = n;
+    a.nkinds = int(s->nkinds);
+    a.table = s->table.p;
+    a.tabs = s->ptabs.p;
+    a.coul = s->coul;
+    a.box = s->box;
+    a.com = s->mol_com.p;
+    a.part = s->molvir_part.p;
+    k_mol_pairs<<<nbp, 256, 0, st>>>(a);
+    if (kspace)
+        k_mol_kspace<<<nbp, 256, 0, st>>>(s->pos.p, s->meta.p, s->fk.p, s->ksplit, n, s->mol_com.p,
+                                          s->molvir_part.p + size_t(nbp) * 6);
+    LG_HIP(hipGetLastError());
+    std::vector<double> h(size_t(nbp) * (6 + (kspace ? 9 : 0)));
+    LG_HIP(hipMemcpyAsync(h.data(), s->molvir_part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipStreamSynchronize(st));
+    double w6[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < nbp; b++)
+        for (int q = 0; q < 6; q++) w6[q] += h[size_t(b) * 6 + q];
+    double e_tail, w_tail;
+    tail_terms(s, &e_tail, &w_tail);
+    double m[9] = {w6[0], w6[3], w6[4], w6[3], w6[1], w6[5], w6[4], w6[5], w6[2]};
+    m[0] += w_tail;
+    m[4] += w_tail;
+    m[8] += w_tail;
+    if (kspace) {
+        double corr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const double* hk = h.data() + size_t(nbp) * 6;
+        for (int b = 0; b < nbp; b++)
+            for (int q = 0; q < 9; q++) corr[q] += hk[size_t(b) * 9 + q];
+        const double wk[9] = {E.wk[0], E.wk[3], E.wk[4], E.wk[3], E.wk[1], E.wk[5], E.wk[4], E.wk[5], E.wk[2]};
+        for (int q = 0; q < 9; q++) m[q] += wk[q] - corr[q];
+    }
     std::memcpy(w9, m, sizeof(m));
     return LG_OK;
 }

@@ -4,6 +4,11 @@
 // The scalar functions restate Lumol's formulas (reference checkout paths):
 //   Lennard-Jones      lumol-core/src/energy/functions.rs:79-89
 //   Harmonic           lumol-core/src/energy/functions.rs:143-151
+//   other potentials   lumol-core/src/energy/functions.rs:198-207 (CosineHarmonic),
+//                      :244-256 (Torsion), :286-300 (Buckingham),
+//                      :353-367 (BornMayerHuggins), :414-424 (Morse),
+//                      :474-482 (Gaussian), :552-566 (Mie)
+//   TableComputation   lumol-core/src/energy/computations.rs:122-146
 //   PairInteraction    lumol-core/src/energy/pairs.rs:185-218 (r >= rc -> 0)
 //   minimum image      lumol-core/src/sys/config/cells.rs:284-300
 //   Ewald real space   lumol-core/src/energy/global/ewald.rs:387-428 (r > rc -> 0)
@@ -21,18 +26,65 @@ constexpr double kFrac2SqrtPi = 1.12837916709551257390;  // 2/sqrt(pi)
 constexpr double kBoltzmann = 8.31446284161522e-7;
 constexpr double kFourPiEps0 = 7.197589831304046;
 
-enum : int { POT_NONE = -1, POT_NULL = 0, POT_LJ = 1, POT_HARMONIC = 2 };
+enum : int {
+    POT_NONE = -1,
+    POT_NULL = 0,
+    POT_LJ = 1,
+    POT_HARMONIC = 2,
+    POT_BUCKINGHAM = 3,
+    POT_BORN = 4,
+    POT_MORSE = 5,
+    POT_GAUSSIAN = 6,
+    POT_MIE = 7,
+    POT_COSINE_HARMONIC = 8,
+    POT_TORSION = 9
+};
 // restrictions.rs:37-50
 enum : int { BP_NONE = 0, BP_SAME = 1, BP_ONE = 2, BP_TWO = 3, BP_THREE = 4, BP_FAR = 5 };
 enum : int { COUL_NONE = 0, COUL_EWALD = 1, COUL_WOLF = 2 };
 
-// One entry per (kind_i, kind_j), symmetric, K*K entries.
+// One entry per (kind_i, kind_j), symmetric, K*K entries.  Parameters in
+// the reference's field order: LJ {sigma, epsilon}; Harmonic {k, x0};
+// Buckingham {a, c, rho}; BornMayerHuggins {a, c, d, sigma, rho};
+// Morse {a, x0, depth}; Gaussian {a, b}; Mie {sigma, n, m, prefac}.
+// tab_n > 0: a TableComputation of the potential, whose energies are
+// tabs[tab_off + i] and forces tabs[tab_off + tab_n + i] at r = i tab_delta.
 struct PairParam {
     int pot;           // POT_* ; POT_NONE when no interaction is set
     int inter_excl;    // 1 when the restriction excludes inter-molecular pairs
-    double p0, p1;     // LJ {sigma, epsilon}; Harmonic {k, x0}
+    double p0, p1;
     double rc;         // PairInteraction cutoff
     double shift;      // V(rc) when shifted, else 0
+    double p2, p3, p4;
+    int tab_n, tab_off;
+    double tab_delta;
+};
+
+// The fields of PairParam that the LJ/Harmonic/Null pair kernels read
+// (the table they stage in LDS).
+struct PairFast {
+    int pot, inter_excl;
+    double p0, p1, rc, shift;
+};
+// Kinds the fast pair kernels evaluate; a system with another kind or a
+// table uses the general pair kernel.
+__host__ __device__ __forceinline__ bool pot_is_fast(int pot, int tab_n) {
+    return tab_n == 0 && pot <= POT_HARMONIC;
+}
+// Marks, in a PairFast, an entry that needs the general evaluation from the
+// full PairParam (the Monte Carlo kernels, which run every kind).
+constexpr int POT_GENERAL = 64;
+__host__ __device__ __forceinline__ PairFast pair_fast(const PairParam& p) {
+    return PairFast{pot_is_fast(p.pot, p.tab_n) ? p.pot : POT_GENERAL, p.inter_excl, p.p0, p.p1, p.rc, p.shift};
+}
+
+// Bond, angle and dihedral potential: Harmonic {k, x0}, Morse {a, x0, depth},
+// CosineHarmonic {k, cos(x0)}, Torsion {k, delta, n}, Null.
+struct BondedParam {
+    int pot;  // POT_* (POT_NONE: no potential)
+    int tab_n, tab_off;  // always 0 (no tables for bonded terms)
+    double p0, p1, p2, p3, p4;
+    double tab_delta;
 };
 
 struct CoulParam {
@@ -111,6 +163,86 @@ __device__ __forceinline__ void pot_eval(int pot, double p0, double p1, double r
         e = 0.0;
         f_over_r = 0.0;
     }
+}
+
+// Every potential kind and TableComputation: energy and force / r (with
+// inv_r = 1 the plain force, -dV/dx, for angles and dihedrals).  Operation
+// order as the reference; P is PairParam or BondedParam.
+template <class P>
+__device__ __forceinline__ void pot_eval_any(const P& q, const double* __restrict__ tabs, double r, double inv_r,
+                                             double& e, double& f_over_r) {
+    if (q.tab_n > 0) {
+        // computations.rs:122-146: `floor(r / delta) as usize` saturates at 0
+        const double u = floor(r / q.tab_delta);
+        const double bin = u > 0.0 ? u : 0.0;  // NaN -> 0
+        e = 0.0;
+        double f = 0.0;
+        if (bin < (double)(q.tab_n - 1)) {
+            const int b = (int)bin;
+            const double dx = r - bin * q.tab_delta;
+            const double* te = tabs + q.tab_off;
+            const double* tf = te + q.tab_n;
+            e = te[b] + dx * ((te[b + 1] - te[b]) / q.tab_delta);
+            f = tf[b] + dx * ((tf[b + 1] - tf[b]) / q.tab_delta);
+        }
+        f_over_r = f * inv_r;
+        return;
+    }
+    double f = 0.0;
+    switch (q.pot) {
+        case POT_LJ: pot_eval(POT_LJ, q.p0, q.p1, r, inv_r, e, f_over_r); return;
+        case POT_HARMONIC: pot_eval(POT_HARMONIC, q.p0, q.p1, r, inv_r, e, f_over_r); return;
+        case POT_BUCKINGHAM: {
+            const double r3 = r * r * r;
+            const double ex = exp(-r / q.p2);
+            e = q.p0 * ex - q.p1 / (r3 * r3);
+            f = q.p0 / q.p2 * ex - 6.0 * q.p1 / (r3 * r3 * r);
+            break;
+        }
+        case POT_BORN: {
+            const double r2 = r * r;
+            const double r6 = r2 * r2 * r2;
+            const double r7 = r6 * r;
+            const double ex = exp((q.p3 - r) / q.p4);
+            e = q.p0 * ex - q.p1 / r6 + q.p2 / (r6 * r2);
+            f = q.p0 / q.p4 * ex - 6.0 * q.p1 / r7 + 8.0 * q.p2 / (r7 * r2);
+            break;
+        }
+        case POT_MORSE: {
+            const double ex = exp((q.p1 - r) * q.p0);
+            const double m = 1.0 - ex;
+            e = q.p2 * m * m;
+            f = 2.0 * q.p2 * (1.0 - ex * ex) * q.p0;
+            break;
+        }
+        case POT_GAUSSIAN: {
+            e = -q.p0 * exp(-q.p1 * r * r);
+            f = 2.0 * q.p1 * r * e;
+            break;
+        }
+        case POT_MIE: {
+            const double sr = q.p0 / r;
+            const double rep = pow(sr, q.p1), att = pow(sr, q.p2);
+            e = q.p3 * (rep - att);
+            f = q.p3 * (q.p1 * rep - q.p2 * att) / r;
+            break;
+        }
+        case POT_COSINE_HARMONIC: {
+            const double c = cos(r);
+            const double dr = c - q.p1;
+            e = 0.5 * q.p0 * dr * dr;
+            f = q.p0 * dr * sin(r);
+            break;
+        }
+        case POT_TORSION: {
+            const double x = q.p2 * r - q.p1;
+            e = q.p0 * (1.0 + cos(x));
+            f = q.p0 * q.p2 * sin(x);
+            break;
+        }
+        default: e = 0.0;
+    }
+    f_over_r = f * inv_r;
 }
 
 // Ewald real-space pair: energy and the factor multiplying d in the force

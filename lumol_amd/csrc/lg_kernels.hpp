@@ -17,6 +17,8 @@
 // brute-force `for j in i+1..n` loops (lumol-core/src/sys/compute.rs:40).
 #pragma once
 
+#include <type_traits>
+
 #include "lg_device.hpp"
 
 namespace lg {
@@ -985,11 +987,6 @@ __global__ __launch_bounds__(kCellBlock) void k_nlist_direct(DirectArgs da) {
 // are deterministic without atomics.  Terms are sorted by type on the host so
 // that waves do not diverge.
 // ---------------------------------------------------------------------------------
-struct BondedParam {
-    int pot;  // POT_* (POT_NONE: no potential)
-    double p0, p1;
-};
-
 enum : int { TERM_PAIR = 0, TERM_BOND = 1, TERM_ANGLE = 2, TERM_DIHEDRAL = 3 };
 
 struct Term {
@@ -1011,6 +1008,7 @@ struct IntraArgs {
     const BondedParam* bparam;
     const BondedParam* aparam;
     const BondedParam* dparam;
+    const double* tabs;  // TableComputation tables of the pair potentials
     CoulParam coul;
     int coul_restr;
     double coul_s14;
@@ -1044,7 +1042,7 @@ __device__ __forceinline__ void intra_block(const IntraArgs& a, int blk) {
                 restriction_info(a.pair_restr[tk], a.pair_s14[tk], path, excl, scaling);
                 if (!excl && r < pp.rc && pp.pot != POT_NULL) {
                     double e, f;
-                    pot_eval(pp.pot, pp.p0, pp.p1, r, inv_r, e, f);
+                    pot_eval_any(pp, a.tabs, r, inv_r, e, f);
                     fs += scaling * f;
                     acc[0] += scaling * (e - pp.shift);
                 }
@@ -1085,7 +1083,7 @@ __device__ __forceinline__ void intra_block(const IntraArgs& a, int blk) {
                 min_image(a.box, dx, dy, dz);
                 double r = sqrt(norm2_ordered(dx, dy, dz));
                 double e, f;
-                pot_eval(bp.pot, bp.p0, bp.p1, r, 1.0 / r, e, f);
+                pot_eval_any(bp, nullptr, r, 1.0 / r, e, f);
                 fx = f * dx;
                 fy = f * dy;
                 fz = f * dz;
@@ -1121,7 +1119,7 @@ __device__ __forceinline__ void intra_block(const IntraArgs& a, int blk) {
                 double theta = acos(cs);
                 double e, f;
                 // with inv_r = 1 pot_eval returns the plain -dV/dtheta
-                pot_eval(ap.pot, ap.p0, ap.p1, theta, 1.0, e, f);
+                pot_eval_any(ap, nullptr, theta, 1.0, e, f);
                 c0 = make_double4(f * d1x, f * d1y, f * d1z, 0.0);
                 c1 = make_double4(-f * (d1x + d3x), -f * (d1y + d3y), -f * (d1z + d3z), 0.0);
                 c2 = make_double4(f * d3x, f * d3y, f * d3z, 0.0);
@@ -1160,7 +1158,7 @@ __device__ __forceinline__ void intra_block(const IntraArgs& a, int blk) {
                 double d3x = k3 * d4x + k4 * d1x, d3y = k3 * d4y + k4 * d1y, d3z = k3 * d4z + k4 * d1z;
                 double phi = atan2(r23n * (vx * r12x + vy * r12y + vz * r12z), ux * vx + uy * vy + uz * vz);
                 double e, f;
-                pot_eval(dp.pot, dp.p0, dp.p1, phi, 1.0, e, f);
+                pot_eval_any(dp, nullptr, phi, 1.0, e, f);
                 c0 = make_double4(f * d1x, f * d1y, f * d1z, 0.0);
                 c1 = make_double4(f * d2x, f * d2y, f * d2z, 0.0);
                 c2 = make_double4(f * d3x, f * d3y, f * d3z, 0.0);
@@ -1317,6 +1315,7 @@ struct PairArgs {
     const double* ecx64;
     const float* ecx32;
     double ecx_inv_h;
+    const double* tabs;  // TableComputation tables (general instance)
 };
 
 constexpr int kMaxKinds = 16;
@@ -1340,9 +1339,13 @@ __host__ __forceinline__ int coulomb_variant(const CoulParam& c) {
 // Held to 128 VGPRs (four waves per SIMD instead of three): the kernel waits
 // on its gathers most of the time, and the extra resident wave outweighs the
 // few spilled registers (SPC/E FP64 pairs 44 -> 40 us per launch).
-template <bool kEnergy, int TPA, int CK>
+// kGen: the general instance for systems with a pair potential other than
+// LJ/Harmonic/Null or a TableComputation (any kind, FP64, both forms); the
+// fast instances stage only the LJ/Harmonic fields.
+template <bool kEnergy, int TPA, int CK, bool kGen = false>
 __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pairs(PairArgs a) {
-    __shared__ PairParam tab[kMaxKinds * kMaxKinds];
+    using TabEntry = typename std::conditional<kGen, PairParam, PairFast>::type;
+    __shared__ TabEntry tab[kMaxKinds * kMaxKinds];
     __shared__ double red[kPairBlock / 64][8];
     constexpr bool kTab = CK == CK_EWALD || CK == CK_WOLF;
     __shared__ double ecx[kTab ? kErfcxPieces * (kErfcxDeg64 + 1) : 1];
@@ -1356,7 +1359,12 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(4)))
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.rebuild) *a.rebuild = 0;
     int nk2 = a.nkinds * a.nkinds;
-    for (int x = threadIdx.x; x < nk2; x += blockDim.x) tab[x] = a.table[x];
+    for (int x = threadIdx.x; x < nk2; x += blockDim.x) {
+        if constexpr (kGen)
+            tab[x] = a.table[x];
+        else
+            tab[x] = pair_fast(a.table[x]);
+    }
     const bool use_tab = kTab && a.ecx64 != nullptr;  // uniform
     if (use_tab)
         for (int x = threadIdx.x; x < kErfcxPieces * (kErfcxDeg64 + 1); x += blockDim.x) ecx[x] = a.ecx64[x];
@@ -1395,10 +1403,13 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(4)))
             double inv_r = 1.0 / r;
             double fs = 0.0;
             double e_p = 0.0, e_c = 0.0;
-            const PairParam& pp = tab[ki * a.nkinds + kj];
+            const TabEntry& pp = tab[ki * a.nkinds + kj];
             if (pp.pot > POT_NULL && !pp.inter_excl && r < pp.rc) {
                 double e, f;
-                pot_eval(pp.pot, pp.p0, pp.p1, r, inv_r, e, f);
+                if constexpr (kGen)
+                    pot_eval_any(pp, a.tabs, r, inv_r, e, f);
+                else
+                    pot_eval(pp.pot, pp.p0, pp.p1, r, inv_r, e, f);
                 fs += f;
                 e_p = e - pp.shift;
             }
@@ -1717,6 +1728,142 @@ __global__ void k_force_combine(double4* __restrict__ force, const double4* __re
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     force[i] = total_force(i, n, force, contrib, slot_start, fk, ksplit, pos);
+}
+
+// ---------------------------------------------------------------------------------
+// MolecularVirial (compute.rs:278-364).  k_mol_com: centre of mass of each
+// molecule, masses and positions summed in atom order (molecules.rs:256-263).
+// k_mol_pairs: every inter-molecular pair of the Verlet list once (the atom
+// of the lower-numbered molecule owns it) adds w_ab (r_ab . r_ij) / r_ab^2,
+// w_ab being the pair virial of the pair potential (compute.rs:289-307), the
+// Ewald real space (ewald.rs:533-570) or Wolf (wolf.rs:285-324) and r_ij the
+// minimum image of the two centres of mass.  k_mol_kspace: the k-space
+// correction sum_i F_i (x) (r_i - com) of ewald.rs:736-753.  Both write
+// per-block partial sums that the host adds in block order.
+// ---------------------------------------------------------------------------------
+__global__ void k_mol_com(const double4* __restrict__ pos, const double4* __restrict__ vel,
+                          const int* __restrict__ mol_start, int nmol, double4* __restrict__ com) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    double total = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int i = mol_start[m]; i < mol_start[m + 1]; i++) {
+        const double4 p = pos[i];
+        const double mass = vel[i].w;
+        total += mass;
+        cx += mass * p.x;
+        cy += mass * p.y;
+        cz += mass * p.z;
+    }
+    com[m] = make_double4(cx / total, cy / total, cz / total, total);
+}
+
+struct MolVirialArgs {
+    const double4* pos;
+    const int2* meta;
+    const int* sorted;
+    const int* nbr;
+    const int* nnbr;
+    int cap, n, nkinds;
+    const PairParam* table;
+    const double* tabs;
+    CoulParam coul;
+    Box box;
+    const double4* com;
+    double* part;  // 6 per block: xx, yy, zz, xy, xz, yz
+};
+
+__device__ __forceinline__ void block_sum_store(double* v, int nv, double* out) {
+    __shared__ double red[4][9];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int q = 0; q < nv; q++) {
+        double sum = wave_sum(v[q]);
+        if (lane == 0) red[wid][q] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x < nv) {
+        double sum = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) sum += red[w][threadIdx.x];
+        out[blockIdx.x * nv + threadIdx.x] = sum;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mol_pairs(MolVirialArgs a) {
+    double w[6] = {0, 0, 0, 0, 0, 0};
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < a.n) {
+        const int i = a.sorted[t];
+        const double4 pi = a.pos[i];
+        const int2 mi = a.meta[i];
+        const double4 ci = a.com[mi.y];
+        const int* row = a.nbr + (size_t)t * a.cap;
+        const int nn = a.nnbr[t];
+        for (int k = 0; k < nn; k++) {
+            const int e = row[k];
+            const int j = nbr_atom(e);
+            const int mj = a.meta[j].y;
+            if (mj <= mi.y) continue;
+            const double4 pj = a.pos[j];
+            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            min_image(a.box, dx, dy, dz);
+            const double r2 = norm2_ordered(dx, dy, dz);
+            const double r = sqrt(r2);
+            const double inv_r = 1.0 / r;
+            double fs = 0.0;
+            const PairParam& pp = a.table[mi.x * a.nkinds + nbr_kind(e)];
+            if (pp.pot > POT_NULL && !pp.inter_excl && r < pp.rc) {
+                double en, f;
+                pot_eval_any(pp, a.tabs, r, inv_r, en, f);
+                fs += f;
+            }
+            const double qq = pi.w * pj.w;
+            if (a.coul.kind != COUL_NONE && qq != 0.0 && r <= a.coul.rc) {
+                double en, f;
+                if (a.coul.kind == COUL_EWALD) {
+                    ewald_real(a.coul.alpha, qq, r, inv_r, a.coul.inter_excl != 0, en, f);
+                    fs += f;
+                } else if (!a.coul.inter_excl) {
+                    wolf_pair(a.coul, qq, r, inv_r, en, f);
+                    fs += f;
+                }
+            }
+            if (fs == 0.0) continue;
+            const double4 cj = a.com[mj];
+            double rx = ci.x - cj.x, ry = ci.y - cj.y, rz = ci.z - cj.z;
+            min_image(a.box, rx, ry, rz);
+            const double wgt = (dx * rx + dy * ry + dz * rz) / r2;
+            w[0] += fs * dx * dx * wgt;
+            w[1] += fs * dy * dy * wgt;
+            w[2] += fs * dz * dz * wgt;
+            w[3] += fs * dx * dy * wgt;
+            w[4] += fs * dx * dz * wgt;
+            w[5] += fs * dy * dz * wgt;
+        }
+    }
+    block_sum_store(w, 6, a.part);
+}
+
+__global__ __launch_bounds__(256) void k_mol_kspace(const double4* __restrict__ pos, const int2* __restrict__ meta,
+                                                     const double4* __restrict__ fk, int ksplit, int n,
+                                                     const double4* __restrict__ com, double* __restrict__ part) {
+    double c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        double sx = 0, sy = 0, sz = 0;
+        for (int g = 0; g < ksplit; g++) {
+            const double4 p = fk[(size_t)g * n + i];
+            sx += p.x;
+            sy += p.y;
+            sz += p.z;
+        }
+        const double4 p = pos[i];
+        const double fx = (p.w * sx) / kFourPiEps0, fy = (p.w * sy) / kFourPiEps0, fz = (p.w * sz) / kFourPiEps0;
+        const double4 cm = com[meta[i].y];
+        const double d[3] = {p.x - cm.x, p.y - cm.y, p.z - cm.z};
+        const double f[3] = {fx, fy, fz};
+        for (int u = 0; u < 3; u++)
+            for (int v = 0; v < 3; v++) c[3 * u + v] = f[u] * d[v];
+    }
+    block_sum_store(c, 9, part);
 }
 
 // ---------------------------------------------------------------------------------

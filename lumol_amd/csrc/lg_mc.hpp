@@ -72,6 +72,7 @@ struct McChain {
     int n, nmol;
     Box box;
     const PairParam* table;
+    const double* tabs;     // TableComputation tables of the pair potentials
     int nkinds;
     CoulParam coul;
     // Ewald k-space (nk == 0: none)
@@ -110,13 +111,19 @@ struct McChain {
 // Pair energy of an inter-molecular pair at distance r: the pair potential
 // (cut at r >= rc, shifted) and the real-space Coulomb term (cut at r > rc),
 // as k_pairs; restrictions reduce to the flags for "different molecules".
-__device__ __forceinline__ void mc_pair_energy(const PairParam* tab, int nkinds, const CoulParam& c, int ki, int kj,
-                                               double qiqj, double r, double& ep, double& ec) {
+// `tab` holds the fast fields; an entry marked POT_GENERAL is evaluated from
+// the full table `full` (and the TableComputation tables `tabs`).
+__device__ __forceinline__ void mc_pair_energy(const PairFast* tab, const PairParam* full, const double* tabs,
+                                               int nkinds, const CoulParam& c, int ki, int kj, double qiqj, double r,
+                                               double& ep, double& ec) {
     double inv_r = 1.0 / r;
-    const PairParam& pp = tab[ki * nkinds + kj];
+    const PairFast& pp = tab[ki * nkinds + kj];
     if (pp.pot > POT_NULL && !pp.inter_excl && r < pp.rc) {
         double e, f;
-        pot_eval(pp.pot, pp.p0, pp.p1, r, inv_r, e, f);
+        if (pp.pot == POT_GENERAL)
+            pot_eval_any(full[ki * nkinds + kj], tabs, r, inv_r, e, f);
+        else
+            pot_eval(pp.pot, pp.p0, pp.p1, r, inv_r, e, f);
         ep += e - pp.shift;
     }
     if (c.kind != COUL_NONE && qiqj != 0.0 && r <= c.rc) {
@@ -155,15 +162,15 @@ __device__ __forceinline__ double mc_r2_near(const Box& b, double4 p, double4 q)
 // Energy change of one candidate j: e(new) - e(old), each evaluated with the
 // reference's arithmetic when the candidate passes the distance filter.
 template <bool kCoul>
-__device__ __forceinline__ void mc_candidate(const McChain& c, const PairParam* tab, const CoulParam& cp, int ki,
+__device__ __forceinline__ void mc_candidate(const McChain& c, const PairFast* tab, const CoulParam& cp, int ki,
                                              int kj, double4 po, double4 pn, double4 pj, double& d_p,
                                              double& d_c) {
     double qq = kCoul ? po.w * pj.w : 0.0;
     double eo_p = 0, eo_c = 0, en_p = 0, en_c = 0;
     if (mc_r2_near(c.box, po, pj) < c.r2_filter)
-        mc_pair_energy(tab, c.nkinds, cp, ki, kj, qq, mc_pair_r(c.box, po, pj), eo_p, eo_c);
+        mc_pair_energy(tab, c.table, c.tabs, c.nkinds, cp, ki, kj, qq, mc_pair_r(c.box, po, pj), eo_p, eo_c);
     if (mc_r2_near(c.box, pn, pj) < c.r2_filter)
-        mc_pair_energy(tab, c.nkinds, cp, ki, kj, qq, mc_pair_r(c.box, pn, pj), en_p, en_c);
+        mc_pair_energy(tab, c.table, c.tabs, c.nkinds, cp, ki, kj, qq, mc_pair_r(c.box, pn, pj), en_p, en_c);
     d_p += en_p - eo_p;
     if (kCoul) d_c += en_c - eo_c;
 }
@@ -195,7 +202,7 @@ __device__ __forceinline__ int mc_cell_of(const McChain& c, double4 p) {
 }
 
 struct McShared {
-    PairParam tab[kMaxKinds * kMaxKinds];
+    PairFast tab[kMaxKinds * kMaxKinds];
     double4 newp[kMcMaxMol];       // trial positions (then final, on accept)
     double4 oldp[kMcMaxMol];
     double4 committed[kMcMaxMol];  // positions of the last accepted move
@@ -205,7 +212,7 @@ struct McShared {
 };
 
 __device__ __forceinline__ void mc_load_table(const McChain& c, McShared& sh) {
-    for (int x = threadIdx.x; x < c.nkinds * c.nkinds; x += kMcBlock) sh.tab[x] = c.table[x];
+    for (int x = threadIdx.x; x < c.nkinds * c.nkinds; x += kMcBlock) sh.tab[x] = pair_fast(c.table[x]);
 }
 
 // ---- energy change of a move -----------------------------------------------------------

@@ -4,7 +4,10 @@ gfx950 library.
 Names and argument meaning follow the reference so that code written
 against lumol reads the same (paths relative to the reference checkout):
 
-  LennardJones / Harmonic / NullPotential   lumol-core/src/energy/functions.rs
+  LennardJones / Harmonic / NullPotential,
+  Buckingham / BornMayerHuggins / Morse /
+  Gaussian / Mie / CosineHarmonic / Torsion lumol-core/src/energy/functions.rs
+  TableComputation                          lumol-core/src/energy/computations.rs
   PairInteraction (new, shifted, tail)      lumol-core/src/energy/pairs.rs:57-297
   PairRestriction                           lumol-core/src/energy/restrictions.rs:13-33
   Ewald / SharedEwald / Wolf                lumol-core/src/energy/global/{ewald,wolf}.rs
@@ -111,6 +114,109 @@ class Harmonic:
 
     def params(self):
         return (self.k, self.x0)
+
+
+class Buckingham:
+    """V = A exp(-r/rho) - C/r^6 (functions.rs:260-327; its tail virial keeps
+    the reference's constant `+ 8.0`)."""
+    kind = nat.LG_POT_BUCKINGHAM
+
+    def __init__(self, a, c, rho):
+        self.a, self.c, self.rho = float(a), float(c), float(rho)
+
+    def params(self):
+        return (self.a, self.c, self.rho)
+
+
+class BornMayerHuggins:
+    """V = A exp((sigma - r)/rho) - C/r^6 + D/r^8 (functions.rs:330-395)."""
+    kind = nat.LG_POT_BORN
+
+    def __init__(self, a, c, d, sigma, rho):
+        self.a, self.c, self.d, self.sigma, self.rho = float(a), float(c), float(d), float(sigma), float(rho)
+
+    def params(self):
+        return (self.a, self.c, self.d, self.sigma, self.rho)
+
+
+class Morse:
+    """V = depth (1 - exp(a (x0 - x)))^2 (functions.rs:397-437)."""
+    kind = nat.LG_POT_MORSE
+
+    def __init__(self, a, x0, depth):
+        self.a, self.x0, self.depth = float(a), float(x0), float(depth)
+
+    def params(self):
+        return (self.a, self.x0, self.depth)
+
+
+class Gaussian:
+    """V = -a exp(-b x^2), b > 0 (functions.rs:439-499)."""
+    kind = nat.LG_POT_GAUSSIAN
+
+    def __init__(self, a, b):
+        if not b > 0:
+            raise ValueError('"b" has to be positive in Gaussian potential')
+        self.a, self.b = float(a), float(b)
+
+    def params(self):
+        return (self.a, self.b)
+
+
+class Mie:
+    """V = eps n/(n-m) (n/m)^(m/(n-m)) ((s/r)^n - (s/r)^m), m < n
+    (functions.rs:501-592)."""
+    kind = nat.LG_POT_MIE
+
+    def __init__(self, sigma, epsilon, n, m):
+        if not m < n:
+            raise ValueError("The repulsive exponent n has to be larger than the attractive exponent m")
+        self.sigma, self.epsilon, self.n, self.m = float(sigma), float(epsilon), float(n), float(m)
+
+    def params(self):
+        return (self.sigma, self.epsilon, self.n, self.m)
+
+
+class CosineHarmonic:
+    """V = k/2 (cos x - cos x0)^2, angles and dihedrals (functions.rs:162-210)."""
+    kind = nat.LG_POT_COSINE_HARMONIC
+
+    def __init__(self, k, x0):
+        self.k, self.x0 = float(k), float(x0)
+
+    def params(self):
+        return (self.k, self.x0)
+
+
+class Torsion:
+    """V = k (1 + cos(n x - delta)), dihedrals (functions.rs:212-258)."""
+    kind = nat.LG_POT_TORSION
+
+    def __init__(self, k, delta, n):
+        if int(n) != n or n < 0:
+            raise ValueError("the multiplicity of a torsion must be a non-negative integer")
+        self.k, self.delta, self.n = float(k), float(delta), int(n)
+
+    def params(self):
+        return (self.k, self.delta, float(self.n))
+
+
+class TableComputation:
+    """A pair potential evaluated from tables of `size` points on [0, max)
+    with linear interpolation, zero from the last bin on; tails stay those
+    of the potential (computations.rs:70-175)."""
+
+    def __init__(self, potential, size, max):
+        if int(size) < 1:
+            raise ValueError("the table needs at least one point")
+        if not max > 0:
+            raise ValueError("the table maximum must be positive")
+        self.potential = potential
+        self.table = (int(size), float(max))
+        self.kind = potential.kind
+
+    def params(self):
+        return self.potential.params()
 
 
 class PairRestriction:
@@ -267,6 +373,8 @@ class System:
         self.angle_potentials = {}
         self.dihedral_potentials = {}
         self.coulomb = None
+        # DegreesOfFreedom of system.rs:47-50, set by the propagator's setup
+        self.simulated_degrees_of_freedom = "particles"
         self.precision = nat.LG_PRECISION_FP64
         self._handle = None
         self._dirty = True
@@ -443,19 +551,26 @@ class System:
         bonds = np.array(self.bonds, dtype=np.uint32).reshape(-1, 2)
         nat.check(L.lg_system_set_bonds(h, len(bonds), nat.u32ptr(np.ascontiguousarray(bonds))))
         for (na, nb), inter in self.pairs.items():
-            pr = np.array(inter.potential.params(), dtype=float)
-            nat.check(L.lg_pair_set(h, self.kinds[na], self.kinds[nb], inter.potential.kind, nat.dptr(pr),
-                                    inter.cutoff(), int(inter.shifted_), int(inter.tail),
-                                    int(inter.restriction_), float(inter.scale14)))
+            pot = inter.potential
+            pr = np.array(pot.params() or (0.0,), dtype=float)
+            table = getattr(pot, "table", None)
+            if table is not None:
+                nat.check(L.lg_pair_set_table(h, self.kinds[na], self.kinds[nb], pot.kind, nat.dptr(pr),
+                                              table[0], table[1], inter.cutoff(), int(inter.shifted_),
+                                              int(inter.tail), int(inter.restriction_), float(inter.scale14)))
+            else:
+                nat.check(L.lg_pair_set(h, self.kinds[na], self.kinds[nb], pot.kind, nat.dptr(pr),
+                                        inter.cutoff(), int(inter.shifted_), int(inter.tail),
+                                        int(inter.restriction_), float(inter.scale14)))
         for names, pot in self.bond_potentials.items():
-            pr = np.array(pot.params(), dtype=float)
+            pr = np.array(pot.params() or (0.0,), dtype=float)
             nat.check(L.lg_bond_set(h, self.kinds[names[0]], self.kinds[names[1]], pot.kind, nat.dptr(pr)))
         for names, pot in self.angle_potentials.items():
-            pr = np.array(pot.params(), dtype=float)
+            pr = np.array(pot.params() or (0.0,), dtype=float)
             k = [self.kinds[x] for x in names]
             nat.check(L.lg_angle_set(h, k[0], k[1], k[2], pot.kind, nat.dptr(pr)))
         for names, pot in self.dihedral_potentials.items():
-            pr = np.array(pot.params(), dtype=float)
+            pr = np.array(pot.params() or (0.0,), dtype=float)
             k = [self.kinds[x] for x in names]
             nat.check(L.lg_dihedral_set(h, k[0], k[1], k[2], k[3], pot.kind, nat.dptr(pr)))
         if isinstance(self.coulomb, Ewald):
@@ -502,11 +617,33 @@ class System:
     def potential_energy(self):
         return self.energy_terms().total
 
-    def virial(self):
+    def atomic_virial(self):
+        """AtomicVirial (compute.rs:195-255)."""
         h = self._sync()
         out = np.zeros(9)
         nat.check(nat.lib().lg_virial(h, nat.dptr(out)))
         return out.reshape(3, 3)
+
+    def molecular_virial(self):
+        """MolecularVirial (compute.rs:278-364)."""
+        h = self._sync()
+        out = np.zeros(9)
+        nat.check(nat.lib().lg_molecular_virial(h, nat.dptr(out)))
+        return out.reshape(3, 3)
+
+    def virial(self):
+        """Virial (compute.rs:372-381): the molecular definition when the
+        simulated degrees of freedom are molecules (Monte Carlo moves of whole
+        molecules), else the atomic one."""
+        if self.simulated_degrees_of_freedom == "molecules":
+            return self.molecular_virial()
+        return self.atomic_virial()
+
+    def degrees_of_freedom(self):
+        """system.rs:249-255: 3 per particle, or 3 per molecule."""
+        if self.simulated_degrees_of_freedom == "molecules":
+            return 3 * len(self.molecules())
+        return 3 * self.size()
 
     def kinetic_energy(self):
         h = self._sync()
@@ -526,11 +663,38 @@ class System:
     def volume(self):
         return self.cell.volume()
 
-    def pressure(self):
-        """Virial pressure (compute.rs:383-425): (dof kB T + tr W) / (3 V)."""
+    def pressure_at_temperature(self, temperature):
+        """PressureAtTemperature (compute.rs:396-405): (dof kB T + tr W) / (3 V)."""
+        if self.cell.is_infinite():
+            raise ValueError("Can not compute pressure for infinite cell")
+        if not temperature >= 0:
+            raise ValueError("the temperature must be positive")
         w = self.virial()
-        dof = 3 * self.size()
-        return (dof * K_BOLTZMANN * self.temperature() + np.trace(w)) / (3.0 * self.volume())
+        return (self.degrees_of_freedom() * K_BOLTZMANN * temperature + np.trace(w)) / (3.0 * self.volume())
+
+    def pressure(self):
+        """Pressure (compute.rs:418-425) at the instantaneous temperature."""
+        return self.pressure_at_temperature(self.temperature())
+
+    def stress_at_temperature(self, temperature):
+        """StressAtTemperature (compute.rs:442-452): (dof/3 kB T 1 + W) / V."""
+        if self.cell.is_infinite():
+            raise ValueError("Can not compute stress for infinite cell")
+        if not temperature >= 0:
+            raise ValueError("the temperature must be positive")
+        w = self.virial()
+        kinetic = self.degrees_of_freedom() / 3.0 * K_BOLTZMANN * temperature * np.eye(3)
+        return (kinetic + w) / self.volume()
+
+    def stress(self):
+        """Stress (compute.rs:465-478): (sum_i m_i v_i (x) v_i + W) / V."""
+        if self.cell.is_infinite():
+            raise ValueError("Can not compute stress for infinite cell")
+        v = self.velocities
+        kinetic = np.zeros((3, 3))
+        for m, vi in zip(self.masses, v):
+            kinetic += m * np.outer(vi, vi)
+        return (kinetic + self.virial()) / self.volume()
 
     def pairs_within(self, cutoff):
         """Inter-molecular pairs (i < j) with r < cutoff found by the device
@@ -742,9 +906,12 @@ class Simulation:
         h = system._sync()
         md = self.propagator
         if isinstance(md, MonteCarlo):
+            # monte_carlo.rs:247-280: Translate moves whole molecules
+            system.simulated_degrees_of_freedom = "molecules"
             system.cache_init()
             self._setup_for = (id(system), h.value)
             return
+        system.simulated_degrees_of_freedom = "particles"  # molecular_dynamics.rs:53-57
         T, tau = (md.thermostat.temperature, md.thermostat.tau) if md.thermostat else (-1.0, 1.0)
         nat.check(nat.lib().lg_md_setup(h, md.integrator.code, md.integrator.timestep, T, tau))
         self._setup_for = (id(system), h.value)

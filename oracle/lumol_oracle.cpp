@@ -105,56 +105,278 @@ inline Cx polar(double r, double phi) { return {r * std::cos(phi), r * std::sin(
 inline double cnorm2(Cx a) { return a.re * a.re + a.im * a.im; }
 
 // --- potentials ------------------------------------------------------------
-enum PotKind { POT_NULL = 0, POT_LJ = 1, POT_HARMONIC = 2 };
+enum PotKind {
+    POT_NULL = 0,
+    POT_LJ = 1,
+    POT_HARMONIC = 2,
+    POT_BUCKINGHAM = 3,
+    POT_BORN = 4,
+    POT_MORSE = 5,
+    POT_GAUSSIAN = 6,
+    POT_MIE = 7,
+    POT_COSINE_HARMONIC = 8,
+    POT_TORSION = 9
+};
 
+// Parameters, in the reference's field order:
+//   LJ {sigma, epsilon}; Harmonic {k, x0}; Buckingham {a, c, rho};
+//   BornMayerHuggins {a, c, d, sigma, rho}; Morse {a, x0, depth};
+//   Gaussian {a, b}; Mie {sigma, n, m, prefac} (prefac from Mie::new);
+//   CosineHarmonic {k, cos_x0} (from CosineHarmonic::new); Torsion {k, delta, n}.
+// A TableComputation (computations.rs:70-175) wraps any pair potential: the
+// tables hold energy/force at r = i * delta, and `tabulated` switches
+// energy/force to the table while the tails stay with the wrapped potential.
 struct Potential {
     int kind = POT_NULL;
-    double p0 = 0, p1 = 0;  // LJ: sigma, epsilon.  Harmonic: k, x0.
+    double p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0;
+    bool tabulated = false;
+    double t_delta = 0, t_max = 0;
+    std::vector<double> t_energy, t_force;
 
-    // functions.rs:79-89 (LJ), :143-151 (Harmonic), :31-38 (Null)
-    double energy(double r) const {
-        if (kind == POT_LJ) {
-            double x = p0 / r;
-            double x2 = x * x;
-            double x4 = x2 * x2;
-            double s6 = x2 * x4;  // powi(x, 6) by binary exponentiation
-            return 4.0 * p1 * (s6 * s6 - s6);
-        } else if (kind == POT_HARMONIC) {
-            double dx = r - p1;
-            return 0.5 * p0 * dx * dx;
+    // functions.rs:79-89 (LJ), :135-145 (Harmonic), :31-38 (Null),
+    // :198-207 (CosineHarmonic), :244-256 (Torsion), :286-300 (Buckingham),
+    // :353-367 (BornMayerHuggins), :414-424 (Morse), :474-482 (Gaussian),
+    // :552-566 (Mie)
+    double direct_energy(double r) const {
+        switch (kind) {
+            case POT_LJ: {
+                double x = p0 / r;
+                double x2 = x * x;
+                double x4 = x2 * x2;
+                double s6 = x2 * x4;  // powi(x, 6) by binary exponentiation
+                return 4.0 * p1 * (s6 * s6 - s6);
+            }
+            case POT_HARMONIC: {
+                double dx = r - p1;
+                return 0.5 * p0 * dx * dx;
+            }
+            case POT_BUCKINGHAM: {
+                double r3 = r * r * r;
+                double r6 = r3 * r3;
+                double ex = std::exp(-r / p2);
+                return p0 * ex - p1 / r6;
+            }
+            case POT_BORN: {
+                double r2 = r * r;
+                double r6 = r2 * r2 * r2;
+                double ex = std::exp((p3 - r) / p4);
+                return p0 * ex - p1 / r6 + p2 / (r6 * r2);
+            }
+            case POT_MORSE: {
+                double rc = 1.0 - std::exp((p1 - r) * p0);
+                return p2 * rc * rc;
+            }
+            case POT_GAUSSIAN: return -p0 * std::exp(-p1 * r * r);
+            case POT_MIE: {
+                double sigma_r = p0 / r;
+                double rep = std::pow(sigma_r, p1);
+                double att = std::pow(sigma_r, p2);
+                return p3 * (rep - att);
+            }
+            case POT_COSINE_HARMONIC: {
+                double dr = std::cos(r) - p1;
+                return 0.5 * p0 * dr * dr;
+            }
+            case POT_TORSION: {
+                double c = std::cos(p2 * r - p1);
+                return p0 * (1.0 + c);
+            }
         }
         return 0.0;
     }
-    double force(double r) const {
-        if (kind == POT_LJ) {
-            double x = p0 / r;
-            double x2 = x * x;
-            double x4 = x2 * x2;
-            double s6 = x2 * x4;
-            return -24.0 * p1 * (s6 - 2.0 * (s6 * s6)) / r;
-        } else if (kind == POT_HARMONIC) {
-            return p0 * (p1 - r);
+    double direct_force(double r) const {
+        switch (kind) {
+            case POT_LJ: {
+                double x = p0 / r;
+                double x2 = x * x;
+                double x4 = x2 * x2;
+                double s6 = x2 * x4;
+                return -24.0 * p1 * (s6 - 2.0 * (s6 * s6)) / r;
+            }
+            case POT_HARMONIC: return p0 * (p1 - r);
+            case POT_BUCKINGHAM: {
+                double r3 = r * r * r;
+                double r7 = r3 * r3 * r;
+                double ex = std::exp(-r / p2);
+                return p0 / p2 * ex - 6.0 * p1 / r7;
+            }
+            case POT_BORN: {
+                double r2 = r * r;
+                double r7 = r2 * r2 * r2 * r;
+                double ex = std::exp((p3 - r) / p4);
+                return p0 / p4 * ex - 6.0 * p1 / r7 + 8.0 * p2 / (r7 * r2);
+            }
+            case POT_MORSE: {
+                double ex = std::exp((p1 - r) * p0);
+                return 2.0 * p2 * (1.0 - ex * ex) * p0;
+            }
+            case POT_GAUSSIAN: return 2.0 * p1 * r * direct_energy(r);
+            case POT_MIE: {
+                double sigma_r = p0 / r;
+                double rep = std::pow(sigma_r, p1);
+                double att = std::pow(sigma_r, p2);
+                return p3 * (p1 * rep - p2 * att) / r;
+            }
+            case POT_COSINE_HARMONIC: return p0 * (std::cos(r) - p1) * std::sin(r);
+            case POT_TORSION: return p0 * p2 * std::sin(p2 * r - p1);
         }
         return 0.0;
     }
-    // functions.rs:92-106
+    // computations.rs:122-146: linear interpolation in the bin of r; zero
+    // from the last bin on.  `floor(r / delta) as usize` saturates at 0.
+    static double table_lookup(const std::vector<double>& t, double delta, double r) {
+        double q = std::floor(r / delta);
+        size_t bin = q > 0.0 ? size_t(q) : 0;  // NaN and negatives -> 0
+        if (q >= 1e18) return 0.0;
+        if (bin < t.size() - 1) {
+            double dx = r - double(bin) * delta;
+            double slope = (t[bin + 1] - t[bin]) / delta;
+            return t[bin] + dx * slope;
+        }
+        return 0.0;
+    }
+    double energy(double r) const { return tabulated ? table_lookup(t_energy, t_delta, r) : direct_energy(r); }
+    double force(double r) const { return tabulated ? table_lookup(t_force, t_delta, r) : direct_force(r); }
+    // computations.rs:96-117 (TableComputation::new)
+    void make_table(size_t size, double max) {
+        t_delta = max / double(size);
+        t_max = max;
+        t_energy.resize(size);
+        t_force.resize(size);
+        for (size_t i = 0; i < size; i++) {
+            double r = double(i) * t_delta;
+            t_energy[i] = direct_energy(r);
+            t_force[i] = direct_force(r);
+        }
+        tabulated = true;
+    }
+    // functions.rs:92-106 (LJ), :302-327 (Buckingham, with its `+ 8.0`),
+    // :369-395 (BornMayerHuggins), :484-499 (Gaussian), :568-592 (Mie);
+    // zero for the others (Null, Harmonic, Morse).  A table keeps the tails
+    // of the potential it wraps (computations.rs:148-175).
     double tail_energy(double rc) const {
-        if (kind != POT_LJ) return 0.0;
-        double s3 = p0 * p0 * p0;
-        double rc3 = rc * rc * rc;
-        double s9 = s3 * s3 * s3;
-        double rc9 = rc3 * rc3 * rc3;
-        return 4.0 / 3.0 * p1 * s3 * (1.0 / 3.0 * s9 / rc9 - s3 / rc3);
+        switch (kind) {
+            case POT_LJ: {
+                double s3 = p0 * p0 * p0;
+                double rc3 = rc * rc * rc;
+                double s9 = s3 * s3 * s3;
+                double rc9 = rc3 * rc3 * rc3;
+                return 4.0 / 3.0 * p1 * s3 * (1.0 / 3.0 * s9 / rc9 - s3 / rc3);
+            }
+            case POT_BUCKINGHAM: {
+                double rc2 = rc * rc;
+                double rc3 = rc2 * rc;
+                double ex = std::exp(-rc / p2);
+                double factor = rc2 - 2.0 * rc * p2 + 2.0 * p2 * p2;
+                return p0 * p2 * ex * factor - p1 / (3.0 * rc3);
+            }
+            case POT_BORN: {
+                double rc2 = rc * rc;
+                double rc3 = rc2 * rc;
+                double ex = std::exp((p3 - rc) / p4);
+                double factor = rc2 - 2.0 * rc * p4 + 2.0 * p4 * p4;
+                return p0 * p4 * ex * factor - p1 / (3.0 * rc3) + p2 / (5.0 * rc2 * rc3);
+            }
+            case POT_GAUSSIAN:
+                return direct_energy(rc) * rc / (2.0 * p1) -
+                       p0 * std::sqrt(PI) * std::erfc(std::sqrt(p1) * rc) / (4.0 * std::pow(p1, 3.0 / 2.0));
+            case POT_MIE: {
+                if (p2 <= 3.0) return 0.0;
+                double sigma_rc = p0 / rc;
+                double n_3 = p1 - 3.0;
+                double m_3 = p2 - 3.0;
+                double rep = std::pow(sigma_rc, n_3);
+                double att = std::pow(sigma_rc, m_3);
+                return p3 * (p0 * p0 * p0) * (rep / n_3 - att / m_3);
+            }
+        }
+        return 0.0;
     }
     double tail_virial(double rc) const {
-        if (kind != POT_LJ) return 0.0;
-        double s3 = p0 * p0 * p0;
-        double rc3 = rc * rc * rc;
-        double s9 = s3 * s3 * s3;
-        double rc9 = rc3 * rc3 * rc3;
-        return 8.0 * p1 * s3 * (2.0 / 3.0 * s9 / rc9 - s3 / rc3);
+        switch (kind) {
+            case POT_LJ: {
+                double s3 = p0 * p0 * p0;
+                double rc3 = rc * rc * rc;
+                double s9 = s3 * s3 * s3;
+                double rc9 = rc3 * rc3 * rc3;
+                return 8.0 * p1 * s3 * (2.0 / 3.0 * s9 / rc9 - s3 / rc3);
+            }
+            case POT_BUCKINGHAM: {
+                double rc2 = rc * rc;
+                double rc3 = rc2 * rc;
+                double ex = std::exp(-rc / p2);
+                double factor = rc3 + 3.0 * rc2 * p2 + 6.0 * rc * p2 * p2 + 6.0 * p2 * p2 * p2;
+                return p0 * ex * factor - 20.0 * p1 / rc3 + 8.0;
+            }
+            case POT_BORN: {
+                double rc2 = rc * rc;
+                double rc3 = rc2 * rc;
+                double ex = std::exp((p3 - rc) / p4);
+                double factor = rc3 + 3.0 * rc2 * p4 + 6.0 * rc * p4 * p4 + 6.0 * p4 * p4 * p4;
+                return p0 * ex * factor - 20.0 * p1 / rc3 + 8.0 * p2 / (5.0 * rc2 * rc3);
+            }
+            case POT_GAUSSIAN:
+                return 3.0 * std::sqrt(PI) * p0 * std::erfc(std::sqrt(p1) * rc) / (4.0 * std::pow(p1, 3.0 / 2.0)) -
+                       direct_energy(rc) * rc * (2.0 * p1 * rc * rc + 3.0) / (2.0 * p1);
+            case POT_MIE: {
+                if (p2 <= 3.0) return 0.0;
+                double sigma_rc = p0 / rc;
+                double n_3 = p1 - 3.0;
+                double m_3 = p2 - 3.0;
+                double rep = std::pow(sigma_rc, n_3);
+                double att = std::pow(sigma_rc, m_3);
+                return p3 * (p0 * p0 * p0) * (rep * p1 / n_3 - att * p2 / m_3);
+            }
+        }
+        return 0.0;
     }
 };
+
+// Parameters as the C ABI passes them -> the reference's fields.  Returns
+// false for invalid parameters, where the reference's constructors assert
+// (Gaussian::new b > 0, functions.rs:465; Mie::new m < n, :544).
+bool set_potential(Potential& p, int kind, const double* a) {
+    p = Potential{};
+    p.kind = kind;
+    switch (kind) {
+        case POT_NULL: return true;
+        case POT_LJ: case POT_HARMONIC: case POT_GAUSSIAN:
+            p.p0 = a[0];
+            p.p1 = a[1];
+            return kind != POT_GAUSSIAN || p.p1 > 0.0;
+        case POT_BUCKINGHAM: case POT_MORSE:
+            p.p0 = a[0];
+            p.p1 = a[1];
+            p.p2 = a[2];
+            return true;
+        case POT_BORN:
+            p.p0 = a[0];
+            p.p1 = a[1];
+            p.p2 = a[2];
+            p.p3 = a[3];
+            p.p4 = a[4];
+            return true;
+        case POT_MIE: {  // a = {sigma, epsilon, n, m}; functions.rs:543-552
+            double sigma = a[0], eps = a[1], n = a[2], m = a[3];
+            if (!(m < n)) return false;
+            p.p0 = sigma;
+            p.p1 = n;
+            p.p2 = m;
+            p.p3 = n / (n - m) * std::pow(n / m, m / (n - m)) * eps;
+            return true;
+        }
+        case POT_COSINE_HARMONIC:  // a = {k, x0}; functions.rs:189-196
+            p.p0 = a[0];
+            p.p1 = std::cos(a[1]);
+            return true;
+        case POT_TORSION:  // a = {k, delta, n}
+            p.p0 = a[0];
+            p.p1 = a[1];
+            p.p2 = double(uint64_t(a[2]));
+            return a[2] >= 0.0 && a[2] == std::floor(a[2]);
+    }
+    return false;
+}
 
 // restrictions.rs:13-33
 enum Restriction {
@@ -1066,6 +1288,124 @@ M3 atomic_virial(System& s) {
     return w;
 }
 
+// --- MolecularVirial (compute.rs:278-364) ------------------------------------------
+// molecules.rs:256-263: centre of mass, masses and positions summed in atom order
+V3 center_of_mass(const System& s, size_t m) {
+    double total = 0.0;
+    V3 com;
+    for (size_t i = s.mol_start[m]; i < s.mol_start[m + 1]; i++) {
+        total += s.mass[i];
+        com += s.mass[i] * s.pos[i];
+    }
+    return com / total;
+}
+
+// w_ab * (r_ab . r_ij) / |r_ab|^2 for one atom pair of two molecules
+inline M3 molecular_weight(const M3& w_ab, V3 r_ab, V3 r_ij) {
+    double d = dot(r_ab, r_ij);
+    double n2 = norm2(r_ab);
+    M3 r;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) r.m[a][b] = w_ab.m[a][b] * d / n2;
+    return r;
+}
+
+// Loop of compute.rs:285-310 (pairs), ewald.rs:533-570 (real space) and
+// wolf.rs:285-324 over molecule pairs i < j and their atoms; `pair` returns
+// the virial w_ab of one atom pair.
+template <class F>
+M3 molecule_pairs_virial(const System& s, F&& pair) {
+    size_t nmol = s.mol_start.size() - 1;
+    std::vector<V3> com(nmol);
+    for (size_t m = 0; m < nmol; m++) com[m] = center_of_mass(s, m);
+    int nt = std::max(1, s.nthreads);
+    std::vector<M3> partial(nt);
+    parallel_for_i(nt, nmol, [&](int t, size_t i0, size_t n_, size_t step) {
+        M3 acc;
+        for (size_t i = i0; i < n_; i += step) {
+            M3 local;
+            for (size_t j = i + 1; j < nmol; j++) {
+                V3 r_ij = com[i] - com[j];
+                s.vector_image(r_ij);
+                for (size_t a = s.mol_start[i]; a < s.mol_start[i + 1]; a++)
+                    for (size_t b = s.mol_start[j]; b < s.mol_start[j + 1]; b++) {
+                        V3 r_ab = s.nearest_image(a, b);
+                        M3 w;
+                        if (pair(a, b, r_ab, w)) local += molecular_weight(w, r_ab, r_ij);
+                    }
+            }
+            acc += local;
+        }
+        partial[t] = acc;
+    });
+    M3 w;
+    for (auto& p : partial) w += p;
+    return w;
+}
+
+M3 molecular_virial(System& s) {
+    // pair potentials (compute.rs:285-310)
+    M3 w = molecule_pairs_virial(s, [&](size_t a, size_t b, V3 r_ab, M3& out) {
+        const PairInteraction* p = s.pair_potential(a, b);
+        if (!p) return false;
+        RInfo info = information(p->restriction, p->scale14, s.bond_path(a, b));
+        if (info.excluded) return false;
+        out = scale(info.scaling, p->virial(r_ab));
+        return true;
+    });
+    // tails (compute.rs:313-322), as the atomic virial's
+    double volume = s.volume();
+    auto comp = composition(s);
+    for (auto& [ki, ni] : comp)
+        for (auto& [kj, nj] : comp) {
+            double two_pi_density = 2.0 * PI * ni * nj / volume;
+            auto it = s.pairs.find({std::min(ki, kj), std::max(ki, kj)});
+            if (it != s.pairs.end()) {
+                double diag = it->second.tail_virial_scalar() * (1.0 / 3.0);
+                for (int k = 0; k < 3; k++) w.m[k][k] += two_pi_density * diag;
+            }
+        }
+    // bonds (compute.rs:325-345) only warn; angles and dihedrals do not enter
+    if (s.coulomb == 1) {
+        Ewald& ew = s.ewald;
+        ewald_prepare(s, ew);
+        // ewald.rs:533-570: every charged pair, restriction info for the real-space force
+        M3 real = molecule_pairs_virial(s, [&](size_t a, size_t b, V3 r_ab, M3& out) {
+            double qa = s.charge[a], qb = s.charge[b];
+            if (qa == 0.0 || qb == 0.0) return false;
+            RInfo info = information(ew.restriction, ew.scale14, s.bond_path(a, b));
+            V3 force = ewald_real_force_pair(ew, info, qa * qb, norm(r_ab)) * r_ab;
+            out = tensorial(force, r_ab);
+            return true;
+        });
+        // ewald.rs:736-753: atomic k-space virial minus sum_i F_i (x) (r_i - com)
+        M3 k = ewald_k_virial(s, ew);
+        std::vector<V3> fk(s.n);
+        ewald_k_forces(s, ew, fk);
+        M3 corr;
+        for (size_t m = 0; m + 1 < s.mol_start.size(); m++) {
+            V3 com = center_of_mass(s, m);
+            for (size_t i = s.mol_start[m]; i < s.mol_start[m + 1]; i++) corr += tensorial(fk[i], s.pos[i] - com);
+        }
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) k.m[a][b] = k.m[a][b] - corr.m[a][b];
+        real += k;
+        w += real;
+    } else if (s.coulomb == 2) {
+        const Wolf& wf = s.wolf;
+        w += molecule_pairs_virial(s, [&](size_t a, size_t b, V3 r_ab, M3& out) {
+            double qa = s.charge[a], qb = s.charge[b];
+            if (qa == 0.0 || qb == 0.0) return false;
+            RInfo info = information(wf.restriction, wf.scale14, s.bond_path(a, b));
+            if (info.excluded) return false;
+            V3 force = (info.scaling * wolf_force_pair(wf, qa * qb, norm(r_ab))) * r_ab;
+            out = tensorial(force, r_ab);
+            return true;
+        });
+    }
+    return w;
+}
+
 double kinetic_energy(const System& s) {
     double e = 0.0;
     for (size_t i = 0; i < s.n; i++) e += 0.5 * s.mass[i] * norm2(s.vel[i]);
@@ -1520,9 +1860,10 @@ int lo_set_pair(lo_system* h, uint32_t ki, uint32_t kj, int pot, const double* p
                 int shifted, int tail, int restriction, double scale14) {
     System& s = S(h);
     PairInteraction p;
-    p.pot.kind = pot;
-    p.pot.p0 = params ? params[0] : 0.0;
-    p.pot.p1 = params ? params[1] : 0.0;
+    if (!set_potential(p.pot, pot, params)) {
+        g_error = "invalid pair potential parameters";
+        return 1;
+    }
     p.cutoff = cutoff;
     p.shifted = shifted != 0;
     p.shift = p.shifted ? p.pot.energy(cutoff) : 0.0;  // pairs.rs:86-95
@@ -1532,28 +1873,42 @@ int lo_set_pair(lo_system* h, uint32_t ki, uint32_t kj, int pot, const double* p
     s.pairs[{std::min(ki, kj), std::max(ki, kj)}] = p;
     return 0;
 }
+// PairInteraction over TableComputation::new(potential, size, max)
+// (computations.rs:96-117); the shift is the table's energy at the cutoff.
+int lo_set_pair_table(lo_system* h, uint32_t ki, uint32_t kj, int pot, const double* params, uint64_t size,
+                      double max, double cutoff, int shifted, int tail, int restriction, double scale14) {
+    System& s = S(h);
+    PairInteraction p;
+    if (!set_potential(p.pot, pot, params) || size < 1 || !(max > 0.0)) {
+        g_error = "invalid tabulated pair potential";
+        return 1;
+    }
+    p.pot.make_table(size_t(size), max);
+    p.cutoff = cutoff;
+    p.shifted = shifted != 0;
+    p.shift = p.shifted ? p.pot.energy(cutoff) : 0.0;
+    p.tail = tail != 0;
+    p.restriction = restriction;
+    p.scale14 = scale14;
+    s.pairs[{std::min(ki, kj), std::max(ki, kj)}] = p;
+    return 0;
+}
 int lo_set_bond_potential(lo_system* h, uint32_t ki, uint32_t kj, int pot, const double* params) {
     Potential p;
-    p.kind = pot;
-    p.p0 = params[0];
-    p.p1 = params[1];
+    if (!set_potential(p, pot, params)) return 1;
     S(h).bond_pots[{std::min(ki, kj), std::max(ki, kj)}] = p;
     return 0;
 }
 int lo_set_angle_potential(lo_system* h, uint32_t ki, uint32_t kj, uint32_t kk, int pot, const double* params) {
     Potential p;
-    p.kind = pot;
-    p.p0 = params[0];
-    p.p1 = params[1];
+    if (!set_potential(p, pot, params)) return 1;
     S(h).angle_pots[{std::min(ki, kk), kj, std::max(ki, kk)}] = p;
     return 0;
 }
 int lo_set_dihedral_potential(lo_system* h, uint32_t ki, uint32_t kj, uint32_t kk, uint32_t km, int pot,
                               const double* params) {
     Potential p;
-    p.kind = pot;
-    p.p0 = params[0];
-    p.p1 = params[1];
+    if (!set_potential(p, pot, params)) return 1;
     S(h).dihedral_pots[System::dihedral_key(ki, kj, kk, km)] = p;
     return 0;
 }
@@ -1657,6 +2012,18 @@ int lo_ewald_kvectors(lo_system* h, int* out_count) {
     return 0;
 }
 
+// MolecularVirial (compute.rs:278-364)
+int lo_molecular_virial(lo_system* h, double* w9) {
+    System& s = S(h);
+    if (s.infinite()) {
+        g_error = "Can not compute virial for infinite cell";
+        return 1;
+    }
+    M3 w = molecular_virial(s);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) w9[3 * a + b] = w.m[a][b];
+    return 0;
+}
 int lo_virial(lo_system* h, double* w9) {
     System& s = S(h);
     M3 w = atomic_virial(s);
@@ -1816,6 +2183,20 @@ int lo_mc_run(lo_system* h, uint64_t seed, uint64_t attempt0, uint64_t nattempts
 void lo_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox4x32_10(ctr, key, out); }
 
 // Scalar functions for the unit golden values.
+// Scalar values of one potential (energy/force at r, tails at rc), and of
+// its TableComputation when size > 0; NaN for invalid parameters.
+double lo_potential_value(int pot, const double* params, uint64_t size, double max, int what, double x) {
+    Potential p;
+    if (!set_potential(p, pot, params)) return std::nan("");
+    if (size > 0) p.make_table(size_t(size), max);
+    switch (what) {
+        case 0: return p.energy(x);
+        case 1: return p.force(x);
+        case 2: return p.tail_energy(x);
+        case 3: return p.tail_virial(x);
+    }
+    return std::nan("");
+}
 double lo_lj_energy(double sigma, double eps, double r) {
     Potential p;
     p.kind = POT_LJ;

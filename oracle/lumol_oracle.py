@@ -45,6 +45,8 @@ def lib():
             "lo_set_kinds": (I, [P, U32P, U32]),
             "lo_set_bonds": (I, [P, SZ, U32P]),
             "lo_set_pair": (I, [P, U32, U32, I, DP, D, I, I, I, D]),
+            "lo_set_pair_table": (I, [P, U32, U32, I, DP, U64, D, D, I, I, I, D]),
+            "lo_potential_value": (D, [I, DP, U64, D, I, D]),
             "lo_set_bond_potential": (I, [P, U32, U32, I, DP]),
             "lo_set_angle_potential": (I, [P, U32, U32, U32, I, DP]),
             "lo_set_dihedral_potential": (I, [P, U32, U32, U32, U32, I, DP]),
@@ -55,6 +57,7 @@ def lib():
             "lo_ewald_components": (I, [P, DP, DP, DP]),
             "lo_ewald_kvectors": (I, [P, ctypes.POINTER(I)]),
             "lo_virial": (I, [P, DP]),
+            "lo_molecular_virial": (I, [P, DP]),
             "lo_kinetic_energy": (D, [P]),
             "lo_temperature": (D, [P]),
             "lo_pair_counts": (I, [P, ctypes.POINTER(U64), ctypes.POINTER(U64)]),
@@ -88,6 +91,12 @@ def _chk(status):
         raise RuntimeError("oracle: " + lib().lo_last_error().decode())
 
 
+def _padded(params):
+    pr = np.zeros(8)
+    pr[:len(params)] = params
+    return pr
+
+
 def _dp(a):
     return a.ctypes.data_as(DP)
 
@@ -115,20 +124,29 @@ class OracleSystem:
         bonds = np.ascontiguousarray(np.array(system.bonds, dtype=np.uint32).reshape(-1, 2))
         _chk(L.lo_set_bonds(self.h, len(bonds), bonds.ctypes.data_as(U32P)))
         for (na, nb), inter in system.pairs.items():
-            pr = np.array(inter.potential.params(), dtype=float)
-            _chk(L.lo_set_pair(self.h, system.kinds[na], system.kinds[nb], inter.potential.kind, _dp(pr),
-                               inter.cutoff(), int(inter.shifted_), int(inter.tail), int(inter.restriction_),
-                               float(inter.scale14)))
+            pot = inter.potential
+            table = getattr(pot, "table", None)  # TableComputation: (size, max)
+            if table is not None:
+                pot = pot.potential
+            pr = _padded(pot.params())
+            if table is not None:
+                _chk(L.lo_set_pair_table(self.h, system.kinds[na], system.kinds[nb], pot.kind, _dp(pr),
+                                         int(table[0]), float(table[1]), inter.cutoff(), int(inter.shifted_),
+                                         int(inter.tail), int(inter.restriction_), float(inter.scale14)))
+            else:
+                _chk(L.lo_set_pair(self.h, system.kinds[na], system.kinds[nb], pot.kind, _dp(pr),
+                                   inter.cutoff(), int(inter.shifted_), int(inter.tail), int(inter.restriction_),
+                                   float(inter.scale14)))
         for names, pot in system.bond_potentials.items():
-            pr = np.array(pot.params(), dtype=float)
+            pr = _padded(pot.params())
             _chk(L.lo_set_bond_potential(self.h, system.kinds[names[0]], system.kinds[names[1]], pot.kind,
                                          _dp(pr)))
         for names, pot in system.angle_potentials.items():
-            pr = np.array(pot.params(), dtype=float)
+            pr = _padded(pot.params())
             k = [system.kinds[x] for x in names]
             _chk(L.lo_set_angle_potential(self.h, k[0], k[1], k[2], pot.kind, _dp(pr)))
         for names, pot in system.dihedral_potentials.items():
-            pr = np.array(pot.params(), dtype=float)
+            pr = _padded(pot.params())
             k = [system.kinds[x] for x in names]
             _chk(L.lo_set_dihedral_potential(self.h, k[0], k[1], k[2], k[3], pot.kind, _dp(pr)))
         coul = system.coulomb
@@ -163,6 +181,11 @@ class OracleSystem:
     def virial(self):
         out = np.zeros(9)
         _chk(lib().lo_virial(self.h, _dp(out)))
+        return out.reshape(3, 3)
+
+    def molecular_virial(self):
+        out = np.zeros(9)
+        _chk(lib().lo_molecular_virial(self.h, _dp(out)))
         return out.reshape(3, 3)
 
     def ewald_components(self):
@@ -238,6 +261,17 @@ class OracleSystem:
         acc = U64()
         _chk(lib().lo_mc_run(self.h, seed, attempt0, nattempts, beta, delta, ctypes.byref(acc)))
         return acc.value
+
+
+def potential_value(kind, params, what, x, table=None):
+    """Scalar value of one potential: what = "energy", "force",
+    "tail_energy" or "tail_virial"; table = (size, max) evaluates its
+    TableComputation instead."""
+    pr = np.zeros(8)
+    pr[:len(params)] = params
+    size, mx = table if table is not None else (0, 0.0)
+    code = ["energy", "force", "tail_energy", "tail_virial"].index(what)
+    return lib().lo_potential_value(kind, _dp(pr), int(size), float(mx), code, float(x))
 
 
 def philox4x32_10(ctr, key):

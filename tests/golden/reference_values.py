@@ -202,3 +202,95 @@ MD_NACL_SMALL = {
     "charges": {"Na": 1.0, "Cl": -1.0}, "masses": {"Na": 22.98976928, "Cl": 35.45},
     "drift_tol": 5e-3,
 }
+
+# --- other potentials (lumol-core/src/energy/functions.rs tests, doc tests) ----
+# Each: (constructor parameters in the reference's order, [(what, x, value, check)])
+# check: "eq" assert_eq!, "ulps" assert_ulps_eq!, ("abs", eps) |a - b| <= eps,
+# ("rel", eps) approx::assert_relative_eq!(.., epsilon = eps).
+OTHER_POTENTIALS = {
+    # functions.rs:698-716
+    "buckingham": ({"a": 2.0, "c": 1.0, "rho": 2.0}, [
+        ("energy", 2.0, 0.7201338823428847, "eq"),
+        ("force", 2.0, 0.32100444117144233, "eq"),
+        ("tail_energy", 10.0, 1.8323882504179136, "eq"),
+        ("tail_virial", 10.0, 33.422487868546725, "eq"),
+    ]),
+    # functions.rs:272-274 (doc test, Buckingham{a 2, c 1, rho 5.3})
+    "buckingham_doc": ({"a": 2.0, "c": 1.0, "rho": 5.3}, [
+        ("energy", 2.2, 1.3117360696239022, "eq"),
+        ("force", 2.2, 0.2251072178835946, "eq"),
+    ]),
+    # functions.rs:718-738
+    "born": ({"a": 2.0, "c": 1.0, "d": 0.5, "sigma": 2.0, "rho": 2.0}, [
+        ("energy", 2.0, 1.986328125, "eq"),
+        ("force", 2.0, 0.9609375, "eq"),
+        ("tail_energy", 10.0, 4.981521444402363, "eq"),
+        ("tail_virial", 10.0, 69.13986044386026, "eq"),
+    ]),
+    # functions.rs:335-337 (doc test)
+    "born_doc": ({"a": 2.0, "c": 1.0, "d": 0.5, "sigma": 1.5, "rho": 5.3}, [
+        ("energy", 2.2, 1.7446409593340713, "eq"),
+        ("force", 2.2, 0.30992873382584607, "eq"),
+    ]),
+    # functions.rs:740-758
+    "morse": ({"a": 2.0, "x0": 1.3, "depth": 4.0}, [
+        ("energy", 1.0, 2.703517287822119, "eq"),
+        ("force", 1.0, -37.12187076378477, "eq"),
+        ("tail_energy", 1.0, 0.0, "eq"),
+        ("tail_virial", 1.0, 0.0, "eq"),
+    ]),
+    # functions.rs:760-778
+    "gaussian": ({"a": 8.0, "b": 2.0}, [
+        ("energy", 0.0, -8.0, "eq"),
+        ("force", 0.0, 0.0, "eq"),
+        ("tail_energy", 2.5, -1.93518e-5, ("rel", 1e-10)),
+        ("tail_virial", 2.5, 5.23887e-4, ("rel", 1e-10)),
+    ]),
+    # functions.rs:781-802
+    "mie": ({"sigma": 2.0, "epsilon": 0.8, "n": 12.0, "m": 6.0}, [
+        ("energy", 2.0, 0.0, "eq"),
+        ("energy", 2.5, -0.6189584744448002, "eq"),
+        ("tail_energy", 1.0, 1388.0888889, ("rel", 1e-6)),
+        ("tail_energy", 2.0, -5.688888888888889, ("rel", 1e-6)),
+        ("tail_energy", 14.42, -0.022767318648783084, ("rel", 1e-6)),
+        ("tail_virial", 1.0, 17066.666666666668, ("rel", 1e-6)),
+        ("tail_virial", 2.0, -17.06666666666667, ("rel", 1e-6)),
+        ("tail_virial", 14.42, -0.1366035877536718, ("rel", 1e-6)),
+        ("force", 2.5, -0.95773475733504, "ulps"),
+    ]),
+    # functions.rs:519-524 (doc test, Mie(sigma 2, epsilon 10, n 12, m 6))
+    "mie_doc": ({"sigma": 2.0, "epsilon": 10.0, "n": 12.0, "m": 6.0}, [
+        ("energy", 2.0, 0.0, "eq"),
+        ("energy", 3.0, -3.203365942785746, ("abs", 1e-8)),
+        ("force", 2.0, 120.0, "eq"),
+    ]),
+    # functions.rs:172-177 (doc test, CosineHarmonic::new(100, 2))
+    "cosine_harmonic_doc": ({"k": 100.0, "x0": 2.0}, [
+        ("energy", 2.0, 0.0, "eq"),
+        ("energy", 3.0, 16.464942078100552, "eq"),
+        ("force", 2.0, 0.0, "eq"),
+    ]),
+    # functions.rs:225-231 (doc test, Torsion{delta pi/2, k 10, n 3})
+    "torsion_doc": ({"k": 10.0, "delta": 1.5707963267948966, "n": 3}, [
+        ("energy", 1.5707963267948966, 0.0, "eq"),
+        ("energy", 1.0471975511965976, 10.0, "eq"),
+        ("force", 1.5707963267948966, 0.0, ("abs", 1e-12)),
+    ]),
+    # functions.rs:680-696 (Torsion{k 5, n 3, delta 3})
+    "torsion": ({"k": 5.0, "delta": 3.0, "n": 3}, [
+        ("energy", 1.0, 10.0, "eq"),
+        ("force", 1.0, 0.0, "eq"),
+    ]),
+}
+
+# lumol-core/src/energy/computations.rs:178-206 (Harmonic k 50 x0 2, 1000 points,
+# max 4) and :94-101 (doc test: Harmonic x0 0.5 k 4.2, 1000 points, max 2).
+# (x, energy, force), assert_eq!.
+TABLE_HARMONIC = {"k": 50.0, "x0": 2.0, "size": 1000, "max": 4.0, "values": [
+    (2.5, 6.25, -25.0),
+    (4.0 - 2.0 * (4.0 / 1000.0), 99.2016, -99.6),
+    (4.0 - 4.0 / 1000.0, 0.0, 0.0),
+    (4.0, 0.0, 0.0),
+    (4.1, 0.0, 0.0),
+]}
+TABLE_DOC = {"k": 4.2, "x0": 0.5, "size": 1000, "max": 2.0, "values": [(1.0, 0.525, None), (3.0, 0.0, None)]}
