@@ -29,56 +29,42 @@ import numpy as np
 
 from . import _native as nat
 
-K_BOLTZMANN = 8.31446284161522e-7          # lumol-core/src/consts.rs:9
-FOUR_PI_EPSILON_0 = 7.197589831304046      # lumol-core/src/consts.rs:15
+from . import unitconv as _units
+from .consts import FOUR_PI_EPSILON_0, K_BOLTZMANN  # noqa: F401
 
-# Standard atomic weights (IUPAC), the values lumol's mass table holds.
+# Standard atomic weights (IUPAC) of the elements, the values of lumol's mass
+# table (lumol-core/src/sys/config/mass.rs); a particle whose name is not an
+# element gets mass 0 (particles.rs:61).
 ATOMIC_MASSES = {
     "H": 1.008, "He": 4.002602, "Li": 6.94, "Be": 9.012182, "B": 10.81, "C": 12.011,
-    "N": 14.007, "O": 15.999, "F": 18.9984032, "Ne": 20.1797, "Na": 22.98976928,
-    "Mg": 24.305, "Al": 26.9815386, "Si": 28.085, "P": 30.973762, "S": 32.06,
-    "Cl": 35.45, "Ar": 39.948, "K": 39.0983, "Ca": 40.078, "Kr": 83.798, "Xe": 131.293,
+    "N": 14.007, "O": 15.999, "F": 18.9984032, "Ne": 20.1797, "Na": 22.98976928, "Mg": 24.305,
+    "Al": 26.9815386, "Si": 28.085, "P": 30.973762, "S": 32.06, "Cl": 35.45, "Ar": 39.948,
+    "K": 39.0983, "Ca": 40.078, "Sc": 44.955912, "Ti": 47.867, "V": 50.9415, "Cr": 51.9961,
+    "Mn": 54.938045, "Fe": 55.845, "Co": 58.933195, "Ni": 58.6934, "Cu": 63.546, "Zn": 65.38,
+    "Ga": 69.723, "Ge": 72.63, "As": 74.9216, "Se": 78.96, "Br": 79.904, "Kr": 83.798,
+    "Rb": 85.4678, "Sr": 87.62, "Y": 88.90585, "Zr": 91.224, "Nb": 92.90638, "Mo": 95.96,
+    "Tc": 97.0, "Ru": 101.07, "Rh": 102.9055, "Pd": 106.42, "Ag": 107.8682, "Cd": 112.411,
+    "In": 114.818, "Sn": 118.71, "Sb": 121.76, "Te": 127.6, "I": 126.90447, "Xe": 131.293,
+    "Cs": 132.9054519, "Ba": 137.327, "La": 138.90547, "Ce": 140.116, "Pr": 140.90765, "Nd": 144.242,
+    "Pm": 145.0, "Sm": 150.36, "Eu": 151.964, "Gd": 157.25, "Tb": 158.92535, "Dy": 162.5,
+    "Ho": 164.93032, "Er": 167.259, "Tm": 168.93421, "Yb": 173.054, "Lu": 174.9668, "Hf": 178.49,
+    "Ta": 180.94788, "W": 183.84, "Re": 186.207, "Os": 190.23, "Ir": 192.217, "Pt": 195.084,
+    "Au": 196.966569, "Hg": 200.592, "Tl": 204.38, "Pb": 207.2, "Bi": 208.9804, "Po": 209.0,
+    "At": 210.0, "Rn": 222.0, "Fr": 223.0, "Ra": 226.0, "Ac": 227.0, "Th": 232.03806,
+    "Pa": 231.03588, "U": 238.02891, "Np": 237.0, "Pu": 244.0, "Am": 243.0, "Cm": 247.0,
+    "Bk": 247.0, "Cf": 251.0, "Es": 252.0, "Fm": 257.0, "Md": 258.0, "No": 259.0,
+    "Lr": 262.0, "Rf": 267.0, "Db": 270.0, "Sg": 271.0, "Bh": 270.0, "Hs": 277.0,
+    "Mt": 276.0, "Ds": 281.0, "Rg": 282.0, "Cn": 285.0, "Uut": 285.0, "Fl": 289.0,
+    "Mc": 289.0, "Lv": 293.0, "Ts": 294.0, "Og": 294.0,
 }
 
 
-# ---------------------------------------------------------------------------------
-# units (lumol-core/src/units.rs:30-80)
-# ---------------------------------------------------------------------------------
 class units:
-    """Conversion factors to lumol's internal units (A, fs, u, K)."""
-    U_IN_KG = 1.660538782e-27
-    AVOGADRO = 6.02214179e23
-    _energy = {"kJ/mol": 1e-7 / U_IN_KG / AVOGADRO,
-               "kcal/mol": 4.184 * 1e-7 / U_IN_KG / AVOGADRO,
-               "K": K_BOLTZMANN}
-
-    @classmethod
-    def from_(cls, value, unit):
-        if unit in cls._energy:
-            return value * cls._energy[unit]
-        if unit == "kJ/mol/A^2":
-            return value * cls._energy["kJ/mol"]
-        if unit == "kcal/mol/A^2":
-            return value * cls._energy["kcal/mol"]
-        if unit == "kJ/mol/A":
-            return value * cls._energy["kJ/mol"]
-        if unit == "kcal/mol/A":
-            return value * cls._energy["kcal/mol"]
-        if unit == "atm":
-            return value * 101325.0 * 1e-40 / cls.U_IN_KG
-        if unit == "bar":
-            return value * 1e-35 / cls.U_IN_KG
-        if unit == "deg":
-            return value * math.pi / 180.0
-        if unit in ("A", "fs", "u", "rad"):
-            return value
-        if unit == "ps":
-            return value * 1e3
-        raise ValueError(f"unknown unit {unit}")
-
-    @classmethod
-    def to(cls, value, unit):
-        return value / cls.from_(1.0, unit)
+    """Conversion to and from lumol's internal units (lumol-core/src/units.rs);
+    unit strings such as "kcal/mol/A^2" or "(kJ/mol/A)^2"."""
+    from_ = staticmethod(_units.from_)
+    from_str = staticmethod(_units.from_str)
+    to = staticmethod(_units.to)
 
 
 # ---------------------------------------------------------------------------------
@@ -375,6 +361,8 @@ class System:
         self.coulomb = None
         # DegreesOfFreedom of system.rs:47-50, set by the propagator's setup
         self.simulated_degrees_of_freedom = "particles"
+        # external temperature of system.rs:47 (set by a Monte Carlo setup)
+        self.external_temperature = None
         self.precision = nat.LG_PRECISION_FP64
         self._handle = None
         self._dirty = True
@@ -454,6 +442,23 @@ class System:
             raise ValueError("The cutoff for the coulomb potential is bigger than half the cell length")
         self.coulomb = potential
         self._dirty = True
+
+    def cell_changed(self):
+        """Push a new `cell` (assigned to the attribute) to the device: the
+        k-vectors, cell list and cutoff checks follow at the next evaluation."""
+        if self._handle is not None and not self._dirty:
+            a, b, c = self.cell.lengths
+            nat.check(nat.lib().lg_system_set_cell(self._handle, a, b, c))
+        else:
+            self._dirty = True
+
+    def maximum_cutoff(self):
+        """Interactions::maximum_cutoff (interactions.rs:166-196): the largest
+        pair or coulomb cutoff, None without potentials."""
+        cuts = [p.cutoff() for p in self.pairs.values()]
+        if self.coulomb is not None:
+            cuts.append(self.coulomb.cutoff)
+        return max(cuts) if cuts else None
 
     def set_precision(self, precision):
         self.precision = precision
@@ -655,6 +660,10 @@ class System:
         return self.kinetic_energy() + self.potential_energy()
 
     def temperature(self):
+        """System::temperature (system.rs:273-278): the external temperature a
+        Monte Carlo propagator sets, else the kinetic temperature."""
+        if self.external_temperature is not None:
+            return self.external_temperature
         h = self._sync()
         out = ctypes.c_double()
         nat.check(nat.lib().lg_temperature(h, ctypes.byref(out)))
@@ -673,7 +682,9 @@ class System:
         return (self.degrees_of_freedom() * K_BOLTZMANN * temperature + np.trace(w)) / (3.0 * self.volume())
 
     def pressure(self):
-        """Pressure (compute.rs:418-425) at the instantaneous temperature."""
+        """System::pressure (system.rs:290-302): at the external temperature
+        when one is set, else Pressure (compute.rs:418-425) at the kinetic
+        temperature."""
         return self.pressure_at_temperature(self.temperature())
 
     def stress_at_temperature(self, temperature):
@@ -687,7 +698,11 @@ class System:
         return (kinetic + w) / self.volume()
 
     def stress(self):
-        """Stress (compute.rs:465-478): (sum_i m_i v_i (x) v_i + W) / V."""
+        """System::stress (system.rs:304-315): StressAtTemperature at the
+        external temperature when one is set, else Stress (compute.rs:465-478),
+        (sum_i m_i v_i (x) v_i + W) / V."""
+        if self.external_temperature is not None:
+            return self.stress_at_temperature(self.external_temperature)
         if self.cell.is_infinite():
             raise ValueError("Can not compute stress for infinite cell")
         v = self.velocities
@@ -850,14 +865,27 @@ class BerendsenThermostat:
 
 
 class MolecularDynamics:
-    """molecular_dynamics.rs:11-83: integrator + optional thermostat."""
+    """molecular_dynamics.rs:11-83: integrator + optional thermostat +
+    controls, applied in that order each step.  VelocityVerlet and LeapFrog
+    with no thermostat or a Berendsen one run fused on the device; the other
+    integrators, thermostats and controls (lumol_amd.propagators) step on the
+    host around device force evaluations."""
 
     def __init__(self, integrator):
         self.integrator = integrator
         self.thermostat = None
+        self.controls = []
+
+    @classmethod
+    def new(cls, timestep):
+        """MolecularDynamics::new: a VelocityVerlet integrator."""
+        return cls(VelocityVerlet(timestep))
 
     def set_thermostat(self, thermostat):
         self.thermostat = thermostat
+
+    def add_control(self, control):
+        self.controls.append(control)
 
 
 # ---------------------------------------------------------------------------------
@@ -874,33 +902,63 @@ class Translate:
 
 
 class MonteCarlo:
-    """monte_carlo.rs:17-330: Metropolis sampling at `temperature` (K).  The
-    device supports one Translate move; random numbers come from Philox4x32-10
-    keyed by `seed` (one counter per attempt)."""
+    """monte_carlo.rs:17-345: Metropolis sampling at `temperature` (K) over
+    moves picked by frequency.  A single Translate move without amplitude
+    updates runs as the device chain (random numbers from Philox4x32-10 keyed
+    by `seed`, one counter per attempt); other move sets (Rotate, Resize,
+    several moves, amplitude updates) are driven from the host with device
+    energy changes (lumol_amd.propagators)."""
 
     def __init__(self, temperature, seed=0):
-        if temperature < 0:
-            raise ValueError("Monte Carlo temperature must be positive")
+        if not temperature > 0:
+            raise ValueError(f"Monte Carlo temperature must be positive, got {temperature}")
         self.temperature = float(temperature)
         self.beta = 1.0 / (K_BOLTZMANN * self.temperature)
         self.seed = int(seed)
-        self.move = None
+        self.moves = []  # (move, frequency, target_acceptance)
+        self.update_frequency = 0
         self.attempts = 0
         self.accepted = 0
 
-    def add(self, move, frequency=1.0):
-        if not isinstance(move, Translate) or self.move is not None:
-            raise NotImplementedError("the device Monte Carlo runs a single Translate move")
-        self.move = move
+    def add(self, move, frequency=1.0, target_acceptance=None):
+        if target_acceptance is not None and not 0.0 < target_acceptance < 1.0:
+            raise ValueError(f"target acceptance ratio must be between 0 and 1, got {target_acceptance}")
+        self.moves.append((move, float(frequency), target_acceptance))
+
+    def set_amplitude_update_frequency(self, frequency):
+        self.update_frequency = int(frequency)
+
+    @property
+    def move(self):
+        return self.moves[0][0] if self.moves else None
+
+    def device_chain(self):
+        """True when the run is the device chain kernel: one Translate move."""
+        return (len(self.moves) == 1 and isinstance(self.moves[0][0], Translate)
+                and (self.update_frequency == 0 or self.moves[0][2] is None))
 
 
 class Simulation:
-    """simulations.rs:54-159: runs `nsteps` propagator steps on the device
-    (MD steps, or Monte Carlo attempts)."""
+    """simulations.rs:54-159: runs `nsteps` propagator steps (MD steps, or
+    Monte Carlo attempts)."""
 
     def __init__(self, propagator):
         self.propagator = propagator
         self._setup_for = None
+        self._host = None
+
+    def _md_mode(self):
+        from . import propagators as P
+        md = self.propagator
+        fused_int = isinstance(md.integrator, (VelocityVerlet, LeapFrog))
+        th = md.thermostat
+        if fused_int and (th is None or isinstance(th, BerendsenThermostat)) and not md.controls:
+            return "device"
+        if fused_int:
+            return "hybrid"
+        if not isinstance(md.integrator, (P.Verlet, P.BerendsenBarostat, P.AnisoBerendsenBarostat)):
+            raise ValueError(f"unknown integrator {type(md.integrator).__name__}")
+        return "host"
 
     def setup(self, system):
         h = system._sync()
@@ -908,12 +966,22 @@ class Simulation:
         if isinstance(md, MonteCarlo):
             # monte_carlo.rs:247-280: Translate moves whole molecules
             system.simulated_degrees_of_freedom = "molecules"
+            system.external_temperature = md.temperature  # simulations.rs:70-75
             system.cache_init()
             self._setup_for = (id(system), h.value)
+            self._host = None
             return
         system.simulated_degrees_of_freedom = "particles"  # molecular_dynamics.rs:53-57
-        T, tau = (md.thermostat.temperature, md.thermostat.tau) if md.thermostat else (-1.0, 1.0)
-        nat.check(nat.lib().lg_md_setup(h, md.integrator.code, md.integrator.timestep, T, tau))
+        system.external_temperature = None
+        mode = self._md_mode()
+        if mode == "host":
+            md.integrator.setup(system)
+        else:
+            # the device integrator; a Berendsen thermostat runs on the device
+            # right after it, other thermostats and the controls on the host
+            th = md.thermostat
+            T, tau = (th.temperature, th.tau) if isinstance(th, BerendsenThermostat) else (-1.0, 1.0)
+            nat.check(nat.lib().lg_md_setup(h, md.integrator.code, md.integrator.timestep, T, tau))
         self._setup_for = (id(system), h.value)
 
     def run(self, system, nsteps):
@@ -922,10 +990,67 @@ class Simulation:
             self.setup(system)
         p = self.propagator
         if isinstance(p, MonteCarlo):
-            if p.move is None:
+            if not p.moves:
                 raise ValueError("no move added to the Monte Carlo propagator")
-            p.accepted += system.mc_run(p.seed, p.attempts, int(nsteps), p.beta, p.move.delta)
-            p.attempts += int(nsteps)
+            if p.device_chain():
+                p.accepted += system.mc_run(p.seed, p.attempts, int(nsteps), p.beta, p.moves[0][0].delta)
+                p.attempts += int(nsteps)
+            else:
+                self._run_mc_host(system, int(nsteps))
             return
-        nat.check(nat.lib().lg_md_run(h, int(nsteps)))
+        mode = self._md_mode()
+        if mode == "device":
+            nat.check(nat.lib().lg_md_run(h, int(nsteps)))
+            system._device_state = True
+            return
+        th = p.thermostat if not isinstance(p.thermostat, BerendsenThermostat) or mode == "host" else None
+        if mode == "host" and isinstance(p.thermostat, BerendsenThermostat):
+            from .propagators import HostBerendsen
+            th = HostBerendsen(p.thermostat.temperature, p.thermostat.tau)
+        for _ in range(int(nsteps)):
+            if mode == "hybrid":
+                nat.check(nat.lib().lg_md_run(system._sync(), 1))
+                system._device_state = True
+            else:
+                p.integrator.integrate(system)
+            if th is not None:
+                th.apply(system)
+            for c in p.controls:
+                c.control(system)
+
+    def _run_mc_host(self, system, nsteps):
+        """MonteCarlo::propagate (monte_carlo.rs:289-330) on the host."""
+        from . import propagators as P
+        mc = self.propagator
+        if self._host is None:
+            counters = []
+            for move, freq, target in mc.moves:
+                hm = P.HostTranslate(move.delta) if isinstance(move, Translate) else move
+                counters.append(P.MoveCounter(hm, target))
+            freqs = np.array([f for _, f, _ in mc.moves], dtype=float)
+            freqs = freqs / freqs.sum()
+            cum = np.cumsum(freqs)
+            cum[-1] = 1.0
+            self._host = (counters, cum, np.random.default_rng(mc.seed))
+        counters, cum, rng = self._host
+        mols = system.molecules()
+        for _ in range(nsteps):
+            u = rng.random()
+            k = int(np.searchsorted(cum, u, side="left"))
+            c = counters[min(k, len(counters) - 1)]
+            mv = c.move
+            if not mv.prepare(system, rng, mols):
+                continue
+            cost = mv.cost(system, mc.beta)
+            if cost <= 0.0 or rng.random() < math.exp(-cost):
+                mv.apply(system)
+                c.accept()
+                mc.accepted += 1
+            else:
+                mv.restore(system)
+                c.reject()
+            mc.attempts += 1
+            if c.attempted == mc.update_frequency:
+                mv.update_amplitude(c.scaling_factor(), system)
+                c.attempted = c.accepted = 0
         system._device_state = True

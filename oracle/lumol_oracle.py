@@ -232,6 +232,13 @@ class OracleSystem:
         pos = np.ascontiguousarray(pos, dtype=float)
         _chk(lib().lo_set_positions(self.h, _dp(pos)))
 
+    def set_velocities(self, vel):
+        vel = np.ascontiguousarray(vel, dtype=float)
+        _chk(lib().lo_set_velocities(self.h, _dp(vel)))
+
+    def set_cell(self, a, b, c):
+        _chk(lib().lo_set_cell(self.h, a, b, c))
+
     def md_reset(self):
         _chk(lib().lo_md_reset(self.h))
 
