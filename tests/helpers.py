@@ -183,14 +183,18 @@ def butane_system(n_mol=27, L=22.0, seed=11, restriction=PairRestriction.NONE, c
 
 
 def other_potentials_system(variant=1, coulomb="wolf", n_mol=27, L=22.0, seed=13,
-                            restriction=PairRestriction.EXCLUDE13, precision=None):
+                            restriction=PairRestriction.EXCLUDE13, precision=None, morse=True):
     """The butane-like chains of butane_system with the potential kinds that
     the benchmark systems do not use (functions.rs, computations.rs):
     variant 1: Buckingham (tail) CA-CA, Mie n 14 m 7 (shifted, tail) CA-CB,
     Born-Mayer-Huggins (tail) CB-CB; Morse bonds, CosineHarmonic angles,
     Torsion (n 3) dihedrals.  Variant 2: Gaussian (tail) CA-CA, Morse
     (shifted) CA-CB, a TableComputation of LJ (2000 points, tail) CB-CB;
-    Harmonic bonds, Morse angles, CosineHarmonic dihedrals."""
+    Harmonic bonds, Morse angles, CosineHarmonic dihedrals.  morse=False
+    puts Harmonic bonds and angles and a Mie CA-CB pair in place of the
+    Morse terms, for dynamics: the reference's Morse force
+    2 depth a (1 - e^2) (functions.rs:420-423) attracts a compressed bond,
+    so trajectories with Morse terms collapse."""
     from lumol_amd import (BornMayerHuggins, Buckingham, CosineHarmonic, Gaussian, Mie, Morse,
                            TableComputation, Torsion)
     s = butane_system(n_mol=n_mol, L=L, seed=seed, restriction=restriction, coulomb=coulomb)
@@ -209,15 +213,20 @@ def other_potentials_system(variant=1, coulomb="wolf", n_mol=27, L=22.0, seed=13
         pair(("CA", "CB"), Mie(3.85, kj(0.555), 14.0, 7.0), shifted=True, tail=True)
         pair(("CB", "CB"), BornMayerHuggins(a=kj(20.0), c=kj(2000.0), d=kj(1000.0), sigma=3.2, rho=0.35),
              tail=True)
-        s.set_bond_potential(("CA", "CB"), Morse(a=1.9, x0=1.54, depth=kj(350.0)))
-        s.set_bond_potential(("CB", "CB"), Morse(a=1.9, x0=1.54, depth=kj(350.0)))
+        if morse:
+            s.set_bond_potential(("CA", "CB"), Morse(a=1.9, x0=1.54, depth=kj(350.0)))
+            s.set_bond_potential(("CB", "CB"), Morse(a=1.9, x0=1.54, depth=kj(350.0)))
         s.set_angle_potential(("CA", "CB", "CB"), CosineHarmonic(k=kj(500.0), x0=np.radians(114.0)))
         s.set_dihedral_potential(("CA", "CB", "CB", "CA"), Torsion(k=kj(5.0), delta=0.0, n=3))
     else:
         pair(("CA", "CA"), Gaussian(a=kj(2.0), b=0.05), tail=True)
-        pair(("CA", "CB"), Morse(a=1.0, x0=4.2, depth=kj(0.6)), shifted=True)
+        if morse:
+            pair(("CA", "CB"), Morse(a=1.0, x0=4.2, depth=kj(0.6)), shifted=True)
+        else:
+            pair(("CA", "CB"), Mie(3.85, kj(0.555), 10.0, 6.0), shifted=True)
         pair(("CB", "CB"), TableComputation(LennardJones(3.95, kj(0.382)), 2000, rc + 0.5), tail=True)
-        s.set_angle_potential(("CA", "CB", "CB"), Morse(a=2.0, x0=np.radians(114.0), depth=kj(200.0)))
+        if morse:
+            s.set_angle_potential(("CA", "CB", "CB"), Morse(a=2.0, x0=np.radians(114.0), depth=kj(200.0)))
         s.set_dihedral_potential(("CA", "CB", "CB", "CA"), CosineHarmonic(k=kj(30.0), x0=np.radians(180.0)))
     if precision is not None:
         s.set_precision(precision)

@@ -52,7 +52,7 @@ def test_other_potentials_mixed_form():
 
 @pytest.mark.parametrize("variant", [1, 2])
 def test_other_potentials_md_100_steps(variant):
-    s = other_potentials_system(variant=variant, coulomb="wolf")
+    s = other_potentials_system(variant=variant, coulomb="wolf", morse=False)
     rng = np.random.default_rng(3)
     s.velocities = rng.normal(scale=2e-3, size=(s.size(), 3))
     md_compare(s, Simulation(MolecularDynamics(VelocityVerlet(1.0))), 0, 1.0, 100)
