@@ -201,6 +201,18 @@ int lg_forces(lg_system* sys, double* forces /* 3N */, int accumulate);
 /* System::potential_energy and the EnergyEvaluator terms
  * (system.rs:263, compute.rs:111-128, energy.rs:47-158). */
 int lg_potential_energy(lg_system* sys, lg_energy_terms* out);
+/* lg_potential_energy that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The per-block energy and virial partial sums go to a static device array of
+ * 2^20 doubles (8 MiB per GPU, held until the process ends) instead of
+ * buffers the library allocates; the results are bit-identical to
+ * lg_potential_energy's.  The system must have been evaluated since its last
+ * change (lg_potential_energy or lg_forces), else LG_E_STATE; partial sums
+ * beyond the array return LG_E_INVALID; a neighbour list that would have to
+ * grow returns LG_E_STATE.  Calls that overlap on one GPU (from several host
+ * threads or contexts) race on the shared array. */
+int lg_potential_energy_noalloc(lg_system* sys, lg_energy_terms* out);
 /* System::virial with the atomic definition (system.rs:286,
  * compute.rs:195-255), row-major 3x3. */
 int lg_virial(lg_system* sys, double* virial /* 9 */);

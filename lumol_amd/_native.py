@@ -34,7 +34,7 @@ EXPORTS = [
     "lg_system_set_kinds", "lg_system_set_bonds", "lg_system_molecule_count",
     "lg_system_set_precision", "lg_system_set_tuning", "lg_system_get_tuning", "lg_pair_set", "lg_pair_set_table", "lg_bond_set", "lg_angle_set", "lg_dihedral_set",
     "lg_coulomb_ewald", "lg_coulomb_wolf", "lg_coulomb_clear", "lg_forces",
-    "lg_potential_energy", "lg_virial", "lg_molecular_virial", "lg_kinetic_energy", "lg_temperature",
+    "lg_potential_energy", "lg_potential_energy_noalloc", "lg_virial", "lg_molecular_virial", "lg_kinetic_energy", "lg_temperature",
     "lg_ewald_kvector_count", "lg_pairs_within", "lg_md_setup", "lg_md_run", "lg_cache_init", "lg_move_cost",
     "lg_move_commit", "lg_move_discard", "lg_cache_energy", "lg_mc_run", "lg_mc_run_batch", "lg_profile_begin",
     "lg_profile_end",
@@ -115,6 +115,7 @@ def _declare(lib):
         "lg_coulomb_clear": (I, [P]),
         "lg_forces": (I, [P, DP, I]),
         "lg_potential_energy": (I, [P, ctypes.POINTER(EnergyTerms)]),
+        "lg_potential_energy_noalloc": (I, [P, ctypes.POINTER(EnergyTerms)]),
         "lg_virial": (I, [P, DP]),
         "lg_molecular_virial": (I, [P, DP]),
         "lg_kinetic_energy": (I, [P, DP]),

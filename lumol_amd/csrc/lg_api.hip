@@ -16,6 +16,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <set>
 #include <string>
@@ -293,8 +294,40 @@ int make_pot(int pot, const double* params, int role, PotSpec& out) {
 
 }  // namespace
 
+// ---- scratch of lg_potential_energy_noalloc --------------------------------------------
+// Per-block energy and virial partial sums of the allocation-free energy
+// entry point, and their 18 totals.  A file-scope device array: every GPU
+// the library runs on holds its own copy until the process ends.  Host code
+// never uses the name itself (a host-side stand-in): kernels get the device
+// address that hipGetSymbolAddress returns for the current device, looked
+// up once per device when a context is created.
+constexpr size_t kNoallocScratchDoubles = size_t(1) << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current).
+int scratch_of_device(int device, double** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(LG_E_UNSUPPORTED, "device index above the supported range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(LG_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return LG_OK;
+}
+}  // namespace
+
 struct lg_context {
     int device = 0;
+    double* scratch = nullptr;  // this device's address of `scratch`
     hipStream_t stream = nullptr;
     size_t max_shared = 65536;
     // Monte Carlo launches: chain descriptors and per-chain counters
@@ -382,6 +415,10 @@ struct lg_system {
     int ld = 0;
     // energy partials
     DevBuf<double> epart_pairs, epart_mol, epart_k, ke_part, lambda;
+    // when set, the energy partials go here instead (lg_potential_energy_noalloc)
+    double* ep_pairs_at = nullptr;
+    double* ep_mol_at = nullptr;
+    double* ep_k_at = nullptr;
     // molecular virial: centres of mass and per-block partial sums
     DevBuf<double4> mol_com;
     DevBuf<double> molvir_part;
@@ -1355,7 +1392,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     pa.box = s->box;
     pa.n = n;
     pa.force = s->force.p;
-    pa.epart = s->epart_pairs.p;
+    pa.epart = s->ep_pairs_at ? s->ep_pairs_at : s->epart_pairs.p;
     pa.rebuild = s->flags.p;
     pa.ecx64 = s->ecx_ok ? s->ecx64.p : nullptr;
     pa.ecx32 = s->ecx_ok ? s->ecx32.p : nullptr;
@@ -1380,7 +1417,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     ia.coul_s14 = s->coul_s14;
     ia.box = s->box;
     ia.contrib = s->contrib.p;
-    ia.epart = s->epart_mol.p;
+    ia.epart = s->ep_mol_at ? s->ep_mol_at : s->epart_mol.p;
     const int iblocks = s->nterms > 0 ? blocks_for(s->nterms, 256) : 0;
     // rho(k) reduction blocks
     const bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
@@ -1398,7 +1435,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         rr.rec32 = mixed ? s->krec32.p : nullptr;
         rr.efac = s->efac.p;
         rr.vfac = s->vfac.p;
-        rr.epart = s->epart_k.p;
+        rr.epart = s->ep_k_at ? s->ep_k_at : s->epart_k.p;
         rblocks = blocks_for(s->nk, kRhoReduceK);
     }
     long prof = prof_start(s, LG_KERNEL_PAIRS);
@@ -1786,8 +1823,11 @@ int lg_context_create(int device, lg_context** out) {
     LG_HIP(hipSetDevice(device));
     hipDeviceProp_t prop;
     LG_HIP(hipGetDeviceProperties(&prop, device));
+    double* sp = nullptr;
+    LG_TRY(scratch_of_device(device, &sp));
     lg_context* c = new lg_context();
     c->device = device;
+    c->scratch = sp;
     c->max_shared = prop.sharedMemPerBlock;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -2177,12 +2217,12 @@ int lg_forces(lg_system* s, double* out, int accumulate) {
     return pull_vec3(s->force, s->n, out, S(s), accumulate != 0);
 }
 
-int lg_potential_energy(lg_system* s, lg_energy_terms* out) {
-    if (!s || !out) return fail(LG_E_INVALID, "null argument");
-    LG_HIP(hipSetDevice(s->ctx->device));
-    LG_TRY(compute(s, true));
-    Energies E;
-    LG_TRY(gather_energies(s, E));
+}  // extern "C"
+
+namespace {
+// EnergyEvaluator terms and their sum (energy.rs:47-158, compute.rs:114-127)
+// from the gathered device sums.
+int energy_terms_of(lg_system* s, const Energies& E, lg_energy_terms* out) {
     double e_tail, w_tail;
     tail_terms(s, &e_tail, &w_tail);
     double self = coulomb_self(s);
@@ -2208,6 +2248,102 @@ int lg_potential_energy(lg_system* s, lg_energy_terms* out) {
     *out = t;
     if (!std::isfinite(total)) return fail(LG_E_NONFINITE, "Potential energy is infinite!");
     return LG_OK;
+}
+
+// The 18 sums of gather_energies, added on the device in the same order
+// (block order, pair blocks then intra-molecular blocks): one thread per sum.
+__global__ void k_energy_gather(const double* __restrict__ pp, int nbp, const double* __restrict__ pm, int nbm,
+                                const double* __restrict__ pk, int nbk, double* __restrict__ out) {
+    const int q = threadIdx.x;
+    if (q >= 18) return;
+    double v = 0.0;
+    if (q < 2 || q >= 6) {  // pairs, coulomb real space, w[0..5]: pair then intra blocks
+        const int ip = q < 2 ? q : q - 6 + 2;
+        const int im = q < 2 ? q : q - 6 + 5;
+        if (q < 12) {
+            for (int b = 0; b < nbp; b++) v += pp[b * 8 + ip];
+            for (int b = 0; b < nbm; b++) v += pm[b * 11 + im];
+        } else {  // wk[0..5]: k-space blocks
+            for (int b = 0; b < nbk; b++) v += pk[b * 7 + 1 + (q - 12)];
+        }
+    } else if (q < 5) {  // bonds, angles, dihedrals: intra blocks
+        for (int b = 0; b < nbm; b++) v += pm[b * 11 + q];
+    } else {  // q == 5: k-space energy
+        for (int b = 0; b < nbk; b++) v += pk[b * 7];
+    }
+    out[q] = v;
+}
+}  // namespace
+
+extern "C" {
+
+int lg_potential_energy(lg_system* s, lg_energy_terms* out) {
+    if (!s || !out) return fail(LG_E_INVALID, "null argument");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    LG_TRY(compute(s, true));
+    Energies E;
+    LG_TRY(gather_energies(s, E));
+    return energy_terms_of(s, E, out);
+}
+
+int lg_potential_energy_noalloc(lg_system* s, lg_energy_terms* out) {
+    if (!s || !out) return fail(LG_E_INVALID, "null argument");
+    if (!s->ctx->scratch) return fail(LG_E_STATE, "the context has no scratch address");
+    if (s->topo_dirty || s->meta_dirty || s->params_dirty || s->kspace_dirty || s->nlist_dirty || !s->epart_pairs.p)
+        return fail(LG_E_STATE, "the system changed since its last evaluation: call lg_potential_energy (or "
+                                "lg_forces) once first, which allocates what the evaluation needs");
+    if (s->periodic) {  // the checks of prepare(), which allocates nothing
+        double half = std::min(s->L[0], std::min(s->L[1], s->L[2])) / 2.0;
+        for (auto& [k, p] : s->pairs)
+            if (p.rc > half)
+                return fail(LG_E_INVALID, "The cutoff for this pair potential is bigger than half the cell length");
+        if (s->coul_kind != COUL_NONE && s->coul_rc > half)
+            return fail(LG_E_INVALID, "The cutoff for the coulomb potential is bigger than half the cell length");
+    }
+    const int nbp = blocks_for(s->n * size_t(s->tpa), kPairBlock);
+    const int nbm = s->nterms > 0 ? blocks_for(s->nterms, 256) : 0;
+    const int nbk = (s->coul_kind == COUL_EWALD && s->nk > 0) ? blocks_for(s->nk, kRhoReduceK) : 0;
+    const size_t need = size_t(nbp) * 8 + size_t(nbm) * 11 + size_t(nbk) * 7 + 18;
+    if (need > kNoallocScratchDoubles)
+        return fail(LG_E_INVALID, "the energy partial sums of this system (" + std::to_string(need) +
+                                      " doubles) exceed the static scratch of " +
+                                      std::to_string(kNoallocScratchDoubles) + " doubles");
+    LG_HIP(hipSetDevice(s->ctx->device));
+    double* base = s->ctx->scratch;
+    s->ep_pairs_at = base;
+    s->ep_mol_at = base + size_t(nbp) * 8;
+    s->ep_k_at = s->ep_mol_at + size_t(nbm) * 11;
+    double* sums = s->ep_k_at + size_t(nbk) * 7;
+    const uint32_t mask = s->prof_mask;
+    s->prof_mask = 0;  // no event creation in this call
+    int rc = launch_nlist(s, true);
+    if (rc == LG_OK) rc = launch_forces(s, true);
+    s->prof_mask = mask;
+    s->ep_pairs_at = s->ep_mol_at = s->ep_k_at = nullptr;
+    if (rc != LG_OK) return rc;
+    hipStream_t st = S(s);
+    k_energy_gather<<<1, 64, 0, st>>>(base, nbp, base + size_t(nbp) * 8, nbm, base + size_t(nbp) * 8 + size_t(nbm) * 11,
+                                      nbk, sums);
+    LG_HIP(hipGetLastError());
+    double h[18];
+    int fl = 0;
+    LG_HIP(hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipMemcpyAsync(&fl, s->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipStreamSynchronize(st));
+    if (fl > 0)  // growing the neighbour list would allocate: leave that to lg_potential_energy
+        return fail(LG_E_STATE, "the neighbour list is full: call lg_potential_energy once to grow it");
+    Energies E;
+    E.pairs = h[0];
+    E.coul_real = h[1];
+    E.bonds = h[2];
+    E.angles = h[3];
+    E.dihedrals = h[4];
+    E.kspace = h[5] / kFourPiEps0;
+    for (int q = 0; q < 6; q++) {
+        E.w[q] = h[6 + q];
+        E.wk[q] = h[12 + q] / kFourPiEps0;
+    }
+    return energy_terms_of(s, E, out);
 }
 
 int lg_virial(lg_system* s, double* w9) {
