@@ -496,7 +496,8 @@ def run_batched(batch, steps, warmup, ref, md_graph=0):
     knobs = tuning.autotune_concurrent(systems, lambda: concurrent(200, warm=20))
     # the stated error: the first system's initial state with these knobs
     chk = make(3, 1)
-    chk.set_tuning(knobs["pair_lanes"], knobs["kspace_split"], knobs["rho_width"], 0, md_graph)
+    chk.set_tuning(knobs["pair_lanes"], knobs["kspace_split"], knobs["rho_width"], knobs["nlist_build"], md_graph,
+                   rho_kind=knobs["rho_kind"])
     err = force_error(chk, *ref)
     chk.close()
     concurrent(warmup)
@@ -539,31 +540,38 @@ def bench_c3(args):
     beside it."""
     from lumol_amd._native import LG_PRECISION_FP64, LG_PRECISION_MIXED
     from lumol_amd.workloads import spce
+    forms = set(args.forms.split(","))
+    if "mixed" not in forms or not forms <= {"fp64", "mixed", "batched"}:
+        raise SystemExit("--forms: a comma list of fp64, mixed, batched that includes mixed")
     ref = oracle_reference("spce")
-    fp64 = run_form("spce", "fp64", args.steps, args.warmup, *ref)
+    fp64 = run_form("spce", "fp64", args.steps, args.warmup, *ref) if "fp64" in forms else None
     mixed = run_form("spce", "mixed", args.steps, args.warmup, *ref)
-    bat = run_batched(args.batch, args.steps, args.warmup, ref, args.md_graph)
-    drift = {"fp64": nve_drift(spce, LG_PRECISION_FP64), "mixed": nve_drift(spce, LG_PRECISION_MIXED)}
+    bat = run_batched(args.batch, args.steps, args.warmup, ref, args.md_graph) if "batched" in forms else None
+    drift = {} if args.no_drift else {
+        "fp64": nve_drift(spce, LG_PRECISION_FP64), "mixed": nve_drift(spce, LG_PRECISION_MIXED)}
     keep = ("value", "ms_per_step", "error_vs_fp64_oracle", "roofline", "kernel_us_per_step", "tuning")
+    head = bat or mixed
     line = {
-        "metric": METRIC, "value": bat["value"], "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": bat["ms_per_step"], "higher_is_better": True,
+        "metric": METRIC, "value": head["value"], "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic (seeded jittered grid, random orientations, Maxwell-Boltzmann velocities)",
-        "config": {"workload": WORKLOADS["spce"], "form": "mixed, batched", "batch": args.batch,
+        "config": {"workload": WORKLOADS["spce"], "form": "mixed, batched" if bat else "mixed",
+                   "batch": args.batch if bat else 1,
                    "atoms_per_system": mixed["atoms"], "pairs_within_rc": mixed["pairs_within_rc"],
                    "kvectors": mixed["kvectors"]},
-        "error_vs_fp64_oracle": bat["error_vs_fp64_oracle"],
+        "error_vs_fp64_oracle": head["error_vs_fp64_oracle"],
         "roofline": mixed["roofline"],
-        "forms": {
-            "fp64_single": dict({k: fp64[k] for k in keep}, nve_drift=drift["fp64"]),
-            "mixed_single": dict({k: mixed[k] for k in keep}, nve_drift=drift["mixed"]),
-            "mixed_batched": bat,
-            "mixed_batched_over_fp64": bat["value"] / fp64["value"],
-            "mixed_single_over_fp64": mixed["value"] / fp64["value"],
-        },
+        "forms": {"mixed_single": dict({k: mixed[k] for k in keep}, nve_drift=drift.get("mixed"))},
         "roofline_note": "dominant kernel of the mixed single-system run (the kernel measured alone on the device)",
     }
+    if fp64:
+        line["forms"]["fp64_single"] = dict({k: fp64[k] for k in keep}, nve_drift=drift.get("fp64"))
+        line["forms"]["mixed_single_over_fp64"] = mixed["value"] / fp64["value"]
+    if bat:
+        line["forms"]["mixed_batched"] = bat
+        if fp64:
+            line["forms"]["mixed_batched_over_fp64"] = bat["value"] / fp64["value"]
     if not args.no_serve:
         line["serve"] = run_serve(args.serve_jobs, args.serve_scale, args.serve_seconds, 0, 0, args.md_lanes,
                                   args.slice, "mixed")
@@ -742,6 +750,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serve: span of the arrivals")
     ap.add_argument("--form", choices=["fp64", "mixed"], default="mixed", help="serve: MD arithmetic form")
     ap.add_argument("--no-serve", action="store_true", help="spce: leave out the serving stream")
+    ap.add_argument("--forms", default="fp64,mixed,batched",
+                    help="spce: the forms measured (a comma list; mixed alone for a profiled run)")
+    ap.add_argument("--no-drift", action="store_true", help="spce: leave out the NVE drift runs")
     ap.add_argument("--serve-jobs", type=int, default=256, help="spce: MD jobs of the serving stream")
     ap.add_argument("--serve-scale", type=float, default=0.005,
                     help="spce: factor on the serving stream's 1k..20k MD steps")

@@ -341,11 +341,11 @@ int lg_system_get_tuning(lg_system* sys, lg_tuning* tuning);
  * the launch count per class. */
 enum {
     LG_KERNEL_NLIST = 0,        /* displacement check + cell list + Verlet list */
-    LG_KERNEL_PAIRS = 1,        /* k_pairs: LJ + Ewald real space / Wolf, and the
-                                   intra-molecular terms (extra blocks of the same
-                                   launch)                                        */
-    LG_KERNEL_MOLECULES = 2,    /* standalone k_intra_terms (not launched in the
-                                   force path: timed with LG_KERNEL_PAIRS)        */
+    LG_KERNEL_PAIRS = 1,        /* k_pairs: LJ + Ewald real space / Wolf, and, in
+                                   the FP64 form, the intra-molecular terms (extra
+                                   blocks of the same launch)                     */
+    LG_KERNEL_MOLECULES = 2,    /* k_intra_terms: the intra-molecular terms in a
+                                   launch of their own (mixed form)               */
     LG_KERNEL_KSPACE_RHO = 3,   /* k_rho_partial + k_rho_reduce                  */
     LG_KERNEL_KSPACE_FORCE = 4, /* k_kspace_force                                */
     LG_KERNEL_INTEGRATE = 5,    /* integrator halves + thermostat                */

@@ -22,7 +22,7 @@ FLAGS = [
     "--offload-arch=gfx950",
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function",
+    "-Wall", "-Wno-unused-function", "-Werror=misleading-indentation",
     "-I", os.path.join(ROOT, "include"),
 ]
 

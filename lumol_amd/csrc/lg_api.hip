@@ -8,6 +8,7 @@
 //   Ewald k-vector set and factors  lumol-core/src/energy/global/ewald.rs:130-183, 353-378
 //   tail corrections                lumol-core/src/sys/energy.rs:63-79, compute.rs:218-228
 //   Ewald / Wolf self terms         ewald.rs:619-626, wolf.rs:100-102, 177-207
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -372,6 +373,7 @@ struct lg_system {
     int cap = 0;
     DevBuf<int> cellid, slot, count, start, tmp, sorted, nbr, nnbr, flags;  // flags[0] rebuild, [1] overflow
     DevBuf<float4> cpos;  // list-build candidate records in sorted order
+    DevBuf<float4> frac;  // positions as the mixed pair kernel reads them (frac_pos)
     // parameters
     DevBuf<PairParam> table;
     DevBuf<double> ptabs;  // TableComputation tables: energies then forces per kind pair
@@ -481,6 +483,32 @@ long prof_start(lg_system* s, int cls) {
 void prof_stop(lg_system* s, long rec) {
     if (rec < 0) return;
     (void)hipEventRecord(s->prof_events[s->prof_records[size_t(rec)].second + 1], s->ctx->stream);
+}
+
+// A launch timed, when its class is profiled, by the kernel's own start and
+// end timestamps (the events of hipExtLaunchKernelGGL), which agree with a
+// kernel trace; an event pair recorded around the launch would add the
+// dispatch of the event packets to every kernel's time.
+template <typename... KArgs, typename... Args>
+void launch_timed(lg_system* s, int cls, void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32_t shmem,
+                  Args... args) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (s->prof_mask & (1u << cls)) {
+        bool ok = true;
+        while (ok && s->prof_events.size() < s->prof_used + 2) {
+            hipEvent_t e;
+            ok = hipEventCreate(&e) == hipSuccess;
+            if (ok) s->prof_events.push_back(e);
+        }
+        if (ok) {
+            const size_t idx = s->prof_used;
+            s->prof_used += 2;
+            s->prof_records.push_back({cls, idx});
+            e0 = s->prof_events[idx];
+            e1 = s->prof_events[idx + 1];
+        }
+    }
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, S(s), e0, e1, 0u, args...);
 }
 
 int blocks_for(size_t n, int b) { return int((n + b - 1) / b); }
@@ -1061,6 +1089,7 @@ int prepare_nlist(lg_system* s) {
     LG_TRY(s->tmp.ensure(n));
     LG_TRY(s->sorted.ensure(n));
     LG_TRY(s->cpos.ensure(n));
+    LG_TRY(s->frac.ensure(n));
     LG_TRY(s->nbr.ensure(size_t(s->cap) * n));
     LG_TRY(s->nnbr.ensure(n));
     LG_TRY(s->flags.ensure(4));
@@ -1135,7 +1164,10 @@ static bool cell_build(const lg_system* s) {
 }
 // Where the displacement tests write the atoms' cell tags: only for the
 // direct build.
-static CellTag cell_tags(lg_system* s) { return CellTag{direct_build(s) ? s->cellid.p : nullptr, s->grid, s->box}; }
+static CellTag cell_tags(lg_system* s) {
+    return CellTag{direct_build(s) ? s->cellid.p : nullptr, s->grid, s->box,
+                   s->precision == LG_PRECISION_MIXED ? s->frac.p : nullptr};
+}
 // Workgroups per cell of the direct build: about 256 workgroups in all (one
 // per CU of the MI355X), at most 8 per cell.
 static int direct_split(const lg_system* s) { return std::max(1, std::min(8, 256 / std::max(1, s->grid.ncell))); }
@@ -1264,6 +1296,8 @@ int launch_nlist(lg_system* s, bool check) {
     return LG_OK;
 }
 
+// phase_done: the MD first half-step has already written the phase table
+// and the FP32 positions (frac) of the current positions.
 int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_done = false) {
     hipStream_t st = S(s);
     int n = int(s->n);
@@ -1278,7 +1312,6 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         int w = rho_chunk(s);
         nchunks = blocks_for(s->ld, w);
         unsigned grid = xcd_grid(nub, nchunks);
-        long pr = prof_start(s, LG_KERNEL_KSPACE_RHO);
         // with a deferred Verlet build: one launch of k_cells_rho, the list
         // blocks first (cells rounded up to 8), then the rho blocks
         const bool direct = s->direct_pending;
@@ -1291,78 +1324,77 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
             // the matrix-core rho shares its launch with the cell-list build
             // only; a direct build runs first on its own
             if (direct && near)
-                k_nlist_direct<true><<<direct_blocks(s), kCellBlock, 0, st>>>(da);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_nlist_direct<true>, direct_blocks(s), kCellBlock, 0, da);
             else if (direct)
-                k_nlist_direct<false><<<direct_blocks(s), kCellBlock, 0, st>>>(da);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_nlist_direct<false>, direct_blocks(s), kCellBlock, 0, da);
             const bool cells = s->cells_pending;
             const RhoShape sh = rho_mfma_shape(s, mixed);
             nchunks = sh.nchunks;
             const unsigned mgrid = xcd_grid(sh.ngroups, sh.nchunks);
             if (mixed) {
                 if (!phase_done)
-                    k_phase32<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                    launch_timed(s, LG_KERNEL_KSPACE_RHO, k_phase32, blocks_for(size_t(3) * s->ld, 256), 256, 0, s->pos.p, n, s->ld, s->box, kmax,
                                                                                   s->phase_tab32.p, s->phase_q32.p);
                 RhoMfmaArgs<float, float2> ra{s->pos.p, n, s->box, s->ld, kmax, sh.wlog, sh.cpb,
                                               s->mfma_nt, sh.ngroups, sh.nchunks, s->mcols.p, s->mkmap.p, s->nk,
                                               s->rho_partial.p};
                 const size_t lds = rho_mfma_bytes(sh.width, kmax, true);
                 if (!cells)
-                    k_rho_mfma<float, float2><<<mgrid, 512, lds, st>>>(ra);
+                    launch_timed(s, LG_KERNEL_KSPACE_RHO, k_rho_mfma<float, float2>, mgrid, 512, lds, ra);
                 else if (near)
-                    k_cells_rho_mfma<true, float, float2><<<ncb + mgrid, kCellBlock, lds, st>>>(ca, ra, ncb);
+                    launch_timed(s, LG_KERNEL_KSPACE_RHO, k_cells_rho_mfma<true, float, float2>, ncb + mgrid, kCellBlock, lds, ca, ra, ncb);
                 else
-                    k_cells_rho_mfma<false, float, float2><<<ncb + mgrid, kCellBlock, lds, st>>>(ca, ra, ncb);
+                    launch_timed(s, LG_KERNEL_KSPACE_RHO, k_cells_rho_mfma<false, float, float2>, ncb + mgrid, kCellBlock, lds, ca, ra, ncb);
             } else {
                 if (!phase_done)
-                    k_phase<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                    launch_timed(s, LG_KERNEL_KSPACE_RHO, k_phase, blocks_for(size_t(3) * s->ld, 256), 256, 0, s->pos.p, n, s->ld, s->box, kmax,
                                                                                 s->phase_tab.p, s->phase_q.p);
                 RhoMfmaArgs<double, double2> ra{s->pos.p, n, s->box, s->ld, kmax, sh.wlog, sh.cpb,
                                                 s->mfma_nt, sh.ngroups, sh.nchunks, s->mcols.p, s->mkmap.p, s->nk,
                                                 s->rho_partial.p};
                 const size_t lds = rho_mfma_bytes(sh.width, kmax, false);
                 if (!cells)
-                    k_rho_mfma<double, double2><<<mgrid, 512, lds, st>>>(ra);
+                    launch_timed(s, LG_KERNEL_KSPACE_RHO, k_rho_mfma<double, double2>, mgrid, 512, lds, ra);
                 else if (near)
-                    k_cells_rho_mfma<true, double, double2><<<ncb + mgrid, kCellBlock, lds, st>>>(ca, ra, ncb);
+                    launch_timed(s, LG_KERNEL_KSPACE_RHO, k_cells_rho_mfma<true, double, double2>, ncb + mgrid, kCellBlock, lds, ca, ra, ncb);
                 else
-                    k_cells_rho_mfma<false, double, double2><<<ncb + mgrid, kCellBlock, lds, st>>>(ca, ra, ncb);
+                    launch_timed(s, LG_KERNEL_KSPACE_RHO, k_cells_rho_mfma<false, double, double2>, ncb + mgrid, kCellBlock, lds, ca, ra, ncb);
             }
         } else if (mixed) {
             if (!phase_done)
-                k_phase32<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_phase32, blocks_for(size_t(3) * s->ld, 256), 256, 0, s->pos.p, n, s->ld, s->box, kmax,
                                                                               s->phase_tab32.p, s->phase_q32.p);
             RhoRowsArgs<float, float2> ra{s->phase_tab32.p, s->phase_q32.p, s->ld, kmax, w, s->units.p,
                                           s->nunits, s->rho_partial.p, nub, nchunks};
             if (!cells)
-                k_rho_rows<float, float2><<<grid, 512, 0, st>>>(ra);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_rho_rows<float, float2>, grid, 512, 0, ra);
             else if (direct && near)
-                k_direct_rho<true, float, float2><<<ncb + grid, kCellBlock, 0, st>>>(da, ra, ncb);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<true, float, float2>, ncb + grid, kCellBlock, 0, da, ra, ncb);
             else if (direct)
-                k_direct_rho<false, float, float2><<<ncb + grid, kCellBlock, 0, st>>>(da, ra, ncb);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<false, float, float2>, ncb + grid, kCellBlock, 0, da, ra, ncb);
             else if (near)
-                k_cells_rho<true, float, float2><<<ncb + grid, kCellBlock, 0, st>>>(ca, ra, ncb);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_cells_rho<true, float, float2>, ncb + grid, kCellBlock, 0, ca, ra, ncb);
             else
-                k_cells_rho<false, float, float2><<<ncb + grid, kCellBlock, 0, st>>>(ca, ra, ncb);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_cells_rho<false, float, float2>, ncb + grid, kCellBlock, 0, ca, ra, ncb);
         } else {
             if (!phase_done)
-                k_phase<<<blocks_for(size_t(3) * s->ld, 256), 256, 0, st>>>(s->pos.p, n, s->ld, s->box, kmax,
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_phase, blocks_for(size_t(3) * s->ld, 256), 256, 0, s->pos.p, n, s->ld, s->box, kmax,
                                                                             s->phase_tab.p, s->phase_q.p);
             RhoRowsArgs<double, double2> ra{s->phase_tab.p, s->phase_q.p, s->ld, kmax, w, s->units.p,
                                             s->nunits, s->rho_partial.p, nub, nchunks};
             if (!cells)
-                k_rho_rows<double, double2><<<grid, 512, 0, st>>>(ra);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_rho_rows<double, double2>, grid, 512, 0, ra);
             else if (direct && near)
-                k_direct_rho<true, double, double2><<<ncb + grid, kCellBlock, 0, st>>>(da, ra, ncb);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<true, double, double2>, ncb + grid, kCellBlock, 0, da, ra, ncb);
             else if (direct)
-                k_direct_rho<false, double, double2><<<ncb + grid, kCellBlock, 0, st>>>(da, ra, ncb);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<false, double, double2>, ncb + grid, kCellBlock, 0, da, ra, ncb);
             else if (near)
-                k_cells_rho<true, double, double2><<<ncb + grid, kCellBlock, 0, st>>>(ca, ra, ncb);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_cells_rho<true, double, double2>, ncb + grid, kCellBlock, 0, ca, ra, ncb);
             else
-                k_cells_rho<false, double, double2><<<ncb + grid, kCellBlock, 0, st>>>(ca, ra, ncb);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_cells_rho<false, double, double2>, ncb + grid, kCellBlock, 0, ca, ra, ncb);
         }
         s->cells_pending = false;
         s->direct_pending = false;
-        prof_stop(s, pr);
     }
     if (s->cells_pending) {  // deferred build without k-space (not expected): launch it alone
         if (s->cells_nearest)
@@ -1397,6 +1429,12 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     pa.ecx64 = s->ecx_ok ? s->ecx64.p : nullptr;
     pa.ecx32 = s->ecx_ok ? s->ecx32.p : nullptr;
     pa.ecx_inv_h = s->ecx_inv_h;
+    pa.frac = s->frac.p;
+    if (mixed && !s->gen_pairs && !phase_done) {
+        // the MD half-steps write the FP32 positions beside the new ones;
+        // any other caller converts the current positions here
+        launch_timed(s, LG_KERNEL_PAIRS, k_frac, blocks_for(n, 256), 256, 0, s->pos.p, n, s->box, s->frac.p);
+    }
     // the intra-molecular terms run as the last blocks of the same launch
     IntraArgs& ia = pa.intra;
     ia = IntraArgs{};
@@ -1438,22 +1476,30 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         rr.epart = s->ep_k_at ? s->ep_k_at : s->epart_k.p;
         rblocks = blocks_for(s->nk, kRhoReduceK);
     }
-    long prof = prof_start(s, LG_KERNEL_PAIRS);
+    // the mixed instance leaves the intra-molecular terms to a launch of
+    // their own (see k_pairs_mixed)
+    const bool intra_apart = mixed && !s->gen_pairs;
+    if (intra_apart && iblocks > 0) {
+        if (energy)
+            launch_timed(s, LG_KERNEL_MOLECULES, k_intra_terms<true>, iblocks, 256, 0, ia);
+        else
+            launch_timed(s, LG_KERNEL_MOLECULES, k_intra_terms<false>, iblocks, 256, 0, ia);
+    }
     const int pblocks = blocks_for(size_t(n) * s->tpa, kPairBlock);
     pa.pair_blocks = pblocks;
-    pa.intra_blocks = iblocks;
-    const int nblocks = pblocks + iblocks + rblocks;
+    pa.intra_blocks = intra_apart ? 0 : iblocks;
+    const int nblocks = pblocks + pa.intra_blocks + rblocks;
     const int ck = coulomb_variant(s->coul);
 #define LG_PAIRS_CK(T, CK)                                                      \
     if (ck == CK) {                                                             \
         if (mixed && energy)                                                    \
-            k_pairs_mixed<true, T, CK><<<nblocks, kPairBlock, 0, st>>>(pa);     \
+            launch_timed(s, LG_KERNEL_PAIRS, k_pairs_mixed<true, T, CK>, nblocks, kPairBlock, 0, pa);     \
         else if (mixed)                                                         \
-            k_pairs_mixed<false, T, CK><<<nblocks, kPairBlock, 0, st>>>(pa);    \
+            launch_timed(s, LG_KERNEL_PAIRS, k_pairs_mixed<false, T, CK>, nblocks, kPairBlock, 0, pa);    \
         else if (energy)                                                        \
-            k_pairs<true, T, CK><<<nblocks, kPairBlock, 0, st>>>(pa);           \
+            launch_timed(s, LG_KERNEL_PAIRS, k_pairs<true, T, CK>, nblocks, kPairBlock, 0, pa);           \
         else                                                                    \
-            k_pairs<false, T, CK><<<nblocks, kPairBlock, 0, st>>>(pa);          \
+            launch_timed(s, LG_KERNEL_PAIRS, k_pairs<false, T, CK>, nblocks, kPairBlock, 0, pa);          \
     }
 #define LG_PAIRS_CASE(T)                 \
     case T:                              \
@@ -1466,15 +1512,15 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         // any pair potential kind or a table: the general FP64 instance, in
         // both forms
         if (energy) {
-            if (ck == CK_NONE) k_pairs<true, 16, CK_NONE, true><<<nblocks, kPairBlock, 0, st>>>(pa);
-            if (ck == CK_EWALD) k_pairs<true, 16, CK_EWALD, true><<<nblocks, kPairBlock, 0, st>>>(pa);
-            if (ck == CK_EWALD_ERF) k_pairs<true, 16, CK_EWALD_ERF, true><<<nblocks, kPairBlock, 0, st>>>(pa);
-            if (ck == CK_WOLF) k_pairs<true, 16, CK_WOLF, true><<<nblocks, kPairBlock, 0, st>>>(pa);
+            if (ck == CK_NONE) launch_timed(s, LG_KERNEL_PAIRS, k_pairs<true, 16, CK_NONE, true>, nblocks, kPairBlock, 0, pa);
+            if (ck == CK_EWALD) launch_timed(s, LG_KERNEL_PAIRS, k_pairs<true, 16, CK_EWALD, true>, nblocks, kPairBlock, 0, pa);
+            if (ck == CK_EWALD_ERF) launch_timed(s, LG_KERNEL_PAIRS, k_pairs<true, 16, CK_EWALD_ERF, true>, nblocks, kPairBlock, 0, pa);
+            if (ck == CK_WOLF) launch_timed(s, LG_KERNEL_PAIRS, k_pairs<true, 16, CK_WOLF, true>, nblocks, kPairBlock, 0, pa);
         } else {
-            if (ck == CK_NONE) k_pairs<false, 16, CK_NONE, true><<<nblocks, kPairBlock, 0, st>>>(pa);
-            if (ck == CK_EWALD) k_pairs<false, 16, CK_EWALD, true><<<nblocks, kPairBlock, 0, st>>>(pa);
-            if (ck == CK_EWALD_ERF) k_pairs<false, 16, CK_EWALD_ERF, true><<<nblocks, kPairBlock, 0, st>>>(pa);
-            if (ck == CK_WOLF) k_pairs<false, 16, CK_WOLF, true><<<nblocks, kPairBlock, 0, st>>>(pa);
+            if (ck == CK_NONE) launch_timed(s, LG_KERNEL_PAIRS, k_pairs<false, 16, CK_NONE, true>, nblocks, kPairBlock, 0, pa);
+            if (ck == CK_EWALD) launch_timed(s, LG_KERNEL_PAIRS, k_pairs<false, 16, CK_EWALD, true>, nblocks, kPairBlock, 0, pa);
+            if (ck == CK_EWALD_ERF) launch_timed(s, LG_KERNEL_PAIRS, k_pairs<false, 16, CK_EWALD_ERF, true>, nblocks, kPairBlock, 0, pa);
+            if (ck == CK_WOLF) launch_timed(s, LG_KERNEL_PAIRS, k_pairs<false, 16, CK_WOLF, true>, nblocks, kPairBlock, 0, pa);
         }
     } else switch (s->tpa) {
         LG_PAIRS_CASE(4)
@@ -1487,22 +1533,19 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     }
 #undef LG_PAIRS_CASE
 #undef LG_PAIRS_CK
-    prof_stop(s, prof);
     if (s->coul_kind == COUL_EWALD && s->nk > 0) {
         int kmax = s->coul_kmax;
-        long pf = prof_start(s, LG_KERNEL_KSPACE_FORCE);
         constexpr int kApl = 1;  // atoms per lane (2 measured slower at 4k-9k atoms)
         int nab = blocks_for(n, 64 * kApl);
         unsigned fgrid = xcd_grid(s->ksplit, nab);
         if (mixed)
-            k_kspace_force<float, float2, KRec32, kApl><<<fgrid, 256, 0, st>>>(
+            launch_timed(s, LG_KERNEL_KSPACE_FORCE, k_kspace_force<float, float2, KRec32, kApl>, fgrid, 256, 0, 
                 s->phase_tab32.p, s->ld, n, kmax, s->rows.p, s->nrows, s->krec32.p, s->kg[0], s->kg[1], s->kg[2],
                 s->ksplit, nab, s->fk.p);
         else
-            k_kspace_force<double, double2, KRec, kApl><<<fgrid, 256, 0, st>>>(
+            launch_timed(s, LG_KERNEL_KSPACE_FORCE, k_kspace_force<double, double2, KRec, kApl>, fgrid, 256, 0, 
                 s->phase_tab.p, s->ld, n, kmax, s->rows.p, s->nrows, s->krec.p, s->kg[0], s->kg[1], s->kg[2],
                 s->ksplit, nab, s->fk.p);
-        prof_stop(s, pf);
     }
     if (combine && (kspace || s->nterms > 0)) {
         // timed with the k-space forces: it adds their partials
@@ -1915,6 +1958,7 @@ int lg_system_destroy(lg_system* s) {
     s->mc_out.release();
     s->mc_cell_pos.release();
     s->cpos.release();
+    s->frac.release();
     s->mc_cell_ids.release();
     s->bparam.release();
     s->aparam.release();

@@ -77,16 +77,45 @@ __host__ __device__ __forceinline__ int cell_coord(double x, double L, double in
 // kernels that see each new position (the displacement test, the first half
 // of an MD step): cx | cy << 10 | cz << 20 (cells per axis <= 256).  Off
 // when tag is null.
+// What the MD half-steps write beside a new position: the cell tag of the
+// direct Verlet build (tag, or null) and the position as the mixed pair
+// kernel reads it (frac, or null; see frac_pos).
 struct CellTag {
     int* tag;
     Grid g;
     Box b;
+    float4* frac;
 };
 __host__ __device__ __forceinline__ int cell_tag(const CellTag& t, double4 p) {
     const int cx = cell_coord(p.x, t.b.L[0], t.b.inv[0], t.g.n[0], t.g.inv_size[0], t.b.periodic);
     const int cy = cell_coord(p.y, t.b.L[1], t.b.inv[1], t.g.n[1], t.g.inv_size[1], t.b.periodic);
     const int cz = cell_coord(p.z, t.b.L[2], t.b.inv[2], t.g.n[2], t.g.inv_size[2], t.b.periodic);
     return cx | (cy << 10) | (cz << 20);
+}
+
+// Position of an atom as the mixed pair kernel reads it, 16 bytes instead
+// of 32 (the pair kernel's gathers move half the bytes).  In a periodic
+// cell: the fractional coordinate x / L - floor(x / L), formed in FP64, as a
+// 32-bit fixed-point fraction (units of L / 2^32, stored in the bits of the
+// float lanes); the difference of two such words as a signed 32-bit integer
+// is then the minimum image, exactly, at a resolution of L / 2^32, and the
+// one rounding of the pair distance is its conversion to FP32, as when an
+// FP64 difference is rounded.  In an infinite cell: Cartesian FP32.  The
+// charge rides in w.
+__device__ __forceinline__ unsigned fixed_frac(double x, double inv) {
+    double t = x * inv;
+    t = (t - floor(t)) * 4294967296.0;                   // [0, 2^32]
+    return (unsigned)(unsigned long long)(long long)rint(t);  // 2^32 wraps to 0
+}
+__device__ __forceinline__ float4 frac_pos(const Box& b, double4 p) {
+    if (!b.periodic) return make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+    return make_float4(__uint_as_float(fixed_frac(p.x, b.inv[0])), __uint_as_float(fixed_frac(p.y, b.inv[1])),
+                       __uint_as_float(fixed_frac(p.z, b.inv[2])), (float)p.w);
+}
+
+__global__ void k_frac(const double4* __restrict__ pos, int n, Box b, float4* __restrict__ frac) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) frac[i] = frac_pos(b, pos[i]);
 }
 
 __global__ void k_nlist_check(const double4* __restrict__ pos, const double4* __restrict__ ref, int n,
@@ -1316,6 +1345,7 @@ struct PairArgs {
     const float* ecx32;
     double ecx_inv_h;
     const double* tabs;  // TableComputation tables (general instance)
+    const float4* frac;  // positions as frac_pos (mixed instance)
 };
 
 constexpr int kMaxKinds = 16;
@@ -1492,47 +1522,74 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(4)))
     }
 }
 
-// Mixed form of k_pairs: the minimum-image difference is taken in FP64 and
-// rounded to FP32; distance, LJ, erfc and the Ewald/Wolf force factor are
-// FP32 (erfc from erfc_cheb); the FP32 pair forces are accumulated in FP64
-// (two operations per component, against seven for a compensated FP32 sum,
-// and no less accurate).  Energies and virial (energy mode) are accumulated
-// in FP64.
+// Mixed form of k_pairs: positions are read as 32-bit fixed-point
+// fractions (frac_pos, 16-byte gathers) whose wrapped difference is the
+// minimum image, rounded to FP32 once; distance, LJ, erfc and the
+// Ewald/Wolf force factor are FP32; the FP32 pair forces are accumulated in
+// FP64 (two operations per component, against seven for a compensated FP32
+// sum, and no less accurate).  Energies and virial (energy mode) are
+// accumulated in FP64.
+//
+// The kernel issues few instructions per pair (it is bound by the vector
+// ALU when several systems share the device): a kind pair is one 16-byte
+// LDS record {48 eps s^12, 24 eps s^6, rc^2 (0 when the kind pair has no
+// term: Null, or excluded between molecules), shift}, the LJ force is
+// (A r^-6 - B) r^-8 evaluated for every pair and selected by r^2 < rc^2, so
+// that a Null or excluded kind pair adds an exact 0; Harmonic pair terms,
+// rare, sit in a second record read only when the system has one (a
+// uniform flag).  The Coulomb term is evaluated for every pair and selected
+// by r <= rc_coulomb (q_i q_j = 0 gives an exact 0).  1/r is the hardware
+// reciprocal square root (1 ulp).
+//
+// The launch carries the rho(k) reduction blocks but not the
+// intra-molecular terms (k_intra_terms, a launch of its own): their code
+// needs 104 VGPRs, which would hold every pair block of the launch to four
+// waves per SIMD, against eight for the pair code alone.
 template <bool kEnergy, int TPA, int CK>
 __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
-    struct P32 {
-        int pot, inter_excl;
-        float p0, p1, rc2, shift;
-    };
-    __shared__ P32 tab[kMaxKinds * kMaxKinds];
+    __shared__ float4 ljt[kMaxKinds * kMaxKinds];
+    __shared__ float2 hmt[kMaxKinds * kMaxKinds];  // Harmonic {k, x0}, k = 0 otherwise
     __shared__ double red[kPairBlock / 64][8];
     constexpr bool kTab = CK == CK_EWALD || CK == CK_WOLF;
-    __shared__ float ecx[kTab ? kErfcxPieces * (kErfcxDeg32 + 1) : 1];
+    __shared__ __align__(16) float ecx[kTab ? kErfcxPieces * (kErfcxDeg32 + 1) : 4];
     if ((int)blockIdx.x >= a.pair_blocks) {  // uniform over the block
-        const int b = blockIdx.x - a.pair_blocks;
-        if (b < a.intra_blocks)
-            intra_block<kEnergy>(a.intra, b);
-        else
-            rho_reduce_block<kEnergy>(a.rr, b - a.intra_blocks);
+        rho_reduce_block<kEnergy>(a.rr, blockIdx.x - a.pair_blocks);
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.rebuild) *a.rebuild = 0;
-    int nk2 = a.nkinds * a.nkinds;
+    const int nk2 = a.nkinds * a.nkinds;
+    int harmonic = 0;
     for (int x = threadIdx.x; x < nk2; x += blockDim.x) {
-        PairParam pp = a.table[x];
-        float rc = (float)pp.rc;
-        tab[x] = P32{pp.pot, pp.inter_excl, (float)pp.p0, (float)pp.p1, rc * rc, (float)pp.shift};
+        const PairParam pp = a.table[x];
+        const bool live = pp.pot > POT_NULL && !pp.inter_excl;
+        const float rc = (float)pp.rc;
+        const bool lj = pp.pot == POT_LJ;
+        const double s6 = lj ? pp.p0 * pp.p0 * pp.p0 * pp.p0 * pp.p0 * pp.p0 : 0.0;
+        ljt[x] = make_float4(lj ? (float)(48.0 * pp.p1 * s6 * s6) : 0.f, lj ? (float)(24.0 * pp.p1 * s6) : 0.f,
+                             live ? rc * rc : 0.f, (float)pp.shift);
+        const bool hm = live && pp.pot == POT_HARMONIC;
+        hmt[x] = make_float2(hm ? (float)pp.p0 : 0.f, hm ? (float)pp.p1 : 0.f);
+        harmonic |= hm;
     }
     const bool use_tab = kTab && a.ecx32 != nullptr;  // uniform
-    if (use_tab)
-        for (int x = threadIdx.x; x < kErfcxPieces * (kErfcxDeg32 + 1); x += blockDim.x) ecx[x] = a.ecx32[x];
-    __syncthreads();
+    static_assert(kErfcxDeg32 + 1 == 4, "a piece of the FP32 erfcx table is one float4");
+    if (use_tab) {
+        // one piece (4 coefficients) per 16-byte copy
+        for (int x = threadIdx.x; x < kErfcxPieces; x += blockDim.x)
+            reinterpret_cast<float4*>(ecx)[x] = reinterpret_cast<const float4*>(a.ecx32)[x];
+    }
+    harmonic = __syncthreads_or(harmonic);  // also the barrier of the LDS tables
     const float ecx_inv_h = (float)a.ecx_inv_h;
     const float alpha = (float)a.coul.alpha;
     const float rcc2 = (float)(a.coul.rc * a.coul.rc);
     const float g0 = (float)(a.coul.alpha * kFrac2SqrtPi);
     const float inv4pe = (float)(1.0 / kFourPiEps0);
     const float we = (float)a.coul.e_const, wf = (float)a.coul.f_const;
+    const bool periodic = a.box.periodic;  // uniform
+    // L / 2^32: one unit of the fixed-point fractions
+    const float Lx = (float)(a.box.L[0] * 0x1p-32), Ly = (float)(a.box.L[1] * 0x1p-32),
+                Lz = (float)(a.box.L[2] * 0x1p-32);
+    const float4* __restrict__ frac = a.frac;
 
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     int t = g / TPA;
@@ -1543,11 +1600,13 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
     int i = -1;
     if (t < a.n) {
         i = a.sorted[t];
-        double4 pi = a.pos[i];
-        int ki = a.meta[i].x;
-        float qi = (float)pi.w;
-        int nn = a.nnbr[t];
+        const float4 pi = frac[i];
+        const int ki = a.meta[i].x;
+        const float qi = pi.w;
+        const int nn = a.nnbr[t];
         const int* row = a.nbr + (size_t)t * a.cap;
+        const float4* __restrict__ ljrow = ljt + ki * a.nkinds;
+        const float2* __restrict__ hmrow = hmt + ki * a.nkinds;
         // Software pipeline over the row: the row entries two passes ahead
         // and the next pass's neighbour positions are loaded while the
         // current pair is evaluated, unrolled by two on alternating
@@ -1558,30 +1617,38 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
             int v = row[min(k, max(nn - 1, 0))];
             return k < nn ? v : nbr_entry(i, ki);
         };
-        auto pair = [&](int j, const double4& pj, int kj) {
-            double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
-            min_image_fast(a.box, ddx, ddy, ddz);
-            float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            float inv_r = rsqrtf(r2);
-            float inv_r2 = inv_r * inv_r;
-            float fs = 0.f, e_p = 0.f, e_c = 0.f;
-            const P32& pp = tab[ki * a.nkinds + kj];
-            if (pp.pot > POT_NULL && !pp.inter_excl && r2 < pp.rc2) {
-                if (pp.pot == POT_LJ) {
-                    float s2 = pp.p0 * pp.p0 * inv_r2;
-                    float s6 = s2 * s2 * s2;
-                    fs += 24.f * pp.p1 * (2.f * s6 * s6 - s6) * inv_r2;
-                    if (kEnergy) e_p = 4.f * pp.p1 * (s6 * s6 - s6) - pp.shift;
-                } else {
-                    float r = r2 * inv_r;
-                    fs += pp.p0 * (pp.p1 - r) * inv_r;
-                    if (kEnergy) e_p = 0.5f * pp.p0 * (r - pp.p1) * (r - pp.p1) - pp.shift;
-                }
+        auto pair = [&](int j, const float4& pj, int kj) {
+            float dx, dy, dz;
+            if (periodic) {
+                // the wrapped 32-bit difference is the minimum image
+                dx = (float)(int)(__float_as_uint(pi.x) - __float_as_uint(pj.x)) * Lx;
+                dy = (float)(int)(__float_as_uint(pi.y) - __float_as_uint(pj.y)) * Ly;
+                dz = (float)(int)(__float_as_uint(pi.z) - __float_as_uint(pj.z)) * Lz;
+            } else {
+                dx = pi.x - pj.x;
+                dy = pi.y - pj.y;
+                dz = pi.z - pj.z;
             }
-            float qiqj = qi * (float)pj.w;
-            if (CK != CK_NONE && qiqj != 0.f && r2 <= rcc2) {
-                float r = r2 * inv_r;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float inv_r = __builtin_amdgcn_rsqf(r2);
+            const float inv_r2 = inv_r * inv_r;
+            const float4 P = ljrow[kj];
+            const bool in_pair = r2 < P.z;
+            const float i6 = inv_r2 * inv_r2 * inv_r2;
+            float fp = (P.x * i6 - P.y) * (i6 * inv_r2);
+            float e_p = 0.f, e_c = 0.f;
+            if (kEnergy) e_p = i6 * (P.x * (1.f / 12.f) * i6 - P.y * (1.f / 6.f));
+            if (harmonic) {  // uniform
+                const float2 H = hmrow[kj];
+                const float r = r2 * inv_r;
+                fp += H.x * (H.y - r) * inv_r;
+                if (kEnergy) e_p += 0.5f * H.x * (r - H.y) * (r - H.y);
+            }
+            float fs = in_pair ? fp : 0.f;
+            if (kEnergy) e_p = in_pair ? e_p - P.w : 0.f;
+            const float qiqj = qi * pj.w;
+            if (CK != CK_NONE) {
+                const float r = r2 * inv_r;
                 float ex2, erfcv;
                 if (kTab && use_tab) {
                     const float x = alpha * r;
@@ -1590,19 +1657,23 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
                 } else {
                     erfcv = erfc_cheb(alpha * r, ex2);
                 }
-                float pref = qiqj * inv4pe;
+                const float pref = qiqj * inv4pe;
+                float fc;
                 if (CK == CK_EWALD_ERF) {
-                    float erfv = 1.f - erfcv;
-                    fs += pref * inv_r2 * (g0 * ex2 - erfv * inv_r);
+                    const float erfv = 1.f - erfcv;
+                    fc = pref * inv_r2 * (g0 * ex2 - erfv * inv_r);
                     if (kEnergy) e_c = -pref * erfv * inv_r;
                 } else if (CK == CK_EWALD) {
-                    fs += pref * inv_r2 * (g0 * ex2 + erfcv * inv_r);
+                    fc = pref * inv_r2 * (g0 * ex2 + erfcv * inv_r);
                     if (kEnergy) e_c = pref * erfcv * inv_r;
                 } else {
-                    float factor = erfcv * inv_r2 + g0 * ex2 * inv_r;
-                    fs += pref * (factor - wf) * inv_r;
+                    const float factor = erfcv * inv_r2 + g0 * ex2 * inv_r;
+                    fc = pref * (factor - wf) * inv_r;
                     if (kEnergy) e_c = pref * (erfcv * inv_r - we);
                 }
+                const bool in_coul = r2 <= rcc2;
+                fs += in_coul ? fc : 0.f;
+                if (kEnergy) e_c = in_coul ? e_c : 0.f;
             }
             fx += (double)(fs * dx);
             fy += (double)(fs * dy);
@@ -1610,7 +1681,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
             if (kEnergy && j > i) {
                 e_pair += e_p;
                 e_coul += e_c;
-                double f = fs;
+                const double f = fs, ddx = dx, ddy = dy, ddz = dz;
                 wxx += f * ddx * ddx;
                 wyy += f * ddy * ddy;
                 wzz += f * ddz * ddz;
@@ -1621,13 +1692,13 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
         };
         int k = sub;
         int eA = jat(k), eB = jat(k + TPA);  // list entries: atom and kind
-        double4 pA = a.pos[nbr_atom(eA)], pB = a.pos[nbr_atom(eB)];
+        float4 pA = frac[nbr_atom(eA)], pB = frac[nbr_atom(eB)];
         for (; k < nn; k += 2 * TPA) {
             const int eA2 = jat(k + 2 * TPA), eB2 = jat(k + 3 * TPA);
             pair(nbr_atom(eA), pA, nbr_kind(eA));
-            pA = a.pos[nbr_atom(eA2)];
+            pA = frac[nbr_atom(eA2)];
             if (k + TPA < nn) pair(nbr_atom(eB), pB, nbr_kind(eB));
-            pB = a.pos[nbr_atom(eB2)];
+            pB = frac[nbr_atom(eB2)];
             eA = eA2;
             eB = eB2;
         }
@@ -2615,6 +2686,7 @@ __global__ __launch_bounds__(512) void k_vv_first(double4* __restrict__ pos, dou
             moved = moved_beyond(p, ref, i, half_skin2);
             bad = nonfinite3(p.x, p.y, p.z) || nonfinite3(v.x, v.y, v.z);
             if (tg.tag) tg.tag[i] = cell_tag(tg, p);
+            if (tg.frac) tg.frac[i] = frac_pos(tg.b, p);
         }
         if (__any(moved) && lane == 0) atomicOr(flags, 1);
         flag_nonfinite(bad, flags);
@@ -2660,6 +2732,7 @@ __global__ __launch_bounds__(512) void k_lf_first(double4* __restrict__ pos, dou
             moved = moved_beyond(p, ref, i, half_skin2);
             bad = nonfinite3(p.x, p.y, p.z) || nonfinite3(v.x, v.y, v.z);
             if (tg.tag) tg.tag[i] = cell_tag(tg, p);
+            if (tg.frac) tg.frac[i] = frac_pos(tg.b, p);
         }
         if (__any(moved) && lane == 0) atomicOr(flags, 1);
         flag_nonfinite(bad, flags);
@@ -2777,6 +2850,7 @@ __global__ __launch_bounds__(512) void k_md_fused(const double4* __restrict__ fo
             moved = moved_beyond(p, ref, i, half_skin2);
             bad = nonfinite3(p.x, p.y, p.z) || nonfinite3(v.x, v.y, v.z);
             if (tg.tag) tg.tag[i] = cell_tag(tg, p);
+            if (tg.frac) tg.frac[i] = frac_pos(tg.b, p);
         }
         if (__any(moved) && lane == 0) atomicOr(flags, 1);
         flag_nonfinite(bad, flags);

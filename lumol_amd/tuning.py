@@ -22,6 +22,8 @@ from . import _native as nat
 PAIR_LANES = (8, 16, 32, 64)
 KSPACE_SPLITS = (2, 4, 8, 16, 32)
 RHO_WIDTHS = (64, 128, 256, 512)
+NLIST_BUILDS = (0, 2)  # automatic, cell list (lg_tuning.nlist_build)
+RHO_KINDS = (1, 2)  # rows, matrix cores (lg_tuning.rho_kind)
 
 _cache = {}
 _lock = threading.Lock()
@@ -112,19 +114,27 @@ def autotune_concurrent(systems, run, rounds=2):
     not those for several sharing the device (four SPC/E systems: wider rho
     blocks).  `run()` advances every system by the same fixed work,
     concurrently, and returns its wall time.  The knobs are chosen in turn
-    (pair lanes, then the k-space split and rho width), every candidate timed
+    (pair lanes, then the k-space split, rho width and rho form, then the
+    list build), every candidate timed
     in `rounds` interleaved passes and the fastest pass kept.  Returns the
     knobs (dict)."""
-    keep = [tuple(getattr(s, "tuning_", (0, 0, 0, 0, 0, 0))[3:5]) for s in systems]
-    best = {"pair_lanes": 0, "kspace_split": 0, "rho_width": 0}
+    graphs = [getattr(s, "tuning_", (0, 0, 0, 0, 0, 0))[4] for s in systems]
+    best = {"pair_lanes": 0, "kspace_split": 0, "rho_width": 0, "nlist_build": 0, "rho_kind": 0}
 
     def apply(b):
-        for s, k in zip(systems, keep):
-            s.set_tuning(b["pair_lanes"], b["kspace_split"], b["rho_width"], *k)
+        for s, g in zip(systems, graphs):
+            s.set_tuning(b["pair_lanes"], b["kspace_split"], b["rho_width"], b["nlist_build"], g,
+                         rho_kind=b["rho_kind"])
 
     fields = [("pair_lanes", PAIR_LANES)]
     if bucket(systems[0])[0]:
-        fields += [("kspace_split", KSPACE_SPLITS), ("rho_width", RHO_WIDTHS)]
+        # the matrix-core rho(k) leaves the vector ALU to the other systems'
+        # kernels: it can win in a batch where it loses alone
+        fields += [("kspace_split", KSPACE_SPLITS), ("rho_width", RHO_WIDTHS), ("rho_kind", RHO_KINDS)]
+    # the list build: the automatic choice against the cell-list build, whose
+    # rebuilds cost the vector ALU less (the direct build reads every
+    # atom's cell tag in every cell's workgroup) at one more launch per step
+    fields += [("nlist_build", NLIST_BUILDS)]
     for field, values in fields:
         times = {v: math.inf for v in values}
         for _ in range(rounds):

@@ -68,15 +68,19 @@ def test_autotune_concurrent_applies_one_choice():
         return time.perf_counter() - t0
 
     best = tuning.autotune_concurrent(systems, run, rounds=1)
-    assert len(calls) == len(tuning.PAIR_LANES) + len(tuning.KSPACE_SPLITS) + len(tuning.RHO_WIDTHS)
+    assert len(calls) == (len(tuning.PAIR_LANES) + len(tuning.KSPACE_SPLITS) + len(tuning.RHO_WIDTHS)
+                          + len(tuning.RHO_KINDS) + len(tuning.NLIST_BUILDS))
     assert best["pair_lanes"] in tuning.PAIR_LANES and best["kspace_split"] in tuning.KSPACE_SPLITS
-    assert best["rho_width"] in tuning.RHO_WIDTHS
+    assert best["rho_width"] in tuning.RHO_WIDTHS and best["nlist_build"] in tuning.NLIST_BUILDS
     for s in systems:
         t = s.tuning()
         assert (t["pair_lanes"], t["kspace_split"], t["rho_width"]) == (
             best["pair_lanes"], best["kspace_split"], best["rho_width"])
+        if best["nlist_build"]:
+            assert t["nlist_build"] == best["nlist_build"]
     # fresh copies of the initial states with the chosen knobs
     for seed, f in zip((21, 22), f0):
         s = spce(nmol=300, L=20.8, cutoff=8.0, kmax=7, seed=seed)
-        s.set_tuning(pair_lanes=best["pair_lanes"], kspace_split=best["kspace_split"], rho_width=best["rho_width"])
+        s.set_tuning(pair_lanes=best["pair_lanes"], kspace_split=best["kspace_split"], rho_width=best["rho_width"],
+                     nlist_build=best["nlist_build"], rho_kind=best["rho_kind"])
         assert frel(s.forces(), f) < 1e-12
