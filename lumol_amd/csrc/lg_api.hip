@@ -954,8 +954,9 @@ int prepare_kspace(lg_system* s) {
     s->nrows = int(rows.size());
     std::vector<KUnit> units;
     for (const KRow& r : rows)
-        for (int j = 0; j < r.count; j += 4)
-            units.push_back(KUnit{r.ikx, r.iky, short(r.ikz0 + j), short(std::min(4, r.count - j)), r.kbase + j, 0});
+        for (int j = 0; j < r.count; j += kRhoUnitK)
+            units.push_back(
+                KUnit{r.ikx, r.iky, short(r.ikz0 + j), short(std::min(kRhoUnitK, r.count - j)), r.kbase + j, 0});
     s->nunits = int(units.size());
     std::vector<int> kslot(s->nk);
     for (int u = 0; u < s->nunits; u++)
@@ -1011,7 +1012,7 @@ int prepare_kspace(lg_system* s) {
     s->ld = blocks_for(s->n, 64) * 64;
     // rho partials: the row form's narrowest atom chunk (64) or the
     // matrix-core form's chunks, whichever needs more
-    size_t npart = size_t(blocks_for(s->ld, 64)) * 4 * s->nunits;
+    size_t npart = size_t(blocks_for(s->ld, 64)) * kRhoUnitK * s->nunits;
     if (s->mfma_nt > 0)
         for (int m = 0; m < 2; m++) npart = std::max(npart, size_t(rho_mfma_shape(s, m == 1).nchunks) * s->nk);
     LG_TRY(s->rho_partial.ensure(npart));
@@ -1465,7 +1466,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     if (kspace) {
         rr.partial = s->rho_partial.p;
         rr.nchunks = nchunks;
-        rr.nslot = use_mfma ? s->nk : 4 * s->nunits;
+        rr.nslot = use_mfma ? s->nk : kRhoUnitK * s->nunits;
         rr.kslot = use_mfma ? s->kslot_id.p : s->kslot.p;
         rr.nk = s->nk;
         rr.rho = s->rho.p;

@@ -1525,9 +1525,9 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(4)))
 // Mixed form of k_pairs: positions are read as 32-bit fixed-point
 // fractions (frac_pos, 16-byte gathers) whose wrapped difference is the
 // minimum image, rounded to FP32 once; distance, LJ, erfc and the
-// Ewald/Wolf force factor are FP32; the FP32 pair forces are accumulated in
-// FP64 (two operations per component, against seven for a compensated FP32
-// sum, and no less accurate).  Energies and virial (energy mode) are
+// Ewald/Wolf force factor are FP32; the FP32 pair forces of each pass (two
+// pairs) are added in FP32 and accumulated in FP64 (two operations per
+// component and pass, against seven per pair for a compensated FP32 sum).  Energies and virial (energy mode) are
 // accumulated in FP64.
 //
 // The kernel issues few instructions per pair (it is bound by the vector
@@ -1617,8 +1617,10 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
             int v = row[min(k, max(nn - 1, 0))];
             return k < nn ? v : nbr_entry(i, ki);
         };
-        auto pair = [&](int j, const float4& pj, int kj) {
-            float dx, dy, dz;
+        // one list entry: returns the force factor f/r, with the difference
+        // vector in dx, dy, dz (the energies and virial, in energy mode,
+        // are accumulated here in FP64)
+        auto pair = [&](int j, const float4& pj, int kj, float& dx, float& dy, float& dz) -> float {
             if (periodic) {
                 // the wrapped 32-bit difference is the minimum image
                 dx = (float)(int)(__float_as_uint(pi.x) - __float_as_uint(pj.x)) * Lx;
@@ -1675,9 +1677,6 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
                 fs += in_coul ? fc : 0.f;
                 if (kEnergy) e_c = in_coul ? e_c : 0.f;
             }
-            fx += (double)(fs * dx);
-            fy += (double)(fs * dy);
-            fz += (double)(fs * dz);
             if (kEnergy && j > i) {
                 e_pair += e_p;
                 e_coul += e_c;
@@ -1689,16 +1688,30 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
                 wxz += f * ddx * ddz;
                 wyz += f * ddy * ddz;
             }
+            return fs;
         };
         int k = sub;
         int eA = jat(k), eB = jat(k + TPA);  // list entries: atom and kind
         float4 pA = frac[nbr_atom(eA)], pB = frac[nbr_atom(eB)];
         for (; k < nn; k += 2 * TPA) {
             const int eA2 = jat(k + 2 * TPA), eB2 = jat(k + 3 * TPA);
-            pair(nbr_atom(eA), pA, nbr_kind(eA));
+            // the two pair forces of a pass are added in FP32 (one rounding
+            // of their sum), then to the FP64 accumulators
+            float ax, ay, az;
+            const float fa = pair(nbr_atom(eA), pA, nbr_kind(eA), ax, ay, az);
+            float tx = fa * ax, ty = fa * ay, tz = fa * az;
             pA = frac[nbr_atom(eA2)];
-            if (k + TPA < nn) pair(nbr_atom(eB), pB, nbr_kind(eB));
+            if (k + TPA < nn) {
+                float bx, by, bz;
+                const float fb = pair(nbr_atom(eB), pB, nbr_kind(eB), bx, by, bz);
+                tx += fb * bx;
+                ty += fb * by;
+                tz += fb * bz;
+            }
             pB = frac[nbr_atom(eB2)];
+            fx += (double)tx;
+            fy += (double)ty;
+            fz += (double)tz;
             eA = eA2;
             eB = eB2;
         }
@@ -2060,21 +2073,26 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// rho partial sums by row units.  A unit is up to 4 consecutive k-vectors of
-// one row (fixed ikx, iky; consecutive ikz): per atom, the lane forms
-// q e^{i k0.r} = q (Ex * Ey) * Ez[ikz0] and steps to the next ikz by one
-// product with Ez[1], 7.5 operations and one table read per (atom, k).
-// Block = 64 units (one per lane) x 8 waves; the block's atom chunk of
-// `width` atoms is split in eight and each wave sums its eighth; the eight
-// wave sums are added in order in LDS and wave j writes k-vector j of every
-// unit: partial[chunk * 4 * nunits + j * nunits + unit] (slots with j past
-// the unit's count are not written).  Table rows are read straight from L2:
-// a wave's lanes read few distinct rows at one atom column.  The table is
-// padded with zero charges to `ld` atoms.  Grid: xcd_grid(nub, nchunks),
-// so all unit blocks of an atom chunk share one L2.  Mixed form (T = float):
-// each wave's quarter is summed in FP32, the rest in FP64.
+// rho partial sums by row units.  A unit is up to kRhoUnitK (5) consecutive
+// k-vectors of one row (fixed ikx, iky; consecutive ikz): per atom, the lane
+// forms q e^{i k0.r} = q (Ex * Ey) * Ez[ikz0] and steps to the next ikz by
+// one product with Ez[1], 7.2 operations and 0.6 table reads per (atom, k).
+// Five per unit pads the k set least (units fill whole 64-lane blocks more
+// often than with four: SPC/E kmax 10, 480 units in 512 lanes against 589
+// in 640; 8.9 against 9.2 operations per k-vector and atom, lane padding
+// included).  Block = 64 units (one per lane) x 8 waves; the block's atom
+// chunk of `width` atoms is split in eight and each wave sums its eighth;
+// the eight wave sums are added in order in LDS and wave j < 5 writes
+// k-vector j of every unit: partial[chunk * 5 * nunits + j * nunits + unit]
+// (slots with j past the unit's count are not written).  Table rows are
+// read straight from L2: a wave's lanes read few distinct rows at one atom
+// column.  The table is padded with zero charges to `ld` atoms.  Grid:
+// xcd_grid(nub, nchunks), so all unit blocks of an atom chunk share one L2.
+// Mixed form (T = float): each wave's eighth is summed in FP32, the rest in
+// FP64.
+constexpr int kRhoUnitK = 5;
 struct KUnit {
-    short ikx, iky, kz0, nv;  // row, first ikz, k-vectors in the unit (1..4)
+    short ikx, iky, kz0, nv;  // row, first ikz, k-vectors in the unit (1..kRhoUnitK)
     int kb, pad;              // index of the unit's first k-vector
 };
 
@@ -2092,7 +2110,9 @@ struct RhoRowsArgs {
 // NW waves per block (8 in k_rho_rows and k_cells_rho), logical
 // block index L of the xcd_grid(nub, nchunks) layout.
 template <typename T, typename T2, int NW>
-__device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int L, double2 (*red)[4][64]) {
+__device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int L,
+                                               double2 (*red)[kRhoUnitK][64]) {
+    static_assert(NW >= kRhoUnitK, "one wave writes each k-vector of a unit");
     const T2* __restrict__ tab = ra.tab;
     const T* __restrict__ q = ra.q;
     const int ld = ra.ld, kmax = ra.kmax, width = ra.width, nunits = ra.nunits;
@@ -2112,7 +2132,9 @@ __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int
     // an atom are then scalar loads, three vector loads per atom remain
     const int a0 = __builtin_amdgcn_readfirstlane(chunk * width + w * per);
     const int a1 = min(a0 + per, ld);
-    T r0 = 0, i0 = 0, r1 = 0, i1 = 0, r2 = 0, i2 = 0, r3 = 0, i3 = 0;
+    T re[kRhoUnitK], im[kRhoUnitK];
+#pragma unroll
+    for (int v = 0; v < kRhoUnitK; v++) re[v] = im[v] = 0;
 #pragma unroll 4
     for (int a = a0; a < a1; a++) {
         const T qa = q[a];
@@ -2120,22 +2142,15 @@ __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int
         T2 e = cmul(cmul(cx[a], cy[a]), cz[a]);
         e.x *= qa;
         e.y *= qa;
-        r0 += e.x;
-        i0 += e.y;
-        e = cmul(e, e1);
-        r1 += e.x;
-        i1 += e.y;
-        e = cmul(e, e1);
-        r2 += e.x;
-        i2 += e.y;
-        e = cmul(e, e1);
-        r3 += e.x;
-        i3 += e.y;
+#pragma unroll
+        for (int v = 0; v < kRhoUnitK; v++) {
+            if (v > 0) e = cmul(e, e1);
+            re[v] += e.x;
+            im[v] += e.y;
+        }
     }
-    red[w][0][lane] = make_double2(r0, i0);
-    red[w][1][lane] = make_double2(r1, i1);
-    red[w][2][lane] = make_double2(r2, i2);
-    red[w][3][lane] = make_double2(r3, i3);
+#pragma unroll
+    for (int v = 0; v < kRhoUnitK; v++) red[w][v][lane] = make_double2(re[v], im[v]);
     __syncthreads();
     if (u < nunits && w < un.nv) {
         double2 sum = red[0][w][lane];
@@ -2144,7 +2159,7 @@ __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int
             sum.x += red[v][w][lane].x;
             sum.y += red[v][w][lane].y;
         }
-        partial[(size_t)chunk * 4 * nunits + (size_t)w * nunits + u] = sum;
+        partial[(size_t)chunk * kRhoUnitK * nunits + (size_t)w * nunits + u] = sum;
     }
 }
 
@@ -2152,7 +2167,7 @@ __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int
 // sums do not depend on which launch computed them.
 template <typename T, typename T2>
 __global__ __launch_bounds__(512) void k_rho_rows(RhoRowsArgs<T, T2> ra) {
-    __shared__ double2 red[8][4][64];
+    __shared__ double2 red[8][kRhoUnitK][64];
     rho_rows_block<T, T2, 8>(ra, blockIdx.x, red);
 }
 
@@ -2161,25 +2176,26 @@ __global__ __launch_bounds__(512) void k_rho_rows(RhoRowsArgs<T, T2> ra) {
 // rho blocks that follow keep the XCD layout of xcd_grid), the rest compute
 // the rho(k) partials with 8 waves per block.  Both read only what the
 // first half-step and the cell list wrote.
+constexpr size_t kRhoRed = sizeof(double2) * (kCellBlock / 64) * kRhoUnitK * 64;
 template <bool kMinImage, typename T, typename T2>
 __global__ __launch_bounds__(kCellBlock) void k_cells_rho(CellsArgs ca, RhoRowsArgs<T, T2> ra, int ncb) {
-    constexpr size_t kRed = sizeof(double2) * (kCellBlock / 64) * 4 * 64;
-    __shared__ __align__(16) unsigned char buf[sizeof(CellsLds) > kRed ? sizeof(CellsLds) : kRed];
+    __shared__ __align__(16) unsigned char buf[sizeof(CellsLds) > kRhoRed ? sizeof(CellsLds) : kRhoRed];
     if ((int)blockIdx.x < ncb)
         nlist_cells_block<kMinImage>(ca, blockIdx.x, ncb, *reinterpret_cast<CellsLds*>(buf));
     else
-        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb, reinterpret_cast<double2(*)[4][64]>(buf));
+        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb,
+                                               reinterpret_cast<double2(*)[kRhoUnitK][64]>(buf));
 }
 
 // The same with the direct Verlet build in the first ncb blocks.
 template <bool kMinImage, typename T, typename T2>
 __global__ __launch_bounds__(kCellBlock) void k_direct_rho(DirectArgs da, RhoRowsArgs<T, T2> ra, int ncb) {
-    constexpr size_t kRed = sizeof(double2) * (kCellBlock / 64) * 4 * 64;
-    __shared__ __align__(16) unsigned char buf[sizeof(DirectLds) > kRed ? sizeof(DirectLds) : kRed];
+    __shared__ __align__(16) unsigned char buf[sizeof(DirectLds) > kRhoRed ? sizeof(DirectLds) : kRhoRed];
     if ((int)blockIdx.x < ncb)
         nlist_direct_block<kMinImage>(da, blockIdx.x, ncb, *reinterpret_cast<DirectLds*>(buf));
     else
-        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb, reinterpret_cast<double2(*)[4][64]>(buf));
+        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb,
+                                               reinterpret_cast<double2(*)[kRhoUnitK][64]>(buf));
 }
 
 
@@ -2456,7 +2472,8 @@ __device__ __forceinline__ void kforce_step(const R& r, T2& e, T2 ez1, T& A, T& 
 }
 
 // T = double: FP64 form; T = float: mixed form (FP32 phases, rho and
-// factors; each row's sums are added to FP64 accumulators).  Records are
+// factors; a wave's rows, a tenth of the k-vectors or fewer, are summed in
+// FP32 and its sum joins the FP64 sum of the block's waves).  Records are
 // read four at a time with one batch of scalar loads, so that a wave waits
 // once per four k-vectors (a copy of the block's records in LDS measured no
 // faster).  APL atoms per lane (block = 64 APL atoms): the row headers,
@@ -2482,7 +2499,7 @@ __global__ __launch_bounds__(256) void k_kspace_force(const T2* __restrict__ tab
         ez1[u] = col[u][(size_t)phase_row(2, 1, kmax) * ld];
     }
     const T tgx = (T)gx, tgy = (T)gy, tgz = (T)gz;
-    double fx[APL], fy[APL], fz[APL];
+    T fx[APL], fy[APL], fz[APL];
     // software pipeline: the next row's header and phase columns are loaded
     // while the current row is summed
     KRow nrow = rows[r0 < r1 ? r0 : 0];
@@ -2533,15 +2550,15 @@ __global__ __launch_bounds__(256) void k_kspace_force(const T2* __restrict__ tab
         }
 #pragma unroll
         for (int u = 0; u < APL; u++) {
-            fx[u] += (double)(A[u] * (tgx * (T)row.ikx));
-            fy[u] += (double)(A[u] * (tgy * (T)row.iky));
-            fz[u] += (double)(B[u] * tgz);
+            fx[u] += A[u] * (tgx * (T)row.ikx);
+            fy[u] += A[u] * (tgy * (T)row.iky);
+            fz[u] += B[u] * tgz;
         }
     }
 #pragma unroll
     for (int u = 0; u < APL; u++) {
         if (u > 0) __syncthreads();  // the LDS of the previous store is read before it is reused
-        kspace_block_store(fx[u], fy[u], fz[u], n, ablock * APL + u, split, fk);
+        kspace_block_store((double)fx[u], (double)fy[u], (double)fz[u], n, ablock * APL + u, split, fk);
     }
 }
 
