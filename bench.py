@@ -493,7 +493,9 @@ def run_batched(batch, steps, warmup, ref, md_graph=0):
             t.join()
         return time.perf_counter() - t0
 
-    knobs = tuning.autotune_concurrent(systems, lambda: concurrent(200, warm=20))
+    # 500 steps a candidate, best of three passes: with 200 steps and two
+    # passes the choice moved between runs by the noise of the batch
+    knobs = tuning.autotune_concurrent(systems, lambda: concurrent(500, warm=50), rounds=3)
     # the stated error: the first system's initial state with these knobs
     chk = make(3, 1)
     chk.set_tuning(knobs["pair_lanes"], knobs["kspace_split"], knobs["rho_width"], knobs["nlist_build"], md_graph,

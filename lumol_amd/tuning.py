@@ -114,8 +114,8 @@ def autotune_concurrent(systems, run, rounds=2):
     not those for several sharing the device (four SPC/E systems: wider rho
     blocks).  `run()` advances every system by the same fixed work,
     concurrently, and returns its wall time.  The knobs are chosen in turn
-    (pair lanes, then the k-space split, rho width and rho form, then the
-    list build), every candidate timed
+    (the list build, pair lanes, then the k-space split, rho width and rho
+    form), every candidate timed
     in `rounds` interleaved passes and the fastest pass kept.  Returns the
     knobs (dict)."""
     graphs = [getattr(s, "tuning_", (0, 0, 0, 0, 0, 0))[4] for s in systems]
@@ -126,15 +126,16 @@ def autotune_concurrent(systems, run, rounds=2):
             s.set_tuning(b["pair_lanes"], b["kspace_split"], b["rho_width"], b["nlist_build"], g,
                          rho_kind=b["rho_kind"])
 
-    fields = [("pair_lanes", PAIR_LANES)]
+    # the list build first (it sets the cost of every rebuild step, which
+    # the other knobs' timings include): the automatic choice against the
+    # cell-list build, whose rebuilds cost the vector ALU less (the direct
+    # build reads every atom's cell tag in every cell's workgroup) at one
+    # more launch per step
+    fields = [("nlist_build", NLIST_BUILDS), ("pair_lanes", PAIR_LANES)]
     if bucket(systems[0])[0]:
         # the matrix-core rho(k) leaves the vector ALU to the other systems'
         # kernels: it can win in a batch where it loses alone
         fields += [("kspace_split", KSPACE_SPLITS), ("rho_width", RHO_WIDTHS), ("rho_kind", RHO_KINDS)]
-    # the list build: the automatic choice against the cell-list build, whose
-    # rebuilds cost the vector ALU less (the direct build reads every
-    # atom's cell tag in every cell's workgroup) at one more launch per step
-    fields += [("nlist_build", NLIST_BUILDS)]
     for field, values in fields:
         times = {v: math.inf for v in values}
         for _ in range(rounds):
