@@ -404,6 +404,12 @@ struct lg_system {
     DevBuf<double> ecx64;
     DevBuf<float> ecx32;
     double ecx_inv_h = 0.0;
+    // the mixed pair kernel's real-space force factor
+    // h(r) = erfc(alpha r) + (2 alpha / sqrt(pi)) r exp(-alpha^2 r^2) on
+    // [0, rc], FP32 pieces (hforce_ld); hf_ok when the fit met 1e-7
+    DevBuf<float> hf32;
+    double hf_inv_h = 0.0;
+    bool hf_ok = false;
     bool ecx_ok = false;
     double kg[3] = {0, 0, 0};    // 2 pi / L per axis
     DevBuf<double> efac, vfac;
@@ -719,13 +725,15 @@ int prepare_meta(lg_system* s) {
     return LG_OK;
 }
 
-// erfcx(x) = exp(x^2) erfc(x) on [0, xmax] as kErfcxPieces polynomials of
-// degree `deg` in t in [-1, 1] per piece: the piece's Chebyshev interpolant
-// (deg + 1 Chebyshev nodes, long double), expanded in powers of t.  Returns
-// the largest relative error against erfcl over 64 points per piece.
+// A function f on [0, xmax] as kErfcxPieces polynomials of degree `deg` in
+// t in [-1, 1] per piece: the piece's Chebyshev interpolant (deg + 1
+// Chebyshev nodes, long double), expanded in powers of t.  Returns the
+// largest relative error against f (long double) over 64 points per piece.
+// Used for erfcx(x) = exp(x^2) erfc(x) and for the real-space force factor
+// of the mixed pair kernel (hforce_ld).
 static long double erfcx_ld(long double x) { return erfcl(x) * expl(x * x); }
-template <typename T>
-static double fit_erfcx(double xmax, int deg, std::vector<T>& out) {
+template <typename T, typename F>
+static double fit_pieces(double xmax, int deg, F fn, std::vector<T>& out) {
     const int M = kErfcxPieces, D = deg;
     const long double pi = 3.141592653589793238462643383279502884L;
     const long double h = (long double)xmax / M;
@@ -734,7 +742,7 @@ static double fit_erfcx(double xmax, int deg, std::vector<T>& out) {
     std::vector<long double> f(D + 1), c(D + 1), mono(D + 1), tprev(D + 1), tcur(D + 1), tnext(D + 1);
     for (int k = 0; k < M; k++) {
         const long double mid = (k + 0.5L) * h, half = 0.5L * h;
-        for (int j = 0; j <= D; j++) f[j] = erfcx_ld(mid + half * cosl(pi * (j + 0.5L) / (D + 1)));
+        for (int j = 0; j <= D; j++) f[j] = fn(mid + half * cosl(pi * (j + 0.5L) / (D + 1)));
         for (int m = 0; m <= D; m++) {
             long double acc = 0;
             for (int j = 0; j <= D; j++) acc += f[j] * cosl(m * pi * (j + 0.5L) / (D + 1));
@@ -764,12 +772,17 @@ static double fit_erfcx(double xmax, int deg, std::vector<T>& out) {
             const long double t = -1.0L + 2.0L * (z + 0.5L) / 64.0L;
             long double p = (long double)out[size_t(k) * (D + 1) + D];
             for (int q = D - 1; q >= 0; q--) p = p * t + (long double)out[size_t(k) * (D + 1) + q];
-            const long double ref = erfcx_ld(mid + half * t);
+            const long double ref = fn(mid + half * t);
             worst = std::max(worst, double(fabsl(p - ref) / ref));
         }
     }
     return worst;
 }
+template <typename T>
+static double fit_erfcx(double xmax, int deg, std::vector<T>& out) {
+    return fit_pieces(xmax, deg, erfcx_ld, out);
+}
+
 
 int prepare_params(lg_system* s) {
     uint32_t K = s->nkinds;
@@ -834,6 +847,7 @@ int prepare_params(lg_system* s) {
     // erfcx tables for the pair kernels' real-space erfc (Ewald and Wolf);
     // kept only if the fit meets 1e-14 (FP64) and 1e-7 (FP32) relative
     s->ecx_ok = false;
+    s->hf_ok = false;
     if ((s->coul_kind == COUL_EWALD || s->coul_kind == COUL_WOLF) && c.alpha > 0.0 && c.rc > 0.0) {
         const double xmax = c.alpha * c.rc * (1.0 + 1e-9);
         std::vector<double> t64;
@@ -845,6 +859,20 @@ int prepare_params(lg_system* s) {
             LG_TRY(upload(s->ecx32, t32, S(s)));
             s->ecx_inv_h = kErfcxPieces / xmax;
             s->ecx_ok = true;
+        }
+        // h(r): erfc(alpha r) r^-3 + g0 exp(-alpha^2 r^2) r^-2 = h(r) r^-3,
+        // the Ewald real-space force over r (and the r-dependent part of
+        // Wolf's), in one table lookup without an exponential
+        const long double al = c.alpha, g0 = 2.0L * al / sqrtl(3.141592653589793238462643383279502884L);
+        auto hforce_ld = [al, g0](long double r) { return erfcl(al * r) + g0 * r * expl(-al * al * r * r); };
+        const double rmax = c.rc * (1.0 + 1e-9);
+        std::vector<float> th;
+        const double eh = fit_pieces(rmax, kErfcxDeg32, hforce_ld, th);
+        s->hf_ok = false;
+        if (s->ecx_ok && eh < 1e-7) {
+            LG_TRY(upload(s->hf32, th, S(s)));
+            s->hf_inv_h = kErfcxPieces / rmax;
+            s->hf_ok = true;
         }
     }
     s->params_dirty = false;
@@ -1430,6 +1458,8 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     pa.ecx64 = s->ecx_ok ? s->ecx64.p : nullptr;
     pa.ecx32 = s->ecx_ok ? s->ecx32.p : nullptr;
     pa.ecx_inv_h = s->ecx_inv_h;
+    pa.hf32 = s->hf_ok ? s->hf32.p : nullptr;
+    pa.hf_inv_h = s->hf_inv_h;
     pa.frac = s->frac.p;
     if (mixed && !s->gen_pairs && !phase_done) {
         // the MD half-steps write the FP32 positions beside the new ones;
@@ -1934,6 +1964,7 @@ int lg_system_destroy(lg_system* s) {
     s->units.release();
     s->ecx64.release();
     s->ecx32.release();
+    s->hf32.release();
     s->phase_tab.release();
     s->phase_q.release();
     s->phase_tab32.release();

@@ -1346,6 +1346,10 @@ struct PairArgs {
     double ecx_inv_h;
     const double* tabs;  // TableComputation tables (general instance)
     const float4* frac;  // positions as frac_pos (mixed instance)
+    // the real-space force factor h(r) = erfc(alpha r) + g0 r exp(-alpha^2 r^2)
+    // in FP32 pieces over [0, rc] (mixed instance, Ewald and Wolf), or null
+    const float* hf32;
+    double hf_inv_h;
 };
 
 constexpr int kMaxKinds = 16;
@@ -1538,8 +1542,12 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(4)))
 // that a Null or excluded kind pair adds an exact 0; Harmonic pair terms,
 // rare, sit in a second record read only when the system has one (a
 // uniform flag).  The Coulomb term is evaluated for every pair and selected
-// by r <= rc_coulomb (q_i q_j = 0 gives an exact 0).  1/r is the hardware
-// reciprocal square root (1 ulp).
+// by r <= rc_coulomb (q_i q_j = 0 gives an exact 0); in the Ewald and Wolf
+// instances its r-dependence is h(r) r^-3 with h(r) = erfc(alpha r) + (2
+// alpha / sqrt(pi)) r exp(-alpha^2 r^2) read from a table of FP32 pieces
+// (fitted on the host to 1e-7 relative), one lookup and no exponential;
+// energies keep exp(-x^2) erfcx(x).  1/r is the hardware reciprocal square
+// root (1 ulp).
 //
 // The launch carries the rho(k) reduction blocks but not the
 // intra-molecular terms (k_intra_terms, a launch of its own): their code
@@ -1551,7 +1559,9 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
     __shared__ float2 hmt[kMaxKinds * kMaxKinds];  // Harmonic {k, x0}, k = 0 otherwise
     __shared__ double red[kPairBlock / 64][8];
     constexpr bool kTab = CK == CK_EWALD || CK == CK_WOLF;
+    // erfcx pieces (energies, and forces without the h table) and h pieces
     __shared__ __align__(16) float ecx[kTab ? kErfcxPieces * (kErfcxDeg32 + 1) : 4];
+    __shared__ __align__(16) float hft[kTab ? kErfcxPieces * (kErfcxDeg32 + 1) : 4];
     if ((int)blockIdx.x >= a.pair_blocks) {  // uniform over the block
         rho_reduce_block<kEnergy>(a.rr, blockIdx.x - a.pair_blocks);
         return;
@@ -1571,13 +1581,20 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
         hmt[x] = make_float2(hm ? (float)pp.p0 : 0.f, hm ? (float)pp.p1 : 0.f);
         harmonic |= hm;
     }
-    const bool use_tab = kTab && a.ecx32 != nullptr;  // uniform
-    static_assert(kErfcxDeg32 + 1 == 4, "a piece of the FP32 erfcx table is one float4");
-    if (use_tab) {
+    // uniform: the tables are there (the h table implies the erfcx table)
+    const bool use_h = kTab && a.hf32 != nullptr && a.ecx32 != nullptr;
+    const bool use_tab = kTab && a.ecx32 != nullptr;
+    static_assert(kErfcxDeg32 + 1 == 4, "a piece of the FP32 tables is one float4");
+    if (use_tab && (kEnergy || !use_h)) {
         // one piece (4 coefficients) per 16-byte copy
         for (int x = threadIdx.x; x < kErfcxPieces; x += blockDim.x)
             reinterpret_cast<float4*>(ecx)[x] = reinterpret_cast<const float4*>(a.ecx32)[x];
     }
+    if (use_h) {
+        for (int x = threadIdx.x; x < kErfcxPieces; x += blockDim.x)
+            reinterpret_cast<float4*>(hft)[x] = reinterpret_cast<const float4*>(a.hf32)[x];
+    }
+    const float hf_inv_h = (float)a.hf_inv_h;
     harmonic = __syncthreads_or(harmonic);  // also the barrier of the LDS tables
     const float ecx_inv_h = (float)a.ecx_inv_h;
     const float alpha = (float)a.coul.alpha;
@@ -1649,7 +1666,21 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
             float fs = in_pair ? fp : 0.f;
             if (kEnergy) e_p = in_pair ? e_p - P.w : 0.f;
             const float qiqj = qi * pj.w;
-            if (CK != CK_NONE) {
+            if (kTab && use_h) {
+                // forces from the h table: no exponential, one lookup
+                const float r = r2 * inv_r;
+                const float h = erfcx_tab(hft, r, hf_inv_h);
+                const float pref = qiqj * inv4pe;
+                const float inv_r3 = inv_r * inv_r2;
+                const float fc = CK == CK_EWALD ? pref * (h * inv_r3) : pref * (h * inv_r3 - wf * inv_r);
+                const bool in_coul = r2 <= rcc2;
+                fs += in_coul ? fc : 0.f;
+                if (kEnergy) {
+                    const float x = alpha * r;
+                    const float erfcv = __expf(-x * x) * erfcx_tab(ecx, x, ecx_inv_h);
+                    e_c = in_coul ? (CK == CK_EWALD ? pref * erfcv * inv_r : pref * (erfcv * inv_r - we)) : 0.f;
+                }
+            } else if (CK != CK_NONE) {
                 const float r = r2 * inv_r;
                 float ex2, erfcv;
                 if (kTab && use_tab) {
