@@ -493,9 +493,11 @@ def run_batched(batch, steps, warmup, ref, md_graph=0):
             t.join()
         return time.perf_counter() - t0
 
-    # 500 steps a candidate, best of three passes: with 200 steps and two
-    # passes the choice moved between runs by the noise of the batch
-    knobs = tuning.autotune_concurrent(systems, lambda: concurrent(500, warm=50), rounds=3)
+    # 500 steps a candidate, best of two timings, two passes of coordinate
+    # descent: with 200 steps and one pass the choice moved between runs by
+    # the noise of the batch, and one pass stopped at pair lanes 16 / split 8
+    # / rho width 128 (28.9 us per system-step) against 32 / 4 / 256 (28.0)
+    knobs = tuning.autotune_concurrent(systems, lambda: concurrent(500, warm=50), rounds=2, passes=2)
     # the stated error: the first system's initial state with these knobs
     chk = make(3, 1)
     chk.set_tuning(knobs["pair_lanes"], knobs["kspace_split"], knobs["rho_width"], knobs["nlist_build"], md_graph,

@@ -336,9 +336,12 @@ int lg_system_get_tuning(lg_system* sys, lg_tuning* tuning);
 /* ---- kernel timing ---------------------------------------------------------------
  * No counterpart in the reference (its benches time whole calls with
  * criterion, benches/argon.rs etc.).  Between begin and end, every launch of a kernel
- * class in `class_mask` (bit = LG_KERNEL_*) is bracketed by HIP events on the
- * system's stream; end synchronises and returns the summed device time and
- * the launch count per class. */
+ * class in `class_mask` (bit = LG_KERNEL_*) is timed on the system's stream:
+ * the pair, intra-molecular, rho(k) and k-space force launches by their
+ * kernels' own start and end timestamps (the events of
+ * hipExtLaunchKernelGGL, which agree with a kernel trace), the others by a
+ * HIP event pair around the launch; end synchronises and returns the summed
+ * device time and the launch count per class. */
 enum {
     LG_KERNEL_NLIST = 0,        /* displacement check + cell list + Verlet list */
     LG_KERNEL_PAIRS = 1,        /* k_pairs: LJ + Ewald real space / Wolf, and, in

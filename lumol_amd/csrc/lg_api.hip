@@ -374,6 +374,7 @@ struct lg_system {
     DevBuf<int> cellid, slot, count, start, tmp, sorted, nbr, nnbr, flags;  // flags[0] rebuild, [1] overflow
     DevBuf<float4> cpos;  // list-build candidate records in sorted order
     DevBuf<float4> frac;  // positions as the mixed pair kernel reads them (frac_pos)
+    double3 frac_origin = make_double3(0.0, 0.0, 0.0);  // infinite cells: atom 0 at the last set
     // parameters
     DevBuf<PairParam> table;
     DevBuf<double> ptabs;  // TableComputation tables: energies then forces per kind pair
@@ -1195,7 +1196,7 @@ static bool cell_build(const lg_system* s) {
 // direct build.
 static CellTag cell_tags(lg_system* s) {
     return CellTag{direct_build(s) ? s->cellid.p : nullptr, s->grid, s->box,
-                   s->precision == LG_PRECISION_MIXED ? s->frac.p : nullptr};
+                   s->precision == LG_PRECISION_MIXED ? s->frac.p : nullptr, s->frac_origin};
 }
 // Workgroups per cell of the direct build: about 256 workgroups in all (one
 // per CU of the MI355X), at most 8 per cell.
@@ -1464,7 +1465,8 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     if (mixed && !s->gen_pairs && !phase_done) {
         // the MD half-steps write the FP32 positions beside the new ones;
         // any other caller converts the current positions here
-        launch_timed(s, LG_KERNEL_PAIRS, k_frac, blocks_for(n, 256), 256, 0, s->pos.p, n, s->box, s->frac.p);
+        launch_timed(s, LG_KERNEL_PAIRS, k_frac, blocks_for(n, 256), 256, 0, s->pos.p, n, s->box, s->frac_origin,
+                     s->frac.p);
     }
     // the intra-molecular terms run as the last blocks of the same launch
     IntraArgs& ia = pa.intra;
@@ -2032,6 +2034,7 @@ int lg_system_set_positions(lg_system* s, const double* xyz) {
         if (!std::isfinite(xyz[i])) return fail(LG_E_NONFINITE, "non-finite position");
     LG_HIP(hipSetDevice(s->ctx->device));
     LG_TRY(push_vec3(s->pos, s->n, xyz, s->charge, S(s)));
+    if (s->n > 0) s->frac_origin = make_double3(xyz[0], xyz[1], xyz[2]);
     s->mc_ready = s->mc_staged = s->mc_cells_valid = false;
     if (!s->nlist_dirty && s->flags.p) LG_TRY(force_rebuild(s));
     return LG_OK;

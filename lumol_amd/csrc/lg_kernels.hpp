@@ -85,6 +85,7 @@ struct CellTag {
     Grid g;
     Box b;
     float4* frac;
+    double3 fo;  // origin of the FP32 Cartesian coordinates in an infinite cell
 };
 __host__ __device__ __forceinline__ int cell_tag(const CellTag& t, double4 p) {
     const int cx = cell_coord(p.x, t.b.L[0], t.b.inv[0], t.g.n[0], t.g.inv_size[0], t.b.periodic);
@@ -100,22 +101,25 @@ __host__ __device__ __forceinline__ int cell_tag(const CellTag& t, double4 p) {
 // float lanes); the difference of two such words as a signed 32-bit integer
 // is then the minimum image, exactly, at a resolution of L / 2^32, and the
 // one rounding of the pair distance is its conversion to FP32, as when an
-// FP64 difference is rounded.  In an infinite cell: Cartesian FP32.  The
-// charge rides in w.
+// FP64 difference is rounded.  In an infinite cell: FP32 Cartesian
+// coordinates relative to an origin `fo` (the first atom's position when
+// the positions were last set), formed in FP64, so that a cluster far from
+// the coordinate origin keeps the precision of one near it.  The charge
+// rides in w.
 __device__ __forceinline__ unsigned fixed_frac(double x, double inv) {
     double t = x * inv;
     t = (t - floor(t)) * 4294967296.0;                   // [0, 2^32]
     return (unsigned)(unsigned long long)(long long)rint(t);  // 2^32 wraps to 0
 }
-__device__ __forceinline__ float4 frac_pos(const Box& b, double4 p) {
-    if (!b.periodic) return make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+__device__ __forceinline__ float4 frac_pos(const Box& b, double3 fo, double4 p) {
+    if (!b.periodic) return make_float4((float)(p.x - fo.x), (float)(p.y - fo.y), (float)(p.z - fo.z), (float)p.w);
     return make_float4(__uint_as_float(fixed_frac(p.x, b.inv[0])), __uint_as_float(fixed_frac(p.y, b.inv[1])),
                        __uint_as_float(fixed_frac(p.z, b.inv[2])), (float)p.w);
 }
 
-__global__ void k_frac(const double4* __restrict__ pos, int n, Box b, float4* __restrict__ frac) {
+__global__ void k_frac(const double4* __restrict__ pos, int n, Box b, double3 fo, float4* __restrict__ frac) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) frac[i] = frac_pos(b, pos[i]);
+    if (i < n) frac[i] = frac_pos(b, fo, pos[i]);
 }
 
 __global__ void k_nlist_check(const double4* __restrict__ pos, const double4* __restrict__ ref, int n,
@@ -2734,7 +2738,7 @@ __global__ __launch_bounds__(512) void k_vv_first(double4* __restrict__ pos, dou
             moved = moved_beyond(p, ref, i, half_skin2);
             bad = nonfinite3(p.x, p.y, p.z) || nonfinite3(v.x, v.y, v.z);
             if (tg.tag) tg.tag[i] = cell_tag(tg, p);
-            if (tg.frac) tg.frac[i] = frac_pos(tg.b, p);
+            if (tg.frac) tg.frac[i] = frac_pos(tg.b, tg.fo, p);
         }
         if (__any(moved) && lane == 0) atomicOr(flags, 1);
         flag_nonfinite(bad, flags);
@@ -2780,7 +2784,7 @@ __global__ __launch_bounds__(512) void k_lf_first(double4* __restrict__ pos, dou
             moved = moved_beyond(p, ref, i, half_skin2);
             bad = nonfinite3(p.x, p.y, p.z) || nonfinite3(v.x, v.y, v.z);
             if (tg.tag) tg.tag[i] = cell_tag(tg, p);
-            if (tg.frac) tg.frac[i] = frac_pos(tg.b, p);
+            if (tg.frac) tg.frac[i] = frac_pos(tg.b, tg.fo, p);
         }
         if (__any(moved) && lane == 0) atomicOr(flags, 1);
         flag_nonfinite(bad, flags);
@@ -2898,7 +2902,7 @@ __global__ __launch_bounds__(512) void k_md_fused(const double4* __restrict__ fo
             moved = moved_beyond(p, ref, i, half_skin2);
             bad = nonfinite3(p.x, p.y, p.z) || nonfinite3(v.x, v.y, v.z);
             if (tg.tag) tg.tag[i] = cell_tag(tg, p);
-            if (tg.frac) tg.frac[i] = frac_pos(tg.b, p);
+            if (tg.frac) tg.frac[i] = frac_pos(tg.b, tg.fo, p);
         }
         if (__any(moved) && lane == 0) atomicOr(flags, 1);
         flag_nonfinite(bad, flags);

@@ -108,14 +108,14 @@ def autotune(system, reps=5, use_cache=True, rounds=3):
     return dict(hit)
 
 
-def autotune_concurrent(systems, run, rounds=2):
+def autotune_concurrent(systems, run, rounds=2, passes=2):
     """Picks and applies one set of knobs for systems that run at the same
     time, each on its own stream: the knobs fastest for one system alone are
     not those for several sharing the device (four SPC/E systems: wider rho
     blocks).  `run()` advances every system by the same fixed work,
     concurrently, and returns its wall time.  The knobs are chosen in turn
     (the list build, pair lanes, then the k-space split, rho width and rho
-    form), every candidate timed
+    form; `passes` rounds of this coordinate descent), every candidate timed
     in `rounds` interleaved passes and the fastest pass kept.  Returns the
     knobs (dict)."""
     graphs = [getattr(s, "tuning_", (0, 0, 0, 0, 0, 0))[4] for s in systems]
@@ -136,13 +136,16 @@ def autotune_concurrent(systems, run, rounds=2):
         # the matrix-core rho(k) leaves the vector ALU to the other systems'
         # kernels: it can win in a batch where it loses alone
         fields += [("kspace_split", KSPACE_SPLITS), ("rho_width", RHO_WIDTHS), ("rho_kind", RHO_KINDS)]
-    for field, values in fields:
-        times = {v: math.inf for v in values}
-        for _ in range(rounds):
-            for v in values:
-                apply(dict(best, **{field: v}))
-                times[v] = min(times[v], run())
-        best[field] = min(times, key=times.get)
+    # coordinate descent, `passes` times over the knobs: the first pass
+    # tunes each knob with the later ones still at their built-in choice
+    for _ in range(passes):
+        for field, values in fields:
+            times = {v: math.inf for v in values}
+            for _ in range(rounds):
+                for v in values:
+                    apply(dict(best, **{field: v}))
+                    times[v] = min(times[v], run())
+            best[field] = min(times, key=times.get)
     apply(best)
     return dict(best)
 

@@ -85,3 +85,35 @@ def test_mixed_nve_drift_within_twice_fp64():
     dmx, semx = drift(LG_PRECISION_MIXED)
     print(f"drift per atom-step: fp64 {d64:.3e} (+-{se64:.1e}), mixed {dmx:.3e} (+-{semx:.1e})")
     assert abs(dmx) <= 2 * abs(d64) + 3 * (se64 + semx)
+
+
+@pytest.mark.parametrize("offset", [0.0, 700.0])
+def test_mixed_infinite_cell_far_from_origin(offset):
+    # An infinite cell: the mixed pair kernel reads FP32 coordinates relative
+    # to the first atom's position (formed in FP64), so a charged LJ cluster
+    # far from the coordinate origin keeps the error bound of one near it
+    # (FP32 coordinates of magnitude 700 A alone would carry ~3e-5 A of
+    # rounding per coordinate).
+    from lumol_amd import LennardJones, PairInteraction, System, UnitCell, Wolf, units
+    rng = np.random.default_rng(17)
+    g = np.arange(6) * 3.6
+    pos = np.array([(x, y, z) for x in g for y in g for z in g]) + rng.uniform(-0.3, 0.3, (216, 3))
+    pos += offset * np.array([1.0, -0.6, 0.8])
+    names = ["Na" if k % 2 == 0 else "Cl" for k in range(216)]
+    charges = [0.7 if k % 2 == 0 else -0.7 for k in range(216)]
+    s = System.from_arrays(names, pos, UnitCell.infinite(), charges=charges)
+    lj = PairInteraction(LennardJones(3.3, units.from_(0.5, "kJ/mol")), 9.0)
+    for pair in (("Na", "Na"), ("Na", "Cl"), ("Cl", "Cl")):
+        s.set_pair_potential(pair, lj)
+    s.set_coulomb_potential(Wolf(9.0))
+    # forces and energy (the reference has no virial for an infinite cell,
+    # compute.rs:198)
+    o = lo.OracleSystem(s, nthreads=16)
+    s.set_precision(LG_PRECISION_MIXED)
+    fg, fo = s.forces(), o.forces()
+    d = fg - fo
+    rms = math.sqrt((d * d).sum() / (fo * fo).sum())
+    e = abs(s.potential_energy() - o.potential_energy()) / abs(o.potential_energy())
+    print(f"infinite cell, offset {offset}: force rms rel {rms:.2e}, energy rel {e:.2e}")
+    assert rms < FORCE_RMS
+    assert e < ENERGY_REL

@@ -67,7 +67,7 @@ def test_autotune_concurrent_applies_one_choice():
         calls.append(1)
         return time.perf_counter() - t0
 
-    best = tuning.autotune_concurrent(systems, run, rounds=1)
+    best = tuning.autotune_concurrent(systems, run, rounds=1, passes=1)
     assert len(calls) == (len(tuning.PAIR_LANES) + len(tuning.KSPACE_SPLITS) + len(tuning.RHO_WIDTHS)
                           + len(tuning.RHO_KINDS) + len(tuning.NLIST_BUILDS))
     assert best["pair_lanes"] in tuning.PAIR_LANES and best["kspace_split"] in tuning.KSPACE_SPLITS
