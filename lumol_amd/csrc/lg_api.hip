@@ -1326,6 +1326,31 @@ int launch_nlist(lg_system* s, bool check) {
     return LG_OK;
 }
 
+// The arguments of the intra-molecular terms (k_intra_terms, or blocks of
+// another launch).
+static IntraArgs intra_args(lg_system* s) {
+    IntraArgs ia{};
+    ia.pos = s->pos.p;
+    ia.meta = s->meta.p;
+    ia.nterms = s->nterms;
+    ia.terms = s->terms.p;
+    ia.table = s->table.p;
+    ia.tabs = s->ptabs.p;
+    ia.nkinds = int(s->nkinds);
+    ia.pair_restr = s->pair_restr.p;
+    ia.pair_s14 = s->pair_s14.p;
+    ia.bparam = s->bparam.p;
+    ia.aparam = s->aparam.p;
+    ia.dparam = s->dparam.p;
+    ia.coul = s->coul;
+    ia.coul_restr = s->coul_restr;
+    ia.coul_s14 = s->coul_s14;
+    ia.box = s->box;
+    ia.contrib = s->contrib.p;
+    ia.epart = s->ep_mol_at ? s->ep_mol_at : s->epart_mol.p;
+    return ia;
+}
+
 // phase_done: the MD first half-step has already written the phase table
 // and the FP32 positions (frac) of the current positions.
 int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_done = false) {
@@ -1336,6 +1361,13 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     // pair launch, and the k-space forces after it
     int nchunks = 0;
     const bool use_mfma = s->coul_kind == COUL_EWALD && s->nk > 0 && rho_use_mfma(s, mixed);
+    // mixed form, forces only: the intra-molecular terms ride as the last
+    // blocks of the direct-build list-and-rho launch (its 64 KB of LDS hold
+    // it to 4 waves per SIMD, which the terms' 104 VGPRs do not lower)
+    const bool intra_fused = mixed && !energy && !s->gen_pairs && s->nterms > 0 && s->coul_kind == COUL_EWALD &&
+                             s->nk > 0 && !use_mfma && s->direct_pending;
+    const IntraArgs fia = intra_fused ? intra_args(s) : IntraArgs{};
+    const int fib = intra_fused ? blocks_for(s->nterms, kCellBlock) : 0;
     if (s->coul_kind == COUL_EWALD && s->nk > 0) {
         int kmax = s->coul_kmax;
         int nub = blocks_for(s->nunits, 64);
@@ -1399,9 +1431,11 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
             if (!cells)
                 launch_timed(s, LG_KERNEL_KSPACE_RHO, k_rho_rows<float, float2>, grid, 512, 0, ra);
             else if (direct && near)
-                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<true, float, float2>, ncb + grid, kCellBlock, 0, da, ra, ncb);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<true, float, float2>, ncb + grid + fib, kCellBlock,
+                             0, da, ra, ncb, int(grid), fia);
             else if (direct)
-                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<false, float, float2>, ncb + grid, kCellBlock, 0, da, ra, ncb);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<false, float, float2>, ncb + grid + fib, kCellBlock,
+                             0, da, ra, ncb, int(grid), fia);
             else if (near)
                 launch_timed(s, LG_KERNEL_KSPACE_RHO, k_cells_rho<true, float, float2>, ncb + grid, kCellBlock, 0, ca, ra, ncb);
             else
@@ -1415,9 +1449,11 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
             if (!cells)
                 launch_timed(s, LG_KERNEL_KSPACE_RHO, k_rho_rows<double, double2>, grid, 512, 0, ra);
             else if (direct && near)
-                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<true, double, double2>, ncb + grid, kCellBlock, 0, da, ra, ncb);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<true, double, double2>, ncb + grid, kCellBlock, 0, da, ra,
+                             ncb, int(grid), IntraArgs{});
             else if (direct)
-                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<false, double, double2>, ncb + grid, kCellBlock, 0, da, ra, ncb);
+                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<false, double, double2>, ncb + grid, kCellBlock, 0, da, ra,
+                             ncb, int(grid), IntraArgs{});
             else if (near)
                 launch_timed(s, LG_KERNEL_KSPACE_RHO, k_cells_rho<true, double, double2>, ncb + grid, kCellBlock, 0, ca, ra, ncb);
             else
@@ -1468,27 +1504,11 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         launch_timed(s, LG_KERNEL_PAIRS, k_frac, blocks_for(n, 256), 256, 0, s->pos.p, n, s->box, s->frac_origin,
                      s->frac.p);
     }
-    // the intra-molecular terms run as the last blocks of the same launch
+    // the intra-molecular terms run as the last blocks of the pair launch
+    // (FP64 form), of the list-and-rho launch (mixed form, forces only,
+    // direct build: intra_fused) or in a launch of their own
     IntraArgs& ia = pa.intra;
-    ia = IntraArgs{};
-    ia.pos = s->pos.p;
-    ia.meta = s->meta.p;
-    ia.nterms = s->nterms;
-    ia.terms = s->terms.p;
-    ia.table = s->table.p;
-    ia.tabs = s->ptabs.p;
-    ia.nkinds = int(s->nkinds);
-    ia.pair_restr = s->pair_restr.p;
-    ia.pair_s14 = s->pair_s14.p;
-    ia.bparam = s->bparam.p;
-    ia.aparam = s->aparam.p;
-    ia.dparam = s->dparam.p;
-    ia.coul = s->coul;
-    ia.coul_restr = s->coul_restr;
-    ia.coul_s14 = s->coul_s14;
-    ia.box = s->box;
-    ia.contrib = s->contrib.p;
-    ia.epart = s->ep_mol_at ? s->ep_mol_at : s->epart_mol.p;
+    ia = intra_args(s);
     const int iblocks = s->nterms > 0 ? blocks_for(s->nterms, 256) : 0;
     // rho(k) reduction blocks
     const bool kspace = s->coul_kind == COUL_EWALD && s->nk > 0;
@@ -1512,7 +1532,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
     // the mixed instance leaves the intra-molecular terms to a launch of
     // their own (see k_pairs_mixed)
     const bool intra_apart = mixed && !s->gen_pairs;
-    if (intra_apart && iblocks > 0) {
+    if (intra_apart && iblocks > 0 && !intra_fused) {
         if (energy)
             launch_timed(s, LG_KERNEL_MOLECULES, k_intra_terms<true>, iblocks, 256, 0, ia);
         else

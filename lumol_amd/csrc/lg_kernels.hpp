@@ -2222,15 +2222,21 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_rho(CellsArgs ca, RhoRowsA
                                                reinterpret_cast<double2(*)[kRhoUnitK][64]>(buf));
 }
 
-// The same with the direct Verlet build in the first ncb blocks.
+// The same with the direct Verlet build in the first ncb blocks; the rho
+// partials in the next nrho blocks, and any blocks past those run the
+// intra-molecular terms (forces only, kCellBlock terms per block), which
+// need only the positions of the first half-step.
 template <bool kMinImage, typename T, typename T2>
-__global__ __launch_bounds__(kCellBlock) void k_direct_rho(DirectArgs da, RhoRowsArgs<T, T2> ra, int ncb) {
+__global__ __launch_bounds__(kCellBlock) void k_direct_rho(DirectArgs da, RhoRowsArgs<T, T2> ra, int ncb, int nrho,
+                                                           IntraArgs ia) {
     __shared__ __align__(16) unsigned char buf[sizeof(DirectLds) > kRhoRed ? sizeof(DirectLds) : kRhoRed];
     if ((int)blockIdx.x < ncb)
         nlist_direct_block<kMinImage>(da, blockIdx.x, ncb, *reinterpret_cast<DirectLds*>(buf));
-    else
+    else if ((int)blockIdx.x < ncb + nrho)
         rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb,
                                                reinterpret_cast<double2(*)[kRhoUnitK][64]>(buf));
+    else
+        intra_block<false>(ia, blockIdx.x - ncb - nrho);
 }
 
 
