@@ -307,12 +307,14 @@ def run_form(workload, form, steps, warmup, oracle_forces, oracle_energy, oracle
     n_pairs = len(s.pairs_within(rc))
     nk = s.ewald_kvector_count()
     err = force_error(s, oracle_forces, oracle_energy, oracle_virial)
-    # launch knobs for this size bucket (lumol_amd.tuning), chosen before timing
+    # launch knobs for this size bucket (lumol_amd.tuning), chosen before
+    # timing, each candidate timed by the wall time of 100 MD steps (the
+    # launch mix of the timed region, list rebuilds included), best of 3
     from lumol_amd import tuning
-    knobs = tuning.autotune(s, reps=10, rounds=5)
+    sim = make_sim(workload)
+    knobs = tuning.autotune(s, rounds=3, step=lambda: sim.run(s, 100))
     if TUNING_CACHE:
         tuning.save_cache(TUNING_CACHE)
-    sim = make_sim(workload)
     sim.run(s, warmup)
     # timed region: no instrumentation (an event pair per launch costs ~10%
     # of the step rate of these small systems)

@@ -16,6 +16,7 @@ unchanged.
 """
 import math
 import threading
+import time
 
 from . import _native as nat
 
@@ -37,9 +38,16 @@ def bucket(system):
     return (ewald, int(system.precision), p2(n), p2(nk))
 
 
-def _time_all(system, reps, classes=None):
+def _time_all(system, reps, classes=None, step=None):
     """Device time (ms) of one force evaluation, summed over `classes` (the
-    kernel classes a knob changes; all by default)."""
+    kernel classes a knob changes; all by default); or, given `step` (MD
+    steps of the system, which return once the stream is synchronised), the
+    wall time of one call of it: the MD step's own objective, list rebuilds
+    and launch gaps included."""
+    if step is not None:
+        t0 = time.perf_counter()
+        step()
+        return (time.perf_counter() - t0) * 1e3
     system.profile_begin(list(classes or nat.KERNEL_CLASSES))
     for _ in range(reps):
         system.forces()
@@ -57,8 +65,18 @@ _KNOB_CLASSES = {
 }
 
 
-def autotune(system, reps=5, use_cache=True, rounds=3):
-    """Picks and applies the knobs for `system`; returns them (dict)."""
+
+def autotune(system, reps=5, use_cache=True, rounds=3, step=None):
+    """Picks and applies the knobs for `system`; returns them (dict).
+
+    With `step` (a callable that advances the system by MD steps and
+    returns once they are done, e.g. ``lambda: sim.run(system, 100)``) every
+    candidate is timed by the wall time of those steps -- the launches of
+    lg_md_run, with the list rebuilds, the fused first half-step and the
+    force-only launch mix -- instead of by the device time of `reps`
+    standalone force evaluations; the system then advances during tuning.
+    Enough steps to hold several list rebuilds keep their count from
+    deciding between close candidates."""
     key = bucket(system)
     keep = tuple(getattr(system, "tuning_", (0, 0, 0, 0, 0, 0))[3:5])  # nlist_build, md_graph: left as set
 
@@ -67,8 +85,11 @@ def autotune(system, reps=5, use_cache=True, rounds=3):
 
     with _lock:
         hit = _cache.get(key) if use_cache else None
+    # after a change of knobs: the lists and tables are rebuilt before timing
+    warm = system.forces if step is None else step
+    classes = _KNOB_CLASSES
     if hit is None:
-        system.forces()  # builds the lists and tables once
+        warm()  # builds the lists and tables once
         best = {"pair_lanes": 0, "kspace_split": 0, "rho_width": 0, "rho_kind": 0}
 
         def pick(field, values, **extra):
@@ -80,8 +101,8 @@ def autotune(system, reps=5, use_cache=True, rounds=3):
                 for v in values:
                     trial = dict(best, **{field: v}, **extra)
                     apply(trial)
-                    system.forces()
-                    times[v] = min(times[v], _time_all(system, reps, _KNOB_CLASSES[field]))
+                    warm()
+                    times[v] = min(times[v], _time_all(system, reps, classes[field], step))
             return min(times, key=times.get), times
 
         # pair kernel lanes
@@ -92,11 +113,11 @@ def autotune(system, reps=5, use_cache=True, rounds=3):
             width, times = pick("rho_width", RHO_WIDTHS, rho_kind=1)
             trial = dict(best, rho_kind=2)
             apply(trial)
-            system.forces()
+            warm()
             t_mfma = float("inf")
             if system.tuning()["rho_kind"] == 2:
                 for _ in range(rounds):
-                    t_mfma = min(t_mfma, _time_all(system, reps, _KNOB_CLASSES["rho_kind"]))
+                    t_mfma = min(t_mfma, _time_all(system, reps, classes["rho_kind"], step))
             if t_mfma < times[width]:
                 best["rho_kind"] = 2
             else:

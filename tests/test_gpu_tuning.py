@@ -42,6 +42,26 @@ def test_autotune_choice_and_cache():
     assert tuning.bucket(t) in tuning.cached()
 
 
+def test_autotune_on_md_steps():
+    # candidates timed on MD steps (the launch mix of lg_md_run); the choice
+    # changes forces only by summation order
+    from lumol_amd import MolecularDynamics, Simulation, VelocityVerlet
+    from lumol_amd._native import LG_PRECISION_MIXED
+    s = spce(nmol=300, L=20.8, cutoff=8.0, kmax=7, seed=12)
+    s.set_precision(LG_PRECISION_MIXED)
+    sim = Simulation(MolecularDynamics(VelocityVerlet(0.5)))
+    best = tuning.autotune(s, use_cache=False, rounds=1, step=lambda: sim.run(s, 5))
+    assert best["pair_lanes"] in tuning.PAIR_LANES and best["kspace_split"] in tuning.KSPACE_SPLITS
+    assert best["rho_kind"] in (1, 2)
+    t = spce(nmol=300, L=20.8, cutoff=8.0, kmax=7, seed=12)
+    t.set_precision(LG_PRECISION_MIXED)
+    f0 = t.forces()
+    t.set_tuning(best["pair_lanes"], best["kspace_split"], best["rho_width"], rho_kind=best["rho_kind"])
+    # the mixed form's sums differ by FP32 rounding between launch shapes
+    assert frel(t.forces(), f0) < 1e-5
+    assert np.isfinite(s.positions).all()
+
+
 def test_autotune_concurrent_applies_one_choice():
     # two water systems on their own streams: one set of knobs for both,
     # chosen by timing them together; forces change only by summation order
