@@ -520,6 +520,11 @@ void launch_timed(lg_system* s, int cls, void (*kernel)(KArgs...), dim3 grid, di
 
 int blocks_for(size_t n, int b) { return int((n + b - 1) / b); }
 
+// Blocks of the pair part of a pair launch: rounded up to a multiple of 8,
+// so that the kernels' XCD-contiguous block order (xcd_contiguous) is a
+// permutation of them.
+int pair_blocks_of(size_t n, int tpa) { return (blocks_for(n * size_t(tpa), kPairBlock) + 7) / 8 * 8; }
+
 // ---- topology (bonding.rs:78-155, configuration.rs:124-144) ---------------------------
 int build_topology(lg_system* s, std::vector<std::tuple<int, int, int>>& ipairs_out) {
     size_t n = s->n;
@@ -1128,7 +1133,7 @@ int prepare_nlist(lg_system* s) {
     s->tpa = n <= 8192 ? 64 : n <= 32768 ? 32 : n <= 131072 ? 16 : n <= 524288 ? 8 : 4;
     if (s->tune.pair_lanes > 0) s->tpa = s->tune.pair_lanes;
     if (s->gen_pairs) s->tpa = 16;  // the general pair kernel has one instance
-    LG_TRY(s->epart_pairs.ensure(size_t(blocks_for(n * size_t(s->tpa), kPairBlock)) * 8));
+    LG_TRY(s->epart_pairs.ensure(size_t(pair_blocks_of(n, s->tpa)) * 8));
     int init[4] = {1, 0, 0, 0};  // force a rebuild; clear overflow
     LG_HIP(hipMemcpyAsync(s->flags.p, init, sizeof(init), hipMemcpyHostToDevice, S(s)));
     LG_HIP(hipMemsetAsync(s->count.p, 0, g.ncell * sizeof(int), S(s)));
@@ -1538,7 +1543,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
         else
             launch_timed(s, LG_KERNEL_MOLECULES, k_intra_terms<false>, iblocks, 256, 0, ia);
     }
-    const int pblocks = blocks_for(size_t(n) * s->tpa, kPairBlock);
+    const int pblocks = pair_blocks_of(size_t(n), s->tpa);
     pa.pair_blocks = pblocks;
     pa.intra_blocks = intra_apart ? 0 : iblocks;
     const int nblocks = pblocks + pa.intra_blocks + rblocks;
@@ -1653,7 +1658,7 @@ struct Energies {
 
 int gather_energies(lg_system* s, Energies& E) {
     hipStream_t st = S(s);
-    int nbp = blocks_for(s->n * size_t(s->tpa), kPairBlock);
+    int nbp = pair_blocks_of(s->n, s->tpa);
     std::vector<double> hp(size_t(nbp) * 8);
     LG_HIP(hipMemcpyAsync(hp.data(), s->epart_pairs.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     std::vector<double> hm;
@@ -2399,7 +2404,7 @@ int lg_potential_energy_noalloc(lg_system* s, lg_energy_terms* out) {
         if (s->coul_kind != COUL_NONE && s->coul_rc > half)
             return fail(LG_E_INVALID, "The cutoff for the coulomb potential is bigger than half the cell length");
     }
-    const int nbp = blocks_for(s->n * size_t(s->tpa), kPairBlock);
+    const int nbp = pair_blocks_of(s->n, s->tpa);
     const int nbm = s->nterms > 0 ? blocks_for(s->nterms, 256) : 0;
     const int nbk = (s->coul_kind == COUL_EWALD && s->nk > 0) ? blocks_for(s->nk, kRhoReduceK) : 0;
     const size_t need = size_t(nbp) * 8 + size_t(nbm) * 11 + size_t(nbk) * 7 + 18;

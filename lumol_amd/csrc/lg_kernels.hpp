@@ -1359,6 +1359,15 @@ struct PairArgs {
 constexpr int kMaxKinds = 16;
 constexpr int kPairBlock = 256;
 
+// Blocks are dealt round-robin to the eight XCDs (block b runs on XCD b %
+// 8).  The pair kernels take block b as logical block (b % 8) * (P / 8) +
+// b / 8 of P (a multiple of 8), so that each XCD works on one contiguous
+// eighth of the atoms: in a spatially ordered system its L2 then holds the
+// positions of one region and its neighbours (a 1M-atom Lennard-Jones box:
+// about a fifth of the 16 MB of positions per 4 MB L2) instead of the
+// whole box.
+__device__ __forceinline__ int xcd_contiguous(int b, int P) { return (b & 7) * (P >> 3) + (b >> 3); }
+
 // Coulomb variant of a pair kernel instance, fixed per system: none, Ewald
 // real space (erfc), Ewald for inter-molecular pairs that the restriction
 // excludes (the -erf removal term only), Wolf.  A Wolf system whose
@@ -1408,7 +1417,8 @@ __global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(4)))
         for (int x = threadIdx.x; x < kErfcxPieces * (kErfcxDeg64 + 1); x += blockDim.x) ecx[x] = a.ecx64[x];
     __syncthreads();
 
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-contiguous order of the pair blocks (the launch has a multiple of 8)
+    int g = xcd_contiguous(blockIdx.x, a.pair_blocks) * blockDim.x + threadIdx.x;
     int t = g / TPA;
     int sub = g % TPA;
     double fx = 0, fy = 0, fz = 0;
@@ -1612,7 +1622,8 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
                 Lz = (float)(a.box.L[2] * 0x1p-32);
     const float4* __restrict__ frac = a.frac;
 
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-contiguous order of the pair blocks (the launch has a multiple of 8)
+    int g = xcd_contiguous(blockIdx.x, a.pair_blocks) * blockDim.x + threadIdx.x;
     int t = g / TPA;
     int sub = g % TPA;
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -2235,7 +2246,7 @@ __global__ __launch_bounds__(kCellBlock) void k_direct_rho(DirectArgs da, RhoRow
     else if ((int)blockIdx.x < ncb + nrho)
         rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb,
                                                reinterpret_cast<double2(*)[kRhoUnitK][64]>(buf));
-    else
+    else if constexpr (std::is_same<T, float>::value)  // the FP64 form keeps its terms in the pair launch
         intra_block<false>(ia, blockIdx.x - ncb - nrho);
 }
 
