@@ -1435,12 +1435,22 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
                                           s->nunits, s->rho_partial.p, nub, nchunks};
             if (!cells)
                 launch_timed(s, LG_KERNEL_KSPACE_RHO, k_rho_rows<float, float2>, grid, 512, 0, ra);
-            else if (direct && near)
-                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<true, float, float2>, ncb + grid + fib, kCellBlock,
-                             0, da, ra, ncb, int(grid), fia);
-            else if (direct)
-                launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<false, float, float2>, ncb + grid + fib, kCellBlock,
-                             0, da, ra, ncb, int(grid), fia);
+            else if (direct && near) {
+                if (fib > 0)
+                    launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<true, float, float2, true>, ncb + grid + fib,
+                                 kCellBlock, 0, da, ra, ncb, int(grid), fia);
+                else
+                    launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<true, float, float2, false>, ncb + grid,
+                                 kCellBlock, 0, da, ra, ncb, int(grid), fia);
+            }
+            else if (direct) {
+                if (fib > 0)
+                    launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<false, float, float2, true>, ncb + grid + fib,
+                                 kCellBlock, 0, da, ra, ncb, int(grid), fia);
+                else
+                    launch_timed(s, LG_KERNEL_KSPACE_RHO, k_direct_rho<false, float, float2, false>, ncb + grid,
+                                 kCellBlock, 0, da, ra, ncb, int(grid), fia);
+            }
             else if (near)
                 launch_timed(s, LG_KERNEL_KSPACE_RHO, k_cells_rho<true, float, float2>, ncb + grid, kCellBlock, 0, ca, ra, ncb);
             else

@@ -2237,7 +2237,9 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_rho(CellsArgs ca, RhoRowsA
 // partials in the next nrho blocks, and any blocks past those run the
 // intra-molecular terms (forces only, kCellBlock terms per block), which
 // need only the positions of the first half-step.
-template <bool kMinImage, typename T, typename T2>
+// kIntra: the instance with the intra-molecular blocks (mixed form); the
+// others leave their code out (it costs the rho blocks registers).
+template <bool kMinImage, typename T, typename T2, bool kIntra = false>
 __global__ __launch_bounds__(kCellBlock) void k_direct_rho(DirectArgs da, RhoRowsArgs<T, T2> ra, int ncb, int nrho,
                                                            IntraArgs ia) {
     __shared__ __align__(16) unsigned char buf[sizeof(DirectLds) > kRhoRed ? sizeof(DirectLds) : kRhoRed];
@@ -2246,7 +2248,7 @@ __global__ __launch_bounds__(kCellBlock) void k_direct_rho(DirectArgs da, RhoRow
     else if ((int)blockIdx.x < ncb + nrho)
         rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb,
                                                reinterpret_cast<double2(*)[kRhoUnitK][64]>(buf));
-    else if constexpr (std::is_same<T, float>::value)  // the FP64 form keeps its terms in the pair launch
+    else if constexpr (kIntra)
         intra_block<false>(ia, blockIdx.x - ncb - nrho);
 }
 
