@@ -300,8 +300,8 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
  *                 16 for systems with a pair potential other than LJ,
  *                 Harmonic and Null, or a table)
  *   kspace_split  row ranges of the k-space force kernel (1..32)
- *   rho_width     atoms per block of the row form of the rho(k) kernel (64,
- *                 128, 256, 512)
+ *   rho_width     atoms per block of the row form of the rho(k) kernel (a
+ *                 multiple of 64 up to 512)
  *   nlist_build   neighbour-list build: 1 lanes per atom, 2 one workgroup per
  *                 cell with candidates staged in LDS, 3 direct: workgroups
  *                 per cell over the atoms' cell tags, no cell list (up to

@@ -2174,8 +2174,11 @@ int lg_system_set_tuning(lg_system* s, const lg_tuning* t) {
     if (l != 0 && l != 4 && l != 8 && l != 16 && l != 32 && l != 64)
         return fail(LG_E_INVALID, "pair_lanes must be 0, 4, 8, 16, 32 or 64");
     if (t->kspace_split < 0 || t->kspace_split > 32) return fail(LG_E_INVALID, "kspace_split must be in 0..32");
-    if (t->rho_width != 0 && t->rho_width != 64 && t->rho_width != 128 && t->rho_width != 256 && t->rho_width != 512)
-        return fail(LG_E_INVALID, "rho_width must be 0, 64, 128, 256 or 512");
+    // any multiple of 64 up to 512: the row kernel's eight waves then each
+    // take a multiple of 8 atoms, and the partial-sum buffer is sized for
+    // the narrowest width
+    if (t->rho_width < 0 || t->rho_width > 512 || t->rho_width % 64 != 0)
+        return fail(LG_E_INVALID, "rho_width must be 0 or a multiple of 64 up to 512");
     if (t->nlist_build < 0 || t->nlist_build > 3) return fail(LG_E_INVALID, "nlist_build must be 0, 1, 2 or 3");
     if (t->md_graph < 0 || t->md_graph > 2) return fail(LG_E_INVALID, "md_graph must be 0, 1 or 2");
     if (t->rho_kind < 0 || t->rho_kind > 2) return fail(LG_E_INVALID, "rho_kind must be 0, 1 or 2");

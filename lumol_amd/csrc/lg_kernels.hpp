@@ -2565,6 +2565,14 @@ __global__ __launch_bounds__(256) void k_kspace_force(const T2* __restrict__ tab
         pey[u] = col[u][(size_t)phase_row(1, nrow.iky, kmax) * ld];
         pez[u] = col[u][(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
     }
+    // A full vector-memory wait before the row loop.  Without it the
+    // compiler issues the Ez[1] load after the first row's columns and then
+    // places a full wait inside the k loop (for Ez[1] in the first row),
+    // which also drains the next row's prefetch at the start of every row;
+    // with it, the loop waits for the columns once, at the top of the first
+    // row, and each row's prefetch stays in flight while the row is summed.
+    // 0x0F70: vmcnt(0), expcnt and lgkmcnt not waited on.
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     for (int r = r0; r < r1; r++) {
         KRow row = nrow;
         T2 e[APL];

@@ -22,7 +22,7 @@ from . import _native as nat
 
 PAIR_LANES = (8, 16, 32, 64)
 KSPACE_SPLITS = (2, 4, 8, 16, 32)
-RHO_WIDTHS = (64, 128, 256, 512)
+RHO_WIDTHS = (64, 128, 192, 256, 512)
 NLIST_BUILDS = (0, 2)  # automatic, cell list (lg_tuning.nlist_build)
 RHO_KINDS = (1, 2)  # rows, matrix cores (lg_tuning.rho_kind)
 
