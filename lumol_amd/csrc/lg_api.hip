@@ -1050,9 +1050,12 @@ int prepare_kspace(lg_system* s) {
     if (s->mfma_nt > 0)
         for (int m = 0; m < 2; m++) npart = std::max(npart, size_t(rho_mfma_shape(s, m == 1).nchunks) * s->nk);
     LG_TRY(s->rho_partial.ensure(npart));
-    LG_TRY(s->phase_tab.ensure(size_t(phase_rows(kmax)) * s->ld));
+    // the staged row kernel loads kRhoStageRows rows per round (the rows
+    // past the phase rows are loaded and not used)
+    const size_t trows = size_t(std::max(phase_rows(kmax), kRhoStageRows));
+    LG_TRY(s->phase_tab.ensure(trows * s->ld));
     LG_TRY(s->phase_q.ensure(s->ld));
-    LG_TRY(s->phase_tab32.ensure(size_t(phase_rows(kmax)) * s->ld));
+    LG_TRY(s->phase_tab32.ensure(trows * s->ld));
     LG_TRY(s->phase_q32.ensure(s->ld));
     LG_TRY(s->epart_k.ensure(size_t(blocks_for(s->nk, kRhoReduceK)) * 7));
     return LG_OK;

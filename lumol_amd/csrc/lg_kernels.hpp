@@ -2153,12 +2153,31 @@ struct RhoRowsArgs {
     int nub, nchunks;
 };
 
-// NW waves per block (8 in k_rho_rows and k_cells_rho), logical
-// block index L of the xcd_grid(nub, nchunks) layout.
+// Phase rows the staged row kernel loads per round; the phase tables are
+// allocated with at least this many rows (kmax <= 12 staged).
+constexpr int kRhoStageRows = 64;
+// Atoms per staging round of the row kernel: every phase row of S
+// consecutive atoms is copied to LDS at once.
+template <typename T>
+constexpr int rho_stage_atoms() { return sizeof(T) == 4 ? 8 : 4; }
+// LDS bytes of one wave's staging area (rows padded to S + 1 entries so that
+// lanes reading different rows at one atom fall in different banks; the
+// charges follow the rows), rounded to 16 bytes.
+template <typename T, typename T2>
+__host__ __device__ constexpr size_t rho_stage_wave_bytes(int kmax) {
+    return ((size_t)phase_rows(kmax) * (rho_stage_atoms<T>() + 1) * sizeof(T2) + rho_stage_atoms<T>() * sizeof(T) +
+            15) / 16 * 16;
+}
+
+// NW waves per block (8 in k_rho_rows, k_cells_rho and k_direct_rho),
+// logical block index L of the xcd_grid(nub, nchunks) layout.  buf: the
+// launch's LDS (buf_bytes), which holds the eight wave sums at the end and,
+// when the staging areas of all NW waves fit in it, the staged phase rows.
 template <typename T, typename T2, int NW>
-__device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int L,
-                                               double2 (*red)[kRhoUnitK][64]) {
+__device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int L, unsigned char* buf,
+                                               size_t buf_bytes) {
     static_assert(NW >= kRhoUnitK, "one wave writes each k-vector of a unit");
+    double2(*red)[kRhoUnitK][64] = reinterpret_cast<double2(*)[kRhoUnitK][64]>(buf);
     const T2* __restrict__ tab = ra.tab;
     const T* __restrict__ q = ra.q;
     const int ld = ra.ld, kmax = ra.kmax, width = ra.width, nunits = ra.nunits;
@@ -2169,23 +2188,18 @@ __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int u = ub * 64 + lane;
     const KUnit un = units[u < nunits ? u : nunits - 1];
-    const T2* cx = tab + (size_t)phase_row(0, un.ikx, kmax) * ld;
-    const T2* cy = tab + (size_t)phase_row(1, un.iky, kmax) * ld;
-    const T2* cz = tab + (size_t)phase_row(2, un.kz0, kmax) * ld;
-    const T2* c1 = tab + (size_t)phase_row(2, 1, kmax) * ld;
     const int per = width / NW;
-    // the wave's atom range made visibly uniform: the charge and Ez[1] of
-    // an atom are then scalar loads, three vector loads per atom remain
+    // the wave's atom range, visibly uniform.  a1 - a0 is a multiple of 8
+    // or <= 0: widths are multiples of 64 (as is ld), so per and a0 are
+    // multiples of 8
     const int a0 = __builtin_amdgcn_readfirstlane(chunk * width + w * per);
     const int a1 = min(a0 + per, ld);
     T re[kRhoUnitK], im[kRhoUnitK];
 #pragma unroll
     for (int v = 0; v < kRhoUnitK; v++) re[v] = im[v] = 0;
-#pragma unroll 4
-    for (int a = a0; a < a1; a++) {
-        const T qa = q[a];
-        const T2 e1 = c1[a];
-        T2 e = cmul(cmul(cx[a], cy[a]), cz[a]);
+    // one atom's terms, in the order of the reference's triple product
+    auto add_atom = [&](T2 x, T2 y, T2 z, T2 e1, T qa) {
+        T2 e = cmul(cmul(x, y), z);
         e.x *= qa;
         e.y *= qa;
 #pragma unroll
@@ -2194,7 +2208,56 @@ __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int
             re[v] += e.x;
             im[v] += e.y;
         }
+    };
+    constexpr int S = rho_stage_atoms<T>(), SP = S + 1;
+    constexpr int kStep = 64 / S;  // rows per load instruction of a wave
+    const size_t wave_bytes = rho_stage_wave_bytes<T, T2>(kmax);
+    const int prow = phase_rows(kmax);
+    if ((size_t)NW * wave_bytes <= buf_bytes && prow <= kRhoStageRows) {
+        // Staged: each round copies every phase row of the next S atoms
+        // (and their charges) to the wave's own LDS area, one load per 64
+        // entries (S consecutive atoms of 64 / S rows), all in flight
+        // together; then each lane reads its unit's rows from LDS: one
+        // memory latency per S atoms.  Atoms are added in index order, as in
+        // the unstaged loop.  The loads cover kRhoStageRows rows, which the
+        // table is allocated to hold; rows past the phase rows are not
+        // stored.  Only this wave reads its area, and a wave's LDS
+        // operations complete in order, so a round's reads precede the next
+        // round's writes.
+        T2* tile = reinterpret_cast<T2*>(buf + (size_t)w * wave_bytes);
+        T* qt = reinterpret_cast<T*>(tile + (size_t)prow * SP);
+        const int rx = phase_row(0, un.ikx, kmax) * SP, ry = phase_row(1, un.iky, kmax) * SP;
+        const int rz = phase_row(2, un.kz0, kmax) * SP, r1 = phase_row(2, 1, kmax) * SP;
+        const int lr = lane / S, lj = lane - lr * S, lq = min(lane, S - 1);
+        const T2* src = tab + (size_t)lr * ld + lj;
+        for (int a = a0; a < a1; a += S) {
+            T2 v[kRhoStageRows / kStep];
+#pragma unroll
+            for (int it = 0; it < kRhoStageRows / kStep; it++) v[it] = src[(size_t)(it * kStep) * ld + a];
+            const T qv = q[a + lq];
+#pragma unroll
+            for (int it = 0; it < kRhoStageRows / kStep; it++) {
+                const int r = lr + it * kStep;
+                if (r < prow) tile[r * SP + lj] = v[it];
+            }
+            qt[lq] = qv;
+            // the wave's LDS writes done before any lane reads them
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int j = 0; j < S; j++) add_atom(tile[rx + j], tile[ry + j], tile[rz + j], tile[r1 + j], qt[j]);
+            __builtin_amdgcn_wave_barrier();
+        }
+    } else {
+        const T2* cx = tab + (size_t)phase_row(0, un.ikx, kmax) * ld;
+        const T2* cy = tab + (size_t)phase_row(1, un.iky, kmax) * ld;
+        const T2* cz = tab + (size_t)phase_row(2, un.kz0, kmax) * ld;
+        const T2* c1 = tab + (size_t)phase_row(2, 1, kmax) * ld;
+#pragma unroll 4
+        for (int a = a0; a < a1; a++) add_atom(cx[a], cy[a], cz[a], c1[a], q[a]);
     }
+    // every wave is past its staging area before the sums overwrite it
+    __syncthreads();
 #pragma unroll
     for (int v = 0; v < kRhoUnitK; v++) red[w][v][lane] = make_double2(re[v], im[v]);
     __syncthreads();
@@ -2209,12 +2272,15 @@ __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int
     }
 }
 
+// LDS of the eight wave sums of a row-kernel block
+constexpr size_t kRhoRed = sizeof(double2) * (kCellBlock / 64) * kRhoUnitK * 64;
+
 // Same 8-wave blocks as the rho part of k_cells_rho, so that the partial
 // sums do not depend on which launch computed them.
 template <typename T, typename T2>
 __global__ __launch_bounds__(512) void k_rho_rows(RhoRowsArgs<T, T2> ra) {
-    __shared__ double2 red[8][kRhoUnitK][64];
-    rho_rows_block<T, T2, 8>(ra, blockIdx.x, red);
+    __shared__ __align__(16) unsigned char buf[kRhoRed];
+    rho_rows_block<T, T2, 8>(ra, blockIdx.x, buf, sizeof(buf));
 }
 
 // One launch for two independent passes of a force evaluation: blocks
@@ -2222,15 +2288,13 @@ __global__ __launch_bounds__(512) void k_rho_rows(RhoRowsArgs<T, T2> ra) {
 // rho blocks that follow keep the XCD layout of xcd_grid), the rest compute
 // the rho(k) partials with 8 waves per block.  Both read only what the
 // first half-step and the cell list wrote.
-constexpr size_t kRhoRed = sizeof(double2) * (kCellBlock / 64) * kRhoUnitK * 64;
 template <bool kMinImage, typename T, typename T2>
 __global__ __launch_bounds__(kCellBlock) void k_cells_rho(CellsArgs ca, RhoRowsArgs<T, T2> ra, int ncb) {
     __shared__ __align__(16) unsigned char buf[sizeof(CellsLds) > kRhoRed ? sizeof(CellsLds) : kRhoRed];
     if ((int)blockIdx.x < ncb)
         nlist_cells_block<kMinImage>(ca, blockIdx.x, ncb, *reinterpret_cast<CellsLds*>(buf));
     else
-        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb,
-                                               reinterpret_cast<double2(*)[kRhoUnitK][64]>(buf));
+        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb, buf, sizeof(buf));
 }
 
 // The same with the direct Verlet build in the first ncb blocks; the rho
@@ -2246,8 +2310,7 @@ __global__ __launch_bounds__(kCellBlock) void k_direct_rho(DirectArgs da, RhoRow
     if ((int)blockIdx.x < ncb)
         nlist_direct_block<kMinImage>(da, blockIdx.x, ncb, *reinterpret_cast<DirectLds*>(buf));
     else if ((int)blockIdx.x < ncb + nrho)
-        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb,
-                                               reinterpret_cast<double2(*)[kRhoUnitK][64]>(buf));
+        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb, buf, sizeof(buf));
     else if constexpr (kIntra)
         intra_block<false>(ia, blockIdx.x - ncb - nrho);
 }
