@@ -2173,7 +2173,10 @@ __host__ __device__ constexpr size_t rho_stage_wave_bytes(int kmax) {
 // logical block index L of the xcd_grid(nub, nchunks) layout.  buf: the
 // launch's LDS (buf_bytes), which holds the eight wave sums at the end and,
 // when the staging areas of all NW waves fit in it, the staged phase rows.
-template <typename T, typename T2, int NW>
+// kPrefetch: the staged loop loads the next round while it sums the current
+// one (more registers held: used where the launch's LDS, not its registers,
+// sets the residency).
+template <typename T, typename T2, int NW, bool kPrefetch = false>
 __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int L, unsigned char* buf,
                                                size_t buf_bytes) {
     static_assert(NW >= kRhoUnitK, "one wave writes each k-vector of a unit");
@@ -2230,13 +2233,17 @@ __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int
         const int rz = phase_row(2, un.kz0, kmax) * SP, r1 = phase_row(2, 1, kmax) * SP;
         const int lr = lane / S, lj = lane - lr * S, lq = min(lane, S - 1);
         const T2* src = tab + (size_t)lr * ld + lj;
-        for (int a = a0; a < a1; a += S) {
-            T2 v[kRhoStageRows / kStep];
+        constexpr int kR = kRhoStageRows / kStep;
+        T2 v[kR];
+        T qv;
+        auto load = [&](int a) {
 #pragma unroll
-            for (int it = 0; it < kRhoStageRows / kStep; it++) v[it] = src[(size_t)(it * kStep) * ld + a];
-            const T qv = q[a + lq];
+            for (int it = 0; it < kR; it++) v[it] = src[(size_t)(it * kStep) * ld + a];
+            qv = q[a + lq];
+        };
+        auto stage = [&]() {
 #pragma unroll
-            for (int it = 0; it < kRhoStageRows / kStep; it++) {
+            for (int it = 0; it < kR; it++) {
                 const int r = lr + it * kStep;
                 if (r < prow) tile[r * SP + lj] = v[it];
             }
@@ -2244,9 +2251,36 @@ __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int
             // the wave's LDS writes done before any lane reads them
             __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
             __builtin_amdgcn_wave_barrier();
+        };
+        auto sum = [&]() {
 #pragma unroll
             for (int j = 0; j < S; j++) add_atom(tile[rx + j], tile[ry + j], tile[rz + j], tile[r1 + j], qt[j]);
             __builtin_amdgcn_wave_barrier();
+        };
+        if constexpr (kPrefetch) {
+            // the next round's loads in flight while this round is summed
+            // (one memory latency per wave).  The first round's values go
+            // through an empty asm statement once they have landed, so that
+            // the values entering the loop are not all loads and the
+            // compiler keeps the early loads where they are; the last round
+            // reloads its own atoms (in bounds, not used).
+            if (a0 < a1) {
+                load(a0);
+#pragma unroll
+                for (int it = 0; it < kR; it++) asm volatile("" : "+v"(v[it].x), "+v"(v[it].y));
+                asm volatile("" : "+v"(qv));
+            }
+            for (int a = a0; a < a1; a += S) {
+                stage();
+                load(a + S < a1 ? a + S : a);
+                sum();
+            }
+        } else {
+            for (int a = a0; a < a1; a += S) {
+                load(a);
+                stage();
+                sum();
+            }
         }
     } else {
         const T2* cx = tab + (size_t)phase_row(0, un.ikx, kmax) * ld;
@@ -2310,7 +2344,7 @@ __global__ __launch_bounds__(kCellBlock) void k_direct_rho(DirectArgs da, RhoRow
     if ((int)blockIdx.x < ncb)
         nlist_direct_block<kMinImage>(da, blockIdx.x, ncb, *reinterpret_cast<DirectLds*>(buf));
     else if ((int)blockIdx.x < ncb + nrho)
-        rho_rows_block<T, T2, kCellBlock / 64>(ra, blockIdx.x - ncb, buf, sizeof(buf));
+        rho_rows_block<T, T2, kCellBlock / 64, true>(ra, blockIdx.x - ncb, buf, sizeof(buf));
     else if constexpr (kIntra)
         intra_block<false>(ia, blockIdx.x - ncb - nrho);
 }
