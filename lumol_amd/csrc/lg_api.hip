@@ -1241,12 +1241,15 @@ static int force_rebuild(lg_system* s) {
 
 int launch_nlist(lg_system* s, bool check) {
     hipStream_t st = S(s);
-    long prof = prof_start(s, LG_KERNEL_NLIST);
+    // profiled intervals only around launches (an empty event pair would
+    // count the event packets' own dispatch as list time)
+    long prof = -1;
     int n = int(s->n);
     int* flag = s->flags.p;
     const Grid& g = s->grid;
     const bool direct = direct_build(s);
     if (check) {
+        prof = prof_start(s, LG_KERNEL_NLIST);
         double hs = 0.5 * s->skin;
         k_nlist_check<<<blocks_for(n, 256), 256, 0, st>>>(s->pos.p, s->ref.p, n, hs * hs, flag, cell_tags(s));
     }
@@ -1275,6 +1278,7 @@ int launch_nlist(lg_system* s, bool check) {
         LG_HIP(hipGetLastError());
         return LG_OK;
     }
+    if (prof < 0) prof = prof_start(s, LG_KERNEL_NLIST);
     // small systems: the whole cell list in one workgroup
     size_t small_lds = size_t(8) * g.ncell + 4 + size_t(8) * n;
     if (g.ncell < 65536 && n <= 32768 && small_lds <= s->ctx->max_shared) {
@@ -2661,38 +2665,36 @@ static int md_chunk(lg_system* s, uint64_t todo) {
     // halves run alone
     bool first_pending = true;
     for (uint64_t k = 0; k < todo; k++) {
+        // integrator launches timed, when profiled, by their own kernel
+        // timestamps (launch_timed), like the force launches
         if (first_pending) {
             // the Berendsen scale of the previous step is applied by the first half
             th.ke_part = (thermo && k > 0) ? s->ke_part.p : nullptr;
-            long p1 = prof_start(s, LG_KERNEL_INTEGRATE);
             if (!lf)
-                k_vv_first<<<nbf, tbf, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
-                                               s->flags.p, ph, tags);
+                launch_timed(s, LG_KERNEL_INTEGRATE, k_vv_first, nbf, tbf, 0, s->pos.p, s->vel.p, s->acc.p, n, s->dt,
+                             th, s->ref.p, hs * hs, s->flags.p, ph, tags);
             else
-                k_lf_first<<<nbf, tbf, 0, st>>>(s->pos.p, s->vel.p, s->acc.p, n, s->dt, th, s->ref.p, hs * hs,
-                                               s->flags.p, ph, tags);
-            prof_stop(s, p1);
+                launch_timed(s, LG_KERNEL_INTEGRATE, k_lf_first, nbf, tbf, 0, s->pos.p, s->vel.p, s->acc.p, n, s->dt,
+                             th, s->ref.p, hs * hs, s->flags.p, ph, tags);
         }
         LG_TRY(launch_nlist(s, false));
         LG_TRY(launch_forces(s, false, false, true));
-        long p2 = prof_start(s, LG_KERNEL_INTEGRATE);
         const bool fuse = !thermo && k + 1 < todo;
         if (fuse && !lf)
-            k_md_fused<false><<<nbf, tbf, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks, s->vel.p,
-                                                   s->acc.p, n, s->dt, s->ref.p, hs * hs, s->flags.p, ph, tags);
+            launch_timed(s, LG_KERNEL_INTEGRATE, k_md_fused<false>, nbf, tbf, 0, s->force.p, s->pos.p, s->contrib.p,
+                         sl, s->fk.p, ks, s->vel.p, s->acc.p, n, s->dt, s->ref.p, hs * hs, s->flags.p, ph, tags);
         else if (fuse)
-            k_md_fused<true><<<nbf, tbf, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks, s->vel.p,
-                                                  s->acc.p, n, s->dt, s->ref.p, hs * hs, s->flags.p, ph, tags);
+            launch_timed(s, LG_KERNEL_INTEGRATE, k_md_fused<true>, nbf, tbf, 0, s->force.p, s->pos.p, s->contrib.p,
+                         sl, s->fk.p, ks, s->vel.p, s->acc.p, n, s->dt, s->ref.p, hs * hs, s->flags.p, ph, tags);
         else {
             double* kp = thermo ? s->ke_part.p : nullptr;
             if (!lf)
-                k_md_second<false><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
-                                                       s->vel.p, s->acc.p, n, s->dt, kp, s->flags.p);
+                launch_timed(s, LG_KERNEL_INTEGRATE, k_md_second<false>, nb, 64, 0, s->force.p, s->pos.p,
+                             s->contrib.p, sl, s->fk.p, ks, s->vel.p, s->acc.p, n, s->dt, kp, s->flags.p);
             else
-                k_md_second<true><<<nb, 64, 0, st>>>(s->force.p, s->pos.p, s->contrib.p, sl, s->fk.p, ks,
-                                                      s->vel.p, s->acc.p, n, s->dt, kp, s->flags.p);
+                launch_timed(s, LG_KERNEL_INTEGRATE, k_md_second<true>, nb, 64, 0, s->force.p, s->pos.p,
+                             s->contrib.p, sl, s->fk.p, ks, s->vel.p, s->acc.p, n, s->dt, kp, s->flags.p);
         }
-        prof_stop(s, p2);
         first_pending = !fuse;
     }
     if (thermo) {
