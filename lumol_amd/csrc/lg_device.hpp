@@ -329,7 +329,9 @@ __host__ __device__ __forceinline__ double erfcx_tab(const double* __restrict__ 
 }
 __host__ __device__ __forceinline__ float erfcx_tab(const float* __restrict__ c, float x, float inv_h) {
     const float u = x * inv_h;
-    const int k = (int)fminf(fmaxf(u, 0.f), (float)(kErfcxPieces - 1));
+    // x >= 0 (a distance or alpha times one), so only the top needs a
+    // clamp; a NaN u gives the last piece (fminf returns the other operand)
+    const int k = (int)fminf(u, (float)(kErfcxPieces - 1));
     const float t = 2.0f * (u - (float)k) - 1.0f;
     const float* q = c + k * (kErfcxDeg32 + 1);
     float p = q[kErfcxDeg32];

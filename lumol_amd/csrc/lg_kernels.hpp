@@ -1635,6 +1635,8 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
         const float4 pi = frac[i];
         const int ki = a.meta[i].x;
         const float qi = pi.w;
+        // the home charge with 1 / (4 pi eps0): one product per pair fewer
+        const float qi_k = qi * inv4pe;
         const int nn = a.nnbr[t];
         const int* row = a.nbr + (size_t)t * a.cap;
         const float4* __restrict__ ljrow = ljt + ki * a.nkinds;
@@ -1680,12 +1682,11 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
             }
             float fs = in_pair ? fp : 0.f;
             if (kEnergy) e_p = in_pair ? e_p - P.w : 0.f;
-            const float qiqj = qi * pj.w;
+            const float pref = qi_k * pj.w;  // q_i q_j / (4 pi eps0)
             if (kTab && use_h) {
                 // forces from the h table: no exponential, one lookup
                 const float r = r2 * inv_r;
                 const float h = erfcx_tab(hft, r, hf_inv_h);
-                const float pref = qiqj * inv4pe;
                 const float inv_r3 = inv_r * inv_r2;
                 const float fc = CK == CK_EWALD ? pref * (h * inv_r3) : pref * (h * inv_r3 - wf * inv_r);
                 const bool in_coul = r2 <= rcc2;
@@ -1705,7 +1706,6 @@ __global__ __launch_bounds__(kPairBlock) void k_pairs_mixed(PairArgs a) {
                 } else {
                     erfcv = erfc_cheb(alpha * r, ex2);
                 }
-                const float pref = qiqj * inv4pe;
                 float fc;
                 if (CK == CK_EWALD_ERF) {
                     const float erfv = 1.f - erfcv;
