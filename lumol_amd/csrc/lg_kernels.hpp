@@ -2670,6 +2670,16 @@ __global__ __launch_bounds__(256) void k_kspace_force(const T2* __restrict__ tab
     // row, and each row's prefetch stays in flight while the row is summed.
     // 0x0F70: vmcnt(0), expcnt and lgkmcnt not waited on.
     __builtin_amdgcn_s_waitcnt(0x0F70);
+    // The first row's columns go through an empty asm statement, so that
+    // the values entering the row loop are not all loads: the compiler then
+    // keeps the prefetch in place and in the same registers from row to row
+    // (with a conditional prefetch it copied six registers in and out of
+    // every row).  The prefetch is unconditional: the last row reloads its
+    // own columns (in bounds, not used).
+#pragma unroll
+    for (int u = 0; u < APL; u++)
+        asm volatile("" : "+v"(pex[u].x), "+v"(pex[u].y), "+v"(pey[u].x), "+v"(pey[u].y), "+v"(pez[u].x),
+                     "+v"(pez[u].y));
     for (int r = r0; r < r1; r++) {
         KRow row = nrow;
         T2 e[APL];
@@ -2680,14 +2690,12 @@ __global__ __launch_bounds__(256) void k_kspace_force(const T2* __restrict__ tab
             A[u] = 0;
             B[u] = 0;
         }
-        if (r + 1 < r1) {
-            nrow = rows[r + 1];
+        nrow = rows[r + 1 < r1 ? r + 1 : r];
 #pragma unroll
-            for (int u = 0; u < APL; u++) {
-                pex[u] = col[u][(size_t)phase_row(0, nrow.ikx, kmax) * ld];
-                pey[u] = col[u][(size_t)phase_row(1, nrow.iky, kmax) * ld];
-                pez[u] = col[u][(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
-            }
+        for (int u = 0; u < APL; u++) {
+            pex[u] = col[u][(size_t)phase_row(0, nrow.ikx, kmax) * ld];
+            pey[u] = col[u][(size_t)phase_row(1, nrow.iky, kmax) * ld];
+            pez[u] = col[u][(size_t)phase_row(2, nrow.ikz0, kmax) * ld];
         }
         const R* rp = rec + row.kbase;
         const int cnt = row.count;
