@@ -3,7 +3,9 @@
 forces at the FP64 parity bar, the mixed form within its force-error bar,
 across kmax (column tiles from one to many, rows ikx up to 15), atom counts
 below one block's stage and off the stage width, and the fall-back to the row
-form above kmax 16."""
+form above kmax 16.  The row form stages the phase rows in LDS up to kmax 12
+(63 rows) and reads them from the table above, so kmax 3..12 and 16..17
+cover both of its loops."""
 import numpy as np
 import pytest
 
@@ -18,6 +20,8 @@ CASES = {
     "nacl-64-kmax3": (lambda: nacl(ncell=2, cutoff=5.0, kmax=3), 2),
     "nacl-512-kmax7": (lambda: nacl(ncell=4, cutoff=10.0, kmax=7), 2),
     "nacl-4096-kmax9": (lambda: nacl(ncell=8, cutoff=12.0, kmax=9), 2),
+    # 63 phase rows: the largest table the row form stages in LDS
+    "nacl-512-kmax12": (lambda: nacl(ncell=4, cutoff=10.0, kmax=12), 2),
     "water-1500-kmax10": (lambda: spce(nmol=500, L=24.7, cutoff=8.0, kmax=10), 2),
     "water-3000-kmax16": (lambda: spce(nmol=1000, L=31.07, cutoff=7.0, kmax=16), 2),
     "water-ragged-kmax5": (lambda: spce(nmol=333, L=21.6, cutoff=7.0, kmax=5), 2),
