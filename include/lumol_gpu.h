@@ -314,10 +314,13 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
  *                 launches while a kernel profile is active).  On ROCm 7.0
  *                 the replay measured 4-18% slower than eager launches, so
  *                 it is off by default.
- *   rho_kind      rho(k) kernel: 1 row form (one lane per run of four
- *                 k-vectors, a loop over atoms; the default), 2 matrix-core
- *                 form (rho as a complex product over atoms on v_mfma
- *                 16x16x4; kmax <= 16, else the row form).
+ *   rho_kind      rho(k) kernel: 1 row form (one lane per run of five
+ *                 k-vectors, a loop over atoms, the phase rows of a few
+ *                 atoms at a time staged in LDS up to kmax 12; the default),
+ *                 2 matrix-core form (rho as a complex product over atoms on
+ *                 v_mfma 16x16x4; kmax <= 16, else the row form), 3 the row
+ *                 form reading the phase rows from the table (no staging;
+ *                 the same sums, bit for bit).
  * lumol_amd.tuning picks them per size bucket by timing the candidates.
  * Results do not depend on them beyond summation order. */
 typedef struct lg_tuning {

@@ -492,15 +492,32 @@ void prof_stop(lg_system* s, long rec) {
     (void)hipEventRecord(s->prof_events[s->prof_records[size_t(rec)].second + 1], s->ctx->stream);
 }
 
+// An ordinary launch (hipLaunchKernel) of `kernel` with its arguments
+// converted to the kernel's own parameter types.
+template <typename... KArgs, size_t... I>
+void launch_plain(const void* kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_t st,
+                  std::tuple<KArgs...>& a, std::index_sequence<I...>) {
+    void* argv[sizeof...(KArgs) > 0 ? sizeof...(KArgs) : 1] = {static_cast<void*>(&std::get<I>(a))...};
+    (void)hipLaunchKernel(kernel, grid, block, argv, shmem, st);
+}
+
 // A launch timed, when its class is profiled, by the kernel's own start and
 // end timestamps (the events of hipExtLaunchKernelGGL), which agree with a
 // kernel trace; an event pair recorded around the launch would add the
-// dispatch of the event packets to every kernel's time.
+// dispatch of the event packets to every kernel's time.  Unprofiled, an
+// ordinary launch: hipExtLaunchKernelGGL costs more host time per launch,
+// which small systems (a step every ~30 us, five launches) notice.
 template <typename... KArgs, typename... Args>
 void launch_timed(lg_system* s, int cls, void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32_t shmem,
                   Args... args) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (s->prof_mask & (1u << cls)) {
+    if (!(s->prof_mask & (1u << cls))) {
+        std::tuple<KArgs...> a(args...);
+        launch_plain(reinterpret_cast<const void*>(kernel), grid, block, shmem, S(s), a,
+                     std::index_sequence_for<KArgs...>{});
+        return;
+    }
+    {
         bool ok = true;
         while (ok && s->prof_events.size() < s->prof_used + 2) {
             hipEvent_t e;
@@ -1439,7 +1456,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
                 launch_timed(s, LG_KERNEL_KSPACE_RHO, k_phase32, blocks_for(size_t(3) * s->ld, 256), 256, 0, s->pos.p, n, s->ld, s->box, kmax,
                                                                               s->phase_tab32.p, s->phase_q32.p);
             RhoRowsArgs<float, float2> ra{s->phase_tab32.p, s->phase_q32.p, s->ld, kmax, w, s->units.p,
-                                          s->nunits, s->rho_partial.p, nub, nchunks};
+                                          s->nunits, s->rho_partial.p, nub, nchunks, s->tune.rho_kind != 3};
             if (!cells)
                 launch_timed(s, LG_KERNEL_KSPACE_RHO, k_rho_rows<float, float2>, grid, 512, 0, ra);
             else if (direct && near) {
@@ -1467,7 +1484,7 @@ int launch_forces(lg_system* s, bool energy, bool combine = true, bool phase_don
                 launch_timed(s, LG_KERNEL_KSPACE_RHO, k_phase, blocks_for(size_t(3) * s->ld, 256), 256, 0, s->pos.p, n, s->ld, s->box, kmax,
                                                                             s->phase_tab.p, s->phase_q.p);
             RhoRowsArgs<double, double2> ra{s->phase_tab.p, s->phase_q.p, s->ld, kmax, w, s->units.p,
-                                            s->nunits, s->rho_partial.p, nub, nchunks};
+                                            s->nunits, s->rho_partial.p, nub, nchunks, s->tune.rho_kind != 3};
             if (!cells)
                 launch_timed(s, LG_KERNEL_KSPACE_RHO, k_rho_rows<double, double2>, grid, 512, 0, ra);
             else if (direct && near)
@@ -2188,7 +2205,7 @@ int lg_system_set_tuning(lg_system* s, const lg_tuning* t) {
         return fail(LG_E_INVALID, "rho_width must be 0 or a multiple of 64 up to 512");
     if (t->nlist_build < 0 || t->nlist_build > 3) return fail(LG_E_INVALID, "nlist_build must be 0, 1, 2 or 3");
     if (t->md_graph < 0 || t->md_graph > 2) return fail(LG_E_INVALID, "md_graph must be 0, 1 or 2");
-    if (t->rho_kind < 0 || t->rho_kind > 2) return fail(LG_E_INVALID, "rho_kind must be 0, 1 or 2");
+    if (t->rho_kind < 0 || t->rho_kind > 3) return fail(LG_E_INVALID, "rho_kind must be 0, 1, 2 or 3");
     s->tune = *t;
     s->kspace_dirty = s->nlist_dirty = true;
     return LG_OK;
@@ -2205,7 +2222,7 @@ int lg_system_get_tuning(lg_system* s, lg_tuning* t) {
     t->md_graph = s->tune.md_graph == 2 ? 2 : 1;
     t->rho_width = (s->coul_kind == COUL_EWALD && s->nk > 0) ? rho_chunk(s) : 0;
     t->rho_kind = (s->coul_kind == COUL_EWALD && s->nk > 0)
-                      ? (rho_use_mfma(s, s->precision == LG_PRECISION_MIXED) ? 2 : 1)
+                      ? (rho_use_mfma(s, s->precision == LG_PRECISION_MIXED) ? 2 : (s->tune.rho_kind == 3 ? 3 : 1))
                       : 0;
     return LG_OK;
 }

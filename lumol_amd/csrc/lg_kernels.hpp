@@ -2151,6 +2151,7 @@ struct RhoRowsArgs {
     int nunits;
     double2* partial;
     int nub, nchunks;
+    int stage;  // 1: phase rows staged in LDS when they fit (rho_kind 1), 0: read from the table (rho_kind 3)
 };
 
 // Phase rows the staged row kernel loads per round; the phase tables are
@@ -2216,7 +2217,7 @@ __device__ __forceinline__ void rho_rows_block(const RhoRowsArgs<T, T2>& ra, int
     constexpr int kStep = 64 / S;  // rows per load instruction of a wave
     const size_t wave_bytes = rho_stage_wave_bytes<T, T2>(kmax);
     const int prow = phase_rows(kmax);
-    if ((size_t)NW * wave_bytes <= buf_bytes && prow <= kRhoStageRows) {
+    if (ra.stage && (size_t)NW * wave_bytes <= buf_bytes && prow <= kRhoStageRows) {
         // Staged: each round copies every phase row of the next S atoms
         // (and their charges) to the wave's own LDS area, one load per 64
         // entries (S consecutive atoms of 64 / S rows), all in flight
@@ -2337,8 +2338,10 @@ __global__ __launch_bounds__(kCellBlock) void k_cells_rho(CellsArgs ca, RhoRowsA
 // need only the positions of the first half-step.
 // kIntra: the instance with the intra-molecular blocks (mixed form); the
 // others leave their code out (it costs the rho blocks registers).
+// Four waves per SIMD: its 64 KB of LDS allow two blocks (sixteen waves) per
+// CU, so registers up to 128 cost nothing and above it would cost a block.
 template <bool kMinImage, typename T, typename T2, bool kIntra = false>
-__global__ __launch_bounds__(kCellBlock) void k_direct_rho(DirectArgs da, RhoRowsArgs<T, T2> ra, int ncb, int nrho,
+__global__ __launch_bounds__(kCellBlock, 4) void k_direct_rho(DirectArgs da, RhoRowsArgs<T, T2> ra, int ncb, int nrho,
                                                            IntraArgs ia) {
     __shared__ __align__(16) unsigned char buf[sizeof(DirectLds) > kRhoRed ? sizeof(DirectLds) : kRhoRed];
     if ((int)blockIdx.x < ncb)

@@ -470,7 +470,8 @@ class System:
         of the rho(k) kernel; nlist_build: 1 lanes per atom, 2 one workgroup
         per cell over the cell list, 3 direct (workgroups per cell over the
         atoms' cell tags, up to 32768 atoms); md_graph: 1 eager launches (default), 2 HIP-graph replay of
-        MD chunks; rho_kind: 1 row form, 2 matrix-core form (kmax <= 16)."""
+        MD chunks; rho_kind: 1 row form (phase rows staged in LDS), 2
+        matrix-core form (kmax <= 16), 3 row form without the staging."""
         self.tuning_ = (int(pair_lanes), int(kspace_split), int(rho_width), int(nlist_build), int(md_graph),
                         int(rho_kind))
         if self._handle is not None and not self._dirty:

@@ -24,7 +24,7 @@ PAIR_LANES = (8, 16, 32, 64)
 KSPACE_SPLITS = (2, 4, 8, 16, 32)
 RHO_WIDTHS = (64, 128, 192, 256, 512)
 NLIST_BUILDS = (0, 2)  # automatic, cell list (lg_tuning.nlist_build)
-RHO_KINDS = (1, 2)  # rows, matrix cores (lg_tuning.rho_kind)
+RHO_KINDS = (1, 2, 3)  # rows staged in LDS, matrix cores, rows from the table (lg_tuning.rho_kind)
 
 _cache = {}
 _lock = threading.Lock()
@@ -109,19 +109,22 @@ def autotune(system, reps=5, use_cache=True, rounds=3, step=None):
         best["pair_lanes"], _ = pick("pair_lanes", PAIR_LANES)
         if key[0]:
             best["kspace_split"], _ = pick("kspace_split", KSPACE_SPLITS)
-            # rho(k): each width of the row form, then the matrix-core form
+            # rho(k): each width of the row form, then at the best width
+            # the row form without its LDS staging, then the matrix-core form
             width, times = pick("rho_width", RHO_WIDTHS, rho_kind=1)
-            trial = dict(best, rho_kind=2)
-            apply(trial)
-            warm()
-            t_mfma = float("inf")
-            if system.tuning()["rho_kind"] == 2:
-                for _ in range(rounds):
-                    t_mfma = min(t_mfma, _time_all(system, reps, classes["rho_kind"], step))
-            if t_mfma < times[width]:
-                best["rho_kind"] = 2
-            else:
-                best["rho_kind"], best["rho_width"] = 1, width
+            kind, t_kind = 1, times[width]
+            for k in (3, 2):
+                trial = dict(best, rho_width=width if k != 2 else 0, rho_kind=k)
+                apply(trial)
+                warm()
+                t = float("inf")
+                if system.tuning()["rho_kind"] == k:
+                    for _ in range(rounds):
+                        t = min(t, _time_all(system, reps, classes["rho_kind"], step))
+                if t < t_kind:
+                    kind, t_kind = k, t
+            best["rho_kind"] = kind
+            best["rho_width"] = width if kind != 2 else 0
         hit = best
         with _lock:
             _cache[key] = hit

@@ -74,6 +74,21 @@ def test_matrix_core_rho_mixed(case):
     assert abs(e2.coulomb_kspace - e1.coulomb_kspace) <= 1e-5 * abs(e1.coulomb_kspace)
 
 
+@pytest.mark.parametrize("case", ["nacl-512-kmax7", "water-1500-kmax10", "nacl-512-kmax12"])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_staged_rows_equal_table_rows(case, mixed):
+    # the row form with its phase rows staged in LDS (rho_kind 1) and read
+    # from the table (rho_kind 3) adds the same terms in the same order:
+    # the same bits, in both forms
+    make, _ = CASES[case]
+    s1, f1, e1, w1 = run(make, 1, mixed=mixed)
+    s3, f3, e3, w3 = run(make, 3, mixed=mixed)
+    assert s3.tuning()["rho_kind"] == 3 and s1.tuning()["rho_kind"] == 1
+    assert np.array_equal(f1, f3)
+    assert e1.coulomb_kspace == e3.coulomb_kspace
+    assert np.array_equal(w1, w3)
+
+
 def test_kmax_above_16_uses_rows():
     s = nacl(ncell=4, cutoff=10.0, kmax=17)
     s.set_tuning(rho_kind=2)

@@ -20,7 +20,7 @@ def frel(f, g):
 def test_every_knob_setting_meets_parity(lanes, split):
     s = nacl(ncell=4, cutoff=10.0, kmax=7)
     fo = lo.OracleSystem(s, nthreads=8).forces()
-    kind = 1 + (lanes + split) % 2
+    kind = 1 + (lanes + split) % 3
     s.set_tuning(pair_lanes=lanes, kspace_split=split, rho_width=(64, 128, 256)[lanes % 3], rho_kind=kind)
     assert frel(s.forces(), fo) < 1e-10
     t = s.tuning()
@@ -33,8 +33,8 @@ def test_autotune_choice_and_cache():
     f0 = s.forces()
     best = tuning.autotune(s, reps=3)
     assert best["pair_lanes"] in tuning.PAIR_LANES
-    assert best["kspace_split"] in tuning.KSPACE_SPLITS and best["rho_kind"] in (1, 2)
-    assert best["rho_width"] in tuning.RHO_WIDTHS if best["rho_kind"] == 1 else best["rho_width"] == 0
+    assert best["kspace_split"] in tuning.KSPACE_SPLITS and best["rho_kind"] in (1, 2, 3)
+    assert best["rho_width"] in tuning.RHO_WIDTHS if best["rho_kind"] != 2 else best["rho_width"] == 0
     assert frel(s.forces(), f0) < 1e-12
     # a second system of the same bucket reuses the choice
     t = spce(nmol=300, L=20.8, cutoff=8.0, kmax=7, seed=9)
@@ -52,7 +52,7 @@ def test_autotune_on_md_steps():
     sim = Simulation(MolecularDynamics(VelocityVerlet(0.5)))
     best = tuning.autotune(s, use_cache=False, rounds=1, step=lambda: sim.run(s, 5))
     assert best["pair_lanes"] in tuning.PAIR_LANES and best["kspace_split"] in tuning.KSPACE_SPLITS
-    assert best["rho_kind"] in (1, 2)
+    assert best["rho_kind"] in (1, 2, 3)
     t = spce(nmol=300, L=20.8, cutoff=8.0, kmax=7, seed=12)
     t.set_precision(LG_PRECISION_MIXED)
     f0 = t.forces()
