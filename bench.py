@@ -760,8 +760,9 @@ def main():
                     help="spce: the forms measured (a comma list; mixed alone for a profiled run)")
     ap.add_argument("--no-drift", action="store_true", help="spce: leave out the NVE drift runs")
     ap.add_argument("--serve-jobs", type=int, default=256, help="spce: MD jobs of the serving stream")
-    ap.add_argument("--serve-scale", type=float, default=0.005,
-                    help="spce: factor on the serving stream's 1k..20k MD steps")
+    ap.add_argument("--serve-scale", type=float, default=0.05,
+                    help="spce: factor on the serving stream's 1k..20k MD steps (0.05: 50..1000 steps, about "
+                         "a third of one GPU's serving capacity over the 20 s of arrivals)")
     ap.add_argument("--tuning-cache", default=None,
                     help="JSON file of autotuner choices: read if it exists, written after tuning (so that a "
                          "profiled rerun launches the same kernel variants as the timed run)")
