@@ -140,3 +140,24 @@ def test_failing_jobs_do_not_disturb_the_mix():
     for j in good:
         assert j.error is None, (j.name, j.error)
         same(j.result, run_alone(j))
+
+
+def test_large_water_system_sets_up_in_milliseconds():
+    # A 36 000-atom water job's set-up (topology, tables, first list build
+    # and forces) once took ~2 s, all in the angle/dihedral search of the
+    # topology, which compared every pair of bonds; over the bonds incident to
+    # each bond's atoms it takes tens of milliseconds.  The bound leaves a
+    # wide margin on a loaded host.  Its angles, and their forces, are those
+    # of the oracle.
+    from lumol_amd.workloads import water_box
+    import lumol_oracle as lo
+    s = water_box(12000, seed=7)
+    s.set_precision(LG_PRECISION_MIXED)
+    t0 = time.perf_counter()
+    s.forces()
+    assert time.perf_counter() - t0 < 0.5
+    s.close()
+    small = water_box(300, seed=7)
+    o = lo.OracleSystem(small, nthreads=8)
+    f, fo = small.forces(), o.forces()
+    assert np.abs(f - fo).max() / np.abs(fo).max() < 1e-10
