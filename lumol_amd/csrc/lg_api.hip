@@ -2960,10 +2960,16 @@ int lg_mc_run_batch(lg_system* const* systems, size_t count, const uint64_t* see
     for (size_t b = 0; b < count; b++) {
         if (!systems[b]) return fail(LG_E_INVALID, "null system");
         if (systems[b]->ctx != ctx) return fail(LG_E_INVALID, "all chains of a batch must share one context");
-        for (size_t q = 0; q < b; q++)
-            if (systems[q] == systems[b]) return fail(LG_E_INVALID, "a system appears twice in one batch");
         if (!(beta[b] >= 0.0) || !std::isfinite(beta[b]) || !(delta[b] >= 0.0) || !std::isfinite(delta[b]))
             return fail(LG_E_INVALID, "invalid temperature or amplitude");
+    }
+    {
+        // duplicates by sorting (a pairwise check cost ~0.5 ms of host time
+        // per launch of 1024 chains)
+        std::vector<lg_system*> seen(systems, systems + count);
+        std::sort(seen.begin(), seen.end());
+        if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+            return fail(LG_E_INVALID, "a system appears twice in one batch");
     }
     LG_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
