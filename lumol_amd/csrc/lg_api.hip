@@ -614,7 +614,7 @@ int build_topology(lg_system* s, std::vector<std::tuple<int, int, int>>& ipairs_
             else
                 return;
             angles.insert(angle);
-            auto [ai, aj, ak] = angle;
+            const int ai = std::get<0>(angle), aj = std::get<1>(angle), ak = std::get<2>(angle);
             incident(ak, ai, [&](const std::pair<int, int>& b3) {
                 if (b2 == b3) return;
                 if (ak == b3.first && aj != b3.second)
