@@ -1193,8 +1193,10 @@ int prepare_nlist(lg_system* s) {
     LG_TRY(s->nnbr.ensure(n));
     LG_TRY(s->flags.ensure(4));
     LG_TRY(s->ref.ensure(n));
-    // lanes per atom: enough waves to fill 256 CUs, fewer lanes for long lists
-    s->tpa = n <= 8192 ? 64 : n <= 32768 ? 32 : n <= 131072 ? 16 : n <= 524288 ? 8 : 4;
+    // lanes per atom: enough waves to fill 256 CUs, fewer lanes for long
+    // lists; not fewer than 8 (1M-atom Kob-Andersen: 4 lanes 660/720 us per
+    // mixed/FP64 step, 8 lanes 562/625 us; lumol_amd.tuning picks per bucket)
+    s->tpa = n <= 8192 ? 64 : n <= 32768 ? 32 : n <= 131072 ? 16 : 8;
     if (s->tune.pair_lanes > 0) s->tpa = s->tune.pair_lanes;
     if (s->gen_pairs) s->tpa = 16;  // the general pair kernel has one instance
     LG_TRY(s->epart_pairs.ensure(size_t(pair_blocks_of(n, s->tpa)) * 8));
