@@ -613,6 +613,8 @@ def bench_ka(args):
         # mixed step against 512 us with 16 lanes
         from lumol_amd import tuning
         knobs = tuning.autotune(s, rounds=2, step=lambda: sim.run(s, 50))
+        if TUNING_CACHE:
+            tuning.save_cache(TUNING_CACHE)
         s.profile_begin(["nlist", "pairs", "integrate"])
         sim.run(s, args.warmup)
         warm = s.profile_end()
